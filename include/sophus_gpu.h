@@ -1,0 +1,345 @@
+/*
+ * sophus_gpu.h — C ABI of the MI355X (gfx950) Schur-complement Levenberg–Marquardt
+ * NLLS library.  Every entry point takes plain pointers and sizes; no C++ or torch
+ * types cross this boundary.
+ *
+ * The surface mirrors sophus_opt's NLLS API:
+ *   optimize_nlls            crates/sophus_opt/src/nlls.rs:73-80
+ *   Optimizer::new/step      crates/sophus_opt/src/nlls/optimizer.rs:71-120,345-377
+ *   Optimizer::build_solution crates/sophus_opt/src/nlls/optimizer.rs:609-645
+ *   OptParams (+ defaults)   crates/sophus_opt/src/nlls/optimizer_types.rs:161-240
+ *   VarKind                  crates/sophus_opt/src/variables.rs:17-24
+ *   VarFamily::new_with_const_ids crates/sophus_opt/src/variables/var_family.rs:26-46
+ *   CostFn / CostTerms       crates/sophus_opt/src/nlls/cost/cost_fn.rs:105-151,
+ *                            crates/sophus_opt/src/nlls/cost/cost_term.rs:74-105
+ *   RobustKernel::Huber      crates/sophus_opt/src/robust_kernel.rs:28-57
+ *   TerminationReason        crates/sophus_opt/src/nlls/optimizer_types.rs:93-122
+ *   NllsError                crates/sophus_opt/src/nlls/optimizer_types.rs:422-504
+ *
+ * Conventions (identical to the reference):
+ *   - SE(3) members are 7 doubles (qw, qx, qy, qz, tx, ty, tz) = world_from_camera;
+ *     tangent order [omega; nu] (rotation first); update T <- exp(delta) * T.
+ *   - Vec3 members are 3 doubles; pinhole members are 4 doubles (fx, fy, cx, cy).
+ *   - Family partitions: Free families in lexicographic name order, then
+ *     Marginalized families; Conditioned families and const members get no blocks.
+ *   - The cost has no 1/2 factor: cost = sum rho(|r|) (Huber) or |r|^2.
+ *
+ * Threading: an sgpu_optimizer handle owns its device buffers and one HIP stream;
+ * calls on one handle must not overlap; distinct handles are independent.
+ */
+#ifndef SOPHUS_GPU_H
+#define SOPHUS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SGPU_ABI_VERSION 1
+
+/* Status codes.  One code per NllsError variant the NLLS path can raise
+ * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
+typedef enum sgpu_status {
+  SGPU_OK = 0,
+  SGPU_ERR_INVALID_ARGUMENT = 1,
+  /* NllsError::SchurMissingFreeOrMarg  (optimizer.rs:680-697) */
+  SGPU_ERR_SCHUR_MISSING_FREE_OR_MARG = 2,
+  /* NllsError::SchurCostConnectsTwoMargFamilies (optimizer.rs:699-714) */
+  SGPU_ERR_SCHUR_COST_CONNECTS_TWO_MARG_FAMILIES = 3,
+  /* VarFamilyError::UnknownFamily (var_families.rs) */
+  SGPU_ERR_UNKNOWN_FAMILY = 4,
+  /* VarFamilyError::TypeError */
+  SGPU_ERR_FAMILY_TYPE = 5,
+  /* an entity index is >= the family's member count */
+  SGPU_ERR_INDEX_OUT_OF_RANGE = 6,
+  /* a layout this library does not implement (see sgpu_last_error) */
+  SGPU_ERR_UNSUPPORTED = 7,
+  SGPU_ERR_HIP = 8,
+  SGPU_ERR_NO_DEVICE = 9,
+  SGPU_ERR_OUT_OF_MEMORY = 10,
+  /* a non-finite residual or pivot (LdltDecompositionError::NonFinitePivot) */
+  SGPU_ERR_NON_FINITE = 11,
+  /* the handle has already terminated */
+  SGPU_ERR_TERMINATED = 12
+} sgpu_status;
+
+/* VarKind (crates/sophus_opt/src/variables.rs:17-24) */
+typedef enum sgpu_var_kind {
+  SGPU_VAR_FREE = 0,
+  SGPU_VAR_CONDITIONED = 1,
+  SGPU_VAR_MARGINALIZED = 2
+} sgpu_var_kind;
+
+/* Variable types on the Schur BA path. */
+typedef enum sgpu_var_type {
+  SGPU_VAR_ISOMETRY3 = 0, /* Isometry3F64: 7 params, 6 dof */
+  SGPU_VAR_VEC3 = 1,      /* VecF64<3>:   3 params, 3 dof */
+  SGPU_VAR_PINHOLE = 2    /* PinholeCameraF64: 4 params, 4 dof (additive) */
+} sgpu_var_type;
+
+/* Cost term kinds (closed set instead of HasResidualFn trait objects). */
+typedef enum sgpu_cost_kind {
+  /* PinholeCameraReprojectionCostTerm: args (intrinsics, world_from_camera, point);
+   * measurement uv[2].  pinhole_reprojection.rs:42-95 */
+  SGPU_COST_PINHOLE_REPROJECTION = 0,
+  /* PoseGraph3CostTerm: args (world_from_pose_m, world_from_pose_n);
+   * measurement pose_m_from_pose_n[7].  pose_graph3.rs:33-79 */
+  SGPU_COST_POSE_GRAPH3 = 1
+} sgpu_cost_kind;
+
+typedef enum sgpu_robust_kernel {
+  SGPU_ROBUST_NONE = 0,
+  SGPU_ROBUST_HUBER = 1 /* RobustKernel::Huber(delta), robust_kernel.rs:28-35 */
+} sgpu_robust_kernel;
+
+/* LinearSolverEnum subset (crates/sophus_solver/src/lib.rs:295-318) plus the
+ * iterative RCS solvers of this library. */
+typedef enum sgpu_solver {
+  /* SchurBlockSparseLdlt: Schur complement + block-sparse LDLt of the RCS */
+  SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT = 0,
+  /* Schur complement + block-Jacobi preconditioned CG on the RCS */
+  SGPU_SOLVER_SCHUR_PCG_JACOBI = 1,
+  /* Schur complement + PCG with the learned (MLP-corrected) block-Jacobi */
+  SGPU_SOLVER_SCHUR_PCG_LEARNED = 2
+} sgpu_solver;
+
+typedef enum sgpu_precision {
+  /* FP64 everywhere (parity path) */
+  SGPU_PRECISION_F64 = 0,
+  /* FP32 residuals/Jacobians/assembly/Schur products; FP64 RCS, pivots, cost
+   * sums, LM control */
+  SGPU_PRECISION_MIXED = 1
+} sgpu_precision;
+
+/* TerminationReason (optimizer_types.rs:93-122) */
+typedef enum sgpu_termination {
+  SGPU_TERM_NONE = 0,
+  SGPU_TERM_MAX_ITERATIONS = 1,
+  SGPU_TERM_FUNCTION_TOLERANCE = 2,
+  SGPU_TERM_GRADIENT_TOLERANCE = 3,
+  SGPU_TERM_PARAMETER_TOLERANCE = 4,
+  SGPU_TERM_MAX_CONSECUTIVE_REJECTS = 5
+} sgpu_termination;
+
+/* VarFamily (var_family.rs:15-46).  `members` is num_members x params AoS. */
+typedef struct sgpu_var_family {
+  const char* name;
+  int32_t type; /* sgpu_var_type */
+  int32_t kind; /* sgpu_var_kind */
+  int64_t num_members;
+  const double* members;
+  int64_t num_const_ids;
+  const int64_t* const_ids; /* members held constant (VarFamily::new_with_const_ids) */
+} sgpu_var_family;
+
+/* CostFn::new_boxed(global_constants, CostTerms::new(family_names, terms), kernel)
+ * (cost_fn.rs:105-151).  entity_indices: num_terms x arity; measurements:
+ * num_terms x (2 | 7). */
+typedef struct sgpu_cost_fn {
+  int32_t kind;          /* sgpu_cost_kind */
+  int32_t robust_kernel; /* sgpu_robust_kernel */
+  double huber_delta;
+  const char* family_names[3]; /* arity 3 (reprojection) or 2 (pose graph) */
+  int64_t num_terms;
+  const int64_t* entity_indices;
+  const double* measurements;
+} sgpu_cost_fn;
+
+/* OptParams (optimizer_types.rs:161-189) + build-only fields. */
+typedef struct sgpu_opt_params {
+  int64_t num_iterations;        /* default 100 */
+  double initial_lm_damping;     /* default 10.0 */
+  int32_t parallelize;           /* default 1 (accepted, no effect on the GPU) */
+  int32_t solver;                /* sgpu_solver, default SCHUR_BLOCK_SPARSE_LDLT */
+  int32_t skip_final_hessian;    /* default 0 */
+  int32_t precision;             /* sgpu_precision, default F64 */
+  double function_tolerance;     /* default 1e-10 */
+  double gradient_tolerance;     /* default 1e-14 */
+  double parameter_tolerance;    /* default 1e-12 */
+  int64_t max_consecutive_rejects; /* default 20 */
+  double pcg_relative_tolerance; /* default 1e-8 (PCG solvers only) */
+  int64_t pcg_max_iterations;    /* default 500 */
+  int32_t device;                /* HIP device ordinal, default 0 */
+  int32_t reserved0;
+} sgpu_opt_params;
+
+/* StepInfo (optimizer_types.rs:67-83) plus per-step diagnostics. */
+typedef struct sgpu_step_info {
+  double merit;
+  double smooth_cost;
+  int64_t inner_step;
+  double lm_damping;     /* damping after the step */
+  int32_t termination;   /* sgpu_termination, NONE while running */
+  int32_t accepted;      /* 1 if the step was accepted */
+  double new_merit;      /* merit of the proposed variables (NaN if not computed) */
+  double gain_ratio;     /* NaN if not computed */
+  double max_abs_neg_grad;
+  double max_abs_delta;
+  int64_t pcg_iterations; /* 0 for the direct solver */
+  /* per-phase device milliseconds (HIP events on the handle's stream) */
+  double ms_linearize;   /* residual/Jacobian + assembly + Schur elimination */
+  double ms_factor;      /* RCS factorization or PCG */
+  double ms_solve;       /* triangular solves + back-substitution + update */
+  double ms_cost;        /* merit of the proposed variables */
+  double ms_total;       /* wall time of the step on the host */
+} sgpu_step_info;
+
+/* OptimizationSolution (optimizer_types.rs:243-257), scalar part. */
+typedef struct sgpu_solution {
+  double initial_cost;
+  double final_cost;
+  int64_t num_steps;
+  int32_t termination; /* sgpu_termination */
+  int32_t reserved0;
+  double final_max_abs_neg_grad; /* from the final (nu = 0) linearization */
+} sgpu_solution;
+
+typedef struct sgpu_optimizer sgpu_optimizer;
+
+/* ---- parameters / diagnostics ------------------------------------------------ */
+
+/* OptParams::default() (optimizer_types.rs:225-240). */
+void sgpu_opt_params_default(sgpu_opt_params* params);
+
+/* Message of the last error raised on this thread ("" if none). */
+const char* sgpu_last_error(void);
+
+int32_t sgpu_abi_version(void);
+
+/* ---- full solve ----------------------------------------------------------------
+ * optimize_nlls(variables, cost_fns, [], params) (nlls.rs:73-80).
+ * out_members[i] (may be NULL) receives family i's final members
+ * (num_members x params).  Solver failures inside LM are rejects, not errors
+ * (optimizer_helpers.rs:176-185). */
+sgpu_status sgpu_optimize_nlls(const sgpu_var_family* families, int64_t num_families,
+                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                               const sgpu_opt_params* params, double* const* out_members,
+                               sgpu_solution* solution);
+
+/* ---- stateful optimizer --------------------------------------------------------
+ * Optimizer::new (optimizer.rs:71-120): validates, sorts terms, uploads, computes
+ * the initial merit and the symbolic structure. */
+sgpu_status sgpu_optimizer_create(const sgpu_var_family* families, int64_t num_families,
+                                  const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                  const sgpu_opt_params* params, sgpu_optimizer** out);
+
+/* Optimizer::step (optimizer.rs:345-377 + optimizer_step.rs:366-443). */
+sgpu_status sgpu_optimizer_step(sgpu_optimizer* opt, sgpu_step_info* info);
+
+/* Optimizer::variables(): copy family `name`'s members to `out` (capacity doubles). */
+sgpu_status sgpu_optimizer_get_members(sgpu_optimizer* opt, const char* name, double* out,
+                                       int64_t capacity);
+
+/* Optimizer::merit() / lm_damping() */
+double sgpu_optimizer_merit(const sgpu_optimizer* opt);
+double sgpu_optimizer_lm_damping(const sgpu_optimizer* opt);
+
+/* Optimizer::eval_cost_at (optimizer.rs:570): merit of the handle's cost terms at
+ * caller-provided family values (same family order/shape as at create). */
+sgpu_status sgpu_optimizer_eval_cost_at(sgpu_optimizer* opt, const double* const* members,
+                                        double* cost);
+
+/* Optimizer::build_solution (optimizer.rs:609-645): one more linearization with
+ * nu = 0 unless skip_final_hessian. */
+sgpu_status sgpu_optimizer_build_solution(sgpu_optimizer* opt, sgpu_solution* solution);
+
+/* Structure of the reduced camera system of this problem. */
+typedef struct sgpu_rcs_stats {
+  int64_t num_free_blocks;   /* RCS block rows (6x6 blocks) */
+  int64_t num_marg_blocks;   /* eliminated point blocks */
+  int64_t num_observations;  /* reduced reprojection terms */
+  int64_t num_pose_terms;    /* reduced pose-graph terms */
+  int64_t nnz_s_blocks;      /* lower-triangular S blocks incl. diagonal */
+  int64_t nnz_l_blocks;      /* strictly-lower L blocks after ordering */
+  int64_t num_schur_pairs;   /* (point, cam_i, cam_j<=cam_i) contributions */
+  int64_t num_levels;        /* etree levels of the RCS factorization */
+  double factor_flops;       /* FP64 flops of one RCS factorization */
+} sgpu_rcs_stats;
+
+sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* stats);
+
+/* Per-kernel device timing recorded with HIP events on the handle's stream since the
+ * last reset: names[i] (static strings), count[i] launches, total_ms[i].
+ * Returns the number of kernels written (<= capacity). */
+int32_t sgpu_optimizer_kernel_times(const sgpu_optimizer* opt, const char** names,
+                                    int64_t* counts, double* total_ms, int32_t capacity);
+void sgpu_optimizer_reset_kernel_times(sgpu_optimizer* opt);
+/* 1 enables per-kernel event timing (adds one event pair per launch), 0 disables. */
+void sgpu_optimizer_set_kernel_timing(sgpu_optimizer* opt, int32_t enable);
+
+void sgpu_optimizer_destroy(sgpu_optimizer* opt);
+
+/* ---- stage-level entry points (parity tests of one stage at a time) -----------
+ * All take host arrays and copy them to/from the device internally. */
+
+/* Kernel (a): per-term residual, Jacobians and the term's Hessian/gradient blocks
+ * for pinhole reprojection terms, evaluated on the GPU in FP64 (precision = F64) or
+ * FP32 (MIXED).  Mirrors PinholeCameraReprojectionCostTerm::eval + make
+ * (pinhole_reprojection.rs:60-95, evaluated_term.rs:286-327).
+ * Per term t: residual_out[2t..], jac_pose_out[12t..] (2x6 row-major),
+ * jac_point_out[6t..] (2x3 row-major), cost_out[t] (rho(|r|)), weight_out[t].
+ * Any output pointer may be NULL. */
+sgpu_status sgpu_eval_pinhole_terms(int32_t precision, int64_t num_terms,
+                                    const double* intrinsics /*4 per term*/,
+                                    const double* world_from_camera /*7 per term*/,
+                                    const double* point_in_world /*3 per term*/,
+                                    const double* uv /*2 per term*/, int32_t robust_kernel,
+                                    double huber_delta, double* residual_out,
+                                    double* jac_pose_out, double* jac_point_out,
+                                    double* cost_out, double* weight_out);
+
+/* Kernel (a'): SE(3) between factor (pose_graph3.rs:33-79) on the GPU (FP64).
+ * Per term: residual_out[6t..], jac_m_out[36t..], jac_n_out[36t..] (6x6 row-major). */
+sgpu_status sgpu_eval_pose_graph3_terms(int64_t num_terms, const double* world_from_pose_m,
+                                        const double* world_from_pose_n,
+                                        const double* pose_m_from_pose_n,
+                                        double* residual_out, double* jac_m_out,
+                                        double* jac_n_out);
+
+/* Kernel (d): block-sparse LDLt solve of a symmetric system with 6x6 blocks on the
+ * GPU in FP64 (BlockSparseLdlt::factorize_impl + solve_inplace,
+ * block_sparse_ldlt.rs:96-190,479-535).  Lower blocks in CSC-by-block-column:
+ * col_ptr[nb+1], row_idx[nnz] (row >= col, diagonal included), values[nnz*36]
+ * (each block 6x6 column-major).  rhs[nb*6] -> x[nb*6]. */
+sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
+                                  const int64_t* row_idx, const double* values,
+                                  const double* rhs, double* x);
+
+/* ---- synthetic problems ---------------------------------------------------------
+ * Seeded BA generator (SplitMix64 -> xoshiro256**, Box-Muller), the input of the
+ * parity tests and the benchmark.  Two-call protocol: call with NULL arrays to get
+ * the sizes, then with arrays of those sizes. */
+typedef struct sgpu_ba_spec {
+  int32_t layout;       /* 0 = ring of cameras (cube of points), 1 = random-walk */
+  int32_t num_cameras;
+  int64_t num_points;
+  int32_t obs_per_point_ring; /* layout 0: distinct cameras per point */
+  int32_t min_obs, max_obs;   /* layout 1: clip of the geometric draw */
+  double geometric_p;         /* layout 1: success probability */
+  double pixel_noise;         /* px sigma */
+  double rot_noise;           /* rad sigma of the pose perturbation */
+  double trans_noise;         /* m sigma of the pose perturbation */
+  double point_noise;         /* m sigma of the point perturbation */
+  double step_length;         /* layout 1 */
+  double yaw_jitter;          /* layout 1, rad sigma */
+  uint64_t seed;
+} sgpu_ba_spec;
+
+typedef struct sgpu_ba_sizes {
+  int64_t num_cameras;
+  int64_t num_points;
+  int64_t num_obs;
+} sgpu_ba_sizes;
+
+sgpu_status sgpu_generate_ba(const sgpu_ba_spec* spec, sgpu_ba_sizes* sizes,
+                             double* intrinsics /*4*/, double* poses_gt /*7 x cams*/,
+                             double* poses_init, double* points_gt /*3 x pts*/,
+                             double* points_init, int64_t* obs_idx /*3 x obs: cam,pose,pt*/,
+                             double* obs_uv /*2 x obs*/);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* SOPHUS_GPU_H */

@@ -1,0 +1,314 @@
+// Seeded synthetic bundle-adjustment problems.
+//
+// Behavioural counterpart of the reference's example problems
+// (crates/sophus_opt/src/example_problems/ba_problem.rs:154-381 and
+// crates/sophus_opt/src/example_problems/cam_calib.rs:63-151): a pinhole camera
+// (fx, fy, cx, cy) shared by all observations, SE(3) world_from_camera poses, 3-D
+// points, pixel observations with Gaussian noise and perturbed initial values.
+//
+// RNG: SplitMix64 seeds a xoshiro256** generator; normals come from Box-Muller.
+// The draw order is fixed (ground truth, observations, pose perturbations, point
+// perturbations), so one seed gives one bit-identical problem on every machine.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "lie.hpp"
+
+namespace sgpu {
+namespace {
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+struct Rng {
+  uint64_t s[4];
+  bool has_spare = false;
+  double spare = 0.0;
+  explicit Rng(uint64_t seed) {
+    uint64_t x = seed;
+    for (auto& v : s) v = splitmix64(x);
+  }
+  uint64_t next() {
+    const uint64_t result = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return result;
+  }
+  // [0, 1)
+  double uniform() { return double(next() >> 11) * 0x1.0p-53; }
+  double uniform(double lo, double hi) { return lo + (hi - lo) * uniform(); }
+  // [0, n)
+  int64_t below(int64_t n) { return int64_t(uniform() * double(n)); }
+  double normal() {
+    if (has_spare) {
+      has_spare = false;
+      return spare;
+    }
+    double u1 = 1.0 - uniform();  // (0, 1]
+    double u2 = uniform();
+    double r = std::sqrt(-2.0 * std::log(u1));
+    spare = r * std::sin(2.0 * M_PI * u2);
+    has_spare = true;
+    return r * std::cos(2.0 * M_PI * u2);
+  }
+};
+
+void normalize3(double v[3]) {
+  double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= n;
+  v[1] /= n;
+  v[2] /= n;
+}
+
+void cross3(const double a[3], const double b[3], double c[3]) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Unit quaternion (w, x, y, z) of a row-major rotation matrix (Shepperd's method).
+void quat_from_matrix(const double R[9], double q[4]) {
+  const double tr = R[0] + R[4] + R[8];
+  if (tr > 0) {
+    double s = std::sqrt(tr + 1.0) * 2;
+    q[0] = 0.25 * s;
+    q[1] = (R[7] - R[5]) / s;
+    q[2] = (R[2] - R[6]) / s;
+    q[3] = (R[3] - R[1]) / s;
+  } else if (R[0] > R[4] && R[0] > R[8]) {
+    double s = std::sqrt(1.0 + R[0] - R[4] - R[8]) * 2;
+    q[0] = (R[7] - R[5]) / s;
+    q[1] = 0.25 * s;
+    q[2] = (R[1] + R[3]) / s;
+    q[3] = (R[2] + R[6]) / s;
+  } else if (R[4] > R[8]) {
+    double s = std::sqrt(1.0 + R[4] - R[0] - R[8]) * 2;
+    q[0] = (R[2] - R[6]) / s;
+    q[1] = (R[1] + R[3]) / s;
+    q[2] = 0.25 * s;
+    q[3] = (R[5] + R[7]) / s;
+  } else {
+    double s = std::sqrt(1.0 + R[8] - R[0] - R[4]) * 2;
+    q[0] = (R[3] - R[1]) / s;
+    q[1] = (R[2] + R[6]) / s;
+    q[2] = (R[5] + R[7]) / s;
+    q[3] = 0.25 * s;
+  }
+  double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  for (int i = 0; i < 4; ++i) q[i] /= n;
+}
+
+// world_from_camera looking along `forward` with world +z up; camera x right,
+// y down, z forward.
+void look_along(const double pos[3], const double fwd_in[3], double pose[7]) {
+  double f[3] = {fwd_in[0], fwd_in[1], fwd_in[2]};
+  normalize3(f);
+  const double up[3] = {0, 0, 1};
+  double right[3], down[3];
+  cross3(f, up, right);
+  normalize3(right);
+  cross3(f, right, down);
+  const double R[9] = {right[0], down[0], f[0], right[1], down[1], f[1],
+                       right[2], down[2], f[2]};
+  quat_from_matrix(R, pose);
+  pose[4] = pos[0];
+  pose[5] = pos[1];
+  pose[6] = pos[2];
+}
+
+struct Problem {
+  double intrinsics[4];
+  std::vector<double> poses_gt, poses_init, points_gt, points_init, uv;
+  std::vector<int64_t> idx;
+};
+
+constexpr double kWidth = 640, kHeight = 480;
+
+// Project x_w with camera `pose` (world_from_camera); returns false if behind
+// or outside the image.
+bool project(const double K[4], const double pose[7], const double xw[3], double uv[2],
+             double min_depth) {
+  double cw[7];
+  se3_inverse(pose, cw);
+  double R[9], xc[3];
+  so3_matrix(cw, R);
+  mat3_vec(R, xw, xc);
+  xc[0] += cw[4];
+  xc[1] += cw[5];
+  xc[2] += cw[6];
+  if (xc[2] <= min_depth) return false;
+  uv[0] = xc[0] / xc[2] * K[0] + K[2];
+  uv[1] = xc[1] / xc[2] * K[1] + K[3];
+  return uv[0] >= 0 && uv[0] < kWidth && uv[1] >= 0 && uv[1] < kHeight;
+}
+
+void perturb(Rng& rng, const sgpu_ba_spec& s, Problem& p, int64_t ncam, int64_t npt) {
+  p.poses_init = p.poses_gt;
+  for (int64_t c = 1; c < ncam; ++c) {
+    double d[6];
+    for (int k = 0; k < 3; ++k) d[k] = s.rot_noise * rng.normal();
+    for (int k = 3; k < 6; ++k) d[k] = s.trans_noise * rng.normal();
+    se3_oplus(&p.poses_gt[7 * c], d, &p.poses_init[7 * c]);
+  }
+  p.points_init = p.points_gt;
+  for (int64_t i = 0; i < 3 * npt; ++i) p.points_init[i] += s.point_noise * rng.normal();
+}
+
+void gen_ring(const sgpu_ba_spec& s, Problem& p) {
+  Rng rng(s.seed);
+  const int64_t ncam = s.num_cameras, npt = s.num_points;
+  p.poses_gt.resize(7 * ncam);
+  for (int64_t j = 0; j < ncam; ++j) {
+    const double th = 2.0 * M_PI * double(j) / double(ncam);
+    const double pos[3] = {2.0 * std::cos(th), 2.0 * std::sin(th), 0.0};
+    const double fwd[3] = {-pos[0], -pos[1], -pos[2]};
+    look_along(pos, fwd, &p.poses_gt[7 * j]);
+  }
+  p.points_gt.resize(3 * npt);
+  for (auto& v : p.points_gt) v = rng.uniform(-0.5, 0.5);
+  const int k = s.obs_per_point_ring;
+  std::vector<int64_t> cams(ncam);
+  for (int64_t b = 0; b < npt; ++b) {
+    for (int64_t c = 0; c < ncam; ++c) cams[c] = c;
+    for (int j = 0; j < k; ++j) {  // partial Fisher-Yates: k distinct cameras
+      int64_t r = j + rng.below(ncam - j);
+      std::swap(cams[j], cams[r]);
+      const int64_t c = cams[j];
+      double uv[2];
+      project(p.intrinsics, &p.poses_gt[7 * c], &p.points_gt[3 * b], uv, -1e300);
+      p.idx.insert(p.idx.end(), {0, c, b});
+      p.uv.push_back(uv[0] + s.pixel_noise * rng.normal());
+      p.uv.push_back(uv[1] + s.pixel_noise * rng.normal());
+    }
+  }
+  perturb(rng, s, p, ncam, npt);
+}
+
+void gen_walk(const sgpu_ba_spec& s, Problem& p) {
+  Rng rng(s.seed);
+  const int64_t ncam = s.num_cameras, npt = s.num_points;
+  p.poses_gt.resize(7 * ncam);
+  double pos[3] = {0, 0, 0}, psi = 0.0;
+  for (int64_t j = 0; j < ncam; ++j) {
+    const double fwd[3] = {std::cos(psi), std::sin(psi), 0.0};
+    look_along(pos, fwd, &p.poses_gt[7 * j]);
+    psi += s.yaw_jitter * rng.normal();
+    pos[0] += s.step_length * std::cos(psi);
+    pos[1] += s.step_length * std::sin(psi);
+  }
+  p.points_gt.resize(3 * npt);
+  const double log1mp = std::log(1.0 - s.geometric_p);
+  std::vector<int64_t> seen;
+  std::vector<double> seen_uv;
+  for (int64_t b = 0; b < npt; ++b) {
+    for (int attempt = 0;; ++attempt) {
+      const int64_t a = rng.below(ncam);
+      const double u = rng.uniform(0, kWidth), v = rng.uniform(0, kHeight);
+      const double depth = rng.uniform(2.0, 10.0);
+      double xc[3] = {depth * (u - p.intrinsics[2]) / p.intrinsics[0],
+                      depth * (v - p.intrinsics[3]) / p.intrinsics[1], depth};
+      const double* T = &p.poses_gt[7 * a];
+      double R[9], xw[3];
+      so3_matrix(T, R);
+      mat3_vec(R, xc, xw);
+      xw[0] += T[4];
+      xw[1] += T[5];
+      xw[2] += T[6];
+      // geometric draw on {1, 2, ...}, clipped
+      double g = std::ceil(std::log(1.0 - rng.uniform()) / log1mp);
+      int64_t want = g < 1 ? 1 : int64_t(g);
+      if (want < s.min_obs) want = s.min_obs;
+      if (want > s.max_obs) want = s.max_obs;
+      seen.clear();
+      seen_uv.clear();
+      // nearest-in-time cameras: a, a-1, a+1, a-2, a+2, ...
+      const int64_t window = 4 * s.max_obs + 16;
+      for (int64_t d = 0; d <= window && int64_t(seen.size()) < want; ++d) {
+        for (int sgn = -1; sgn <= 1 && int64_t(seen.size()) < want; sgn += 2) {
+          if (d == 0 && sgn == 1) break;
+          const int64_t c = a + sgn * d;
+          if (c < 0 || c >= ncam) continue;
+          double uv[2];
+          if (project(p.intrinsics, &p.poses_gt[7 * c], xw, uv, 0.5)) {
+            seen.push_back(c);
+            seen_uv.push_back(uv[0]);
+            seen_uv.push_back(uv[1]);
+          }
+        }
+      }
+      if (int64_t(seen.size()) >= s.min_obs || attempt >= 64) {
+        if (seen.empty()) continue;
+        p.points_gt[3 * b] = xw[0];
+        p.points_gt[3 * b + 1] = xw[1];
+        p.points_gt[3 * b + 2] = xw[2];
+        for (size_t j = 0; j < seen.size(); ++j) {
+          p.idx.insert(p.idx.end(), {0, seen[j], b});
+          p.uv.push_back(seen_uv[2 * j] + s.pixel_noise * rng.normal());
+          p.uv.push_back(seen_uv[2 * j + 1] + s.pixel_noise * rng.normal());
+        }
+        break;
+      }
+    }
+  }
+  perturb(rng, s, p, ncam, npt);
+}
+
+}  // namespace
+}  // namespace sgpu
+
+extern "C" sgpu_status sgpu_generate_ba(const sgpu_ba_spec* spec, sgpu_ba_sizes* sizes,
+                                        double* intrinsics, double* poses_gt,
+                                        double* poses_init, double* points_gt,
+                                        double* points_init, int64_t* obs_idx,
+                                        double* obs_uv) {
+  using namespace sgpu;
+  clear_error();
+  if (!spec || !sizes) return set_error(SGPU_ERR_INVALID_ARGUMENT, "spec and sizes required");
+  if (spec->num_cameras < 2 || spec->num_points < 1)
+    return set_error(SGPU_ERR_INVALID_ARGUMENT, "need >= 2 cameras and >= 1 point");
+  if (spec->layout == 0 &&
+      (spec->obs_per_point_ring < 1 || spec->obs_per_point_ring > spec->num_cameras))
+    return set_error(SGPU_ERR_INVALID_ARGUMENT, "obs_per_point_ring out of range");
+  if (spec->layout == 1 && (spec->geometric_p <= 0 || spec->geometric_p >= 1 ||
+                            spec->min_obs < 1 || spec->max_obs < spec->min_obs))
+    return set_error(SGPU_ERR_INVALID_ARGUMENT, "bad geometric observation spec");
+  if (spec->layout != 0 && spec->layout != 1)
+    return set_error(SGPU_ERR_INVALID_ARGUMENT, "unknown layout");
+  Problem p;
+  p.intrinsics[0] = 500.0;
+  p.intrinsics[1] = 500.0;
+  p.intrinsics[2] = 320.0;
+  p.intrinsics[3] = 240.0;
+  if (spec->layout == 0)
+    gen_ring(*spec, p);
+  else
+    gen_walk(*spec, p);
+  sizes->num_cameras = spec->num_cameras;
+  sizes->num_points = spec->num_points;
+  sizes->num_obs = int64_t(p.uv.size() / 2);
+  auto put = [](double* dst, const std::vector<double>& v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  if (intrinsics) std::memcpy(intrinsics, p.intrinsics, sizeof(p.intrinsics));
+  put(poses_gt, p.poses_gt);
+  put(poses_init, p.poses_init);
+  put(points_gt, p.points_gt);
+  put(points_init, p.points_init);
+  put(obs_uv, p.uv);
+  if (obs_idx) std::memcpy(obs_idx, p.idx.data(), p.idx.size() * sizeof(int64_t));
+  return SGPU_OK;
+}

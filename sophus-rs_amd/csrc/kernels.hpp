@@ -1,0 +1,131 @@
+// Device kernels of the Schur LM iteration (launch wrappers; all asynchronous on `stream`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sgpu {
+
+// Per-iteration device scalars (one struct in device memory, copied back once per step).
+struct DevScalars {
+  double cost;            // merit of the evaluated variables
+  double dd, dg;          // delta.delta, delta.neg_grad
+  unsigned long long max_delta_bits;  // max |delta| as the bits of a non-negative double
+  unsigned long long max_grad_bits;   // max |neg_grad|
+  int32_t singular_marg;  // an H_mm block had det == 0
+  int32_t bad_pivot;      // non-finite RCS pivot
+  int32_t pcg_iterations;
+  int32_t pad;
+};
+
+// Read-only problem arrays on the device.
+struct DevProblem {
+  int32_t nb, npt, nobs, npose, nintr;
+  const int32_t* obs_pose;
+  const int32_t* obs_point;
+  const int32_t* obs_intr;
+  const double* obs_uv;
+  const int32_t* pt_ptr;
+  const int32_t* pt_nu;
+  const int32_t* pt_block;  // point member -> active index or -1
+  const int32_t* cam_nu;
+  const int32_t* pose_block;
+  const int32_t* cam_member;
+  const int32_t* cam_ptr;
+  const int32_t* cam_obs;
+  const int32_t* s_ptr;
+  const int32_t* s_col;
+  const int32_t* s_dest;
+  int robust;
+  double delta;
+};
+
+// RCS factorization structure on the device.
+struct DevFactor {
+  int32_t nb, nlev, nnz_l;
+  const int32_t* perm;      // perm[new] = old
+  const int32_t* l_ptr;
+  const int32_t* l_row;
+  const int32_t* r_ptr;
+  const int32_t* r_col;
+  const int32_t* r_lpos;
+  const int32_t* u_ptr;
+  const int32_t* u_src;
+  const int32_t* u_m;
+  const int32_t* lvl_ptr;
+  const int32_t* lvl_col;
+  const int32_t* h_lvl_ptr;  // host copy of lvl_ptr (launch sizes)
+  double* C;       // unified blocks: nb diagonal C's then nnz_l L blocks (36 each)
+  double* M;       // per reach entry: D_k L_jk^T (36 each)
+  double* Ljj;     // nb x 36 unit-lower
+  double* Dfull;   // nb x 36
+  double* dvec;    // nb x 6
+};
+
+// Linearization workspace.
+struct DevLin {
+  double* cw;         // npose x 12: camera_from_world rotation (row-major) + translation
+  double* hinv;       // npt x 9
+  double* pt_s;       // npt x 3: neg-gradient of the point (sum of A^T r~)
+  double* pt_y;       // npt x 3: hinv * s
+  double* rhs_nat;    // nb x 6: reduced rhs in camera order
+  double* grad_cam;   // nb x 6: neg-gradient of the cameras
+  double* x_perm;     // nb x 6 solve buffer in permuted order
+  double* x_aux;      // nb x 6 second solve buffer
+  double* dx_cam;     // nb x 6 camera step (camera order)
+  double* partial;    // reduction partials
+  int32_t partial_cap;
+};
+
+void launch_cam_cache(hipStream_t s, int npose, const double* poses, double* cw);
+// precision: 0 = FP64, 1 = FP32 Jacobians/assembly/Schur products
+void launch_point_prep(hipStream_t s, int precision, const DevProblem& p, const double* poses,
+                       const double* points, const double* intr, const double* cw, double nu,
+                       DevLin& w, DevScalars* sc);
+void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const double* points,
+                        const double* intr, const double* cw, double nu, DevLin& w,
+                        DevFactor& f, DevScalars* sc);
+void launch_fill(hipStream_t s, double* ptr, int64_t n, double v);
+void launch_factor_level(hipStream_t s, const DevFactor& f, int level, DevScalars* sc);
+void launch_triangular_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat,
+                             double* x_perm, double* x_aux, double* dx_cam);
+void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p,
+                           const double* points, const double* intr, const double* cw,
+                           const DevLin& w, double* points_new, DevScalars* sc);
+void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses,
+                         const DevLin& w, double* poses_new, DevScalars* sc);
+void launch_cost(hipStream_t s, const DevProblem& p, const double* poses, const double* points,
+                 const double* intr, double* partial, int32_t partial_cap, DevScalars* sc);
+void launch_finalize_step(hipStream_t s, const DevLin& w, int nb, int npt, DevScalars* sc);
+void launch_finalize_cost(hipStream_t s, const double* partial, int nblocks, DevScalars* sc);
+void launch_max_grad(hipStream_t s, const double* grad_cam, int nb, DevScalars* sc);
+void launch_zero_scalars(hipStream_t s, DevScalars* sc);
+
+// pose-graph terms (SE(3) between factor)
+struct DevPoseGraph {
+  int32_t n;
+  const int32_t* m;
+  const int32_t* nn;
+  const double* meas;
+  const uint8_t* first;
+  const int32_t* dest;  // 3 per term: (m,m), (n,n), (m,n) unified codes or -1
+  int robust;
+  double delta;
+};
+void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const DevProblem& p,
+                                 const double* poses, double nu, DevLin& w, DevFactor& f);
+void launch_pose_graph_cost(hipStream_t s, const DevPoseGraph& g, const double* poses,
+                            double* partial, int32_t partial_cap, DevScalars* sc);
+
+// stage-level evaluation kernels
+void launch_eval_pinhole(hipStream_t s, int precision, int64_t n, const double* intr,
+                         const double* pose, const double* pt, const double* uv, int robust,
+                         double delta, double* res, double* jpose, double* jpt, double* cost,
+                         double* weight);
+void launch_eval_pose_graph(hipStream_t s, int64_t n, const double* m, const double* nn,
+                            const double* mn, double* res, double* jm, double* jn);
+
+constexpr int kCostBlocks = 1024;
+
+}  // namespace sgpu

@@ -1,0 +1,578 @@
+// C ABI: Optimizer::new / step / build_solution and optimize_nlls on the GPU.
+//
+// The LM control (optimizer_step.rs:366-443, optimizer.rs:345-377,463-520,
+// optimizer_helpers.rs:159-173) runs on the host in FP64 from a handful of scalars the
+// device reduces per step; everything that touches observations, points or the reduced
+// camera system runs in the kernels of kernels.hip on the handle's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "kernels.hpp"
+#include "problem.hpp"
+
+namespace sgpu {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void clear_error() { g_last_error.clear(); }
+sgpu_status set_error(sgpu_status status, const std::string& message) {
+  g_last_error = message;
+  return status;
+}
+
+namespace {
+
+enum KernelId {
+  K_LINEARIZE_POINTS = 0,
+  K_CAMERA_ROWS,
+  K_FACTOR,
+  K_TRIANGULAR,
+  K_BACKSUB,
+  K_UPDATE,
+  K_COST,
+  K_COUNT
+};
+const char* kKernelNames[K_COUNT] = {"point_prep",   "camera_rows", "rcs_factor", "rcs_solve",
+                                     "backsub",      "pose_update", "cost"};
+
+double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof(d));
+  return d;
+}
+
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  template <typename T>
+  T* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    SGPU_HIP(hipMalloc(&p, n * sizeof(T)));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& v, hipStream_t s) {
+    T* p = alloc<T>(v.size());
+    (void)s;  // one-time upload: blocking copy, so temporaries may die right after
+    if (!v.empty()) SGPU_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  ~DeviceArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+}  // namespace sgpu
+
+using namespace sgpu;
+
+struct sgpu_optimizer {
+  HostProblem P;
+  Symbolic S;
+  sgpu_opt_params params{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DeviceArena arena;
+  DevProblem dp{};
+  DevFactor df{};
+  DevLin dl{};
+  std::vector<DevPoseGraph> dpg;
+  double* poses[2] = {nullptr, nullptr};
+  double* points[2] = {nullptr, nullptr};
+  int cur = 0;
+  double* intr = nullptr;
+  double* cost_partial = nullptr;
+  DevScalars* d_sc = nullptr;
+  DevScalars* h_sc = nullptr;  // pinned
+  double merit = 0, nu = 0;
+  int64_t rejects = 0, steps = 0;
+  bool terminated = false;
+  int32_t last_termination = SGPU_TERM_NONE;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[2 * K_COUNT] = {};
+  double kernel_ms[K_COUNT] = {};
+  int64_t kernel_count[K_COUNT] = {};
+  hipEvent_t phase_ev[5] = {};
+
+  ~sgpu_optimizer() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : phase_ev)
+      if (e) (void)hipEventDestroy(e);
+    if (h_sc) (void)hipHostFree(h_sc);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void tic(int k) {
+    if (timing) SGPU_HIP(hipEventRecord(ev[2 * k], stream));
+  }
+  void toc(int k) {
+    if (timing) SGPU_HIP(hipEventRecord(ev[2 * k + 1], stream));
+  }
+  // accumulate after a stream sync; kernels recorded this step only
+  void collect(const bool* used) {
+    if (!timing) return;
+    for (int k = 0; k < K_COUNT; ++k) {
+      if (!used[k]) continue;
+      float ms = 0;
+      SGPU_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+      kernel_ms[k] += ms;
+      kernel_count[k] += 1;
+    }
+  }
+
+  const Family& pose_fam() const { return P.fams[P.f_pose]; }
+  const Family& point_fam() const { return P.fams[P.f_point]; }
+
+  void linearize(const double* ps, const double* pts, double nu_lin, bool* used) {
+    launch_zero_scalars(stream, d_sc);
+    launch_cam_cache(stream, S.npose, ps, dl.cw);
+    launch_fill(stream, df.C, int64_t(36) * (S.nb + df.nnz_l), 0.0);
+    tic(K_LINEARIZE_POINTS);
+    launch_point_prep(stream, params.precision, dp, ps, pts, intr, dl.cw, nu_lin, dl, d_sc);
+    toc(K_LINEARIZE_POINTS);
+    tic(K_CAMERA_ROWS);
+    launch_camera_rows(stream, params.precision, dp, pts, intr, dl.cw, nu_lin, dl, df, d_sc);
+    toc(K_CAMERA_ROWS);
+    for (auto& g : dpg) launch_pose_graph_linearize(stream, g, dp, ps, nu_lin, dl, df);
+    launch_max_grad(stream, dl.grad_cam, S.nb, d_sc);
+    used[K_LINEARIZE_POINTS] = used[K_CAMERA_ROWS] = true;
+  }
+
+  void cost_of(const double* ps, const double* pts, bool* used) {
+    tic(K_COST);
+    launch_cost(stream, dp, ps, pts, intr, cost_partial, 2 * kCostBlocks, d_sc);
+    toc(K_COST);
+    int nblk = kCostBlocks;
+    for (auto& g : dpg) {
+      launch_pose_graph_cost(stream, g, ps, cost_partial + 2 * nblk, 2 * kCostBlocks, d_sc);
+      nblk += kCostBlocks;
+    }
+    launch_finalize_cost(stream, cost_partial, nblk, d_sc);
+    if (used) used[K_COST] = true;
+  }
+
+  double compute_merit(const double* ps, const double* pts) {
+    bool used[K_COUNT] = {};
+    launch_zero_scalars(stream, d_sc);
+    cost_of(ps, pts, used);
+    SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipStreamSynchronize(stream));
+    collect(used);
+    return h_sc->cost;
+  }
+
+  void init(const sgpu_var_family* families, int64_t nf, const sgpu_cost_fn* cfs, int64_t nc,
+            const sgpu_opt_params* prm) {
+    params = *prm;
+    if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT)
+      throw Error(SGPU_ERR_UNSUPPORTED, "only the Schur block-sparse LDLt solver is available");
+    if (params.precision != SGPU_PRECISION_F64 && params.precision != SGPU_PRECISION_MIXED)
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown precision");
+    P = build_host_problem(families, nf, cfs, nc);
+    S = analyze(P);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      throw Error(SGPU_ERR_NO_DEVICE, "no HIP device");
+    device = params.device;
+    if (device < 0 || device >= ndev) throw Error(SGPU_ERR_NO_DEVICE, "device ordinal out of range");
+    SGPU_HIP(hipSetDevice(device));
+    SGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    SGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_sc), sizeof(DevScalars), hipHostMallocDefault));
+    for (auto& e : ev) SGPU_HIP(hipEventCreate(&e));
+    for (auto& e : phase_ev) SGPU_HIP(hipEventCreate(&e));
+    // variables
+    const Family& pf = pose_fam();
+    const Family& xf = point_fam();
+    const Family& kf = P.fams[P.f_intr];
+    for (int i = 0; i < 2; ++i) {
+      poses[i] = arena.alloc<double>(pf.members.size());
+      points[i] = arena.alloc<double>(xf.members.size());
+    }
+    SGPU_HIP(hipMemcpy(poses[0], pf.members.data(), pf.members.size() * 8, hipMemcpyHostToDevice));
+    SGPU_HIP(hipMemcpy(points[0], xf.members.data(), xf.members.size() * 8, hipMemcpyHostToDevice));
+    intr = arena.upload(kf.members, stream);
+    // problem arrays
+    std::vector<int32_t> pt_block(xf.block.begin(), xf.block.end());
+    dp.nb = S.nb;
+    dp.npt = S.npt;
+    dp.nobs = S.nobs;
+    dp.npose = S.npose;
+    dp.nintr = int32_t(kf.n);
+    dp.obs_pose = arena.upload(P.obs_pose, stream);
+    dp.obs_point = arena.upload(P.obs_point, stream);
+    dp.obs_intr = arena.upload(P.obs_intr, stream);
+    dp.obs_uv = arena.upload(P.obs_uv, stream);
+    dp.pt_ptr = arena.upload(S.pt_ptr, stream);
+    dp.pt_nu = arena.upload(S.pt_nu, stream);
+    dp.pt_block = arena.upload(pt_block, stream);
+    dp.cam_nu = arena.upload(S.cam_nu, stream);
+    dp.pose_block = arena.upload(S.pose_block, stream);
+    dp.cam_member = arena.upload(P.cam_member, stream);
+    dp.cam_ptr = arena.upload(S.cam_ptr, stream);
+    dp.cam_obs = arena.upload(S.cam_obs, stream);
+    dp.s_ptr = arena.upload(S.s_ptr, stream);
+    dp.s_col = arena.upload(S.s_col, stream);
+    dp.s_dest = arena.upload(S.s_dest, stream);
+    dp.robust = P.robust;
+    dp.delta = P.delta;
+    // factor structure
+    df.nb = S.nb;
+    df.nlev = int32_t(S.lvl_ptr.size()) - 1;
+    df.nnz_l = S.l_ptr[S.nb];
+    df.perm = arena.upload(S.perm, stream);
+    df.l_ptr = arena.upload(S.l_ptr, stream);
+    df.l_row = arena.upload(S.l_row, stream);
+    df.r_ptr = arena.upload(S.r_ptr, stream);
+    df.r_col = arena.upload(S.r_col, stream);
+    df.r_lpos = arena.upload(S.r_lpos, stream);
+    df.u_ptr = arena.upload(S.u_ptr, stream);
+    df.u_src = arena.upload(S.u_src, stream);
+    df.u_m = arena.upload(S.u_m, stream);
+    df.lvl_ptr = arena.upload(S.lvl_ptr, stream);
+    df.lvl_col = arena.upload(S.lvl_col, stream);
+    df.h_lvl_ptr = S.lvl_ptr.data();
+    df.C = arena.alloc<double>(size_t(36) * (S.nb + df.nnz_l));
+    df.M = arena.alloc<double>(size_t(36) * std::max<size_t>(1, S.r_col.size()));
+    df.Ljj = arena.alloc<double>(size_t(36) * S.nb);
+    df.Dfull = arena.alloc<double>(size_t(36) * S.nb);
+    df.dvec = arena.alloc<double>(size_t(6) * S.nb);
+    // workspace
+    dl.cw = arena.alloc<double>(size_t(12) * S.npose);
+    dl.hinv = arena.alloc<double>(size_t(9) * S.npt);
+    dl.pt_s = arena.alloc<double>(size_t(3) * S.npt);
+    dl.pt_y = arena.alloc<double>(size_t(3) * S.npt);
+    dl.rhs_nat = arena.alloc<double>(size_t(6) * S.nb);
+    dl.grad_cam = arena.alloc<double>(size_t(6) * S.nb);
+    dl.x_perm = arena.alloc<double>(size_t(6) * S.nb);
+    dl.x_aux = arena.alloc<double>(size_t(6) * S.nb);
+    dl.dx_cam = arena.alloc<double>(size_t(6) * S.nb);
+    dl.partial_cap = 4 * kCostBlocks;
+    dl.partial = arena.alloc<double>(dl.partial_cap);
+    cost_partial = arena.alloc<double>(size_t(2) * kCostBlocks * (1 + P.pg.size()));
+    d_sc = arena.alloc<DevScalars>(1);
+    for (size_t i = 0; i < P.pg.size(); ++i) {
+      const auto& g = P.pg[i];
+      DevPoseGraph q{};
+      q.n = int32_t(g.m.size());
+      q.m = arena.upload(g.m, stream);
+      q.nn = arena.upload(g.n, stream);
+      q.meas = arena.upload(g.meas, stream);
+      q.first = arena.upload(g.first, stream);
+      std::vector<int32_t> dest;
+      for (auto& d : S.pg_dest[i]) dest.insert(dest.end(), {d.mm, d.nn, d.mn});
+      q.dest = arena.upload(dest, stream);
+      q.robust = g.robust;
+      q.delta = g.delta;
+      dpg.push_back(q);
+    }
+    nu = params.initial_lm_damping;
+    merit = compute_merit(poses[0], points[0]);
+  }
+
+  void step(sgpu_step_info* info) {
+    if (terminated) throw Error(SGPU_ERR_TERMINATED, "optimizer already terminated");
+    const auto t0 = std::chrono::steady_clock::now();
+    bool used[K_COUNT] = {};
+    const int nxt = 1 - cur;
+    SGPU_HIP(hipEventRecord(phase_ev[0], stream));
+    linearize(poses[cur], points[cur], nu, used);
+    SGPU_HIP(hipEventRecord(phase_ev[1], stream));
+    tic(K_FACTOR);
+    for (int l = 0; l < df.nlev; ++l) launch_factor_level(stream, df, l, d_sc);
+    toc(K_FACTOR);
+    SGPU_HIP(hipEventRecord(phase_ev[2], stream));
+    tic(K_TRIANGULAR);
+    launch_triangular_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.x_aux, dl.dx_cam);
+    toc(K_TRIANGULAR);
+    tic(K_BACKSUB);
+    launch_backsub_points(stream, params.precision, dp, points[cur], intr, dl.cw, dl, points[nxt], d_sc);
+    toc(K_BACKSUB);
+    SGPU_HIP(hipMemcpyAsync(poses[nxt], poses[cur], pose_fam().members.size() * 8,
+                            hipMemcpyDeviceToDevice, stream));
+    tic(K_UPDATE);
+    launch_update_poses(stream, dp, poses[cur], dl, poses[nxt], d_sc);
+    toc(K_UPDATE);
+    launch_finalize_step(stream, dl, S.nb, S.npt, d_sc);
+    SGPU_HIP(hipEventRecord(phase_ev[3], stream));
+    cost_of(poses[nxt], points[nxt], used);
+    SGPU_HIP(hipEventRecord(phase_ev[4], stream));
+    SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipStreamSynchronize(stream));
+    used[K_FACTOR] = used[K_TRIANGULAR] = used[K_BACKSUB] = used[K_UPDATE] = true;
+    collect(used);
+    const DevScalars sc = *h_sc;
+    // LM control, step_nlls_inner order
+    int reason = SGPU_TERM_NONE;
+    int accepted = 0;
+    double new_merit = NAN, rho = NAN;
+    const double max_grad = bits_to_double(sc.max_grad_bits);
+    const double max_step = bits_to_double(sc.max_delta_bits);
+    if (max_grad < params.gradient_tolerance) {
+      reason = SGPU_TERM_GRADIENT_TOLERANCE;
+    } else if (sc.singular_marg || sc.bad_pivot) {
+      nu *= 2.0;  // handle_solve_failure -> reject_step
+      ++rejects;
+      if (rejects >= params.max_consecutive_rejects) reason = SGPU_TERM_MAX_CONSECUTIVE_REJECTS;
+    } else {
+      new_merit = sc.cost;
+      if (max_step < params.parameter_tolerance) {
+        if (new_merit < merit) {
+          cur = nxt;
+          merit = new_merit;
+          accepted = 1;
+        }
+        reason = SGPU_TERM_PARAMETER_TOLERANCE;
+      } else {
+        const double pred = nu * sc.dd + sc.dg;
+        const double actual = merit - new_merit;
+        rho = std::fabs(pred) > DBL_MIN ? actual / pred : (actual > 0 ? 1.0 : 0.0);
+        if (rho > 0.0 && new_merit < merit) {
+          const double f = 1.0 - std::pow(2.0 * rho - 1.0, 3);
+          nu *= std::max(f, 1.0 / 3.0);
+          rejects = 0;
+          const double rel = (merit - new_merit) / (std::fabs(merit) + DBL_MIN);
+          if (rel < params.function_tolerance) reason = SGPU_TERM_FUNCTION_TOLERANCE;
+          cur = nxt;
+          merit = new_merit;
+          accepted = 1;
+        } else {
+          nu *= 2.0;
+          ++rejects;
+          if (rejects >= params.max_consecutive_rejects) reason = SGPU_TERM_MAX_CONSECUTIVE_REJECTS;
+        }
+      }
+    }
+    ++steps;
+    if (steps >= params.num_iterations && reason == SGPU_TERM_NONE) reason = SGPU_TERM_MAX_ITERATIONS;
+    if (reason != SGPU_TERM_NONE) {
+      terminated = true;
+      last_termination = reason;
+    }
+    if (info) {
+      float a = 0, b = 0, c = 0, d = 0;
+      SGPU_HIP(hipEventElapsedTime(&a, phase_ev[0], phase_ev[1]));
+      SGPU_HIP(hipEventElapsedTime(&b, phase_ev[1], phase_ev[2]));
+      SGPU_HIP(hipEventElapsedTime(&c, phase_ev[2], phase_ev[3]));
+      SGPU_HIP(hipEventElapsedTime(&d, phase_ev[3], phase_ev[4]));
+      info->merit = merit;
+      info->smooth_cost = merit;
+      info->inner_step = steps - 1;
+      info->lm_damping = nu;
+      info->termination = reason;
+      info->accepted = accepted;
+      info->new_merit = new_merit;
+      info->gain_ratio = rho;
+      info->max_abs_neg_grad = max_grad;
+      info->max_abs_delta = max_step;
+      info->pcg_iterations = 0;
+      info->ms_linearize = a;
+      info->ms_factor = b;
+      info->ms_solve = c;
+      info->ms_cost = d;
+      info->ms_total =
+          std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+  }
+
+  void copy_members(int fam_sorted, double* out, int64_t capacity) {
+    const Family& f = P.fams[fam_sorted];
+    const int64_t need = int64_t(f.members.size());
+    if (capacity < need) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
+    if (fam_sorted == P.f_pose || fam_sorted == P.f_point) {
+      const double* src = fam_sorted == P.f_pose ? poses[cur] : points[cur];
+      SGPU_HIP(hipMemcpyAsync(out, src, need * 8, hipMemcpyDeviceToHost, stream));
+      SGPU_HIP(hipStreamSynchronize(stream));
+    } else {
+      std::memcpy(out, f.members.data(), need * 8);
+    }
+  }
+
+  double final_max_grad(double nu_lin) {
+    bool used[K_COUNT] = {};
+    linearize(poses[cur], points[cur], nu_lin, used);
+    SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipStreamSynchronize(stream));
+    collect(used);
+    return bits_to_double(h_sc->max_grad_bits);
+  }
+};
+
+extern "C" {
+
+int32_t sgpu_abi_version(void) { return SGPU_ABI_VERSION; }
+
+const char* sgpu_last_error(void) { return g_last_error.c_str(); }
+
+void sgpu_opt_params_default(sgpu_opt_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->num_iterations = 100;
+  p->initial_lm_damping = 10.0;
+  p->parallelize = 1;
+  p->solver = SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT;
+  p->skip_final_hessian = 0;
+  p->precision = SGPU_PRECISION_F64;
+  p->function_tolerance = 1e-10;
+  p->gradient_tolerance = 1e-14;
+  p->parameter_tolerance = 1e-12;
+  p->max_consecutive_rejects = 20;
+  p->pcg_relative_tolerance = 1e-8;
+  p->pcg_max_iterations = 500;
+  p->device = 0;
+}
+
+sgpu_status sgpu_optimizer_create(const sgpu_var_family* families, int64_t num_families,
+                                  const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                  const sgpu_opt_params* params, sgpu_optimizer** out) {
+  return guarded([&] {
+    if (!out || !params) throw Error(SGPU_ERR_INVALID_ARGUMENT, "params and out are required");
+    *out = nullptr;
+    auto opt = std::make_unique<sgpu_optimizer>();
+    opt->init(families, num_families, cost_fns, num_cost_fns, params);
+    *out = opt.release();
+  });
+}
+
+sgpu_status sgpu_optimizer_step(sgpu_optimizer* opt, sgpu_step_info* info) {
+  return guarded([&] {
+    if (!opt) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null optimizer");
+    SGPU_HIP(hipSetDevice(opt->device));
+    opt->step(info);
+  });
+}
+
+sgpu_status sgpu_optimizer_get_members(sgpu_optimizer* opt, const char* name, double* out,
+                                       int64_t capacity) {
+  return guarded([&] {
+    if (!opt || !name || !out) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    for (size_t i = 0; i < opt->P.fams.size(); ++i)
+      if (opt->P.fams[i].name == name) {
+        SGPU_HIP(hipSetDevice(opt->device));
+        opt->copy_members(int(i), out, capacity);
+        return;
+      }
+    throw Error(SGPU_ERR_UNKNOWN_FAMILY, std::string("unknown family '") + name + "'");
+  });
+}
+
+double sgpu_optimizer_merit(const sgpu_optimizer* opt) { return opt ? opt->merit : NAN; }
+double sgpu_optimizer_lm_damping(const sgpu_optimizer* opt) { return opt ? opt->nu : NAN; }
+
+sgpu_status sgpu_optimizer_eval_cost_at(sgpu_optimizer* opt, const double* const* members,
+                                        double* cost) {
+  return guarded([&] {
+    if (!opt || !members || !cost) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    SGPU_HIP(hipSetDevice(opt->device));
+    const int nxt = 1 - opt->cur;
+    for (size_t i = 0; i < opt->P.sorted_of_input.size(); ++i) {
+      const int s = opt->P.sorted_of_input[i];
+      const auto& f = opt->P.fams[s];
+      if (!members[i]) throw Error(SGPU_ERR_INVALID_ARGUMENT, "missing family values");
+      if (s == opt->P.f_pose)
+        SGPU_HIP(hipMemcpyAsync(opt->poses[nxt], members[i], f.members.size() * 8, hipMemcpyHostToDevice, opt->stream));
+      else if (s == opt->P.f_point)
+        SGPU_HIP(hipMemcpyAsync(opt->points[nxt], members[i], f.members.size() * 8, hipMemcpyHostToDevice, opt->stream));
+      else if (s == opt->P.f_intr &&
+               std::memcmp(members[i], f.members.data(), f.members.size() * 8) != 0)
+        throw Error(SGPU_ERR_UNSUPPORTED, "conditioned intrinsics differ from the handle's");
+    }
+    *cost = opt->compute_merit(opt->poses[nxt], opt->points[nxt]);
+  });
+}
+
+sgpu_status sgpu_optimizer_build_solution(sgpu_optimizer* opt, sgpu_solution* sol) {
+  return guarded([&] {
+    if (!opt || !sol) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    SGPU_HIP(hipSetDevice(opt->device));
+    sol->final_cost = opt->merit;
+    sol->num_steps = opt->steps;
+    sol->termination = opt->last_termination;
+    sol->final_max_abs_neg_grad = opt->final_max_grad(opt->params.skip_final_hessian ? opt->nu : 0.0);
+  });
+}
+
+sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* st) {
+  return guarded([&] {
+    if (!opt || !st) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    const Symbolic& S = opt->S;
+    st->num_free_blocks = S.nb;
+    st->num_marg_blocks = opt->P.num_marg_blocks;
+    st->num_observations = S.nobs;
+    int64_t npg = 0;
+    for (auto& g : opt->P.pg) npg += int64_t(g.m.size());
+    st->num_pose_terms = npg;
+    st->nnz_s_blocks = int64_t(S.s_col.size());
+    st->nnz_l_blocks = S.l_ptr[S.nb];
+    st->num_schur_pairs = S.num_schur_pairs;
+    st->num_levels = int64_t(S.lvl_ptr.size()) - 1;
+    st->factor_flops = S.factor_flops;
+  });
+}
+
+int32_t sgpu_optimizer_kernel_times(const sgpu_optimizer* opt, const char** names, int64_t* counts,
+                                    double* total_ms, int32_t capacity) {
+  if (!opt) return 0;
+  int32_t n = 0;
+  for (int k = 0; k < K_COUNT && n < capacity; ++k, ++n) {
+    if (names) names[n] = kKernelNames[k];
+    if (counts) counts[n] = opt->kernel_count[k];
+    if (total_ms) total_ms[n] = opt->kernel_ms[k];
+  }
+  return n;
+}
+
+void sgpu_optimizer_reset_kernel_times(sgpu_optimizer* opt) {
+  if (!opt) return;
+  for (int k = 0; k < K_COUNT; ++k) {
+    opt->kernel_ms[k] = 0;
+    opt->kernel_count[k] = 0;
+  }
+}
+
+void sgpu_optimizer_set_kernel_timing(sgpu_optimizer* opt, int32_t enable) {
+  if (opt) opt->timing = enable != 0;
+}
+
+void sgpu_optimizer_destroy(sgpu_optimizer* opt) { delete opt; }
+
+sgpu_status sgpu_optimize_nlls(const sgpu_var_family* families, int64_t num_families,
+                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                               const sgpu_opt_params* params, double* const* out_members,
+                               sgpu_solution* solution) {
+  return guarded([&] {
+    if (!params || !solution) throw Error(SGPU_ERR_INVALID_ARGUMENT, "params and solution required");
+    auto opt = std::make_unique<sgpu_optimizer>();
+    opt->init(families, num_families, cost_fns, num_cost_fns, params);
+    solution->initial_cost = opt->merit;
+    while (!opt->terminated) opt->step(nullptr);
+    solution->final_cost = opt->merit;
+    solution->num_steps = opt->steps;
+    solution->termination = opt->last_termination;
+    solution->final_max_abs_neg_grad =
+        opt->final_max_grad(params->skip_final_hessian ? opt->nu : 0.0);
+    if (out_members)
+      for (int64_t i = 0; i < num_families; ++i)
+        if (out_members[i]) {
+          const int s = opt->P.sorted_of_input[i];
+          opt->copy_members(s, out_members[i], int64_t(opt->P.fams[s].members.size()));
+        }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,561 @@
+// Problem validation, variable layout, term sorting and symbolic analysis.
+#include "problem.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <queue>
+
+namespace sgpu {
+namespace {
+
+int var_dof(int type) { return type == SGPU_VAR_ISOMETRY3 ? 6 : type == SGPU_VAR_VEC3 ? 3 : 4; }
+int var_params(int type) {
+  return type == SGPU_VAR_ISOMETRY3 ? 7 : type == SGPU_VAR_VEC3 ? 3 : 4;
+}
+
+[[noreturn]] void raise(sgpu_status s, const std::string& m) { throw Error(s, m); }
+
+}  // namespace
+
+HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_families,
+                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns) {
+  HostProblem P;
+  if (!families || num_families <= 0) raise(SGPU_ERR_INVALID_ARGUMENT, "no variable families");
+  if (!cost_fns || num_cost_fns <= 0) raise(SGPU_ERR_INVALID_ARGUMENT, "no cost functions");
+  std::vector<Family> in(num_families);
+  for (int64_t i = 0; i < num_families; ++i) {
+    const sgpu_var_family& s = families[i];
+    if (!s.name || s.type < 0 || s.type > 2 || s.kind < 0 || s.kind > 2 || s.num_members < 0 ||
+        (s.num_members > 0 && !s.members) || (s.num_const_ids > 0 && !s.const_ids))
+      raise(SGPU_ERR_INVALID_ARGUMENT, "bad variable family descriptor");
+    if (s.num_members > INT32_MAX / 8) raise(SGPU_ERR_UNSUPPORTED, "family too large");
+    Family& f = in[i];
+    f.name = s.name;
+    f.type = s.type;
+    f.kind = s.kind;
+    f.dof = var_dof(s.type);
+    f.nparams = var_params(s.type);
+    f.n = s.num_members;
+    f.members.assign(s.members, s.members + f.n * f.nparams);
+    f.is_const.assign(f.n, 0);
+    for (int64_t k = 0; k < s.num_const_ids; ++k) {
+      if (s.const_ids[k] < 0 || s.const_ids[k] >= f.n)
+        raise(SGPU_ERR_INDEX_OUT_OF_RANGE, "const id out of range in family '" + f.name + "'");
+      f.is_const[s.const_ids[k]] = 1;
+    }
+  }
+  std::vector<int> order(num_families);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return in[a].name < in[b].name; });
+  for (size_t k = 1; k < order.size(); ++k)
+    if (in[order[k]].name == in[order[k - 1]].name)
+      raise(SGPU_ERR_INVALID_ARGUMENT, "duplicate family name '" + in[order[k]].name + "'");
+  P.sorted_of_input.assign(num_families, 0);
+  for (int k = 0; k < num_families; ++k) {
+    P.sorted_of_input[order[k]] = k;
+    P.fams.push_back(std::move(in[order[k]]));
+  }
+  // layout: active members of Free / Marginalized families get block indices
+  int64_t free_scalars = 0, marg_scalars = 0;
+  for (auto& f : P.fams) {
+    f.block.assign(f.n, -1);
+    if (f.kind == SGPU_VAR_CONDITIONED) continue;
+    int32_t b = 0;
+    for (int64_t i = 0; i < f.n; ++i)
+      if (!f.is_const[i]) f.block[i] = b++;
+    f.num_active = b;
+    (f.kind == SGPU_VAR_FREE ? free_scalars : marg_scalars) += int64_t(b) * f.dof;
+  }
+  auto find = [&](const char* name) -> int {
+    for (size_t i = 0; i < P.fams.size(); ++i)
+      if (P.fams[i].name == name) return int(i);
+    return -1;
+  };
+  // validate_solver_config (optimizer.rs:649-717)
+  if (free_scalars == 0 || marg_scalars == 0)
+    raise(SGPU_ERR_SCHUR_MISSING_FREE_OR_MARG,
+          "Schur solver requires at least one active free and one active marginalized variable");
+  for (int64_t i = 0; i < num_cost_fns; ++i) {
+    const sgpu_cost_fn& c = cost_fns[i];
+    const int arity = c.kind == SGPU_COST_PINHOLE_REPROJECTION ? 3 : 2;
+    int marg = 0;
+    for (int a = 0; a < arity; ++a) {
+      if (!c.family_names[a]) raise(SGPU_ERR_INVALID_ARGUMENT, "missing cost family name");
+      const int fi = find(c.family_names[a]);
+      if (fi >= 0 && P.fams[fi].kind == SGPU_VAR_MARGINALIZED) ++marg;
+    }
+    if (marg > 1)
+      raise(SGPU_ERR_SCHUR_COST_CONNECTS_TWO_MARG_FAMILIES,
+            "cost term connects two marginalized families");
+  }
+  for (int64_t i = 0; i < num_cost_fns; ++i) {
+    const sgpu_cost_fn& c = cost_fns[i];
+    if (c.kind != SGPU_COST_PINHOLE_REPROJECTION && c.kind != SGPU_COST_POSE_GRAPH3)
+      raise(SGPU_ERR_INVALID_ARGUMENT, "unknown cost kind");
+    if (c.robust_kernel != SGPU_ROBUST_NONE && c.robust_kernel != SGPU_ROBUST_HUBER)
+      raise(SGPU_ERR_INVALID_ARGUMENT, "unknown robust kernel");
+    const int arity = c.kind == SGPU_COST_PINHOLE_REPROJECTION ? 3 : 2;
+    const int want[2][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+                            {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1}};
+    int fi[3] = {-1, -1, -1};
+    for (int a = 0; a < arity; ++a) {
+      fi[a] = find(c.family_names[a]);
+      if (fi[a] < 0)
+        raise(SGPU_ERR_UNKNOWN_FAMILY, std::string("unknown family '") + c.family_names[a] + "'");
+      if (P.fams[fi[a]].type != want[c.kind][a])
+        raise(SGPU_ERR_FAMILY_TYPE, std::string("family '") + c.family_names[a] +
+                                        "' has the wrong variable type");
+    }
+    if (c.num_terms <= 0 || !c.entity_indices || !c.measurements)
+      raise(SGPU_ERR_INVALID_ARGUMENT, "empty cost function");
+    if (c.num_terms > INT32_MAX / 4) raise(SGPU_ERR_UNSUPPORTED, "too many terms");
+    for (int64_t t = 0; t < c.num_terms; ++t)
+      for (int a = 0; a < arity; ++a) {
+        const int64_t v = c.entity_indices[t * arity + a];
+        if (v < 0 || v >= P.fams[fi[a]].n)
+          raise(SGPU_ERR_INDEX_OUT_OF_RANGE, "entity index out of range");
+      }
+    // The GPU path implements the BA layout: SE(3) poses Free, points Marginalized,
+    // pinhole intrinsics Conditioned.
+    if (c.kind == SGPU_COST_PINHOLE_REPROJECTION) {
+      if (P.has_reproj)
+        raise(SGPU_ERR_UNSUPPORTED, "more than one pinhole reprojection cost function");
+      if (P.fams[fi[0]].kind != SGPU_VAR_CONDITIONED || P.fams[fi[1]].kind != SGPU_VAR_FREE ||
+          P.fams[fi[2]].kind != SGPU_VAR_MARGINALIZED)
+        raise(SGPU_ERR_UNSUPPORTED,
+              "reprojection terms need Conditioned intrinsics, Free poses and Marginalized "
+              "points");
+      if ((P.f_pose >= 0 && P.f_pose != fi[1]))
+        raise(SGPU_ERR_UNSUPPORTED, "cost functions use different pose families");
+      P.has_reproj = true;
+      P.f_intr = fi[0];
+      P.f_pose = fi[1];
+      P.f_point = fi[2];
+      P.robust = c.robust_kernel;
+      P.delta = c.huber_delta;
+      // CostFn::sort: stable sort by (point, pose, intrinsics)
+      const int64_t nt = c.num_terms;
+      std::vector<int32_t> ord(nt);
+      std::iota(ord.begin(), ord.end(), 0);
+      const int64_t* E = c.entity_indices;
+      std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+        if (E[3 * x + 2] != E[3 * y + 2]) return E[3 * x + 2] < E[3 * y + 2];
+        if (E[3 * x + 1] != E[3 * y + 1]) return E[3 * x + 1] < E[3 * y + 1];
+        return E[3 * x] < E[3 * y];
+      });
+      P.obs_pose.resize(nt);
+      P.obs_point.resize(nt);
+      P.obs_intr.resize(nt);
+      P.obs_uv.resize(2 * nt);
+      for (int64_t t = 0; t < nt; ++t) {
+        const int32_t o = ord[t];
+        P.obs_intr[t] = int32_t(E[3 * o]);
+        P.obs_pose[t] = int32_t(E[3 * o + 1]);
+        P.obs_point[t] = int32_t(E[3 * o + 2]);
+        P.obs_uv[2 * t] = c.measurements[2 * o];
+        P.obs_uv[2 * t + 1] = c.measurements[2 * o + 1];
+      }
+    } else {
+      if (P.fams[fi[0]].kind != SGPU_VAR_FREE || fi[0] != fi[1])
+        raise(SGPU_ERR_UNSUPPORTED, "pose-graph terms need one Free SE(3) family");
+      if (P.f_pose >= 0 && P.f_pose != fi[0])
+        raise(SGPU_ERR_UNSUPPORTED, "cost functions use different pose families");
+      P.f_pose = fi[0];
+      PoseGraphFn g;
+      g.robust = c.robust_kernel;
+      g.delta = c.huber_delta;
+      const int64_t nt = c.num_terms;
+      std::vector<int32_t> ord(nt);
+      std::iota(ord.begin(), ord.end(), 0);
+      const int64_t* E = c.entity_indices;
+      std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+        if (E[2 * x] != E[2 * y]) return E[2 * x] < E[2 * y];
+        return E[2 * x + 1] < E[2 * y + 1];
+      });
+      for (int64_t t = 0; t < nt; ++t) {
+        const int32_t o = ord[t];
+        if (E[2 * o] == E[2 * o + 1])
+          raise(SGPU_ERR_UNSUPPORTED, "pose-graph term connects a pose to itself");
+        g.m.push_back(int32_t(E[2 * o]));
+        g.n.push_back(int32_t(E[2 * o + 1]));
+        for (int k = 0; k < 7; ++k) g.meas.push_back(c.measurements[7 * o + k]);
+        g.first.push_back(t == 0 || g.m[t] != g.m[t - 1] || g.n[t] != g.n[t - 1]);
+      }
+      P.pg.push_back(std::move(g));
+    }
+  }
+  // every Free / Marginalized family must be the pose / point family of this layout
+  for (size_t i = 0; i < P.fams.size(); ++i) {
+    const Family& f = P.fams[i];
+    if (f.kind == SGPU_VAR_FREE && int(i) != P.f_pose && f.num_active > 0)
+      raise(SGPU_ERR_UNSUPPORTED, "free family '" + f.name + "' is not the pose family");
+    if (f.kind == SGPU_VAR_MARGINALIZED && int(i) != P.f_point && f.num_active > 0)
+      raise(SGPU_ERR_UNSUPPORTED, "marginalized family '" + f.name + "' is not the point family");
+  }
+  if (P.f_point < 0) raise(SGPU_ERR_UNSUPPORTED, "no reprojection cost function");
+  P.num_free_blocks = P.fams[P.f_pose].num_active;
+  P.num_marg_blocks = P.fams[P.f_point].num_active;
+  const Family& fp = P.fams[P.f_pose];
+  P.cam_member.resize(P.num_free_blocks);
+  for (int64_t i = 0; i < fp.n; ++i)
+    if (fp.block[i] >= 0) P.cam_member[fp.block[i]] = int32_t(i);
+  return P;
+}
+
+// ---- nested dissection ----------------------------------------------------------------
+
+std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<int32_t>>& adj,
+                                       int32_t leaf_size) {
+  std::vector<int32_t> out;
+  out.reserve(n);
+  std::vector<int32_t> mark(n, -1), level(n, -1);
+  int32_t stamp = 0;
+  // explicit stack of node sets to avoid deep recursion
+  struct Task {
+    std::vector<int32_t> nodes;
+    std::vector<int32_t> sep;  // appended after both halves (pending separator)
+    int stage;
+  };
+  // Recursive lambda via std::function-free approach
+  std::vector<int32_t> all(n);
+  std::iota(all.begin(), all.end(), 0);
+  auto bfs = [&](const std::vector<int32_t>& nodes, int32_t tag, int32_t src,
+                 std::vector<int32_t>& orderv) {
+    // nodes carry mark == tag; returns BFS order, sets level[]
+    orderv.clear();
+    std::queue<int32_t> q;
+    for (int32_t v : nodes) level[v] = -1;
+    level[src] = 0;
+    q.push(src);
+    while (!q.empty()) {
+      int32_t v = q.front();
+      q.pop();
+      orderv.push_back(v);
+      for (int32_t u : adj[v])
+        if (mark[u] == tag && level[u] < 0) {
+          level[u] = level[v] + 1;
+          q.push(u);
+        }
+    }
+  };
+  // recursive worker
+  std::vector<int32_t> orderv;
+  std::function<void(std::vector<int32_t>&)> rec;
+  rec = [&](std::vector<int32_t>& nodes) {
+    if (nodes.empty()) return;
+    std::sort(nodes.begin(), nodes.end());
+    if (int32_t(nodes.size()) <= leaf_size) {
+      out.insert(out.end(), nodes.begin(), nodes.end());
+      return;
+    }
+    const int32_t tag = ++stamp;
+    for (int32_t v : nodes) mark[v] = tag;
+    bfs(nodes, tag, nodes[0], orderv);
+    if (orderv.size() < nodes.size()) {  // disconnected: split into components
+      std::vector<std::vector<int32_t>> comps;
+      std::vector<char> seen(0);
+      std::vector<int32_t> rest = nodes;
+      while (!rest.empty()) {
+        const int32_t t2 = ++stamp;
+        for (int32_t v : rest) mark[v] = t2;
+        bfs(rest, t2, rest[0], orderv);
+        comps.push_back(orderv);
+        std::vector<int32_t> nxt;
+        for (int32_t v : rest)
+          if (level[v] < 0) nxt.push_back(v);
+        rest.swap(nxt);
+      }
+      for (auto& c : comps) rec(c);
+      return;
+    }
+    // pseudo-peripheral start: two BFS sweeps
+    int32_t src = orderv.back();
+    bfs(nodes, tag, src, orderv);
+    src = orderv.back();
+    bfs(nodes, tag, src, orderv);
+    const int32_t maxlev = level[orderv.back()];
+    if (maxlev < 2) {  // (nearly) a clique: no useful separator
+      out.insert(out.end(), nodes.begin(), nodes.end());
+      return;
+    }
+    std::vector<int32_t> cnt(maxlev + 1, 0);
+    for (int32_t v : nodes) ++cnt[level[v]];
+    int32_t acc = 0, m = 1;
+    for (int32_t l = 0; l <= maxlev; ++l) {
+      acc += cnt[l];
+      if (2 * acc >= int32_t(nodes.size())) {
+        m = l;
+        break;
+      }
+    }
+    m = std::max(1, std::min(m, maxlev - 1));
+    std::vector<int32_t> left, right, sep;
+    for (int32_t v : nodes) {
+      if (level[v] < m)
+        left.push_back(v);
+      else if (level[v] > m)
+        right.push_back(v);
+      else
+        sep.push_back(v);
+    }
+    rec(left);
+    rec(right);
+    std::sort(sep.begin(), sep.end());
+    out.insert(out.end(), sep.begin(), sep.end());
+  };
+  rec(all);
+  return out;
+}
+
+// ---- symbolic analysis -------------------------------------------------------------------
+
+Symbolic analyze(const HostProblem& P) {
+  Symbolic S;
+  const Family& poses = P.fams[P.f_pose];
+  const Family& points = P.fams[P.f_point];
+  S.nb = int32_t(P.num_free_blocks);
+  S.npt = int32_t(points.n);
+  S.npose = int32_t(poses.n);
+  S.nobs = int32_t(P.obs_point.size());
+  S.pose_block = poses.block;
+  // point-major CSR and nu multiplicities (one per reduced (point, pose) term)
+  S.pt_ptr.assign(S.npt + 1, 0);
+  for (int32_t o = 0; o < S.nobs; ++o) ++S.pt_ptr[P.obs_point[o] + 1];
+  for (int32_t b = 0; b < S.npt; ++b) S.pt_ptr[b + 1] += S.pt_ptr[b];
+  S.pt_nu.assign(S.npt, 0);
+  S.cam_nu.assign(S.nb, 0);
+  for (int32_t o = 0; o < S.nobs; ++o) {
+    const bool first = o == 0 || P.obs_point[o] != P.obs_point[o - 1] ||
+                       P.obs_pose[o] != P.obs_pose[o - 1];
+    if (!first) continue;
+    ++S.pt_nu[P.obs_point[o]];
+    const int32_t c = poses.block[P.obs_pose[o]];
+    if (c >= 0) ++S.cam_nu[c];
+  }
+  // camera-major lists
+  S.cam_ptr.assign(S.nb + 1, 0);
+  for (int32_t o = 0; o < S.nobs; ++o) {
+    const int32_t c = poses.block[P.obs_pose[o]];
+    if (c >= 0) ++S.cam_ptr[c + 1];
+  }
+  for (int32_t c = 0; c < S.nb; ++c) S.cam_ptr[c + 1] += S.cam_ptr[c];
+  S.cam_obs.resize(S.cam_ptr[S.nb]);
+  {
+    std::vector<int32_t> fill(S.cam_ptr.begin(), S.cam_ptr.end() - 1);
+    for (int32_t o = 0; o < S.nobs; ++o) {
+      const int32_t c = poses.block[P.obs_pose[o]];
+      if (c >= 0) S.cam_obs[fill[c]++] = o;
+    }
+  }
+  // S pattern: diagonal, co-observed pairs of active points, pose-graph pairs
+  std::vector<std::vector<int32_t>> rows(S.nb);
+  for (int32_t c = 0; c < S.nb; ++c) rows[c].push_back(c);
+  std::vector<int32_t> cams;
+  for (int32_t b = 0; b < S.npt; ++b) {
+    if (points.block[b] < 0) continue;
+    cams.clear();
+    for (int32_t o = S.pt_ptr[b]; o < S.pt_ptr[b + 1]; ++o) {
+      const int32_t c = poses.block[P.obs_pose[o]];
+      if (c >= 0 && (cams.empty() || cams.back() != c)) cams.push_back(c);
+    }
+    for (size_t i = 0; i < cams.size(); ++i)
+      for (size_t j = 0; j < i; ++j) rows[cams[i]].push_back(cams[j]);
+    // pairs counted per (obs, obs) with cam_j <= cam_i, as the Schur kernel visits them
+    int64_t k = 0;
+    for (int32_t o = S.pt_ptr[b]; o < S.pt_ptr[b + 1]; ++o)
+      if (poses.block[P.obs_pose[o]] >= 0) ++k;
+    S.num_schur_pairs += k * (k + 1) / 2;
+  }
+  for (const auto& g : P.pg)
+    for (size_t t = 0; t < g.m.size(); ++t) {
+      const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
+      if (a >= 0 && b >= 0) rows[std::max(a, b)].push_back(std::min(a, b));
+    }
+  S.s_ptr.assign(S.nb + 1, 0);
+  for (int32_t c = 0; c < S.nb; ++c) {
+    auto& r = rows[c];
+    std::sort(r.begin(), r.end());
+    r.erase(std::unique(r.begin(), r.end()), r.end());
+    S.s_ptr[c + 1] = S.s_ptr[c] + int32_t(r.size());
+  }
+  S.s_col.resize(S.s_ptr[S.nb]);
+  for (int32_t c = 0; c < S.nb; ++c) std::copy(rows[c].begin(), rows[c].end(), &S.s_col[S.s_ptr[c]]);
+  analyze_rcs(S);
+  // pose-graph destinations
+  auto s_slot = [&](int32_t r, int32_t c) -> int32_t {
+    auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return int32_t(it - S.s_col.begin());
+  };
+  for (const auto& g : P.pg) {
+    std::vector<Symbolic::PgDest> d(g.m.size());
+    for (size_t t = 0; t < g.m.size(); ++t) {
+      const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
+      d[t].mm = a >= 0 ? S.s_dest[s_slot(a, a)] : -1;
+      d[t].nn = b >= 0 ? S.s_dest[s_slot(b, b)] : -1;
+      if (a >= 0 && b >= 0) {
+        // block (m, n) stored at lower (max, min); transpose when m is the column
+        const int32_t code = S.s_dest[s_slot(std::max(a, b), std::min(a, b))];
+        d[t].mn = a > b ? code : (code ^ 1);
+      } else {
+        d[t].mn = -1;
+      }
+    }
+    S.pg_dest.push_back(std::move(d));
+  }
+  return S;
+}
+
+void analyze_rcs(Symbolic& S) {
+  S.max_slots_per_row = 0;
+  for (int32_t c = 0; c < S.nb; ++c)
+    S.max_slots_per_row = std::max(S.max_slots_per_row, S.s_ptr[c + 1] - S.s_ptr[c]);
+  // ordering
+  std::vector<std::vector<int32_t>> adj(S.nb);
+  for (int32_t c = 0; c < S.nb; ++c)
+    for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k)
+      if (S.s_col[k] != c) {
+        adj[c].push_back(S.s_col[k]);
+        adj[S.s_col[k]].push_back(c);
+      }
+  S.perm = nested_dissection(S.nb, adj, 4);
+  S.perm_inv.assign(S.nb, 0);
+  for (int32_t i = 0; i < S.nb; ++i) S.perm_inv[S.perm[i]] = i;
+  // permuted lower pattern by column: upper[i] = columns j < i with A(i, j) != 0
+  std::vector<std::vector<int32_t>> acol(S.nb), upper(S.nb);
+  for (int32_t c = 0; c < S.nb; ++c)
+    for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k) {
+      const int32_t pi = S.perm_inv[c], pj = S.perm_inv[S.s_col[k]];
+      const int32_t i = std::max(pi, pj), j = std::min(pi, pj);
+      if (i != j) {
+        acol[j].push_back(i);
+        upper[i].push_back(j);
+      }
+    }
+  // elimination tree
+  std::vector<int32_t> parent(S.nb, -1), ancestor(S.nb, -1);
+  for (int32_t j = 0; j < S.nb; ++j) {
+    std::sort(upper[j].begin(), upper[j].end());
+    for (int32_t i0 : upper[j]) {
+      int32_t i = i0;
+      while (i != j) {
+        const int32_t next = ancestor[i];
+        ancestor[i] = j;
+        if (next < 0) {
+          parent[i] = j;
+          break;
+        }
+        i = next;
+      }
+    }
+  }
+  // column structure of L: struct(A(:,j)) U struct(L(:,child)) minus {<= j}
+  std::vector<std::vector<int32_t>> lcol(S.nb), children(S.nb);
+  for (int32_t j = 0; j < S.nb; ++j)
+    if (parent[j] >= 0) children[parent[j]].push_back(j);
+  for (int32_t j = 0; j < S.nb; ++j) {
+    std::vector<int32_t> s = acol[j];
+    for (int32_t c : children[j])
+      for (int32_t i : lcol[c])
+        if (i > j) s.push_back(i);
+    std::sort(s.begin(), s.end());
+    s.erase(std::unique(s.begin(), s.end()), s.end());
+    lcol[j].swap(s);
+  }
+  S.l_ptr.assign(S.nb + 1, 0);
+  for (int32_t j = 0; j < S.nb; ++j) S.l_ptr[j + 1] = S.l_ptr[j] + int32_t(lcol[j].size());
+  S.l_row.resize(S.l_ptr[S.nb]);
+  for (int32_t j = 0; j < S.nb; ++j) std::copy(lcol[j].begin(), lcol[j].end(), &S.l_row[S.l_ptr[j]]);
+  auto lpos = [&](int32_t i, int32_t j) -> int32_t {  // L index of block (i, j), i > j
+    auto b = S.l_row.begin() + S.l_ptr[j], e = S.l_row.begin() + S.l_ptr[j + 1];
+    auto it = std::lower_bound(b, e, i);
+    if (it == e || *it != i) return -1;
+    return int32_t(it - S.l_row.begin());
+  };
+  // reach lists (row structure of L), ascending k
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> reach(S.nb);
+  for (int32_t k = 0; k < S.nb; ++k)
+    for (int32_t p = S.l_ptr[k]; p < S.l_ptr[k + 1]; ++p) reach[S.l_row[p]].push_back({k, p});
+  S.r_ptr.assign(S.nb + 1, 0);
+  for (int32_t j = 0; j < S.nb; ++j) S.r_ptr[j + 1] = S.r_ptr[j] + int32_t(reach[j].size());
+  S.r_col.resize(S.r_ptr[S.nb]);
+  S.r_lpos.resize(S.r_ptr[S.nb]);
+  std::vector<int32_t> r_index_of_lpos(S.l_ptr[S.nb], -1);  // L(j,k) -> reach entry of j
+  for (int32_t j = 0; j < S.nb; ++j)
+    for (size_t e = 0; e < reach[j].size(); ++e) {
+      const int32_t at = S.r_ptr[j] + int32_t(e);
+      S.r_col[at] = reach[j][e].first;
+      S.r_lpos[at] = reach[j][e].second;
+      r_index_of_lpos[reach[j][e].second] = at;
+    }
+  // gather-form update lists: target (i, j) <- L(i, k) * M(j, k) for i, j in struct(L(:,k)), i >= j
+  const int32_t nslots = S.nb + S.l_ptr[S.nb];
+  std::vector<int32_t> ucount(nslots + 1, 0);
+  for (int32_t k = 0; k < S.nb; ++k) {
+    const int32_t m = S.l_ptr[k + 1] - S.l_ptr[k];
+    for (int32_t b = 0; b < m; ++b) {
+      const int32_t j = S.l_row[S.l_ptr[k] + b];
+      ucount[j + 1]++;  // diagonal target
+      for (int32_t a = b + 1; a < m; ++a) ucount[S.nb + lpos(S.l_row[S.l_ptr[k] + a], j) + 1]++;
+    }
+    S.factor_flops += double(m) * (m + 1) / 2.0 * 432.0 + double(m) * 432.0;
+  }
+  S.factor_flops += double(S.nb) * 300.0;
+  for (int32_t s = 0; s < nslots; ++s) ucount[s + 1] += ucount[s];
+  S.u_ptr = ucount;
+  S.u_src.resize(ucount[nslots]);
+  S.u_m.resize(ucount[nslots]);
+  {
+    std::vector<int32_t> fill(ucount.begin(), ucount.end() - 1);
+    for (int32_t k = 0; k < S.nb; ++k) {  // ascending k: each target's list is k-sorted
+      const int32_t m = S.l_ptr[k + 1] - S.l_ptr[k];
+      for (int32_t b = 0; b < m; ++b) {
+        const int32_t pj = S.l_ptr[k] + b;  // L(j, k)
+        const int32_t j = S.l_row[pj];
+        const int32_t mi = r_index_of_lpos[pj];
+        int32_t at = fill[j]++;
+        S.u_src[at] = pj;
+        S.u_m[at] = mi;
+        for (int32_t a = b + 1; a < m; ++a) {
+          const int32_t pi = S.l_ptr[k] + a;  // L(i, k)
+          const int32_t tgt = S.nb + lpos(S.l_row[pi], j);
+          at = fill[tgt]++;
+          S.u_src[at] = pi;
+          S.u_m[at] = mi;
+        }
+      }
+    }
+  }
+  // level schedule
+  std::vector<int32_t> level(S.nb, 0);
+  int32_t nlev = 0;
+  for (int32_t j = 0; j < S.nb; ++j) {
+    if (parent[j] >= 0) level[parent[j]] = std::max(level[parent[j]], level[j] + 1);
+    nlev = std::max(nlev, level[j] + 1);
+  }
+  S.lvl_ptr.assign(nlev + 1, 0);
+  for (int32_t j = 0; j < S.nb; ++j) ++S.lvl_ptr[level[j] + 1];
+  for (int32_t l = 0; l < nlev; ++l) S.lvl_ptr[l + 1] += S.lvl_ptr[l];
+  S.lvl_col.resize(S.nb);
+  {
+    std::vector<int32_t> fill(S.lvl_ptr.begin(), S.lvl_ptr.end() - 1);
+    for (int32_t j = 0; j < S.nb; ++j) S.lvl_col[fill[level[j]]++] = j;
+  }
+  // S slot -> unified destination (diag j, or nb + L index), transpose flag
+  S.s_dest.resize(S.s_col.size());
+  for (int32_t c = 0; c < S.nb; ++c)
+    for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k) {
+      const int32_t pi = S.perm_inv[c], pj = S.perm_inv[S.s_col[k]];
+      if (pi == pj) {
+        S.s_dest[k] = pi * 2;
+      } else if (pi > pj) {
+        S.s_dest[k] = (S.nb + lpos(pi, pj)) * 2;
+      } else {
+        S.s_dest[k] = (S.nb + lpos(pj, pi)) * 2 + 1;
+      }
+    }
+}
+
+}  // namespace sgpu

@@ -1,0 +1,205 @@
+// Stage-level C entry points: one kernel at a time on host arrays, for parity tests
+// of a single stage against the oracle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "api_internal.hpp"
+#include "kernels.hpp"
+#include "problem.hpp"
+
+using namespace sgpu;
+
+namespace {
+
+struct Buf {
+  void* p = nullptr;
+  ~Buf() {
+    if (p) (void)hipFree(p);
+  }
+  template <typename T>
+  T* get(size_t n) {
+    SGPU_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T*>(p);
+  }
+};
+
+template <typename T>
+T* upload(Buf& b, const T* host, size_t n) {
+  T* d = b.get<T>(n);
+  SGPU_HIP(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+struct DeviceBufs {
+  std::vector<void*> ptrs;
+  ~DeviceBufs() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename T>
+  T* alloc(size_t n) {
+    void* p = nullptr;
+    SGPU_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* up(const std::vector<T>& v) {
+    T* d = alloc<T>(v.size());
+    if (!v.empty()) SGPU_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+  }
+};
+
+void require_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) throw Error(SGPU_ERR_NO_DEVICE, "no HIP device");
+}
+
+}  // namespace
+
+extern "C" sgpu_status sgpu_eval_pinhole_terms(int32_t precision, int64_t n, const double* intr,
+                                               const double* pose, const double* pt,
+                                               const double* uv, int32_t robust, double delta,
+                                               double* res, double* jpose, double* jpt,
+                                               double* cost, double* weight) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!intr || !pose || !pt || !uv)))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return;
+    require_device();
+    Buf bi, bp, bx, bu, br, bj, bk, bc, bw;
+    const double* di = upload(bi, intr, 4 * n);
+    const double* dpo = upload(bp, pose, 7 * n);
+    const double* dx = upload(bx, pt, 3 * n);
+    const double* du = upload(bu, uv, 2 * n);
+    double* dr = br.get<double>(2 * n);
+    double* dj = bj.get<double>(12 * n);
+    double* dk = bk.get<double>(6 * n);
+    double* dc = bc.get<double>(n);
+    double* dw = bw.get<double>(n);
+    launch_eval_pinhole(nullptr, precision, n, di, dpo, dx, du, robust, delta, dr, dj, dk, dc, dw);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    if (res) SGPU_HIP(hipMemcpy(res, dr, 2 * n * 8, hipMemcpyDeviceToHost));
+    if (jpose) SGPU_HIP(hipMemcpy(jpose, dj, 12 * n * 8, hipMemcpyDeviceToHost));
+    if (jpt) SGPU_HIP(hipMemcpy(jpt, dk, 6 * n * 8, hipMemcpyDeviceToHost));
+    if (cost) SGPU_HIP(hipMemcpy(cost, dc, n * 8, hipMemcpyDeviceToHost));
+    if (weight) SGPU_HIP(hipMemcpy(weight, dw, n * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" sgpu_status sgpu_eval_pose_graph3_terms(int64_t n, const double* m, const double* nn,
+                                                   const double* mn, double* res, double* jm,
+                                                   double* jn) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!m || !nn || !mn || !res || !jm || !jn)))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return;
+    require_device();
+    Buf b1, b2, b3, b4, b5, b6;
+    const double* d1 = upload(b1, m, 7 * n);
+    const double* d2 = upload(b2, nn, 7 * n);
+    const double* d3 = upload(b3, mn, 7 * n);
+    double* d4 = b4.get<double>(6 * n);
+    double* d5 = b5.get<double>(36 * n);
+    double* d6 = b6.get<double>(36 * n);
+    launch_eval_pose_graph(nullptr, n, d1, d2, d3, d4, d5, d6);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    SGPU_HIP(hipMemcpy(res, d4, 6 * n * 8, hipMemcpyDeviceToHost));
+    SGPU_HIP(hipMemcpy(jm, d5, 36 * n * 8, hipMemcpyDeviceToHost));
+    SGPU_HIP(hipMemcpy(jn, d6, 36 * n * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
+                                             const int64_t* row_idx, const double* values,
+                                             const double* rhs, double* x) {
+  return guarded([&] {
+    if (num_blocks <= 0 || !col_ptr || !row_idx || !values || !rhs || !x)
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (num_blocks > INT32_MAX / 64) throw Error(SGPU_ERR_UNSUPPORTED, "too many blocks");
+    const int32_t nb = int32_t(num_blocks);
+    const int64_t nnz = col_ptr[nb];
+    if (col_ptr[0] != 0 || nnz < 0) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad col_ptr");
+    std::vector<std::vector<int32_t>> rows(nb);
+    for (int32_t c = 0; c < nb; ++c) {
+      if (col_ptr[c + 1] < col_ptr[c]) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad col_ptr");
+      rows[c].push_back(c);
+      for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
+        const int64_t r = row_idx[k];
+        if (r < c || r >= nb)
+          throw Error(SGPU_ERR_INDEX_OUT_OF_RANGE, "block row must satisfy col <= row < nb");
+        rows[r].push_back(c);
+      }
+    }
+    Symbolic S;
+    S.nb = nb;
+    S.s_ptr.assign(nb + 1, 0);
+    for (int32_t r = 0; r < nb; ++r) {
+      std::sort(rows[r].begin(), rows[r].end());
+      rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+      S.s_ptr[r + 1] = S.s_ptr[r] + int32_t(rows[r].size());
+    }
+    S.s_col.resize(S.s_ptr[nb]);
+    for (int32_t r = 0; r < nb; ++r) std::copy(rows[r].begin(), rows[r].end(), &S.s_col[S.s_ptr[r]]);
+    analyze_rcs(S);
+    require_device();
+    const int32_t nnz_l = S.l_ptr[nb];
+    std::vector<double> C(size_t(36) * (nb + nnz_l), 0.0);
+    for (int32_t c = 0; c < nb; ++c)
+      for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
+        const int32_t r = int32_t(row_idx[k]);
+        auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
+        const int32_t slot = int32_t(std::lower_bound(b, e, c) - S.s_col.begin());
+        const int32_t code = S.s_dest[slot];
+        double* dst = &C[size_t(36) * (code >> 1)];
+        for (int a = 0; a < 6; ++a)
+          for (int bb = 0; bb < 6; ++bb) {
+            const double v = values[36 * k + 6 * bb + a];  // input column-major
+            if (code & 1)
+              dst[6 * bb + a] += v;
+            else
+              dst[6 * a + bb] += v;
+          }
+      }
+    DeviceBufs bufs;
+    DevFactor df{};
+    df.nb = nb;
+    df.nlev = int32_t(S.lvl_ptr.size()) - 1;
+    df.nnz_l = nnz_l;
+    df.perm = bufs.up(S.perm);
+    df.l_ptr = bufs.up(S.l_ptr);
+    df.l_row = bufs.up(S.l_row);
+    df.r_ptr = bufs.up(S.r_ptr);
+    df.r_col = bufs.up(S.r_col);
+    df.r_lpos = bufs.up(S.r_lpos);
+    df.u_ptr = bufs.up(S.u_ptr);
+    df.u_src = bufs.up(S.u_src);
+    df.u_m = bufs.up(S.u_m);
+    df.lvl_ptr = bufs.up(S.lvl_ptr);
+    df.lvl_col = bufs.up(S.lvl_col);
+    df.h_lvl_ptr = S.lvl_ptr.data();
+    df.C = bufs.up(C);
+    df.M = bufs.alloc<double>(size_t(36) * std::max<size_t>(1, S.r_col.size()));
+    df.Ljj = bufs.alloc<double>(size_t(36) * nb);
+    df.Dfull = bufs.alloc<double>(size_t(36) * nb);
+    df.dvec = bufs.alloc<double>(size_t(6) * nb);
+    std::vector<double> b(rhs, rhs + 6 * num_blocks);
+    double* d_rhs = bufs.up(b);
+    double* d_y = bufs.alloc<double>(size_t(6) * nb);
+    double* d_x = bufs.alloc<double>(size_t(6) * nb);
+    DevScalars* d_sc = bufs.alloc<DevScalars>(1);
+    launch_zero_scalars(nullptr, d_sc);
+    for (int l = 0; l < df.nlev; ++l) launch_factor_level(nullptr, df, l, d_sc);
+    launch_triangular_solve(nullptr, df, d_rhs, d_y, nullptr, d_x);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    DevScalars sc;
+    SGPU_HIP(hipMemcpy(&sc, d_sc, sizeof(sc), hipMemcpyDeviceToHost));
+    if (sc.bad_pivot) throw Error(SGPU_ERR_NON_FINITE, "non-finite pivot");
+    SGPU_HIP(hipMemcpy(x, d_x, size_t(6) * nb * 8, hipMemcpyDeviceToHost));
+  });
+}

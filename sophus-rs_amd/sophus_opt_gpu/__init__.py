@@ -1,0 +1,757 @@
+"""Python mirror of sophus_opt's NLLS surface over libsophus_gpu's C ABI.
+
+Names follow the reference crate so that calling code reads the same:
+
+- ``VarKind`` (crates/sophus_opt/src/variables.rs:17-24)
+- ``VarFamily`` / ``VarBuilder`` / ``VarFamilies``
+  (crates/sophus_opt/src/variables/var_family.rs:15-46, var_builder.rs:36)
+- ``RobustKernel.huber`` (crates/sophus_opt/src/robust_kernel.rs:28-57)
+- ``CostFn`` with ``PinholeCameraReprojectionCostTerm`` / ``PoseGraph3CostTerm`` terms
+  (crates/sophus_opt/src/nlls/cost/cost_fn.rs:105-151,
+  crates/sophus_opt/src/nlls/functor_library/costs/pinhole_reprojection.rs,
+  crates/sophus_opt/src/nlls/functor_library/costs/pose_graph3.rs)
+- ``OptParams`` with the reference defaults (crates/sophus_opt/src/nlls/optimizer_types.rs:225-240)
+- ``optimize_nlls`` (crates/sophus_opt/src/nlls.rs:73-80), ``Optimizer`` (optimizer.rs:71-645)
+- ``NllsError`` variants (optimizer_types.rs:422-504), ``TerminationReason`` (:93-122)
+
+Every compute call goes to the HIP library; there is no CPU fallback.  Importing the
+module only loads the shared library (no GPU is touched until a solve is requested).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import math
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE.parent / "libsophus_gpu.so"
+HEADER_PATH = _HERE.parent.parent / "include" / "sophus_gpu.h"
+
+# ---- ABI structs (include/sophus_gpu.h) -------------------------------------------------
+
+
+class _VarFamily(C.Structure):
+    _fields_ = [
+        ("name", C.c_char_p),
+        ("type", C.c_int32),
+        ("kind", C.c_int32),
+        ("num_members", C.c_int64),
+        ("members", C.POINTER(C.c_double)),
+        ("num_const_ids", C.c_int64),
+        ("const_ids", C.POINTER(C.c_int64)),
+    ]
+
+
+class _CostFn(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("robust_kernel", C.c_int32),
+        ("huber_delta", C.c_double),
+        ("family_names", C.c_char_p * 3),
+        ("num_terms", C.c_int64),
+        ("entity_indices", C.POINTER(C.c_int64)),
+        ("measurements", C.POINTER(C.c_double)),
+    ]
+
+
+class _OptParams(C.Structure):
+    _fields_ = [
+        ("num_iterations", C.c_int64),
+        ("initial_lm_damping", C.c_double),
+        ("parallelize", C.c_int32),
+        ("solver", C.c_int32),
+        ("skip_final_hessian", C.c_int32),
+        ("precision", C.c_int32),
+        ("function_tolerance", C.c_double),
+        ("gradient_tolerance", C.c_double),
+        ("parameter_tolerance", C.c_double),
+        ("max_consecutive_rejects", C.c_int64),
+        ("pcg_relative_tolerance", C.c_double),
+        ("pcg_max_iterations", C.c_int64),
+        ("device", C.c_int32),
+        ("reserved0", C.c_int32),
+    ]
+
+
+class _StepInfo(C.Structure):
+    _fields_ = [
+        ("merit", C.c_double),
+        ("smooth_cost", C.c_double),
+        ("inner_step", C.c_int64),
+        ("lm_damping", C.c_double),
+        ("termination", C.c_int32),
+        ("accepted", C.c_int32),
+        ("new_merit", C.c_double),
+        ("gain_ratio", C.c_double),
+        ("max_abs_neg_grad", C.c_double),
+        ("max_abs_delta", C.c_double),
+        ("pcg_iterations", C.c_int64),
+        ("ms_linearize", C.c_double),
+        ("ms_factor", C.c_double),
+        ("ms_solve", C.c_double),
+        ("ms_cost", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+
+class _Solution(C.Structure):
+    _fields_ = [
+        ("initial_cost", C.c_double),
+        ("final_cost", C.c_double),
+        ("num_steps", C.c_int64),
+        ("termination", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("final_max_abs_neg_grad", C.c_double),
+    ]
+
+
+class _RcsStats(C.Structure):
+    _fields_ = [
+        ("num_free_blocks", C.c_int64),
+        ("num_marg_blocks", C.c_int64),
+        ("num_observations", C.c_int64),
+        ("num_pose_terms", C.c_int64),
+        ("nnz_s_blocks", C.c_int64),
+        ("nnz_l_blocks", C.c_int64),
+        ("num_schur_pairs", C.c_int64),
+        ("num_levels", C.c_int64),
+        ("factor_flops", C.c_double),
+    ]
+
+
+class _BaSpec(C.Structure):
+    _fields_ = [
+        ("layout", C.c_int32),
+        ("num_cameras", C.c_int32),
+        ("num_points", C.c_int64),
+        ("obs_per_point_ring", C.c_int32),
+        ("min_obs", C.c_int32),
+        ("max_obs", C.c_int32),
+        ("geometric_p", C.c_double),
+        ("pixel_noise", C.c_double),
+        ("rot_noise", C.c_double),
+        ("trans_noise", C.c_double),
+        ("point_noise", C.c_double),
+        ("step_length", C.c_double),
+        ("yaw_jitter", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+class _BaSizes(C.Structure):
+    _fields_ = [("num_cameras", C.c_int64), ("num_points", C.c_int64), ("num_obs", C.c_int64)]
+
+
+_DP = C.POINTER(C.c_double)
+_I64P = C.POINTER(C.c_int64)
+
+# Every entry point declared in include/sophus_gpu.h with its ctypes signature.
+_SIGNATURES = {
+    "sgpu_abi_version": (C.c_int32, []),
+    "sgpu_last_error": (C.c_char_p, []),
+    "sgpu_opt_params_default": (None, [C.POINTER(_OptParams)]),
+    "sgpu_optimize_nlls": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                                     C.POINTER(_OptParams), C.POINTER(_DP), C.POINTER(_Solution)]),
+    "sgpu_optimizer_create": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                                        C.POINTER(_OptParams), C.POINTER(C.c_void_p)]),
+    "sgpu_optimizer_step": (C.c_int, [C.c_void_p, C.POINTER(_StepInfo)]),
+    "sgpu_optimizer_get_members": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.c_int64]),
+    "sgpu_optimizer_merit": (C.c_double, [C.c_void_p]),
+    "sgpu_optimizer_lm_damping": (C.c_double, [C.c_void_p]),
+    "sgpu_optimizer_eval_cost_at": (C.c_int, [C.c_void_p, C.POINTER(_DP), _DP]),
+    "sgpu_optimizer_build_solution": (C.c_int, [C.c_void_p, C.POINTER(_Solution)]),
+    "sgpu_optimizer_rcs_stats": (C.c_int, [C.c_void_p, C.POINTER(_RcsStats)]),
+    "sgpu_optimizer_kernel_times": (C.c_int32, [C.c_void_p, C.POINTER(C.c_char_p), _I64P, _DP, C.c_int32]),
+    "sgpu_optimizer_reset_kernel_times": (None, [C.c_void_p]),
+    "sgpu_optimizer_set_kernel_timing": (None, [C.c_void_p, C.c_int32]),
+    "sgpu_optimizer_destroy": (None, [C.c_void_p]),
+    "sgpu_eval_pinhole_terms": (C.c_int, [C.c_int32, C.c_int64, _DP, _DP, _DP, _DP, C.c_int32, C.c_double,
+                                          _DP, _DP, _DP, _DP, _DP]),
+    "sgpu_eval_pose_graph3_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
+    "sgpu_generate_ba": (C.c_int, [C.POINTER(_BaSpec), C.POINTER(_BaSizes), _DP, _DP, _DP, _DP, _DP,
+                                   _I64P, _DP]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def library() -> C.CDLL:
+    """Load libsophus_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with sophus-rs_amd/build.py")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> List[str]:
+    return list(_SIGNATURES)
+
+
+# ---- errors ---------------------------------------------------------------------------------
+
+
+class NllsError(RuntimeError):
+    """Error raised by the NLLS entry points (NllsError, optimizer_types.rs:422-504)."""
+
+    status = -1
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+        self.message = message
+
+
+class InvalidArgument(NllsError):
+    pass
+
+
+class SchurMissingFreeOrMarg(NllsError):
+    pass
+
+
+class SchurCostConnectsTwoMargFamilies(NllsError):
+    pass
+
+
+class UnknownFamily(NllsError):
+    pass
+
+
+class FamilyTypeError(NllsError):
+    pass
+
+
+class IndexOutOfRange(NllsError):
+    pass
+
+
+class Unsupported(NllsError):
+    pass
+
+
+class DeviceError(NllsError):
+    pass
+
+
+class Terminated(NllsError):
+    pass
+
+
+_ERRORS = {
+    1: InvalidArgument,
+    2: SchurMissingFreeOrMarg,
+    3: SchurCostConnectsTwoMargFamilies,
+    4: UnknownFamily,
+    5: FamilyTypeError,
+    6: IndexOutOfRange,
+    7: Unsupported,
+    8: DeviceError,
+    9: DeviceError,
+    10: DeviceError,
+    11: NllsError,
+    12: Terminated,
+}
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = library().sgpu_last_error().decode(errors="replace")
+        raise _ERRORS.get(status, NllsError)(status, msg)
+
+
+# ---- problem description -----------------------------------------------------------------------
+
+
+class VarKind(enum.IntEnum):
+    Free = 0
+    Conditioned = 1
+    Marginalized = 2
+
+
+class VarType(enum.IntEnum):
+    Isometry3 = 0
+    Vec3 = 1
+    Pinhole = 2
+
+
+_PARAMS = {VarType.Isometry3: 7, VarType.Vec3: 3, VarType.Pinhole: 4}
+
+
+@dataclass
+class VarFamily:
+    """A family of variables of one type (VarFamily::new / new_with_const_ids)."""
+
+    kind: VarKind
+    type: VarType
+    members: np.ndarray
+    const_ids: Sequence[int] = ()
+
+    def __post_init__(self):
+        self.members = np.ascontiguousarray(self.members, dtype=np.float64).reshape(-1, _PARAMS[self.type])
+        self.const_ids = np.ascontiguousarray(np.asarray(self.const_ids, dtype=np.int64).reshape(-1))
+
+    @staticmethod
+    def new(kind: VarKind, type: VarType, members) -> "VarFamily":
+        return VarFamily(kind, type, members)
+
+    @staticmethod
+    def new_with_const_ids(kind: VarKind, type: VarType, members, const_ids) -> "VarFamily":
+        return VarFamily(kind, type, members, const_ids)
+
+
+class VarBuilder:
+    """VarBuilder::new().add_family(name, family).build() (var_builder.rs:36)."""
+
+    def __init__(self):
+        self.families: Dict[str, VarFamily] = {}
+
+    def add_family(self, name: str, family: VarFamily) -> "VarBuilder":
+        self.families[name] = family
+        return self
+
+    def build(self) -> "VarFamilies":
+        return VarFamilies(dict(self.families))
+
+
+@dataclass
+class VarFamilies:
+    collection: Dict[str, VarFamily]
+
+    def get_members(self, name: str) -> np.ndarray:
+        return self.collection[name].members
+
+
+@dataclass(frozen=True)
+class RobustKernel:
+    """RobustKernel::Huber(HuberKernel::new(delta)) (robust_kernel.rs:40-57)."""
+
+    delta: float
+
+    @staticmethod
+    def huber(delta: float) -> "RobustKernel":
+        return RobustKernel(float(delta))
+
+
+class CostKind(enum.IntEnum):
+    PinholeReprojection = 0
+    PoseGraph3 = 1
+
+
+@dataclass
+class CostFn:
+    """CostFn::new_boxed((), CostTerms::new(family_names, terms), robust_kernel).
+
+    Terms are stored vectorised: ``entity_indices`` (N x arity) and ``measurements``
+    (N x 2 pixel uv for reprojection, N x 7 pose_m_from_pose_n for the pose graph).
+    """
+
+    kind: CostKind
+    family_names: Sequence[str]
+    entity_indices: np.ndarray
+    measurements: np.ndarray
+    robust_kernel: Optional[RobustKernel] = None
+
+    def __post_init__(self):
+        arity = 3 if self.kind == CostKind.PinholeReprojection else 2
+        mlen = 2 if self.kind == CostKind.PinholeReprojection else 7
+        self.entity_indices = np.ascontiguousarray(self.entity_indices, dtype=np.int64).reshape(-1, arity)
+        self.measurements = np.ascontiguousarray(self.measurements, dtype=np.float64).reshape(-1, mlen)
+        if len(self.family_names) != arity:
+            raise ValueError(f"{self.kind.name} needs {arity} family names")
+
+    @staticmethod
+    def pinhole_reprojection(family_names, entity_indices, uv, robust_kernel=None) -> "CostFn":
+        """PinholeCameraReprojectionCostTerm terms: (intrinsics, world_from_camera, point)."""
+        return CostFn(CostKind.PinholeReprojection, family_names, entity_indices, uv, robust_kernel)
+
+    @staticmethod
+    def pose_graph3(family_names, entity_indices, pose_m_from_pose_n, robust_kernel=None) -> "CostFn":
+        """PoseGraph3CostTerm terms: (world_from_pose_m, world_from_pose_n)."""
+        return CostFn(CostKind.PoseGraph3, family_names, entity_indices, pose_m_from_pose_n, robust_kernel)
+
+
+class Solver(enum.IntEnum):
+    SchurBlockSparseLdlt = 0
+    SchurPcgJacobi = 1
+    SchurPcgLearned = 2
+
+
+class Precision(enum.IntEnum):
+    F64 = 0
+    Mixed = 1
+
+
+class TerminationReason(enum.IntEnum):
+    NoTermination = 0
+    MaxIterations = 1
+    FunctionTolerance = 2
+    GradientTolerance = 3
+    ParameterTolerance = 4
+    MaxConsecutiveRejects = 5
+
+
+@dataclass
+class OptParams:
+    """OptParams with the reference defaults (optimizer_types.rs:225-240)."""
+
+    num_iterations: int = 100
+    initial_lm_damping: float = 10.0
+    parallelize: bool = True
+    solver: Solver = Solver.SchurBlockSparseLdlt
+    skip_final_hessian: bool = False
+    function_tolerance: float = 1e-10
+    gradient_tolerance: float = 1e-14
+    parameter_tolerance: float = 1e-12
+    max_consecutive_rejects: int = 20
+    precision: Precision = Precision.F64
+    pcg_relative_tolerance: float = 1e-8
+    pcg_max_iterations: int = 500
+    device: int = 0
+
+    def _c(self) -> _OptParams:
+        p = _OptParams()
+        library().sgpu_opt_params_default(C.byref(p))
+        p.num_iterations = int(self.num_iterations)
+        p.initial_lm_damping = float(self.initial_lm_damping)
+        p.parallelize = int(bool(self.parallelize))
+        p.solver = int(self.solver)
+        p.skip_final_hessian = int(bool(self.skip_final_hessian))
+        p.precision = int(self.precision)
+        p.function_tolerance = float(self.function_tolerance)
+        p.gradient_tolerance = float(self.gradient_tolerance)
+        p.parameter_tolerance = float(self.parameter_tolerance)
+        p.max_consecutive_rejects = int(self.max_consecutive_rejects)
+        p.pcg_relative_tolerance = float(self.pcg_relative_tolerance)
+        p.pcg_max_iterations = int(self.pcg_max_iterations)
+        p.device = int(self.device)
+        return p
+
+
+@dataclass
+class StepInfo:
+    merit: float
+    smooth_cost: float
+    inner_step: int
+    lm_damping: float
+    termination: Optional[TerminationReason]
+    accepted: bool
+    new_merit: float
+    gain_ratio: float
+    max_abs_neg_grad: float
+    max_abs_delta: float
+    pcg_iterations: int
+    ms_linearize: float
+    ms_factor: float
+    ms_solve: float
+    ms_cost: float
+    ms_total: float
+
+
+@dataclass
+class OptimizationSolution:
+    variables: VarFamilies
+    initial_cost: float
+    final_cost: float
+    num_steps: int
+    termination: TerminationReason
+    final_max_abs_neg_grad: float
+
+
+class _Marshal:
+    """Keeps the ctypes views of one problem alive for the duration of a call."""
+
+    def __init__(self, variables: VarFamilies, cost_fns: Sequence[CostFn]):
+        self.names = list(variables.collection)
+        self.keep = []
+        fams = (_VarFamily * len(self.names))()
+        for i, name in enumerate(self.names):
+            f = variables.collection[name]
+            nm = name.encode()
+            self.keep += [nm, f.members, f.const_ids]
+            fams[i].name = nm
+            fams[i].type = int(f.type)
+            fams[i].kind = int(f.kind)
+            fams[i].num_members = f.members.shape[0]
+            fams[i].members = f.members.ctypes.data_as(_DP)
+            fams[i].num_const_ids = f.const_ids.size
+            fams[i].const_ids = f.const_ids.ctypes.data_as(_I64P)
+        costs = (_CostFn * len(cost_fns))()
+        for i, c in enumerate(cost_fns):
+            costs[i].kind = int(c.kind)
+            costs[i].robust_kernel = 1 if c.robust_kernel is not None else 0
+            costs[i].huber_delta = c.robust_kernel.delta if c.robust_kernel is not None else 0.0
+            for a, n in enumerate(c.family_names):
+                b = n.encode()
+                self.keep.append(b)
+                costs[i].family_names[a] = b
+            costs[i].num_terms = c.entity_indices.shape[0]
+            costs[i].entity_indices = c.entity_indices.ctypes.data_as(_I64P)
+            costs[i].measurements = c.measurements.ctypes.data_as(_DP)
+            self.keep += [c.entity_indices, c.measurements]
+        self.fams, self.costs = fams, costs
+
+
+def _termination(v: int) -> Optional[TerminationReason]:
+    return None if v == 0 else TerminationReason(v)
+
+
+class Optimizer:
+    """Stateful LM optimizer on the GPU (Optimizer::new / step / build_solution)."""
+
+    def __init__(self, variables: VarFamilies, cost_fns: Sequence[CostFn], params: Optional[OptParams] = None):
+        self.params = params or OptParams()
+        self._variables = variables
+        self._m = _Marshal(variables, cost_fns)
+        h = C.c_void_p()
+        _check(library().sgpu_optimizer_create(self._m.fams, len(self._m.names), self._m.costs,
+                                               len(cost_fns), C.byref(self.params._c()), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            library().sgpu_optimizer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self) -> StepInfo:
+        s = _StepInfo()
+        _check(library().sgpu_optimizer_step(self._h, C.byref(s)))
+        return StepInfo(s.merit, s.smooth_cost, s.inner_step, s.lm_damping, _termination(s.termination),
+                        bool(s.accepted), s.new_merit, s.gain_ratio, s.max_abs_neg_grad, s.max_abs_delta,
+                        s.pcg_iterations, s.ms_linearize, s.ms_factor, s.ms_solve, s.ms_cost, s.ms_total)
+
+    def merit(self) -> float:
+        return library().sgpu_optimizer_merit(self._h)
+
+    def lm_damping(self) -> float:
+        return library().sgpu_optimizer_lm_damping(self._h)
+
+    def members(self, name: str) -> np.ndarray:
+        f = self._variables.collection[name]
+        out = np.empty_like(f.members)
+        _check(library().sgpu_optimizer_get_members(self._h, name.encode(), out.ctypes.data_as(_DP), out.size))
+        return out
+
+    def variables(self) -> VarFamilies:
+        return VarFamilies({n: VarFamily(f.kind, f.type, self.members(n), f.const_ids)
+                            for n, f in self._variables.collection.items()})
+
+    def eval_cost_at(self, variables: VarFamilies) -> float:
+        arrs = [np.ascontiguousarray(variables.collection[n].members, dtype=np.float64) for n in self._m.names]
+        ptrs = (_DP * len(arrs))(*[a.ctypes.data_as(_DP) for a in arrs])
+        out = C.c_double()
+        _check(library().sgpu_optimizer_eval_cost_at(self._h, ptrs, C.byref(out)))
+        return out.value
+
+    def rcs_stats(self) -> dict:
+        s = _RcsStats()
+        _check(library().sgpu_optimizer_rcs_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
+
+    def set_kernel_timing(self, enable: bool) -> None:
+        library().sgpu_optimizer_set_kernel_timing(self._h, int(enable))
+
+    def reset_kernel_times(self) -> None:
+        library().sgpu_optimizer_reset_kernel_times(self._h)
+
+    def kernel_times(self) -> Dict[str, tuple]:
+        n = 16
+        names = (C.c_char_p * n)()
+        counts = (C.c_int64 * n)()
+        ms = (C.c_double * n)()
+        k = library().sgpu_optimizer_kernel_times(self._h, names, counts, ms, n)
+        return {names[i].decode(): (int(counts[i]), float(ms[i])) for i in range(k)}
+
+    def build_solution(self, initial_cost: float = math.nan) -> OptimizationSolution:
+        s = _Solution()
+        _check(library().sgpu_optimizer_build_solution(self._h, C.byref(s)))
+        return OptimizationSolution(self.variables(), initial_cost, s.final_cost, s.num_steps,
+                                    TerminationReason(s.termination), s.final_max_abs_neg_grad)
+
+
+def optimize_nlls(variables: VarFamilies, cost_fns: Sequence[CostFn],
+                  params: Optional[OptParams] = None) -> OptimizationSolution:
+    """optimize_nlls(variables, cost_fns, vec![], params) on the GPU (nlls.rs:73-80)."""
+    params = params or OptParams()
+    m = _Marshal(variables, cost_fns)
+    outs = [np.empty_like(variables.collection[n].members) for n in m.names]
+    ptrs = (_DP * len(outs))(*[o.ctypes.data_as(_DP) for o in outs])
+    sol = _Solution()
+    _check(library().sgpu_optimize_nlls(m.fams, len(m.names), m.costs, len(cost_fns),
+                                        C.byref(params._c()), ptrs, C.byref(sol)))
+    fams = {n: VarFamily(variables.collection[n].kind, variables.collection[n].type, o,
+                         variables.collection[n].const_ids) for n, o in zip(m.names, outs)}
+    return OptimizationSolution(VarFamilies(fams), sol.initial_cost, sol.final_cost, sol.num_steps,
+                                TerminationReason(sol.termination), sol.final_max_abs_neg_grad)
+
+
+# ---- stage-level entry points ---------------------------------------------------------------
+
+
+def eval_pinhole_terms(intrinsics, poses, points, uv, robust_kernel: Optional[RobustKernel] = None,
+                       precision: Precision = Precision.F64) -> dict:
+    """Kernel (a) per term on the GPU: residual, J_pose (2x6), J_point (2x3), cost, weight."""
+    intr = np.ascontiguousarray(intrinsics, dtype=np.float64).reshape(-1, 4)
+    T = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 7)
+    x = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    z = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    n = z.shape[0]
+    out = {"residual": np.empty((n, 2)), "jac_pose": np.empty((n, 2, 6)), "jac_point": np.empty((n, 2, 3)),
+           "cost": np.empty(n), "weight": np.empty(n)}
+    _check(library().sgpu_eval_pinhole_terms(
+        int(precision), n, intr.ctypes.data_as(_DP), T.ctypes.data_as(_DP), x.ctypes.data_as(_DP),
+        z.ctypes.data_as(_DP), 1 if robust_kernel else 0, robust_kernel.delta if robust_kernel else 0.0,
+        *[out[k].ctypes.data_as(_DP) for k in ("residual", "jac_pose", "jac_point", "cost", "weight")]))
+    return out
+
+
+def eval_pose_graph3_terms(world_from_m, world_from_n, m_from_n) -> dict:
+    """Kernel (a') per term on the GPU: residual (6), J_m, J_n (6x6)."""
+    a = np.ascontiguousarray(world_from_m, dtype=np.float64).reshape(-1, 7)
+    b = np.ascontiguousarray(world_from_n, dtype=np.float64).reshape(-1, 7)
+    c = np.ascontiguousarray(m_from_n, dtype=np.float64).reshape(-1, 7)
+    n = a.shape[0]
+    out = {"residual": np.empty((n, 6)), "jac_m": np.empty((n, 6, 6)), "jac_n": np.empty((n, 6, 6))}
+    _check(library().sgpu_eval_pose_graph3_terms(n, a.ctypes.data_as(_DP), b.ctypes.data_as(_DP),
+                                                 c.ctypes.data_as(_DP),
+                                                 *[out[k].ctypes.data_as(_DP) for k in ("residual", "jac_m", "jac_n")]))
+    return out
+
+
+def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Kernel (d): block-sparse LDLt solve of a symmetric matrix with 6x6 blocks on the GPU.
+
+    Structurally zero 6x6 blocks of ``A_dense`` are skipped.
+    """
+    A = np.asarray(A_dense, dtype=np.float64)
+    n = A.shape[0]
+    if n % 6:
+        raise ValueError("dimension must be a multiple of 6")
+    nb = n // 6
+    col_ptr, rows, vals = [0], [], []
+    for c in range(nb):
+        for r in range(c, nb):
+            blk = A[6 * r:6 * r + 6, 6 * c:6 * c + 6]
+            if r == c or np.any(blk != 0):
+                rows.append(r)
+                vals.append(blk.T.copy())  # column-major block
+        col_ptr.append(len(rows))
+    col_ptr = np.asarray(col_ptr, dtype=np.int64)
+    rows = np.asarray(rows, dtype=np.int64)
+    vals = np.ascontiguousarray(np.stack(vals), dtype=np.float64)
+    rhs = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+    x = np.empty(n)
+    _check(library().sgpu_block_ldlt_solve(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P),
+                                           vals.ctypes.data_as(_DP), rhs.ctypes.data_as(_DP),
+                                           x.ctypes.data_as(_DP)))
+    return x
+
+
+# ---- synthetic problems ------------------------------------------------------------------------
+
+
+@dataclass
+class BaSpec:
+    layout: int = 0
+    num_cameras: int = 10
+    num_points: int = 200
+    obs_per_point_ring: int = 5
+    min_obs: int = 2
+    max_obs: int = 30
+    geometric_p: float = 0.1
+    pixel_noise: float = 0.5
+    rot_noise: float = 0.05
+    trans_noise: float = 0.1
+    point_noise: float = 0.1
+    step_length: float = 0.3
+    yaw_jitter: float = math.radians(5.0)
+    seed: int = 0
+
+    def _c(self) -> _BaSpec:
+        s = _BaSpec()
+        for k, _ in _BaSpec._fields_:
+            setattr(s, k, getattr(self, k))
+        return s
+
+
+@dataclass
+class BaProblem:
+    intrinsics: np.ndarray
+    poses_gt: np.ndarray
+    poses_init: np.ndarray
+    points_gt: np.ndarray
+    points_init: np.ndarray
+    obs_idx: np.ndarray
+    obs_uv: np.ndarray
+
+    @property
+    def num_obs(self) -> int:
+        return self.obs_uv.shape[0]
+
+    def variables(self, poses=None, points=None) -> VarFamilies:
+        """Families 'cams' (Conditioned), 'points' (Marginalized), 'poses' (Free, camera 0 fixed)."""
+        return (VarBuilder()
+                .add_family("cams", VarFamily(VarKind.Conditioned, VarType.Pinhole, self.intrinsics))
+                .add_family("poses", VarFamily(VarKind.Free, VarType.Isometry3,
+                                               self.poses_init if poses is None else poses, [0]))
+                .add_family("points", VarFamily(VarKind.Marginalized, VarType.Vec3,
+                                                self.points_init if points is None else points))
+                .build())
+
+    def cost_fns(self, huber_delta: Optional[float] = 1.0) -> List[CostFn]:
+        rk = RobustKernel.huber(huber_delta) if huber_delta is not None else None
+        return [CostFn.pinhole_reprojection(["cams", "poses", "points"], self.obs_idx, self.obs_uv, rk)]
+
+
+def generate_ba(spec: BaSpec) -> BaProblem:
+    """Seeded synthetic BA problem (host code of the library)."""
+    lib = library()
+    cs = spec._c()
+    sizes = _BaSizes()
+    _check(lib.sgpu_generate_ba(C.byref(cs), C.byref(sizes), *([None] * 6), None))
+    nc, npt, no = sizes.num_cameras, sizes.num_points, sizes.num_obs
+    intr = np.empty(4)
+    pg, pi = np.empty((nc, 7)), np.empty((nc, 7))
+    xg, xi = np.empty((npt, 3)), np.empty((npt, 3))
+    idx = np.empty((no, 3), dtype=np.int64)
+    uv = np.empty((no, 2))
+    _check(lib.sgpu_generate_ba(C.byref(cs), C.byref(sizes), intr.ctypes.data_as(_DP), pg.ctypes.data_as(_DP),
+                                pi.ctypes.data_as(_DP), xg.ctypes.data_as(_DP), xi.ctypes.data_as(_DP),
+                                idx.ctypes.data_as(_I64P), uv.ctypes.data_as(_DP)))
+    return BaProblem(intr.reshape(1, 4), pg, pi, xg, xi, idx, uv)
+
+
+def config_spec(index: int, seed: int = 0) -> BaSpec:
+    """The workloads: 0 = 10-camera ring / 200 points, 1 = 500-camera walk / 100k points,
+    2 = 2000-camera walk / 800k points (about 6M observations)."""
+    if index == 0:
+        return BaSpec(layout=0, num_cameras=10, num_points=200, obs_per_point_ring=5, seed=seed)
+    if index == 1:
+        return BaSpec(layout=1, num_cameras=500, num_points=100_000, geometric_p=0.1, seed=seed)
+    if index == 2:
+        return BaSpec(layout=1, num_cameras=2000, num_points=800_000, geometric_p=0.135, seed=seed)
+    raise ValueError(index)

@@ -1,0 +1,214 @@
+"""GPU parity tests: the HIP path against the FP64 oracle on the same seeded inputs.
+
+Tolerances: FP64 kernels 1e-10 relative (FMA contraction and summation order differ
+from the oracle); full LM runs: final cost 1e-9 relative, parameters 1e-7; the FP32
+mixed path: final cost 1e-6 relative to the FP64 oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(1e-300, float(np.max(np.abs(b)))))
+
+
+def _random_terms(n, seed):
+    rng = np.random.default_rng(seed)
+    intr = np.tile([500.0, 505.0, 320.0, 240.0], (n, 1))
+    w = rng.normal(0, 0.3, (n, 3))
+    t = rng.normal(0, 1.0, (n, 3))
+    import oracle
+
+    poses = np.stack([oracle.se3_exp(np.concatenate([w[i], t[i]])) for i in range(n)])
+    # points in front of each camera
+    pts = []
+    for i in range(n):
+        xc = np.array([rng.uniform(-1, 1), rng.uniform(-1, 1), rng.uniform(2, 8)])
+        R = oracle.so3_matrix(poses[i][:4])
+        pts.append(R @ xc + poses[i][4:])
+    pts = np.array(pts)
+    uv = rng.uniform([0, 0], [640, 480], (n, 2))
+    return intr, poses, pts, uv
+
+
+def test_pinhole_kernel_fp64_matches_oracle(sg, orc):
+    intr, poses, pts, uv = _random_terms(512, 1)
+    for delta in (None, 1.0, 50.0):
+        rk = sg.RobustKernel.huber(delta) if delta is not None else None
+        g = sg.eval_pinhole_terms(intr, poses, pts, uv, rk)
+        o = orc.eval_pinhole_terms(intr, poses, pts, uv, delta)
+        for k in ("residual", "jac_pose", "jac_point", "cost", "weight"):
+            assert _rel(g[k], o[k]) < 1e-10, (k, delta, _rel(g[k], o[k]))
+
+
+def test_pinhole_kernel_fp32_close_to_oracle(sg, orc):
+    intr, poses, pts, uv = _random_terms(512, 2)
+    g = sg.eval_pinhole_terms(intr, poses, pts, uv, None, sg.Precision.Mixed)
+    o = orc.eval_pinhole_terms(intr, poses, pts, uv, None)
+    assert _rel(g["residual"], o["residual"]) < 1e-12  # residual stays FP64
+    assert _rel(g["jac_pose"], o["jac_pose"]) < 1e-5
+    assert _rel(g["jac_point"], o["jac_point"]) < 1e-5
+
+
+def test_pose_graph_kernel_matches_oracle(sg, orc):
+    rng = np.random.default_rng(3)
+    n = 256
+    m = np.stack([orc.se3_exp(rng.normal(0, 0.5, 6)) for _ in range(n)])
+    nn = np.stack([orc.se3_exp(rng.normal(0, 0.5, 6)) for _ in range(n)])
+    # measurements near the true relative pose (small residuals hit the series branches)
+    mn = []
+    for i in range(n):
+        rel = orc.se3_mul(orc.se3_inverse(m[i]), nn[i])
+        scale = 1e-4 if i % 2 else 0.2
+        mn.append(orc.se3_mul(rel, orc.se3_exp(rng.normal(0, scale, 6))))
+    mn = np.array(mn)
+    g = sg.eval_pose_graph3_terms(m, nn, mn)
+    r, jm, jn = orc.eval_pose_graph3_terms(m, nn, mn)
+    assert _rel(g["residual"], r) < 1e-10
+    assert _rel(g["jac_m"], jm) < 1e-10
+    assert _rel(g["jac_n"], jn) < 1e-10
+
+
+def _spd_block_matrix(nb, band, seed, extra=0):
+    rng = np.random.default_rng(seed)
+    n = 6 * nb
+    A = np.zeros((n, n))
+    for i in range(nb):
+        for j in range(max(0, i - band), i + 1):
+            B = rng.normal(0, 1, (6, 6))
+            A[6 * i:6 * i + 6, 6 * j:6 * j + 6] += B
+            if i != j:
+                A[6 * j:6 * j + 6, 6 * i:6 * i + 6] += B.T
+    for _ in range(extra):
+        i, j = sorted(rng.integers(0, nb, 2))
+        if i != j:
+            B = rng.normal(0, 1, (6, 6))
+            A[6 * j:6 * j + 6, 6 * i:6 * i + 6] += B
+            A[6 * i:6 * i + 6, 6 * j:6 * j + 6] += B.T
+    A = 0.5 * (A + A.T)
+    A += np.eye(n) * (np.abs(A).sum(axis=1).max() + 1.0)
+    return A
+
+
+@pytest.mark.parametrize("nb,band,extra", [(1, 0, 0), (7, 2, 0), (60, 5, 20), (300, 12, 50)])
+def test_block_ldlt_kernel_matches_dense_solve(sg, orc, nb, band, extra):
+    A = _spd_block_matrix(nb, band, nb, extra)
+    b = np.random.default_rng(nb).normal(size=6 * nb)
+    x = sg.block_ldlt_solve(A, b)
+    x_np = np.linalg.solve(A, b)
+    assert _rel(x, x_np) < 1e-10
+    x_or = orc.block_ldlt_solve_dense(A, [6] * nb, b)
+    assert _rel(x, x_or) < 1e-10
+
+
+def test_block_ldlt_kernel_zero_pivot_rule(sg, orc):
+    # rank-deficient PSD block diagonal: zero pivots solve to 0 in both implementations
+    nb = 4
+    rng = np.random.default_rng(5)
+    A = np.zeros((24, 24))
+    for i in range(nb):
+        B = rng.normal(size=(6, 4))
+        A[6 * i:6 * i + 6, 6 * i:6 * i + 6] = B @ B.T
+    b = A @ rng.normal(size=24)
+    x = sg.block_ldlt_solve(A, b)
+    x_or = orc.block_ldlt_solve_dense(A, [6] * nb, b)
+    assert np.allclose(A @ x, b, atol=1e-8)
+    assert _rel(x, x_or) < 1e-6
+
+
+def _run_gpu(sg, problem, params, steps=None):
+    opt = sg.Optimizer(problem.variables(), problem.cost_fns(1.0), params)
+    trace = []
+    while True:
+        info = opt.step()
+        trace.append(info)
+        if info.termination is not None:
+            break
+    out = {n: opt.members(n) for n in ("poses", "points")}
+    merit = opt.merit()
+    opt.close()
+    return merit, out, trace
+
+
+def _sim3_align(X, Y):
+    """Similarity transform minimising |s R X + t - Y|."""
+    mx, my = X.mean(0), Y.mean(0)
+    Xc, Yc = X - mx, Y - my
+    U, S, Vt = np.linalg.svd(Yc.T @ Xc)
+    D = np.eye(3)
+    D[2, 2] = np.sign(np.linalg.det(U @ Vt))
+    R = U @ D @ Vt
+    s = np.trace(np.diag(S) @ D) / np.sum(Xc ** 2)
+    return s, R, my - s * R @ mx
+
+
+def test_ba_ring_fp64_trajectory_matches_oracle(sg, orc):
+    """10 cameras / 200 points / 1000 obs, Huber 1 px, nu0 = 1, 20 iterations."""
+    p = sg.generate_ba(sg.config_spec(0))
+    params = sg.OptParams(num_iterations=20, initial_lm_damping=1.0)
+    merit, out, trace = _run_gpu(sg, p, params)
+    sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), orc.ba_costs(p),
+                                           orc.default_params(num_iterations=20, initial_lm_damping=1.0))
+    assert len(trace) == len(otrace)
+    for g, o in zip(trace, otrace):
+        assert g.accepted == o["accepted"]
+        assert abs(g.merit - o["merit"]) <= 1e-9 * o["merit"]
+        assert abs(g.lm_damping - o["lm_damping"]) <= 1e-6 * o["lm_damping"]
+    assert abs(merit - sol["final_cost"]) <= 1e-9 * sol["final_cost"]
+    assert np.max(np.abs(out["poses"] - ovars["poses"])) < 1e-7
+    assert np.max(np.abs(out["points"] - ovars["points"])) < 1e-7
+    # the reference's own BA gate (ba_problem.rs:518-547): pose RMS < 0.01
+    rms = np.sqrt(np.mean((out["poses"][:, 4:] - p.poses_gt[:, 4:]) ** 2))
+    assert rms < 0.01
+
+
+def test_ba_walk_fp64_matches_oracle(sg, orc):
+    """A 60-camera random walk with 3000 points (the medium generator, small)."""
+    spec = sg.BaSpec(layout=1, num_cameras=60, num_points=3000, geometric_p=0.15, seed=7)
+    p = sg.generate_ba(spec)
+    params = sg.OptParams(num_iterations=15, initial_lm_damping=1.0)
+    merit, out, trace = _run_gpu(sg, p, params)
+    sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), orc.ba_costs(p),
+                                           orc.default_params(num_iterations=15, initial_lm_damping=1.0))
+    assert abs(merit - sol["final_cost"]) <= 1e-9 * sol["final_cost"]
+    for g, o in zip(trace, otrace):
+        assert g.accepted == o["accepted"]
+    # parameters after similarity alignment (scale is a gauge direction)
+    s, R, t = _sim3_align(out["points"], ovars["points"])
+    aligned = s * out["points"] @ R.T + t
+    assert np.max(np.abs(aligned - ovars["points"])) < 1e-6
+
+
+def test_ba_ring_mixed_precision_final_cost(sg, orc):
+    p = sg.generate_ba(sg.config_spec(0))
+    params = sg.OptParams(num_iterations=30, initial_lm_damping=1.0, precision=sg.Precision.Mixed)
+    merit, out, trace = _run_gpu(sg, p, params)
+    sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), orc.ba_costs(p),
+                                           orc.default_params(num_iterations=30, initial_lm_damping=1.0))
+    assert abs(merit - sol["final_cost"]) <= 1e-6 * sol["final_cost"]
+
+
+def test_pose_graph_plus_ba_matches_oracle(sg, orc):
+    """Reprojection terms plus SE(3) between factors between consecutive cameras."""
+    spec = sg.BaSpec(layout=1, num_cameras=30, num_points=800, geometric_p=0.2, seed=11)
+    p = sg.generate_ba(spec)
+    pairs = np.array([[i, i + 1] for i in range(29)], dtype=np.int64)
+    meas = np.stack([orc.se3_mul(orc.se3_inverse(p.poses_gt[i]), p.poses_gt[j]) for i, j in pairs])
+    costs = p.cost_fns(1.0) + [sg.CostFn.pose_graph3(["poses", "poses"], pairs, meas,
+                                                     sg.RobustKernel.huber(0.5))]
+    params = sg.OptParams(num_iterations=12, initial_lm_damping=1.0)
+    opt = sg.Optimizer(p.variables(), costs, params)
+    while opt.step().termination is None:
+        pass
+    merit = opt.merit()
+    opt.close()
+    ocosts = orc.ba_costs(p) + [(1, ["poses", "poses"], pairs, meas, 0.5)]
+    sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), ocosts,
+                                           orc.default_params(num_iterations=12, initial_lm_damping=1.0))
+    assert abs(merit - sol["final_cost"]) <= 1e-9 * sol["final_cost"]
