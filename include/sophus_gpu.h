@@ -253,11 +253,20 @@ typedef struct sgpu_rcs_stats {
   int64_t nnz_s_blocks;      /* lower-triangular S blocks incl. diagonal */
   int64_t nnz_l_blocks;      /* strictly-lower L blocks after ordering */
   int64_t num_schur_pairs;   /* (point, cam_i, cam_j<=cam_i) contributions */
-  int64_t num_levels;        /* etree levels of the RCS factorization */
+  int64_t num_levels;        /* levels of the supernodal elimination tree */
   double factor_flops;       /* FP64 flops of one RCS factorization */
+  int64_t num_supernodes;    /* supernodes (dense fronts) of the factorization */
+  int64_t max_front_blocks;  /* largest front, in 6x6 blocks */
+  int64_t front_bytes;       /* device bytes of all fronts */
 } sgpu_rcs_stats;
 
 sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* stats);
+
+/* Host-only symbolic analysis of a problem (validation, layout, RCS ordering and
+ * supernodes) without touching a device; same errors as sgpu_optimizer_create. */
+sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
+                         const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                         sgpu_rcs_stats* stats);
 
 /* Per-kernel device timing recorded with HIP events on the handle's stream since the
  * last reset: names[i] (static strings), count[i] launches, total_ms[i].

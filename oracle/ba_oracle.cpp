@@ -30,6 +30,7 @@
 //     triggers for indefinite blocks, and every RCS block here is PD (damped).
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1119,9 +1120,15 @@ int step(Optimizer& O, StepRecord& rec) {
 
 using namespace oracle;
 
+namespace {
+double g_step_seconds = 0.0;  // wall time spent inside step() by the last optimize call
+}
+
 extern "C" {
 
 const char* oracle_last_error(void) { return g_error.c_str(); }
+
+double oracle_last_step_seconds(void) { return g_step_seconds; }
 
 typedef struct oracle_step_record {
   double merit, lm_damping, new_merit, gain_ratio, max_abs_delta;
@@ -1147,9 +1154,13 @@ sgpu_status oracle_optimize_nlls(const sgpu_var_family* fams, int64_t nf, const 
   sol->initial_cost = O.merit_v;
   int64_t n = 0;
   int reason = SGPU_TERM_NONE;
+  g_step_seconds = 0.0;
   while (reason == SGPU_TERM_NONE) {
     StepRecord rec;
+    const auto t0 = std::chrono::steady_clock::now();
     reason = step(O, rec);
+    g_step_seconds +=
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (trace && n < trace_capacity) {
       trace[n] = {rec.merit, rec.lm_damping, rec.new_merit, rec.gain_ratio, rec.max_abs_delta,
                   rec.accepted, rec.termination};

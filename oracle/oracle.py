@@ -69,6 +69,7 @@ def lib() -> C.CDLL:
             build()
         L = C.CDLL(str(LIB))
         L.oracle_last_error.restype = C.c_char_p
+        L.oracle_last_step_seconds.restype = C.c_double
         L.oracle_optimize_nlls.restype = C.c_int
         L.oracle_optimize_nlls.argtypes = [C.POINTER(VarFamilyC), C.c_int64, C.POINTER(CostFnC), C.c_int64,
                                            C.POINTER(OptParamsC), C.POINTER(DP), C.POINTER(SolutionC),
@@ -194,6 +195,11 @@ def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trac
     solution = dict(initial_cost=sol.initial_cost, final_cost=sol.final_cost, num_steps=sol.num_steps,
                     termination=sol.termination, final_max_abs_neg_grad=sol.final_max_abs_neg_grad)
     return solution, {f[0]: o for f, o in zip(families, outs)}, recs
+
+
+def last_step_seconds() -> float:
+    """Wall time spent inside LM steps by the last optimize_nlls call."""
+    return lib().oracle_last_step_seconds()
 
 
 def linearize_dense(families, costs, nu: float) -> Tuple[np.ndarray, np.ndarray]:

@@ -156,13 +156,22 @@ bool project(const double K[4], const double pose[7], const double xw[3], double
   return uv[0] >= 0 && uv[0] < kWidth && uv[1] >= 0 && uv[1] < kHeight;
 }
 
+// Pose perturbation exp(d) * T_gt (ring: cameras 2 m from the origin) or, for the
+// walk, T_gt * exp(d): rotation about the camera's own centre, so that cameras far
+// from the world origin get the same image-space perturbation as near ones.
 void perturb(Rng& rng, const sgpu_ba_spec& s, Problem& p, int64_t ncam, int64_t npt) {
   p.poses_init = p.poses_gt;
   for (int64_t c = 1; c < ncam; ++c) {
     double d[6];
     for (int k = 0; k < 3; ++k) d[k] = s.rot_noise * rng.normal();
     for (int k = 3; k < 6; ++k) d[k] = s.trans_noise * rng.normal();
-    se3_oplus(&p.poses_gt[7 * c], d, &p.poses_init[7 * c]);
+    if (s.layout == 0) {
+      se3_oplus(&p.poses_gt[7 * c], d, &p.poses_init[7 * c]);
+    } else {
+      double e[7];
+      se3_exp(d, e);
+      se3_mul(&p.poses_gt[7 * c], e, &p.poses_init[7 * c]);
+    }
   }
   p.points_init = p.points_gt;
   for (int64_t i = 0; i < 3 * npt; ++i) p.points_init[i] += s.point_noise * rng.normal();

@@ -110,7 +110,8 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_point_prep(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
                  const double* __restrict__ cw, double nu, double* __restrict__ hinv_out,
-                 double* __restrict__ s_out, double* __restrict__ y_out, DevScalars* sc) {
+                 double* __restrict__ hll_out, double* __restrict__ s_out,
+                 double* __restrict__ y_out, DevScalars* sc) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= p.npt) return;
   if (p.pt_block[b] < 0) return;
@@ -156,6 +157,12 @@ __global__ void __launch_bounds__(kThreads)
     inv[8] = (m11 * m22 - m21 * m12) / det;
   }
   for (int k = 0; k < 9; ++k) hinv_out[9 * b + k] = inv[k];
+  hll_out[6 * b + 0] = m11;
+  hll_out[6 * b + 1] = m12;
+  hll_out[6 * b + 2] = m13;
+  hll_out[6 * b + 3] = m22;
+  hll_out[6 * b + 4] = m23;
+  hll_out[6 * b + 5] = m33;
   for (int k = 0; k < 3; ++k) {
     s_out[3 * b + k] = double(s[k]);
     y_out[3 * b + k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) +
@@ -207,151 +214,137 @@ __device__ __forceinline__ void mat3_mul_t(const T A[9], const T B[9], T C[9]) {
       C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
 }
 
-// ---- kernel: one S row (camera c1) with its reduced rhs -----------------------------------
-// One workgroup per free camera: visits the camera's observations and, for each, the
-// co-observations of the same point with camera block <= c1.  Row blocks accumulate in
-// LDS (row slots found by binary search) and are written once to the factor's
-// unified storage (permuted, transposed where the permutation flips the block).
+// ---- kernel: the reduced camera system, one wavefront per S block ------------------------
+// Block (c1, c2) of S sums over the (obs_i, obs_j) pairs of the points seen by both
+// cameras (cam(obs_i) = c1, cam(obs_j) = c2):
+//   i != j:  -P^T (M_i H_ll^-1 M_j) P
+//   i == j:  +P^T (M_i H_ll^-1 (H_ll - M_i)) P      (= P^T (M_i - M_i H_ll^-1 M_i) P, written
+//                                                   without the cancellation)
+// and the (o, o) pairs of a diagonal block also carry the camera's gradient
+// -P^T v_o and its reduced rhs P^T (M_o y_b - v_o).  Lanes stride over the block's
+// pairs, accumulate in FP64 registers and butterfly-reduce; no atomics, fixed order.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_camera_rows(DevProblem p, const int32_t* __restrict__ obs_point,
-                  const double* __restrict__ points, const double* __restrict__ intr,
+    k_schur_slots(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
                   const double* __restrict__ cw, double nu, const double* __restrict__ hinv_g,
-                  const double* __restrict__ y_g, double* __restrict__ C,
-                  double* __restrict__ rhs_nat, double* __restrict__ grad_cam, DevScalars* sc) {
-  __shared__ int32_t cols[kMaxRowSlots];
-  __shared__ T acc[kMaxRowSlots * 36];
-  __shared__ T racc[6], gacc[6];
-  const int c1 = blockIdx.x;
-  const int row0 = p.s_ptr[c1], nrow = p.s_ptr[c1 + 1] - row0;
-  const bool use_lds = nrow <= kMaxRowSlots;
-  const int tid = threadIdx.x;
-  if (use_lds) {
-    for (int k = tid; k < nrow; k += blockDim.x) cols[k] = p.s_col[row0 + k];
-    for (int k = tid; k < nrow * 36; k += blockDim.x) acc[k] = T(0);
-  }
-  if (tid < 6) {
-    racc[tid] = T(0);
-    gacc[tid] = T(0);
-  }
-  __syncthreads();
-  const int32_t* gcols = p.s_col + row0;
-  auto slot_of = [&](int c2) -> int {
-    int lo = 0, hi = nrow - 1;
-    const int32_t* cc = use_lds ? cols : gcols;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (cc[mid] < c2)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    return lo;
-  };
-  auto add_block = [&](int slot, const T x[3], const T N[9], T sign) {
-    if (use_lds) {
-      T* dst = acc + 36 * slot;
-      add_lift(x, N, sign, [&](int e, T v) { lds_add(dst + e, v); });
-    } else {
-      const int code = p.s_dest[row0 + slot];
-      double* dst = C + 36 * (code >> 1);
-      const bool tr = code & 1;
-      add_lift(x, N, sign, [&](int e, T v) {
-        const int idx = tr ? (e % 6) * 6 + e / 6 : e;
-        atomic_add_d(dst + idx, double(v));
-      });
-    }
-  };
-  const double* cw1 = cw + 12 * p.cam_member[c1];
-  for (int idx = p.cam_ptr[c1] + tid; idx < p.cam_ptr[c1 + 1]; idx += blockDim.x) {
-    const int o = p.cam_obs[idx];
-    const int b = obs_point[o];
+                  const double* __restrict__ hll_g, const double* __restrict__ y_g,
+                  double* __restrict__ front, double* __restrict__ rhs_nat,
+                  double* __restrict__ grad_cam) {
+  const int lane = threadIdx.x & 63;
+  const int64_t k = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (k >= p.nslot) return;  // whole wavefront leaves together
+  const int c1 = p.s_row[k];
+  const bool diag = p.s_col[k] == c1;
+  double acc[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  double racc[6] = {0, 0, 0, 0, 0, 0}, gacc[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
+    const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
+    const int b = p.obs_point[i];
     const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
     const T x[3] = {T(xd[0]), T(xd[1]), T(xd[2])};
-    const double* in = intr + 4 * p.obs_intr[o];
-    double xc[3];
-    to_camera(cw1, xd, xc);
-    T A[6], v[3], Mo[9];
-    obs_jacobian<T>(cw1, in, xc, A);
-    obs_residual<T>(in, p.obs_uv + 2 * o, xc, p.robust, p.delta, A, v);
-    gram(A, Mo);
-    // own gradient: neg_grad_p -= P^T v  (P^T t = [x cross t; t])
-    T g[6];
-    g[0] = -(x[1] * v[2] - x[2] * v[1]);
-    g[1] = -(x[2] * v[0] - x[0] * v[2]);
-    g[2] = -(x[0] * v[1] - x[1] * v[0]);
-    g[3] = -v[0];
-    g[4] = -v[1];
-    g[5] = -v[2];
+    const double* cwi = cw + 12 * p.obs_pose[i];
+    const double* ini = intr + 4 * p.obs_intr[i];
+    double xci[3];
+    to_camera(cwi, xd, xci);
+    double Ad[6];
+    obs_jacobian<double>(cwi, ini, xci, Ad);
+    T Ai[6], Mi[9];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) lds_add(&gacc[k], g[k]);
-    if (p.pt_block[b] < 0) {  // constant point: H_pp only, no Schur term
+    for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
+    gram(Ai, Mi);
+    const bool active = p.pt_block[b] >= 0;
+    T N[9];
+    T sign = T(1);
+    if (i == j) {
+      double v[3];
+      obs_residual<double>(ini, p.obs_uv + 2 * i, xci, p.robust, p.delta, Ad, v);
+      double t[3];
+      if (active) {
+        double Md[9];
+        gram(Ad, Md);
+        const double* yb = y_g + 3 * b;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) lds_add(&racc[k], g[k]);
-      add_block(nrow - 1, x, Mo, T(1));
-      continue;
-    }
-    T Hi[9], y[3];
+        for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * yb[0] + Md[3 * r + 1] * yb[1] + Md[3 * r + 2] * yb[2] - v[r];
+        // R = H_ll - M_i (FP64, then to T), N = M_i H_ll^-1 R
+        const double* H = hll_g + 6 * b;
+        const double Hf[9] = {H[0], H[1], H[2], H[1], H[3], H[4], H[2], H[4], H[5]};
+        T R[9], Hi[9], Z[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Hi[k] = T(hinv_g[9 * b + k]);
+        for (int e = 0; e < 9; ++e) {
+          R[e] = T(Hf[e] - Md[e]);
+          Hi[e] = T(hinv_g[9 * b + e]);
+        }
+        mat3_mul_t(Mi, Hi, Z);
+        mat3_mul_t(Z, R, N);
+      } else {  // constant point: own H_pp term only
 #pragma unroll
-    for (int k = 0; k < 3; ++k) y[k] = T(y_g[3 * b + k]);
-    // rhs += P^T (M_o y - v)
-    T t[3];
+        for (int r = 0; r < 3; ++r) t[r] = -v[r];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) t[k] = Mo[3 * k] * y[0] + Mo[3 * k + 1] * y[1] + Mo[3 * k + 2] * y[2] - v[k];
-    lds_add(&racc[0], x[1] * t[2] - x[2] * t[1]);
-    lds_add(&racc[1], x[2] * t[0] - x[0] * t[2]);
-    lds_add(&racc[2], x[0] * t[1] - x[1] * t[0]);
-    lds_add(&racc[3], t[0]);
-    lds_add(&racc[4], t[1]);
-    lds_add(&racc[5], t[2]);
-    T Z[9];
-    mat3_mul_t(Mo, Hi, Z);
-    for (int j = p.pt_ptr[b]; j < p.pt_ptr[b + 1]; ++j) {
-      const int pj = p.obs_pose[j];
-      const int c2 = p.pose_block[pj];
-      if (c2 < 0) continue;
-      if (c2 > c1) break;
-      T Mj[9];
-      if (j == o) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Mj[k] = Mo[k];
-      } else {
-        const double* cwj = cw + 12 * pj;
-        double xcj[3];
-        to_camera(cwj, xd, xcj);
-        T Aj[6];
-        obs_jacobian<T>(cwj, intr + 4 * p.obs_intr[j], xcj, Aj);
-        gram(Aj, Mj);
+        for (int e = 0; e < 9; ++e) N[e] = Mi[e];
       }
-      T N[9];
+      // P^T t = [x cross t; t]
+      racc[0] += xd[1] * t[2] - xd[2] * t[1];
+      racc[1] += xd[2] * t[0] - xd[0] * t[2];
+      racc[2] += xd[0] * t[1] - xd[1] * t[0];
+      racc[3] += t[0];
+      racc[4] += t[1];
+      racc[5] += t[2];
+      gacc[0] -= xd[1] * v[2] - xd[2] * v[1];
+      gacc[1] -= xd[2] * v[0] - xd[0] * v[2];
+      gacc[2] -= xd[0] * v[1] - xd[1] * v[0];
+      gacc[3] -= v[0];
+      gacc[4] -= v[1];
+      gacc[5] -= v[2];
+    } else {
+      const double* cwj = cw + 12 * p.obs_pose[j];
+      double xcj[3];
+      to_camera(cwj, xd, xcj);
+      T Aj[6], Mj[9], Hi[9], Z[9];
+      obs_jacobian<T>(cwj, intr + 4 * p.obs_intr[j], xcj, Aj);
+      gram(Aj, Mj);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Hi[e] = T(hinv_g[9 * b + e]);
+      mat3_mul_t(Mi, Hi, Z);
       mat3_mul_t(Z, Mj, N);
-      if (j == o) {
+      sign = T(-1);
+    }
+    add_lift(x, N, sign, [&](int e, T val) { acc[e] += double(val); });
+  }
+  // butterfly reduction: every lane ends with all sums
 #pragma unroll
-        for (int k = 0; k < 9; ++k) N[k] -= Mo[k];
+  for (int m = 32; m > 0; m >>= 1) {
+#pragma unroll
+    for (int e = 0; e < 36; ++e) acc[e] += __shfl_xor(acc[e], m, 64);
+    if (diag) {
+#pragma unroll
+      for (int e = 0; e < 6; ++e) {
+        racc[e] += __shfl_xor(racc[e], m, 64);
+        gacc[e] += __shfl_xor(gacc[e], m, 64);
       }
-      add_block(slot_of(c2), x, N, T(-1));
     }
   }
-  __syncthreads();
-  const T damp = T(nu * double(p.cam_nu[c1]));
-  if (use_lds) {
-    if (tid < 6) acc[36 * (nrow - 1) + 7 * tid] += damp;
-    __syncthreads();
-    for (int k = tid; k < nrow * 36; k += blockDim.x) {
-      const int slot = k / 36, e = k % 36;
-      const int code = p.s_dest[row0 + slot];
-      const int idx = (code & 1) ? (e % 6) * 6 + e / 6 : e;
-      C[36 * (code >> 1) + idx] = double(acc[k]);
+  double mine = 0.0;
+#pragma unroll
+  for (int e = 0; e < 36; ++e)
+    if (lane == e) mine = acc[e];
+  if (diag) {
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      if (lane == 36 + e) mine = racc[e];
+      if (lane == 42 + e) mine = gacc[e];
     }
-  } else if (tid < 6) {
-    const int code = p.s_dest[row0 + nrow - 1];
-    atomic_add_d(C + 36 * (code >> 1) + 7 * tid, double(damp));
   }
-  if (tid < 6) {
-    rhs_nat[6 * c1 + tid] = double(racc[tid]);
-    grad_cam[6 * c1 + tid] = double(gacc[tid]);
+  if (lane < 36) {
+    const int a = lane / 6, bb = lane % 6;
+    if (diag && a == bb) mine += nu * double(p.cam_nu[c1]);
+    const int ldc = p.s_ld[k], ld = ldc >> 1;
+    front[p.s_off[k] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = mine;
+  } else if (diag && lane < 42) {
+    rhs_nat[6 * c1 + lane - 36] = mine;
+  } else if (diag && lane < 48) {
+    grad_cam[6 * c1 + lane - 42] = mine;
   }
 }
 
@@ -408,155 +401,429 @@ __device__ bool ldlt6(const double* A, double* L, double* d) {
   return true;
 }
 
-__global__ void __launch_bounds__(kThreads) k_factor_level(DevFactor f, int level, DevScalars* sc) {
-  const int j = f.lvl_col[f.lvl_ptr[level] + blockIdx.x];
-  const int nb = f.nb;
-  const int tid = threadIdx.x;
-  const int r0 = f.r_ptr[j], r1 = f.r_ptr[j + 1];
-  // phase 1: M_jk = D_k L_jk^T for k in reach(j)
-  for (int e = tid; e < (r1 - r0) * 36; e += blockDim.x) {
-    const int r = r0 + e / 36, q = e % 36, a = q / 6, bc = q % 6;
-    const double* Dk = f.Dfull + 36 * f.r_col[r];
-    const double* Ljk = f.C + 36 * (nb + f.r_lpos[r]);
-    double s = 0;
+// ---- supernodal multifrontal LDLt ---------------------------------------------------------
+// Fronts are dense column-major F x F (F = 6 * front rows); the first K = 6 * node columns
+// are eliminated with 6x6 pivots (the reference's block LDLt, block_sparse_ldlt.rs:614-747):
+// for pivot j: (L_jj, d_j) = ldlt6(C_jj); L_ij = C_ij L_jj^-T D^-1 L_jj^-1 for rows i > j;
+// trailing C_ik -= L_ij C_kj^T.  Panels of kPanelBlocks pivots are factored by one
+// workgroup per node; the rank-24 trailing update runs as 64x64 tiles over many workgroups.
+
+__device__ __forceinline__ void right_solve6(const double* L, const double* d, double x[6]) {
 #pragma unroll
-    for (int t = 0; t < 6; ++t) s += Dk[6 * a + t] * Ljk[6 * bc + t];
-    f.M[36 * r + q] = s;
+  for (int c = 0; c < 6; ++c) {
+    double v = x[c];
+    for (int t = 0; t < c; ++t) v -= x[t] * L[6 * c + t];
+    x[c] = v;
   }
-  __syncthreads();
-  // phase 2: C(:, j) -= sum_k L(:, k) M_jk over the gather lists of column j's slots
-  const int nslot = 1 + f.l_ptr[j + 1] - f.l_ptr[j];
-  for (int e = tid; e < nslot * 36; e += blockDim.x) {
-    const int s = e / 36, q = e % 36, a = q / 6, bc = q % 6;
-    const int u = s == 0 ? j : nb + f.l_ptr[j] + s - 1;
-    double acc = f.C[36 * u + q];
-    for (int w = f.u_ptr[u]; w < f.u_ptr[u + 1]; ++w) {
-      const double* Lik = f.C + 36 * (nb + f.u_src[w]) + 6 * a;
-      const double* Mjk = f.M + 36 * f.u_m[w] + bc;
-      double t = 0;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) t += Lik[k] * Mjk[6 * k];
-      acc -= t;
-    }
-    f.C[36 * u + q] = acc;
-  }
-  __syncthreads();
-  // phase 3: pivot block
-  if (tid == 0) {
-    double L[36], d[6];
-    if (!ldlt6(f.C + 36 * j, L, d)) {
-      atomicOr(&sc->bad_pivot, 1);
-      for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
-      for (int k = 0; k < 6; ++k) d[k] = 0.0;
-    }
-    for (int k = 0; k < 36; ++k) f.Ljj[36 * j + k] = L[k];
-    for (int k = 0; k < 6; ++k) f.dvec[6 * j + k] = d[k];
-    for (int a = 0; a < 6; ++a)
-      for (int b = 0; b < 6; ++b) {
-        double s = 0;
-        for (int t = 0; t < 6; ++t) s += L[6 * a + t] * d[t] * L[6 * b + t];
-        f.Dfull[36 * j + 6 * a + b] = s;
-      }
-  }
-  __syncthreads();
-  // phase 4: L_ij = C_ij L_jj^-T D_j^-1 L_jj^-1, one row of one block per thread
-  // (ldlt_right_matsolve_inplace, dense_ldlt.rs:300-308)
-  const double* L = f.Ljj + 36 * j;
-  const double* d = f.dvec + 6 * j;
-  for (int e = tid; e < (nslot - 1) * 6; e += blockDim.x) {
-    const int s = 1 + e / 6, a = e % 6;
-    double* X = f.C + 36 * (nb + f.l_ptr[j] + s - 1) + 6 * a;
-    double x[6];
+  for (int c = 0; c < 6; ++c) x[c] *= d[c] == 0.0 ? 0.0 : 1.0 / d[c];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) x[k] = X[k];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      double v = x[c];
-      for (int t = 0; t < c; ++t) v -= x[t] * L[6 * c + t];
-      x[c] = v;
-    }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) x[c] *= d[c] == 0.0 ? 0.0 : 1.0 / d[c];
-#pragma unroll
-    for (int c = 5; c >= 0; --c) {
-      double v = x[c];
-      for (int t = c + 1; t < 6; ++t) v -= x[t] * L[6 * t + c];
-      x[c] = v;
-    }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) X[k] = x[k];
+  for (int c = 5; c >= 0; --c) {
+    double v = x[c];
+    for (int t = c + 1; t < 6; ++t) v -= x[t] * L[6 * t + c];
+    x[c] = v;
   }
 }
 
-// ---- triangular solves (single workgroup walking the etree levels) ---------------------
-__global__ void __launch_bounds__(1024)
-    k_triangular_solve(DevFactor f, const double* __restrict__ rhs_nat, double* __restrict__ y,
-                       double* __restrict__ dx_cam) {
-  const int nb = f.nb, tid = threadIdx.x, nt = blockDim.x;
-  for (int e = tid; e < 6 * nb; e += nt) y[e] = rhs_nat[6 * f.perm[e / 6] + e % 6];
+constexpr int kPW = 6 * kPanelBlocks;
+
+// View a [kPW][kTile + 1] LDS tile as a [kPW][kPW + 1] square (kPW <= kTile).
+__device__ __forceinline__ double (*as_square(double (*t)[kTile + 1]))[kPW + 1] {
+  return reinterpret_cast<double (*)[kPW + 1]>(&t[0][0]);
+}
+
+// Factor the np x np diagonal block of the panel at column k0 of one front in LDS (block
+// LDLt with 6x6 pivots and the PSD rule), then store L_PP (strictly below the pivot
+// blocks) into the front and the pivots (L_jj, d_j) into piv_L / piv_d.  Called by all
+// threads of one workgroup.
+__device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, int k0, int np,
+                                  DevScalars* sc, double (*D)[kPW + 1], double (*Wd)[kPW + 1],
+                                  double* sL, double* sd) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < np * np; e += blockDim.x) {
+    const int r = e % np, c = e / np;
+    D[r][c] = r >= c ? A[int64_t(k0 + c) * F + k0 + r] : 0.0;
+  }
   __syncthreads();
-  // L y = b (gather over the row structure of L)
-  for (int lev = 0; lev < f.nlev; ++lev) {
-    const int c0 = f.lvl_ptr[lev], cn = f.lvl_ptr[lev + 1] - c0;
-    for (int e = tid; e < cn * 6; e += nt) {
-      const int j = f.lvl_col[c0 + e / 6], c = e % 6;
-      double acc = y[6 * j + c];
-      for (int r = f.r_ptr[j]; r < f.r_ptr[j + 1]; ++r) {
-        const double* Ljk = f.C + 36 * (nb + f.r_lpos[r]) + 6 * c;
-        const double* yk = y + 6 * f.r_col[r];
-        double t = 0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) t += Ljk[k] * yk[k];
-        acc -= t;
+  for (int j0 = 0; j0 < np; j0 += 6) {
+    if (tid == 0) {
+      double a6[36], L[36], d[6];
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) a6[6 * a + b] = a >= b ? D[j0 + a][j0 + b] : D[j0 + b][j0 + a];
+      if (!ldlt6(a6, L, d)) {
+        atomicOr(&sc->bad_pivot, 1);
+        for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
+        for (int k = 0; k < 6; ++k) d[k] = 0.0;
       }
-      y[6 * j + c] = acc;
+      const int gb = f.sn_c0[X] + (k0 + j0) / 6;
+      for (int k = 0; k < 36; ++k) {
+        sL[k] = L[k];
+        f.piv_L[36 * gb + k] = L[k];
+      }
+      for (int k = 0; k < 6; ++k) {
+        sd[k] = d[k];
+        f.piv_d[6 * gb + k] = d[k];
+      }
+    }
+    __syncthreads();
+    for (int r = j0 + 6 + tid; r < np; r += blockDim.x) {
+      double x[6];
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        x[t] = D[r][j0 + t];
+        Wd[r][j0 + t] = x[t];
+      }
+      right_solve6(sL, sd, x);
+#pragma unroll
+      for (int t = 0; t < 6; ++t) D[r][j0 + t] = x[t];
+    }
+    __syncthreads();
+    for (int e = tid; e < np * np; e += blockDim.x) {
+      const int r = e % np, c = e / np;
+      if (c < j0 + 6 || r < c) continue;
+      double acc = 0;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) acc += D[r][j0 + t] * Wd[c][j0 + t];
+      D[r][c] -= acc;
     }
     __syncthreads();
   }
-  // D z = y per block (lower, diag with d == 0 -> 0, lower transpose)
-  for (int j = tid; j < nb; j += nt) {
-    const double* L = f.Ljj + 36 * j;
-    const double* d = f.dvec + 6 * j;
+  double* Lout = f.lbuf + f.sn_loff[X];
+  for (int e = tid; e < np * np; e += blockDim.x) {
+    const int r = e % np, c = e / np;
+    if (r / 6 > c / 6) Lout[int64_t(k0 + c) * F + k0 + r] = D[r][c];
+  }
+}
+
+// First panel diagonal of every node of a level (after the level's fronts are assembled).
+__global__ void __launch_bounds__(kThreads)
+    k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc) {
+  const int X = nodes[blockIdx.x];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
+  __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[36], sd[6];
+  factor_panel_diag(f, X, f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL, sd);
+}
+
+// W_r (unscaled) and L_r of one row below the panel from C_rP, L_PP and the pivots:
+// W_r,j = C_r,j - sum_{i<j} W_r,i L_PP(j, i)^T ; L_r,j = W_r,j L_jj^-T D_j^-1 L_jj^-1.
+__device__ __forceinline__ void panel_row(const double* __restrict__ A, int F, int k0, int np,
+                                          int r, const double (*Lpp)[kPW + 1],
+                                          const double (*pL)[36], const double (*pd)[6],
+                                          double w[kPW], double l[kPW]) {
+#pragma unroll
+  for (int c = 0; c < kPW; ++c) w[c] = c < np ? A[int64_t(k0 + c) * F + r] : 0.0;
+#pragma unroll
+  for (int j0 = 6; j0 < kPW; j0 += 6) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      double v = w[j0 + a];
+#pragma unroll
+      for (int c = 0; c < j0; ++c) v -= w[c] * Lpp[j0 + a][c];
+      w[j0 + a] = v;
+    }
+  }
+#pragma unroll
+  for (int jb = 0; jb < kPanelBlocks; ++jb) {
+    double x[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) x[t] = w[6 * jb + t];
+    right_solve6(pL[jb], pd[jb], x);
+#pragma unroll
+    for (int t = 0; t < 6; ++t) l[6 * jb + t] = x[t];
+  }
+}
+
+// One panel step of the nodes of a level, one workgroup per 64x64 tile of the lower
+// trailing region: form the tile rows' L and the tile columns' W from the panel
+// columns, store L (tile column 0 only), update A[r, c] -= L_r W_c^T, and (tile (0, 0),
+// which holds the next panel's diagonal block) factor the next panel's diagonal.
+__global__ void __launch_bounds__(kThreads)
+    k_mf_step(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc) {
+  const int X = tiles[3 * blockIdx.x], tr = tiles[3 * blockIdx.x + 1], tc = tiles[3 * blockIdx.x + 2];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
+  double* A = f.front + f.sn_off[X];
+  double* Lo = f.lbuf + f.sn_loff[X];
+  const int k0 = kPW * panel;
+  const int kend = min(K, k0 + kPW);
+  const int np = kend - k0;
+  const int r0 = kend + kTile * tr, c0 = kend + kTile * tc;
+  const int tid = threadIdx.x;
+  __shared__ double Lpp[kPW][kPW + 1];
+  __shared__ double pL[kPanelBlocks][36], pd[kPanelBlocks][6];
+  __shared__ double sA[kPW][kTile + 1], sB[kPW][kTile + 1];
+  for (int e = tid; e < kPW * kPW; e += blockDim.x) {
+    const int r = e % kPW, c = e / kPW;
+    Lpp[r][c] = (r < np && c < np && r / 6 > c / 6) ? Lo[int64_t(k0 + c) * F + k0 + r] : 0.0;
+  }
+  for (int e = tid; e < kPanelBlocks * 42; e += blockDim.x) {
+    const int jb = e / 42, q = e % 42;
+    const int gb = f.sn_c0[X] + k0 / 6 + jb;
+    const bool ok = 6 * jb < np;
+    if (q < 36)
+      pL[jb][q] = ok ? f.piv_L[36 * gb + q] : (q % 7 == 0 ? 1.0 : 0.0);
+    else
+      pd[jb][q - 36] = ok ? f.piv_d[6 * gb + q - 36] : 0.0;
+  }
+  __syncthreads();
+  // threads 0..63: tile rows (L into sA, and W into sB on diagonal tiles);
+  // threads 64..127: tile columns (W into sB) on off-diagonal tiles
+  if (tid < 2 * kTile) {
+    const bool is_row = tid < kTile;
+    const int q = is_row ? tid : tid - kTile;
+    const int r = (is_row ? r0 : c0) + q;
+    if (is_row || tr != tc) {
+      double w[kPW], l[kPW];
+      const bool inside = r < F;
+      if (inside) panel_row(A, F, k0, np, r, Lpp, pL, pd, w, l);
+#pragma unroll
+      for (int t = 0; t < kPW; ++t) {
+        const double lv = inside ? l[t] : 0.0, wv = inside ? w[t] : 0.0;
+        if (is_row) {
+          sA[t][q] = lv;
+          if (tr == tc) sB[t][q] = wv;
+        } else {
+          sB[t][q] = wv;
+        }
+      }
+      if (inside && is_row && tc == 0)
+#pragma unroll
+        for (int t = 0; t < kPW; ++t)
+          if (t < np) Lo[int64_t(k0 + t) * F + r] = l[t];
+    }
+  }
+  __syncthreads();
+  const int ty = tid / 16, tx = tid % 16;
+  double acc[4][4] = {};
+  for (int t = 0; t < np; ++t) {
+    double a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = sA[t][4 * ty + i];
+      b[i] = sB[t][4 * tx + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + 4 * ty + i, c = c0 + 4 * tx + j;
+      if (r < F && c < F && r >= c) A[int64_t(c) * F + r] -= acc[i][j];
+    }
+  if (tr == 0 && tc == 0 && kend < K) {
+    __syncthreads();  // every update of this tile is in memory before it is re-read
+    factor_panel_diag(f, X, A, F, kend, min(K, kend + kPW) - kend, sc, as_square(sA),
+                      as_square(sB), &pL[0][0], &pd[0][0]);
+  }
+}
+
+// Extend-add: scatter each finished node's update matrix into its parent's front.
+__global__ void __launch_bounds__(kThreads)
+    k_mf_extend(DevFactor f, const int32_t* __restrict__ nodes) {
+  const int X = nodes[blockIdx.y];
+  const int P = f.sn_parent[X];
+  if (P < 0) return;
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], m = F - K;
+  const int FP = 6 * f.sn_nf[P];
+  const double* U = f.front + f.sn_off[X] + int64_t(K) * F + K;
+  double* B = f.front + f.sn_off[P];
+  const int32_t* up = f.sn_upos + f.sn_upos_ptr[X];
+  for (int c = blockIdx.x; c < m; c += gridDim.x) {
+    const int pc = 6 * up[c / 6] + c % 6;
+    for (int r = c + threadIdx.x; r < m; r += blockDim.x) {
+      const double v = U[int64_t(c) * F + r];
+      if (v != 0.0) atomic_add_d(B + int64_t(pc) * FP + 6 * up[r / 6] + r % 6, v);
+    }
+  }
+}
+
+// ---- solves with the supernodal factor ----------------------------------------------------
+// Per level (children first): y_K <- L_KK^-1 y_K (node's own rows, one workgroup per
+// node), then the rows below: y_rows -= L_rect y_K (many workgroups, atomics into the
+// ancestors' entries), z_K = D^-1 y_K.  Backward (parents first): t_K = z_K - L_rect^T
+// x_rows (column tiles), then x_K = L_KK^-T t_K (one workgroup per node).
+
+__device__ __forceinline__ void diag_solve6(const double* L, const double* d, double v[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double s = v[i];
+    for (int k = 0; k < i; ++k) s -= L[6 * i + k] * v[k];
+    v[i] = s;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v[i] = d[i] == 0.0 ? 0.0 : v[i] / d[i];
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double s = v[i];
+    for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * v[k];
+    v[i] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_mf_fwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y,
+                 double* __restrict__ ytmp) {
+  const int X = nodes[blockIdx.x];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  const double* A = f.lbuf + f.sn_loff[X];
+  __shared__ double yl[6 * 96];
+  __shared__ double Lp[kPW][kPW + 1];
+  for (int i = threadIdx.x; i < K; i += blockDim.x) yl[i] = y[6 * c0 + i];
+  for (int k0 = 0; k0 < K; k0 += kPW) {
+    const int kend = min(K, k0 + kPW), np = kend - k0;
+    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
+      const int r = e % np, c = e / np;
+      Lp[r][c] = r / 6 > c / 6 ? A[int64_t(k0 + c) * F + k0 + r] : 0.0;
+    }
+    __syncthreads();
+    for (int j0 = 6; j0 < np; j0 += 6) {
+      if (threadIdx.x < 6) {
+        const int r = j0 + threadIdx.x;
+        double v = yl[k0 + r];
+        for (int c = 0; c < j0; ++c) v -= Lp[r][c] * yl[k0 + c];
+        yl[k0 + r] = v;
+      }
+      __syncthreads();
+    }
+    for (int r = kend + threadIdx.x; r < K; r += blockDim.x) {
+      double s = 0;
+      for (int c = k0; c < kend; ++c) s += A[int64_t(c) * F + r] * yl[c];
+      yl[r] -= s;
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < K; i += blockDim.x) ytmp[6 * c0 + i] = yl[i];
+  for (int jb = threadIdx.x; jb < K / 6; jb += blockDim.x) {
     double v[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = y[6 * j + k];
+    for (int k = 0; k < 6; ++k) v[k] = yl[6 * jb + k];
+    diag_solve6(f.piv_L + 36 * (c0 + jb), f.piv_d + 6 * (c0 + jb), v);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      double s = v[i];
-      for (int k = 0; k < i; ++k) s -= L[6 * i + k] * v[k];
-      v[i] = s;
-    }
-#pragma unroll
-    for (int i = 0; i < 6; ++i) v[i] = d[i] == 0.0 ? 0.0 : v[i] / d[i];
-#pragma unroll
-    for (int i = 5; i >= 0; --i) {
-      double s = v[i];
-      for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * v[k];
-      v[i] = s;
-    }
-#pragma unroll
-    for (int k = 0; k < 6; ++k) y[6 * j + k] = v[k];
+    for (int k = 0; k < 6; ++k) y[6 * (c0 + jb) + k] = v[k];
   }
+}
+
+// rows below the node's own: y[ancestor rows] -= L_rect y_K; items = (node, row tile)
+__global__ void __launch_bounds__(kThreads)
+    k_mf_fwd_rect(DevFactor f, const int32_t* __restrict__ items, double* __restrict__ y,
+                  const double* __restrict__ ytmp) {
+  const int X = items[2 * blockIdx.x], tile = items[2 * blockIdx.x + 1];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  const double* A = f.lbuf + f.sn_loff[X];
+  const int32_t* rows = f.sn_rows + f.sn_rows_ptr[X];
+  __shared__ double yk[6 * 96];
+  for (int i = threadIdx.x; i < K; i += blockDim.x) yk[i] = ytmp[6 * c0 + i];
   __syncthreads();
-  // L^T x = z (gather over the column structure of L), levels top-down
-  for (int lev = f.nlev - 1; lev >= 0; --lev) {
-    const int c0 = f.lvl_ptr[lev], cn = f.lvl_ptr[lev + 1] - c0;
-    for (int e = tid; e < cn * 6; e += nt) {
-      const int j = f.lvl_col[c0 + e / 6], c = e % 6;
-      double acc = y[6 * j + c];
-      for (int q = f.l_ptr[j]; q < f.l_ptr[j + 1]; ++q) {
-        const double* Lij = f.C + 36 * (nb + q) + c;
-        const double* xi = y + 6 * f.l_row[q];
-        double t = 0;
+  const int r = K + kThreads * tile + threadIdx.x;
+  if (r >= F) return;
+  double s = 0;
+  for (int c = 0; c < K; ++c) s += A[int64_t(c) * F + r] * yk[c];
+  if (s != 0.0) atomic_add_d(y + 6 * rows[r / 6] + r % 6, -s);
+}
+
+// t_K = z_K - L_rect^T x_rows for one tile of 64 node columns (one wavefront per column
+// at a time); items = (node, column tile)
+__global__ void __launch_bounds__(kThreads)
+    k_mf_bwd_rect(DevFactor f, const int32_t* __restrict__ items, double* __restrict__ y) {
+  const int X = items[2 * blockIdx.x], tile = items[2 * blockIdx.x + 1];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  const double* A = f.lbuf + f.sn_loff[X];
+  const int32_t* rows = f.sn_rows + f.sn_rows_ptr[X];
+  __shared__ double xr[6 * 512];
+  const int m = F - K;
+  double* xs = m <= 6 * 512 ? xr : nullptr;
+  if (xs)
+    for (int i = threadIdx.x; i < m; i += blockDim.x) xs[i] = y[6 * rows[(K + i) / 6] + (K + i) % 6];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int cc = wid; cc < kTile; cc += kThreads / 64) {
+    const int c = kTile * tile + cc;
+    if (c >= K) break;
+    double s = 0;
+    for (int i = lane; i < m; i += 64) {
+      const double xv = xs ? xs[i] : y[6 * rows[(K + i) / 6] + (K + i) % 6];
+      s += A[int64_t(c) * F + K + i] * xv;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[6 * c0 + c] -= s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_mf_bwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y) {
+  const int X = nodes[blockIdx.x];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  const double* A = f.lbuf + f.sn_loff[X];
+  __shared__ double xl[6 * 96];
+  __shared__ double Lp[kPW][kPW + 1];
+  __shared__ double red[kThreads / 64][kPW];
+  for (int i = threadIdx.x; i < K; i += blockDim.x) xl[i] = y[6 * c0 + i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int npan = (K + kPW - 1) / kPW;
+  for (int pn = npan - 1; pn >= 0; --pn) {
+    const int k0 = kPW * pn, kend = min(K, k0 + kPW), np = kend - k0;
+    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
+      const int r = e % np, c = e / np;
+      Lp[r][c] = r / 6 > c / 6 ? A[int64_t(k0 + c) * F + k0 + r] : 0.0;
+    }
+    double part[kPW];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) t += Lij[6 * k] * xi[k];
-        acc -= t;
-      }
-      y[6 * j + c] = acc;
+    for (int t = 0; t < kPW; ++t) part[t] = 0.0;
+    for (int r = kend + threadIdx.x; r < K; r += blockDim.x) {
+      const double xv = xl[r];
+#pragma unroll
+      for (int t = 0; t < kPW; ++t)
+        if (t < np) part[t] += A[int64_t(k0 + t) * F + r] * xv;
+    }
+#pragma unroll
+    for (int t = 0; t < kPW; ++t)
+      for (int off = 32; off > 0; off >>= 1) part[t] += __shfl_down(part[t], off, 64);
+    if (lane == 0)
+      for (int t = 0; t < kPW; ++t) red[wid][t] = part[t];
+    __syncthreads();
+    if (int(threadIdx.x) < np) {
+      double s = 0;
+      for (int w = 0; w < int(blockDim.x / 64); ++w) s += red[w][threadIdx.x];
+      xl[k0 + threadIdx.x] -= s;
     }
     __syncthreads();
+    for (int j0 = ((np - 1) / 6 - 1) * 6; j0 >= 0; j0 -= 6) {
+      if (threadIdx.x < 6) {
+        const int c = j0 + threadIdx.x;
+        double v = xl[k0 + c];
+        for (int r = j0 + 6; r < np; ++r) v -= Lp[r][c] * xl[k0 + r];
+        xl[k0 + c] = v;
+      }
+      __syncthreads();
+    }
   }
-  for (int e = tid; e < 6 * nb; e += nt) dx_cam[6 * f.perm[e / 6] + e % 6] = y[e];
+  for (int i = threadIdx.x; i < K; i += blockDim.x) y[6 * c0 + i] = xl[i];
+}
+
+__global__ void k_permute_in(int nb, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 6 * nb) dst[e] = src[6 * perm[e / 6] + e % 6];
+}
+
+__global__ void k_permute_out(int nb, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                              double* __restrict__ dst) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 6 * nb) dst[6 * perm[e / 6] + e % 6] = src[e];
+}
+
+__global__ void k_scatter_s(DevProblem p, int64_t nslots, const double* __restrict__ blocks,
+                            double* __restrict__ front) {
+  const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (e >= nslots * 36) return;
+  const int64_t k = e / 36;
+  const int q = int(e % 36), a = q / 6, b = q % 6;
+  const int ldc = p.s_ld[k], ld = ldc >> 1;
+  front[p.s_off[k] + ((ldc & 1) ? a * ld + b : b * ld + a)] += blocks[e];
 }
 
 // block reduction of two doubles into partial[2 * blockIdx.x + {0, 1}]
@@ -689,32 +956,47 @@ __global__ void __launch_bounds__(kThreads)
   block_reduce2(acc, 0.0, partial);
 }
 
-// Sum partials in a fixed order; finish the step scalars.
-__global__ void k_finalize(const double* __restrict__ partial, int n_pt_blocks, int n_pose_blocks,
-                           int n_cost_blocks, const double* __restrict__ grad_cam, int nb,
-                           DevScalars* sc, int what) {
+// Sum partials in a fixed order (strided per thread, then a fixed tree); finish the
+// step scalars.
+__device__ double block_sum_fixed(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (int(threadIdx.x) < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256)
+    k_finalize(const double* __restrict__ partial, int n_pt_blocks, int n_pose_blocks,
+               int n_cost_blocks, const double* __restrict__ grad_cam, int nb, DevScalars* sc,
+               int what) {
   __shared__ double red[256];
   if (what == 0) {  // dd, dg from back-substitution + pose update partials
+    double dd = 0, dg = 0;
+    for (int i = threadIdx.x; i < n_pt_blocks; i += blockDim.x) {
+      dd += partial[2 * i];
+      dg += partial[2 * i + 1];
+    }
+    const double* q = partial + 2 * n_pt_blocks;
+    for (int i = threadIdx.x; i < n_pose_blocks; i += blockDim.x) {
+      dd += q[2 * i];
+      dg += q[2 * i + 1];
+    }
+    dd = block_sum_fixed(dd, red);
+    dg = block_sum_fixed(dg, red);
     if (threadIdx.x == 0) {
-      double dd = 0, dg = 0;
-      for (int i = 0; i < n_pt_blocks; ++i) {
-        dd += partial[2 * i];
-        dg += partial[2 * i + 1];
-      }
-      const double* q = partial + 2 * n_pt_blocks;
-      for (int i = 0; i < n_pose_blocks; ++i) {
-        dd += q[2 * i];
-        dg += q[2 * i + 1];
-      }
       sc->dd = dd;
       sc->dg = dg;
     }
   } else if (what == 1) {  // cost
-    if (threadIdx.x == 0) {
-      double c = 0;
-      for (int i = 0; i < n_cost_blocks; ++i) c += partial[2 * i];
-      sc->cost = c;
-    }
+    double c = 0;
+    for (int i = threadIdx.x; i < n_cost_blocks; i += blockDim.x) c += partial[2 * i];
+    c = block_sum_fixed(c, red);
+    if (threadIdx.x == 0) sc->cost = c;
   } else {  // max |neg_grad| over the cameras
     double m = 0;
     for (int e = threadIdx.x; e < 6 * nb; e += blockDim.x) m = fmax(m, fabs(grad_cam[e]));
@@ -779,16 +1061,19 @@ __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const doubl
     }
   }
   const int cm = p.pose_block[g.m[t]], cn = p.pose_block[g.nn[t]];
-  const int* dst = g.dest + 3 * t;
+  const int* dst = g.dest + 4 * t;
   const double damp = g.first[t] ? nu : 0.0;
-  auto add = [&](int code, double sign, double diag) {
-    if (code < 0) return;
-    double* B = C + 36 * (code >> 1);
+  // H_mm = H_nn = D^T D (+ nu I), H_mn = J_m^T J_n = -D^T D (symmetric, so the stored
+  // orientation of the (max, min) block does not matter)
+  auto add = [&](int slot, double sign, double diag) {
+    if (slot < 0) return;
+    double* B = C + p.s_off[slot];
+    const int ldc = p.s_ld[slot], ld = ldc >> 1;
     for (int e = 0; e < 36; ++e) {
-      const int idx = (code & 1) ? (e % 6) * 6 + e / 6 : e;
+      const int a = e / 6, b = e % 6;
       double v = sign * DtD[e];
-      if (e % 7 == 0) v += diag;
-      if (v != 0.0) atomic_add_d(B + idx, v);
+      if (a == b) v += diag;
+      if (v != 0.0) atomic_add_d(B + ((ldc & 1) ? a * ld + b : b * ld + a), v);
     }
   };
   add(dst[0], 1.0, damp);
@@ -911,25 +1196,26 @@ void launch_point_prep(hipStream_t s, int precision, const DevProblem& p, const 
   if (p.npt == 0) return;
   const dim3 g(blocks_for(p.npt, kThreads)), b(kThreads);
   if (precision == 0)
-    hipLaunchKernelGGL(k_point_prep<double>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.pt_s,
-                       w.pt_y, sc);
+    hipLaunchKernelGGL(k_point_prep<double>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.hll,
+                       w.pt_s, w.pt_y, sc);
   else
-    hipLaunchKernelGGL(k_point_prep<float>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.pt_s,
-                       w.pt_y, sc);
+    hipLaunchKernelGGL(k_point_prep<float>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.hll,
+                       w.pt_s, w.pt_y, sc);
 }
 
 void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const double* points,
                         const double* intr, const double* cw, double nu, DevLin& w, DevFactor& f,
                         DevScalars* sc) {
   (void)sc;
-  if (p.nb == 0) return;
-  const int32_t* obs_point = p.obs_point;
+  if (p.nslot == 0) return;
+  const int wpb = kThreads / 64;
+  const dim3 g(unsigned((p.nslot + wpb - 1) / wpb));
   if (precision == 0)
-    hipLaunchKernelGGL(k_camera_rows<double>, dim3(p.nb), dim3(kThreads), 0, s, p, obs_point,
-                       points, intr, cw, nu, w.hinv, w.pt_y, f.C, w.rhs_nat, w.grad_cam, sc);
+    hipLaunchKernelGGL(k_schur_slots<double>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
+                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam);
   else
-    hipLaunchKernelGGL(k_camera_rows<float>, dim3(p.nb), dim3(kThreads), 0, s, p, obs_point,
-                       points, intr, cw, nu, w.hinv, w.pt_y, f.C, w.rhs_nat, w.grad_cam, sc);
+    hipLaunchKernelGGL(k_schur_slots<float>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
+                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam);
 }
 
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v) {
@@ -942,16 +1228,58 @@ void launch_zero_scalars(hipStream_t s, DevScalars* sc) {
   hipLaunchKernelGGL(k_zero_scalars, dim3(1), dim3(64), 0, s, sc);
 }
 
-void launch_factor_level(hipStream_t s, const DevFactor& f, int level, DevScalars* sc) {
-  const int ncol = f.h_lvl_ptr[level + 1] - f.h_lvl_ptr[level];
-  if (ncol == 0) return;
-  hipLaunchKernelGGL(k_factor_level, dim3(ncol), dim3(kThreads), 0, s, f, level, sc);
+void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
+  const Symbolic& S = *f.host;
+  int level = -1;
+  for (const auto& L : S.launches) {
+    if (L.level != level) {
+      // extend-add of the previous level's nodes into their parents, then the first
+      // panel diagonal of this level's nodes
+      if (level >= 0) {
+        const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
+        hipLaunchKernelGGL(k_mf_extend, dim3(32, n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level]);
+      }
+      level = L.level;
+      const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
+      hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level], sc);
+    }
+    if (L.tile_count > 0)
+      hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
+                         f.launch_tiles + 3 * int64_t(L.tile_begin), sc);
+  }
 }
 
-void launch_triangular_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat,
-                             double* x_perm, double* x_aux, double* dx_cam) {
-  (void)x_aux;
-  hipLaunchKernelGGL(k_triangular_solve, dim3(1), dim3(1024), 0, s, f, rhs_nat, x_perm, dx_cam);
+void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, double* y,
+                      double* dx_cam) {
+  const Symbolic& S = *f.host;
+  const int g = blocks_for(6 * int64_t(f.nb), kThreads);
+  hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, rhs_nat, y);
+  const int nlev = int(S.lvl_ptr.size()) - 1;
+  for (int l = 0; l < nlev; ++l) {
+    const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
+    hipLaunchKernelGGL(k_mf_fwd_tri, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[l], y,
+                       f.ytmp);
+    const int nr = S.solve_rows_ptr[l + 1] - S.solve_rows_ptr[l];
+    if (nr > 0)
+      hipLaunchKernelGGL(k_mf_fwd_rect, dim3(nr), dim3(kThreads), 0, s, f,
+                         f.solve_rows + 2 * int64_t(S.solve_rows_ptr[l]), y, f.ytmp);
+  }
+  for (int l = nlev - 1; l >= 0; --l) {
+    const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
+    const int nc = S.solve_cols_ptr[l + 1] - S.solve_cols_ptr[l];
+    if (nc > 0)
+      hipLaunchKernelGGL(k_mf_bwd_rect, dim3(nc), dim3(kThreads), 0, s, f,
+                         f.solve_cols + 2 * int64_t(S.solve_cols_ptr[l]), y);
+    hipLaunchKernelGGL(k_mf_bwd_tri, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[l], y);
+  }
+  hipLaunchKernelGGL(k_permute_out, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, y, dx_cam);
+}
+
+void launch_scatter_s(hipStream_t s, const DevProblem& p, int64_t nslots, const double* blocks,
+                      DevFactor& f) {
+  if (nslots == 0) return;
+  hipLaunchKernelGGL(k_scatter_s, dim3(blocks_for(nslots * 36, kThreads)), dim3(kThreads), 0, s, p,
+                     nslots, blocks, f.front);
 }
 
 void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p, const double* points,
@@ -1002,7 +1330,7 @@ void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const Dev
                                  const double* poses, double nu, DevLin& w, DevFactor& f) {
   if (g.n == 0) return;
   hipLaunchKernelGGL(k_pose_graph_linearize, dim3(blocks_for(g.n, 128)), dim3(128), 0, s, g, p,
-                     poses, nu, f.C, w.rhs_nat, w.grad_cam);
+                     poses, nu, f.front, w.rhs_nat, w.grad_cam);
 }
 
 void launch_pose_graph_cost(hipStream_t s, const DevPoseGraph& g, const double* poses,

@@ -5,6 +5,8 @@
 
 #include <cstdint>
 
+#include "problem.hpp"
+
 namespace sgpu {
 
 // Per-iteration device scalars (one struct in device memory, copied back once per step).
@@ -36,37 +38,48 @@ struct DevProblem {
   const int32_t* cam_obs;
   const int32_t* s_ptr;
   const int32_t* s_col;
-  const int32_t* s_dest;
+  const int32_t* s_row;     // slot -> row camera
+  const int64_t* pair_ptr;  // slot -> pair range
+  const int32_t* pairs;     // (obs_i, obs_j) per pair
+  int64_t nslot;
+  const int64_t* s_off;
+  const int32_t* s_ld;
   int robust;
   double delta;
 };
 
-// RCS factorization structure on the device.
+// RCS supernodal multifrontal factorization on the device.
 struct DevFactor {
-  int32_t nb, nlev, nnz_l;
-  const int32_t* perm;      // perm[new] = old
-  const int32_t* l_ptr;
-  const int32_t* l_row;
-  const int32_t* r_ptr;
-  const int32_t* r_col;
-  const int32_t* r_lpos;
-  const int32_t* u_ptr;
-  const int32_t* u_src;
-  const int32_t* u_m;
-  const int32_t* lvl_ptr;
-  const int32_t* lvl_col;
-  const int32_t* h_lvl_ptr;  // host copy of lvl_ptr (launch sizes)
-  double* C;       // unified blocks: nb diagonal C's then nnz_l L blocks (36 each)
-  double* M;       // per reach entry: D_k L_jk^T (36 each)
-  double* Ljj;     // nb x 36 unit-lower
-  double* Dfull;   // nb x 36
-  double* dvec;    // nb x 6
+  int32_t nb, nsn;
+  const int32_t* perm;  // perm[new] = old
+  const int32_t* sn_c0;
+  const int32_t* sn_nc;
+  const int32_t* sn_nf;
+  const int64_t* sn_off;
+  const int64_t* sn_loff;
+  const int32_t* sn_rows_ptr;
+  const int32_t* sn_rows;
+  const int32_t* sn_parent;
+  const int32_t* sn_upos_ptr;
+  const int32_t* sn_upos;
+  const int32_t* lvl_sn;
+  const int32_t* launch_tiles;
+  const int32_t* solve_rows;  // (node, 256-row tile) per level, rows below the node
+  const int32_t* solve_cols;  // (node, 64-column tile) per level
+  double* ytmp;               // nb x 6 solve scratch
+  double* front;   // all fronts, column-major (workspace)
+  double* lbuf;    // per node F x K: the factor's L columns (column-major, ld F)
+  double* piv_L;   // nb x 36 unit-lower pivot factors (row-major)
+  double* piv_d;   // nb x 6
+  int64_t front_elems;
+  const Symbolic* host;  // launch schedule
 };
 
 // Linearization workspace.
 struct DevLin {
   double* cw;         // npose x 12: camera_from_world rotation (row-major) + translation
   double* hinv;       // npt x 9
+  double* hll;        // npt x 6: damped H_ll (00, 01, 02, 11, 12, 22)
   double* pt_s;       // npt x 3: neg-gradient of the point (sum of A^T r~)
   double* pt_y;       // npt x 3: hinv * s
   double* rhs_nat;    // nb x 6: reduced rhs in camera order
@@ -87,9 +100,11 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
                         const double* intr, const double* cw, double nu, DevLin& w,
                         DevFactor& f, DevScalars* sc);
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v);
-void launch_factor_level(hipStream_t s, const DevFactor& f, int level, DevScalars* sc);
-void launch_triangular_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat,
-                             double* x_perm, double* x_aux, double* dx_cam);
+// Full numeric factorization (all levels and panels) of the assembled fronts.
+void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc);
+// Solve with the factor: rhs in camera order -> dx in camera order (y: nb*6 scratch).
+void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, double* y,
+                      double* dx_cam);
 void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p,
                            const double* points, const double* intr, const double* cw,
                            const DevLin& w, double* points_new, DevScalars* sc);
@@ -109,12 +124,15 @@ struct DevPoseGraph {
   const int32_t* nn;
   const double* meas;
   const uint8_t* first;
-  const int32_t* dest;  // 3 per term: (m,m), (n,n), (m,n) unified codes or -1
+  const int32_t* dest;  // 4 per term: S slots of (m,m), (n,n), (max,min) or -1; flip
   int robust;
   double delta;
 };
 void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const DevProblem& p,
                                  const double* poses, double nu, DevLin& w, DevFactor& f);
+// Scatter S blocks given per slot (36 doubles, row-major, camera order) into the fronts.
+void launch_scatter_s(hipStream_t s, const DevProblem& p, int64_t nslots, const double* blocks,
+                      DevFactor& f);
 void launch_pose_graph_cost(hipStream_t s, const DevPoseGraph& g, const double* poses,
                             double* partial, int32_t partial_cap, DevScalars* sc);
 

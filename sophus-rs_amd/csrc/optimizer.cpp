@@ -142,7 +142,7 @@ struct sgpu_optimizer {
   void linearize(const double* ps, const double* pts, double nu_lin, bool* used) {
     launch_zero_scalars(stream, d_sc);
     launch_cam_cache(stream, S.npose, ps, dl.cw);
-    launch_fill(stream, df.C, int64_t(36) * (S.nb + df.nnz_l), 0.0);
+    launch_fill(stream, df.front, df.front_elems, 0.0);
     tic(K_LINEARIZE_POINTS);
     launch_point_prep(stream, params.precision, dp, ps, pts, intr, dl.cw, nu_lin, dl, d_sc);
     toc(K_LINEARIZE_POINTS);
@@ -228,33 +228,48 @@ struct sgpu_optimizer {
     dp.cam_obs = arena.upload(S.cam_obs, stream);
     dp.s_ptr = arena.upload(S.s_ptr, stream);
     dp.s_col = arena.upload(S.s_col, stream);
-    dp.s_dest = arena.upload(S.s_dest, stream);
+    {
+      std::vector<int32_t> srow(S.s_col.size());
+      for (int32_t c = 0; c < S.nb; ++c)
+        for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k) srow[k] = c;
+      dp.s_row = arena.upload(srow, stream);
+    }
+    dp.pair_ptr = arena.upload(S.pair_ptr, stream);
+    dp.pairs = arena.upload(S.pairs, stream);
+    dp.nslot = int64_t(S.s_col.size());
+    dp.s_off = arena.upload(S.s_off, stream);
+    dp.s_ld = arena.upload(S.s_ld, stream);
     dp.robust = P.robust;
     dp.delta = P.delta;
     // factor structure
     df.nb = S.nb;
-    df.nlev = int32_t(S.lvl_ptr.size()) - 1;
-    df.nnz_l = S.l_ptr[S.nb];
+    df.nsn = S.nsn;
     df.perm = arena.upload(S.perm, stream);
-    df.l_ptr = arena.upload(S.l_ptr, stream);
-    df.l_row = arena.upload(S.l_row, stream);
-    df.r_ptr = arena.upload(S.r_ptr, stream);
-    df.r_col = arena.upload(S.r_col, stream);
-    df.r_lpos = arena.upload(S.r_lpos, stream);
-    df.u_ptr = arena.upload(S.u_ptr, stream);
-    df.u_src = arena.upload(S.u_src, stream);
-    df.u_m = arena.upload(S.u_m, stream);
-    df.lvl_ptr = arena.upload(S.lvl_ptr, stream);
-    df.lvl_col = arena.upload(S.lvl_col, stream);
-    df.h_lvl_ptr = S.lvl_ptr.data();
-    df.C = arena.alloc<double>(size_t(36) * (S.nb + df.nnz_l));
-    df.M = arena.alloc<double>(size_t(36) * std::max<size_t>(1, S.r_col.size()));
-    df.Ljj = arena.alloc<double>(size_t(36) * S.nb);
-    df.Dfull = arena.alloc<double>(size_t(36) * S.nb);
-    df.dvec = arena.alloc<double>(size_t(6) * S.nb);
+    df.sn_c0 = arena.upload(S.sn_c0, stream);
+    df.sn_nc = arena.upload(S.sn_nc, stream);
+    df.sn_nf = arena.upload(S.sn_nf, stream);
+    df.sn_off = arena.upload(S.sn_off, stream);
+    df.sn_loff = arena.upload(S.sn_loff, stream);
+    df.sn_rows_ptr = arena.upload(S.sn_rows_ptr, stream);
+    df.sn_rows = arena.upload(S.sn_rows, stream);
+    df.sn_parent = arena.upload(S.sn_parent, stream);
+    df.sn_upos_ptr = arena.upload(S.sn_upos_ptr, stream);
+    df.sn_upos = arena.upload(S.sn_upos, stream);
+    df.lvl_sn = arena.upload(S.lvl_sn, stream);
+    df.launch_tiles = arena.upload(S.launch_tiles, stream);
+    df.solve_rows = arena.upload(S.solve_rows, stream);
+    df.solve_cols = arena.upload(S.solve_cols, stream);
+    df.ytmp = arena.alloc<double>(size_t(6) * S.nb);
+    df.front = arena.alloc<double>(size_t(S.front_elems));
+    df.lbuf = arena.alloc<double>(size_t(S.l_elems));
+    df.piv_L = arena.alloc<double>(size_t(36) * S.nb);
+    df.piv_d = arena.alloc<double>(size_t(6) * S.nb);
+    df.front_elems = S.front_elems;
+    df.host = &S;
     // workspace
     dl.cw = arena.alloc<double>(size_t(12) * S.npose);
     dl.hinv = arena.alloc<double>(size_t(9) * S.npt);
+    dl.hll = arena.alloc<double>(size_t(6) * S.npt);
     dl.pt_s = arena.alloc<double>(size_t(3) * S.npt);
     dl.pt_y = arena.alloc<double>(size_t(3) * S.npt);
     dl.rhs_nat = arena.alloc<double>(size_t(6) * S.nb);
@@ -275,7 +290,7 @@ struct sgpu_optimizer {
       q.meas = arena.upload(g.meas, stream);
       q.first = arena.upload(g.first, stream);
       std::vector<int32_t> dest;
-      for (auto& d : S.pg_dest[i]) dest.insert(dest.end(), {d.mm, d.nn, d.mn});
+      for (auto& d : S.pg_dest[i]) dest.insert(dest.end(), {d.mm, d.nn, d.mn, d.mn_flip});
       q.dest = arena.upload(dest, stream);
       q.robust = g.robust;
       q.delta = g.delta;
@@ -294,11 +309,11 @@ struct sgpu_optimizer {
     linearize(poses[cur], points[cur], nu, used);
     SGPU_HIP(hipEventRecord(phase_ev[1], stream));
     tic(K_FACTOR);
-    for (int l = 0; l < df.nlev; ++l) launch_factor_level(stream, df, l, d_sc);
+    launch_rcs_factor(stream, df, d_sc);
     toc(K_FACTOR);
     SGPU_HIP(hipEventRecord(phase_ev[2], stream));
     tic(K_TRIANGULAR);
-    launch_triangular_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.x_aux, dl.dx_cam);
+    launch_rcs_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.dx_cam);
     toc(K_TRIANGULAR);
     tic(K_BACKSUB);
     launch_backsub_points(stream, params.precision, dp, points[cur], intr, dl.cw, dl, points[nxt], d_sc);
@@ -507,21 +522,38 @@ sgpu_status sgpu_optimizer_build_solution(sgpu_optimizer* opt, sgpu_solution* so
   });
 }
 
+static void fill_stats(const HostProblem& P, const Symbolic& S, sgpu_rcs_stats* st) {
+  st->num_free_blocks = S.nb;
+  st->num_marg_blocks = P.num_marg_blocks;
+  st->num_observations = S.nobs;
+  int64_t npg = 0;
+  for (auto& g : P.pg) npg += int64_t(g.m.size());
+  st->num_pose_terms = npg;
+  st->nnz_s_blocks = int64_t(S.s_col.size());
+  st->nnz_l_blocks = S.nnz_l;
+  st->num_schur_pairs = S.num_schur_pairs;
+  st->num_levels = int64_t(S.lvl_ptr.size()) - 1;
+  st->factor_flops = S.factor_flops;
+  st->num_supernodes = S.nsn;
+  st->max_front_blocks = S.max_front;
+  st->front_bytes = S.front_elems * 8;
+}
+
 sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* st) {
   return guarded([&] {
     if (!opt || !st) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
-    const Symbolic& S = opt->S;
-    st->num_free_blocks = S.nb;
-    st->num_marg_blocks = opt->P.num_marg_blocks;
-    st->num_observations = S.nobs;
-    int64_t npg = 0;
-    for (auto& g : opt->P.pg) npg += int64_t(g.m.size());
-    st->num_pose_terms = npg;
-    st->nnz_s_blocks = int64_t(S.s_col.size());
-    st->nnz_l_blocks = S.l_ptr[S.nb];
-    st->num_schur_pairs = S.num_schur_pairs;
-    st->num_levels = int64_t(S.lvl_ptr.size()) - 1;
-    st->factor_flops = S.factor_flops;
+    fill_stats(opt->P, opt->S, st);
+  });
+}
+
+sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
+                         const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                         sgpu_rcs_stats* st) {
+  return guarded([&] {
+    if (!st) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    const HostProblem P = build_host_problem(families, num_families, cost_fns, num_cost_fns);
+    const Symbolic S = analyze(P);
+    fill_stats(P, S, st);
   });
 }
 

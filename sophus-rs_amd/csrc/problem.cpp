@@ -362,11 +362,6 @@ Symbolic analyze(const HostProblem& P) {
     }
     for (size_t i = 0; i < cams.size(); ++i)
       for (size_t j = 0; j < i; ++j) rows[cams[i]].push_back(cams[j]);
-    // pairs counted per (obs, obs) with cam_j <= cam_i, as the Schur kernel visits them
-    int64_t k = 0;
-    for (int32_t o = S.pt_ptr[b]; o < S.pt_ptr[b + 1]; ++o)
-      if (poses.block[P.obs_pose[o]] >= 0) ++k;
-    S.num_schur_pairs += k * (k + 1) / 2;
   }
   for (const auto& g : P.pg)
     for (size_t t = 0; t < g.m.size(); ++t) {
@@ -382,8 +377,45 @@ Symbolic analyze(const HostProblem& P) {
   }
   S.s_col.resize(S.s_ptr[S.nb]);
   for (int32_t c = 0; c < S.nb; ++c) std::copy(rows[c].begin(), rows[c].end(), &S.s_col[S.s_ptr[c]]);
+  // per-slot observation pairs (two passes: count, then fill in point order)
+  {
+    const int64_t nslot = int64_t(S.s_col.size());
+    auto slot_of = [&](int32_t r, int32_t c) -> int64_t {
+      auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
+      return int64_t(std::lower_bound(b, e, c) - S.s_col.begin());
+    };
+    S.pair_ptr.assign(nslot + 1, 0);
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<int64_t> fill;
+      if (pass == 1) {
+        for (int64_t k = 0; k < nslot; ++k) S.pair_ptr[k + 1] += S.pair_ptr[k];
+        S.pairs.assign(2 * S.pair_ptr[nslot], 0);
+        fill.assign(S.pair_ptr.begin(), S.pair_ptr.end() - 1);
+      }
+      for (int32_t b = 0; b < S.npt; ++b) {
+        const bool active = points.block[b] >= 0;
+        for (int32_t i = S.pt_ptr[b]; i < S.pt_ptr[b + 1]; ++i) {
+          const int32_t ci = poses.block[P.obs_pose[i]];
+          if (ci < 0) continue;
+          for (int32_t j = S.pt_ptr[b]; j < S.pt_ptr[b + 1]; ++j) {
+            const int32_t cj = poses.block[P.obs_pose[j]];
+            if (cj < 0 || cj > ci || (!active && j != i)) continue;
+            const int64_t k = slot_of(ci, cj);
+            if (pass == 0) {
+              ++S.pair_ptr[k + 1];
+            } else {
+              const int64_t at = fill[k]++;
+              S.pairs[2 * at] = i;
+              S.pairs[2 * at + 1] = j;
+            }
+          }
+        }
+      }
+    }
+  }
+  S.num_schur_pairs = int64_t(S.pairs.size() / 2);
   analyze_rcs(S);
-  // pose-graph destinations
+  // pose-graph destinations (S slots)
   auto s_slot = [&](int32_t r, int32_t c) -> int32_t {
     auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
     auto it = std::lower_bound(b, e, c);
@@ -393,15 +425,10 @@ Symbolic analyze(const HostProblem& P) {
     std::vector<Symbolic::PgDest> d(g.m.size());
     for (size_t t = 0; t < g.m.size(); ++t) {
       const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
-      d[t].mm = a >= 0 ? S.s_dest[s_slot(a, a)] : -1;
-      d[t].nn = b >= 0 ? S.s_dest[s_slot(b, b)] : -1;
-      if (a >= 0 && b >= 0) {
-        // block (m, n) stored at lower (max, min); transpose when m is the column
-        const int32_t code = S.s_dest[s_slot(std::max(a, b), std::min(a, b))];
-        d[t].mn = a > b ? code : (code ^ 1);
-      } else {
-        d[t].mn = -1;
-      }
+      d[t].mm = a >= 0 ? s_slot(a, a) : -1;
+      d[t].nn = b >= 0 ? s_slot(b, b) : -1;
+      d[t].mn = (a >= 0 && b >= 0) ? s_slot(std::max(a, b), std::min(a, b)) : -1;
+      d[t].mn_flip = a < b ? 1 : 0;
     }
     S.pg_dest.push_back(std::move(d));
   }
@@ -420,10 +447,10 @@ void analyze_rcs(Symbolic& S) {
         adj[c].push_back(S.s_col[k]);
         adj[S.s_col[k]].push_back(c);
       }
-  S.perm = nested_dissection(S.nb, adj, 4);
+  S.perm = nested_dissection(S.nb, adj, 24);
   S.perm_inv.assign(S.nb, 0);
   for (int32_t i = 0; i < S.nb; ++i) S.perm_inv[S.perm[i]] = i;
-  // permuted lower pattern by column: upper[i] = columns j < i with A(i, j) != 0
+  // permuted lower pattern: acol[j] = rows i > j of A(:, j); upper[i] = columns j < i
   std::vector<std::vector<int32_t>> acol(S.nb), upper(S.nb);
   for (int32_t c = 0; c < S.nb; ++c)
     for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k) {
@@ -434,8 +461,10 @@ void analyze_rcs(Symbolic& S) {
         upper[i].push_back(j);
       }
     }
-  // elimination tree
-  std::vector<int32_t> parent(S.nb, -1), ancestor(S.nb, -1);
+  // elimination tree (elimination_tree.rs:20-60)
+  std::vector<int32_t>& parent = S.etree_parent;
+  parent.assign(S.nb, -1);
+  std::vector<int32_t> ancestor(S.nb, -1);
   for (int32_t j = 0; j < S.nb; ++j) {
     std::sort(upper[j].begin(), upper[j].end());
     for (int32_t i0 : upper[j]) {
@@ -451,110 +480,158 @@ void analyze_rcs(Symbolic& S) {
       }
     }
   }
-  // column structure of L: struct(A(:,j)) U struct(L(:,child)) minus {<= j}
+  // column structure of L: struct(A(:,j)) U struct(L(:,child)) minus rows <= j
   std::vector<std::vector<int32_t>> lcol(S.nb), children(S.nb);
   for (int32_t j = 0; j < S.nb; ++j)
     if (parent[j] >= 0) children[parent[j]].push_back(j);
+  S.nnz_l = 0;
   for (int32_t j = 0; j < S.nb; ++j) {
-    std::vector<int32_t> s = acol[j];
+    std::vector<int32_t> st = acol[j];
     for (int32_t c : children[j])
       for (int32_t i : lcol[c])
-        if (i > j) s.push_back(i);
-    std::sort(s.begin(), s.end());
-    s.erase(std::unique(s.begin(), s.end()), s.end());
-    lcol[j].swap(s);
+        if (i > j) st.push_back(i);
+    std::sort(st.begin(), st.end());
+    st.erase(std::unique(st.begin(), st.end()), st.end());
+    lcol[j].swap(st);
+    S.nnz_l += int64_t(lcol[j].size());
   }
-  S.l_ptr.assign(S.nb + 1, 0);
-  for (int32_t j = 0; j < S.nb; ++j) S.l_ptr[j + 1] = S.l_ptr[j] + int32_t(lcol[j].size());
-  S.l_row.resize(S.l_ptr[S.nb]);
-  for (int32_t j = 0; j < S.nb; ++j) std::copy(lcol[j].begin(), lcol[j].end(), &S.l_row[S.l_ptr[j]]);
-  auto lpos = [&](int32_t i, int32_t j) -> int32_t {  // L index of block (i, j), i > j
-    auto b = S.l_row.begin() + S.l_ptr[j], e = S.l_row.begin() + S.l_ptr[j + 1];
-    auto it = std::lower_bound(b, e, i);
-    if (it == e || *it != i) return -1;
-    return int32_t(it - S.l_row.begin());
-  };
-  // reach lists (row structure of L), ascending k
-  std::vector<std::vector<std::pair<int32_t, int32_t>>> reach(S.nb);
-  for (int32_t k = 0; k < S.nb; ++k)
-    for (int32_t p = S.l_ptr[k]; p < S.l_ptr[k + 1]; ++p) reach[S.l_row[p]].push_back({k, p});
-  S.r_ptr.assign(S.nb + 1, 0);
-  for (int32_t j = 0; j < S.nb; ++j) S.r_ptr[j + 1] = S.r_ptr[j] + int32_t(reach[j].size());
-  S.r_col.resize(S.r_ptr[S.nb]);
-  S.r_lpos.resize(S.r_ptr[S.nb]);
-  std::vector<int32_t> r_index_of_lpos(S.l_ptr[S.nb], -1);  // L(j,k) -> reach entry of j
-  for (int32_t j = 0; j < S.nb; ++j)
-    for (size_t e = 0; e < reach[j].size(); ++e) {
-      const int32_t at = S.r_ptr[j] + int32_t(e);
-      S.r_col[at] = reach[j][e].first;
-      S.r_lpos[at] = reach[j][e].second;
-      r_index_of_lpos[reach[j][e].second] = at;
-    }
-  // gather-form update lists: target (i, j) <- L(i, k) * M(j, k) for i, j in struct(L(:,k)), i >= j
-  const int32_t nslots = S.nb + S.l_ptr[S.nb];
-  std::vector<int32_t> ucount(nslots + 1, 0);
-  for (int32_t k = 0; k < S.nb; ++k) {
-    const int32_t m = S.l_ptr[k + 1] - S.l_ptr[k];
-    for (int32_t b = 0; b < m; ++b) {
-      const int32_t j = S.l_row[S.l_ptr[k] + b];
-      ucount[j + 1]++;  // diagonal target
-      for (int32_t a = b + 1; a < m; ++a) ucount[S.nb + lpos(S.l_row[S.l_ptr[k] + a], j) + 1]++;
-    }
-    S.factor_flops += double(m) * (m + 1) / 2.0 * 432.0 + double(m) * 432.0;
+  // supernodes: runs j, j+1, ... with parent(j) == j+1.  Along such a chain
+  // struct(L(:,k)) \ {k+1} is contained in struct(L(:,k+1)), so the run's columns plus
+  // the structure of its last column hold every column's structure; entries outside a
+  // column's own structure stay exact zeros through the dense front updates.
+  S.sn_c0.clear();
+  S.sn_nc.clear();
+  S.blk_sn.assign(S.nb, -1);
+  constexpr int32_t kMaxSupernode = 96;
+  for (int32_t j = 0; j < S.nb;) {
+    int32_t e = j + 1;
+    while (e < S.nb && parent[e - 1] == e && e - j < kMaxSupernode) ++e;
+    const int32_t sn = int32_t(S.sn_c0.size());
+    S.sn_c0.push_back(j);
+    S.sn_nc.push_back(e - j);
+    for (int32_t k = j; k < e; ++k) S.blk_sn[k] = sn;
+    j = e;
   }
-  S.factor_flops += double(S.nb) * 300.0;
-  for (int32_t s = 0; s < nslots; ++s) ucount[s + 1] += ucount[s];
-  S.u_ptr = ucount;
-  S.u_src.resize(ucount[nslots]);
-  S.u_m.resize(ucount[nslots]);
-  {
-    std::vector<int32_t> fill(ucount.begin(), ucount.end() - 1);
-    for (int32_t k = 0; k < S.nb; ++k) {  // ascending k: each target's list is k-sorted
-      const int32_t m = S.l_ptr[k + 1] - S.l_ptr[k];
-      for (int32_t b = 0; b < m; ++b) {
-        const int32_t pj = S.l_ptr[k] + b;  // L(j, k)
-        const int32_t j = S.l_row[pj];
-        const int32_t mi = r_index_of_lpos[pj];
-        int32_t at = fill[j]++;
-        S.u_src[at] = pj;
-        S.u_m[at] = mi;
-        for (int32_t a = b + 1; a < m; ++a) {
-          const int32_t pi = S.l_ptr[k] + a;  // L(i, k)
-          const int32_t tgt = S.nb + lpos(S.l_row[pi], j);
-          at = fill[tgt]++;
-          S.u_src[at] = pi;
-          S.u_m[at] = mi;
-        }
+  S.nsn = int32_t(S.sn_c0.size());
+  S.sn_nf.assign(S.nsn, 0);
+  S.sn_parent.assign(S.nsn, -1);
+  S.sn_rows_ptr.assign(S.nsn + 1, 0);
+  S.sn_rows.clear();
+  for (int32_t x = 0; x < S.nsn; ++x) {
+    const int32_t c0 = S.sn_c0[x], nc = S.sn_nc[x], last = c0 + nc - 1;
+    for (int32_t k = c0; k <= last; ++k) S.sn_rows.push_back(k);
+    for (int32_t i : lcol[last]) S.sn_rows.push_back(i);
+    S.sn_nf[x] = nc + int32_t(lcol[last].size());
+    S.sn_rows_ptr[x + 1] = int32_t(S.sn_rows.size());
+    if (!lcol[last].empty()) S.sn_parent[x] = S.blk_sn[lcol[last][0]];
+    S.max_front = std::max(S.max_front, S.sn_nf[x]);
+  }
+  // positions of each node's update rows in its parent's front
+  S.sn_upos_ptr.assign(S.nsn + 1, 0);
+  S.sn_upos.clear();
+  for (int32_t x = 0; x < S.nsn; ++x) {
+    const int32_t p = S.sn_parent[x];
+    if (p >= 0) {
+      const int32_t* prow = &S.sn_rows[S.sn_rows_ptr[p]];
+      const int32_t pn = S.sn_nf[p];
+      for (int32_t q = S.sn_rows_ptr[x] + S.sn_nc[x]; q < S.sn_rows_ptr[x + 1]; ++q) {
+        const int32_t r = S.sn_rows[q];
+        const int32_t pos = int32_t(std::lower_bound(prow, prow + pn, r) - prow);
+        if (pos >= pn || prow[pos] != r) throw Error(SGPU_ERR_INVALID_ARGUMENT, "front structure");
+        S.sn_upos.push_back(pos);
       }
     }
+    S.sn_upos_ptr[x + 1] = int32_t(S.sn_upos.size());
   }
-  // level schedule
-  std::vector<int32_t> level(S.nb, 0);
+  // front storage (column-major F x F) and the factor's L columns (F x K per node)
+  S.sn_off.assign(S.nsn, 0);
+  S.sn_loff.assign(S.nsn, 0);
+  S.front_elems = 0;
+  S.l_elems = 0;
+  S.factor_flops = 0;
+  for (int32_t x = 0; x < S.nsn; ++x) {
+    const int64_t F = 6 * int64_t(S.sn_nf[x]), K = 6 * int64_t(S.sn_nc[x]);
+    S.sn_off[x] = S.front_elems;
+    S.sn_loff[x] = S.l_elems;
+    S.front_elems += F * F;
+    S.l_elems += F * K;
+    // K pivots of a front of F rows: ~ sum_k (F - k)^2 multiply-adds on the lower part
+    for (int64_t k = 0; k < K; ++k) S.factor_flops += double(F - k) * double(F - k);
+  }
+  // levels of the supernodal tree (children before parents)
+  S.sn_level.assign(S.nsn, 0);
   int32_t nlev = 0;
-  for (int32_t j = 0; j < S.nb; ++j) {
-    if (parent[j] >= 0) level[parent[j]] = std::max(level[parent[j]], level[j] + 1);
-    nlev = std::max(nlev, level[j] + 1);
+  for (int32_t x = 0; x < S.nsn; ++x) {  // parents have larger indices
+    if (S.sn_parent[x] >= 0)
+      S.sn_level[S.sn_parent[x]] = std::max(S.sn_level[S.sn_parent[x]], S.sn_level[x] + 1);
+    nlev = std::max(nlev, S.sn_level[x] + 1);
   }
   S.lvl_ptr.assign(nlev + 1, 0);
-  for (int32_t j = 0; j < S.nb; ++j) ++S.lvl_ptr[level[j] + 1];
+  for (int32_t x = 0; x < S.nsn; ++x) ++S.lvl_ptr[S.sn_level[x] + 1];
   for (int32_t l = 0; l < nlev; ++l) S.lvl_ptr[l + 1] += S.lvl_ptr[l];
-  S.lvl_col.resize(S.nb);
+  S.lvl_sn.assign(S.nsn, 0);
   {
     std::vector<int32_t> fill(S.lvl_ptr.begin(), S.lvl_ptr.end() - 1);
-    for (int32_t j = 0; j < S.nb; ++j) S.lvl_col[fill[level[j]]++] = j;
+    for (int32_t x = 0; x < S.nsn; ++x) S.lvl_sn[fill[S.sn_level[x]]++] = x;
   }
-  // S slot -> unified destination (diag j, or nb + L index), transpose flag
-  S.s_dest.resize(S.s_col.size());
+  // launch schedule: per level, per panel: the nodes with that panel and the lower
+  // triangle of trailing tiles
+  S.launches.clear();
+  S.launch_tiles.clear();
+  for (int32_t l = 0; l < nlev; ++l) {
+    int32_t maxpan = 0;
+    for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q)
+      maxpan = std::max(maxpan, (S.sn_nc[S.lvl_sn[q]] + kPanelBlocks - 1) / kPanelBlocks);
+    for (int32_t pn = 0; pn < maxpan; ++pn) {
+      Symbolic::PanelLaunch L{l, pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0};
+      for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
+        const int32_t x = S.lvl_sn[q];
+        const int32_t npan = (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks;
+        if (pn >= npan) continue;
+        const int32_t kend = 6 * std::min(S.sn_nc[x], (pn + 1) * kPanelBlocks);
+        const int32_t T = 6 * S.sn_nf[x] - kend;
+        const int32_t nt = (T + kTile - 1) / kTile;
+        for (int32_t tr = 0; tr < nt; ++tr)
+          for (int32_t tc = 0; tc <= tr; ++tc) {
+            S.launch_tiles.insert(S.launch_tiles.end(), {x, tr, tc});
+            ++L.tile_count;
+          }
+      }
+      S.launches.push_back(L);
+    }
+  }
+  // solve work items
+  S.solve_rows_ptr.assign(nlev + 1, 0);
+  S.solve_cols_ptr.assign(nlev + 1, 0);
+  S.solve_rows.clear();
+  S.solve_cols.clear();
+  for (int32_t l = 0; l < nlev; ++l) {
+    for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
+      const int32_t x = S.lvl_sn[q];
+      const int32_t m = 6 * (S.sn_nf[x] - S.sn_nc[x]), K = 6 * S.sn_nc[x];
+      for (int32_t t = 0; t * 256 < m; ++t) S.solve_rows.insert(S.solve_rows.end(), {x, t});
+      if (m > 0)
+        for (int32_t t = 0; t * kTile < K; ++t) S.solve_cols.insert(S.solve_cols.end(), {x, t});
+    }
+    S.solve_rows_ptr[l + 1] = int32_t(S.solve_rows.size() / 2);
+    S.solve_cols_ptr[l + 1] = int32_t(S.solve_cols.size() / 2);
+  }
+  // S slot -> front element offset of the block's (0, 0) entry
+  std::vector<int32_t> pos_in_front(S.nb, -1);  // position of block j within its own node
+  for (int32_t x = 0; x < S.nsn; ++x)
+    for (int32_t k = 0; k < S.sn_nc[x]; ++k) pos_in_front[S.sn_c0[x] + k] = k;
+  S.s_off.resize(S.s_col.size());
+  S.s_ld.resize(S.s_col.size());
   for (int32_t c = 0; c < S.nb; ++c)
     for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k) {
       const int32_t pi = S.perm_inv[c], pj = S.perm_inv[S.s_col[k]];
-      if (pi == pj) {
-        S.s_dest[k] = pi * 2;
-      } else if (pi > pj) {
-        S.s_dest[k] = (S.nb + lpos(pi, pj)) * 2;
-      } else {
-        S.s_dest[k] = (S.nb + lpos(pj, pi)) * 2 + 1;
-      }
+      const int32_t row = std::max(pi, pj), col = std::min(pi, pj);
+      const int32_t x = S.blk_sn[col];
+      const int32_t* rows = &S.sn_rows[S.sn_rows_ptr[x]];
+      const int32_t rp = int32_t(std::lower_bound(rows, rows + S.sn_nf[x], row) - rows);
+      const int64_t F = 6 * int64_t(S.sn_nf[x]);
+      S.s_off[k] = S.sn_off[x] + 6 * int64_t(pos_in_front[col]) * F + 6 * int64_t(rp);
+      S.s_ld[k] = int32_t(F) * 2 + (pi < pj ? 1 : 0);
     }
 }
 

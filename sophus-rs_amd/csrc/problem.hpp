@@ -61,34 +61,61 @@ struct Symbolic {
   std::vector<int32_t> cam_ptr, cam_obs;
   // RCS S pattern in camera order: row c, columns <= c, sorted
   std::vector<int32_t> s_ptr, s_col;
-  std::vector<int32_t> s_dest;  // per S slot: unified slot * 2 + transpose flag
-  // ordering (perm[new] = old)
+  // per S slot, the (obs_i, obs_j) pairs of one point with cam(obs_i) = row and
+  // cam(obs_j) = column, ordered by point; (o, o) pairs carry the camera's own terms
+  std::vector<int64_t> pair_ptr;
+  std::vector<int32_t> pairs;  // 2 per pair
+  // ordering (perm[new] = old) and column elimination tree
   std::vector<int32_t> perm, perm_inv;
-  // L structure in the permuted order
-  std::vector<int32_t> l_ptr, l_row;  // CSC, rows > column
-  std::vector<int32_t> r_ptr, r_col, r_lpos;  // reach of each column: (k, L index of L_jk)
-  // gather-form update lists over unified slots (0..nb-1 diagonal, nb + L index)
-  std::vector<int32_t> u_ptr, u_src, u_m;
-  std::vector<int32_t> lvl_ptr, lvl_col;
+  std::vector<int32_t> etree_parent;
+  int64_t nnz_l = 0;  // strictly-lower L blocks (column structure)
   int32_t max_slots_per_row = 0;
   double factor_flops = 0;
   int64_t num_schur_pairs = 0;
-  // pose-graph terms: per term the unified destinations of (m,m), (n,n), (m,n)
+  // destination of each S slot in the supernodal fronts: element offset of the block's
+  // (0,0) entry, and leading dimension * 2 + transpose flag
+  std::vector<int64_t> s_off;
+  std::vector<int32_t> s_ld;
+  // supernodes: contiguous column chains of the elimination tree
+  int32_t nsn = 0;
+  std::vector<int32_t> sn_c0, sn_nc, sn_nf, sn_parent, sn_level;
+  std::vector<int64_t> sn_off, sn_loff;       // front / L-panel element offsets
+  std::vector<int32_t> sn_rows_ptr, sn_rows;  // front row blocks (cols first, then ancestors)
+  std::vector<int32_t> sn_upos_ptr, sn_upos;  // update-row blocks -> position in parent front
+  std::vector<int32_t> blk_sn;                // permuted block -> supernode
+  int64_t front_elems = 0, l_elems = 0;
+  int32_t max_front = 0;                      // largest front (blocks)
+  // schedule: nodes per level; one panel launch per (level, panel)
+  std::vector<int32_t> lvl_ptr, lvl_sn;
+  struct PanelLaunch {
+    int32_t level, panel, node_begin, node_count, tile_begin, tile_count;
+  };
+  std::vector<PanelLaunch> launches;
+  std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
+  // solve work items per level: (node, 256-row tile below K), (node, 64-column tile of K)
+  std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;
+  // pose-graph terms: per term the S slots of (m,m), (n,n) and (max,min), -1 if absent;
+  // mn_flip = 1 when the stored (max,min) block is the transpose of J_m^T J_n
   struct PgDest {
-    int32_t mm, nn, mn;  // unified slot * 2 + transpose, -1 if absent
+    int32_t mm, nn, mn, mn_flip;
   };
   std::vector<std::vector<PgDest>> pg_dest;
 };
 
 Symbolic analyze(const HostProblem& P);
 
-// Ordering, elimination tree, L structure, update lists and levels of the RCS whose
-// lower pattern is S.nb, S.s_ptr, S.s_col (camera order); fills the remaining fields.
+// Ordering, elimination tree, supernodes, fronts and the launch schedule of the RCS
+// whose lower pattern is S.nb, S.s_ptr, S.s_col (camera order).
 void analyze_rcs(Symbolic& S);
 
 // Nested-dissection ordering of an undirected graph (adjacency lists, no self
 // loops).  Returns perm[new] = old.
 std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<int32_t>>& adj,
                                        int32_t leaf_size);
+
+// Panel width (in 6x6 pivot blocks) and trailing-update tile edge (scalars) of the
+// multifrontal factorization.
+constexpr int32_t kPanelBlocks = 4;
+constexpr int32_t kTile = 64;
 
 }  // namespace sgpu

@@ -147,54 +147,58 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     for (int32_t r = 0; r < nb; ++r) std::copy(rows[r].begin(), rows[r].end(), &S.s_col[S.s_ptr[r]]);
     analyze_rcs(S);
     require_device();
-    const int32_t nnz_l = S.l_ptr[nb];
-    std::vector<double> C(size_t(36) * (nb + nnz_l), 0.0);
+    // S blocks in slot order (row-major 6x6), scattered into the fronts on the device
+    std::vector<double> blocks(size_t(36) * S.s_col.size(), 0.0);
     for (int32_t c = 0; c < nb; ++c)
       for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
         const int32_t r = int32_t(row_idx[k]);
         auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
-        const int32_t slot = int32_t(std::lower_bound(b, e, c) - S.s_col.begin());
-        const int32_t code = S.s_dest[slot];
-        double* dst = &C[size_t(36) * (code >> 1)];
+        const int64_t slot = int64_t(std::lower_bound(b, e, c) - S.s_col.begin());
         for (int a = 0; a < 6; ++a)
-          for (int bb = 0; bb < 6; ++bb) {
-            const double v = values[36 * k + 6 * bb + a];  // input column-major
-            if (code & 1)
-              dst[6 * bb + a] += v;
-            else
-              dst[6 * a + bb] += v;
-          }
+          for (int bb = 0; bb < 6; ++bb)
+            blocks[36 * slot + 6 * a + bb] += values[36 * k + 6 * bb + a];  // input column-major
       }
     DeviceBufs bufs;
+    DevProblem dp{};
+    dp.nb = nb;
+    dp.s_off = bufs.up(S.s_off);
+    dp.s_ld = bufs.up(S.s_ld);
     DevFactor df{};
     df.nb = nb;
-    df.nlev = int32_t(S.lvl_ptr.size()) - 1;
-    df.nnz_l = nnz_l;
+    df.nsn = S.nsn;
     df.perm = bufs.up(S.perm);
-    df.l_ptr = bufs.up(S.l_ptr);
-    df.l_row = bufs.up(S.l_row);
-    df.r_ptr = bufs.up(S.r_ptr);
-    df.r_col = bufs.up(S.r_col);
-    df.r_lpos = bufs.up(S.r_lpos);
-    df.u_ptr = bufs.up(S.u_ptr);
-    df.u_src = bufs.up(S.u_src);
-    df.u_m = bufs.up(S.u_m);
-    df.lvl_ptr = bufs.up(S.lvl_ptr);
-    df.lvl_col = bufs.up(S.lvl_col);
-    df.h_lvl_ptr = S.lvl_ptr.data();
-    df.C = bufs.up(C);
-    df.M = bufs.alloc<double>(size_t(36) * std::max<size_t>(1, S.r_col.size()));
-    df.Ljj = bufs.alloc<double>(size_t(36) * nb);
-    df.Dfull = bufs.alloc<double>(size_t(36) * nb);
-    df.dvec = bufs.alloc<double>(size_t(6) * nb);
+    df.sn_c0 = bufs.up(S.sn_c0);
+    df.sn_nc = bufs.up(S.sn_nc);
+    df.sn_nf = bufs.up(S.sn_nf);
+    df.sn_off = bufs.up(S.sn_off);
+    df.sn_loff = bufs.up(S.sn_loff);
+    df.sn_rows_ptr = bufs.up(S.sn_rows_ptr);
+    df.sn_rows = bufs.up(S.sn_rows);
+    df.sn_parent = bufs.up(S.sn_parent);
+    df.sn_upos_ptr = bufs.up(S.sn_upos_ptr);
+    df.sn_upos = bufs.up(S.sn_upos);
+    df.lvl_sn = bufs.up(S.lvl_sn);
+    df.launch_tiles = bufs.up(S.launch_tiles);
+    df.solve_rows = bufs.up(S.solve_rows);
+    df.solve_cols = bufs.up(S.solve_cols);
+    df.ytmp = bufs.alloc<double>(size_t(6) * nb);
+    df.front = bufs.alloc<double>(size_t(S.front_elems));
+    df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
+    df.piv_L = bufs.alloc<double>(size_t(36) * nb);
+    df.piv_d = bufs.alloc<double>(size_t(6) * nb);
+    df.front_elems = S.front_elems;
+    df.host = &S;
+    double* d_blocks = bufs.up(blocks);
     std::vector<double> b(rhs, rhs + 6 * num_blocks);
     double* d_rhs = bufs.up(b);
     double* d_y = bufs.alloc<double>(size_t(6) * nb);
     double* d_x = bufs.alloc<double>(size_t(6) * nb);
     DevScalars* d_sc = bufs.alloc<DevScalars>(1);
     launch_zero_scalars(nullptr, d_sc);
-    for (int l = 0; l < df.nlev; ++l) launch_factor_level(nullptr, df, l, d_sc);
-    launch_triangular_solve(nullptr, df, d_rhs, d_y, nullptr, d_x);
+    launch_fill(nullptr, df.front, df.front_elems, 0.0);
+    launch_scatter_s(nullptr, dp, int64_t(S.s_col.size()), d_blocks, df);
+    launch_rcs_factor(nullptr, df, d_sc);
+    launch_rcs_solve(nullptr, df, d_rhs, d_y, d_x);
     SGPU_HIP(hipGetLastError());
     SGPU_HIP(hipDeviceSynchronize());
     DevScalars sc;

@@ -121,6 +121,9 @@ class _RcsStats(C.Structure):
         ("num_schur_pairs", C.c_int64),
         ("num_levels", C.c_int64),
         ("factor_flops", C.c_double),
+        ("num_supernodes", C.c_int64),
+        ("max_front_blocks", C.c_int64),
+        ("front_bytes", C.c_int64),
     ]
 
 
@@ -166,6 +169,8 @@ _SIGNATURES = {
     "sgpu_optimizer_eval_cost_at": (C.c_int, [C.c_void_p, C.POINTER(_DP), _DP]),
     "sgpu_optimizer_build_solution": (C.c_int, [C.c_void_p, C.POINTER(_Solution)]),
     "sgpu_optimizer_rcs_stats": (C.c_int, [C.c_void_p, C.POINTER(_RcsStats)]),
+    "sgpu_analyze": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                               C.POINTER(_RcsStats)]),
     "sgpu_optimizer_kernel_times": (C.c_int32, [C.c_void_p, C.POINTER(C.c_char_p), _I64P, _DP, C.c_int32]),
     "sgpu_optimizer_reset_kernel_times": (None, [C.c_void_p]),
     "sgpu_optimizer_set_kernel_timing": (None, [C.c_void_p, C.c_int32]),
@@ -591,6 +596,14 @@ class Optimizer:
         _check(library().sgpu_optimizer_build_solution(self._h, C.byref(s)))
         return OptimizationSolution(self.variables(), initial_cost, s.final_cost, s.num_steps,
                                     TerminationReason(s.termination), s.final_max_abs_neg_grad)
+
+
+def analyze(variables: VarFamilies, cost_fns: Sequence[CostFn]) -> dict:
+    """Host-only validation and symbolic analysis (no device needed)."""
+    m = _Marshal(variables, cost_fns)
+    s = _RcsStats()
+    _check(library().sgpu_analyze(m.fams, len(m.names), m.costs, len(cost_fns), C.byref(s)))
+    return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
 
 
 def optimize_nlls(variables: VarFamilies, cost_fns: Sequence[CostFn],
