@@ -171,28 +171,30 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// P^T N P for P = [hat(-x) | I] and a 3x3 N: [[H N H^T, H N], [N H^T, N]] with H = hat(x).
-// Adds sign * (P^T N P) to a 6x6 row-major accumulator via `add(idx, value)`.
+// P_i^T N P_j for P = [hat(-x) | I] and a 3x3 N:
+//   [[H_i N H_j^T, H_i N], [N H_j^T, N]]  with H = hat(x).
+// Adds sign * (P_i^T N P_j) to a 6x6 row-major accumulator via `add(idx, value)`.
 template <typename T, typename Add>
-__device__ __forceinline__ void add_lift(const T x[3], const T N[9], T sign, Add add) {
-  // HN = hat(x) N ; NHt = N hat(x)^T = -N hat(x)
+__device__ __forceinline__ void add_lift(const T xi[3], const T xj[3], const T N[9], T sign,
+                                         Add add) {
+  // HN = hat(x_i) N ; NHt = N hat(x_j)^T ; HNHt = hat(x_i) N hat(x_j)^T
   T HN[9], NHt[9], HNHt[9];
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     // hat(x) rows: [0,-x2,x1], [x2,0,-x0], [-x1,x0,0]
-    HN[j] = -x[2] * N[3 + j] + x[1] * N[6 + j];
-    HN[3 + j] = x[2] * N[j] - x[0] * N[6 + j];
-    HN[6 + j] = -x[1] * N[j] + x[0] * N[3 + j];
+    HN[j] = -xi[2] * N[3 + j] + xi[1] * N[6 + j];
+    HN[3 + j] = xi[2] * N[j] - xi[0] * N[6 + j];
+    HN[6 + j] = -xi[1] * N[j] + xi[0] * N[3 + j];
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    // (N hat(x)^T)[i][j] = sum_t N[i][t] hat(x)[j][t]
-    NHt[3 * i + 0] = -N[3 * i + 1] * x[2] + N[3 * i + 2] * x[1];
-    NHt[3 * i + 1] = N[3 * i + 0] * x[2] - N[3 * i + 2] * x[0];
-    NHt[3 * i + 2] = -N[3 * i + 0] * x[1] + N[3 * i + 1] * x[0];
-    HNHt[3 * i + 0] = -HN[3 * i + 1] * x[2] + HN[3 * i + 2] * x[1];
-    HNHt[3 * i + 1] = HN[3 * i + 0] * x[2] - HN[3 * i + 2] * x[0];
-    HNHt[3 * i + 2] = -HN[3 * i + 0] * x[1] + HN[3 * i + 1] * x[0];
+    // (N hat(x_j)^T)[i][j] = sum_t N[i][t] hat(x_j)[j][t]
+    NHt[3 * i + 0] = -N[3 * i + 1] * xj[2] + N[3 * i + 2] * xj[1];
+    NHt[3 * i + 1] = N[3 * i + 0] * xj[2] - N[3 * i + 2] * xj[0];
+    NHt[3 * i + 2] = -N[3 * i + 0] * xj[1] + N[3 * i + 1] * xj[0];
+    HNHt[3 * i + 0] = -HN[3 * i + 1] * xj[2] + HN[3 * i + 2] * xj[1];
+    HNHt[3 * i + 1] = HN[3 * i + 0] * xj[2] - HN[3 * i + 2] * xj[0];
+    HNHt[3 * i + 2] = -HN[3 * i + 0] * xj[1] + HN[3 * i + 1] * xj[0];
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -203,6 +205,24 @@ __device__ __forceinline__ void add_lift(const T x[3], const T N[9], T sign, Add
       add(6 * (i + 3) + j, sign * NHt[3 * i + j]);
       add(6 * (i + 3) + 3 + j, sign * N[3 * i + j]);
     }
+}
+
+// World position of a camera from its camera_from_world cache: c = -R_cw^T t_cw.
+__device__ __forceinline__ void cam_center(const double* __restrict__ cw, double c[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) c[k] = -(cw[k] * cw[9] + cw[3 + k] * cw[10] + cw[6 + k] * cw[11]);
+}
+
+// (G^T G)[a][b] for G = [[I, 0], [hat(c), I]]:  [[|c|^2 I - c c^T + I, hat(c)^T], [hat(c), I]].
+__device__ __forceinline__ double centered_gram(const double c[3], int a, int b) {
+  if (a < 3 && b < 3) return (a == b ? 1.0 + c[0] * c[0] + c[1] * c[1] + c[2] * c[2] : 0.0) - c[a] * c[b];
+  if (a >= 3 && b >= 3) return a == b ? 1.0 : 0.0;
+  // hat(c)[r][q]
+  auto hatc = [&](int r, int q) -> double {
+    const double h[9] = {0.0, -c[2], c[1], c[2], 0.0, -c[0], -c[1], c[0], 0.0};
+    return h[3 * r + q];
+  };
+  return a < 3 ? hatc(b - 3, a) : hatc(a - 3, b);
 }
 
 template <typename T>
@@ -223,13 +243,18 @@ __device__ __forceinline__ void mat3_mul_t(const T A[9], const T B[9], T C[9]) {
 // and the (o, o) pairs of a diagonal block also carry the camera's gradient
 // -P^T v_o and its reduced rhs P^T (M_o y_b - v_o).  Lanes stride over the block's
 // pairs, accumulate in FP64 registers and butterfly-reduce; no atomics, fixed order.
+// With `centered`, each camera's tangent is taken about its own centre c
+// (delta = G delta', G = [[I, 0], [hat(c), I]]): P becomes [hat(c - x) | I], whose entries
+// are scene-depth sized instead of world-coordinate sized, so FP32 products keep their
+// relative accuracy in S' = G^T S G and rhs' = G^T rhs.  The gradient stays in the
+// original coordinates.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_schur_slots(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
                   const double* __restrict__ cw, double nu, const double* __restrict__ hinv_g,
                   const double* __restrict__ hll_g, const double* __restrict__ y_g,
                   double* __restrict__ front, double* __restrict__ rhs_nat,
-                  double* __restrict__ grad_cam) {
+                  double* __restrict__ grad_cam, int centered) {
   const int lane = threadIdx.x & 63;
   const int64_t k = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (k >= p.nslot) return;  // whole wavefront leaves together
@@ -243,8 +268,16 @@ __global__ void __launch_bounds__(kThreads)
     const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
     const int b = p.obs_point[i];
     const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
-    const T x[3] = {T(xd[0]), T(xd[1]), T(xd[2])};
     const double* cwi = cw + 12 * p.obs_pose[i];
+    double xl[3] = {xd[0], xd[1], xd[2]};  // lift point of obs i (relative to c_i if centered)
+    if (centered) {
+      double c[3];
+      cam_center(cwi, c);
+#pragma unroll
+      for (int e = 0; e < 3; ++e) xl[e] -= c[e];
+    }
+    const T x[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
+    T xj[3] = {x[0], x[1], x[2]};
     const double* ini = intr + 4 * p.obs_intr[i];
     double xci[3];
     to_camera(cwi, xd, xci);
@@ -285,9 +318,9 @@ __global__ void __launch_bounds__(kThreads)
         for (int e = 0; e < 9; ++e) N[e] = Mi[e];
       }
       // P^T t = [x cross t; t]
-      racc[0] += xd[1] * t[2] - xd[2] * t[1];
-      racc[1] += xd[2] * t[0] - xd[0] * t[2];
-      racc[2] += xd[0] * t[1] - xd[1] * t[0];
+      racc[0] += xl[1] * t[2] - xl[2] * t[1];
+      racc[1] += xl[2] * t[0] - xl[0] * t[2];
+      racc[2] += xl[0] * t[1] - xl[1] * t[0];
       racc[3] += t[0];
       racc[4] += t[1];
       racc[5] += t[2];
@@ -301,6 +334,12 @@ __global__ void __launch_bounds__(kThreads)
       const double* cwj = cw + 12 * p.obs_pose[j];
       double xcj[3];
       to_camera(cwj, xd, xcj);
+      if (centered) {
+        double c[3];
+        cam_center(cwj, c);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) xj[e] = T(xd[e] - c[e]);
+      }
       T Aj[6], Mj[9], Hi[9], Z[9];
       obs_jacobian<T>(cwj, intr + 4 * p.obs_intr[j], xcj, Aj);
       gram(Aj, Mj);
@@ -310,7 +349,7 @@ __global__ void __launch_bounds__(kThreads)
       mat3_mul_t(Z, Mj, N);
       sign = T(-1);
     }
-    add_lift(x, N, sign, [&](int e, T val) { acc[e] += double(val); });
+    add_lift(x, xj, N, sign, [&](int e, T val) { acc[e] += double(val); });
   }
   // butterfly reduction: every lane ends with all sums
 #pragma unroll
@@ -338,7 +377,13 @@ __global__ void __launch_bounds__(kThreads)
   }
   if (lane < 36) {
     const int a = lane / 6, bb = lane % 6;
-    if (diag && a == bb) mine += nu * double(p.cam_nu[c1]);
+    if (diag && centered) {
+      double c[3];
+      cam_center(cw + 12 * p.cam_member[c1], c);
+      mine += nu * double(p.cam_nu[c1]) * centered_gram(c, a, bb);
+    } else if (diag && a == bb) {
+      mine += nu * double(p.cam_nu[c1]);
+    }
     const int ldc = p.s_ld[k], ld = ldc >> 1;
     front[p.s_off[k] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = mine;
   } else if (diag && lane < 42) {
@@ -864,7 +909,6 @@ __global__ void __launch_bounds__(kThreads)
       for (int k = 0; k < 3; ++k) points_new[3 * b + k] = xd[k];
       continue;
     }
-    const T x[3] = {T(xd[0]), T(xd[1]), T(xd[2])};
     T acc[3] = {T(pt_s[3 * b]), T(pt_s[3 * b + 1]), T(pt_s[3 * b + 2])};
     for (int o = p.pt_ptr[b]; o < p.pt_ptr[b + 1]; ++o) {
       const int pm = p.obs_pose[o];
@@ -877,11 +921,11 @@ __global__ void __launch_bounds__(kThreads)
       obs_jacobian<T>(cwo, intr + 4 * p.obs_intr[o], xc, A);
       gram(A, M);
       const double* dx = dx_cam + 6 * c;
-      const T w0 = T(dx[0]), w1 = T(dx[1]), w2 = T(dx[2]);
-      // P dx = omega x point + nu
-      const T q0 = w1 * x[2] - w2 * x[1] + T(dx[3]);
-      const T q1 = w2 * x[0] - w0 * x[2] + T(dx[4]);
-      const T q2 = w0 * x[1] - w1 * x[0] + T(dx[5]);
+      // P dx = omega x point + nu, in FP64 (omega x point and nu nearly cancel far from
+      // the world origin)
+      const T q0 = T(dx[1] * xd[2] - dx[2] * xd[1] + dx[3]);
+      const T q1 = T(dx[2] * xd[0] - dx[0] * xd[2] + dx[4]);
+      const T q2 = T(dx[0] * xd[1] - dx[1] * xd[0] + dx[5]);
 #pragma unroll
       for (int k = 0; k < 3; ++k) acc[k] += M[3 * k] * q0 + M[3 * k + 1] * q1 + M[3 * k + 2] * q2;
     }
@@ -1037,9 +1081,11 @@ __device__ void pose_graph_eval(const double* m, const double* n, const double* 
     }
 }
 
+// With `centered` (cw != nullptr), the blocks and rhs are those of the centred camera
+// tangents: G_m^T H_mn G_n and G_m^T b_m (see k_schur_slots).
 __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const double* __restrict__ poses,
                                        double nu, double* __restrict__ C, double* __restrict__ rhs_nat,
-                                       double* __restrict__ grad_cam) {
+                                       double* __restrict__ grad_cam, const double* __restrict__ cw) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.n) return;
   double r[6], D[36];
@@ -1065,31 +1111,84 @@ __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const doubl
   const double damp = g.first[t] ? nu : 0.0;
   // H_mm = H_nn = D^T D (+ nu I), H_mn = J_m^T J_n = -D^T D (symmetric, so the stored
   // orientation of the (max, min) block does not matter)
-  auto add = [&](int slot, double sign, double diag) {
+  // camera centres (zero when not centred: G = I)
+  double cmc[3] = {0, 0, 0}, cnc[3] = {0, 0, 0};
+  if (cw) {
+    cam_center(cw + 12 * g.m[t], cmc);
+    cam_center(cw + 12 * g.nn[t], cnc);
+  }
+  // right-multiply a 6x6 row-major block by G(c) = [[I, 0], [hat(c), I]]:
+  // (X G)[:, 0:3] = X[:, 0:3] + X[:, 3:6] hat(c), (X G)[:, 3:6] = X[:, 3:6]
+  auto right_g = [](const double* X, const double c[3], double* Y) {
+    for (int r = 0; r < 6; ++r) {
+      const double* x = X + 6 * r;
+      Y[6 * r + 0] = x[0] + (x[4] * c[2] - x[5] * c[1]);
+      Y[6 * r + 1] = x[1] + (-x[3] * c[2] + x[5] * c[0]);
+      Y[6 * r + 2] = x[2] + (x[3] * c[1] - x[4] * c[0]);
+      for (int k = 3; k < 6; ++k) Y[6 * r + k] = x[k];
+    }
+  };
+  // G(ca)^T X G(cb) (X row-major 6x6)
+  auto sandwich = [&](const double* X, const double ca[3], const double cb[3], double* Y) {
+    double XG[36], Xt[36], T1[36];
+    right_g(X, cb, XG);
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) Xt[6 * j + i] = XG[6 * i + j];
+    right_g(Xt, ca, T1);  // (X G_b)^T G_a = (G_a^T X G_b)^T
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) Y[6 * i + j] = T1[6 * j + i];
+  };
+  // stores element (a, b) of the row-m / column-n block; `flip` when the slot holds its
+  // transpose
+  auto add = [&](int slot, const double* X, int flip) {
     if (slot < 0) return;
     double* B = C + p.s_off[slot];
     const int ldc = p.s_ld[slot], ld = ldc >> 1;
     for (int e = 0; e < 36; ++e) {
       const int a = e / 6, b = e % 6;
-      double v = sign * DtD[e];
-      if (a == b) v += diag;
+      const double v = flip ? X[6 * b + a] : X[e];
       if (v != 0.0) atomic_add_d(B + ((ldc & 1) ? a * ld + b : b * ld + a), v);
     }
   };
-  add(dst[0], 1.0, damp);
-  add(dst[1], 1.0, damp);
-  add(dst[2], -1.0, 0.0);
-  // neg_grad_m -= J_m^T r~ = +D^T r~ ; neg_grad_n -= D^T r~
-  if (cm >= 0)
+  // H_mm = H_nn = D^T D (+ nu I), H_mn = J_m^T J_n = -D^T D
+  double Hmm[36], Hmn[36], X[36];
+  for (int e = 0; e < 36; ++e) {
+    Hmm[e] = DtD[e] + ((e / 6 == e % 6) ? damp : 0.0);
+    Hmn[e] = -DtD[e];
+  }
+  sandwich(Hmm, cmc, cmc, X);
+  add(dst[0], X, 0);
+  sandwich(Hmm, cnc, cnc, X);
+  add(dst[1], X, 0);
+  sandwich(Hmn, cmc, cnc, X);
+  add(dst[2], X, dst[3]);
+  // neg_grad_m -= J_m^T r~ = +D^T r~ ; neg_grad_n -= D^T r~ ; rhs' = G^T rhs:
+  // [b_w + b_v x c; b_v]
+  auto add_rhs = [&](int cb, double sign, const double c[3]) {
+    const double* v = Dtr;
+    const double r[6] = {v[0] + (v[4] * c[2] - v[5] * c[1]), v[1] + (v[5] * c[0] - v[3] * c[2]),
+                         v[2] + (v[3] * c[1] - v[4] * c[0]), v[3], v[4], v[5]};
     for (int k = 0; k < 6; ++k) {
-      atomic_add_d(rhs_nat + 6 * cm + k, Dtr[k]);
-      atomic_add_d(grad_cam + 6 * cm + k, Dtr[k]);
+      atomic_add_d(rhs_nat + 6 * cb + k, sign * r[k]);
+      atomic_add_d(grad_cam + 6 * cb + k, sign * v[k]);
     }
-  if (cn >= 0)
-    for (int k = 0; k < 6; ++k) {
-      atomic_add_d(rhs_nat + 6 * cn + k, -Dtr[k]);
-      atomic_add_d(grad_cam + 6 * cn + k, -Dtr[k]);
-    }
+  };
+  if (cm >= 0) add_rhs(cm, 1.0, cmc);
+  if (cn >= 0) add_rhs(cn, -1.0, cnc);
+}
+
+// delta = G delta': nu += c x omega for every free camera (centred solves).
+__global__ void k_uncenter(int nb, const int32_t* __restrict__ cam_member,
+                           const double* __restrict__ cw, double* __restrict__ dx) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nb) return;
+  double ctr[3];
+  cam_center(cw + 12 * cam_member[c], ctr);
+  double* d = dx + 6 * c;
+  const double w0 = d[0], w1 = d[1], w2 = d[2];
+  d[3] += ctr[1] * w2 - ctr[2] * w1;
+  d[4] += ctr[2] * w0 - ctr[0] * w2;
+  d[5] += ctr[0] * w1 - ctr[1] * w0;
 }
 
 __global__ void k_pose_graph_cost(DevPoseGraph g, const double* __restrict__ poses,
@@ -1212,10 +1311,10 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
   const dim3 g(unsigned((p.nslot + wpb - 1) / wpb));
   if (precision == 0)
     hipLaunchKernelGGL(k_schur_slots<double>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
-                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam);
+                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 0);
   else
     hipLaunchKernelGGL(k_schur_slots<float>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
-                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam);
+                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 1);
 }
 
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v) {
@@ -1327,10 +1426,17 @@ void launch_max_grad(hipStream_t s, const double* grad_cam, int nb, DevScalars* 
 }
 
 void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const DevProblem& p,
-                                 const double* poses, double nu, DevLin& w, DevFactor& f) {
+                                 const double* poses, double nu, DevLin& w, DevFactor& f,
+                                 const double* centered_cw) {
   if (g.n == 0) return;
   hipLaunchKernelGGL(k_pose_graph_linearize, dim3(blocks_for(g.n, 128)), dim3(128), 0, s, g, p,
-                     poses, nu, f.front, w.rhs_nat, w.grad_cam);
+                     poses, nu, f.front, w.rhs_nat, w.grad_cam, centered_cw);
+}
+
+void launch_uncenter(hipStream_t s, const DevProblem& p, const double* cw, double* dx_cam) {
+  if (p.nb == 0) return;
+  hipLaunchKernelGGL(k_uncenter, dim3(blocks_for(p.nb, kThreads)), dim3(kThreads), 0, s, p.nb,
+                     p.cam_member, cw, dx_cam);
 }
 
 void launch_pose_graph_cost(hipStream_t s, const DevPoseGraph& g, const double* poses,

@@ -129,7 +129,10 @@ struct DevPoseGraph {
   double delta;
 };
 void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const DevProblem& p,
-                                 const double* poses, double nu, DevLin& w, DevFactor& f);
+                                 const double* poses, double nu, DevLin& w, DevFactor& f,
+                                 const double* centered_cw);
+// Converts centred camera steps back: delta = G delta' (nu += c x omega).
+void launch_uncenter(hipStream_t s, const DevProblem& p, const double* cw, double* dx_cam);
 // Scatter S blocks given per slot (36 doubles, row-major, camera order) into the fronts.
 void launch_scatter_s(hipStream_t s, const DevProblem& p, int64_t nslots, const double* blocks,
                       DevFactor& f);

@@ -149,7 +149,8 @@ struct sgpu_optimizer {
     tic(K_CAMERA_ROWS);
     launch_camera_rows(stream, params.precision, dp, pts, intr, dl.cw, nu_lin, dl, df, d_sc);
     toc(K_CAMERA_ROWS);
-    for (auto& g : dpg) launch_pose_graph_linearize(stream, g, dp, ps, nu_lin, dl, df);
+    const double* centered = params.precision == SGPU_PRECISION_MIXED ? dl.cw : nullptr;
+    for (auto& g : dpg) launch_pose_graph_linearize(stream, g, dp, ps, nu_lin, dl, df, centered);
     launch_max_grad(stream, dl.grad_cam, S.nb, d_sc);
     used[K_LINEARIZE_POINTS] = used[K_CAMERA_ROWS] = true;
   }
@@ -314,6 +315,8 @@ struct sgpu_optimizer {
     SGPU_HIP(hipEventRecord(phase_ev[2], stream));
     tic(K_TRIANGULAR);
     launch_rcs_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.dx_cam);
+    // the mixed path solves for centred camera tangents (see k_schur_slots)
+    if (params.precision == SGPU_PRECISION_MIXED) launch_uncenter(stream, dp, dl.cw, dl.dx_cam);
     toc(K_TRIANGULAR);
     tic(K_BACKSUB);
     launch_backsub_points(stream, params.precision, dp, points[cur], intr, dl.cw, dl, points[nxt], d_sc);

@@ -212,3 +212,23 @@ def test_pose_graph_plus_ba_matches_oracle(sg, orc):
     sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), ocosts,
                                            orc.default_params(num_iterations=12, initial_lm_damping=1.0))
     assert abs(merit - sol["final_cost"]) <= 1e-9 * sol["final_cost"]
+
+
+def test_pose_graph_plus_ba_mixed_precision(sg, orc):
+    """The mixed path's centred camera tangents also carry the pose-graph blocks."""
+    spec = sg.BaSpec(layout=1, num_cameras=30, num_points=800, geometric_p=0.2, seed=11)
+    p = sg.generate_ba(spec)
+    pairs = np.array([[i, i + 1] for i in range(29)], dtype=np.int64)
+    meas = np.stack([orc.se3_mul(orc.se3_inverse(p.poses_gt[i]), p.poses_gt[j]) for i, j in pairs])
+    costs = p.cost_fns(1.0) + [sg.CostFn.pose_graph3(["poses", "poses"], pairs, meas,
+                                                     sg.RobustKernel.huber(0.5))]
+    params = sg.OptParams(num_iterations=12, initial_lm_damping=1.0, precision=sg.Precision.Mixed)
+    opt = sg.Optimizer(p.variables(), costs, params)
+    while opt.step().termination is None:
+        pass
+    merit = opt.merit()
+    opt.close()
+    ocosts = orc.ba_costs(p) + [(1, ["poses", "poses"], pairs, meas, 0.5)]
+    sol, _, _ = orc.optimize_nlls(orc.ba_families(p), ocosts,
+                                  orc.default_params(num_iterations=12, initial_lm_damping=1.0))
+    assert abs(merit - sol["final_cost"]) <= 1e-6 * sol["final_cost"]
