@@ -315,6 +315,15 @@ sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
                                   const int64_t* row_idx, const double* values,
                                   const double* rhs, double* x);
 
+/* Kernel (d'), no reference counterpart: block-Jacobi preconditioned conjugate
+ * gradients on the same input, FP64, x0 = 0, stopping at |r| <= relative_tolerance |b|
+ * or max_iterations.  *iterations receives the iterations taken (may be NULL).
+ * Returns SGPU_ERR_NON_FINITE on breakdown (p.Sp <= 0, e.g. an indefinite matrix). */
+sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr,
+                                 const int64_t* row_idx, const double* values,
+                                 const double* rhs, double* x, double relative_tolerance,
+                                 int64_t max_iterations, int64_t* iterations);
+
 /* ---- synthetic problems ---------------------------------------------------------
  * Seeded BA generator (SplitMix64 -> xoshiro256**, Box-Muller), the input of the
  * parity tests and the benchmark.  Two-call protocol: call with NULL arrays to get

@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libsophus_gpu.so"
 
-HIP_SOURCES = ["kernels.hip"]
+HIP_SOURCES = ["kernels.hip", "pcg.hip"]
 HOST_SOURCES = ["problem.cpp", "optimizer.cpp", "stage_api.cpp", "generator.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
@@ -55,7 +55,7 @@ def build_library(verbose: bool = False) -> Path:
         objs.append(obj)
     for src in HOST_SOURCES:
         obj = objdir / (src + ".o")
-        jobs.append([hipcc, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", str(CSRC / src), "-o", str(obj)])
+        jobs.append([hipcc, *COMMON, "--offload-arch=gfx950", "-D__HIP_PLATFORM_AMD__", "-c", str(CSRC / src), "-o", str(obj)])
         objs.append(obj)
     procs = [(cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True))
              for cmd in jobs]

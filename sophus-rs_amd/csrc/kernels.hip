@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(kThreads)
                   const double* __restrict__ cw, double nu, const double* __restrict__ hinv_g,
                   const double* __restrict__ hll_g, const double* __restrict__ y_g,
                   double* __restrict__ front, double* __restrict__ rhs_nat,
-                  double* __restrict__ grad_cam, int centered) {
+                  double* __restrict__ grad_cam, int centered, double* __restrict__ bsr) {
   const int lane = threadIdx.x & 63;
   const int64_t k = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (k >= p.nslot) return;  // whole wavefront leaves together
@@ -384,8 +384,12 @@ __global__ void __launch_bounds__(kThreads)
     } else if (diag && a == bb) {
       mine += nu * double(p.cam_nu[c1]);
     }
-    const int ldc = p.s_ld[k], ld = ldc >> 1;
-    front[p.s_off[k] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = mine;
+    if (bsr) {  // row-major lower block in slot order (iterative solvers)
+      bsr[36 * k + lane] = mine;
+    } else {
+      const int ldc = p.s_ld[k], ld = ldc >> 1;
+      front[p.s_off[k] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = mine;
+    }
   } else if (diag && lane < 42) {
     rhs_nat[6 * c1 + lane - 36] = mine;
   } else if (diag && lane < 48) {
@@ -1085,7 +1089,8 @@ __device__ void pose_graph_eval(const double* m, const double* n, const double* 
 // tangents: G_m^T H_mn G_n and G_m^T b_m (see k_schur_slots).
 __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const double* __restrict__ poses,
                                        double nu, double* __restrict__ C, double* __restrict__ rhs_nat,
-                                       double* __restrict__ grad_cam, const double* __restrict__ cw) {
+                                       double* __restrict__ grad_cam, const double* __restrict__ cw,
+                                       double* __restrict__ bsr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= g.n) return;
   double r[6], D[36];
@@ -1142,6 +1147,14 @@ __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const doubl
   // transpose
   auto add = [&](int slot, const double* X, int flip) {
     if (slot < 0) return;
+    if (bsr) {
+      for (int e = 0; e < 36; ++e) {
+        const int a = e / 6, b = e % 6;
+        const double v = flip ? X[6 * b + a] : X[e];
+        if (v != 0.0) atomic_add_d(bsr + 36 * int64_t(slot) + e, v);
+      }
+      return;
+    }
     double* B = C + p.s_off[slot];
     const int ldc = p.s_ld[slot], ld = ldc >> 1;
     for (int e = 0; e < 36; ++e) {
@@ -1311,10 +1324,10 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
   const dim3 g(unsigned((p.nslot + wpb - 1) / wpb));
   if (precision == 0)
     hipLaunchKernelGGL(k_schur_slots<double>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
-                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 0);
+                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 0, w.bsr);
   else
     hipLaunchKernelGGL(k_schur_slots<float>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
-                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 1);
+                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 1, w.bsr);
 }
 
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v) {
@@ -1430,7 +1443,7 @@ void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const Dev
                                  const double* centered_cw) {
   if (g.n == 0) return;
   hipLaunchKernelGGL(k_pose_graph_linearize, dim3(blocks_for(g.n, 128)), dim3(128), 0, s, g, p,
-                     poses, nu, f.front, w.rhs_nat, w.grad_cam, centered_cw);
+                     poses, nu, f.front, w.rhs_nat, w.grad_cam, centered_cw, w.bsr);
 }
 
 void launch_uncenter(hipStream_t s, const DevProblem& p, const double* cw, double* dx_cam) {

@@ -87,6 +87,7 @@ struct DevLin {
   double* x_perm;     // nb x 6 solve buffer in permuted order
   double* x_aux;      // nb x 6 second solve buffer
   double* dx_cam;     // nb x 6 camera step (camera order)
+  double* bsr;        // nslot x 36 lower S blocks for the iterative solvers (else null)
   double* partial;    // reduction partials
   int32_t partial_cap;
 };

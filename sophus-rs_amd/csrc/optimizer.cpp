@@ -17,6 +17,7 @@
 
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "pcg.hpp"
 #include "problem.hpp"
 
 namespace sgpu {
@@ -97,6 +98,8 @@ struct sgpu_optimizer {
   double* cost_partial = nullptr;
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned
+  DevPcg pcg{};                // iterative RCS solver state (PCG solvers only)
+  PcgState* h_pcg = nullptr;   // pinned
   double merit = 0, nu = 0;
   int64_t rejects = 0, steps = 0;
   bool terminated = false;
@@ -115,6 +118,7 @@ struct sgpu_optimizer {
     for (auto& e : phase_ev)
       if (e) (void)hipEventDestroy(e);
     if (h_sc) (void)hipHostFree(h_sc);
+    if (h_pcg) (void)hipHostFree(h_pcg);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -136,13 +140,16 @@ struct sgpu_optimizer {
     }
   }
 
+  bool iterative() const { return params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT; }
+
   const Family& pose_fam() const { return P.fams[P.f_pose]; }
   const Family& point_fam() const { return P.fams[P.f_point]; }
 
   void linearize(const double* ps, const double* pts, double nu_lin, bool* used) {
     launch_zero_scalars(stream, d_sc);
     launch_cam_cache(stream, S.npose, ps, dl.cw);
-    launch_fill(stream, df.front, df.front_elems, 0.0);
+    if (!iterative()) launch_fill(stream, df.front, df.front_elems, 0.0);
+    else if (!dpg.empty()) launch_fill(stream, dl.bsr, int64_t(36) * dp.nslot, 0.0);
     tic(K_LINEARIZE_POINTS);
     launch_point_prep(stream, params.precision, dp, ps, pts, intr, dl.cw, nu_lin, dl, d_sc);
     toc(K_LINEARIZE_POINTS);
@@ -181,8 +188,12 @@ struct sgpu_optimizer {
   void init(const sgpu_var_family* families, int64_t nf, const sgpu_cost_fn* cfs, int64_t nc,
             const sgpu_opt_params* prm) {
     params = *prm;
-    if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT)
-      throw Error(SGPU_ERR_UNSUPPORTED, "only the Schur block-sparse LDLt solver is available");
+    if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT &&
+        params.solver != SGPU_SOLVER_SCHUR_PCG_JACOBI)
+      throw Error(SGPU_ERR_UNSUPPORTED, "solver not available");
+    if (params.solver == SGPU_SOLVER_SCHUR_PCG_JACOBI &&
+        !(params.pcg_relative_tolerance > 0.0 && params.pcg_max_iterations > 0))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "PCG needs a positive tolerance and iteration cap");
     if (params.precision != SGPU_PRECISION_F64 && params.precision != SGPU_PRECISION_MIXED)
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown precision");
     P = build_host_problem(families, nf, cfs, nc);
@@ -261,8 +272,10 @@ struct sgpu_optimizer {
     df.solve_rows = arena.upload(S.solve_rows, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
     df.ytmp = arena.alloc<double>(size_t(6) * S.nb);
-    df.front = arena.alloc<double>(size_t(S.front_elems));
-    df.lbuf = arena.alloc<double>(size_t(S.l_elems));
+    if (!iterative()) {
+      df.front = arena.alloc<double>(size_t(S.front_elems));
+      df.lbuf = arena.alloc<double>(size_t(S.l_elems));
+    }
     df.piv_L = arena.alloc<double>(size_t(36) * S.nb);
     df.piv_d = arena.alloc<double>(size_t(6) * S.nb);
     df.front_elems = S.front_elems;
@@ -278,6 +291,28 @@ struct sgpu_optimizer {
     dl.x_perm = arena.alloc<double>(size_t(6) * S.nb);
     dl.x_aux = arena.alloc<double>(size_t(6) * S.nb);
     dl.dx_cam = arena.alloc<double>(size_t(6) * S.nb);
+    dl.bsr = nullptr;
+    if (iterative()) {
+      std::vector<int32_t> mv_ptr, mv_ent, diag_slot;
+      build_pcg_pattern(S.nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot);
+      dl.bsr = arena.alloc<double>(size_t(36) * S.s_col.size());
+      pcg.nb = S.nb;
+      pcg.mv_ptr = arena.upload(mv_ptr, stream);
+      pcg.mv_ent = arena.upload(mv_ent, stream);
+      pcg.diag_slot = arena.upload(diag_slot, stream);
+      pcg.bsr = dl.bsr;
+      pcg.minv = arena.alloc<double>(size_t(36) * S.nb);
+      pcg.x = dl.dx_cam;
+      pcg.r = arena.alloc<double>(size_t(6) * S.nb);
+      pcg.z = arena.alloc<double>(size_t(6) * S.nb);
+      pcg.p = arena.alloc<double>(size_t(6) * S.nb);
+      pcg.q = arena.alloc<double>(size_t(6) * S.nb);
+      pcg.part = arena.alloc<double>(size_t(pcg_partial_len(S.nb)));
+      pcg.state = arena.alloc<PcgState>(1);
+      pcg.n_vec_blocks = pcg_vec_blocks(S.nb);
+      pcg.n_mv_blocks = pcg_mv_blocks(S.nb);
+      SGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pcg), sizeof(PcgState), hipHostMallocDefault));
+    }
     dl.partial_cap = 4 * kCostBlocks;
     dl.partial = arena.alloc<double>(dl.partial_cap);
     cost_partial = arena.alloc<double>(size_t(2) * kCostBlocks * (1 + P.pg.size()));
@@ -309,12 +344,25 @@ struct sgpu_optimizer {
     SGPU_HIP(hipEventRecord(phase_ev[0], stream));
     linearize(poses[cur], points[cur], nu, used);
     SGPU_HIP(hipEventRecord(phase_ev[1], stream));
-    tic(K_FACTOR);
-    launch_rcs_factor(stream, df, d_sc);
-    toc(K_FACTOR);
-    SGPU_HIP(hipEventRecord(phase_ev[2], stream));
-    tic(K_TRIANGULAR);
-    launch_rcs_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.dx_cam);
+    int32_t pcg_iters = 0, pcg_bad = 0;
+    if (iterative()) {
+      // block-Jacobi PCG on the resident S (the host polls the device state in chunks)
+      tic(K_FACTOR);
+      launch_pcg_block_jacobi(stream, pcg);
+      pcg_iters = run_pcg(stream, pcg, dl.rhs_nat, params.pcg_relative_tolerance,
+                          int32_t(std::min<int64_t>(params.pcg_max_iterations, INT32_MAX)), h_pcg,
+                          &pcg_bad);
+      toc(K_FACTOR);
+      SGPU_HIP(hipEventRecord(phase_ev[2], stream));
+      tic(K_TRIANGULAR);
+    } else {
+      tic(K_FACTOR);
+      launch_rcs_factor(stream, df, d_sc);
+      toc(K_FACTOR);
+      SGPU_HIP(hipEventRecord(phase_ev[2], stream));
+      tic(K_TRIANGULAR);
+      launch_rcs_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.dx_cam);
+    }
     // the mixed path solves for centred camera tangents (see k_schur_slots)
     if (params.precision == SGPU_PRECISION_MIXED) launch_uncenter(stream, dp, dl.cw, dl.dx_cam);
     toc(K_TRIANGULAR);
@@ -343,7 +391,7 @@ struct sgpu_optimizer {
     const double max_step = bits_to_double(sc.max_delta_bits);
     if (max_grad < params.gradient_tolerance) {
       reason = SGPU_TERM_GRADIENT_TOLERANCE;
-    } else if (sc.singular_marg || sc.bad_pivot) {
+    } else if (sc.singular_marg || sc.bad_pivot || pcg_bad) {
       nu *= 2.0;  // handle_solve_failure -> reject_step
       ++rejects;
       if (rejects >= params.max_consecutive_rejects) reason = SGPU_TERM_MAX_CONSECUTIVE_REJECTS;
@@ -398,7 +446,7 @@ struct sgpu_optimizer {
       info->gain_ratio = rho;
       info->max_abs_neg_grad = max_grad;
       info->max_abs_delta = max_step;
-      info->pcg_iterations = 0;
+      info->pcg_iterations = pcg_iters;
       info->ms_linearize = a;
       info->ms_factor = b;
       info->ms_solve = c;

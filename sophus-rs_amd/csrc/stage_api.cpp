@@ -7,6 +7,7 @@
 
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "pcg.hpp"
 #include "problem.hpp"
 
 using namespace sgpu;
@@ -58,6 +59,52 @@ void require_device() {
 }
 
 }  // namespace
+
+namespace sgpu {
+// Parses lower blocks in CSC-by-block-column (see sgpu_block_ldlt_solve) into the slot
+// pattern (rows of lower blocks) and row-major blocks in slot order.
+static void parse_lower_blocks(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,
+                               const double* values, const double* rhs, const double* x,
+                               Symbolic& S, std::vector<double>& blocks) {
+  if (num_blocks <= 0 || !col_ptr || !row_idx || !values || !rhs || !x)
+    throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+  if (num_blocks > INT32_MAX / 64) throw Error(SGPU_ERR_UNSUPPORTED, "too many blocks");
+  const int32_t nb = int32_t(num_blocks);
+  const int64_t nnz = col_ptr[nb];
+  if (col_ptr[0] != 0 || nnz < 0) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad col_ptr");
+  std::vector<std::vector<int32_t>> rows(nb);
+  for (int32_t c = 0; c < nb; ++c) {
+    if (col_ptr[c + 1] < col_ptr[c]) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad col_ptr");
+    rows[c].push_back(c);
+    for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
+      const int64_t r = row_idx[k];
+      if (r < c || r >= nb)
+        throw Error(SGPU_ERR_INDEX_OUT_OF_RANGE, "block row must satisfy col <= row < nb");
+      rows[r].push_back(c);
+    }
+  }
+  S = Symbolic();
+  S.nb = nb;
+  S.s_ptr.assign(nb + 1, 0);
+  for (int32_t r = 0; r < nb; ++r) {
+    std::sort(rows[r].begin(), rows[r].end());
+    rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
+    S.s_ptr[r + 1] = S.s_ptr[r] + int32_t(rows[r].size());
+  }
+  S.s_col.resize(S.s_ptr[nb]);
+  for (int32_t r = 0; r < nb; ++r) std::copy(rows[r].begin(), rows[r].end(), &S.s_col[S.s_ptr[r]]);
+  blocks.assign(size_t(36) * S.s_col.size(), 0.0);
+  for (int32_t c = 0; c < nb; ++c)
+    for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
+      const int32_t r = int32_t(row_idx[k]);
+      auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
+      const int64_t slot = int64_t(std::lower_bound(b, e, c) - S.s_col.begin());
+      for (int a = 0; a < 6; ++a)
+        for (int bb = 0; bb < 6; ++bb)
+          blocks[36 * slot + 6 * a + bb] += values[36 * k + 6 * bb + a];  // input column-major
+    }
+}
+}  // namespace sgpu
 
 extern "C" sgpu_status sgpu_eval_pinhole_terms(int32_t precision, int64_t n, const double* intr,
                                                const double* pose, const double* pt,
@@ -118,46 +165,13 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
                                              const int64_t* row_idx, const double* values,
                                              const double* rhs, double* x) {
   return guarded([&] {
-    if (num_blocks <= 0 || !col_ptr || !row_idx || !values || !rhs || !x)
-      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
-    if (num_blocks > INT32_MAX / 64) throw Error(SGPU_ERR_UNSUPPORTED, "too many blocks");
-    const int32_t nb = int32_t(num_blocks);
-    const int64_t nnz = col_ptr[nb];
-    if (col_ptr[0] != 0 || nnz < 0) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad col_ptr");
-    std::vector<std::vector<int32_t>> rows(nb);
-    for (int32_t c = 0; c < nb; ++c) {
-      if (col_ptr[c + 1] < col_ptr[c]) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad col_ptr");
-      rows[c].push_back(c);
-      for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
-        const int64_t r = row_idx[k];
-        if (r < c || r >= nb)
-          throw Error(SGPU_ERR_INDEX_OUT_OF_RANGE, "block row must satisfy col <= row < nb");
-        rows[r].push_back(c);
-      }
-    }
     Symbolic S;
-    S.nb = nb;
-    S.s_ptr.assign(nb + 1, 0);
-    for (int32_t r = 0; r < nb; ++r) {
-      std::sort(rows[r].begin(), rows[r].end());
-      rows[r].erase(std::unique(rows[r].begin(), rows[r].end()), rows[r].end());
-      S.s_ptr[r + 1] = S.s_ptr[r] + int32_t(rows[r].size());
-    }
-    S.s_col.resize(S.s_ptr[nb]);
-    for (int32_t r = 0; r < nb; ++r) std::copy(rows[r].begin(), rows[r].end(), &S.s_col[S.s_ptr[r]]);
+    std::vector<double> blocks;
+    parse_lower_blocks(num_blocks, col_ptr, row_idx, values, rhs, x, S, blocks);
+    const int32_t nb = S.nb;
     analyze_rcs(S);
     require_device();
     // S blocks in slot order (row-major 6x6), scattered into the fronts on the device
-    std::vector<double> blocks(size_t(36) * S.s_col.size(), 0.0);
-    for (int32_t c = 0; c < nb; ++c)
-      for (int64_t k = col_ptr[c]; k < col_ptr[c + 1]; ++k) {
-        const int32_t r = int32_t(row_idx[k]);
-        auto b = S.s_col.begin() + S.s_ptr[r], e = S.s_col.begin() + S.s_ptr[r + 1];
-        const int64_t slot = int64_t(std::lower_bound(b, e, c) - S.s_col.begin());
-        for (int a = 0; a < 6; ++a)
-          for (int bb = 0; bb < 6; ++bb)
-            blocks[36 * slot + 6 * a + bb] += values[36 * k + 6 * bb + a];  // input column-major
-      }
     DeviceBufs bufs;
     DevProblem dp{};
     dp.nb = nb;
@@ -205,5 +219,58 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     SGPU_HIP(hipMemcpy(&sc, d_sc, sizeof(sc), hipMemcpyDeviceToHost));
     if (sc.bad_pivot) throw Error(SGPU_ERR_NON_FINITE, "non-finite pivot");
     SGPU_HIP(hipMemcpy(x, d_x, size_t(6) * nb * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,
+                                 const double* values, const double* rhs, double* x,
+                                 double relative_tolerance, int64_t max_iterations,
+                                 int64_t* iterations) {
+  return guarded([&] {
+    if (!(relative_tolerance > 0.0) || max_iterations <= 0)
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "PCG needs a positive tolerance and iteration cap");
+    Symbolic S;
+    std::vector<double> blocks;
+    parse_lower_blocks(num_blocks, col_ptr, row_idx, values, rhs, x, S, blocks);
+    const int32_t nb = S.nb;
+    std::vector<int32_t> mv_ptr, mv_ent, diag_slot;
+    build_pcg_pattern(nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot);
+    require_device();
+    DeviceBufs bufs;
+    DevPcg g{};
+    g.nb = nb;
+    g.mv_ptr = bufs.up(mv_ptr);
+    g.mv_ent = bufs.up(mv_ent);
+    g.diag_slot = bufs.up(diag_slot);
+    g.bsr = bufs.up(blocks);
+    g.minv = bufs.alloc<double>(size_t(36) * nb);
+    g.x = bufs.alloc<double>(size_t(6) * nb);
+    g.r = bufs.alloc<double>(size_t(6) * nb);
+    g.z = bufs.alloc<double>(size_t(6) * nb);
+    g.p = bufs.alloc<double>(size_t(6) * nb);
+    g.q = bufs.alloc<double>(size_t(6) * nb);
+    g.part = bufs.alloc<double>(size_t(pcg_partial_len(nb)));
+    g.state = bufs.alloc<PcgState>(1);
+    g.n_vec_blocks = pcg_vec_blocks(nb);
+    g.n_mv_blocks = pcg_mv_blocks(nb);
+    std::vector<double> b(rhs, rhs + 6 * num_blocks);
+    double* d_rhs = bufs.up(b);
+    PcgState* h_state = nullptr;
+    SGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_state), sizeof(PcgState), hipHostMallocDefault));
+    int32_t bad = 0, it = 0;
+    try {
+      launch_pcg_block_jacobi(nullptr, g);
+      it = run_pcg(nullptr, g, d_rhs, relative_tolerance,
+                   int32_t(std::min<int64_t>(max_iterations, INT32_MAX)), h_state, &bad);
+      SGPU_HIP(hipGetLastError());
+      SGPU_HIP(hipDeviceSynchronize());
+    } catch (...) {
+      (void)hipHostFree(h_state);
+      throw;
+    }
+    (void)hipHostFree(h_state);
+    if (iterations) *iterations = it;
+    if (bad) throw Error(SGPU_ERR_NON_FINITE, "PCG breakdown (p.Sp <= 0)");
+    SGPU_HIP(hipMemcpy(x, g.x, size_t(6) * nb * 8, hipMemcpyDeviceToHost));
   });
 }

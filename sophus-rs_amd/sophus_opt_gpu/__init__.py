@@ -179,6 +179,8 @@ _SIGNATURES = {
                                           _DP, _DP, _DP, _DP, _DP]),
     "sgpu_eval_pose_graph3_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
+    "sgpu_block_pcg_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP, C.c_double, C.c_int64,
+                                       _I64P]),
     "sgpu_generate_ba": (C.c_int, [C.POINTER(_BaSpec), C.POINTER(_BaSizes), _DP, _DP, _DP, _DP, _DP,
                                    _I64P, _DP]),
 }
@@ -655,11 +657,9 @@ def eval_pose_graph3_terms(world_from_m, world_from_n, m_from_n) -> dict:
     return out
 
 
-def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Kernel (d): block-sparse LDLt solve of a symmetric matrix with 6x6 blocks on the GPU.
-
-    Structurally zero 6x6 blocks of ``A_dense`` are skipped.
-    """
+def _lower_blocks(A_dense: np.ndarray):
+    """Lower 6x6 blocks of a dense symmetric matrix in CSC-by-block-column (column-major
+    blocks); structurally zero blocks are skipped."""
     A = np.asarray(A_dense, dtype=np.float64)
     n = A.shape[0]
     if n % 6:
@@ -673,15 +673,33 @@ def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray) -> np.ndarray:
                 rows.append(r)
                 vals.append(blk.T.copy())  # column-major block
         col_ptr.append(len(rows))
-    col_ptr = np.asarray(col_ptr, dtype=np.int64)
-    rows = np.asarray(rows, dtype=np.int64)
-    vals = np.ascontiguousarray(np.stack(vals), dtype=np.float64)
+    return (nb, np.asarray(col_ptr, dtype=np.int64), np.asarray(rows, dtype=np.int64),
+            np.ascontiguousarray(np.stack(vals), dtype=np.float64))
+
+
+def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """Kernel (d): block-sparse LDLt solve of a symmetric matrix with 6x6 blocks on the GPU."""
+    nb, col_ptr, rows, vals = _lower_blocks(A_dense)
     rhs = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
-    x = np.empty(n)
+    x = np.empty(6 * nb)
     _check(library().sgpu_block_ldlt_solve(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P),
                                            vals.ctypes.data_as(_DP), rhs.ctypes.data_as(_DP),
                                            x.ctypes.data_as(_DP)))
     return x
+
+
+def block_pcg_solve(A_dense: np.ndarray, b: np.ndarray, relative_tolerance: float = 1e-8,
+                    max_iterations: int = 500):
+    """Block-Jacobi PCG on the GPU (FP64, x0 = 0); returns (x, iterations)."""
+    nb, col_ptr, rows, vals = _lower_blocks(A_dense)
+    rhs = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+    x = np.empty(6 * nb)
+    it = C.c_int64(0)
+    _check(library().sgpu_block_pcg_solve(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P),
+                                          vals.ctypes.data_as(_DP), rhs.ctypes.data_as(_DP),
+                                          x.ctypes.data_as(_DP), relative_tolerance, max_iterations,
+                                          C.byref(it)))
+    return x, it.value
 
 
 # ---- synthetic problems ------------------------------------------------------------------------

@@ -232,3 +232,66 @@ def test_pose_graph_plus_ba_mixed_precision(sg, orc):
     sol, _, _ = orc.optimize_nlls(orc.ba_families(p), ocosts,
                                   orc.default_params(num_iterations=12, initial_lm_damping=1.0))
     assert abs(merit - sol["final_cost"]) <= 1e-6 * sol["final_cost"]
+
+
+# ---- block-Jacobi PCG on the reduced camera system (no reference counterpart; judged
+# against numpy's dense solve and the FP64 direct path) --------------------------------
+
+@pytest.mark.parametrize("nb,band,extra", [(1, 0, 0), (7, 2, 0), (60, 5, 20), (300, 12, 50)])
+def test_block_pcg_matches_dense_solve(sg, nb, band, extra):
+    A = _spd_block_matrix(nb, band, nb, extra)
+    b = np.random.default_rng(nb + 1).normal(size=6 * nb)
+    x, it = sg.block_pcg_solve(A, b, relative_tolerance=1e-13, max_iterations=2000)
+    x_np = np.linalg.solve(A, b)
+    assert np.linalg.norm(A @ x - b) <= 1e-12 * np.linalg.norm(b) * 10
+    assert _rel(x, x_np) < 1e-10
+    assert 1 <= it <= 2000
+
+
+def test_block_pcg_block_diagonal_converges_in_one_iteration(sg):
+    """With S block diagonal, block-Jacobi is the exact inverse."""
+    A = _spd_block_matrix(20, 0, 3, 0)
+    b = np.random.default_rng(4).normal(size=120)
+    x, it = sg.block_pcg_solve(A, b, relative_tolerance=1e-12)
+    assert it == 1 and _rel(x, np.linalg.solve(A, b)) < 1e-12
+
+
+def test_block_pcg_zero_rhs_and_breakdown(sg):
+    A = _spd_block_matrix(5, 1, 9, 0)
+    x, it = sg.block_pcg_solve(A, np.zeros(30))
+    assert it == 0 and np.all(x == 0)
+    # negative definite: p.Sp < 0 at the first iteration
+    with pytest.raises(sg.NllsError):
+        sg.block_pcg_solve(-np.eye(30), np.ones(30))
+
+
+def test_ba_ring_pcg_matches_direct(sg, orc):
+    p = sg.generate_ba(sg.config_spec(0))
+    params = sg.OptParams(num_iterations=20, initial_lm_damping=1.0, solver=sg.Solver.SchurPcgJacobi,
+                          pcg_relative_tolerance=1e-12, pcg_max_iterations=500)
+    merit, out, trace = _run_gpu(sg, p, params)
+    assert all(t.pcg_iterations > 0 for t in trace)
+    sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), orc.ba_costs(p),
+                                           orc.default_params(num_iterations=20, initial_lm_damping=1.0))
+    assert [t.accepted for t in trace] == [o["accepted"] for o in otrace]
+    assert abs(merit - sol["final_cost"]) <= 1e-8 * sol["final_cost"]
+
+
+def test_ba_walk_pcg_with_pose_graph(sg, orc):
+    spec = sg.BaSpec(layout=1, num_cameras=30, num_points=800, geometric_p=0.2, seed=11)
+    p = sg.generate_ba(spec)
+    pairs = np.array([[i, i + 1] for i in range(29)], dtype=np.int64)
+    meas = np.stack([orc.se3_mul(orc.se3_inverse(p.poses_gt[i]), p.poses_gt[j]) for i, j in pairs])
+    costs = p.cost_fns(1.0) + [sg.CostFn.pose_graph3(["poses", "poses"], pairs, meas,
+                                                     sg.RobustKernel.huber(0.5))]
+    params = sg.OptParams(num_iterations=12, initial_lm_damping=1.0, solver=sg.Solver.SchurPcgJacobi,
+                          pcg_relative_tolerance=1e-12, pcg_max_iterations=1000)
+    opt = sg.Optimizer(p.variables(), costs, params)
+    while opt.step().termination is None:
+        pass
+    merit = opt.merit()
+    opt.close()
+    ocosts = orc.ba_costs(p) + [(1, ["poses", "poses"], pairs, meas, 0.5)]
+    sol, _, _ = orc.optimize_nlls(orc.ba_families(p), ocosts,
+                                  orc.default_params(num_iterations=12, initial_lm_damping=1.0))
+    assert abs(merit - sol["final_cost"]) <= 1e-7 * sol["final_cost"]
