@@ -99,7 +99,7 @@ struct sgpu_optimizer {
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned
   DevPcg pcg{};                // iterative RCS solver state (PCG solvers only)
-  PcgState* h_pcg = nullptr;   // pinned
+  PcgState* h_pcg = nullptr;   // pinned poll buffer
   double merit = 0, nu = 0;
   int64_t rejects = 0, steps = 0;
   bool terminated = false;
@@ -293,12 +293,16 @@ struct sgpu_optimizer {
     dl.dx_cam = arena.alloc<double>(size_t(6) * S.nb);
     dl.bsr = nullptr;
     if (iterative()) {
-      std::vector<int32_t> mv_ptr, mv_ent, diag_slot;
-      build_pcg_pattern(S.nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot);
+      std::vector<int32_t> mv_ptr, mv_ent, diag_slot, mv_row;
+      build_pcg_pattern(S.nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot, mv_row);
       dl.bsr = arena.alloc<double>(size_t(36) * S.s_col.size());
       pcg.nb = S.nb;
       pcg.mv_ptr = arena.upload(mv_ptr, stream);
       pcg.mv_ent = arena.upload(mv_ent, stream);
+      pcg.n_full = mv_ptr[S.nb];
+      pcg.frow = arena.upload(mv_row, stream);
+      pcg.fbsr = arena.alloc<double>(size_t(36) * pcg.n_full);
+      pcg.fcol = arena.alloc<int32_t>(size_t(pcg.n_full));
       pcg.diag_slot = arena.upload(diag_slot, stream);
       pcg.bsr = dl.bsr;
       pcg.minv = arena.alloc<double>(size_t(36) * S.nb);
@@ -344,14 +348,14 @@ struct sgpu_optimizer {
     SGPU_HIP(hipEventRecord(phase_ev[0], stream));
     linearize(poses[cur], points[cur], nu, used);
     SGPU_HIP(hipEventRecord(phase_ev[1], stream));
-    int32_t pcg_iters = 0, pcg_bad = 0;
     if (iterative()) {
-      // block-Jacobi PCG on the resident S (the host polls the device state in chunks)
+      // block-Jacobi PCG on the resident S, one cooperative launch; its iteration count
+      // and breakdown flag travel back with the step scalars
       tic(K_FACTOR);
-      launch_pcg_block_jacobi(stream, pcg);
-      pcg_iters = run_pcg(stream, pcg, dl.rhs_nat, params.pcg_relative_tolerance,
-                          int32_t(std::min<int64_t>(params.pcg_max_iterations, INT32_MAX)), h_pcg,
-                          &pcg_bad);
+      launch_pcg_setup(stream, pcg);
+      launch_pcg(stream, pcg, dl.rhs_nat, params.pcg_relative_tolerance,
+                 int32_t(std::min<int64_t>(params.pcg_max_iterations, INT32_MAX)),
+                 &d_sc->pcg_iterations, &d_sc->bad_pivot, h_pcg);
       toc(K_FACTOR);
       SGPU_HIP(hipEventRecord(phase_ev[2], stream));
       tic(K_TRIANGULAR);
@@ -391,7 +395,7 @@ struct sgpu_optimizer {
     const double max_step = bits_to_double(sc.max_delta_bits);
     if (max_grad < params.gradient_tolerance) {
       reason = SGPU_TERM_GRADIENT_TOLERANCE;
-    } else if (sc.singular_marg || sc.bad_pivot || pcg_bad) {
+    } else if (sc.singular_marg || sc.bad_pivot) {
       nu *= 2.0;  // handle_solve_failure -> reject_step
       ++rejects;
       if (rejects >= params.max_consecutive_rejects) reason = SGPU_TERM_MAX_CONSECUTIVE_REJECTS;
@@ -446,7 +450,7 @@ struct sgpu_optimizer {
       info->gain_ratio = rho;
       info->max_abs_neg_grad = max_grad;
       info->max_abs_delta = max_step;
-      info->pcg_iterations = pcg_iters;
+      info->pcg_iterations = sc.pcg_iterations;
       info->ms_linearize = a;
       info->ms_factor = b;
       info->ms_solve = c;

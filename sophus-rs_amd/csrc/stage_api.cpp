@@ -233,14 +233,18 @@ sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr, con
     std::vector<double> blocks;
     parse_lower_blocks(num_blocks, col_ptr, row_idx, values, rhs, x, S, blocks);
     const int32_t nb = S.nb;
-    std::vector<int32_t> mv_ptr, mv_ent, diag_slot;
-    build_pcg_pattern(nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot);
+    std::vector<int32_t> mv_ptr, mv_ent, diag_slot, mv_row;
+    build_pcg_pattern(nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot, mv_row);
     require_device();
     DeviceBufs bufs;
     DevPcg g{};
     g.nb = nb;
     g.mv_ptr = bufs.up(mv_ptr);
     g.mv_ent = bufs.up(mv_ent);
+    g.n_full = mv_ptr[nb];
+    g.frow = bufs.up(mv_row);
+    g.fbsr = bufs.alloc<double>(size_t(36) * g.n_full);
+    g.fcol = bufs.alloc<int32_t>(size_t(g.n_full));
     g.diag_slot = bufs.up(diag_slot);
     g.bsr = bufs.up(blocks);
     g.minv = bufs.alloc<double>(size_t(36) * nb);
@@ -255,20 +259,14 @@ sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr, con
     g.n_mv_blocks = pcg_mv_blocks(nb);
     std::vector<double> b(rhs, rhs + 6 * num_blocks);
     double* d_rhs = bufs.up(b);
-    PcgState* h_state = nullptr;
-    SGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_state), sizeof(PcgState), hipHostMallocDefault));
-    int32_t bad = 0, it = 0;
-    try {
-      launch_pcg_block_jacobi(nullptr, g);
-      it = run_pcg(nullptr, g, d_rhs, relative_tolerance,
-                   int32_t(std::min<int64_t>(max_iterations, INT32_MAX)), h_state, &bad);
-      SGPU_HIP(hipGetLastError());
-      SGPU_HIP(hipDeviceSynchronize());
-    } catch (...) {
-      (void)hipHostFree(h_state);
-      throw;
-    }
-    (void)hipHostFree(h_state);
+    launch_pcg_setup(nullptr, g);
+    launch_pcg(nullptr, g, d_rhs, relative_tolerance,
+               int32_t(std::min<int64_t>(max_iterations, INT32_MAX)), nullptr, nullptr, nullptr);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    PcgState st;
+    SGPU_HIP(hipMemcpy(&st, g.state, sizeof(st), hipMemcpyDeviceToHost));
+    const int32_t it = st.iters, bad = st.bad;
     if (iterations) *iterations = it;
     if (bad) throw Error(SGPU_ERR_NON_FINITE, "PCG breakdown (p.Sp <= 0)");
     SGPU_HIP(hipMemcpy(x, g.x, size_t(6) * nb * 8, hipMemcpyDeviceToHost));
