@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 1
+#define SGPU_ABI_VERSION 2
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -162,6 +162,9 @@ typedef struct sgpu_opt_params {
   int64_t pcg_max_iterations;    /* default 500 */
   int32_t device;                /* HIP device ordinal, default 0 */
   int32_t reserved0;
+  /* SCHUR_PCG_LEARNED only: SGPU_PRECOND_WEIGHTS FP32 weights of the block-Jacobi
+   * correction (copied at create; see sgpu_precond_train).  Default NULL. */
+  const float* precond_weights;
 } sgpu_opt_params;
 
 /* StepInfo (optimizer_types.rs:67-83) plus per-step diagnostics. */
@@ -323,6 +326,57 @@ sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr,
                                  const int64_t* row_idx, const double* values,
                                  const double* rhs, double* x, double relative_tolerance,
                                  int64_t max_iterations, int64_t* iterations);
+
+/* ---- learned block-Jacobi correction (no reference counterpart) -------------------
+ * For camera block b of the reduced camera system with D = S_bb: J = diag(D)^-1/2,
+ * J D J = L L^T, B = J L^-T.  A 3-layer MLP (32 -> 128 -> 128 -> 21, tanh hidden
+ * layers) maps 32 features of the block (the correlations and log-scales of D, the
+ * neighbour count, norms of the scaled off-diagonal blocks) to a lower-triangular G
+ * (diagonal exp(o_ii), row-major lower-triangle order) and the preconditioner block
+ * is M_b^-1 = B G G^T B^T (G = I: block-Jacobi).  Inference uses bf16 weights with FP32
+ * accumulation.  Weight layout (FP32): W1[128][32], b1[128], W2[128][128], b2[128],
+ * W3[21][128], b3[21]. */
+#define SGPU_PRECOND_FEATURES 32
+#define SGPU_PRECOND_OUTPUTS 21
+#define SGPU_PRECOND_WEIGHTS 23445
+
+/* Features of every camera block of a block system given like sgpu_block_ldlt_solve's
+ * input; features_out is num_blocks x 32. */
+sgpu_status sgpu_precond_features(int64_t num_blocks, const int64_t* col_ptr,
+                                  const int64_t* row_idx, const double* values,
+                                  float* features_out);
+/* MLP outputs (n x 21) for n feature rows; use_bf16 = 1 rounds the weights to bf16. */
+sgpu_status sgpu_precond_forward(int64_t n, const float* features, const float* weights,
+                                 int32_t use_bf16, float* outputs);
+/* M_b^-1 = B G G^T B^T for n diagonal blocks (n x 36, row-major) and outputs (n x 21);
+ * blocks that are not positive definite give NaN. */
+sgpu_status sgpu_precond_blocks(int64_t n, const double* diag_blocks, const float* outputs,
+                                double* minv_out);
+/* Training loss and its gradient for n blocks: loss = sum_b |G G^T B^T u_b - w_b|^2 /
+ * |w_b|^2 (FP32 network, FP64 loss), grad_out = d loss / d weights (SGPU_PRECOND_WEIGHTS). */
+sgpu_status sgpu_precond_loss_grad(int64_t n, const float* features, const double* diag_blocks,
+                                   const double* u, const double* w, const float* weights,
+                                   double* loss_out, float* grad_out);
+
+typedef struct sgpu_precond_train_params {
+  int32_t num_problems;  /* seeded random-walk BA problems generating the snapshots */
+  int32_t num_cameras;
+  int64_t num_points;
+  double geometric_p;
+  int32_t lm_iterations; /* LM steps per problem; one RCS snapshot per step */
+  int32_t epochs;
+  int32_t probes;        /* probe vectors per snapshot and step */
+  int32_t device;
+  double learning_rate;  /* Adam */
+  uint64_t seed;
+} sgpu_precond_train_params;
+
+/* Trains the correction on the GPU (hand-written forward, backward and Adam kernels)
+ * from RCS snapshots the solver generates on seeded problems.  weights_inout holds
+ * the initial weights and receives the trained ones; loss_history (epochs + 1 entries)
+ * receives the mean loss before training and after each epoch. */
+sgpu_status sgpu_precond_train(const sgpu_precond_train_params* params, float* weights_inout,
+                               double* loss_history, int64_t* num_snapshots);
 
 /* ---- synthetic problems ---------------------------------------------------------
  * Seeded BA generator (SplitMix64 -> xoshiro256**, Box-Muller), the input of the

@@ -42,7 +42,8 @@ class OptParamsC(C.Structure):
                 ("precision", C.c_int32), ("function_tolerance", C.c_double),
                 ("gradient_tolerance", C.c_double), ("parameter_tolerance", C.c_double),
                 ("max_consecutive_rejects", C.c_int64), ("pcg_relative_tolerance", C.c_double),
-                ("pcg_max_iterations", C.c_int64), ("device", C.c_int32), ("reserved0", C.c_int32)]
+                ("pcg_max_iterations", C.c_int64), ("device", C.c_int32), ("reserved0", C.c_int32),
+                ("precond_weights", C.c_void_p)]
 
 
 class SolutionC(C.Structure):

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/sophus_gpu.h"
 
@@ -47,5 +48,28 @@ sgpu_status guarded(F&& body) {
     return set_error(SGPU_ERR_INVALID_ARGUMENT, e.what());
   }
 }
+
+// Device allocations owned by one object, freed together.
+struct DeviceArena {
+  std::vector<void*> ptrs;
+  template <typename T>
+  T* alloc(size_t n) {
+    if (n == 0) n = 1;
+    void* p = nullptr;
+    SGPU_HIP(hipMalloc(&p, n * sizeof(T)));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  template <typename T>
+  T* upload(const std::vector<T>& v, hipStream_t s) {
+    T* p = alloc<T>(v.size());
+    (void)s;  // one-time upload: blocking copy, so temporaries may die right after
+    if (!v.empty()) SGPU_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  ~DeviceArena() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+};
 
 }  // namespace sgpu

@@ -17,6 +17,8 @@
 
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "learned.hpp"
+#include "optimizer_internal.hpp"
 #include "pcg.hpp"
 #include "problem.hpp"
 
@@ -53,27 +55,6 @@ double bits_to_double(unsigned long long b) {
   return d;
 }
 
-struct DeviceArena {
-  std::vector<void*> ptrs;
-  template <typename T>
-  T* alloc(size_t n) {
-    if (n == 0) n = 1;
-    void* p = nullptr;
-    SGPU_HIP(hipMalloc(&p, n * sizeof(T)));
-    ptrs.push_back(p);
-    return static_cast<T*>(p);
-  }
-  template <typename T>
-  T* upload(const std::vector<T>& v, hipStream_t s) {
-    T* p = alloc<T>(v.size());
-    (void)s;  // one-time upload: blocking copy, so temporaries may die right after
-    if (!v.empty()) SGPU_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return p;
-  }
-  ~DeviceArena() {
-    for (void* p : ptrs) (void)hipFree(p);
-  }
-};
 
 }  // namespace
 }  // namespace sgpu
@@ -100,6 +81,7 @@ struct sgpu_optimizer {
   DevScalars* h_sc = nullptr;  // pinned
   DevPcg pcg{};                // iterative RCS solver state (PCG solvers only)
   PcgState* h_pcg = nullptr;   // pinned poll buffer
+  DevLearned learned{};        // learned block-Jacobi correction (PCG_LEARNED only)
   double merit = 0, nu = 0;
   int64_t rejects = 0, steps = 0;
   bool terminated = false;
@@ -189,11 +171,14 @@ struct sgpu_optimizer {
             const sgpu_opt_params* prm) {
     params = *prm;
     if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT &&
-        params.solver != SGPU_SOLVER_SCHUR_PCG_JACOBI)
+        params.solver != SGPU_SOLVER_SCHUR_PCG_JACOBI && params.solver != SGPU_SOLVER_SCHUR_PCG_LEARNED)
       throw Error(SGPU_ERR_UNSUPPORTED, "solver not available");
-    if (params.solver == SGPU_SOLVER_SCHUR_PCG_JACOBI &&
-        !(params.pcg_relative_tolerance > 0.0 && params.pcg_max_iterations > 0))
+    if (iterative() && !(params.pcg_relative_tolerance > 0.0 && params.pcg_max_iterations > 0))
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "PCG needs a positive tolerance and iteration cap");
+    if (params.solver == SGPU_SOLVER_SCHUR_PCG_LEARNED && !params.precond_weights)
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "the learned preconditioner needs precond_weights");
+    const float* precond_weights = params.precond_weights;
+    params.precond_weights = nullptr;  // copied below; the caller's array is not kept
     if (params.precision != SGPU_PRECISION_F64 && params.precision != SGPU_PRECISION_MIXED)
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown precision");
     P = build_host_problem(families, nf, cfs, nc);
@@ -316,6 +301,13 @@ struct sgpu_optimizer {
       pcg.n_vec_blocks = pcg_vec_blocks(S.nb);
       pcg.n_mv_blocks = pcg_mv_blocks(S.nb);
       SGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_pcg), sizeof(PcgState), hipHostMallocDefault));
+      if (params.solver == SGPU_SOLVER_SCHUR_PCG_LEARNED) {
+        learned.wbf16 = arena.upload(weights_to_bf16(precond_weights, kPcWeights), stream);
+        learned.feat = arena.alloc<float>(size_t(kPcFeat) * S.nb);
+        learned.h1 = arena.alloc<float>(size_t(kPcHid) * S.nb);
+        learned.h2 = arena.alloc<float>(size_t(kPcHid) * S.nb);
+        learned.out = arena.alloc<float>(size_t(kPcOut) * S.nb);
+      }
     }
     dl.partial_cap = 4 * kCostBlocks;
     dl.partial = arena.alloc<double>(dl.partial_cap);
@@ -353,6 +345,8 @@ struct sgpu_optimizer {
       // and breakdown flag travel back with the step scalars
       tic(K_FACTOR);
       launch_pcg_setup(stream, pcg);
+      if (params.solver == SGPU_SOLVER_SCHUR_PCG_LEARNED)
+        launch_learned_preconditioner(stream, pcg, learned);
       launch_pcg(stream, pcg, dl.rhs_nat, params.pcg_relative_tolerance,
                  int32_t(std::min<int64_t>(params.pcg_max_iterations, INT32_MAX)),
                  &d_sc->pcg_iterations, &d_sc->bad_pivot, h_pcg);
@@ -663,3 +657,9 @@ sgpu_status sgpu_optimize_nlls(const sgpu_var_family* families, int64_t num_fami
 }
 
 }  // extern "C"
+
+namespace sgpu {
+const DevPcg& optimizer_pcg(const sgpu_optimizer* opt) { return opt->pcg; }
+hipStream_t optimizer_stream(const sgpu_optimizer* opt) { return opt->stream; }
+int64_t optimizer_nslot(const sgpu_optimizer* opt) { return opt->dp.nslot; }
+}  // namespace sgpu

@@ -7,6 +7,7 @@
 
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "learned.hpp"
 #include "pcg.hpp"
 #include "problem.hpp"
 
@@ -270,5 +271,38 @@ sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr, con
     if (iterations) *iterations = it;
     if (bad) throw Error(SGPU_ERR_NON_FINITE, "PCG breakdown (p.Sp <= 0)");
     SGPU_HIP(hipMemcpy(x, g.x, size_t(6) * nb * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+sgpu_status sgpu_precond_features(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,
+                                  const double* values, float* features_out) {
+  return guarded([&] {
+    if (!features_out) throw Error(SGPU_ERR_INVALID_ARGUMENT, "features_out required");
+    Symbolic S;
+    std::vector<double> blocks;
+    std::vector<double> dummy(6 * std::max<int64_t>(num_blocks, 1), 0.0);
+    parse_lower_blocks(num_blocks, col_ptr, row_idx, values, dummy.data(), dummy.data(), S, blocks);
+    const int32_t nb = S.nb;
+    std::vector<int32_t> mv_ptr, mv_ent, diag_slot, mv_row;
+    build_pcg_pattern(nb, S.s_ptr, S.s_col, mv_ptr, mv_ent, diag_slot, mv_row);
+    require_device();
+    DeviceBufs bufs;
+    DevPcg g{};
+    g.nb = nb;
+    g.mv_ptr = bufs.up(mv_ptr);
+    g.mv_ent = bufs.up(mv_ent);
+    g.n_full = mv_ptr[nb];
+    g.frow = bufs.up(mv_row);
+    g.fbsr = bufs.alloc<double>(size_t(36) * g.n_full);
+    g.fcol = bufs.alloc<int32_t>(size_t(g.n_full));
+    g.diag_slot = bufs.up(diag_slot);
+    g.bsr = bufs.up(blocks);
+    g.minv = bufs.alloc<double>(size_t(36) * nb);
+    float* feat = bufs.alloc<float>(size_t(kPcFeat) * nb);
+    launch_pcg_setup(nullptr, g);
+    launch_precond_features(nullptr, g, feat);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    SGPU_HIP(hipMemcpy(features_out, feat, size_t(kPcFeat) * nb * 4, hipMemcpyDeviceToHost));
   });
 }

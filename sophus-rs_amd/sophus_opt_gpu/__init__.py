@@ -75,6 +75,7 @@ class _OptParams(C.Structure):
         ("pcg_max_iterations", C.c_int64),
         ("device", C.c_int32),
         ("reserved0", C.c_int32),
+        ("precond_weights", C.POINTER(C.c_float)),
     ]
 
 
@@ -150,8 +151,22 @@ class _BaSizes(C.Structure):
     _fields_ = [("num_cameras", C.c_int64), ("num_points", C.c_int64), ("num_obs", C.c_int64)]
 
 
+class _PrecondTrainParams(C.Structure):
+    _fields_ = [("num_problems", C.c_int32), ("num_cameras", C.c_int32), ("num_points", C.c_int64),
+                ("geometric_p", C.c_double), ("lm_iterations", C.c_int32), ("epochs", C.c_int32),
+                ("probes", C.c_int32), ("device", C.c_int32), ("learning_rate", C.c_double),
+                ("seed", C.c_uint64)]
+
+
 _DP = C.POINTER(C.c_double)
 _I64P = C.POINTER(C.c_int64)
+_FP = C.POINTER(C.c_float)
+
+PRECOND_FEATURES = 32
+PRECOND_HIDDEN = 128
+PRECOND_OUTPUTS = 21
+PRECOND_WEIGHTS = (PRECOND_HIDDEN * PRECOND_FEATURES + PRECOND_HIDDEN + PRECOND_HIDDEN * PRECOND_HIDDEN
+                   + PRECOND_HIDDEN + PRECOND_OUTPUTS * PRECOND_HIDDEN + PRECOND_OUTPUTS)
 
 # Every entry point declared in include/sophus_gpu.h with its ctypes signature.
 _SIGNATURES = {
@@ -181,6 +196,11 @@ _SIGNATURES = {
     "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_block_pcg_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP, C.c_double, C.c_int64,
                                        _I64P]),
+    "sgpu_precond_features": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _FP]),
+    "sgpu_precond_forward": (C.c_int, [C.c_int64, _FP, _FP, C.c_int32, _FP]),
+    "sgpu_precond_blocks": (C.c_int, [C.c_int64, _DP, _FP, _DP]),
+    "sgpu_precond_loss_grad": (C.c_int, [C.c_int64, _FP, _DP, _DP, _DP, _FP, _DP, _FP]),
+    "sgpu_precond_train": (C.c_int, [C.POINTER(_PrecondTrainParams), _FP, _DP, _I64P]),
     "sgpu_generate_ba": (C.c_int, [C.POINTER(_BaSpec), C.POINTER(_BaSizes), _DP, _DP, _DP, _DP, _DP,
                                    _I64P, _DP]),
 }
@@ -427,6 +447,7 @@ class OptParams:
     pcg_relative_tolerance: float = 1e-8
     pcg_max_iterations: int = 500
     device: int = 0
+    precond_weights: Optional[np.ndarray] = None  # SchurPcgLearned: PRECOND_WEIGHTS float32
 
     def _c(self) -> _OptParams:
         p = _OptParams()
@@ -444,6 +465,12 @@ class OptParams:
         p.pcg_relative_tolerance = float(self.pcg_relative_tolerance)
         p.pcg_max_iterations = int(self.pcg_max_iterations)
         p.device = int(self.device)
+        if self.precond_weights is not None:
+            w = np.ascontiguousarray(self.precond_weights, dtype=np.float32).reshape(-1)
+            if w.size != PRECOND_WEIGHTS:
+                raise ValueError(f"precond_weights must hold {PRECOND_WEIGHTS} floats")
+            self._w_keep = w  # the library copies it during the call that takes these params
+            p.precond_weights = w.ctypes.data_as(C.POINTER(C.c_float))
         return p
 
 
@@ -700,6 +727,106 @@ def block_pcg_solve(A_dense: np.ndarray, b: np.ndarray, relative_tolerance: floa
                                           x.ctypes.data_as(_DP), relative_tolerance, max_iterations,
                                           C.byref(it)))
     return x, it.value
+
+
+# ---- learned block-Jacobi correction ------------------------------------------------------
+
+
+def precond_weight_views(w: np.ndarray) -> dict:
+    """Views of the flat weight vector: W1 (128x32), b1, W2 (128x128), b2, W3 (21x128), b3."""
+    w = np.asarray(w).reshape(-1)
+    out, o = {}, 0
+    for name, shape in (("W1", (PRECOND_HIDDEN, PRECOND_FEATURES)), ("b1", (PRECOND_HIDDEN,)),
+                        ("W2", (PRECOND_HIDDEN, PRECOND_HIDDEN)), ("b2", (PRECOND_HIDDEN,)),
+                        ("W3", (PRECOND_OUTPUTS, PRECOND_HIDDEN)), ("b3", (PRECOND_OUTPUTS,))):
+        n = int(np.prod(shape))
+        out[name] = w[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+def init_precond_weights(seed: int = 0, out_scale: float = 0.0) -> np.ndarray:
+    """Scaled-uniform hidden layers; the output layer is zero (times `out_scale` noise), so
+    the untrained correction is exactly block-Jacobi."""
+    rng = np.random.default_rng(seed)
+    w = np.zeros(PRECOND_WEIGHTS, dtype=np.float32)
+    v = precond_weight_views(w)
+    v["W1"][:] = rng.uniform(-1, 1, v["W1"].shape) * np.sqrt(3.0 / PRECOND_FEATURES)
+    v["W2"][:] = rng.uniform(-1, 1, v["W2"].shape) * np.sqrt(3.0 / PRECOND_HIDDEN)
+    v["W3"][:] = rng.uniform(-1, 1, v["W3"].shape) * out_scale
+    return w
+
+
+def precond_features(A_dense: np.ndarray) -> np.ndarray:
+    """Features (num_blocks x 32) of every 6x6 block row of a symmetric block matrix."""
+    nb, col_ptr, rows, vals = _lower_blocks(A_dense)
+    f = np.empty((nb, PRECOND_FEATURES), dtype=np.float32)
+    _check(library().sgpu_precond_features(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P),
+                                           vals.ctypes.data_as(_DP), f.ctypes.data_as(_FP)))
+    return f
+
+
+def precond_forward(features: np.ndarray, weights: np.ndarray, bf16: bool = False) -> np.ndarray:
+    f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PRECOND_FEATURES)
+    w = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+    out = np.empty((f.shape[0], PRECOND_OUTPUTS), dtype=np.float32)
+    _check(library().sgpu_precond_forward(f.shape[0], f.ctypes.data_as(_FP), w.ctypes.data_as(_FP),
+                                          int(bool(bf16)), out.ctypes.data_as(_FP)))
+    return out
+
+
+def precond_blocks(diag_blocks: np.ndarray, outputs: np.ndarray) -> np.ndarray:
+    D = np.ascontiguousarray(diag_blocks, dtype=np.float64).reshape(-1, 6, 6)
+    o = np.ascontiguousarray(outputs, dtype=np.float32).reshape(-1, PRECOND_OUTPUTS)
+    out = np.empty_like(D)
+    _check(library().sgpu_precond_blocks(D.shape[0], D.ctypes.data_as(_DP), o.ctypes.data_as(_FP),
+                                         out.ctypes.data_as(_DP)))
+    return out
+
+
+def precond_loss_grad(features, diag_blocks, u, w, weights):
+    f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PRECOND_FEATURES)
+    D = np.ascontiguousarray(diag_blocks, dtype=np.float64).reshape(-1, 6, 6)
+    uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1, 6)
+    ww = np.ascontiguousarray(w, dtype=np.float64).reshape(-1, 6)
+    wt = np.ascontiguousarray(weights, dtype=np.float32).reshape(-1)
+    g = np.empty(PRECOND_WEIGHTS, dtype=np.float32)
+    loss = C.c_double()
+    _check(library().sgpu_precond_loss_grad(f.shape[0], f.ctypes.data_as(_FP), D.ctypes.data_as(_DP),
+                                            uu.ctypes.data_as(_DP), ww.ctypes.data_as(_DP),
+                                            wt.ctypes.data_as(_FP), C.byref(loss), g.ctypes.data_as(_FP)))
+    return loss.value, g
+
+
+@dataclass
+class PrecondTrainParams:
+    num_problems: int = 16
+    num_cameras: int = 80
+    num_points: int = 3000
+    geometric_p: float = 0.15
+    lm_iterations: int = 10
+    epochs: int = 30
+    probes: int = 4
+    device: int = 0
+    learning_rate: float = 1e-3
+    seed: int = 1000
+
+    def _c(self) -> _PrecondTrainParams:
+        c = _PrecondTrainParams()
+        for k, _ in _PrecondTrainParams._fields_:
+            setattr(c, k, getattr(self, k))
+        return c
+
+
+def train_preconditioner(params: PrecondTrainParams, weights: Optional[np.ndarray] = None):
+    """Train on the GPU; returns (weights, loss history (epochs + 1), snapshots used)."""
+    w = (init_precond_weights(params.seed) if weights is None
+         else np.array(weights, dtype=np.float32).reshape(-1).copy())
+    hist = np.zeros(params.epochs + 1)
+    n = C.c_int64()
+    _check(library().sgpu_precond_train(C.byref(params._c()), w.ctypes.data_as(_FP), hist.ctypes.data_as(_DP),
+                                        C.byref(n)))
+    return w, hist, n.value
 
 
 # ---- synthetic problems ------------------------------------------------------------------------

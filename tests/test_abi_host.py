@@ -37,9 +37,13 @@ def test_abi_version_and_default_params(sg):
     assert p.num_iterations == 100 and p.initial_lm_damping == 10.0 and p.parallelize == 1
     assert p.skip_final_hessian == 0 and p.max_consecutive_rejects == 20
     assert (p.function_tolerance, p.gradient_tolerance, p.parameter_tolerance) == (1e-10, 1e-14, 1e-12)
+    assert not p.precond_weights  # NULL by default
     py = sg.OptParams()._c()
     for k, _ in sg._OptParams._fields_:
-        assert getattr(py, k) == getattr(p, k), k
+        if k == "precond_weights":
+            assert not py.precond_weights
+        else:
+            assert getattr(py, k) == getattr(p, k), k
 
 
 def test_generator_is_deterministic_and_seeded(sg):

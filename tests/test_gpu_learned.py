@@ -1,0 +1,133 @@
+"""GPU tests of the learned block-Jacobi correction (no reference counterpart): the
+HIP features, MLP forward (FP32 and bf16), preconditioner blocks and training gradient
+against the numpy oracle (oracle/precond_oracle.py, itself pinned by torch autograd);
+the untrained network reproduces block-Jacobi PCG; training lowers the loss, and the
+trained correction is compared with plain block-Jacobi on held-out problems.
+
+Tolerances: FP32 network outputs 1e-5 relative, bf16 weights 3e-2; gradients 2e-3
+relative to the largest entry (FP32 accumulation over blocks); FP64 blocks 1e-12.
+"""
+import numpy as np
+import pytest
+
+import precond_oracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd_block_matrix(nb, band, seed):
+    rng = np.random.default_rng(seed)
+    n = 6 * nb
+    A = np.zeros((n, n))
+    for i in range(nb):
+        for j in range(max(0, i - band), i + 1):
+            B = rng.normal(0, 1, (6, 6)) * (3.0 if i == j else 1.0)
+            A[6 * i:6 * i + 6, 6 * j:6 * j + 6] += B
+            if i != j:
+                A[6 * j:6 * j + 6, 6 * i:6 * i + 6] += B.T
+    A = 0.5 * (A + A.T)
+    A += np.eye(n) * (np.abs(A).sum(axis=1).max() + 1.0) * np.repeat(np.exp(rng.normal(size=nb)), 6)
+    return A
+
+
+def test_features_match_oracle(sg):
+    A = _spd_block_matrix(40, 4, 0)
+    f = sg.precond_features(A)
+    fo = po.features(A)
+    assert np.max(np.abs(f - fo)) <= 1e-5 * max(1.0, np.max(np.abs(fo)))
+
+
+def test_forward_matches_oracle(sg):
+    rng = np.random.default_rng(1)
+    F = rng.normal(size=(300, 32)).astype(np.float32)
+    w = sg.init_precond_weights(3, out_scale=0.2)
+    o32 = sg.precond_forward(F, w)
+    oo, _ = po.forward(F, w)
+    scale = np.max(np.abs(oo))
+    assert np.max(np.abs(o32 - oo)) <= 1e-5 * scale
+    o16 = sg.precond_forward(F, w, bf16=True)
+    assert np.max(np.abs(o16 - oo)) <= 3e-2 * scale
+
+
+def test_blocks_match_oracle_and_zero_output_is_block_jacobi(sg):
+    rng = np.random.default_rng(2)
+    Ds = []
+    for _ in range(50):
+        A = rng.normal(size=(6, 6))
+        Ds.append(A @ A.T + 6 * np.eye(6) * np.exp(rng.normal(size=6)))
+    Ds = np.array(Ds)
+    out = rng.normal(size=(50, 21)).astype(np.float32) * 0.3
+    M = sg.precond_blocks(Ds, out)
+    for b in range(50):
+        Mo = po.minv(Ds[b], out[b].astype(np.float64))
+        assert np.max(np.abs(M[b] - Mo)) <= 1e-12 * np.max(np.abs(Mo))
+    M0 = sg.precond_blocks(Ds, np.zeros((50, 21), dtype=np.float32))
+    for b in range(50):
+        Di = np.linalg.inv(Ds[b])
+        assert np.max(np.abs(M0[b] - Di)) <= 1e-12 * np.max(np.abs(Di))
+
+
+def test_loss_gradient_matches_oracle(sg):
+    rng = np.random.default_rng(3)
+    n = 64
+    F = (rng.normal(size=(n, 32)) * 0.5).astype(np.float32)
+    Ds = []
+    for _ in range(n):
+        A = rng.normal(size=(6, 6))
+        Ds.append(A @ A.T + 6 * np.eye(6) * np.exp(rng.normal(size=6)))
+    Ds = np.array(Ds)
+    u = rng.normal(size=(n, 6))
+    w = rng.normal(size=(n, 6))
+    weights = sg.init_precond_weights(4, out_scale=0.1)
+    loss, grad = sg.precond_loss_grad(F, Ds, u, w, weights)
+    lo, go = po.loss_grad(F.astype(np.float64), Ds, u, w, weights.astype(np.float64))
+    assert abs(loss - lo) <= 1e-5 * abs(lo)
+    assert np.max(np.abs(grad - go)) <= 2e-3 * np.max(np.abs(go))
+
+
+def _ba_run(sg, p, solver, weights=None, iters=8, tol=1e-8):
+    params = sg.OptParams(num_iterations=iters, initial_lm_damping=1.0, solver=solver,
+                          pcg_relative_tolerance=tol, pcg_max_iterations=2000, precond_weights=weights)
+    opt = sg.Optimizer(p.variables(), p.cost_fns(1.0), params)
+    trace = []
+    while True:
+        info = opt.step()
+        trace.append(info)
+        if info.termination is not None:
+            break
+    merit = opt.merit()
+    opt.close()
+    return merit, trace
+
+
+def test_untrained_network_reproduces_block_jacobi(sg):
+    p = sg.generate_ba(sg.BaSpec(layout=1, num_cameras=40, num_points=1500, geometric_p=0.2, seed=21))
+    w0 = sg.init_precond_weights(0)  # output layer zero: G = I
+    m_l, t_l = _ba_run(sg, p, sg.Solver.SchurPcgLearned, w0)
+    m_j, t_j = _ba_run(sg, p, sg.Solver.SchurPcgJacobi)
+    assert [t.accepted for t in t_l] == [t.accepted for t in t_j]
+    assert abs(m_l - m_j) <= 1e-9 * m_j
+    for a, b in zip(t_l, t_j):
+        assert abs(a.pcg_iterations - b.pcg_iterations) <= 2
+
+
+def test_training_lowers_loss_and_reports_gain(sg):
+    tp = sg.PrecondTrainParams(num_problems=6, num_cameras=40, num_points=1500, geometric_p=0.2,
+                               lm_iterations=6, epochs=12, probes=4, learning_rate=2e-3, seed=500)
+    w, hist, nsnap = sg.train_preconditioner(tp)
+    print(f"[learned] snapshots {nsnap}, loss history {np.round(hist, 5).tolist()}")
+    assert nsnap >= 6 * 3
+    assert np.all(np.isfinite(hist))
+    assert hist[-1] < hist[0]
+    # held-out problems: PCG iterations with and without the correction (same tolerance)
+    tot_l = tot_j = 0
+    for seed in (900, 901):
+        p = sg.generate_ba(sg.BaSpec(layout=1, num_cameras=40, num_points=1500, geometric_p=0.2, seed=seed))
+        m_l, t_l = _ba_run(sg, p, sg.Solver.SchurPcgLearned, w, iters=6)
+        m_j, t_j = _ba_run(sg, p, sg.Solver.SchurPcgJacobi, iters=6)
+        tot_l += sum(t.pcg_iterations for t in t_l)
+        tot_j += sum(t.pcg_iterations for t in t_j)
+        assert abs(m_l - m_j) <= 1e-6 * m_j  # same solution quality at tol 1e-8
+    print(f"[learned] held-out PCG iterations: learned {tot_l}, block-Jacobi {tot_j}, "
+          f"ratio {tot_l / tot_j:.3f}")
+    assert tot_l <= 1.10 * tot_j
