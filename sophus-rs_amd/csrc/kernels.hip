@@ -669,7 +669,9 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Extend-add: scatter each finished node's update matrix into its parent's front.
+// Extend-add: scatter each finished node's update matrix into its parent's front.  The
+// launch's nodes are distinct children of distinct parents (one sibling rank at a time),
+// so the adds need no atomics and their order is fixed.
 __global__ void __launch_bounds__(kThreads)
     k_mf_extend(DevFactor f, const int32_t* __restrict__ nodes) {
   const int X = nodes[blockIdx.y];
@@ -684,7 +686,8 @@ __global__ void __launch_bounds__(kThreads)
     const int pc = 6 * up[c / 6] + c % 6;
     for (int r = c + threadIdx.x; r < m; r += blockDim.x) {
       const double v = U[int64_t(c) * F + r];
-      if (v != 0.0) atomic_add_d(B + int64_t(pc) * FP + 6 * up[r / 6] + r % 6, v);
+      // one child per parent per launch and distinct targets within a child: plain add
+      if (v != 0.0) B[int64_t(pc) * FP + 6 * up[r / 6] + r % 6] += v;
     }
   }
 }
@@ -720,7 +723,8 @@ __global__ void __launch_bounds__(kThreads)
   const double* A = f.lbuf + f.sn_loff[X];
   __shared__ double yl[6 * 96];
   __shared__ double Lp[kPW][kPW + 1];
-  for (int i = threadIdx.x; i < K; i += blockDim.x) yl[i] = y[6 * c0 + i];
+  const double* ux = f.usolve + f.sn_uoff[X];  // children's updates to this node's rows
+  for (int i = threadIdx.x; i < K; i += blockDim.x) yl[i] = y[6 * c0 + i] - ux[i];
   for (int k0 = 0; k0 < K; k0 += kPW) {
     const int kend = min(K, k0 + kPW), np = kend - k0;
     for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
@@ -755,14 +759,14 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// rows below the node's own: y[ancestor rows] -= L_rect y_K; items = (node, row tile)
+// rows below the node's own: u_X[rows] += L_rect y_K, completing the node's update to
+// its ancestors' rows (children's updates are already in u_X); items = (node, row tile)
 __global__ void __launch_bounds__(kThreads)
-    k_mf_fwd_rect(DevFactor f, const int32_t* __restrict__ items, double* __restrict__ y,
-                  const double* __restrict__ ytmp) {
+    k_mf_fwd_rect(DevFactor f, const int32_t* __restrict__ items, const double* __restrict__ ytmp) {
   const int X = items[2 * blockIdx.x], tile = items[2 * blockIdx.x + 1];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* A = f.lbuf + f.sn_loff[X];
-  const int32_t* rows = f.sn_rows + f.sn_rows_ptr[X];
+  double* ux = f.usolve + f.sn_uoff[X];
   __shared__ double yk[6 * 96];
   for (int i = threadIdx.x; i < K; i += blockDim.x) yk[i] = ytmp[6 * c0 + i];
   __syncthreads();
@@ -770,7 +774,21 @@ __global__ void __launch_bounds__(kThreads)
   if (r >= F) return;
   double s = 0;
   for (int c = 0; c < K; ++c) s += A[int64_t(c) * F + r] * yk[c];
-  if (s != 0.0) atomic_add_d(y + 6 * rows[r / 6] + r % 6, -s);
+  ux[r] += s;
+}
+
+// u_P[parent rows] += u_X[rows below K] for one sibling rank of a level (plain adds).
+__global__ void __launch_bounds__(kThreads)
+    k_mf_fwd_extend(DevFactor f, const int32_t* __restrict__ nodes) {
+  const int X = nodes[blockIdx.y];
+  const int P = f.sn_parent[X];
+  if (P < 0) return;
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], m = F - K;
+  const double* ux = f.usolve + f.sn_uoff[X] + K;
+  double* up_vec = f.usolve + f.sn_uoff[P];
+  const int32_t* up = f.sn_upos + f.sn_upos_ptr[X];
+  for (int r = blockIdx.x * kThreads + threadIdx.x; r < m; r += gridDim.x * kThreads)
+    up_vec[6 * up[r / 6] + r % 6] += ux[r];
 }
 
 // t_K = z_K - L_rect^T x_rows for one tile of 64 node columns (one wavefront per column
@@ -1340,6 +1358,17 @@ void launch_zero_scalars(hipStream_t s, DevScalars* sc) {
   hipLaunchKernelGGL(k_zero_scalars, dim3(1), dim3(64), 0, s, sc);
 }
 
+// extend-add of one level's nodes, one sibling rank per launch
+static void launch_extend_groups(hipStream_t s, const DevFactor& f, int level) {
+  const Symbolic& S = *f.host;
+  for (int q = S.ext_lvl_ptr[level]; q < S.ext_lvl_ptr[level + 1]; ++q) {
+    const int cnt = S.ext_grp_ptr[q + 1] - S.ext_grp_ptr[q];
+    if (cnt > 0)
+      hipLaunchKernelGGL(k_mf_extend, dim3(32, cnt), dim3(kThreads), 0, s, f,
+                         f.ext_nodes + S.ext_grp_ptr[q]);
+  }
+}
+
 void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
   const Symbolic& S = *f.host;
   int level = -1;
@@ -1347,10 +1376,7 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
     if (L.level != level) {
       // extend-add of the previous level's nodes into their parents, then the first
       // panel diagonal of this level's nodes
-      if (level >= 0) {
-        const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
-        hipLaunchKernelGGL(k_mf_extend, dim3(32, n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level]);
-      }
+      if (level >= 0) launch_extend_groups(s, f, level);
       level = L.level;
       const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
       hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level], sc);
@@ -1366,6 +1392,7 @@ void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, 
   const Symbolic& S = *f.host;
   const int g = blocks_for(6 * int64_t(f.nb), kThreads);
   hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, rhs_nat, y);
+  launch_fill(s, f.usolve, S.u_elems, 0.0);
   const int nlev = int(S.lvl_ptr.size()) - 1;
   for (int l = 0; l < nlev; ++l) {
     const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
@@ -1374,7 +1401,13 @@ void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, 
     const int nr = S.solve_rows_ptr[l + 1] - S.solve_rows_ptr[l];
     if (nr > 0)
       hipLaunchKernelGGL(k_mf_fwd_rect, dim3(nr), dim3(kThreads), 0, s, f,
-                         f.solve_rows + 2 * int64_t(S.solve_rows_ptr[l]), y, f.ytmp);
+                         f.solve_rows + 2 * int64_t(S.solve_rows_ptr[l]), f.ytmp);
+    for (int q = S.ext_lvl_ptr[l]; q < S.ext_lvl_ptr[l + 1]; ++q) {
+      const int cnt = S.ext_grp_ptr[q + 1] - S.ext_grp_ptr[q];
+      if (cnt > 0)
+        hipLaunchKernelGGL(k_mf_fwd_extend, dim3(8, cnt), dim3(kThreads), 0, s, f,
+                           f.ext_nodes + S.ext_grp_ptr[q]);
+    }
   }
   for (int l = nlev - 1; l >= 0; --l) {
     const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];

@@ -67,6 +67,9 @@ struct DevFactor {
   const int32_t* solve_rows;  // (node, 256-row tile) per level, rows below the node
   const int32_t* solve_cols;  // (node, 64-column tile) per level
   double* ytmp;               // nb x 6 solve scratch
+  const int32_t* ext_nodes;   // extend-add groups (see Symbolic)
+  const int64_t* sn_uoff;     // forward-solve update vector offsets
+  double* usolve;             // forward-solve update vectors (u_elems)
   double* front;   // all fronts, column-major (workspace)
   double* lbuf;    // per node F x K: the factor's L columns (column-major, ld F)
   double* piv_L;   // nb x 36 unit-lower pivot factors (row-major)

@@ -257,6 +257,9 @@ struct sgpu_optimizer {
     df.solve_rows = arena.upload(S.solve_rows, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
     df.ytmp = arena.alloc<double>(size_t(6) * S.nb);
+    df.ext_nodes = arena.upload(S.ext_nodes, stream);
+    df.sn_uoff = arena.upload(S.sn_uoff, stream);
+    df.usolve = arena.alloc<double>(size_t(S.u_elems));
     if (!iterative()) {
       df.front = arena.alloc<double>(size_t(S.front_elems));
       df.lbuf = arena.alloc<double>(size_t(S.l_elems));

@@ -616,6 +616,31 @@ void analyze_rcs(Symbolic& S) {
     S.solve_rows_ptr[l + 1] = int32_t(S.solve_rows.size() / 2);
     S.solve_cols_ptr[l + 1] = int32_t(S.solve_cols.size() / 2);
   }
+  // extend-add groups (children ranked by node index within their parent)
+  {
+    std::vector<int32_t> rank(S.nsn, -1), seen(S.nsn, 0);
+    for (int32_t x = 0; x < S.nsn; ++x)
+      if (S.sn_parent[x] >= 0) rank[x] = seen[S.sn_parent[x]]++;
+    S.ext_lvl_ptr.assign(nlev + 1, 0);
+    S.ext_grp_ptr.assign(1, 0);
+    S.ext_nodes.clear();
+    for (int32_t l = 0; l < nlev; ++l) {
+      int32_t maxr = -1;
+      for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) maxr = std::max(maxr, rank[S.lvl_sn[q]]);
+      for (int32_t r = 0; r <= maxr; ++r) {
+        for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q)
+          if (rank[S.lvl_sn[q]] == r) S.ext_nodes.push_back(S.lvl_sn[q]);
+        S.ext_grp_ptr.push_back(int32_t(S.ext_nodes.size()));
+      }
+      S.ext_lvl_ptr[l + 1] = int32_t(S.ext_grp_ptr.size()) - 1;
+    }
+    S.sn_uoff.assign(S.nsn, 0);
+    S.u_elems = 0;
+    for (int32_t x = 0; x < S.nsn; ++x) {
+      S.sn_uoff[x] = S.u_elems;
+      S.u_elems += 6 * int64_t(S.sn_nf[x]);
+    }
+  }
   // S slot -> front element offset of the block's (0, 0) entry
   std::vector<int32_t> pos_in_front(S.nb, -1);  // position of block j within its own node
   for (int32_t x = 0; x < S.nsn; ++x)

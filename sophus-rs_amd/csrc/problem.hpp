@@ -94,6 +94,13 @@ struct Symbolic {
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
   // solve work items per level: (node, 256-row tile below K), (node, 64-column tile of K)
   std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;
+  // extend-add groups: per level, nodes grouped by their rank among their parent's
+  // children, so one launch never has two children of one parent (plain adds, fixed
+  // order).  ext_lvl_ptr[l] .. ext_lvl_ptr[l+1] index groups; ext_grp_ptr into ext_nodes.
+  std::vector<int32_t> ext_lvl_ptr, ext_grp_ptr, ext_nodes;
+  // forward-solve update vectors: node x owns 6 * sn_nf[x] entries at sn_uoff[x]
+  std::vector<int64_t> sn_uoff;
+  int64_t u_elems = 0;
   // pose-graph terms: per term the S slots of (m,m), (n,n) and (max,min), -1 if absent;
   // mn_flip = 1 when the stored (max,min) block is the transpose of J_m^T J_n
   struct PgDest {

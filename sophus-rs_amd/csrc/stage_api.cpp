@@ -197,6 +197,9 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.solve_rows = bufs.up(S.solve_rows);
     df.solve_cols = bufs.up(S.solve_cols);
     df.ytmp = bufs.alloc<double>(size_t(6) * nb);
+    df.ext_nodes = bufs.up(S.ext_nodes);
+    df.sn_uoff = bufs.up(S.sn_uoff);
+    df.usolve = bufs.alloc<double>(size_t(S.u_elems));
     df.front = bufs.alloc<double>(size_t(S.front_elems));
     df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
     df.piv_L = bufs.alloc<double>(size_t(36) * nb);

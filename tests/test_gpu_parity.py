@@ -295,3 +295,15 @@ def test_ba_walk_pcg_with_pose_graph(sg, orc):
     sol, _, _ = orc.optimize_nlls(orc.ba_families(p), ocosts,
                                   orc.default_params(num_iterations=12, initial_lm_damping=1.0))
     assert abs(merit - sol["final_cost"]) <= 1e-7 * sol["final_cost"]
+
+
+def test_fp64_direct_path_is_bitwise_deterministic(sg):
+    """No atomics on the FP64 direct path's sums: two runs give identical bits."""
+    spec = sg.BaSpec(layout=1, num_cameras=120, num_points=6000, geometric_p=0.15, seed=3)
+    p = sg.generate_ba(spec)
+    params = sg.OptParams(num_iterations=10, initial_lm_damping=1.0)
+    m1, o1, t1 = _run_gpu(sg, p, params)
+    m2, o2, t2 = _run_gpu(sg, p, params)
+    assert m1 == m2
+    assert [t.merit for t in t1] == [t.merit for t in t2]
+    assert np.array_equal(o1["poses"], o2["poses"])
