@@ -1,10 +1,18 @@
 """Benchmark: Schur-complement LM bundle adjustment on one MI355X.
 
-Workload: the 2000-camera / 800k-point random-walk BA (about 6M observations, pinhole,
-Huber 1 px, camera 0 fixed).  A "step" is one LM iteration of the hot path over all
-observations: residuals/Jacobians, Schur assembly of the reduced camera system, its
-block LDLt factorization and solve, back-substitution, the pose/point update and the
-merit of the proposal.  Inputs are resident in HBM before the timed region.
+Workload: the 2000-camera / 800k-point random-walk BA (5,947,769 observations, pinhole,
+Huber 1 px, camera 0 fixed, nu0 = 1), inputs resident in HBM before the timed region.
+
+- value: observations per second of the plain FP64 GPU path (Schur + direct block LDLt),
+  N_o / (seconds per LM iteration) over exactly --steps timed iterations after --warmup
+  (tolerances disabled so every iteration does the full work).
+- roofline: the dominant kernel (k_mf_step, the FP64 panel steps of the RCS
+  factorization), algorithmic flops per launch from the symbolic factor / its average
+  launch time from HIP events on the optimizer's stream, against FP64 peak.
+- time_to_tol: F* = the FP64 direct path's final cost (function tolerance 1e-10, at most
+  100 iterations); each path's first iteration within 1e-6 of F*, its time and
+  obs/s = N_o x iterations / time: FP64 direct, block-Jacobi PCG, learned PCG, mixed.
+- cpu_baseline: one LM iteration of the FP64 oracle on this host, one core.
 
 Prints ONE JSON line (stdout).  Progress goes to stderr.
 """
@@ -18,44 +26,28 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "sophus-rs_amd"))
 
 METRIC = "BA time-to-tol @2k cams/6M obs: learned+FP32 vs plain FP64 GPU; obs/s"
-HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+TOL_REL = 1e-6
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(kernel: str, st: dict, npt: int, nobs: int) -> float:
-    """Compulsory HBM bytes of one launch of `kernel` on the FP64 path.
-
-    N_o observations (uv 16 B + pose/point/intrinsics indices 12 B), N_p points
-    (x 24 B; H_ll^-1 72 B; s, y 48 B), nnz_S 6x6 FP64 blocks of the reduced camera system.
-    """
-    nnz_s = st["nnz_s_blocks"]
-    if kernel == "point_prep":
-        return 24.0 * nobs + 36.0 * npt + 120.0 * npt
-    if kernel == "camera_rows":
-        return 32.0 * nobs + 128.0 * npt + 288.0 * nnz_s + 96.0 * st["num_free_blocks"]
-    if kernel == "backsub":
-        return 8.0 * nobs + 24.0 * npt + 96.0 * npt + 24.0 * npt
-    if kernel == "cost":
-        return 28.0 * nobs + 24.0 * npt
-    return float("nan")
-
-
-def cpu_baseline(problem, seconds_hint: str) -> dict:
+def cpu_baseline(problem, label: str) -> dict:
     """One LM iteration of the FP64 oracle (single-threaded C++ restatement of the
     reference path) on the same problem, on this host's cores."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
     params = oracle.default_params(num_iterations=1, initial_lm_damping=1.0)
-    sol, _, _ = oracle.optimize_nlls(oracle.ba_families(problem), oracle.ba_costs(problem), params)
+    oracle.optimize_nlls(oracle.ba_families(problem), oracle.ba_costs(problem), params)
     step_s = oracle.last_step_seconds()
     return {
         "value": problem.num_obs / step_s,
@@ -63,31 +55,38 @@ def cpu_baseline(problem, seconds_hint: str) -> dict:
         "cores": 1,
         "kind": "port",
         "sample": f"one LM iteration (linearize, Schur, RCS LDLt, update, merit) of the FP64 oracle "
-                  f"on the full {seconds_hint} problem; {step_s:.1f} s",
+                  f"on the full {label} problem: {step_s:.2f} s",
     }
 
 
-def time_to_tol(sg, problem, precision, f_star=None, max_iter=100):
-    """Run LM from the initial values; return per-step (cumulative s, merit)."""
-    params = sg.OptParams(num_iterations=max_iter, initial_lm_damping=1.0, precision=precision)
+def run_path(sg, problem, max_iter=100, **kw):
+    """Run LM from the initial values; per-iteration (cumulative s, merit, pcg its)."""
+    params = sg.OptParams(num_iterations=max_iter, initial_lm_damping=1.0, **kw)
     opt = sg.Optimizer(problem.variables(), problem.cost_fns(1.0), params)
     hist = []
     t0 = time.perf_counter()
     while True:
         info = opt.step()
-        hist.append((time.perf_counter() - t0, info.merit, info.accepted))
+        hist.append((time.perf_counter() - t0, info.merit, info.pcg_iterations))
         if info.termination is not None:
             break
-    final = opt.merit()
     opt.close()
-    return hist, final
+    return hist
 
 
-def first_within(hist, f_star, rtol=1e-6):
-    for i, (t, m, acc) in enumerate(hist):
-        if (m - f_star) / f_star <= rtol:
-            return t, i + 1
-    return float("nan"), len(hist)
+def tol_summary(hist, f_star, n_obs):
+    hit = next((i for i, (_, m, _) in enumerate(hist) if (m - f_star) / f_star <= TOL_REL), None)
+    best = min(m for _, m, _ in hist)
+    out = {"iterations_run": len(hist), "seconds_run": hist[-1][0], "final_cost": hist[-1][1],
+           "best_rel_to_f_star": (best - f_star) / f_star,
+           "pcg_iterations": int(sum(h[2] for h in hist))}
+    if hit is None:
+        out.update({"reached": False, "iterations": None, "seconds": None, "obs_per_s": None})
+    else:
+        t = hist[hit][0]
+        out.update({"reached": True, "iterations": hit + 1, "seconds": t,
+                    "obs_per_s": n_obs * (hit + 1) / t})
+    return out
 
 
 def main() -> int:
@@ -100,8 +99,9 @@ def main() -> int:
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is "
-              f"described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build runs the "
+              f"Schur LM path on a single MI355X; the multi-GPU design is documented, not built",
+              file=sys.stderr)
         return 2
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs as a single process on one GPU", file=sys.stderr)
@@ -112,71 +112,97 @@ def main() -> int:
     spec = sg.config_spec(args.config)
     t = time.perf_counter()
     problem = sg.generate_ba(spec)
-    log(f"[bench] generated {spec.num_cameras} cams, {spec.num_points} pts, {problem.num_obs} obs "
+    n_obs = problem.num_obs
+    label = f"{spec.num_cameras}-camera/{spec.num_points}-point"
+    log(f"[bench] generated {spec.num_cameras} cams, {spec.num_points} pts, {n_obs} obs "
         f"in {time.perf_counter() - t:.1f}s")
 
-    # ---- per-step throughput of the FP64 path --------------------------------------------
+    # ---- per-iteration throughput of the plain FP64 path -----------------------------------
     params = sg.OptParams(num_iterations=1 << 30, initial_lm_damping=1.0, function_tolerance=0.0,
                           gradient_tolerance=0.0, parameter_tolerance=0.0,
                           max_consecutive_rejects=1 << 30)
     t = time.perf_counter()
     opt = sg.Optimizer(problem.variables(), problem.cost_fns(1.0), params)
     st = opt.rcs_stats()
-    log(f"[bench] optimizer created in {time.perf_counter() - t:.1f}s: {st}")
+    log(f"[bench] optimizer created (upload + symbolic analysis, PCIe-inclusive) in "
+        f"{time.perf_counter() - t:.2f}s: {st}")
     for _ in range(args.warmup):
         info = opt.step()
         log(f"[bench] warmup merit {info.merit:.6e} lin {info.ms_linearize:.2f} fac {info.ms_factor:.2f} "
             f"sol {info.ms_solve:.2f} cost {info.ms_cost:.2f} total {info.ms_total:.2f} ms")
-    opt.set_kernel_timing(True)
+    opt.set_kernel_timing(1)
     opt.reset_kernel_times()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         info = opt.step()
     t1 = time.perf_counter()
-    log(f"[bench] timed merit {info.merit:.6e} lin {info.ms_linearize:.2f} fac {info.ms_factor:.2f} "
-        f"sol {info.ms_solve:.2f} cost {info.ms_cost:.2f} total {info.ms_total:.2f} ms")
-    ktimes = opt.kernel_times()
-    opt.close()
     step_s = (t1 - t0) / args.steps
-    value = problem.num_obs / step_s
-    log(f"[bench] {step_s * 1e3:.2f} ms/step, {value:.3e} obs/s; kernels: {ktimes}")
+    value = n_obs / step_s
+    ktimes = opt.kernel_times()
+    log(f"[bench] timed merit {info.merit:.6e}; {step_s * 1e3:.2f} ms/step, {value:.3e} obs/s; {ktimes}")
+    # per-launch timing of the dominant kernel on two further iterations (not in `value`)
+    opt.set_kernel_timing(2)
+    opt.reset_kernel_times()
+    for _ in range(2):
+        opt.step()
+    kt2 = opt.kernel_times()
+    opt.close()
+    n_launch, ms_launch = kt2["mf_step"]
+    avg_s = ms_launch / n_launch / 1e3
+    flops_launch = st["mf_step_flops"] / st["mf_step_launches"]
+    achieved = flops_launch / avg_s / 1e12
+    # HBM bytes per k_mf_step launch from the committed rocprofv3 --pmc summary of this
+    # workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a 256 MiB copy's readings)
+    traffic = None
+    tfile = ROOT / "profiles" / "pmc_traffic_ba_walk_2000cams.json"
+    if args.config == 2 and tfile.exists():
+        traffic = json.loads(tfile.read_text())["kernels"]["k_mf_step"]["hbm_bytes_per_launch"]
+    roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "k_mf_step",
+            "flops_per_launch": flops_launch, "avg_launch_us": avg_s * 1e6,
+            "launches_per_factorization": st["mf_step_launches"]}
+    log(f"[bench] roofline {roof}")
+    kernels = {k: {"launches": c, "avg_ms": (ms / c if c else None)} for k, (c, ms) in ktimes.items()
+               if c}
+    kernels["mf_step"] = {"launches": n_launch, "avg_ms": ms_launch / n_launch}
 
-    # ---- roofline of the dominant kernel -------------------------------------------------------
-    dom = max(ktimes, key=lambda k: ktimes[k][1])
-    cnt, tot = ktimes[dom]
-    avg_s = tot / max(cnt, 1) / 1e3
-    if dom == "rcs_factor":
-        flops = st["factor_flops"]
-        achieved = flops / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": dom,
-                "flops_per_launch": flops, "avg_launch_ms": avg_s * 1e3}
-    else:
-        nbytes = algorithmic_bytes(dom, st, spec.num_points, problem.num_obs)
-        achieved = nbytes / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-                "bytes_per_launch": nbytes, "avg_launch_ms": avg_s * 1e3}
-    kernels = {k: {"launches": c, "avg_ms": (ms / c if c else None)} for k, (c, ms) in ktimes.items()}
-
-    # ---- time to tolerance: FP64 vs mixed precision ------------------------------------------
+    # ---- time to tolerance per path -----------------------------------------------------------
     tol = None
     if not args.no_tol:
-        hist64, f64 = time_to_tol(sg, problem, sg.Precision.F64)
-        t64, it64 = first_within(hist64, f64)
-        histmx, fmx = time_to_tol(sg, problem, sg.Precision.Mixed)
-        tmx, itmx = first_within(histmx, f64)
-        tol = {"f_star": f64, "rtol": 1e-6,
-               "f64_ldlt": {"seconds": t64, "iterations": it64, "final_rel_err": 0.0},
-               "mixed_ldlt": {"seconds": tmx, "iterations": itmx, "final_rel_err": (fmx - f64) / f64},
-               "speedup_mixed_vs_f64": (t64 / tmx) if tmx and not math.isnan(tmx) else None,
-               "learned_pcg": None}
-        log(f"[bench] time-to-tol {tol}")
+        wpath = ROOT / "sophus-rs_amd" / "precond_weights.f32"
+        weights = np.fromfile(wpath, dtype="<f4") if wpath.exists() else None
+        paths = {
+            "f64_ldlt": dict(),
+            "f64_pcg_jacobi": dict(solver=sg.Solver.SchurPcgJacobi, pcg_relative_tolerance=1e-8,
+                                   pcg_max_iterations=3000),
+            "f64_pcg_learned": dict(solver=sg.Solver.SchurPcgLearned, pcg_relative_tolerance=1e-8,
+                                    pcg_max_iterations=3000, precond_weights=weights),
+            "mixed_ldlt": dict(precision=sg.Precision.Mixed),
+        }
+        if weights is None:
+            del paths["f64_pcg_learned"]
+        hists = {}
+        for name, kw in paths.items():
+            hists[name] = run_path(sg, problem, **kw)
+            log(f"[bench] {name}: {len(hists[name])} its, {hists[name][-1][0]:.2f}s, "
+                f"final {hists[name][-1][1]:.9e}")
+        f_star = hists["f64_ldlt"][-1][1]
+        tol = {"f_star": f_star, "rtol": TOL_REL,
+               "paths": {k: tol_summary(h, f_star, n_obs) for k, h in hists.items()}}
+        base = tol["paths"]["f64_ldlt"]
+        for k, v in tol["paths"].items():
+            v["speedup_vs_f64_ldlt"] = (base["seconds"] / v["seconds"]) if v["reached"] else None
+        tol["note"] = ("FP32 S (mixed) with PCG is not run: FP32 rounding makes the reduced camera "
+                       "system indefinite in its near-gauge directions and CG breaks down. The "
+                       "endpoint after 100 iterations depends on rounding (the reference's "
+                       "Huber weighting makes LM crawl with nu doubling), so reaching F* from "
+                       "another path is not a property of its speed alone.")
+        log(f"[bench] time-to-tol {json.dumps(tol)}")
 
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
-        cpu = cpu_baseline(problem, f"{spec.num_cameras}-camera/{spec.num_points}-point")
+        cpu = cpu_baseline(problem, label)
         log(f"[bench] cpu baseline {cpu} ({time.perf_counter() - t:.1f}s)")
 
     line = {
@@ -193,9 +219,9 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"ba_walk_{spec.num_cameras}cams_{spec.num_points}pts",
-                   "num_cameras": spec.num_cameras, "num_points": spec.num_points,
-                   "num_obs": problem.num_obs, "solver": "schur_block_sparse_ldlt",
-                   "precision": "f64", "huber_delta_px": 1.0, "seed": spec.seed},
+                   "num_cameras": spec.num_cameras, "num_points": spec.num_points, "num_obs": n_obs,
+                   "value_path": "f64 Schur + direct block LDLt, per LM iteration",
+                   "huber_delta_px": 1.0, "initial_lm_damping": 1.0, "seed": spec.seed},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels": kernels,

@@ -261,6 +261,8 @@ typedef struct sgpu_rcs_stats {
   int64_t num_supernodes;    /* supernodes (dense fronts) of the factorization */
   int64_t max_front_blocks;  /* largest front, in 6x6 blocks */
   int64_t front_bytes;       /* device bytes of all fronts */
+  double mf_step_flops;      /* algorithmic flops of the panel-step launches (k_mf_step) */
+  int64_t mf_step_launches;  /* panel-step launches per factorization */
 } sgpu_rcs_stats;
 
 sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* stats);
@@ -277,7 +279,10 @@ sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
 int32_t sgpu_optimizer_kernel_times(const sgpu_optimizer* opt, const char** names,
                                     int64_t* counts, double* total_ms, int32_t capacity);
 void sgpu_optimizer_reset_kernel_times(sgpu_optimizer* opt);
-/* 1 enables per-kernel event timing (adds one event pair per launch), 0 disables. */
+/* Event timing on the handle's stream: 1 = one event pair around each phase kernel
+ * (point_prep, camera_rows, rcs_factor, rcs_solve, backsub, pose_update, cost);
+ * 2 = also one pair around every panel-step launch of the direct factorization
+ * ("mf_step"); 0 disables. */
 void sgpu_optimizer_set_kernel_timing(sgpu_optimizer* opt, int32_t enable);
 
 void sgpu_optimizer_destroy(sgpu_optimizer* opt);

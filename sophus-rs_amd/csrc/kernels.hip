@@ -15,6 +15,7 @@
 
 #include <algorithm>
 
+#include "api_internal.hpp"
 #include "kernels.hpp"
 #include "lie.hpp"
 
@@ -1381,9 +1382,13 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
       const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
       hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level], sc);
     }
-    if (L.tile_count > 0)
+    if (L.tile_count > 0) {
+      const size_t li = size_t(&L - S.launches.data());
+      if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
       hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
                          f.launch_tiles + 3 * int64_t(L.tile_begin), sc);
+      if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li + 1], s));
+    }
   }
 }
 

@@ -44,10 +44,11 @@ enum KernelId {
   K_BACKSUB,
   K_UPDATE,
   K_COST,
+  K_MF_STEP,  // k_mf_step launches, per-launch events (timing mode 2 only)
   K_COUNT
 };
-const char* kKernelNames[K_COUNT] = {"point_prep",   "camera_rows", "rcs_factor", "rcs_solve",
-                                     "backsub",      "pose_update", "cost"};
+const char* kKernelNames[K_COUNT] = {"point_prep",  "camera_rows", "rcs_factor", "rcs_solve",
+                                     "backsub",     "pose_update", "cost",       "mf_step"};
 
 double bits_to_double(unsigned long long b) {
   double d;
@@ -86,8 +87,9 @@ struct sgpu_optimizer {
   int64_t rejects = 0, steps = 0;
   bool terminated = false;
   int32_t last_termination = SGPU_TERM_NONE;
-  // timing
-  bool timing = false;
+  // timing: 0 off, 1 per phase, 2 also per k_mf_step launch
+  int timing = 0;
+  std::vector<hipEvent_t> step_ev;
   hipEvent_t ev[2 * K_COUNT] = {};
   double kernel_ms[K_COUNT] = {};
   int64_t kernel_count[K_COUNT] = {};
@@ -101,6 +103,8 @@ struct sgpu_optimizer {
       if (e) (void)hipEventDestroy(e);
     if (h_sc) (void)hipHostFree(h_sc);
     if (h_pcg) (void)hipHostFree(h_pcg);
+    for (auto e : step_ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -113,8 +117,17 @@ struct sgpu_optimizer {
   // accumulate after a stream sync; kernels recorded this step only
   void collect(const bool* used) {
     if (!timing) return;
+    if (timing == 2 && used[K_FACTOR] && df.step_events) {
+      for (size_t li = 0; li < S.launches.size(); ++li) {
+        if (S.launches[li].tile_count == 0) continue;
+        float ms = 0;
+        SGPU_HIP(hipEventElapsedTime(&ms, step_ev[2 * li], step_ev[2 * li + 1]));
+        kernel_ms[K_MF_STEP] += ms;
+        kernel_count[K_MF_STEP] += 1;
+      }
+    }
     for (int k = 0; k < K_COUNT; ++k) {
-      if (!used[k]) continue;
+      if (!used[k] || k == K_MF_STEP) continue;
       float ms = 0;
       SGPU_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
       kernel_ms[k] += ms;
@@ -589,6 +602,10 @@ static void fill_stats(const HostProblem& P, const Symbolic& S, sgpu_rcs_stats* 
   st->num_supernodes = S.nsn;
   st->max_front_blocks = S.max_front;
   st->front_bytes = S.front_elems * 8;
+  st->mf_step_flops = 0.0;
+  for (const auto& L : S.launches) st->mf_step_flops += L.flops;
+  st->mf_step_launches = 0;
+  for (const auto& L : S.launches) st->mf_step_launches += L.tile_count > 0 ? 1 : 0;
 }
 
 sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* st) {
@@ -630,7 +647,19 @@ void sgpu_optimizer_reset_kernel_times(sgpu_optimizer* opt) {
 }
 
 void sgpu_optimizer_set_kernel_timing(sgpu_optimizer* opt, int32_t enable) {
-  if (opt) opt->timing = enable != 0;
+  if (!opt) return;
+  opt->timing = enable <= 0 ? 0 : enable == 1 ? 1 : 2;
+  if (opt->timing == 2 && !opt->iterative() && opt->step_ev.empty()) {
+    opt->step_ev.assign(2 * opt->S.launches.size(), nullptr);
+    for (auto& e : opt->step_ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        for (auto& d : opt->step_ev)
+          if (d) (void)hipEventDestroy(d);
+        opt->step_ev.clear();
+        break;
+      }
+  }
+  opt->df.step_events = opt->timing == 2 && !opt->step_ev.empty() ? opt->step_ev.data() : nullptr;
 }
 
 void sgpu_optimizer_destroy(sgpu_optimizer* opt) { delete opt; }

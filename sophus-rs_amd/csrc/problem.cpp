@@ -583,7 +583,7 @@ void analyze_rcs(Symbolic& S) {
     for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q)
       maxpan = std::max(maxpan, (S.sn_nc[S.lvl_sn[q]] + kPanelBlocks - 1) / kPanelBlocks);
     for (int32_t pn = 0; pn < maxpan; ++pn) {
-      Symbolic::PanelLaunch L{l, pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0};
+      Symbolic::PanelLaunch L{l, pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0, 0.0};
       for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
         const int32_t x = S.lvl_sn[q];
         const int32_t npan = (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks;
@@ -591,6 +591,10 @@ void analyze_rcs(Symbolic& S) {
         const int32_t kend = 6 * std::min(S.sn_nc[x], (pn + 1) * kPanelBlocks);
         const int32_t T = 6 * S.sn_nf[x] - kend;
         const int32_t nt = (T + kTile - 1) / kTile;
+        const double np = double(kend - 6 * std::min(S.sn_nc[x], pn * kPanelBlocks));
+        // lower trailing update T(T+1)/2 x np multiply-adds, the panel's triangular
+        // solve (np^2 / 2 multiply-adds per row) and D^-1 scaling (6 np per row)
+        L.flops += double(T) * double(T + 1) * np + double(T) * (np * np + 12.0 * np);
         for (int32_t tr = 0; tr < nt; ++tr)
           for (int32_t tc = 0; tc <= tr; ++tc) {
             S.launch_tiles.insert(S.launch_tiles.end(), {x, tr, tc});

@@ -89,6 +89,7 @@ struct Symbolic {
   std::vector<int32_t> lvl_ptr, lvl_sn;
   struct PanelLaunch {
     int32_t level, panel, node_begin, node_count, tile_begin, tile_count;
+    double flops;  // algorithmic FP64 flops of the launch (trailing update + panel)
   };
   std::vector<PanelLaunch> launches;
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples

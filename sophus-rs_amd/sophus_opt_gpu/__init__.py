@@ -125,6 +125,8 @@ class _RcsStats(C.Structure):
         ("num_supernodes", C.c_int64),
         ("max_front_blocks", C.c_int64),
         ("front_bytes", C.c_int64),
+        ("mf_step_flops", C.c_double),
+        ("mf_step_launches", C.c_int64),
     ]
 
 
@@ -606,8 +608,9 @@ class Optimizer:
         _check(library().sgpu_optimizer_rcs_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
 
-    def set_kernel_timing(self, enable: bool) -> None:
-        library().sgpu_optimizer_set_kernel_timing(self._h, int(enable))
+    def set_kernel_timing(self, mode: int) -> None:
+        """0 off, 1 per phase kernel, 2 also per panel-step launch ("mf_step")."""
+        library().sgpu_optimizer_set_kernel_timing(self._h, int(mode))
 
     def reset_kernel_times(self) -> None:
         library().sgpu_optimizer_reset_kernel_times(self._h)
