@@ -383,6 +383,50 @@ typedef struct sgpu_precond_train_params {
 sgpu_status sgpu_precond_train(const sgpu_precond_train_params* params, float* weights_inout,
                                double* loss_history, int64_t* num_snapshots);
 
+/* ---- batched small windows (no reference counterpart) --------------------------
+ * Many independent bundle-adjustment windows solved together: per window one shared
+ * pinhole (Conditioned), SE(3) poses (Free, camera 0 of the window constant), points
+ * (Marginalized) and pinhole reprojection terms (Huber if huber_delta > 0).  Each
+ * window runs the same LM as sgpu_optimize_nlls (same rules, tolerances and
+ * termination) with its reduced camera system solved densely in LDS; at most 31
+ * cameras per window.  Arrays are concatenated over windows with prefix offsets. */
+typedef struct sgpu_batch sgpu_batch;
+typedef struct sgpu_batch_desc {
+  int64_t num_problems;
+  const int64_t* cam_ptr;    /* num_problems + 1 */
+  const int64_t* point_ptr;  /* num_problems + 1 */
+  const int64_t* obs_ptr;    /* num_problems + 1 */
+  const double* intrinsics;  /* num_problems x 4 (fx, fy, cx, cy) */
+  const double* poses;       /* cameras x 7, world_from_camera */
+  const double* points;      /* points x 3 */
+  const int32_t* obs_cam;    /* per observation: camera index within its window */
+  const int32_t* obs_point;  /* per observation: point index within its window */
+  const double* obs_uv;      /* observations x 2 */
+  double huber_delta;        /* <= 0: no robust kernel */
+} sgpu_batch_desc;
+
+/* Uploads the windows and picks the workgroup size (128, 256 or 512 threads) with a
+ * timed LM iteration of the whole batch per candidate (the autotuner).  params: the
+ * LM parameters of every window; solver must be SCHUR_BLOCK_SPARSE_LDLT; precision
+ * MIXED computes Jacobians and Schur products in FP32. */
+sgpu_status sgpu_batch_create(const sgpu_batch_desc* desc, const sgpu_opt_params* params,
+                              sgpu_batch** out);
+/* Advances every unfinished window by up to `iterations` LM iterations (one launch
+ * per iteration; use_graph = 1 replays them from a captured hipGraph). */
+sgpu_status sgpu_batch_run(sgpu_batch* batch, int32_t iterations, int32_t use_graph,
+                           double* seconds);
+/* Restarts every window from its initial values. */
+sgpu_status sgpu_batch_reset(sgpu_batch* batch);
+/* Final variables (same layout as the description), costs, steps and termination
+ * (sgpu_termination, NONE while running) per window; any pointer may be NULL. */
+sgpu_status sgpu_batch_results(sgpu_batch* batch, double* poses_out, double* points_out,
+                               double* initial_cost, double* final_cost, int32_t* num_steps,
+                               int32_t* termination);
+/* The autotuner's choice, its three timings (ms) and the LDS bytes per workgroup. */
+sgpu_status sgpu_batch_info(const sgpu_batch* batch, int32_t* threads, double* tune_ms3,
+                            int64_t* lds_bytes);
+void sgpu_batch_destroy(sgpu_batch* batch);
+
 /* ---- synthetic problems ---------------------------------------------------------
  * Seeded BA generator (SplitMix64 -> xoshiro256**, Box-Muller), the input of the
  * parity tests and the benchmark.  Two-call protocol: call with NULL arrays to get
@@ -391,8 +435,9 @@ typedef struct sgpu_ba_spec {
   int32_t layout;       /* 0 = ring of cameras (cube of points), 1 = random-walk */
   int32_t num_cameras;
   int64_t num_points;
-  int32_t obs_per_point_ring; /* layout 0: distinct cameras per point */
-  int32_t min_obs, max_obs;   /* layout 1: clip of the geometric draw */
+  int32_t obs_per_point_ring; /* layout 0: distinct cameras per point; 0 = uniform in
+                                 [min_obs, max_obs] per point */
+  int32_t min_obs, max_obs;   /* layout 1: clip of the geometric draw (layout 0: range) */
   double geometric_p;         /* layout 1: success probability */
   double pixel_noise;         /* px sigma */
   double rot_noise;           /* rad sigma of the pose perturbation */

@@ -1,0 +1,41 @@
+// 6x6 block LDLt with the reference's PSD pivot rule, shared by the RCS factorizations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sgpu {
+
+// ldlt_decompose_inplace (dense_ldlt.rs:164-256) on a 6x6 block with the PSD pivot rule.
+__device__ inline bool ldlt6(const double* A, double* L, double* d) {
+  double a[36];
+  for (int k = 0; k < 36; ++k) a[k] = A[k];
+  double max_abs_pivot = 1.0;
+  const double tol = 1e-12;
+  for (int j = 0; j < 6; ++j) {
+    for (int k = 0; k < j; ++k) {
+      double s = a[6 * j + k];
+      for (int q = 0; q < k; ++q) s -= a[6 * j + q] * d[q] * a[6 * k + q];
+      const double zt = fmax(max_abs_pivot, 1.0) * tol;
+      a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s / d[k];
+    }
+    double djj = a[6 * j + j];
+    for (int q = 0; q < j; ++q) {
+      const double l = a[6 * j + q];
+      djj -= d[q] * l * l;
+    }
+    if (!isfinite(djj)) return false;
+    const double zt = fmax(max_abs_pivot, 1.0) * tol;
+    if (fabs(djj) <= zt) {
+      d[j] = 0.0;
+    } else {
+      d[j] = djj;
+      max_abs_pivot = fmax(max_abs_pivot, fabs(djj));
+    }
+    a[6 * j + j] = 1.0;
+  }
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) L[6 * r + c] = c > r ? 0.0 : a[6 * r + c];
+  return true;
+}
+
+}  // namespace sgpu

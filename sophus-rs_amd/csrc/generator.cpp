@@ -189,9 +189,11 @@ void gen_ring(const sgpu_ba_spec& s, Problem& p) {
   }
   p.points_gt.resize(3 * npt);
   for (auto& v : p.points_gt) v = rng.uniform(-0.5, 0.5);
-  const int k = s.obs_per_point_ring;
   std::vector<int64_t> cams(ncam);
   for (int64_t b = 0; b < npt; ++b) {
+    // a fixed count, or (obs_per_point_ring == 0) a uniform draw in [min_obs, max_obs]
+    int k = s.obs_per_point_ring;
+    if (k == 0) k = int(std::min<int64_t>(ncam, s.min_obs + rng.below(s.max_obs - s.min_obs + 1)));
     for (int64_t c = 0; c < ncam; ++c) cams[c] = c;
     for (int j = 0; j < k; ++j) {  // partial Fisher-Yates: k distinct cameras
       int64_t r = j + rng.below(ncam - j);
@@ -290,8 +292,11 @@ extern "C" sgpu_status sgpu_generate_ba(const sgpu_ba_spec* spec, sgpu_ba_sizes*
   if (spec->num_cameras < 2 || spec->num_points < 1)
     return set_error(SGPU_ERR_INVALID_ARGUMENT, "need >= 2 cameras and >= 1 point");
   if (spec->layout == 0 &&
-      (spec->obs_per_point_ring < 1 || spec->obs_per_point_ring > spec->num_cameras))
+      (spec->obs_per_point_ring < 0 || spec->obs_per_point_ring > spec->num_cameras))
     return set_error(SGPU_ERR_INVALID_ARGUMENT, "obs_per_point_ring out of range");
+  if (spec->layout == 0 && spec->obs_per_point_ring == 0 &&
+      (spec->min_obs < 1 || spec->max_obs < spec->min_obs))
+    return set_error(SGPU_ERR_INVALID_ARGUMENT, "bad per-point observation range");
   if (spec->layout == 1 && (spec->geometric_p <= 0 || spec->geometric_p >= 1 ||
                             spec->min_obs < 1 || spec->max_obs < spec->min_obs))
     return set_error(SGPU_ERR_INVALID_ARGUMENT, "bad geometric observation spec");

@@ -16,6 +16,7 @@
 #include <algorithm>
 
 #include "api_internal.hpp"
+#include "block6.hpp"
 #include "kernels.hpp"
 #include "lie.hpp"
 
@@ -417,39 +418,6 @@ __global__ void k_zero_scalars(DevScalars* sc) {
   }
 }
 
-// ---- RCS block LDLt: one workgroup per column of an etree level ---------------------------
-// ldlt_decompose_inplace (dense_ldlt.rs:164-256) on a 6x6 block with the PSD pivot rule.
-__device__ bool ldlt6(const double* A, double* L, double* d) {
-  double a[36];
-  for (int k = 0; k < 36; ++k) a[k] = A[k];
-  double max_abs_pivot = 1.0;
-  const double tol = 1e-12;
-  for (int j = 0; j < 6; ++j) {
-    for (int k = 0; k < j; ++k) {
-      double s = a[6 * j + k];
-      for (int q = 0; q < k; ++q) s -= a[6 * j + q] * d[q] * a[6 * k + q];
-      const double zt = fmax(max_abs_pivot, 1.0) * tol;
-      a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s / d[k];
-    }
-    double djj = a[6 * j + j];
-    for (int q = 0; q < j; ++q) {
-      const double l = a[6 * j + q];
-      djj -= d[q] * l * l;
-    }
-    if (!isfinite(djj)) return false;
-    const double zt = fmax(max_abs_pivot, 1.0) * tol;
-    if (fabs(djj) <= zt) {
-      d[j] = 0.0;
-    } else {
-      d[j] = djj;
-      max_abs_pivot = fmax(max_abs_pivot, fabs(djj));
-    }
-    a[6 * j + j] = 1.0;
-  }
-  for (int r = 0; r < 6; ++r)
-    for (int c = 0; c < 6; ++c) L[6 * r + c] = c > r ? 0.0 : a[6 * r + c];
-  return true;
-}
 
 // ---- supernodal multifrontal LDLt ---------------------------------------------------------
 // Fronts are dense column-major F x F (F = 6 * front rows); the first K = 6 * node columns
@@ -1365,7 +1333,7 @@ static void launch_extend_groups(hipStream_t s, const DevFactor& f, int level) {
   for (int q = S.ext_lvl_ptr[level]; q < S.ext_lvl_ptr[level + 1]; ++q) {
     const int cnt = S.ext_grp_ptr[q + 1] - S.ext_grp_ptr[q];
     if (cnt > 0)
-      hipLaunchKernelGGL(k_mf_extend, dim3(32, cnt), dim3(kThreads), 0, s, f,
+      hipLaunchKernelGGL(k_mf_extend, dim3(256, cnt), dim3(kThreads), 0, s, f,
                          f.ext_nodes + S.ext_grp_ptr[q]);
   }
 }

@@ -153,6 +153,15 @@ class _BaSizes(C.Structure):
     _fields_ = [("num_cameras", C.c_int64), ("num_points", C.c_int64), ("num_obs", C.c_int64)]
 
 
+class _BatchDesc(C.Structure):
+    _fields_ = [("num_problems", C.c_int64), ("cam_ptr", C.POINTER(C.c_int64)),
+                ("point_ptr", C.POINTER(C.c_int64)), ("obs_ptr", C.POINTER(C.c_int64)),
+                ("intrinsics", C.POINTER(C.c_double)), ("poses", C.POINTER(C.c_double)),
+                ("points", C.POINTER(C.c_double)), ("obs_cam", C.POINTER(C.c_int32)),
+                ("obs_point", C.POINTER(C.c_int32)), ("obs_uv", C.POINTER(C.c_double)),
+                ("huber_delta", C.c_double)]
+
+
 class _PrecondTrainParams(C.Structure):
     _fields_ = [("num_problems", C.c_int32), ("num_cameras", C.c_int32), ("num_points", C.c_int64),
                 ("geometric_p", C.c_double), ("lm_iterations", C.c_int32), ("epochs", C.c_int32),
@@ -203,6 +212,13 @@ _SIGNATURES = {
     "sgpu_precond_blocks": (C.c_int, [C.c_int64, _DP, _FP, _DP]),
     "sgpu_precond_loss_grad": (C.c_int, [C.c_int64, _FP, _DP, _DP, _DP, _FP, _DP, _FP]),
     "sgpu_precond_train": (C.c_int, [C.POINTER(_PrecondTrainParams), _FP, _DP, _I64P]),
+    "sgpu_batch_create": (C.c_int, [C.POINTER(_BatchDesc), C.POINTER(_OptParams), C.POINTER(C.c_void_p)]),
+    "sgpu_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
+    "sgpu_batch_reset": (C.c_int, [C.c_void_p]),
+    "sgpu_batch_results": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP, C.POINTER(C.c_int32),
+                                     C.POINTER(C.c_int32)]),
+    "sgpu_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _DP, _I64P]),
+    "sgpu_batch_destroy": (None, [C.c_void_p]),
     "sgpu_generate_ba": (C.c_int, [C.POINTER(_BaSpec), C.POINTER(_BaSizes), _DP, _DP, _DP, _DP, _DP,
                                    _I64P, _DP]),
 }
@@ -830,6 +846,109 @@ def train_preconditioner(params: PrecondTrainParams, weights: Optional[np.ndarra
     _check(library().sgpu_precond_train(C.byref(params._c()), w.ctypes.data_as(_FP), hist.ctypes.data_as(_DP),
                                         C.byref(n)))
     return w, hist, n.value
+
+
+# ---- batched small windows -----------------------------------------------------------------
+
+
+class Batch:
+    """Many independent BA windows solved together on the GPU (sgpu_batch_*).
+
+    ``windows``: BaProblem objects (one shared pinhole each, camera 0 constant, Huber).
+    """
+
+    def __init__(self, windows: Sequence["BaProblem"], params: Optional[OptParams] = None,
+                 huber_delta: Optional[float] = 1.0):
+        params = params or OptParams()
+        self.windows = list(windows)
+        nc = np.array([w.poses_init.shape[0] for w in self.windows], dtype=np.int64)
+        npt = np.array([w.points_init.shape[0] for w in self.windows], dtype=np.int64)
+        no = np.array([w.num_obs for w in self.windows], dtype=np.int64)
+        self.cam_ptr = np.concatenate([[0], np.cumsum(nc)]).astype(np.int64)
+        self.point_ptr = np.concatenate([[0], np.cumsum(npt)]).astype(np.int64)
+        self.obs_ptr = np.concatenate([[0], np.cumsum(no)]).astype(np.int64)
+        self.intr = np.ascontiguousarray(np.concatenate([w.intrinsics.reshape(-1) for w in self.windows]))
+        self.poses = np.ascontiguousarray(np.concatenate([w.poses_init for w in self.windows]))
+        self.points = np.ascontiguousarray(np.concatenate([w.points_init for w in self.windows]))
+        self.obs_cam = np.ascontiguousarray(np.concatenate([w.obs_idx[:, 1] for w in self.windows]),
+                                            dtype=np.int32)
+        self.obs_point = np.ascontiguousarray(np.concatenate([w.obs_idx[:, 2] for w in self.windows]),
+                                              dtype=np.int32)
+        self.obs_uv = np.ascontiguousarray(np.concatenate([w.obs_uv for w in self.windows]))
+        d = _BatchDesc()
+        d.num_problems = len(self.windows)
+        d.cam_ptr = self.cam_ptr.ctypes.data_as(_I64P)
+        d.point_ptr = self.point_ptr.ctypes.data_as(_I64P)
+        d.obs_ptr = self.obs_ptr.ctypes.data_as(_I64P)
+        d.intrinsics = self.intr.ctypes.data_as(_DP)
+        d.poses = self.poses.ctypes.data_as(_DP)
+        d.points = self.points.ctypes.data_as(_DP)
+        d.obs_cam = self.obs_cam.ctypes.data_as(C.POINTER(C.c_int32))
+        d.obs_point = self.obs_point.ctypes.data_as(C.POINTER(C.c_int32))
+        d.obs_uv = self.obs_uv.ctypes.data_as(_DP)
+        d.huber_delta = -1.0 if huber_delta is None else float(huber_delta)
+        h = C.c_void_p()
+        _check(library().sgpu_batch_create(C.byref(d), C.byref(params._c()), C.byref(h)))
+        self._h = h
+
+    @property
+    def num_problems(self) -> int:
+        return len(self.windows)
+
+    def run(self, iterations: int, graph: bool = True) -> float:
+        secs = C.c_double()
+        _check(library().sgpu_batch_run(self._h, int(iterations), int(bool(graph)), C.byref(secs)))
+        return secs.value
+
+    def reset(self) -> None:
+        _check(library().sgpu_batch_reset(self._h))
+
+    def results(self) -> dict:
+        P = self.num_problems
+        poses = np.empty_like(self.poses)
+        points = np.empty_like(self.points)
+        c0, c1 = np.empty(P), np.empty(P)
+        steps = np.empty(P, dtype=np.int32)
+        term = np.empty(P, dtype=np.int32)
+        _check(library().sgpu_batch_results(self._h, poses.ctypes.data_as(_DP), points.ctypes.data_as(_DP),
+                                            c0.ctypes.data_as(_DP), c1.ctypes.data_as(_DP),
+                                            steps.ctypes.data_as(C.POINTER(C.c_int32)),
+                                            term.ctypes.data_as(C.POINTER(C.c_int32))))
+        return {"poses": poses, "points": points, "initial_cost": c0, "final_cost": c1, "steps": steps,
+                "termination": term}
+
+    def info(self) -> dict:
+        t = C.c_int32()
+        ms = (C.c_double * 3)()
+        lds = C.c_int64()
+        _check(library().sgpu_batch_info(self._h, C.byref(t), ms, C.byref(lds)))
+        return {"threads": t.value, "tune_ms": {128: ms[0], 256: ms[1], 512: ms[2]}, "lds_bytes": lds.value}
+
+    def window_slice(self, p: int) -> dict:
+        return {"poses": slice(self.cam_ptr[p], self.cam_ptr[p + 1]),
+                "points": slice(self.point_ptr[p], self.point_ptr[p + 1])}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            library().sgpu_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def window_spec(index: int, seed_base: int = 0) -> "BaSpec":
+    """The batched workload's window `index`: the ring geometry with N_c ~ U{5..30}
+    cameras, N_p ~ U{100..2000} points and U{2..8} distinct cameras per point, drawn
+    from a generator seeded by the window id."""
+    rng = np.random.default_rng(seed_base * 1_000_003 + index)
+    nc = int(rng.integers(5, 31))
+    npt = int(rng.integers(100, 2001))
+    return BaSpec(layout=0, num_cameras=nc, num_points=npt, obs_per_point_ring=0, min_obs=2, max_obs=8,
+                  seed=seed_base * 1_000_003 + index)
 
 
 # ---- synthetic problems ------------------------------------------------------------------------

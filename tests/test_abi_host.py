@@ -7,6 +7,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 HEADER = Path(__file__).resolve().parents[1] / "include" / "sophus_gpu.h"
 
@@ -115,3 +116,17 @@ def test_symbolic_analysis_counts(sg):
     # L's strictly-lower blocks hold S's off-diagonal pattern plus the fill-in
     assert st["nnz_l_blocks"] >= st["nnz_s_blocks"] - st["num_free_blocks"]
     assert st["factor_flops"] > 0 and st["num_supernodes"] >= 1 and st["num_levels"] >= 1
+
+
+def test_window_generator_and_batch_validation(sg):
+    """The batched workload's windows: sizes in range, 2..8 distinct cameras per point;
+    windows beyond 31 cameras are rejected before any device work."""
+    for i in range(20):
+        s = sg.window_spec(i)
+        assert 5 <= s.num_cameras <= 30 and 100 <= s.num_points <= 2000
+        w = sg.generate_ba(s)
+        per = np.bincount(w.obs_idx[:, 2], minlength=s.num_points)
+        assert per.min() >= 1 and per.max() <= 8
+    big = sg.generate_ba(sg.BaSpec(layout=0, num_cameras=40, num_points=50, obs_per_point_ring=3))
+    with pytest.raises(sg.Unsupported):
+        sg.Batch([big], sg.OptParams())
