@@ -89,6 +89,46 @@ def tol_summary(hist, f_star, n_obs):
     return out
 
 
+def batch_leg(sg, n_windows: int, iters: int = 20) -> dict:
+    """The batched workload: n_windows independent ring windows (5-30 cameras, 100-2000
+    points, 2-8 cameras per point, seeded by window id), 20 LM iterations each, FP64 and
+    FP32-assembly.  Throughput from a graph replay of all iterations; per-window latency
+    = the time at the end of the iteration in which the window terminated (one launch
+    per iteration, synchronized)."""
+    t = time.perf_counter()
+    windows = [sg.generate_ba(sg.window_spec(i)) for i in range(n_windows)]
+    n_obs = int(sum(w.num_obs for w in windows))
+    gen_s = time.perf_counter() - t
+    out = {"windows": n_windows, "observations": n_obs, "lm_iterations": iters, "generate_s": gen_s}
+    for name, prec in (("f64", sg.Precision.F64), ("fp32_assembly", sg.Precision.Mixed)):
+        t = time.perf_counter()
+        b = sg.Batch(windows, sg.OptParams(num_iterations=iters, initial_lm_damping=1.0, precision=prec))
+        create_s = time.perf_counter() - t
+        secs = b.run(iters, graph=True)
+        res = b.results()
+        # latency: synchronized single iterations from a reset state
+        b.reset()
+        lat = np.full(n_windows, np.nan)
+        t0 = time.perf_counter()
+        for it in range(iters):
+            b.run(1, graph=False)
+            now = time.perf_counter() - t0
+            term = b.termination()
+            lat[np.isnan(lat) & (term != 0)] = now
+        info = b.info()
+        b.close()
+        out[name] = {"seconds": secs, "problems_per_s": n_windows / secs,
+                     "obs_iterations_per_s": float(np.sum(res["steps"].astype(np.float64) *
+                                                          np.diff(b.obs_ptr))) / secs,
+                     "latency_p50_ms": float(np.nanpercentile(lat, 50) * 1e3),
+                     "latency_p99_ms": float(np.nanpercentile(lat, 99) * 1e3),
+                     "mean_steps": float(res["steps"].mean()),
+                     "final_over_initial_cost_median": float(np.median(res["final_cost"] / res["initial_cost"])),
+                     "create_s": create_s, "autotuned_threads": info["threads"],
+                     "autotune_ms": info["tune_ms"]}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +137,8 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2, help="workload index (0, 1, 2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance runs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-window leg")
+    ap.add_argument("--batch-windows", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build runs the "
@@ -199,6 +241,12 @@ def main() -> int:
                        "another path is not a property of its speed alone.")
         log(f"[bench] time-to-tol {json.dumps(tol)}")
 
+    # ---- batched small windows: problems/s and per-window latency -----------------------------
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(sg, args.batch_windows)
+        log(f"[bench] batch {json.dumps(batch)}")
+
     cpu = None
     if not args.no_cpu:
         t = time.perf_counter()
@@ -226,6 +274,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "kernels": kernels,
         "time_to_tol": tol,
+        "batch": batch,
         "rcs": st,
     }
     print(json.dumps(line), flush=True)

@@ -917,6 +917,13 @@ class Batch:
         return {"poses": poses, "points": points, "initial_cost": c0, "final_cost": c1, "steps": steps,
                 "termination": term}
 
+    def termination(self) -> np.ndarray:
+        """Termination code per window (0 while running); no variable copies."""
+        term = np.empty(self.num_problems, dtype=np.int32)
+        _check(library().sgpu_batch_results(self._h, None, None, None, None, None,
+                                            term.ctypes.data_as(C.POINTER(C.c_int32))))
+        return term
+
     def info(self) -> dict:
         t = C.c_int32()
         ms = (C.c_double * 3)()
