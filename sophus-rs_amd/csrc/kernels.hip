@@ -236,146 +236,239 @@ __device__ __forceinline__ void mat3_mul_t(const T A[9], const T B[9], T C[9]) {
       C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
 }
 
-// ---- kernel: the reduced camera system, one wavefront per S block ------------------------
+// ---- the reduced camera system ----------------------------------------------------------
 // Block (c1, c2) of S sums over the (obs_i, obs_j) pairs of the points seen by both
 // cameras (cam(obs_i) = c1, cam(obs_j) = c2):
-//   i != j:  -P^T (M_i H_ll^-1 M_j) P
+//   i != j:  -P_i^T (M_i H_ll^-1 M_j) P_j
 //   i == j:  +P^T (M_i H_ll^-1 (H_ll - M_i)) P      (= P^T (M_i - M_i H_ll^-1 M_i) P, written
-//                                                   without the cancellation)
-// and the (o, o) pairs of a diagonal block also carry the camera's gradient
-// -P^T v_o and its reduced rhs P^T (M_o y_b - v_o).  Lanes stride over the block's
-// pairs, accumulate in FP64 registers and butterfly-reduce; no atomics, fixed order.
+//                                                   without the cancellation; M_i alone
+//                                                   for a constant point)
+// and the (o, o) terms also carry the camera's gradient -P^T v_o and its reduced rhs
+// P^T (M_o y_b - v_o).
+// With H_ll^-1 = L L^T (3x3 Cholesky of the point's inverse) and U_o = L^T M_o,
+//   M_i H_ll^-1 M_j = U_i^T U_j,
+// so a cross pair needs one 12-value record per observation (U_o and its lift point)
+// instead of both observations' Jacobians.  k_cam_obs forms the records (FP64, stored
+// as T) for every observation of a free camera in camera-major order and sums each
+// camera's (o, o) terms per chunk of 256 observations; k_schur_blocks then reduces each
+// block's cross pairs and, on the diagonal, the camera's chunk sums.  Every sum runs in
+// a fixed order (no atomics).
 // With `centered`, each camera's tangent is taken about its own centre c
 // (delta = G delta', G = [[I, 0], [hat(c), I]]): P becomes [hat(c - x) | I], whose entries
 // are scene-depth sized instead of world-coordinate sized, so FP32 products keep their
 // relative accuracy in S' = G^T S G and rhs' = G^T rhs.  The gradient stays in the
 // original coordinates.
+
+constexpr int kObsRec = 12;  // per-observation record: U_o (3x3 row-major), lift point
+
+// Sums v[0..N) over the wavefront.  Halving steps (while the live length is even) leave
+// each lane with a slice of the totals; the remaining masks are a plain butterfly.
+// Returns the slice offset: lane holds the totals of v[off .. off + len) in v[0 .. len).
+template <int N>
+__device__ __forceinline__ int wave_sum_scatter(double (&v)[N], int lane) {
+  int len = N, off = 0;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    if (len % 2 == 0) {
+      const int h = len / 2;
+      const bool up = (lane & m) != 0;
+#pragma unroll
+      for (int k = 0; k < N / 2; ++k) {
+        if (k < h) {
+          const double send = up ? v[k] : v[k + h];
+          const double keep = up ? v[k + h] : v[k];
+          v[k] = keep + __shfl_xor(send, m, 64);
+        }
+      }
+      len = h;
+      off += up ? h : 0;
+    } else {
+#pragma unroll
+      for (int k = 0; k < N; ++k)
+        if (k < len) v[k] += __shfl_xor(v[k], m, 64);
+    }
+  }
+  return off;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-    k_schur_slots(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
-                  const double* __restrict__ cw, double nu, const double* __restrict__ hinv_g,
-                  const double* __restrict__ hll_g, const double* __restrict__ y_g,
-                  double* __restrict__ front, double* __restrict__ rhs_nat,
-                  double* __restrict__ grad_cam, int centered, double* __restrict__ bsr) {
+    k_cam_obs(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
+              const double* __restrict__ cw, const double* __restrict__ hinv_g,
+              const double* __restrict__ hll_g, const double* __restrict__ y_g, int centered,
+              T* __restrict__ vobs, double* __restrict__ own_part) {
+  constexpr int kOwn = 48;  // 36 block entries, 6 reduced rhs, 6 gradient
+  __shared__ T sv[kThreads * kObsRec];
+  __shared__ double red[kThreads / 64][kOwn];
+  const int ch = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int q0 = p.own_chunks[3 * ch + 1], q1 = p.own_chunks[3 * ch + 2];
+  const int q = q0 + tid;
+  double own[kOwn];
+#pragma unroll
+  for (int e = 0; e < kOwn; ++e) own[e] = 0.0;
+  if (q < q1) {
+    const int o = p.cam_obs[q];
+    const int b = p.obs_point[o];
+    const bool active = p.pt_block[b] >= 0;
+    const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
+    const double* c = cw + 12 * p.obs_pose[o];
+    const double* ini = intr + 4 * p.obs_intr[o];
+    double xl[3] = {xd[0], xd[1], xd[2]};  // lift point (relative to the camera if centered)
+    if (centered) {
+      double cc[3];
+      cam_center(c, cc);
+#pragma unroll
+      for (int e = 0; e < 3; ++e) xl[e] -= cc[e];
+    }
+    double xc[3], Ad[6], Md[9], v[3];
+    to_camera(c, xd, xc);
+    obs_jacobian<double>(c, ini, xc, Ad);
+    gram(Ad, Md);
+    obs_residual<double>(ini, p.obs_uv + 2 * o, xc, p.robust, p.delta, Ad, v);
+    // record: U_o row-major, then the lift point (zero for a constant point: it has no
+    // cross pairs)
+    T* vs = sv + kObsRec * tid;
+    if (active) {
+      const double* h = hinv_g + 9 * b;
+      // lower Cholesky factor of H_ll^-1 (zero columns past a non-positive pivot)
+      const double l00 = sqrt(fmax(h[0], 0.0)), i00 = l00 > 0.0 ? 1.0 / l00 : 0.0;
+      const double l10 = h[3] * i00, l20 = h[6] * i00;
+      const double l11 = sqrt(fmax(h[4] - l10 * l10, 0.0)), i11 = l11 > 0.0 ? 1.0 / l11 : 0.0;
+      const double l21 = (h[7] - l20 * l10) * i11;
+      const double l22 = sqrt(fmax(h[8] - l20 * l20 - l21 * l21, 0.0));
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        vs[j] = T(l00 * Md[j] + l10 * Md[3 + j] + l20 * Md[6 + j]);
+        vs[3 + j] = T(l11 * Md[3 + j] + l21 * Md[6 + j]);
+        vs[6 + j] = T(l22 * Md[6 + j]);
+        vs[9 + j] = T(xl[j]);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < kObsRec; ++e) vs[e] = T(0);
+    }
+    // own (o, o) term
+    const T x[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
+    T Ai[6], Mi[9], N[9];
+#pragma unroll
+    for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
+    gram(Ai, Mi);
+    double t[3];
+    if (active) {
+      const double* yb = y_g + 3 * b;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * yb[0] + Md[3 * r + 1] * yb[1] + Md[3 * r + 2] * yb[2] - v[r];
+      // R = H_ll - M_o (FP64, then to T), N = M_o H_ll^-1 R
+      const double* H = hll_g + 6 * b;
+      const double Hf[9] = {H[0], H[1], H[2], H[1], H[3], H[4], H[2], H[4], H[5]};
+      T R[9], Hi[9], Z[9];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) {
+        R[e] = T(Hf[e] - Md[e]);
+        Hi[e] = T(hinv_g[9 * b + e]);
+      }
+      mat3_mul_t(Mi, Hi, Z);
+      mat3_mul_t(Z, R, N);
+    } else {  // constant point: own H_pp term only
+#pragma unroll
+      for (int r = 0; r < 3; ++r) t[r] = -v[r];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) N[e] = Mi[e];
+    }
+    add_lift(x, x, N, T(1), [&](int e, T val) { own[e] = double(val); });
+    // P^T t = [x cross t; t]
+    own[36] = xl[1] * t[2] - xl[2] * t[1];
+    own[37] = xl[2] * t[0] - xl[0] * t[2];
+    own[38] = xl[0] * t[1] - xl[1] * t[0];
+    own[39] = t[0];
+    own[40] = t[1];
+    own[41] = t[2];
+    own[42] = -(xd[1] * v[2] - xd[2] * v[1]);
+    own[43] = -(xd[2] * v[0] - xd[0] * v[2]);
+    own[44] = -(xd[0] * v[1] - xd[1] * v[0]);
+    own[45] = -v[0];
+    own[46] = -v[1];
+    own[47] = -v[2];
+  }
+  __syncthreads();
+  // the chunk's factors are contiguous in camera-major order: coalesced stores
+  T* dst = vobs + kObsRec * int64_t(q0);
+  for (int e = tid; e < kObsRec * (q1 - q0); e += kThreads) dst[e] = sv[e];
+  const int off = wave_sum_scatter(own, lane);
+  // 48 = 16 * 3: after four halvings each lane holds 3 totals, shared by 4 lanes
+  if ((lane & 3) == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) red[wid][off + k] = own[k];
+  __syncthreads();
+  if (tid < kOwn) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) sum += red[w][tid];
+    own_part[kOwn * int64_t(ch) + tid] = sum;
+  }
+}
+
+// acc -= P_i^T (U_i^T U_j) P_j (6x6 row-major) from two observation records
+template <typename T>
+__device__ __forceinline__ void sub_pair(const T* __restrict__ ri, const T* __restrict__ rj,
+                                         double acc[36]) {
+  T a[kObsRec], b[kObsRec];
+#pragma unroll
+  for (int e = 0; e < kObsRec; ++e) {
+    a[e] = ri[e];
+    b[e] = rj[e];
+  }
+  T N[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) N[3 * r + q] = a[r] * b[q] + a[3 + r] * b[3 + q] + a[6 + r] * b[6 + q];
+  add_lift(a + 9, b + 9, N, T(-1), [&](int e, T val) { acc[e] += double(val); });
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; this remaps them so each XCD
+// takes one contiguous range of logical workgroups (neighbouring S rows, which share
+// cameras and therefore V rows, then meet in one XCD's L2).
+__device__ __forceinline__ int xcd_contiguous(int bid, int n) {
+  const int x = bid & 7, i = bid >> 3, q = n >> 3, r = n & 7;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// One wavefront per S block: lanes stride over the block's cross pairs (point order),
+// FP64 accumulation, a fixed-order wavefront reduction; on the diagonal the camera's
+// chunk sums, the damping and the rhs / gradient rows.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_schur_blocks(DevProblem p, const T* __restrict__ vobs, const double* __restrict__ own_part,
+                   const double* __restrict__ cw, double nu, int centered,
+                   double* __restrict__ front, double* __restrict__ rhs_nat,
+                   double* __restrict__ grad_cam, double* __restrict__ bsr) {
   const int lane = threadIdx.x & 63;
-  const int64_t k = int64_t(blockIdx.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t k = int64_t(xcd_contiguous(blockIdx.x, gridDim.x)) * (blockDim.x >> 6) +
+                    (threadIdx.x >> 6);
   if (k >= p.nslot) return;  // whole wavefront leaves together
   const int c1 = p.s_row[k];
   const bool diag = p.s_col[k] == c1;
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
-  double racc[6] = {0, 0, 0, 0, 0, 0}, gacc[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
     const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
-    const int b = p.obs_point[i];
-    const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
-    const double* cwi = cw + 12 * p.obs_pose[i];
-    double xl[3] = {xd[0], xd[1], xd[2]};  // lift point of obs i (relative to c_i if centered)
-    if (centered) {
-      double c[3];
-      cam_center(cwi, c);
-#pragma unroll
-      for (int e = 0; e < 3; ++e) xl[e] -= c[e];
-    }
-    const T x[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
-    T xj[3] = {x[0], x[1], x[2]};
-    const double* ini = intr + 4 * p.obs_intr[i];
-    double xci[3];
-    to_camera(cwi, xd, xci);
-    double Ad[6];
-    obs_jacobian<double>(cwi, ini, xci, Ad);
-    T Ai[6], Mi[9];
-#pragma unroll
-    for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
-    gram(Ai, Mi);
-    const bool active = p.pt_block[b] >= 0;
-    T N[9];
-    T sign = T(1);
-    if (i == j) {
-      double v[3];
-      obs_residual<double>(ini, p.obs_uv + 2 * i, xci, p.robust, p.delta, Ad, v);
-      double t[3];
-      if (active) {
-        double Md[9];
-        gram(Ad, Md);
-        const double* yb = y_g + 3 * b;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * yb[0] + Md[3 * r + 1] * yb[1] + Md[3 * r + 2] * yb[2] - v[r];
-        // R = H_ll - M_i (FP64, then to T), N = M_i H_ll^-1 R
-        const double* H = hll_g + 6 * b;
-        const double Hf[9] = {H[0], H[1], H[2], H[1], H[3], H[4], H[2], H[4], H[5]};
-        T R[9], Hi[9], Z[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) {
-          R[e] = T(Hf[e] - Md[e]);
-          Hi[e] = T(hinv_g[9 * b + e]);
-        }
-        mat3_mul_t(Mi, Hi, Z);
-        mat3_mul_t(Z, R, N);
-      } else {  // constant point: own H_pp term only
-#pragma unroll
-        for (int r = 0; r < 3; ++r) t[r] = -v[r];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) N[e] = Mi[e];
-      }
-      // P^T t = [x cross t; t]
-      racc[0] += xl[1] * t[2] - xl[2] * t[1];
-      racc[1] += xl[2] * t[0] - xl[0] * t[2];
-      racc[2] += xl[0] * t[1] - xl[1] * t[0];
-      racc[3] += t[0];
-      racc[4] += t[1];
-      racc[5] += t[2];
-      gacc[0] -= xd[1] * v[2] - xd[2] * v[1];
-      gacc[1] -= xd[2] * v[0] - xd[0] * v[2];
-      gacc[2] -= xd[0] * v[1] - xd[1] * v[0];
-      gacc[3] -= v[0];
-      gacc[4] -= v[1];
-      gacc[5] -= v[2];
-    } else {
-      const double* cwj = cw + 12 * p.obs_pose[j];
-      double xcj[3];
-      to_camera(cwj, xd, xcj);
-      if (centered) {
-        double c[3];
-        cam_center(cwj, c);
-#pragma unroll
-        for (int e = 0; e < 3; ++e) xj[e] = T(xd[e] - c[e]);
-      }
-      T Aj[6], Mj[9], Hi[9], Z[9];
-      obs_jacobian<T>(cwj, intr + 4 * p.obs_intr[j], xcj, Aj);
-      gram(Aj, Mj);
-#pragma unroll
-      for (int e = 0; e < 9; ++e) Hi[e] = T(hinv_g[9 * b + e]);
-      mat3_mul_t(Mi, Hi, Z);
-      mat3_mul_t(Z, Mj, N);
-      sign = T(-1);
-    }
-    add_lift(x, xj, N, sign, [&](int e, T val) { acc[e] += double(val); });
+    sub_pair(vobs + kObsRec * int64_t(i), vobs + kObsRec * int64_t(j), acc);
   }
-  // butterfly reduction: every lane ends with all sums
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) {
-#pragma unroll
-    for (int e = 0; e < 36; ++e) acc[e] += __shfl_xor(acc[e], m, 64);
-    if (diag) {
-#pragma unroll
-      for (int e = 0; e < 6; ++e) {
-        racc[e] += __shfl_xor(racc[e], m, 64);
-        gacc[e] += __shfl_xor(gacc[e], m, 64);
-      }
-    }
-  }
+  // 36 = 4 * 9: after two halvings lanes 16 m .. 16 m + 15 hold totals 9 m .. 9 m + 8
+  wave_sum_scatter(acc, lane);
   double mine = 0.0;
+  const int src = 16 * (min(lane, 35) / 9), want = min(lane, 35) % 9;
 #pragma unroll
-  for (int e = 0; e < 36; ++e)
-    if (lane == e) mine = acc[e];
-  if (diag) {
-#pragma unroll
-    for (int e = 0; e < 6; ++e) {
-      if (lane == 36 + e) mine = racc[e];
-      if (lane == 42 + e) mine = gacc[e];
-    }
+  for (int e = 0; e < 9; ++e) {
+    const double t = __shfl(acc[e], src, 64);
+    if (want == e) mine = t;
+  }
+  if (lane >= 36) mine = 0.0;
+  if (diag && lane < 48) {
+    for (int ch = p.cam_chunk_ptr[c1]; ch < p.cam_chunk_ptr[c1 + 1]; ++ch)
+      mine += own_part[48 * int64_t(ch) + lane];
   }
   if (lane < 36) {
     const int a = lane / 6, bb = lane % 6;
@@ -684,47 +777,73 @@ __device__ __forceinline__ void diag_solve6(const double* L, const double* d, do
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
+// Node-local triangular solves: one thread per node row (K <= kTriMax, blockDim a
+// multiple of 64 covering K), 32-column solve panels that sit in one half of one
+// wavefront.  Per panel the owning wavefront finishes the panel's 32 values by
+// column-oriented substitution through lane shuffles (and applies them to its other
+// 32 rows), publishes them to LDS, and after one barrier every later wavefront applies
+// them to its rows.  Each thread's L entries for the next panel are loaded while the
+// current panel is processed.  The unit block diagonal of L is implicit (entries with
+// row block == column block are skipped); D^-1 is applied per 6x6 pivot at the end.
+constexpr int kTriMax = 6 * 96;  // largest supernode (96 column blocks)
+constexpr int kSP = 32;
+
+// base[i] when ok, else 0; the load is unconditional (from base[0] when !ok, which the
+// caller keeps valid) so a panel's loads issue back to back without branches
+__device__ __forceinline__ double load_or_zero(const double* base, int i, bool ok) {
+  const double v = base[ok ? i : 0];
+  return ok ? v : 0.0;
+}
+
+__global__ void __launch_bounds__(kTriMax)
     k_mf_fwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y,
                  double* __restrict__ ytmp) {
   const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* A = f.lbuf + f.sn_loff[X];
-  __shared__ double yl[6 * 96];
-  __shared__ double Lp[kPW][kPW + 1];
-  const double* ux = f.usolve + f.sn_uoff[X];  // children's updates to this node's rows
-  for (int i = threadIdx.x; i < K; i += blockDim.x) yl[i] = y[6 * c0 + i] - ux[i];
-  for (int k0 = 0; k0 < K; k0 += kPW) {
-    const int kend = min(K, k0 + kPW), np = kend - k0;
-    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
-      const int r = e % np, c = e / np;
-      Lp[r][c] = r / 6 > c / 6 ? A[int64_t(k0 + c) * F + k0 + r] : 0.0;
-    }
-    __syncthreads();
-    for (int j0 = 6; j0 < np; j0 += 6) {
-      if (threadIdx.x < 6) {
-        const int r = j0 + threadIdx.x;
-        double v = yl[k0 + r];
-        for (int c = 0; c < j0; ++c) v -= Lp[r][c] * yl[k0 + c];
-        yl[k0 + r] = v;
+  __shared__ double yl[kTriMax];
+  const int r = threadIdx.x, wave = r >> 6, rb = r / 6;
+  const bool own = r < K;
+  // children's updates to this node's rows are in u_X
+  double v = own ? y[6 * c0 + r] - f.usolve[f.sn_uoff[X] + r] : 0.0;
+  yl[r] = 0.0;
+  const int npan = (K + kSP - 1) / kSP;
+  double cur[kSP], nxt[kSP];
+#pragma unroll
+  for (int t = 0; t < kSP; ++t)
+    cur[t] = load_or_zero(A + int64_t(min(t, K - 1)) * F, r, own && t < K && t / 6 < rb);
+  __syncthreads();
+  for (int p = 0; p < npan; ++p) {
+    const int k0 = kSP * p, pw = k0 >> 6;
+    if (p + 1 < npan) {
+      const int k1 = k0 + kSP;
+#pragma unroll
+      for (int t = 0; t < kSP; ++t) {
+        const int c = k1 + t;
+        nxt[t] = load_or_zero(A + int64_t(min(c, K - 1)) * F, r, own && c < K && c / 6 < rb);
       }
-      __syncthreads();
     }
-    for (int r = kend + threadIdx.x; r < K; r += blockDim.x) {
-      double s = 0;
-      for (int c = k0; c < kend; ++c) s += A[int64_t(c) * F + r] * yl[c];
-      yl[r] -= s;
+    if (wave == pw) {  // wave-uniform: the panel's owning wavefront
+#pragma unroll
+      for (int t = 0; t < kSP; ++t) v -= cur[t] * __shfl(v, (k0 + t) & 63, 64);
+      if (r >= k0 && r < k0 + kSP) yl[r] = v;
     }
     __syncthreads();
+    if (wave > pw) {
+#pragma unroll
+      for (int t = 0; t < kSP; ++t) v -= cur[t] * yl[k0 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < kSP; ++t) cur[t] = nxt[t];
   }
-  for (int i = threadIdx.x; i < K; i += blockDim.x) ytmp[6 * c0 + i] = yl[i];
-  for (int jb = threadIdx.x; jb < K / 6; jb += blockDim.x) {
-    double v[6];
+  if (own) ytmp[6 * c0 + r] = v;
+  if (r < K / 6) {
+    double w[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) v[k] = yl[6 * jb + k];
-    diag_solve6(f.piv_L + 36 * (c0 + jb), f.piv_d + 6 * (c0 + jb), v);
+    for (int k = 0; k < 6; ++k) w[k] = yl[6 * r + k];
+    diag_solve6(f.piv_L + 36 * (c0 + r), f.piv_d + 6 * (c0 + r), w);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) y[6 * (c0 + jb) + k] = v[k];
+    for (int k = 0; k < 6; ++k) y[6 * (c0 + r) + k] = w[k];
   }
 }
 
@@ -788,56 +907,54 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
+// x_K = L_KK^-T t_K: one thread per node column, panels last to first; thread c
+// holds L[k0 .. k0 + 31, c] of the current panel (contiguous in its column).
+__global__ void __launch_bounds__(kTriMax)
     k_mf_bwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y) {
   const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* A = f.lbuf + f.sn_loff[X];
-  __shared__ double xl[6 * 96];
-  __shared__ double Lp[kPW][kPW + 1];
-  __shared__ double red[kThreads / 64][kPW];
-  for (int i = threadIdx.x; i < K; i += blockDim.x) xl[i] = y[6 * c0 + i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int npan = (K + kPW - 1) / kPW;
-  for (int pn = npan - 1; pn >= 0; --pn) {
-    const int k0 = kPW * pn, kend = min(K, k0 + kPW), np = kend - k0;
-    for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
-      const int r = e % np, c = e / np;
-      Lp[r][c] = r / 6 > c / 6 ? A[int64_t(k0 + c) * F + k0 + r] : 0.0;
-    }
-    double part[kPW];
+  __shared__ double xl[kTriMax];
+  const int c = threadIdx.x, wave = c >> 6, cb = c / 6;
+  const bool own = c < K;
+  double x = own ? y[6 * c0 + c] : 0.0;
+  xl[c] = 0.0;
+  const int npan = (K + kSP - 1) / kSP;
+  const double* col = A + int64_t(min(c, K - 1)) * F;
+  double cur[kSP], nxt[kSP];
+  {
+    const int k0 = kSP * (npan - 1);
 #pragma unroll
-    for (int t = 0; t < kPW; ++t) part[t] = 0.0;
-    for (int r = kend + threadIdx.x; r < K; r += blockDim.x) {
-      const double xv = xl[r];
-#pragma unroll
-      for (int t = 0; t < kPW; ++t)
-        if (t < np) part[t] += A[int64_t(k0 + t) * F + r] * xv;
-    }
-#pragma unroll
-    for (int t = 0; t < kPW; ++t)
-      for (int off = 32; off > 0; off >>= 1) part[t] += __shfl_down(part[t], off, 64);
-    if (lane == 0)
-      for (int t = 0; t < kPW; ++t) red[wid][t] = part[t];
-    __syncthreads();
-    if (int(threadIdx.x) < np) {
-      double s = 0;
-      for (int w = 0; w < int(blockDim.x / 64); ++w) s += red[w][threadIdx.x];
-      xl[k0 + threadIdx.x] -= s;
-    }
-    __syncthreads();
-    for (int j0 = ((np - 1) / 6 - 1) * 6; j0 >= 0; j0 -= 6) {
-      if (threadIdx.x < 6) {
-        const int c = j0 + threadIdx.x;
-        double v = xl[k0 + c];
-        for (int r = j0 + 6; r < np; ++r) v -= Lp[r][c] * xl[k0 + r];
-        xl[k0 + c] = v;
-      }
-      __syncthreads();
+    for (int t = 0; t < kSP; ++t) {
+      const int r = k0 + t;
+      cur[t] = load_or_zero(col, r, own && r < K && r / 6 > cb);
     }
   }
-  for (int i = threadIdx.x; i < K; i += blockDim.x) y[6 * c0 + i] = xl[i];
+  __syncthreads();
+  for (int p = npan - 1; p >= 0; --p) {
+    const int k0 = kSP * p, pw = k0 >> 6;
+    if (p > 0) {
+      const int k1 = k0 - kSP;
+#pragma unroll
+      for (int t = 0; t < kSP; ++t) {
+        const int r = k1 + t;
+        nxt[t] = load_or_zero(col, r, own && r / 6 > cb);
+      }
+    }
+    if (wave == pw) {
+#pragma unroll
+      for (int t = kSP - 1; t >= 0; --t) x -= cur[t] * __shfl(x, (k0 + t) & 63, 64);
+      if (c >= k0 && c < k0 + kSP) xl[c] = x;
+    }
+    __syncthreads();
+    if (wave < pw) {
+#pragma unroll
+      for (int t = 0; t < kSP; ++t) x -= cur[t] * xl[k0 + t];
+    }
+#pragma unroll
+    for (int t = 0; t < kSP; ++t) cur[t] = nxt[t];
+  }
+  if (own) y[6 * c0 + c] = x;
 }
 
 __global__ void k_permute_in(int nb, const int32_t* __restrict__ perm, const double* __restrict__ src,
@@ -1308,13 +1425,22 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
   (void)sc;
   if (p.nslot == 0) return;
   const int wpb = kThreads / 64;
-  const dim3 g(unsigned((p.nslot + wpb - 1) / wpb));
-  if (precision == 0)
-    hipLaunchKernelGGL(k_schur_slots<double>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
-                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 0, w.bsr);
-  else
-    hipLaunchKernelGGL(k_schur_slots<float>, g, dim3(kThreads), 0, s, p, points, intr, cw, nu,
-                       w.hinv, w.hll, w.pt_y, f.front, w.rhs_nat, w.grad_cam, 1, w.bsr);
+  const dim3 go(unsigned(p.n_own_chunks)), gs(unsigned((p.nslot + wpb - 1) / wpb));
+  if (precision == 0) {
+    double* v = static_cast<double*>(w.vobs);
+    if (p.n_own_chunks > 0)
+      hipLaunchKernelGGL(k_cam_obs<double>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.hinv,
+                         w.hll, w.pt_y, 0, v, w.own_part);
+    hipLaunchKernelGGL(k_schur_blocks<double>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 0,
+                       f.front, w.rhs_nat, w.grad_cam, w.bsr);
+  } else {
+    float* v = static_cast<float*>(w.vobs);
+    if (p.n_own_chunks > 0)
+      hipLaunchKernelGGL(k_cam_obs<float>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.hinv,
+                         w.hll, w.pt_y, 1, v, w.own_part);
+    hipLaunchKernelGGL(k_schur_blocks<float>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 1,
+                       f.front, w.rhs_nat, w.grad_cam, w.bsr);
+  }
 }
 
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v) {
@@ -1360,6 +1486,13 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
   }
 }
 
+// whole wavefronts covering the level's largest node (one thread per node row)
+static int tri_threads(const Symbolic& S, int level) {
+  const int K = 6 * S.lvl_max_nc[level];
+  if (K > kTriMax) throw std::runtime_error("supernode larger than the triangular-solve block");
+  return std::max(64, (K + 63) / 64 * 64);
+}
+
 void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, double* y,
                       double* dx_cam) {
   const Symbolic& S = *f.host;
@@ -1369,8 +1502,8 @@ void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, 
   const int nlev = int(S.lvl_ptr.size()) - 1;
   for (int l = 0; l < nlev; ++l) {
     const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
-    hipLaunchKernelGGL(k_mf_fwd_tri, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[l], y,
-                       f.ytmp);
+    const int tt = tri_threads(S, l);
+    hipLaunchKernelGGL(k_mf_fwd_tri, dim3(n), dim3(tt), 0, s, f, f.lvl_sn + S.lvl_ptr[l], y, f.ytmp);
     const int nr = S.solve_rows_ptr[l + 1] - S.solve_rows_ptr[l];
     if (nr > 0)
       hipLaunchKernelGGL(k_mf_fwd_rect, dim3(nr), dim3(kThreads), 0, s, f,
@@ -1388,7 +1521,8 @@ void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, 
     if (nc > 0)
       hipLaunchKernelGGL(k_mf_bwd_rect, dim3(nc), dim3(kThreads), 0, s, f,
                          f.solve_cols + 2 * int64_t(S.solve_cols_ptr[l]), y);
-    hipLaunchKernelGGL(k_mf_bwd_tri, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[l], y);
+    hipLaunchKernelGGL(k_mf_bwd_tri, dim3(n), dim3(tri_threads(S, l)), 0, s, f,
+                       f.lvl_sn + S.lvl_ptr[l], y);
   }
   hipLaunchKernelGGL(k_permute_out, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, y, dx_cam);
 }

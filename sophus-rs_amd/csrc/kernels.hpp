@@ -39,8 +39,11 @@ struct DevProblem {
   const int32_t* s_ptr;
   const int32_t* s_col;
   const int32_t* s_row;     // slot -> row camera
-  const int64_t* pair_ptr;  // slot -> pair range
-  const int32_t* pairs;     // (obs_i, obs_j) per pair
+  const int64_t* pair_ptr;  // slot -> cross-pair range
+  const int32_t* pairs;     // (q_i, q_j) per cross pair: positions in cam_obs
+  const int32_t* own_chunks;     // (camera, q0, q1) per own-term chunk
+  const int32_t* cam_chunk_ptr;  // nb + 1
+  int32_t n_own_chunks;
   int64_t nslot;
   const int64_t* s_off;
   const int32_t* s_ld;
@@ -92,6 +95,8 @@ struct DevLin {
   double* x_aux;      // nb x 6 second solve buffer
   double* dx_cam;     // nb x 6 camera step (camera order)
   double* bsr;        // nslot x 36 lower S blocks for the iterative solvers (else null)
+  void* vobs;         // 12 per camera-major observation: U_o and lift point (double, or float when mixed)
+  double* own_part;   // 48 per own-term chunk: the chunk's (o, o) block, rhs and gradient sums
   double* partial;    // reduction partials
   int32_t partial_cap;
 };

@@ -246,6 +246,9 @@ struct sgpu_optimizer {
     }
     dp.pair_ptr = arena.upload(S.pair_ptr, stream);
     dp.pairs = arena.upload(S.pairs, stream);
+    dp.own_chunks = arena.upload(S.own_chunks, stream);
+    dp.cam_chunk_ptr = arena.upload(S.cam_chunk_ptr, stream);
+    dp.n_own_chunks = int32_t(S.own_chunks.size() / 3);
     dp.nslot = int64_t(S.s_col.size());
     dp.s_off = arena.upload(S.s_off, stream);
     dp.s_ld = arena.upload(S.s_ld, stream);
@@ -292,6 +295,8 @@ struct sgpu_optimizer {
     dl.x_perm = arena.alloc<double>(size_t(6) * S.nb);
     dl.x_aux = arena.alloc<double>(size_t(6) * S.nb);
     dl.dx_cam = arena.alloc<double>(size_t(6) * S.nb);
+    dl.vobs = arena.alloc<double>(size_t(12) * std::max<int32_t>(S.cam_ptr[S.nb], 1));
+    dl.own_part = arena.alloc<double>(size_t(48) * std::max<size_t>(S.own_chunks.size() / 3, 1));
     dl.bsr = nullptr;
     if (iterative()) {
       std::vector<int32_t> mv_ptr, mv_ent, diag_slot, mv_row;

@@ -377,7 +377,18 @@ Symbolic analyze(const HostProblem& P) {
   }
   S.s_col.resize(S.s_ptr[S.nb]);
   for (int32_t c = 0; c < S.nb; ++c) std::copy(rows[c].begin(), rows[c].end(), &S.s_col[S.s_ptr[c]]);
-  // per-slot observation pairs (two passes: count, then fill in point order)
+  // per-slot cross pairs i != j of active points (two passes: count, then fill in point
+  // order), as positions in the camera-major observation list; the (o, o) terms of each
+  // camera are summed per chunk of its camera-major list
+  std::vector<int32_t> cam_pos(S.nobs, -1);
+  for (int32_t q = 0; q < int32_t(S.cam_obs.size()); ++q) cam_pos[S.cam_obs[q]] = q;
+  S.cam_chunk_ptr.assign(S.nb + 1, 0);
+  S.own_chunks.clear();
+  for (int32_t c = 0; c < S.nb; ++c) {
+    for (int32_t q = S.cam_ptr[c]; q < S.cam_ptr[c + 1]; q += kOwnChunk)
+      S.own_chunks.insert(S.own_chunks.end(), {c, q, std::min(q + kOwnChunk, S.cam_ptr[c + 1])});
+    S.cam_chunk_ptr[c + 1] = int32_t(S.own_chunks.size() / 3);
+  }
   {
     const int64_t nslot = int64_t(S.s_col.size());
     auto slot_of = [&](int32_t r, int32_t c) -> int64_t {
@@ -399,21 +410,21 @@ Symbolic analyze(const HostProblem& P) {
           if (ci < 0) continue;
           for (int32_t j = S.pt_ptr[b]; j < S.pt_ptr[b + 1]; ++j) {
             const int32_t cj = poses.block[P.obs_pose[j]];
-            if (cj < 0 || cj > ci || (!active && j != i)) continue;
+            if (cj < 0 || cj > ci || !active || j == i) continue;
             const int64_t k = slot_of(ci, cj);
             if (pass == 0) {
               ++S.pair_ptr[k + 1];
             } else {
               const int64_t at = fill[k]++;
-              S.pairs[2 * at] = i;
-              S.pairs[2 * at + 1] = j;
+              S.pairs[2 * at] = cam_pos[i];
+              S.pairs[2 * at + 1] = cam_pos[j];
             }
           }
         }
       }
     }
   }
-  S.num_schur_pairs = int64_t(S.pairs.size() / 2);
+  S.num_schur_pairs = int64_t(S.pairs.size() / 2) + S.cam_ptr[S.nb];
   analyze_rcs(S);
   // pose-graph destinations (S slots)
   auto s_slot = [&](int32_t r, int32_t c) -> int32_t {
@@ -609,9 +620,11 @@ void analyze_rcs(Symbolic& S) {
   S.solve_cols_ptr.assign(nlev + 1, 0);
   S.solve_rows.clear();
   S.solve_cols.clear();
+  S.lvl_max_nc.assign(nlev, 0);
   for (int32_t l = 0; l < nlev; ++l) {
     for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
       const int32_t x = S.lvl_sn[q];
+      S.lvl_max_nc[l] = std::max(S.lvl_max_nc[l], S.sn_nc[x]);
       const int32_t m = 6 * (S.sn_nf[x] - S.sn_nc[x]), K = 6 * S.sn_nc[x];
       for (int32_t t = 0; t * 256 < m; ++t) S.solve_rows.insert(S.solve_rows.end(), {x, t});
       if (m > 0)

@@ -49,6 +49,8 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
                                const sgpu_cost_fn* cost_fns, int64_t num_cost_fns);
 
 // Symbolic structure of one problem (computed once per solve).
+constexpr int32_t kOwnChunk = 256;  // camera-major observations per own-term workgroup
+
 struct Symbolic {
   int32_t nb = 0;  // free camera blocks
   int32_t npt = 0, nobs = 0, npose = 0;
@@ -61,10 +63,13 @@ struct Symbolic {
   std::vector<int32_t> cam_ptr, cam_obs;
   // RCS S pattern in camera order: row c, columns <= c, sorted
   std::vector<int32_t> s_ptr, s_col;
-  // per S slot, the (obs_i, obs_j) pairs of one point with cam(obs_i) = row and
-  // cam(obs_j) = column, ordered by point; (o, o) pairs carry the camera's own terms
+  // per S slot, the cross pairs (obs_i, obs_j), i != j, of one active point with
+  // cam(obs_i) = row and cam(obs_j) = column, ordered by point, stored as positions in
+  // cam_obs; the camera's own (o, o) terms are summed over chunks of its cam_obs range
   std::vector<int64_t> pair_ptr;
   std::vector<int32_t> pairs;  // 2 per pair
+  std::vector<int32_t> own_chunks;     // (camera, q0, q1) per chunk of <= kOwnChunk
+  std::vector<int32_t> cam_chunk_ptr;  // nb + 1: chunks of each camera
   // ordering (perm[new] = old) and column elimination tree
   std::vector<int32_t> perm, perm_inv;
   std::vector<int32_t> etree_parent;
@@ -95,6 +100,7 @@ struct Symbolic {
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
   // solve work items per level: (node, 256-row tile below K), (node, 64-column tile of K)
   std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;
+  std::vector<int32_t> lvl_max_nc;  // largest node (column blocks) per level
   // extend-add groups: per level, nodes grouped by their rank among their parent's
   // children, so one launch never has two children of one parent (plain adds, fixed
   // order).  ext_lvl_ptr[l] .. ext_lvl_ptr[l+1] index groups; ext_grp_ptr into ext_nodes.
