@@ -6,8 +6,10 @@
 namespace sgpu {
 
 // ldlt_decompose_inplace (dense_ldlt.rs:164-256) on a 6x6 block with the PSD pivot rule.
-__device__ inline bool ldlt6(const double* A, double* L, double* d) {
-  double a[36];
+// The off-diagonal quotients use each pivot's reciprocal (formed once); dinv, if given,
+// receives those reciprocals (0 for a zero pivot).
+__device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv = nullptr) {
+  double a[36], rd[6];
   for (int k = 0; k < 36; ++k) a[k] = A[k];
   double max_abs_pivot = 1.0;
   const double tol = 1e-12;
@@ -16,7 +18,7 @@ __device__ inline bool ldlt6(const double* A, double* L, double* d) {
       double s = a[6 * j + k];
       for (int q = 0; q < k; ++q) s -= a[6 * j + q] * d[q] * a[6 * k + q];
       const double zt = fmax(max_abs_pivot, 1.0) * tol;
-      a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s / d[k];
+      a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s * rd[k];
     }
     double djj = a[6 * j + j];
     for (int q = 0; q < j; ++q) {
@@ -27,12 +29,16 @@ __device__ inline bool ldlt6(const double* A, double* L, double* d) {
     const double zt = fmax(max_abs_pivot, 1.0) * tol;
     if (fabs(djj) <= zt) {
       d[j] = 0.0;
+      rd[j] = 0.0;
     } else {
       d[j] = djj;
+      rd[j] = 1.0 / djj;
       max_abs_pivot = fmax(max_abs_pivot, fabs(djj));
     }
     a[6 * j + j] = 1.0;
   }
+  if (dinv)
+    for (int k = 0; k < 6; ++k) dinv[k] = rd[k];
   for (int r = 0; r < 6; ++r)
     for (int c = 0; c < 6; ++c) L[6 * r + c] = c > r ? 0.0 : a[6 * r + c];
   return true;

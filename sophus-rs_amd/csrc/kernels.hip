@@ -46,6 +46,32 @@ __device__ __forceinline__ double robust_weight(int robust, double delta, double
   return sqrt(a) / nrm;
 }
 
+// base[i] when ok, else 0; the load is unconditional (from base[0] when !ok, which the
+// caller keeps valid) so a panel's loads issue back to back without branches
+__device__ __forceinline__ double load_or_zero(const double* base, int64_t i, bool ok) {
+  const double v = base[ok ? i : 0];
+  return ok ? v : 0.0;
+}
+
+// Copies n (<= N * blockDim.x) values to LDS with every global load of the thread issued
+// before its first LDS store, so the copy waits for memory once instead of once per
+// element: load(e, ok) returns element e (0 when !ok, from a valid address), put(e, v)
+// stores it.
+template <int N, typename Load, typename Put>
+__device__ __forceinline__ void stage_lds(int n, Load load, Put put) {
+  double v[N];
+#pragma unroll
+  for (int it = 0; it < N; ++it) {
+    const int e = int(threadIdx.x) + it * int(blockDim.x);
+    v[it] = load(e < n ? e : 0, e < n);
+  }
+#pragma unroll
+  for (int it = 0; it < N; ++it) {
+    const int e = int(threadIdx.x) + it * int(blockDim.x);
+    if (e < n) put(e, v[it]);
+  }
+}
+
 // camera_from_world cache: R_cw (row-major) + t_cw.
 __global__ void k_cam_cache(int npose, const double* __restrict__ poses, double* __restrict__ cw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -427,8 +453,8 @@ __device__ __forceinline__ void sub_pair(const T* __restrict__ ri, const T* __re
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; this remaps them so each XCD
-// takes one contiguous range of logical workgroups (neighbouring S rows, which share
-// cameras and therefore V rows, then meet in one XCD's L2).
+// takes one contiguous range of logical workgroups, and neighbouring work items that
+// share inputs meet in one XCD's L2.
 __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
   const int x = bid & 7, i = bid >> 3, q = n >> 3, r = n & 7;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
@@ -519,7 +545,9 @@ __global__ void k_zero_scalars(DevScalars* sc) {
 // trailing C_ik -= L_ij C_kj^T.  Panels of kPanelBlocks pivots are factored by one
 // workgroup per node; the rank-24 trailing update runs as 64x64 tiles over many workgroups.
 
-__device__ __forceinline__ void right_solve6(const double* L, const double* d, double x[6]) {
+// x <- x L^-T D^-1 L^-1 for a 6x6 pivot (unit lower L, dinv = reciprocal pivots, 0 for a
+// zero pivot)
+__device__ __forceinline__ void right_solve6(const double* L, const double* dinv, double x[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     double v = x[c];
@@ -527,7 +555,7 @@ __device__ __forceinline__ void right_solve6(const double* L, const double* d, d
     x[c] = v;
   }
 #pragma unroll
-  for (int c = 0; c < 6; ++c) x[c] *= d[c] == 0.0 ? 0.0 : 1.0 / d[c];
+  for (int c = 0; c < 6; ++c) x[c] *= dinv[c];
 #pragma unroll
   for (int c = 5; c >= 0; --c) {
     double v = x[c];
@@ -546,52 +574,66 @@ __device__ __forceinline__ double (*as_square(double (*t)[kTile + 1]))[kPW + 1] 
 // Factor the np x np diagonal block of the panel at column k0 of one front in LDS (block
 // LDLt with 6x6 pivots and the PSD rule), then store L_PP (strictly below the pivot
 // blocks) into the front and the pivots (L_jj, d_j) into piv_L / piv_d.  Called by all
-// threads of one workgroup.
+// threads of one workgroup.  D holds the block's lower triangle already when `loaded`
+// (else it is read from the front).  sL is LDS scratch for one pivot (L_jj, then d_j in
+// sL[36..41]); sdi receives 1 / d_j.
 __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, int k0, int np,
                                   DevScalars* sc, double (*D)[kPW + 1], double (*Wd)[kPW + 1],
-                                  double* sL, double* sd) {
+                                  double* sL, double* sdi, bool loaded) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < np * np; e += blockDim.x) {
-    const int r = e % np, c = e / np;
-    D[r][c] = r >= c ? A[int64_t(k0 + c) * F + k0 + r] : 0.0;
+  if (!loaded) {
+    stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
+        np * np,
+        [&](int e, bool ok) {
+          const int r = e % np, c = e / np;
+          return load_or_zero(A, int64_t(k0 + c) * F + k0 + r, ok && r >= c);
+        },
+        [&](int e, double v) { D[e % np][e / np] = v; });
+    __syncthreads();
   }
-  __syncthreads();
   for (int j0 = 0; j0 < np; j0 += 6) {
     if (tid == 0) {
-      double a6[36], L[36], d[6];
+      double a6[36], L[36], d[6], di[6];
       for (int a = 0; a < 6; ++a)
         for (int b = 0; b < 6; ++b) a6[6 * a + b] = a >= b ? D[j0 + a][j0 + b] : D[j0 + b][j0 + a];
-      if (!ldlt6(a6, L, d)) {
+      if (!ldlt6(a6, L, d, di)) {
         atomicOr(&sc->bad_pivot, 1);
         for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
-        for (int k = 0; k < 6; ++k) d[k] = 0.0;
+        for (int k = 0; k < 6; ++k) d[k] = di[k] = 0.0;
       }
-      const int gb = f.sn_c0[X] + (k0 + j0) / 6;
-      for (int k = 0; k < 36; ++k) {
-        sL[k] = L[k];
-        f.piv_L[36 * gb + k] = L[k];
-      }
+      for (int k = 0; k < 36; ++k) sL[k] = L[k];
       for (int k = 0; k < 6; ++k) {
-        sd[k] = d[k];
-        f.piv_d[6 * gb + k] = d[k];
+        sL[36 + k] = d[k];
+        sdi[k] = di[k];
       }
     }
     __syncthreads();
-    for (int r = j0 + 6 + tid; r < np; r += blockDim.x) {
+    const int gb = f.sn_c0[X] + (k0 + j0) / 6;
+    if (tid < 36)
+      f.piv_L[36 * gb + tid] = sL[tid];
+    else if (tid < 42)
+      f.piv_d[6 * gb + tid - 36] = sL[tid];
+    const int m = np - j0 - 6;  // rows of the block below this pivot
+    if (m <= 0) break;
+    if (tid < m) {
+      const int r = j0 + 6 + tid;
       double x[6];
 #pragma unroll
       for (int t = 0; t < 6; ++t) {
         x[t] = D[r][j0 + t];
         Wd[r][j0 + t] = x[t];
       }
-      right_solve6(sL, sd, x);
+      right_solve6(sL, sdi, x);
 #pragma unroll
       for (int t = 0; t < 6; ++t) D[r][j0 + t] = x[t];
     }
     __syncthreads();
-    for (int e = tid; e < np * np; e += blockDim.x) {
-      const int r = e % np, c = e / np;
-      if (c < j0 + 6 || r < c) continue;
+    // trailing lower triangle of the block: m (m + 1) / 2 <= kPW (kPW - 5) / 2 entries
+    for (int e = tid; e < m * (m + 1) / 2; e += blockDim.x) {
+      int rr = int((sqrtf(8.0f * float(e) + 1.0f) - 1.0f) * 0.5f);
+      while (rr * (rr + 1) / 2 > e) --rr;
+      while ((rr + 1) * (rr + 2) / 2 <= e) ++rr;
+      const int r = j0 + 6 + rr, c = j0 + 6 + e - rr * (rr + 1) / 2;
       double acc = 0;
 #pragma unroll
       for (int t = 0; t < 6; ++t) acc += D[r][j0 + t] * Wd[c][j0 + t];
@@ -611,100 +653,120 @@ __global__ void __launch_bounds__(kThreads)
     k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc) {
   const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
-  __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[36], sd[6];
-  factor_panel_diag(f, X, f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL, sd);
+  __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[42], sdi[6];
+  factor_panel_diag(f, X, f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL, sdi, false);
 }
 
-// W_r (unscaled) and L_r of one row below the panel from C_rP, L_PP and the pivots:
-// W_r,j = C_r,j - sum_{i<j} W_r,i L_PP(j, i)^T ; L_r,j = W_r,j L_jj^-T D_j^-1 L_jj^-1.
-__device__ __forceinline__ void panel_row(const double* __restrict__ A, int F, int k0, int np,
-                                          int r, const double (*Lpp)[kPW + 1],
-                                          const double (*pL)[36], const double (*pd)[6],
-                                          double w[kPW], double l[kPW]) {
-#pragma unroll
-  for (int c = 0; c < kPW; ++c) w[c] = c < np ? A[int64_t(k0 + c) * F + r] : 0.0;
-#pragma unroll
-  for (int j0 = 6; j0 < kPW; j0 += 6) {
-#pragma unroll
-    for (int a = 0; a < 6; ++a) {
-      double v = w[j0 + a];
-#pragma unroll
-      for (int c = 0; c < j0; ++c) v -= w[c] * Lpp[j0 + a][c];
-      w[j0 + a] = v;
-    }
+// L_PP (strictly below the pivot blocks) and the pivots (L_jj, 1 / d_j; 0 for a zero
+// pivot) of the panel at column k0 into LDS.
+__device__ __forceinline__ void load_panel_factor(const DevFactor& f, int X, const double* Lo,
+                                                  int F, int k0, int np, double (*Lpp)[kPW + 1],
+                                                  double (*pL)[36], double (*pdi)[6]) {
+  stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
+      kPW * kPW,
+      [&](int e, bool ok) {
+        const int rr = e % kPW, c = e / kPW;
+        return load_or_zero(Lo, int64_t(k0 + c) * F + k0 + rr, ok && rr < np && c < np && rr / 6 > c / 6);
+      },
+      [&](int e, double v) { Lpp[e % kPW][e / kPW] = v; });
+  const int gb0 = f.sn_c0[X] + k0 / 6, nbp = np / 6;
+  stage_lds<(kPanelBlocks * 36 + kThreads - 1) / kThreads>(
+      kPanelBlocks * 36,
+      [&](int e, bool ok) {
+        const int jb = e / 36, q = e % 36;
+        const double v = load_or_zero(f.piv_L, int64_t(36) * (gb0 + jb) + q, ok && jb < nbp);
+        return jb < nbp ? v : (q % 7 == 0 ? 1.0 : 0.0);
+      },
+      [&](int e, double v) { pL[e / 36][e % 36] = v; });
+  if (threadIdx.x < kPanelBlocks * 6) {
+    const int jb = threadIdx.x / 6, q = threadIdx.x % 6;
+    const double d = load_or_zero(f.piv_d, int64_t(6) * (gb0 + jb) + q, jb < nbp);
+    pdi[jb][q] = d != 0.0 ? 1.0 / d : 0.0;
   }
+}
+
+// Panel rows of one panel step for the level's nodes, one thread per row below the
+// panel's diagonal block (items = (node, 256-row chunk)), each row formed once:
+//   W_r,j = C_r,j - sum_{i<j} W_r,i L_PP(j, i)^T   (unscaled, into the node's W scratch)
+//   L_r,j = W_r,j L_jj^-T D_j^-1 L_jj^-1           (into the factor)
+// with L_PP and the reciprocal pivots (0 for a zero pivot) staged in LDS.  Each result
+// block is stored as soon as it is formed, which keeps the live set to one row.
+__global__ void __launch_bounds__(kThreads)
+    k_mf_panel(DevFactor f, int panel, const int32_t* __restrict__ items) {
+  const int X = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
+  const double* A = f.front + f.sn_off[X];
+  double* Lo = f.lbuf + f.sn_loff[X];
+  double* Wo = f.wbuf + f.sn_woff[X];
+  const int k0 = kPW * panel, kend = min(K, k0 + kPW), np = kend - k0;
+  const int r = kend + kThreads * chunk + threadIdx.x;
+  __shared__ double Lpp[kPW][kPW + 1];
+  __shared__ double pL[kPanelBlocks][36], pdi[kPanelBlocks][6];
+  // the row's panel entries first (independent of the panel factor)
+  const bool inside = r < F;
+  double w[kPW];
+#pragma unroll
+  for (int c = 0; c < kPW; ++c) w[c] = load_or_zero(A, int64_t(k0 + c) * F + r, inside && c < np);
+  load_panel_factor(f, X, Lo, F, k0, np, Lpp, pL, pdi);
+  __syncthreads();
+  if (!inside) return;
 #pragma unroll
   for (int jb = 0; jb < kPanelBlocks; ++jb) {
+    if (6 * jb >= np) break;
     double x[6];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) x[t] = w[6 * jb + t];
-    right_solve6(pL[jb], pd[jb], x);
+    for (int a = 0; a < 6; ++a) {
+      double v = w[6 * jb + a];
 #pragma unroll
-    for (int t = 0; t < 6; ++t) l[6 * jb + t] = x[t];
+      for (int c = 0; c < 6 * jb; ++c) v -= w[c] * Lpp[6 * jb + a][c];
+      w[6 * jb + a] = v;
+      x[a] = v;
+      Wo[int64_t(6 * jb + a) * F + r] = v;
+    }
+    right_solve6(pL[jb], pdi[jb], x);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) Lo[int64_t(k0 + 6 * jb + a) * F + r] = x[a];
   }
 }
 
-// One panel step of the nodes of a level, one workgroup per 64x64 tile of the lower
-// trailing region: form the tile rows' L and the tile columns' W from the panel
-// columns, store L (tile column 0 only), update A[r, c] -= L_r W_c^T, and (tile (0, 0),
-// which holds the next panel's diagonal block) factor the next panel's diagonal.
+// Trailing update of one panel step, one workgroup per 64x64 tile of the lower trailing
+// region: A[r, c] -= L_r W_c^T with the tile's L rows and W columns staged in LDS; the
+// tile (0, 0), which holds the next panel's diagonal block, then factors that block.
 __global__ void __launch_bounds__(kThreads)
-    k_mf_step(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc) {
-  const int X = tiles[3 * blockIdx.x], tr = tiles[3 * blockIdx.x + 1], tc = tiles[3 * blockIdx.x + 2];
+    k_mf_update(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc) {
+  // consecutive tiles (one tile row shares its L rows) run on one XCD
+  const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int X = tiles[3 * bid], tr = tiles[3 * bid + 1], tc = tiles[3 * bid + 2];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   double* A = f.front + f.sn_off[X];
-  double* Lo = f.lbuf + f.sn_loff[X];
+  const double* Lo = f.lbuf + f.sn_loff[X];
+  const double* Wo = f.wbuf + f.sn_woff[X];
   const int k0 = kPW * panel;
   const int kend = min(K, k0 + kPW);
   const int np = kend - k0;
   const int r0 = kend + kTile * tr, c0 = kend + kTile * tc;
-  const int tid = threadIdx.x;
-  __shared__ double Lpp[kPW][kPW + 1];
-  __shared__ double pL[kPanelBlocks][36], pd[kPanelBlocks][6];
+  const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
   __shared__ double sA[kPW][kTile + 1], sB[kPW][kTile + 1];
-  for (int e = tid; e < kPW * kPW; e += blockDim.x) {
-    const int r = e % kPW, c = e / kPW;
-    Lpp[r][c] = (r < np && c < np && r / 6 > c / 6) ? Lo[int64_t(k0 + c) * F + k0 + r] : 0.0;
-  }
-  for (int e = tid; e < kPanelBlocks * 42; e += blockDim.x) {
-    const int jb = e / 42, q = e % 42;
-    const int gb = f.sn_c0[X] + k0 / 6 + jb;
-    const bool ok = 6 * jb < np;
-    if (q < 36)
-      pL[jb][q] = ok ? f.piv_L[36 * gb + q] : (q % 7 == 0 ? 1.0 : 0.0);
-    else
-      pd[jb][q - 36] = ok ? f.piv_d[6 * gb + q - 36] : 0.0;
-  }
-  __syncthreads();
-  // threads 0..63: tile rows (L into sA, and W into sB on diagonal tiles);
-  // threads 64..127: tile columns (W into sB) on off-diagonal tiles
-  if (tid < 2 * kTile) {
-    const bool is_row = tid < kTile;
-    const int q = is_row ? tid : tid - kTile;
-    const int r = (is_row ? r0 : c0) + q;
-    if (is_row || tr != tc) {
-      double w[kPW], l[kPW];
-      const bool inside = r < F;
-      if (inside) panel_row(A, F, k0, np, r, Lpp, pL, pd, w, l);
+  __shared__ double sL[42], sdi[6];
+  // the tile's current values (read-modify-write) are loaded first
+  double acc[4][4];
 #pragma unroll
-      for (int t = 0; t < kPW; ++t) {
-        const double lv = inside ? l[t] : 0.0, wv = inside ? w[t] : 0.0;
-        if (is_row) {
-          sA[t][q] = lv;
-          if (tr == tc) sB[t][q] = wv;
-        } else {
-          sB[t][q] = wv;
-        }
-      }
-      if (inside && is_row && tc == 0)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int t = 0; t < kPW; ++t)
-          if (t < np) Lo[int64_t(k0 + t) * F + r] = l[t];
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + 4 * ty + i, c = c0 + 4 * tx + j;
+      acc[i][j] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
     }
-  }
+  constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
+  stage_lds<kStage>(
+      np * kTile,
+      [&](int e, bool ok) { return load_or_zero(Lo, int64_t(k0 + e / kTile) * F + r0 + e % kTile, ok && r0 + e % kTile < F); },
+      [&](int e, double v) { sA[e / kTile][e % kTile] = v; });
+  stage_lds<kStage>(
+      np * kTile,
+      [&](int e, bool ok) { return load_or_zero(Wo, int64_t(e / kTile) * F + c0 + e % kTile, ok && c0 + e % kTile < F); },
+      [&](int e, double v) { sB[e / kTile][e % kTile] = v; });
   __syncthreads();
-  const int ty = tid / 16, tx = tid % 16;
-  double acc[4][4] = {};
   for (int t = 0; t < np; ++t) {
     double a[4], b[4];
 #pragma unroll
@@ -715,19 +777,30 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+      for (int j = 0; j < 4; ++j) acc[i][j] -= a[i] * b[j];
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int r = r0 + 4 * ty + i, c = c0 + 4 * tx + j;
-      if (r < F && c < F && r >= c) A[int64_t(c) * F + r] -= acc[i][j];
+      if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[i][j];
     }
   if (tr == 0 && tc == 0 && kend < K) {
-    __syncthreads();  // every update of this tile is in memory before it is re-read
-    factor_panel_diag(f, X, A, F, kend, min(K, kend + kPW) - kend, sc, as_square(sA),
-                      as_square(sB), &pL[0][0], &pd[0][0]);
+    // the next panel's diagonal block is the top-left of this tile: hand the updated
+    // values to LDS directly
+    const int npn = min(K, kend + kPW) - kend;
+    double (*D)[kPW + 1] = as_square(sA);
+    __syncthreads();  // every GEMM read of sA / sB is done
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rl = 4 * ty + i, cl = 4 * tx + j;
+        if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[i][j] : 0.0;
+      }
+    __syncthreads();
+    factor_panel_diag(f, X, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true);
   }
 }
 
@@ -788,12 +861,6 @@ __device__ __forceinline__ void diag_solve6(const double* L, const double* d, do
 constexpr int kTriMax = 6 * 96;  // largest supernode (96 column blocks)
 constexpr int kSP = 32;
 
-// base[i] when ok, else 0; the load is unconditional (from base[0] when !ok, which the
-// caller keeps valid) so a panel's loads issue back to back without branches
-__device__ __forceinline__ double load_or_zero(const double* base, int i, bool ok) {
-  const double v = base[ok ? i : 0];
-  return ok ? v : 0.0;
-}
 
 __global__ void __launch_bounds__(kTriMax)
     k_mf_fwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y,
@@ -1478,8 +1545,10 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
     }
     if (L.tile_count > 0) {
       const size_t li = size_t(&L - S.launches.data());
+      hipLaunchKernelGGL(k_mf_panel, dim3(L.item_count), dim3(kThreads), 0, s, f, L.panel,
+                         f.panel_items + 2 * int64_t(L.item_begin));
       if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
-      hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
+      hipLaunchKernelGGL(k_mf_update, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
                          f.launch_tiles + 3 * int64_t(L.tile_begin), sc);
       if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li + 1], s));
     }

@@ -73,7 +73,10 @@ struct DevFactor {
   const int32_t* ext_nodes;   // extend-add groups (see Symbolic)
   const int64_t* sn_uoff;     // forward-solve update vector offsets
   double* usolve;             // forward-solve update vectors (u_elems)
-  hipEvent_t* step_events;    // optional: 2 per panel launch, recorded around k_mf_step
+  hipEvent_t* step_events;    // optional: 2 per panel launch, recorded around k_mf_update
+  const int32_t* panel_items;  // (node, 256-row chunk) per panel-row workgroup
+  double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
+  const int64_t* sn_woff;
   double* front;   // all fronts, column-major (workspace)
   double* lbuf;    // per node F x K: the factor's L columns (column-major, ld F)
   double* piv_L;   // nb x 36 unit-lower pivot factors (row-major)

@@ -589,12 +589,31 @@ void analyze_rcs(Symbolic& S) {
   // triangle of trailing tiles
   S.launches.clear();
   S.launch_tiles.clear();
+  S.panel_items.clear();
+  S.sn_woff.assign(S.nsn, 0);
+  S.w_elems = 0;
+  for (int32_t x = 0; x < S.nsn; ++x) {
+    S.sn_woff[x] = S.w_elems;
+    S.w_elems += int64_t(6) * S.sn_nf[x] * 6 * kPanelBlocks;
+  }
   for (int32_t l = 0; l < nlev; ++l) {
     int32_t maxpan = 0;
     for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q)
       maxpan = std::max(maxpan, (S.sn_nc[S.lvl_sn[q]] + kPanelBlocks - 1) / kPanelBlocks);
     for (int32_t pn = 0; pn < maxpan; ++pn) {
-      Symbolic::PanelLaunch L{l, pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0, 0.0};
+      Symbolic::PanelLaunch L{l,   pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0,
+                              int32_t(S.panel_items.size() / 2), 0, 0.0, 0.0, 0.0};
+      // every node's tile (0, 0) (which factors the next diagonal block) first, so the
+      // critical path is dispatched first; then the other tiles, node-major, row-major
+      for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
+        const int32_t x = S.lvl_sn[q];
+        const int32_t npan = (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks;
+        const int32_t kend = 6 * std::min(S.sn_nc[x], (pn + 1) * kPanelBlocks);
+        if (pn < npan && 6 * S.sn_nf[x] > kend) {
+          S.launch_tiles.insert(S.launch_tiles.end(), {x, 0, 0});
+          ++L.tile_count;
+        }
+      }
       for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
         const int32_t x = S.lvl_sn[q];
         const int32_t npan = (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks;
@@ -603,11 +622,18 @@ void analyze_rcs(Symbolic& S) {
         const int32_t T = 6 * S.sn_nf[x] - kend;
         const int32_t nt = (T + kTile - 1) / kTile;
         const double np = double(kend - 6 * std::min(S.sn_nc[x], pn * kPanelBlocks));
-        // lower trailing update T(T+1)/2 x np multiply-adds, the panel's triangular
+        // lower trailing update T(T+1)/2 x np multiply-adds; the panel rows' triangular
         // solve (np^2 / 2 multiply-adds per row) and D^-1 scaling (6 np per row)
-        L.flops += double(T) * double(T + 1) * np + double(T) * (np * np + 12.0 * np);
+        L.flops += double(T) * double(T + 1) * np;
+        L.bytes += 16.0 * double(T) * double(T + 1) / 2.0 + 16.0 * double(T) * np;
+        L.panel_flops += double(T) * (np * np + 12.0 * np);
+        for (int32_t ch = 0; ch * 256 < T; ++ch) {
+          S.panel_items.insert(S.panel_items.end(), {x, ch});
+          ++L.item_count;
+        }
         for (int32_t tr = 0; tr < nt; ++tr)
           for (int32_t tc = 0; tc <= tr; ++tc) {
+            if (tr == 0 && tc == 0) continue;  // listed first
             S.launch_tiles.insert(S.launch_tiles.end(), {x, tr, tc});
             ++L.tile_count;
           }

@@ -90,14 +90,20 @@ struct Symbolic {
   std::vector<int32_t> blk_sn;                // permuted block -> supernode
   int64_t front_elems = 0, l_elems = 0;
   int32_t max_front = 0;                      // largest front (blocks)
-  // schedule: nodes per level; one panel launch per (level, panel)
   std::vector<int32_t> lvl_ptr, lvl_sn;
+  // schedule: nodes per level; per (level, panel) one panel-row launch over
+  // (node, 256-row chunk) items and one trailing-update launch over 64x64 tiles
   struct PanelLaunch {
-    int32_t level, panel, node_begin, node_count, tile_begin, tile_count;
-    double flops;  // algorithmic FP64 flops of the launch (trailing update + panel)
+    int32_t level, panel, node_begin, node_count, tile_begin, tile_count, item_begin, item_count;
+    double flops;        // algorithmic FP64 flops of the trailing update
+    double bytes;        // its algorithmic HBM bytes (lower trailing RMW + L and W rows)
+    double panel_flops;  // algorithmic FP64 flops of the panel rows
   };
   std::vector<PanelLaunch> launches;
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
+  std::vector<int32_t> panel_items;   // (node, row chunk) pairs
+  std::vector<int64_t> sn_woff;       // W scratch offsets (6 nf x kPW per node)
+  int64_t w_elems = 0;
   // solve work items per level: (node, 256-row tile below K), (node, 64-column tile of K)
   std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;
   std::vector<int32_t> lvl_max_nc;  // largest node (column blocks) per level
@@ -129,7 +135,7 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
 
 // Panel width (in 6x6 pivot blocks) and trailing-update tile edge (scalars) of the
 // multifrontal factorization.
-constexpr int32_t kPanelBlocks = 4;
+constexpr int32_t kPanelBlocks = 8;
 constexpr int32_t kTile = 64;
 
 }  // namespace sgpu

@@ -194,6 +194,8 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.sn_upos = bufs.up(S.sn_upos);
     df.lvl_sn = bufs.up(S.lvl_sn);
     df.launch_tiles = bufs.up(S.launch_tiles);
+    df.panel_items = bufs.up(S.panel_items);
+    df.sn_woff = bufs.up(S.sn_woff);
     df.solve_rows = bufs.up(S.solve_rows);
     df.solve_cols = bufs.up(S.solve_cols);
     df.ytmp = bufs.alloc<double>(size_t(6) * nb);
@@ -202,6 +204,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.usolve = bufs.alloc<double>(size_t(S.u_elems));
     df.front = bufs.alloc<double>(size_t(S.front_elems));
     df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
+    df.wbuf = bufs.alloc<double>(size_t(S.w_elems));
     df.piv_L = bufs.alloc<double>(size_t(36) * nb);
     df.piv_d = bufs.alloc<double>(size_t(6) * nb);
     df.front_elems = S.front_elems;
