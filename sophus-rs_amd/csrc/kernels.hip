@@ -642,9 +642,15 @@ __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, i
     __syncthreads();
   }
   double* Lout = f.lbuf + f.sn_loff[X];
+  double* Lt = f.ltbuf + f.sn_toff[X];
+  const int K = 6 * f.sn_nc[X];
   for (int e = tid; e < np * np; e += blockDim.x) {
     const int r = e % np, c = e / np;
     if (r / 6 > c / 6) Lout[int64_t(k0 + c) * F + k0 + r] = D[r][c];
+  }
+  for (int e = tid; e < np * np; e += blockDim.x) {  // row-major copy, rows contiguous
+    const int r = e / np, c = e % np;
+    if (r / 6 > c / 6) Lt[int64_t(k0 + r) * K + k0 + c] = D[r][c];
   }
 }
 
@@ -698,6 +704,7 @@ __global__ void __launch_bounds__(kThreads)
   const double* A = f.front + f.sn_off[X];
   double* Lo = f.lbuf + f.sn_loff[X];
   double* Wo = f.wbuf + f.sn_woff[X];
+  double* Lt = f.ltbuf + f.sn_toff[X];
   const int k0 = kPW * panel, kend = min(K, k0 + kPW), np = kend - k0;
   const int r = kend + kThreads * chunk + threadIdx.x;
   __shared__ double Lpp[kPW][kPW + 1];
@@ -726,6 +733,9 @@ __global__ void __launch_bounds__(kThreads)
     right_solve6(pL[jb], pdi[jb], x);
 #pragma unroll
     for (int a = 0; a < 6; ++a) Lo[int64_t(k0 + 6 * jb + a) * F + r] = x[a];
+    if (r < K)  // the node's own rows also go to the row-major copy
+#pragma unroll
+      for (int a = 0; a < 6; ++a) Lt[int64_t(r) * K + k0 + 6 * jb + a] = x[a];
   }
 }
 
@@ -915,21 +925,43 @@ __global__ void __launch_bounds__(kTriMax)
 }
 
 // rows below the node's own: u_X[rows] += L_rect y_K, completing the node's update to
-// its ancestors' rows (children's updates are already in u_X); items = (node, row tile)
+// its ancestors' rows (children's updates are already in u_X); items = (node, tile of
+// kSolveRows rows).  Each row's dot product over the K columns is split over the 4
+// wavefronts (columns g, g + 4, ...) and summed in a fixed order.
 __global__ void __launch_bounds__(kThreads)
     k_mf_fwd_rect(DevFactor f, const int32_t* __restrict__ items, const double* __restrict__ ytmp) {
+  static_assert(kSolveRows == 64, "one row per lane");
   const int X = items[2 * blockIdx.x], tile = items[2 * blockIdx.x + 1];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* A = f.lbuf + f.sn_loff[X];
   double* ux = f.usolve + f.sn_uoff[X];
-  __shared__ double yk[6 * 96];
-  for (int i = threadIdx.x; i < K; i += blockDim.x) yk[i] = ytmp[6 * c0 + i];
+  __shared__ double yk[kTriMax];
+  __shared__ double part[kThreads / 64][64];
+  stage_lds<kTriMax / kThreads + 1>(
+      K, [&](int e, bool ok) { return load_or_zero(ytmp, 6 * c0 + e, ok); },
+      [&](int e, double v) { yk[e] = v; });
   __syncthreads();
-  const int r = K + kThreads * tile + threadIdx.x;
-  if (r >= F) return;
-  double s = 0;
-  for (int c = 0; c < K; ++c) s += A[int64_t(c) * F + r] * yk[c];
-  ux[r] += s;
+  const int q = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int r = K + kSolveRows * tile + q;
+  const bool in = r < F;
+  double s0 = 0.0, s1 = 0.0;
+  for (int cb = g; cb < K; cb += 32) {  // 8 columns (stride 4) per batch of loads
+    double av[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int c = cb + 4 * u;
+      av[u] = load_or_zero(A + int64_t(min(c, K - 1)) * F, r, in && c < K);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      const int c = cb + 4 * u;
+      if (c < K) s0 += av[u] * yk[c];
+      if (c + 4 < K) s1 += av[u + 1] * yk[c + 4];
+    }
+  }
+  part[g][q] = s0 + s1;
+  __syncthreads();
+  if (g == 0 && in) ux[r] += (part[0][q] + part[1][q]) + (part[2][q] + part[3][q]);
 }
 
 // u_P[parent rows] += u_X[rows below K] for one sibling rank of a level (plain adds).
@@ -946,55 +978,67 @@ __global__ void __launch_bounds__(kThreads)
     up_vec[6 * up[r / 6] + r % 6] += ux[r];
 }
 
-// t_K = z_K - L_rect^T x_rows for one tile of 64 node columns (one wavefront per column
-// at a time); items = (node, column tile)
+// t_K = z_K - L_rect^T x_rows for kSolveCols node columns per workgroup (4 columns per
+// wavefront, accumulated side by side; lanes stride over the rows below the node);
+// items = (node, column tile).
 __global__ void __launch_bounds__(kThreads)
     k_mf_bwd_rect(DevFactor f, const int32_t* __restrict__ items, double* __restrict__ y) {
+  static_assert(kSolveCols == 4 * (kThreads / 64), "4 columns per wavefront");
   const int X = items[2 * blockIdx.x], tile = items[2 * blockIdx.x + 1];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* A = f.lbuf + f.sn_loff[X];
   const int32_t* rows = f.sn_rows + f.sn_rows_ptr[X];
-  __shared__ double xr[6 * 512];
+  constexpr int kMaxLds = 6 * 512;
+  __shared__ double xr[kMaxLds];
   const int m = F - K;
-  double* xs = m <= 6 * 512 ? xr : nullptr;
-  if (xs)
-    for (int i = threadIdx.x; i < m; i += blockDim.x) xs[i] = y[6 * rows[(K + i) / 6] + (K + i) % 6];
+  const bool in_lds = m <= kMaxLds;
+  if (in_lds)
+    stage_lds<kMaxLds / kThreads>(
+        m,
+        [&](int e, bool ok) { return load_or_zero(y, 6 * rows[(K + e) / 6] + (K + e) % 6, ok); },
+        [&](int e, double v) { xr[e] = v; });
   __syncthreads();
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  for (int cc = wid; cc < kTile; cc += kThreads / 64) {
-    const int c = kTile * tile + cc;
-    if (c >= K) break;
-    double s = 0;
-    for (int i = lane; i < m; i += 64) {
-      const double xv = xs ? xs[i] : y[6 * rows[(K + i) / 6] + (K + i) % 6];
-      s += A[int64_t(c) * F + K + i] * xv;
-    }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if (lane == 0) y[6 * c0 + c] -= s;
+  const int cb = kSolveCols * tile + 4 * wid;
+  const double* col[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) col[j] = A + int64_t(min(cb + j, K - 1)) * F + K;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int i = lane; i < m; i += 64) {
+    const double xv = in_lds ? xr[i] : y[6 * rows[(K + i) / 6] + (K + i) % 6];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s[j] += col[j][i] * xv;
   }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    for (int off = 32; off > 0; off >>= 1) s[j] += __shfl_down(s[j], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (cb + j < K) y[6 * c0 + cb + j] -= s[j];
 }
 
-// x_K = L_KK^-T t_K: one thread per node column, panels last to first; thread c
-// holds L[k0 .. k0 + 31, c] of the current panel (contiguous in its column).
+// x_K = L_KK^-T t_K: one thread per node column, panels last to first, reading the
+// row-major copy of L_KK (thread c takes L[k0 + t, c]; lanes read consecutive columns).
 __global__ void __launch_bounds__(kTriMax)
     k_mf_bwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y) {
   const int X = nodes[blockIdx.x];
-  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
-  const double* A = f.lbuf + f.sn_loff[X];
+  const int K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  const double* Lt = f.ltbuf + f.sn_toff[X];
   __shared__ double xl[kTriMax];
   const int c = threadIdx.x, wave = c >> 6, cb = c / 6;
   const bool own = c < K;
   double x = own ? y[6 * c0 + c] : 0.0;
   xl[c] = 0.0;
   const int npan = (K + kSP - 1) / kSP;
-  const double* col = A + int64_t(min(c, K - 1)) * F;
   double cur[kSP], nxt[kSP];
   {
     const int k0 = kSP * (npan - 1);
 #pragma unroll
     for (int t = 0; t < kSP; ++t) {
-      const int r = k0 + t;
-      cur[t] = load_or_zero(col, r, own && r < K && r / 6 > cb);
+      const int r = min(k0 + t, K - 1);
+      cur[t] = load_or_zero(Lt + int64_t(r) * K, c, own && k0 + t < K && (k0 + t) / 6 > cb);
     }
   }
   __syncthreads();
@@ -1003,10 +1047,8 @@ __global__ void __launch_bounds__(kTriMax)
     if (p > 0) {
       const int k1 = k0 - kSP;
 #pragma unroll
-      for (int t = 0; t < kSP; ++t) {
-        const int r = k1 + t;
-        nxt[t] = load_or_zero(col, r, own && r / 6 > cb);
-      }
+      for (int t = 0; t < kSP; ++t)
+        nxt[t] = load_or_zero(Lt + int64_t(k1 + t) * K, c, own && (k1 + t) / 6 > cb);
     }
     if (wave == pw) {
 #pragma unroll

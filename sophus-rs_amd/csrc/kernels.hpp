@@ -77,6 +77,8 @@ struct DevFactor {
   const int32_t* panel_items;  // (node, 256-row chunk) per panel-row workgroup
   double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
   const int64_t* sn_woff;
+  double* ltbuf;               // per node K x K: L_KK row-major (backward solve)
+  const int64_t* sn_toff;
   double* front;   // all fronts, column-major (workspace)
   double* lbuf;    // per node F x K: the factor's L columns (column-major, ld F)
   double* piv_L;   // nb x 36 unit-lower pivot factors (row-major)

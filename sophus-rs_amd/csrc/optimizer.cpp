@@ -272,6 +272,7 @@ struct sgpu_optimizer {
     df.launch_tiles = arena.upload(S.launch_tiles, stream);
     df.panel_items = arena.upload(S.panel_items, stream);
     df.sn_woff = arena.upload(S.sn_woff, stream);
+    df.sn_toff = arena.upload(S.sn_toff, stream);
     df.solve_rows = arena.upload(S.solve_rows, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
     df.ytmp = arena.alloc<double>(size_t(6) * S.nb);
@@ -282,6 +283,7 @@ struct sgpu_optimizer {
       df.front = arena.alloc<double>(size_t(S.front_elems));
       df.lbuf = arena.alloc<double>(size_t(S.l_elems));
       df.wbuf = arena.alloc<double>(size_t(S.w_elems));
+      df.ltbuf = arena.alloc<double>(size_t(S.t_elems));
     }
     df.piv_L = arena.alloc<double>(size_t(36) * S.nb);
     df.piv_d = arena.alloc<double>(size_t(6) * S.nb);

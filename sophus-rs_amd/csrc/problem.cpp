@@ -591,10 +591,14 @@ void analyze_rcs(Symbolic& S) {
   S.launch_tiles.clear();
   S.panel_items.clear();
   S.sn_woff.assign(S.nsn, 0);
+  S.sn_toff.assign(S.nsn, 0);
   S.w_elems = 0;
+  S.t_elems = 0;
   for (int32_t x = 0; x < S.nsn; ++x) {
     S.sn_woff[x] = S.w_elems;
     S.w_elems += int64_t(6) * S.sn_nf[x] * 6 * kPanelBlocks;
+    S.sn_toff[x] = S.t_elems;
+    S.t_elems += int64_t(36) * S.sn_nc[x] * S.sn_nc[x];
   }
   for (int32_t l = 0; l < nlev; ++l) {
     int32_t maxpan = 0;
@@ -652,9 +656,9 @@ void analyze_rcs(Symbolic& S) {
       const int32_t x = S.lvl_sn[q];
       S.lvl_max_nc[l] = std::max(S.lvl_max_nc[l], S.sn_nc[x]);
       const int32_t m = 6 * (S.sn_nf[x] - S.sn_nc[x]), K = 6 * S.sn_nc[x];
-      for (int32_t t = 0; t * 256 < m; ++t) S.solve_rows.insert(S.solve_rows.end(), {x, t});
+      for (int32_t t = 0; t * kSolveRows < m; ++t) S.solve_rows.insert(S.solve_rows.end(), {x, t});
       if (m > 0)
-        for (int32_t t = 0; t * kTile < K; ++t) S.solve_cols.insert(S.solve_cols.end(), {x, t});
+        for (int32_t t = 0; t * kSolveCols < K; ++t) S.solve_cols.insert(S.solve_cols.end(), {x, t});
     }
     S.solve_rows_ptr[l + 1] = int32_t(S.solve_rows.size() / 2);
     S.solve_cols_ptr[l + 1] = int32_t(S.solve_cols.size() / 2);

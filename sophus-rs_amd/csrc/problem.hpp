@@ -103,8 +103,11 @@ struct Symbolic {
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
   std::vector<int32_t> panel_items;   // (node, row chunk) pairs
   std::vector<int64_t> sn_woff;       // W scratch offsets (6 nf x kPW per node)
+  std::vector<int64_t> sn_toff;       // row-major copies of each node's L_KK (K x K)
+  int64_t t_elems = 0;
   int64_t w_elems = 0;
-  // solve work items per level: (node, 256-row tile below K), (node, 64-column tile of K)
+  // solve work items per level: (node, kSolveRows-row tile below K), (node,
+  // kSolveCols-column tile of K)
   std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;
   std::vector<int32_t> lvl_max_nc;  // largest node (column blocks) per level
   // extend-add groups: per level, nodes grouped by their rank among their parent's
@@ -137,5 +140,8 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
 // multifrontal factorization.
 constexpr int32_t kPanelBlocks = 8;
 constexpr int32_t kTile = 64;
+// forward-solve row tile and backward-solve column tile of the rows below a node
+constexpr int32_t kSolveRows = 64;
+constexpr int32_t kSolveCols = 16;
 
 }  // namespace sgpu

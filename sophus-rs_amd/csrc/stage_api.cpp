@@ -196,6 +196,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.launch_tiles = bufs.up(S.launch_tiles);
     df.panel_items = bufs.up(S.panel_items);
     df.sn_woff = bufs.up(S.sn_woff);
+    df.sn_toff = bufs.up(S.sn_toff);
     df.solve_rows = bufs.up(S.solve_rows);
     df.solve_cols = bufs.up(S.solve_cols);
     df.ytmp = bufs.alloc<double>(size_t(6) * nb);
@@ -205,6 +206,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.front = bufs.alloc<double>(size_t(S.front_elems));
     df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
     df.wbuf = bufs.alloc<double>(size_t(S.w_elems));
+    df.ltbuf = bufs.alloc<double>(size_t(S.t_elems));
     df.piv_L = bufs.alloc<double>(size_t(36) * nb);
     df.piv_d = bufs.alloc<double>(size_t(6) * nb);
     df.front_elems = S.front_elems;
