@@ -740,10 +740,17 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Trailing update of one panel step, one workgroup per 64x64 tile of the lower trailing
-// region: A[r, c] -= L_r W_c^T with the tile's L rows and W columns staged in LDS; the
-// tile (0, 0), which holds the next panel's diagonal block, then factors that block.
+// region: A[r, c] -= L_r W_c^T on the matrix cores.  The tile's L rows and W columns are
+// staged in LDS (zero past the panel's width); each wavefront owns a 32x32 quadrant as
+// 2 x 2 tiles of v_mfma_f64_16x16x4, computing (-W) L^T so that each lane's results run
+// down 16 consecutive rows of the column-major front (coalesced read-modify-write).  The
+// accumulators start from the tile's current values.  The tile (0, 0), which holds the
+// next panel's diagonal block, then factors that block.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+
 __global__ void __launch_bounds__(kThreads)
     k_mf_update(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc) {
+  static_assert(kTile == 64 && kThreads == 256 && kPW % 4 == 0, "4 wavefronts of 32x32");
   // consecutive tiles (one tile row shares its L rows) run on one XCD
   const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
   const int X = tiles[3 * bid], tr = tiles[3 * bid + 1], tc = tiles[3 * bid + 2];
@@ -755,60 +762,78 @@ __global__ void __launch_bounds__(kThreads)
   const int kend = min(K, k0 + kPW);
   const int np = kend - k0;
   const int r0 = kend + kTile * tr, c0 = kend + kTile * tc;
-  const int tid = threadIdx.x, ty = tid / 16, tx = tid % 16;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lc = lane >> 4;
+  const int rb = 32 * (wv >> 1), cb = 32 * (wv & 1);  // the wavefront's quadrant
   __shared__ double sA[kPW][kTile + 1], sB[kPW][kTile + 1];
   __shared__ double sL[42], sdi[6];
-  // the tile's current values (read-modify-write) are loaded first
-  double acc[4][4];
+  // accumulator (mi, nj) component q is the tile element
+  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+  // it starts from the element's current value (loaded first)
+  mfma_d4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = r0 + 4 * ty + i, c = c0 + 4 * tx + j;
-      acc[i][j] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
-    }
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+      }
   constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
   stage_lds<kStage>(
-      np * kTile,
-      [&](int e, bool ok) { return load_or_zero(Lo, int64_t(k0 + e / kTile) * F + r0 + e % kTile, ok && r0 + e % kTile < F); },
+      kPW * kTile,
+      [&](int e, bool ok) {
+        const int t = e / kTile, r = r0 + e % kTile;
+        return load_or_zero(Lo, int64_t(k0 + t) * F + r, ok && t < np && r < F);
+      },
       [&](int e, double v) { sA[e / kTile][e % kTile] = v; });
   stage_lds<kStage>(
-      np * kTile,
-      [&](int e, bool ok) { return load_or_zero(Wo, int64_t(e / kTile) * F + c0 + e % kTile, ok && c0 + e % kTile < F); },
+      kPW * kTile,
+      [&](int e, bool ok) {
+        const int t = e / kTile, c = c0 + e % kTile;
+        return load_or_zero(Wo, int64_t(t) * F + c, ok && t < np && c < F);
+      },
       [&](int e, double v) { sB[e / kTile][e % kTile] = v; });
   __syncthreads();
-  for (int t = 0; t < np; ++t) {
-    double a[4], b[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a[i] = sA[t][4 * ty + i];
-      b[i] = sB[t][4 * tx + i];
-    }
+  for (int s = 0; s < kPW / 4; ++s) {
+    const int t = 4 * s + lc;  // this lane's k index
+    double a[2], b[2];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] -= a[i] * b[j];
+    for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = r0 + 4 * ty + i, c = c0 + 4 * tx + j;
-      if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[i][j];
-    }
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
+      }
   if (tr == 0 && tc == 0 && kend < K) {
     // the next panel's diagonal block is the top-left of this tile: hand the updated
     // values to LDS directly
     const int npn = min(K, kend + kPW) - kend;
     double (*D)[kPW + 1] = as_square(sA);
-    __syncthreads();  // every GEMM read of sA / sB is done
+    __syncthreads();  // every MFMA operand read of sA / sB is done
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int rl = 4 * ty + i, cl = 4 * tx + j;
-        if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[i][j] : 0.0;
-      }
+      for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
+          if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+        }
     __syncthreads();
     factor_panel_diag(f, X, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true);
   }
