@@ -567,6 +567,7 @@ __device__ __forceinline__ void right_solve6(const double* L, const double* dinv
 constexpr int kPW = 6 * kPanelBlocks;
 
 // View a [kPW][kTile + 1] LDS tile as a [kPW][kPW + 1] square (kPW <= kTile).
+static_assert(kPW <= kTile, "the next diagonal block lies inside tile (0, 0)");
 __device__ __forceinline__ double (*as_square(double (*t)[kTile + 1]))[kPW + 1] {
   return reinterpret_cast<double (*)[kPW + 1]>(&t[0][0]);
 }
@@ -609,10 +610,31 @@ __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, i
     }
     __syncthreads();
     const int gb = f.sn_c0[X] + (k0 + j0) / 6;
-    if (tid < 36)
+    if (tid < 36) {
       f.piv_L[36 * gb + tid] = sL[tid];
-    else if (tid < 42)
+    } else if (tid < 42) {
       f.piv_d[6 * gb + tid - 36] = sL[tid];
+    } else if (tid >= 64 && tid < 100) {
+      // the pivot's inverse M = L^-T D^+ L^-1 (entry (a, b)) from the columns of L^-1
+      const int a = (tid - 64) / 6, b = (tid - 64) % 6;
+      double ua[6], ub[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        ua[k] = k == a ? 1.0 : 0.0;
+        ub[k] = k == b ? 1.0 : 0.0;
+      }
+#pragma unroll
+      for (int k = 1; k < 6; ++k)
+#pragma unroll
+        for (int t = 0; t < k; ++t) {
+          ua[k] -= sL[6 * k + t] * ua[t];
+          ub[k] -= sL[6 * k + t] * ub[t];
+        }
+      double m = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) m += ua[k] * sdi[k] * ub[k];
+      f.piv_M[36 * gb + 6 * a + b] = m;
+    }
     const int m = np - j0 - 6;  // rows of the block below this pivot
     if (m <= 0) break;
     if (tid < m) {
@@ -691,51 +713,79 @@ __device__ __forceinline__ void load_panel_factor(const DevFactor& f, int X, con
   }
 }
 
-// Panel rows of one panel step for the level's nodes, one thread per row below the
-// panel's diagonal block (items = (node, 256-row chunk)), each row formed once:
+// Panel rows of one panel step for the level's nodes, four lanes per row below the
+// panel's diagonal block (items = (node, kPanelRows-row chunk)), each row formed once:
 //   W_r,j = C_r,j - sum_{i<j} W_r,i L_PP(j, i)^T   (unscaled, into the node's W scratch)
-//   L_r,j = W_r,j L_jj^-T D_j^-1 L_jj^-1           (into the factor)
-// with L_PP and the reciprocal pivots (0 for a zero pivot) staged in LDS.  Each result
-// block is stored as soon as it is formed, which keeps the live set to one row.
+//   L_r,j = W_r,j M_j,  M_j = L_jj^-T D_j^-1 L_jj^-1 (into the factor)
+// Lane s of a row holds the columns c = s (mod 4): each block's sums are split over the
+// four lanes and combined with two xor shuffles (fixed order); the lane owning a column
+// keeps and stores it.  L_PP and the M_j are staged in LDS.
 __global__ void __launch_bounds__(kThreads)
     k_mf_panel(DevFactor f, int panel, const int32_t* __restrict__ items) {
+  static_assert(kPanelRows * 4 == kThreads && kPW % 4 == 0, "four lanes per row");
   const int X = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   const double* A = f.front + f.sn_off[X];
+  const double* Lg = f.lbuf + f.sn_loff[X];
   double* Lo = f.lbuf + f.sn_loff[X];
   double* Wo = f.wbuf + f.sn_woff[X];
   double* Lt = f.ltbuf + f.sn_toff[X];
   const int k0 = kPW * panel, kend = min(K, k0 + kPW), np = kend - k0;
-  const int r = kend + kThreads * chunk + threadIdx.x;
-  __shared__ double Lpp[kPW][kPW + 1];
-  __shared__ double pL[kPanelBlocks][36], pdi[kPanelBlocks][6];
-  // the row's panel entries first (independent of the panel factor)
+  const int s = threadIdx.x & 3;
+  const int r = kend + kPanelRows * chunk + (threadIdx.x >> 2);
   const bool inside = r < F;
-  double w[kPW];
+  __shared__ double Lpp[kPW][kPW + 1];
+  __shared__ double pM[kPanelBlocks][36];
+  // this lane's columns first (independent of the panel factor)
+  double w[kPW / 4];
 #pragma unroll
-  for (int c = 0; c < kPW; ++c) w[c] = load_or_zero(A, int64_t(k0 + c) * F + r, inside && c < np);
-  load_panel_factor(f, X, Lo, F, k0, np, Lpp, pL, pdi);
+  for (int u = 0; u < kPW / 4; ++u)
+    w[u] = load_or_zero(A, int64_t(k0 + s + 4 * u) * F + r, inside && s + 4 * u < np);
+  stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
+      kPW * kPW,
+      [&](int e, bool ok) {
+        const int rr = e % kPW, c = e / kPW;
+        return load_or_zero(Lg, int64_t(k0 + c) * F + k0 + rr, ok && rr < np && c < np && rr / 6 > c / 6);
+      },
+      [&](int e, double v) { Lpp[e % kPW][e / kPW] = v; });
+  const int gb0 = f.sn_c0[X] + k0 / 6, nbp = np / 6;
+  stage_lds<(kPanelBlocks * 36 + kThreads - 1) / kThreads>(
+      kPanelBlocks * 36,
+      [&](int e, bool ok) { return load_or_zero(f.piv_M, int64_t(36) * gb0 + e, ok && e / 36 < nbp); },
+      [&](int e, double v) { pM[e / 36][e % 36] = v; });
   __syncthreads();
-  if (!inside) return;
+  if (!inside) return;  // all four lanes of a row leave together
 #pragma unroll
   for (int jb = 0; jb < kPanelBlocks; ++jb) {
-    if (6 * jb >= np) break;
-    double x[6];
+    if (jb >= nbp) break;
+    double p[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
-      double v = w[6 * jb + a];
+      const int col = 6 * jb + a;
+      double acc = (col & 3) == s ? w[col >> 2] : 0.0;
 #pragma unroll
-      for (int c = 0; c < 6 * jb; ++c) v -= w[c] * Lpp[6 * jb + a][c];
-      w[6 * jb + a] = v;
-      x[a] = v;
-      Wo[int64_t(6 * jb + a) * F + r] = v;
+      for (int u = 0; u < kPW / 4; ++u)
+        if (4 * u < 6 * jb && s + 4 * u < 6 * jb) acc -= w[u] * Lpp[col][s + 4 * u];
+      p[a] = acc;
     }
-    right_solve6(pL[jb], pdi[jb], x);
 #pragma unroll
-    for (int a = 0; a < 6; ++a) Lo[int64_t(k0 + 6 * jb + a) * F + r] = x[a];
-    if (r < K)  // the node's own rows also go to the row-major copy
+    for (int a = 0; a < 6; ++a) {
+      p[a] += __shfl_xor(p[a], 1, 64);
+      p[a] += __shfl_xor(p[a], 2, 64);
+    }
 #pragma unroll
-      for (int a = 0; a < 6; ++a) Lt[int64_t(r) * K + k0 + 6 * jb + a] = x[a];
+    for (int a = 0; a < 6; ++a) {
+      const int col = 6 * jb + a;
+      if ((col & 3) == s) {
+        w[col >> 2] = p[a];
+        Wo[int64_t(col) * F + r] = p[a];
+        double l = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) l += p[t] * pM[jb][6 * t + a];
+        Lo[int64_t(k0 + col) * F + r] = l;
+        if (r < K) Lt[int64_t(r) * K + k0 + col] = l;  // the node's own rows: row-major copy
+      }
+    }
   }
 }
 

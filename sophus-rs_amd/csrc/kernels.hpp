@@ -74,7 +74,7 @@ struct DevFactor {
   const int64_t* sn_uoff;     // forward-solve update vector offsets
   double* usolve;             // forward-solve update vectors (u_elems)
   hipEvent_t* step_events;    // optional: 2 per panel launch, recorded around k_mf_update
-  const int32_t* panel_items;  // (node, 256-row chunk) per panel-row workgroup
+  const int32_t* panel_items;  // (node, kPanelRows-row chunk) per panel-row workgroup
   double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
   const int64_t* sn_woff;
   double* ltbuf;               // per node K x K: L_KK row-major (backward solve)
@@ -83,6 +83,7 @@ struct DevFactor {
   double* lbuf;    // per node F x K: the factor's L columns (column-major, ld F)
   double* piv_L;   // nb x 36 unit-lower pivot factors (row-major)
   double* piv_d;   // nb x 6
+  double* piv_M;   // nb x 36: (L_jj D_j L_jj^T)^+ = L_jj^-T D_j^-1 L_jj^-1 (0 rows for zero pivots)
   int64_t front_elems;
   const Symbolic* host;  // launch schedule
 };

@@ -287,6 +287,7 @@ struct sgpu_optimizer {
     }
     df.piv_L = arena.alloc<double>(size_t(36) * S.nb);
     df.piv_d = arena.alloc<double>(size_t(6) * S.nb);
+    df.piv_M = arena.alloc<double>(size_t(36) * S.nb);
     df.front_elems = S.front_elems;
     df.host = &S;
     // workspace

@@ -631,7 +631,7 @@ void analyze_rcs(Symbolic& S) {
         L.flops += double(T) * double(T + 1) * np;
         L.bytes += 16.0 * double(T) * double(T + 1) / 2.0 + 16.0 * double(T) * np;
         L.panel_flops += double(T) * (np * np + 12.0 * np);
-        for (int32_t ch = 0; ch * 256 < T; ++ch) {
+        for (int32_t ch = 0; ch * kPanelRows < T; ++ch) {
           S.panel_items.insert(S.panel_items.end(), {x, ch});
           ++L.item_count;
         }

@@ -101,7 +101,7 @@ struct Symbolic {
   };
   std::vector<PanelLaunch> launches;
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
-  std::vector<int32_t> panel_items;   // (node, row chunk) pairs
+  std::vector<int32_t> panel_items;   // (node, kPanelRows-row chunk) pairs
   std::vector<int64_t> sn_woff;       // W scratch offsets (6 nf x kPW per node)
   std::vector<int64_t> sn_toff;       // row-major copies of each node's L_KK (K x K)
   int64_t t_elems = 0;
@@ -142,6 +142,8 @@ constexpr int32_t kPanelBlocks = 8;
 constexpr int32_t kTile = 64;
 // forward-solve row tile and backward-solve column tile of the rows below a node
 constexpr int32_t kSolveRows = 64;
+// rows per panel-row workgroup (4 lanes per row)
+constexpr int32_t kPanelRows = 64;
 constexpr int32_t kSolveCols = 16;
 
 }  // namespace sgpu

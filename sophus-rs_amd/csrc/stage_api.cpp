@@ -209,6 +209,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.ltbuf = bufs.alloc<double>(size_t(S.t_elems));
     df.piv_L = bufs.alloc<double>(size_t(36) * nb);
     df.piv_d = bufs.alloc<double>(size_t(6) * nb);
+    df.piv_M = bufs.alloc<double>(size_t(36) * nb);
     df.front_elems = S.front_elems;
     df.host = &S;
     double* d_blocks = bufs.up(blocks);
