@@ -478,6 +478,7 @@ __global__ void __launch_bounds__(kThreads)
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+#pragma unroll 2
   for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
     const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
     sub_pair(vobs + kObsRec * int64_t(i), vobs + kObsRec * int64_t(j), acc);
@@ -1079,7 +1080,7 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
   for (int j = 0; j < 4; ++j) col[j] = A + int64_t(min(cb + j, K - 1)) * F + K;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
+#pragma unroll 2
   for (int i = lane; i < m; i += 64) {
     const double xv = in_lds ? xr[i] : y[6 * rows[(K + i) / 6] + (K + i) % 6];
 #pragma unroll
