@@ -1080,7 +1080,7 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
   for (int j = 0; j < 4; ++j) col[j] = A + int64_t(min(cb + j, K - 1)) * F + K;
   double s[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 2
+#pragma unroll 4
   for (int i = lane; i < m; i += 64) {
     const double xv = in_lds ? xr[i] : y[6 * rows[(K + i) / 6] + (K + i) % 6];
 #pragma unroll
