@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "api_internal.hpp"
 #include "block6.hpp"
@@ -139,11 +140,16 @@ __global__ void __launch_bounds__(kThreads)
     k_point_prep(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
                  const double* __restrict__ cw, double nu, double* __restrict__ hinv_out,
                  double* __restrict__ hll_out, double* __restrict__ s_out,
-                 double* __restrict__ y_out, DevScalars* sc) {
+                 double* __restrict__ y_out, double* __restrict__ prec, DevScalars* sc) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= p.npt) return;
-  if (p.pt_block[b] < 0) return;
   const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
+  double* rec = prec + 12 * int64_t(b);
+  if (p.pt_block[b] < 0) {  // constant point: H_ll^-1 = 0 and y = 0 in its record
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rec[k] = k < 3 ? x[k] : 0.0;
+    return;
+  }
   T H[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   T s[3] = {T(0), T(0), T(0)};
   for (int o = p.pt_ptr[b]; o < p.pt_ptr[b + 1]; ++o) {
@@ -185,6 +191,13 @@ __global__ void __launch_bounds__(kThreads)
     inv[8] = (m11 * m22 - m21 * m12) / det;
   }
   for (int k = 0; k < 9; ++k) hinv_out[9 * b + k] = inv[k];
+  double yv[3];
+  for (int k = 0; k < 3; ++k)
+    yv[k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) + inv[3 * k + 2] * double(s[2]);
+  const double r12[12] = {x[0],   x[1],   x[2],   inv[0], inv[1], inv[2],
+                          inv[4], inv[5], inv[8], yv[0],  yv[1],  yv[2]};
+#pragma unroll
+  for (int k = 0; k < 12; ++k) rec[k] = r12[k];
   hll_out[6 * b + 0] = m11;
   hll_out[6 * b + 1] = m12;
   hll_out[6 * b + 2] = m13;
@@ -321,87 +334,133 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_cam_obs(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
               const double* __restrict__ cw, const double* __restrict__ hinv_g,
-              const double* __restrict__ hll_g, const double* __restrict__ y_g, int centered,
-              T* __restrict__ vobs, double* __restrict__ own_part) {
+              const double* __restrict__ hll_g, const double* __restrict__ y_g,
+              const double* __restrict__ prec, int centered, T* __restrict__ vobs,
+              double* __restrict__ own_part) {
   constexpr int kOwn = 48;  // 36 block entries, 6 reduced rhs, 6 gradient
   __shared__ T sv[kThreads * kObsRec];
   __shared__ double red[kThreads / 64][kOwn];
   const int ch = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int q0 = p.own_chunks[3 * ch + 1], q1 = p.own_chunks[3 * ch + 2];
+  const int cam = p.own_chunks[3 * ch], q0 = p.own_chunks[3 * ch + 1], q1 = p.own_chunks[3 * ch + 2];
   const int q = q0 + tid;
   double own[kOwn];
 #pragma unroll
   for (int e = 0; e < kOwn; ++e) own[e] = 0.0;
   if (q < q1) {
-    const int o = p.cam_obs[q];
-    const int b = p.obs_point[o];
-    const bool active = p.pt_block[b] >= 0;
-    const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
-    const double* c = cw + 12 * p.obs_pose[o];
-    const double* ini = intr + 4 * p.obs_intr[o];
-    double xl[3] = {xd[0], xd[1], xd[2]};  // lift point (relative to the camera if centered)
-    if (centered) {
-      double cc[3];
-      cam_center(c, cc);
-#pragma unroll
-      for (int e = 0; e < 3; ++e) xl[e] -= cc[e];
-    }
-    double xc[3], Ad[6], Md[9], v[3];
-    to_camera(c, xd, xc);
-    obs_jacobian<double>(c, ini, xc, Ad);
-    gram(Ad, Md);
-    obs_residual<double>(ini, p.obs_uv + 2 * o, xc, p.robust, p.delta, Ad, v);
-    // record: U_o row-major, then the lift point (zero for a constant point: it has no
-    // cross pairs)
+    // camera-major observation fields; one camera per chunk
+    const int b = p.cm_point[q];
+    const double* c = cw + 12 * p.cam_member[cam];
+    const double* ini = intr + 4 * p.cm_intr[q];
+    const double* uvq = p.cm_uv + 2 * int64_t(q);
     T* vs = sv + kObsRec * tid;
-    if (active) {
-      const double* h = hinv_g + 9 * b;
-      // lower Cholesky factor of H_ll^-1 (zero columns past a non-positive pivot)
-      const double l00 = sqrt(fmax(h[0], 0.0)), i00 = l00 > 0.0 ? 1.0 / l00 : 0.0;
-      const double l10 = h[3] * i00, l20 = h[6] * i00;
-      const double l11 = sqrt(fmax(h[4] - l10 * l10, 0.0)), i11 = l11 > 0.0 ? 1.0 / l11 : 0.0;
-      const double l21 = (h[7] - l20 * l10) * i11;
-      const double l22 = sqrt(fmax(h[8] - l20 * l20 - l21 * l21, 0.0));
+    double xd[3], xl[3], xc[3], Ad[6], Md[9], v[3], t[3];
+    if constexpr (std::is_same<T, double>::value) {
+      // packed point record: x, H_ll^-1 upper, y (H_ll^-1 = y = 0 for a constant point)
+      double rc[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) rc[k] = prec[12 * int64_t(b) + k];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) xd[e] = xl[e] = rc[e];
+      if (centered) {
+        double cc[3];
+        cam_center(c, cc);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) xl[e] -= cc[e];
+      }
+      to_camera(c, xd, xc);
+      obs_jacobian<double>(c, ini, xc, Ad);
+      gram(Ad, Md);
+      obs_residual<double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
+      // U = L^T M with H_ll^-1 = L L^T (zero columns past a non-positive pivot)
+      const double l00 = sqrt(fmax(rc[3], 0.0)), i00 = l00 > 0.0 ? 1.0 / l00 : 0.0;
+      const double l10 = rc[4] * i00, l20 = rc[5] * i00;
+      const double l11 = sqrt(fmax(rc[6] - l10 * l10, 0.0)), i11 = l11 > 0.0 ? 1.0 / l11 : 0.0;
+      const double l21 = (rc[7] - l20 * l10) * i11;
+      const double l22 = sqrt(fmax(rc[8] - l20 * l20 - l21 * l21, 0.0));
+      double U[9];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        vs[j] = T(l00 * Md[j] + l10 * Md[3 + j] + l20 * Md[6 + j]);
-        vs[3 + j] = T(l11 * Md[3 + j] + l21 * Md[6 + j]);
-        vs[6 + j] = T(l22 * Md[6 + j]);
-        vs[9 + j] = T(xl[j]);
+        U[j] = l00 * Md[j] + l10 * Md[3 + j] + l20 * Md[6 + j];
+        U[3 + j] = l11 * Md[3 + j] + l21 * Md[6 + j];
+        U[6 + j] = l22 * Md[6 + j];
       }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) vs[k] = U[k];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) vs[9 + e] = xl[e];
+      // own term: M - M H_ll^-1 M = M - U^T U;  t = M y - v
+      double N[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          N[3 * i + j] = Md[3 * i + j] - (U[i] * U[j] + U[3 + i] * U[3 + j] + U[6 + i] * U[6 + j]);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * rc[9] + Md[3 * r + 1] * rc[10] + Md[3 * r + 2] * rc[11] - v[r];
+      add_lift(xl, xl, N, 1.0, [&](int e, double val) { own[e] = val; });
     } else {
+      const bool active = p.pt_block[b] >= 0;
 #pragma unroll
-      for (int e = 0; e < kObsRec; ++e) vs[e] = T(0);
-    }
-    // own (o, o) term
-    const T x[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
-    T Ai[6], Mi[9], N[9];
+      for (int e = 0; e < 3; ++e) xd[e] = xl[e] = points[3 * b + e];
+      if (centered) {
+        double cc[3];
+        cam_center(c, cc);
 #pragma unroll
-    for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
-    gram(Ai, Mi);
-    double t[3];
-    if (active) {
-      const double* yb = y_g + 3 * b;
-#pragma unroll
-      for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * yb[0] + Md[3 * r + 1] * yb[1] + Md[3 * r + 2] * yb[2] - v[r];
-      // R = H_ll - M_o (FP64, then to T), N = M_o H_ll^-1 R
-      const double* H = hll_g + 6 * b;
-      const double Hf[9] = {H[0], H[1], H[2], H[1], H[3], H[4], H[2], H[4], H[5]};
-      T R[9], Hi[9], Z[9];
-#pragma unroll
-      for (int e = 0; e < 9; ++e) {
-        R[e] = T(Hf[e] - Md[e]);
-        Hi[e] = T(hinv_g[9 * b + e]);
+        for (int e = 0; e < 3; ++e) xl[e] -= cc[e];
       }
-      mat3_mul_t(Mi, Hi, Z);
-      mat3_mul_t(Z, R, N);
-    } else {  // constant point: own H_pp term only
+      to_camera(c, xd, xc);
+      obs_jacobian<double>(c, ini, xc, Ad);
+      gram(Ad, Md);
+      obs_residual<double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
+      // record: U_o row-major, then the lift point (zero for a constant point: it has no
+      // cross pairs)
+      if (active) {
+        const double* h = hinv_g + 9 * b;
+        const double l00 = sqrt(fmax(h[0], 0.0)), i00 = l00 > 0.0 ? 1.0 / l00 : 0.0;
+        const double l10 = h[3] * i00, l20 = h[6] * i00;
+        const double l11 = sqrt(fmax(h[4] - l10 * l10, 0.0)), i11 = l11 > 0.0 ? 1.0 / l11 : 0.0;
+        const double l21 = (h[7] - l20 * l10) * i11;
+        const double l22 = sqrt(fmax(h[8] - l20 * l20 - l21 * l21, 0.0));
 #pragma unroll
-      for (int r = 0; r < 3; ++r) t[r] = -v[r];
+        for (int j = 0; j < 3; ++j) {
+          vs[j] = T(l00 * Md[j] + l10 * Md[3 + j] + l20 * Md[6 + j]);
+          vs[3 + j] = T(l11 * Md[3 + j] + l21 * Md[6 + j]);
+          vs[6 + j] = T(l22 * Md[6 + j]);
+          vs[9 + j] = T(xl[j]);
+        }
+      } else {
 #pragma unroll
-      for (int e = 0; e < 9; ++e) N[e] = Mi[e];
+        for (int e = 0; e < kObsRec; ++e) vs[e] = T(0);
+      }
+      // own (o, o) term in T with the FP64 difference R = H_ll - M_o:
+      // N = M_o H_ll^-1 R (M_o alone for a constant point)
+      const T x[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
+      T Ai[6], Mi[9], N[9];
+#pragma unroll
+      for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
+      gram(Ai, Mi);
+      if (active) {
+        const double* yb = y_g + 3 * b;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * yb[0] + Md[3 * r + 1] * yb[1] + Md[3 * r + 2] * yb[2] - v[r];
+        const double* H = hll_g + 6 * b;
+        const double Hf[9] = {H[0], H[1], H[2], H[1], H[3], H[4], H[2], H[4], H[5]};
+        T R[9], Hi[9], Z[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+          R[e] = T(Hf[e] - Md[e]);
+          Hi[e] = T(hinv_g[9 * b + e]);
+        }
+        mat3_mul_t(Mi, Hi, Z);
+        mat3_mul_t(Z, R, N);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) t[r] = -v[r];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) N[e] = Mi[e];
+      }
+      add_lift(x, x, N, T(1), [&](int e, T val) { own[e] = double(val); });
     }
-    add_lift(x, x, N, T(1), [&](int e, T val) { own[e] = double(val); });
     // P^T t = [x cross t; t]
     own[36] = xl[1] * t[2] - xl[2] * t[1];
     own[37] = xl[2] * t[0] - xl[0] * t[2];
@@ -1598,10 +1657,10 @@ void launch_point_prep(hipStream_t s, int precision, const DevProblem& p, const 
   const dim3 g(blocks_for(p.npt, kThreads)), b(kThreads);
   if (precision == 0)
     hipLaunchKernelGGL(k_point_prep<double>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.hll,
-                       w.pt_s, w.pt_y, sc);
+                       w.pt_s, w.pt_y, w.prec, sc);
   else
     hipLaunchKernelGGL(k_point_prep<float>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.hll,
-                       w.pt_s, w.pt_y, sc);
+                       w.pt_s, w.pt_y, w.prec, sc);
 }
 
 void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const double* points,
@@ -1615,14 +1674,14 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
     double* v = static_cast<double*>(w.vobs);
     if (p.n_own_chunks > 0)
       hipLaunchKernelGGL(k_cam_obs<double>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.hinv,
-                         w.hll, w.pt_y, 0, v, w.own_part);
+                         w.hll, w.pt_y, w.prec, 0, v, w.own_part);
     hipLaunchKernelGGL(k_schur_blocks<double>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 0,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   } else {
     float* v = static_cast<float*>(w.vobs);
     if (p.n_own_chunks > 0)
       hipLaunchKernelGGL(k_cam_obs<float>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.hinv,
-                         w.hll, w.pt_y, 1, v, w.own_part);
+                         w.hll, w.pt_y, w.prec, 1, v, w.own_part);
     hipLaunchKernelGGL(k_schur_blocks<float>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 1,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   }

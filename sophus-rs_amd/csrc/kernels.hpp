@@ -42,6 +42,10 @@ struct DevProblem {
   const int64_t* pair_ptr;  // slot -> cross-pair range
   const int32_t* pairs;     // (q_i, q_j) per cross pair: positions in cam_obs
   const int32_t* own_chunks;     // (camera, q0, q1) per own-term chunk
+  // camera-major copies of the observations' point, intrinsics and uv (positions q of cam_obs)
+  const int32_t* cm_point;
+  const int32_t* cm_intr;
+  const double* cm_uv;
   const int32_t* cam_chunk_ptr;  // nb + 1
   int32_t n_own_chunks;
   int64_t nslot;
@@ -93,6 +97,8 @@ struct DevLin {
   double* cw;         // npose x 12: camera_from_world rotation (row-major) + translation
   double* hinv;       // npt x 9
   double* hll;        // npt x 6: damped H_ll (00, 01, 02, 11, 12, 22)
+  double* prec;       // npt x 12 packed point record: x (3), H_ll^-1 upper (00, 01, 02, 11, 12,
+                      // 22), y (3); zeros for a constant point's H_ll^-1 and y
   double* pt_s;       // npt x 3: neg-gradient of the point (sum of A^T r~)
   double* pt_y;       // npt x 3: hinv * s
   double* rhs_nat;    // nb x 6: reduced rhs in camera order

@@ -236,6 +236,21 @@ struct sgpu_optimizer {
     dp.cam_member = arena.upload(P.cam_member, stream);
     dp.cam_ptr = arena.upload(S.cam_ptr, stream);
     dp.cam_obs = arena.upload(S.cam_obs, stream);
+    {
+      const size_t ncm = S.cam_obs.size();
+      std::vector<int32_t> cm_point(ncm), cm_intr(ncm);
+      std::vector<double> cm_uv(2 * ncm);
+      for (size_t q = 0; q < ncm; ++q) {
+        const int32_t o = S.cam_obs[q];
+        cm_point[q] = P.obs_point[o];
+        cm_intr[q] = P.obs_intr[o];
+        cm_uv[2 * q] = P.obs_uv[2 * size_t(o)];
+        cm_uv[2 * q + 1] = P.obs_uv[2 * size_t(o) + 1];
+      }
+      dp.cm_point = arena.upload(cm_point, stream);
+      dp.cm_intr = arena.upload(cm_intr, stream);
+      dp.cm_uv = arena.upload(cm_uv, stream);
+    }
     dp.s_ptr = arena.upload(S.s_ptr, stream);
     dp.s_col = arena.upload(S.s_col, stream);
     {
@@ -294,6 +309,7 @@ struct sgpu_optimizer {
     dl.cw = arena.alloc<double>(size_t(12) * S.npose);
     dl.hinv = arena.alloc<double>(size_t(9) * S.npt);
     dl.hll = arena.alloc<double>(size_t(6) * S.npt);
+    dl.prec = arena.alloc<double>(size_t(12) * S.npt);
     dl.pt_s = arena.alloc<double>(size_t(3) * S.npt);
     dl.pt_y = arena.alloc<double>(size_t(3) * S.npt);
     dl.rhs_nat = arena.alloc<double>(size_t(6) * S.nb);
