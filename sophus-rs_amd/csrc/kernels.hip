@@ -35,8 +35,13 @@ __device__ __forceinline__ void lds_add(T* p, T v) {
   unsafeAtomicAdd(p, v);
 }
 
-__device__ __forceinline__ void atomic_max_abs(unsigned long long* p, double v) {
-  atomicMax(p, (unsigned long long)__double_as_longlong(fabs(v)));
+// max |v| over the wavefront, then one atomic per wavefront (every lane of the wavefront
+// calls it); a per-lane atomic on one address would serialise one instruction per lane group
+__device__ __forceinline__ void wave_atomic_max_abs(unsigned long long* p, double v) {
+  v = fabs(v);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
 // Huber weight sqrt(rho(|r|)) / |r| (robust_kernel.rs:13-35); 1 without a kernel.
@@ -141,18 +146,14 @@ __global__ void __launch_bounds__(kThreads)
                  const double* __restrict__ cw, double nu, double* __restrict__ hinv_out,
                  double* __restrict__ hll_out, double* __restrict__ s_out,
                  double* __restrict__ y_out, double* __restrict__ prec, DevScalars* sc) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= p.npt) return;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = gid < p.npt;
+  const int b = live ? gid : p.npt - 1;  // trailing lanes stay for the wavefront max
   const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
-  double* rec = prec + 12 * int64_t(b);
-  if (p.pt_block[b] < 0) {  // constant point: H_ll^-1 = 0 and y = 0 in its record
-#pragma unroll
-    for (int k = 0; k < 12; ++k) rec[k] = k < 3 ? x[k] : 0.0;
-    return;
-  }
+  const bool active = p.pt_block[b] >= 0;
   T H[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
   T s[3] = {T(0), T(0), T(0)};
-  for (int o = p.pt_ptr[b]; o < p.pt_ptr[b + 1]; ++o) {
+  for (int o = p.pt_ptr[b]; live && active && o < p.pt_ptr[b + 1]; ++o) {
     const double* c = cw + 12 * p.obs_pose[o];
     const double* in = intr + 4 * p.obs_intr[o];
     double xc[3];
@@ -165,6 +166,19 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < 9; ++k) H[k] += M[k];
 #pragma unroll
     for (int k = 0; k < 3; ++k) s[k] += v[k];
+  }
+  {
+    T g = T(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g = fmax(g, fabs(s[k]));
+    wave_atomic_max_abs(&sc->max_grad_bits, live && active ? double(g) : 0.0);
+  }
+  if (!live) return;
+  double* rec = prec + 12 * int64_t(b);
+  if (!active) {  // constant point: H_ll^-1 = 0 and y = 0 in its record
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rec[k] = k < 3 ? x[k] : 0.0;
+    return;
   }
   const T damp = T(nu * double(p.pt_nu[b]));
   H[0] += damp;
@@ -180,15 +194,16 @@ __global__ void __launch_bounds__(kThreads)
     atomicOr(&sc->singular_marg, 1);
     for (int k = 0; k < 9; ++k) inv[k] = 0.0;
   } else {
-    inv[0] = a / det;
-    inv[1] = (m13 * m32 - m33 * m12) / det;
-    inv[2] = (m12 * m23 - m22 * m13) / det;
-    inv[3] = -bb / det;
-    inv[4] = (m11 * m33 - m31 * m13) / det;
-    inv[5] = (m13 * m21 - m23 * m11) / det;
-    inv[6] = c / det;
-    inv[7] = (m12 * m31 - m32 * m11) / det;
-    inv[8] = (m11 * m22 - m21 * m12) / det;
+    const double rdet = 1.0 / det;  // one division, then products
+    inv[0] = a * rdet;
+    inv[1] = (m13 * m32 - m33 * m12) * rdet;
+    inv[2] = (m12 * m23 - m22 * m13) * rdet;
+    inv[3] = -bb * rdet;
+    inv[4] = (m11 * m33 - m31 * m13) * rdet;
+    inv[5] = (m13 * m21 - m23 * m11) * rdet;
+    inv[6] = c * rdet;
+    inv[7] = (m12 * m31 - m32 * m11) * rdet;
+    inv[8] = (m11 * m22 - m21 * m12) * rdet;
   }
   for (int k = 0; k < 9; ++k) hinv_out[9 * b + k] = inv[k];
   double yv[3];
@@ -208,7 +223,7 @@ __global__ void __launch_bounds__(kThreads)
     s_out[3 * b + k] = double(s[k]);
     y_out[3 * b + k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) +
                        inv[3 * k + 2] * double(s[2]);
-    atomic_max_abs(&sc->max_grad_bits, double(s[k]));
+
   }
 }
 
@@ -1254,7 +1269,7 @@ __global__ void __launch_bounds__(kThreads)
               const double* __restrict__ cw, const double* __restrict__ hinv,
               const double* __restrict__ pt_s, const double* __restrict__ dx_cam,
               double* __restrict__ points_new, double* __restrict__ partial, DevScalars* sc) {
-  double dd = 0, dg = 0;
+  double dd = 0, dg = 0, mx = 0;
   for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < p.npt; b += gridDim.x * blockDim.x) {
     const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
     if (p.pt_block[b] < 0) {
@@ -1282,7 +1297,6 @@ __global__ void __launch_bounds__(kThreads)
       for (int k = 0; k < 3; ++k) acc[k] += M[3 * k] * q0 + M[3 * k + 1] * q1 + M[3 * k + 2] * q2;
     }
     const double* Hi = hinv + 9 * b;
-    double mx = 0;
     for (int k = 0; k < 3; ++k) {
       const double dk = Hi[3 * k] * double(acc[0]) + Hi[3 * k + 1] * double(acc[1]) +
                         Hi[3 * k + 2] * double(acc[2]);
@@ -1291,8 +1305,8 @@ __global__ void __launch_bounds__(kThreads)
       dg += dk * pt_s[3 * b + k];
       mx = fmax(mx, fabs(dk));
     }
-    atomic_max_abs(&sc->max_delta_bits, mx);
   }
+  wave_atomic_max_abs(&sc->max_delta_bits, mx);
   block_reduce2(dd, dg, partial);
 }
 
@@ -1302,10 +1316,10 @@ __global__ void __launch_bounds__(kThreads)
                    const double* __restrict__ grad_cam, double* __restrict__ poses_new,
                    double* __restrict__ partial, DevScalars* sc) {
   double dd = 0, dg = 0;
+  double mx = 0;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < p.nb; c += gridDim.x * blockDim.x) {
     const int pm = p.cam_member[c];
     double t[6], T0[7], T1[7];
-    double mx = 0;
     for (int k = 0; k < 6; ++k) {
       t[k] = dx_cam[6 * c + k];
       dd += t[k] * t[k];
@@ -1315,8 +1329,8 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < 7; ++k) T0[k] = poses[7 * pm + k];
     se3_oplus(T0, t, T1);
     for (int k = 0; k < 7; ++k) poses_new[7 * pm + k] = T1[k];
-    atomic_max_abs(&sc->max_delta_bits, mx);
   }
+  wave_atomic_max_abs(&sc->max_delta_bits, mx);
   block_reduce2(dd, dg, partial);
 }
 

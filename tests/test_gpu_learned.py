@@ -103,8 +103,10 @@ def _ba_run(sg, p, solver, weights=None, iters=8, tol=1e-8):
 def test_untrained_network_reproduces_block_jacobi(sg):
     p = sg.generate_ba(sg.BaSpec(layout=1, num_cameras=40, num_points=1500, geometric_p=0.2, seed=21))
     w0 = sg.init_precond_weights(0)  # output layer zero: G = I
-    m_l, t_l = _ba_run(sg, p, sg.Solver.SchurPcgLearned, w0)
-    m_j, t_j = _ba_run(sg, p, sg.Solver.SchurPcgJacobi)
+    # both solves converge far below the comparison bar, so the two LM runs differ only by
+    # the preconditioner's rounding (B B^T vs D^-1), not by where CG happened to stop
+    m_l, t_l = _ba_run(sg, p, sg.Solver.SchurPcgLearned, w0, tol=1e-12)
+    m_j, t_j = _ba_run(sg, p, sg.Solver.SchurPcgJacobi, tol=1e-12)
     assert [t.accepted for t in t_l] == [t.accepted for t in t_j]
     assert abs(m_l - m_j) <= 1e-9 * m_j
     for a, b in zip(t_l, t_j):
