@@ -5,26 +5,34 @@
 
 namespace sgpu {
 
+// 1 / x from the hardware reciprocal refined by two Newton steps (a short dependent chain
+// instead of the division sequence; within an ulp of the quotient).
+__device__ __forceinline__ double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // ldlt_decompose_inplace (dense_ldlt.rs:164-256) on a 6x6 block with the PSD pivot rule.
 // The off-diagonal quotients use each pivot's reciprocal (formed once); dinv, if given,
-// receives those reciprocals (0 for a zero pivot).
+// receives those reciprocals (0 for a zero pivot).  Row j keeps w_q = l_jq d_q, so each
+// inner term is one multiply-add.
 __device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv = nullptr) {
-  double a[36], rd[6];
+  double a[36], rd[6], w[36];
   for (int k = 0; k < 36; ++k) a[k] = A[k];
   double max_abs_pivot = 1.0;
   const double tol = 1e-12;
   for (int j = 0; j < 6; ++j) {
     for (int k = 0; k < j; ++k) {
       double s = a[6 * j + k];
-      for (int q = 0; q < k; ++q) s -= a[6 * j + q] * d[q] * a[6 * k + q];
+      for (int q = 0; q < k; ++q) s -= w[6 * j + q] * a[6 * k + q];
       const double zt = fmax(max_abs_pivot, 1.0) * tol;
       a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s * rd[k];
+      w[6 * j + k] = a[6 * j + k] * d[k];
     }
     double djj = a[6 * j + j];
-    for (int q = 0; q < j; ++q) {
-      const double l = a[6 * j + q];
-      djj -= d[q] * l * l;
-    }
+    for (int q = 0; q < j; ++q) djj -= w[6 * j + q] * a[6 * j + q];
     if (!isfinite(djj)) return false;
     const double zt = fmax(max_abs_pivot, 1.0) * tol;
     if (fabs(djj) <= zt) {
@@ -32,7 +40,7 @@ __device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv
       rd[j] = 0.0;
     } else {
       d[j] = djj;
-      rd[j] = 1.0 / djj;
+      rd[j] = recip_nr(djj);
       max_abs_pivot = fmax(max_abs_pivot, fabs(djj));
     }
     a[6 * j + j] = 1.0;

@@ -667,23 +667,28 @@ __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, i
         [&](int e, double v) { D[e % np][e / np] = v; });
     __syncthreads();
   }
-  for (int j0 = 0; j0 < np; j0 += 6) {
-    if (tid == 0) {
-      double a6[36], L[36], d[6], di[6];
-      for (int a = 0; a < 6; ++a)
-        for (int b = 0; b < 6; ++b) a6[6 * a + b] = a >= b ? D[j0 + a][j0 + b] : D[j0 + b][j0 + a];
-      if (!ldlt6(a6, L, d, di)) {
-        atomicOr(&sc->bad_pivot, 1);
-        for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
-        for (int k = 0; k < 6; ++k) d[k] = di[k] = 0.0;
-      }
-      for (int k = 0; k < 36; ++k) sL[k] = L[k];
-      for (int k = 0; k < 6; ++k) {
-        sL[36 + k] = d[k];
-        sdi[k] = di[k];
-      }
+  // One pivot: thread 0 factors the 6x6 block a6 into sL (L_jj, then d_j) and sdi.
+  auto factor_pivot = [&](double* a6) {
+    double L[36], d[6], di[6];
+    if (!ldlt6(a6, L, d, di)) {
+      atomicOr(&sc->bad_pivot, 1);
+      for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
+      for (int k = 0; k < 6; ++k) d[k] = di[k] = 0.0;
     }
-    __syncthreads();
+    for (int k = 0; k < 36; ++k) sL[k] = L[k];
+    for (int k = 0; k < 6; ++k) {
+      sL[36 + k] = d[k];
+      sdi[k] = di[k];
+    }
+  };
+  if (tid == 0) {
+    double a6[36];
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) a6[6 * a + b] = a >= b ? D[a][b] : D[b][a];
+    factor_pivot(a6);
+  }
+  for (int j0 = 0; j0 < np; j0 += 6) {
+    __syncthreads();  // pivot j0 is in sL / sdi
     const int gb = f.sn_c0[X] + (k0 + j0) / 6;
     if (tid < 36) {
       f.piv_L[36 * gb + tid] = sL[tid];
@@ -705,13 +710,15 @@ __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, i
           ua[k] -= sL[6 * k + t] * ua[t];
           ub[k] -= sL[6 * k + t] * ub[t];
         }
-      double m = 0.0;
+      double mm = 0.0;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) m += ua[k] * sdi[k] * ub[k];
-      f.piv_M[36 * gb + 6 * a + b] = m;
+      for (int k = 0; k < 6; ++k) mm += ua[k] * sdi[k] * ub[k];
+      f.piv_M[36 * gb + 6 * a + b] = mm;
     }
     const int m = np - j0 - 6;  // rows of the block below this pivot
     if (m <= 0) break;
+    // rows below the pivot (beside the inverse above): W = the block's values (kept in
+    // Wd), L = W L_jj^-T D_j^-1 L_jj^-1
     if (tid < m) {
       const int r = j0 + 6 + tid;
       double x[6];
@@ -725,18 +732,48 @@ __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, i
       for (int t = 0; t < 6; ++t) D[r][j0 + t] = x[t];
     }
     __syncthreads();
-    // trailing lower triangle of the block: m (m + 1) / 2 <= kPW (kPW - 5) / 2 entries
-    for (int e = tid; e < m * (m + 1) / 2; e += blockDim.x) {
-      int rr = int((sqrtf(8.0f * float(e) + 1.0f) - 1.0f) * 0.5f);
-      while (rr * (rr + 1) / 2 > e) --rr;
-      while ((rr + 1) * (rr + 2) / 2 <= e) ++rr;
-      const int r = j0 + 6 + rr, c = j0 + 6 + e - rr * (rr + 1) / 2;
-      double acc = 0;
+    const int j1 = j0 + 6;  // the next pivot
+    if (tid < 64) {
+      // look-ahead: the next pivot block, updated by this pivot (one entry per lane of
+      // wavefront 0), gathered into thread 0 and factored at once
+      const int e = tid < 21 ? tid : 0;
+      int a = 0;
+      while ((a + 1) * (a + 2) / 2 <= e) ++a;
+      const int b = e - a * (a + 1) / 2;
+      double acc = D[j1 + a][j1 + b];
 #pragma unroll
-      for (int t = 0; t < 6; ++t) acc += D[r][j0 + t] * Wd[c][j0 + t];
-      D[r][c] -= acc;
+      for (int t = 0; t < 6; ++t) acc -= D[j1 + a][j0 + t] * Wd[j1 + b][j0 + t];
+      double a6[36];
+#pragma unroll
+      for (int q = 0; q < 21; ++q) {
+        int qa = 0;
+        while ((qa + 1) * (qa + 2) / 2 <= q) ++qa;
+        const int qb = q - qa * (qa + 1) / 2;
+        const double v = __shfl(acc, q, 64);
+        a6[6 * qa + qb] = v;
+        a6[6 * qb + qa] = v;
+      }
+      if (tid == 0) factor_pivot(a6);
+    } else {
+      // the rest of the trailing lower triangle (the next pivot block is thread 0's):
+      // kRowSplit threads per row share its columns, the row's six L values in registers
+      constexpr int kRowSplit = (kThreads - 64) / kPW;
+      const int rr = (tid - 64) % kPW, cg = (tid - 64) / kPW;
+      if (rr < m && cg < kRowSplit) {
+        const int r = j1 + rr;
+        double lr6[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) lr6[t] = D[r][j0 + t];
+        for (int cc = cg; cc <= rr; cc += kRowSplit) {
+          if (rr < 6) break;  // row inside the next pivot block
+          const int c = j1 + cc;
+          double acc = 0;
+#pragma unroll
+          for (int t = 0; t < 6; ++t) acc += lr6[t] * Wd[c][j0 + t];
+          D[r][c] -= acc;
+        }
+      }
     }
-    __syncthreads();
   }
   double* Lout = f.lbuf + f.sn_loff[X];
   double* Lt = f.ltbuf + f.sn_toff[X];
