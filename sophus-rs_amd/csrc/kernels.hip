@@ -78,6 +78,27 @@ __device__ __forceinline__ void stage_lds(int n, Load load, Put put) {
   }
 }
 
+// stage_lds for two arrays of n values each, with the loads of both issued first.
+template <int N, typename LoadA, typename PutA, typename LoadB, typename PutB>
+__device__ __forceinline__ void stage_lds2(int n, LoadA load_a, PutA put_a, LoadB load_b,
+                                           PutB put_b) {
+  double va[N], vb[N];
+#pragma unroll
+  for (int it = 0; it < N; ++it) {
+    const int e = int(threadIdx.x) + it * int(blockDim.x);
+    va[it] = load_a(e < n ? e : 0, e < n);
+    vb[it] = load_b(e < n ? e : 0, e < n);
+  }
+#pragma unroll
+  for (int it = 0; it < N; ++it) {
+    const int e = int(threadIdx.x) + it * int(blockDim.x);
+    if (e < n) {
+      put_a(e, va[it]);
+      put_b(e, vb[it]);
+    }
+  }
+}
+
 // camera_from_world cache: R_cw (row-major) + t_cw.
 __global__ void k_cam_cache(int npose, const double* __restrict__ poses, double* __restrict__ cw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -647,15 +668,40 @@ __device__ __forceinline__ double (*as_square(double (*t)[kTile + 1]))[kPW + 1] 
   return reinterpret_cast<double (*)[kPW + 1]>(&t[0][0]);
 }
 
+// The per-node values a diagonal factorization needs, read at kernel entry (ahead of the
+// barriers, which would otherwise keep these loads from being hoisted).
+struct DiagNode {
+  int c0, K;                  // first column block, node columns
+  int64_t loff, toff, poff;   // L panel, row-major L_KK copy, L_PP^-T slots
+};
+__device__ __forceinline__ DiagNode diag_node(const DevFactor& f, int X) {
+  return DiagNode{f.sn_c0[X], 6 * f.sn_nc[X], f.sn_loff[X], f.sn_toff[X], f.sn_poff[X]};
+}
+
 // Factor the np x np diagonal block of the panel at column k0 of one front in LDS (block
 // LDLt with 6x6 pivots and the PSD rule), then store L_PP (strictly below the pivot
-// blocks) into the front and the pivots (L_jj, d_j) into piv_L / piv_d.  Called by all
-// threads of one workgroup.  D holds the block's lower triangle already when `loaded`
-// (else it is read from the front).  sL is LDS scratch for one pivot (L_jj, then d_j in
-// sL[36..41]); sdi receives 1 / d_j.
-__device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, int k0, int np,
-                                  DevScalars* sc, double (*D)[kPW + 1], double (*Wd)[kPW + 1],
-                                  double* sL, double* sdi, bool loaded) {
+// blocks) into the factor and its row-major copy, the pivots (L_jj, d_j, M_j) into
+// piv_L / piv_d / piv_M and L_PP^-T into the panel's slot of tinv.  Called by all threads
+// of one workgroup.  D holds the block's lower triangle already when `loaded` (else it is
+// read from the front).  sL / sdi hold each pivot's factor (L_jj, then d_j in
+// sL[j][36..41]; 1 / d_j in sdi[j]).
+//
+// Per pivot j (one barrier each):
+//   wavefront 0   the next pivot block: lane (a, b), a >= b, of 21 forms row j1 + a of L
+//                 from its W (the row's values before the solve) and subtracts L_a W_b^T;
+//                 thread 0 gathers the 21 entries and factors the block, so its factor is
+//                 ready at the next barrier;
+//   wavefronts 1-3  the trailing update of the rows below the next pivot block (four
+//                 consecutive threads per row, each row's L formed from its W; the updated
+//                 values of the next pivot's columns are also kept in Wd, the next
+//                 pivot's W) and row block j of T = L_PP^-1 (its L columns are final),
+//                 spread from the last thread down so that it falls on the threads whose
+//                 rows have no trailing work.
+// The pivots' inverses M_j and the stores follow the loop.
+__device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double* A, int F, int k0,
+                                  int np, DevScalars* sc, double (*D)[kPW + 1],
+                                  double (*Wd)[kPW + 1], double (*sL)[42], double (*sdi)[6],
+                                  bool loaded) {
   const int tid = threadIdx.x;
   if (!loaded) {
     stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
@@ -667,124 +713,167 @@ __device__ void factor_panel_diag(const DevFactor& f, int X, double* A, int F, i
         [&](int e, double v) { D[e % np][e / np] = v; });
     __syncthreads();
   }
-  // One pivot: thread 0 factors the 6x6 block a6 into sL (L_jj, then d_j) and sdi.
-  auto factor_pivot = [&](double* a6) {
+  // W of the first pivot: its columns below its block
+  for (int e = 36 + tid; e < 6 * np; e += blockDim.x) Wd[e / 6][e % 6] = D[e / 6][e % 6];
+  // L_PP^-T (row-major kPW x kPW) for the panel rows: its unit diagonal blocks and zeros
+  // here, the blocks above the diagonal as T's row blocks are formed
+  double* Tg = f.tinv + dn.poff + int64_t(k0 / kPW) * kPW * kPW;
+  for (int e = tid - 64; e >= 0 && e < kPW * kPW; e += blockDim.x - 64) {  // beside pivot 0
+    const int r = e / kPW, c = e % kPW;
+    if (r / 6 >= c / 6 || c >= np) Tg[e] = r == c && r < np ? 1.0 : 0.0;
+  }
+  double* Lout = f.lbuf + dn.loff;
+  double* Lt = f.ltbuf + dn.toff;
+  const int K = dn.K;
+  // thread 0 factors the 6x6 block a6 into slot s
+  auto factor_pivot = [&](const double* a6, int s) {
     double L[36], d[6], di[6];
     if (!ldlt6(a6, L, d, di)) {
       atomicOr(&sc->bad_pivot, 1);
       for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
       for (int k = 0; k < 6; ++k) d[k] = di[k] = 0.0;
     }
-    for (int k = 0; k < 36; ++k) sL[k] = L[k];
+    for (int k = 0; k < 36; ++k) sL[s][k] = L[k];
     for (int k = 0; k < 6; ++k) {
-      sL[36 + k] = d[k];
-      sdi[k] = di[k];
+      sL[s][36 + k] = d[k];
+      sdi[s][k] = di[k];
     }
   };
   if (tid == 0) {
     double a6[36];
     for (int a = 0; a < 6; ++a)
       for (int b = 0; b < 6; ++b) a6[6 * a + b] = a >= b ? D[a][b] : D[b][a];
-    factor_pivot(a6);
+    factor_pivot(a6, 0);
   }
   for (int j0 = 0; j0 < np; j0 += 6) {
-    __syncthreads();  // pivot j0 is in sL / sdi
-    const int gb = f.sn_c0[X] + (k0 + j0) / 6;
-    if (tid < 36) {
-      f.piv_L[36 * gb + tid] = sL[tid];
-    } else if (tid < 42) {
-      f.piv_d[6 * gb + tid - 36] = sL[tid];
-    } else if (tid >= 64 && tid < 100) {
-      // the pivot's inverse M = L^-T D^+ L^-1 (entry (a, b)) from the columns of L^-1
-      const int a = (tid - 64) / 6, b = (tid - 64) % 6;
-      double ua[6], ub[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        ua[k] = k == a ? 1.0 : 0.0;
-        ub[k] = k == b ? 1.0 : 0.0;
-      }
-#pragma unroll
-      for (int k = 1; k < 6; ++k)
-#pragma unroll
-        for (int t = 0; t < k; ++t) {
-          ua[k] -= sL[6 * k + t] * ua[t];
-          ub[k] -= sL[6 * k + t] * ub[t];
-        }
-      double mm = 0.0;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) mm += ua[k] * sdi[k] * ub[k];
-      f.piv_M[36 * gb + 6 * a + b] = mm;
-    }
-    const int m = np - j0 - 6;  // rows of the block below this pivot
-    if (m <= 0) break;
-    // rows below the pivot (beside the inverse above): W = the block's values (kept in
-    // Wd), L = W L_jj^-T D_j^-1 L_jj^-1
-    if (tid < m) {
-      const int r = j0 + 6 + tid;
-      double x[6];
-#pragma unroll
-      for (int t = 0; t < 6; ++t) {
-        x[t] = D[r][j0 + t];
-        Wd[r][j0 + t] = x[t];
-      }
-      right_solve6(sL, sdi, x);
-#pragma unroll
-      for (int t = 0; t < 6; ++t) D[r][j0 + t] = x[t];
-    }
-    __syncthreads();
-    const int j1 = j0 + 6;  // the next pivot
+    __syncthreads();  // pivot j0 is in slot j0 / 6; the previous pivot's work is done
+    const int jb = j0 / 6;
+    const double* cL = sL[jb];
+    const double* cdi = sdi[jb];
+    const int j1 = j0 + 6, m = np - j1;  // the next pivot; rows of the block below this one
     if (tid < 64) {
-      // look-ahead: the next pivot block, updated by this pivot (one entry per lane of
-      // wavefront 0), gathered into thread 0 and factored at once
-      const int e = tid < 21 ? tid : 0;
-      int a = 0;
-      while ((a + 1) * (a + 2) / 2 <= e) ++a;
-      const int b = e - a * (a + 1) / 2;
-      double acc = D[j1 + a][j1 + b];
+      if (m > 0) {
+        const int e = tid < 21 ? tid : 0;
+        int a = 0;
+        while ((a + 1) * (a + 2) / 2 <= e) ++a;
+        const int b = e - a * (a + 1) / 2;
+        double x[6];
 #pragma unroll
-      for (int t = 0; t < 6; ++t) acc -= D[j1 + a][j0 + t] * Wd[j1 + b][j0 + t];
-      double a6[36];
+        for (int t = 0; t < 6; ++t) x[t] = Wd[j1 + a][j0 + t];
+        right_solve6(cL, cdi, x);
+        if (tid < 21 && b == 0)
 #pragma unroll
-      for (int q = 0; q < 21; ++q) {
-        int qa = 0;
-        while ((qa + 1) * (qa + 2) / 2 <= q) ++qa;
-        const int qb = q - qa * (qa + 1) / 2;
-        const double v = __shfl(acc, q, 64);
-        a6[6 * qa + qb] = v;
-        a6[6 * qb + qa] = v;
+          for (int t = 0; t < 6; ++t) D[j1 + a][j0 + t] = x[t];
+        double acc = D[j1 + a][j1 + b];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) acc -= x[t] * Wd[j1 + b][j0 + t];
+        // gather the 21 entries into thread 0 (uniform reads of lanes 0..20)
+        const int2 bits = __builtin_bit_cast(int2, acc);
+        double a6[36];
+#pragma unroll
+        for (int q = 0; q < 21; ++q) {
+          int qa = 0;
+          while ((qa + 1) * (qa + 2) / 2 <= q) ++qa;
+          const int qb = q - qa * (qa + 1) / 2;
+          const int2 vb = make_int2(__builtin_amdgcn_readlane(bits.x, q),
+                                    __builtin_amdgcn_readlane(bits.y, q));
+          const double v = __builtin_bit_cast(double, vb);
+          a6[6 * qa + qb] = v;
+          a6[6 * qb + qa] = v;
+        }
+        if (tid == 0) factor_pivot(a6, jb + 1);
       }
-      if (tid == 0) factor_pivot(a6);
     } else {
-      // the rest of the trailing lower triangle (the next pivot block is thread 0's):
-      // kRowSplit threads per row share its columns, the row's six L values in registers
-      constexpr int kRowSplit = (kThreads - 64) / kPW;
-      const int rr = (tid - 64) % kPW, cg = (tid - 64) / kPW;
-      if (rr < m && cg < kRowSplit) {
-        const int r = j1 + rr;
-        double lr6[6];
+      const int t3 = tid - 64;  // 0 .. kThreads - 65
+      constexpr int kRowSplit = 4;
+      static_assert((kThreads - 64) == kRowSplit * kPW, "four threads per block row");
+      const int rr = t3 / kRowSplit, cg = t3 % kRowSplit;
+      const int r = j1 + 6 + rr;
+      if (r < np) {
+        double x[6];
 #pragma unroll
-        for (int t = 0; t < 6; ++t) lr6[t] = D[r][j0 + t];
-        for (int cc = cg; cc <= rr; cc += kRowSplit) {
-          if (rr < 6) break;  // row inside the next pivot block
-          const int c = j1 + cc;
-          double acc = 0;
+        for (int t = 0; t < 6; ++t) x[t] = Wd[r][j0 + t];
+        right_solve6(cL, cdi, x);
+        if (cg == 0)
 #pragma unroll
-          for (int t = 0; t < 6; ++t) acc += lr6[t] * Wd[c][j0 + t];
-          D[r][c] -= acc;
+          for (int t = 0; t < 6; ++t) D[r][j0 + t] = x[t];
+        for (int c = j1 + cg; c <= r; c += kRowSplit) {
+          double acc = 0.0;
+#pragma unroll
+          for (int t = 0; t < 6; ++t) acc += x[t] * Wd[c][j0 + t];
+          const double v = D[r][c] - acc;
+          D[r][c] = v;
+          if (c < j1 + 6) Wd[r][c] = v;  // the next pivot's W
+        }
+      }
+      // row block jb of T = L_PP^-1, kept transposed in the free upper triangle:
+      // D[c][r] = T[r][c] for block(c) < block(r); four partial sums per entry.  The
+      // entries go to wavefront 1 first (its rows have the least trailing work), then 3,
+      // then 2.
+      const int wv3 = t3 >> 6, pos = wv3 == 0 ? 0 : (wv3 == 2 ? 1 : 2);
+      for (int e = 64 * pos + (t3 & 63); e < 6 * j0; e += kThreads - 64) {
+        const int a = e % 6, c = e / 6;
+        double acc[4] = {-D[j0 + a][c], 0.0, 0.0, 0.0};
+        const int c1 = 6 * (c / 6 + 1);
+        for (int cp = c1; cp < j0; cp += 4)
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (cp + u < j0) acc[u] -= D[j0 + a][cp + u] * D[c][cp + u];
+        D[c][j0 + a] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+      }
+      if (tid >= 192) {
+        // wavefront 3: this pivot's M_j and stores, column block jb - 1 of L (final since
+        // the previous pivot) and T's row block jb - 1 to memory
+        const int l3 = tid - 192;
+        const int gb = dn.c0 + (k0 + j0) / 6;
+        if (l3 < 36) {
+          // the pivot's inverse M = L^-T D^+ L^-1 (entry (a, b)) from the columns of L^-1
+          const int a = l3 / 6, b = l3 % 6;
+          double ua[6], ub[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) {
+            ua[k] = k == a ? 1.0 : 0.0;
+            ub[k] = k == b ? 1.0 : 0.0;
+          }
+#pragma unroll
+          for (int k = 1; k < 6; ++k)
+#pragma unroll
+            for (int t = 0; t < k; ++t) {
+              ua[k] -= cL[6 * k + t] * ua[t];
+              ub[k] -= cL[6 * k + t] * ub[t];
+            }
+          double mm = 0.0;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) mm += ua[k] * cdi[k] * ub[k];
+          f.piv_M[36 * int64_t(gb) + l3] = mm;
+        }
+        if (l3 < 42) {
+          if (l3 < 36) f.piv_L[36 * int64_t(gb) + l3] = cL[l3];
+          else f.piv_d[6 * int64_t(gb) + l3 - 36] = cL[l3];
+        }
+        if (jb > 0) {
+          const int p0 = j0 - 6, nr = np - j0;
+          for (int e = l3; e < 6 * nr; e += 64) {
+            const int rr2 = e % nr, t = e / nr;
+            Lout[int64_t(k0 + p0 + t) * F + k0 + j0 + rr2] = D[j0 + rr2][p0 + t];
+          }
+          for (int e = l3; e < 6 * nr; e += 64) {  // row-major copy
+            const int rr2 = e / 6, t = e % 6;
+            Lt[int64_t(k0 + j0 + rr2) * K + k0 + p0 + t] = D[j0 + rr2][p0 + t];
+          }
+          for (int e = l3; e < 6 * p0; e += 64) {
+            const int cp = e / 6, a = e % 6;
+            Tg[cp * kPW + p0 + a] = D[cp][p0 + a];
+          }
         }
       }
     }
   }
-  double* Lout = f.lbuf + f.sn_loff[X];
-  double* Lt = f.ltbuf + f.sn_toff[X];
-  const int K = 6 * f.sn_nc[X];
-  for (int e = tid; e < np * np; e += blockDim.x) {
-    const int r = e % np, c = e / np;
-    if (r / 6 > c / 6) Lout[int64_t(k0 + c) * F + k0 + r] = D[r][c];
-  }
-  for (int e = tid; e < np * np; e += blockDim.x) {  // row-major copy, rows contiguous
-    const int r = e / np, c = e % np;
-    if (r / 6 > c / 6) Lt[int64_t(k0 + r) * K + k0 + c] = D[r][c];
+  __syncthreads();  // T's last row block is in D
+  const int pl = np - 6;
+  for (int e = tid; e < 6 * pl; e += blockDim.x) {
+    const int cp = e / 6, a = e % 6;
+    Tg[cp * kPW + pl + a] = D[cp][pl + a];
   }
 }
 
@@ -793,112 +882,128 @@ __global__ void __launch_bounds__(kThreads)
     k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc) {
   const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
-  __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[42], sdi[6];
-  factor_panel_diag(f, X, f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL, sdi, false);
+  __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
+  factor_panel_diag(f, diag_node(f, X), f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL,
+                    sdi, false);
 }
 
-// L_PP (strictly below the pivot blocks) and the pivots (L_jj, 1 / d_j; 0 for a zero
-// pivot) of the panel at column k0 into LDS.
-__device__ __forceinline__ void load_panel_factor(const DevFactor& f, int X, const double* Lo,
-                                                  int F, int k0, int np, double (*Lpp)[kPW + 1],
-                                                  double (*pL)[36], double (*pdi)[6]) {
-  stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
-      kPW * kPW,
-      [&](int e, bool ok) {
-        const int rr = e % kPW, c = e / kPW;
-        return load_or_zero(Lo, int64_t(k0 + c) * F + k0 + rr, ok && rr < np && c < np && rr / 6 > c / 6);
-      },
-      [&](int e, double v) { Lpp[e % kPW][e / kPW] = v; });
-  const int gb0 = f.sn_c0[X] + k0 / 6, nbp = np / 6;
-  stage_lds<(kPanelBlocks * 36 + kThreads - 1) / kThreads>(
-      kPanelBlocks * 36,
-      [&](int e, bool ok) {
-        const int jb = e / 36, q = e % 36;
-        const double v = load_or_zero(f.piv_L, int64_t(36) * (gb0 + jb) + q, ok && jb < nbp);
-        return jb < nbp ? v : (q % 7 == 0 ? 1.0 : 0.0);
-      },
-      [&](int e, double v) { pL[e / 36][e % 36] = v; });
-  if (threadIdx.x < kPanelBlocks * 6) {
-    const int jb = threadIdx.x / 6, q = threadIdx.x % 6;
-    const double d = load_or_zero(f.piv_d, int64_t(6) * (gb0 + jb) + q, jb < nbp);
-    pdi[jb][q] = d != 0.0 ? 1.0 / d : 0.0;
-  }
-}
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
-// Panel rows of one panel step for the level's nodes, four lanes per row below the
-// panel's diagonal block (items = (node, kPanelRows-row chunk)), each row formed once:
-//   W_r,j = C_r,j - sum_{i<j} W_r,i L_PP(j, i)^T   (unscaled, into the node's W scratch)
-//   L_r,j = W_r,j M_j,  M_j = L_jj^-T D_j^-1 L_jj^-1 (into the factor)
-// Lane s of a row holds the columns c = s (mod 4): each block's sums are split over the
-// four lanes and combined with two xor shuffles (fixed order); the lane owning a column
-// keeps and stores it.  L_PP and the M_j are staged in LDS.
+// Panel rows of one panel step for the level's nodes (items = (node, kPanelRows-row
+// chunk) below the panel's diagonal block), as two products instead of a substitution:
+//   W_r = C_r L_PP^-T            (the rows' values after the panel's own eliminations)
+//   L_r,j = W_r,j M_j,  M_j = L_jj^-T D_j^-1 L_jj^-1
+// with L_PP^-T from the diagonal factorization.  Both run on the matrix cores as 4 x 3
+// tiles of v_mfma_f64_16x16x4, three per wavefront.  W goes to the node's W scratch, L to
+// the factor and, for the node's own rows, to its row-major copy.
 __global__ void __launch_bounds__(kThreads)
     k_mf_panel(DevFactor f, int panel, const int32_t* __restrict__ items) {
-  static_assert(kPanelRows * 4 == kThreads && kPW % 4 == 0, "four lanes per row");
+  static_assert(kPanelRows == 64 && kPW == 48 && kThreads == 256, "4 x 3 tiles, 3 per wavefront");
+  static_assert(kPW * kPanelRows % kThreads == 0 && kPW * kPW % kThreads == 0, "whole staging rounds");
   const int X = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   const double* A = f.front + f.sn_off[X];
-  const double* Lg = f.lbuf + f.sn_loff[X];
   double* Lo = f.lbuf + f.sn_loff[X];
   double* Wo = f.wbuf + f.sn_woff[X];
   double* Lt = f.ltbuf + f.sn_toff[X];
+  const double* Tg = f.tinv + f.sn_poff[X] + int64_t(panel) * kPW * kPW;
   const int k0 = kPW * panel, kend = min(K, k0 + kPW), np = kend - k0;
-  const int s = threadIdx.x & 3;
-  const int r = kend + kPanelRows * chunk + (threadIdx.x >> 2);
-  const bool inside = r < F;
-  __shared__ double Lpp[kPW][kPW + 1];
+  const int rbase = kend + kPanelRows * chunk;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
+  __shared__ double sC[kPW][kPanelRows + 1];  // the rows' panel columns, C^T
+  __shared__ double sT[kPW][kPW + 1];         // L_PP^-T
+  __shared__ double sW[kPanelRows][kPW + 1];
   __shared__ double pM[kPanelBlocks][36];
-  // this lane's columns first (independent of the panel factor)
-  double w[kPW / 4];
-#pragma unroll
-  for (int u = 0; u < kPW / 4; ++u)
-    w[u] = load_or_zero(A, int64_t(k0 + s + 4 * u) * F + r, inside && s + 4 * u < np);
-  stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
-      kPW * kPW,
-      [&](int e, bool ok) {
-        const int rr = e % kPW, c = e / kPW;
-        return load_or_zero(Lg, int64_t(k0 + c) * F + k0 + rr, ok && rr < np && c < np && rr / 6 > c / 6);
-      },
-      [&](int e, double v) { Lpp[e % kPW][e / kPW] = v; });
   const int gb0 = f.sn_c0[X] + k0 / 6, nbp = np / 6;
-  stage_lds<(kPanelBlocks * 36 + kThreads - 1) / kThreads>(
-      kPanelBlocks * 36,
-      [&](int e, bool ok) { return load_or_zero(f.piv_M, int64_t(36) * gb0 + e, ok && e / 36 < nbp); },
-      [&](int e, double v) { pM[e / 36][e % 36] = v; });
+  // every global load of the thread first, then the LDS stores
+  constexpr int kNC = kPW * kPanelRows / kThreads, kNT = kPW * kPW / kThreads;
+  constexpr int kNM = (kPanelBlocks * 36 + kThreads - 1) / kThreads;
+  double vc[kNC], vt[kNT], vm[kNM];
+#pragma unroll
+  for (int it = 0; it < kNC; ++it) {
+    const int e = tid + kThreads * it, t = e / kPanelRows, r = rbase + e % kPanelRows;
+    vc[it] = load_or_zero(A, int64_t(k0 + t) * F + r, t < np && r < F);
+  }
+#pragma unroll
+  for (int it = 0; it < kNT; ++it) vt[it] = Tg[tid + kThreads * it];
+#pragma unroll
+  for (int it = 0; it < kNM; ++it) {
+    const int e = tid + kThreads * it;
+    vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
+  }
+#pragma unroll
+  for (int it = 0; it < kNC; ++it) {
+    const int e = tid + kThreads * it;
+    sC[e / kPanelRows][e % kPanelRows] = vc[it];
+  }
+#pragma unroll
+  for (int it = 0; it < kNT; ++it) {
+    const int e = tid + kThreads * it;
+    sT[e / kPW][e % kPW] = vt[it];
+  }
+#pragma unroll
+  for (int it = 0; it < kNM; ++it) {
+    const int e = tid + kThreads * it;
+    if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
+  }
   __syncthreads();
-  if (!inside) return;  // all four lanes of a row leave together
+  // Wavefront wv owns row tile wv (16 rows) and its three 16-column tiles; it reads only
+  // its own rows of C, W and L after the barrier above, so it needs no further barrier.
+  // W = C L_PP^-T: L_PP^-T is block upper triangular, so column tile ct stops at row
+  // kW[ct] (C and L_PP^-T are zero past np).
+  constexpr int kW[3] = {18, 36, 48};
+  const int rw = 16 * wv;
+  mfma_d4 acc[3];
 #pragma unroll
-  for (int jb = 0; jb < kPanelBlocks; ++jb) {
-    if (jb >= nbp) break;
-    double p[6];
+  for (int ct = 0; ct < 3; ++ct) acc[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int a = 0; a < 6; ++a) {
-      const int col = 6 * jb + a;
-      double acc = (col & 3) == s ? w[col >> 2] : 0.0;
+  for (int k4 = 0; k4 < kPW; k4 += 4) {
+    const double av = sC[k4 + lk][rw + lr];
 #pragma unroll
-      for (int u = 0; u < kPW / 4; ++u)
-        if (4 * u < 6 * jb && s + 4 * u < 6 * jb) acc -= w[u] * Lpp[col][s + 4 * u];
-      p[a] = acc;
-    }
+    for (int ct = 0; ct < 3; ++ct)
+      if (k4 < kW[ct])
+        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sT[k4 + lk][16 * ct + lr], acc[ct], 0, 0, 0);
+  }
 #pragma unroll
-    for (int a = 0; a < 6; ++a) {
-      p[a] += __shfl_xor(p[a], 1, 64);
-      p[a] += __shfl_xor(p[a], 2, 64);
-    }
+  for (int ct = 0; ct < 3; ++ct)
 #pragma unroll
-    for (int a = 0; a < 6; ++a) {
-      const int col = 6 * jb + a;
-      if ((col & 3) == s) {
-        w[col >> 2] = p[a];
-        Wo[int64_t(col) * F + r] = p[a];
-        double l = 0.0;
+    for (int q = 0; q < 4; ++q) sW[rw + lk + 4 * q][16 * ct + lr] = acc[ct][q];
+  // L = W blockdiag(M_j): column tile ct meets the pivot blocks with rows
+  // [kL0[ct], kL1[ct]); the block-diagonal operand is read from the M_j directly
+  constexpr int kL0[3] = {0, 12, 30}, kL1[3] = {18, 36, 48};
 #pragma unroll
-        for (int t = 0; t < 6; ++t) l += p[t] * pM[jb][6 * t + a];
-        Lo[int64_t(k0 + col) * F + r] = l;
-        if (r < K) Lt[int64_t(r) * K + k0 + col] = l;  // the node's own rows: row-major copy
+  for (int ct = 0; ct < 3; ++ct) acc[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < kPW; k4 += 4) {
+    const int k = k4 + lk;
+    const double av = sW[rw + lr][k];
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+      if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
+        const int n = 16 * ct + lr;
+        const double bv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
+        acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[ct], 0, 0, 0);
       }
+  }
+  // L^T into this wavefront's columns of sC
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sC[16 * ct + lr][rw + lk + 4 * q] = acc[ct][q];
+  // the wavefront's 16 rows to memory: W and L column-major, and for the node's own rows
+  // the row-major copy of L
+  for (int e = lane; e < 16 * kPW; e += 64) {
+    const int i = rw + e % 16, c = e / 16, r = rbase + i;
+    if (c < np && r < F) {
+      Wo[int64_t(c) * F + r] = sW[i][c];
+      Lo[int64_t(k0 + c) * F + r] = sC[c][i];
     }
   }
+  if (rbase + rw < K)
+    for (int e = lane; e < 16 * kPW; e += 64) {
+      const int c = e % kPW, i = rw + e / kPW, r = rbase + i;
+      if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sC[c][i];
+    }
 }
 
 // Trailing update of one panel step, one workgroup per 64x64 tile of the lower trailing
@@ -908,8 +1013,6 @@ __global__ void __launch_bounds__(kThreads)
 // down 16 consecutive rows of the column-major front (coalesced read-modify-write).  The
 // accumulators start from the tile's current values.  The tile (0, 0), which holds the
 // next panel's diagonal block, then factors that block.
-typedef double mfma_d4 __attribute__((ext_vector_type(4)));
-
 __global__ void __launch_bounds__(kThreads)
     k_mf_update(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc) {
   static_assert(kTile == 64 && kThreads == 256 && kPW % 4 == 0, "4 wavefronts of 32x32");
@@ -920,6 +1023,7 @@ __global__ void __launch_bounds__(kThreads)
   double* A = f.front + f.sn_off[X];
   const double* Lo = f.lbuf + f.sn_loff[X];
   const double* Wo = f.wbuf + f.sn_woff[X];
+  const DiagNode dn = diag_node(f, X);
   const int k0 = kPW * panel;
   const int kend = min(K, k0 + kPW);
   const int np = kend - k0;
@@ -928,7 +1032,7 @@ __global__ void __launch_bounds__(kThreads)
   const int lr = lane & 15, lc = lane >> 4;
   const int rb = 32 * (wv >> 1), cb = 32 * (wv & 1);  // the wavefront's quadrant
   __shared__ double sA[kPW][kTile + 1], sB[kPW][kTile + 1];
-  __shared__ double sL[42], sdi[6];
+  __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   // accumulator (mi, nj) component q is the tile element
   //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
   // it starts from the element's current value (loaded first)
@@ -943,15 +1047,13 @@ __global__ void __launch_bounds__(kThreads)
         acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
       }
   constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
-  stage_lds<kStage>(
+  stage_lds2<kStage>(
       kPW * kTile,
       [&](int e, bool ok) {
         const int t = e / kTile, r = r0 + e % kTile;
         return load_or_zero(Lo, int64_t(k0 + t) * F + r, ok && t < np && r < F);
       },
-      [&](int e, double v) { sA[e / kTile][e % kTile] = v; });
-  stage_lds<kStage>(
-      kPW * kTile,
+      [&](int e, double v) { sA[e / kTile][e % kTile] = v; },
       [&](int e, bool ok) {
         const int t = e / kTile, c = c0 + e % kTile;
         return load_or_zero(Wo, int64_t(t) * F + c, ok && t < np && c < F);
@@ -997,7 +1099,7 @@ __global__ void __launch_bounds__(kThreads)
           if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
         }
     __syncthreads();
-    factor_panel_diag(f, X, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true);
+    factor_panel_diag(f, dn, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true);
   }
 }
 

@@ -81,6 +81,8 @@ struct DevFactor {
   const int32_t* panel_items;  // (node, kPanelRows-row chunk) per panel-row workgroup
   double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
   const int64_t* sn_woff;
+  double* tinv;                // per panel kPW x kPW row-major: L_PP^-T (unit block diagonal)
+  const int64_t* sn_poff;
   double* ltbuf;               // per node K x K: L_KK row-major (backward solve)
   const int64_t* sn_toff;
   double* front;   // all fronts, column-major (workspace)

@@ -288,6 +288,7 @@ struct sgpu_optimizer {
     df.panel_items = arena.upload(S.panel_items, stream);
     df.sn_woff = arena.upload(S.sn_woff, stream);
     df.sn_toff = arena.upload(S.sn_toff, stream);
+    df.sn_poff = arena.upload(S.sn_poff, stream);
     df.solve_rows = arena.upload(S.solve_rows, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
     df.ytmp = arena.alloc<double>(size_t(6) * S.nb);
@@ -299,6 +300,7 @@ struct sgpu_optimizer {
       df.lbuf = arena.alloc<double>(size_t(S.l_elems));
       df.wbuf = arena.alloc<double>(size_t(S.w_elems));
       df.ltbuf = arena.alloc<double>(size_t(S.t_elems));
+      df.tinv = arena.alloc<double>(size_t(S.p_elems));
     }
     df.piv_L = arena.alloc<double>(size_t(36) * S.nb);
     df.piv_d = arena.alloc<double>(size_t(6) * S.nb);

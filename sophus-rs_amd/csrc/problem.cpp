@@ -592,11 +592,16 @@ void analyze_rcs(Symbolic& S) {
   S.panel_items.clear();
   S.sn_woff.assign(S.nsn, 0);
   S.sn_toff.assign(S.nsn, 0);
+  S.sn_poff.assign(S.nsn, 0);
   S.w_elems = 0;
   S.t_elems = 0;
+  S.p_elems = 0;
   for (int32_t x = 0; x < S.nsn; ++x) {
     S.sn_woff[x] = S.w_elems;
     S.w_elems += int64_t(6) * S.sn_nf[x] * 6 * kPanelBlocks;
+    S.sn_poff[x] = S.p_elems;
+    S.p_elems += int64_t(36) * kPanelBlocks * kPanelBlocks *
+                 ((S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks);
     S.sn_toff[x] = S.t_elems;
     S.t_elems += int64_t(36) * S.sn_nc[x] * S.sn_nc[x];
   }

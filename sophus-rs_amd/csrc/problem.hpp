@@ -106,6 +106,8 @@ struct Symbolic {
   std::vector<int64_t> sn_toff;       // row-major copies of each node's L_KK (K x K)
   int64_t t_elems = 0;
   int64_t w_elems = 0;
+  std::vector<int64_t> sn_poff;       // per node: its panels' L_PP^-T (kPW x kPW each)
+  int64_t p_elems = 0;
   // solve work items per level: (node, kSolveRows-row tile below K), (node,
   // kSolveCols-column tile of K)
   std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;

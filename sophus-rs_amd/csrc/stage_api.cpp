@@ -197,6 +197,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.panel_items = bufs.up(S.panel_items);
     df.sn_woff = bufs.up(S.sn_woff);
     df.sn_toff = bufs.up(S.sn_toff);
+    df.sn_poff = bufs.up(S.sn_poff);
     df.solve_rows = bufs.up(S.solve_rows);
     df.solve_cols = bufs.up(S.solve_cols);
     df.ytmp = bufs.alloc<double>(size_t(6) * nb);
@@ -207,6 +208,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
     df.wbuf = bufs.alloc<double>(size_t(S.w_elems));
     df.ltbuf = bufs.alloc<double>(size_t(S.t_elems));
+    df.tinv = bufs.alloc<double>(size_t(S.p_elems));
     df.piv_L = bufs.alloc<double>(size_t(36) * nb);
     df.piv_d = bufs.alloc<double>(size_t(6) * nb);
     df.piv_M = bufs.alloc<double>(size_t(36) * nb);
