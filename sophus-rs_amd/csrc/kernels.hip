@@ -668,6 +668,61 @@ __device__ __forceinline__ double (*as_square(double (*t)[kTile + 1]))[kPW + 1] 
   return reinterpret_cast<double (*)[kPW + 1]>(&t[0][0]);
 }
 
+// Trailing update of pivot JB of a diagonal block: the rows below the next pivot block,
+// four threads per row (thread t3 of 192), each forming the row's L from its W and
+// updating every fourth column; the updated values of the next pivot's columns are also
+// kept in Wd (the next pivot's W).  The pivot index is a template argument so that the
+// column loop has a fixed length and its LDS reads can be issued together.
+template <int JB>
+__device__ __forceinline__ void diag_trailing(int t3, int np, const double* cL, const double* cdi,
+                                              double (*D)[kPW + 1], double (*Wd)[kPW + 1]) {
+  constexpr int J0 = 6 * JB, J1 = J0 + 6, kSplit = 4;
+  const int rr = t3 / kSplit, cg = t3 % kSplit;
+  const int r = J1 + 6 + rr;
+  if (J1 + 6 >= kPW || r >= np) return;
+  double x[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) x[t] = Wd[r][J0 + t];
+  right_solve6(cL, cdi, x);
+  if (cg == 0)
+#pragma unroll
+    for (int t = 0; t < 6; ++t) D[r][J0 + t] = x[t];
+  constexpr int kCols = J1 < kPW ? (kPW - 1 - J1) / kSplit + 1 : 0;
+#pragma unroll
+  for (int i = 0; i < kCols; ++i) {
+    const int c = J1 + cg + kSplit * i;
+    if (c <= r) {
+      double acc = 0.0;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) acc += x[t] * Wd[c][J0 + t];
+      const double v = D[r][c] - acc;
+      D[r][c] = v;
+      if (i < 2 && c < J1 + 6) Wd[r][c] = v;
+    }
+  }
+}
+
+// Row block JB of T = L^-1 for the unit block-lower L held below the pivot blocks of D
+// (its column blocks < JB are final), kept transposed in D's free upper triangle:
+// D[c][r] = T[r][c] for block(c) < block(r).  Entries slot, slot + stride, ...; each a
+// fixed-length sum (terms outside the entry's range masked) in four partial sums.
+template <int JB>
+__device__ __forceinline__ void diag_trows(int slot, int stride, double (*D)[kPW + 1]) {
+  constexpr int J0 = 6 * JB;
+  for (int e = slot; e < 6 * J0; e += stride) {
+    const int a = e % 6, c = e / 6, cb = c / 6;
+    double acc[4] = {-D[J0 + a][c], 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kb = 1; kb < JB; ++kb)
+#pragma unroll
+      for (int t = 0; t < 6; ++t) {
+        const double tv = D[c][6 * kb + t];
+        acc[(6 * kb + t) & 3] -= D[J0 + a][6 * kb + t] * (kb > cb ? tv : 0.0);
+      }
+    D[c][J0 + a] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  }
+}
+
 // The per-node values a diagonal factorization needs, read at kernel entry (ahead of the
 // barriers, which would otherwise keep these loads from being hoisted).
 struct DiagNode {
@@ -784,51 +839,39 @@ __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double
         if (tid == 0) factor_pivot(a6, jb + 1);
       }
     } else {
-      const int t3 = tid - 64;  // 0 .. kThreads - 65
-      constexpr int kRowSplit = 4;
-      static_assert((kThreads - 64) == kRowSplit * kPW, "four threads per block row");
-      const int rr = t3 / kRowSplit, cg = t3 % kRowSplit;
-      const int r = j1 + 6 + rr;
-      if (r < np) {
-        double x[6];
-#pragma unroll
-        for (int t = 0; t < 6; ++t) x[t] = Wd[r][j0 + t];
-        right_solve6(cL, cdi, x);
-        if (cg == 0)
-#pragma unroll
-          for (int t = 0; t < 6; ++t) D[r][j0 + t] = x[t];
-        for (int c = j1 + cg; c <= r; c += kRowSplit) {
-          double acc = 0.0;
-#pragma unroll
-          for (int t = 0; t < 6; ++t) acc += x[t] * Wd[c][j0 + t];
-          const double v = D[r][c] - acc;
-          D[r][c] = v;
-          if (c < j1 + 6) Wd[r][c] = v;  // the next pivot's W
-        }
+      static_assert(kThreads - 64 == 4 * kPW && kPanelBlocks == 8, "four threads per block row");
+      switch (jb) {
+#define SGPU_TRAIL(J) \
+  case J: diag_trailing<J>(tid - 64, np, cL, cdi, D, Wd); break;
+        SGPU_TRAIL(0) SGPU_TRAIL(1) SGPU_TRAIL(2) SGPU_TRAIL(3)
+        SGPU_TRAIL(4) SGPU_TRAIL(5) SGPU_TRAIL(6) SGPU_TRAIL(7)
+#undef SGPU_TRAIL
       }
-      // row block jb of T = L_PP^-1, kept transposed in the free upper triangle:
-      // D[c][r] = T[r][c] for block(c) < block(r); four partial sums per entry.  The
-      // entries go to wavefront 1 first (its rows have the least trailing work), then 3,
-      // then 2.
-      const int wv3 = t3 >> 6, pos = wv3 == 0 ? 0 : (wv3 == 2 ? 1 : 2);
-      for (int e = 64 * pos + (t3 & 63); e < 6 * j0; e += kThreads - 64) {
-        const int a = e % 6, c = e / 6;
-        double acc[4] = {-D[j0 + a][c], 0.0, 0.0, 0.0};
-        const int c1 = 6 * (c / 6 + 1);
-        for (int cp = c1; cp < j0; cp += 4)
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            if (cp + u < j0) acc[u] -= D[j0 + a][cp + u] * D[c][cp + u];
-        D[c][j0 + a] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    }
+    // Beside the look-ahead: row block jb of T = L_PP^-1 (its L columns are final), kept
+    // transposed in the free upper triangle, D[c][r] = T[r][c] for block(c) < block(r),
+    // four partial sums per entry; then the outputs of pivot jb - 1 (M_j, L_jj, d_j) and
+    // the stores of L's column block jb - 1 and T's row block jb - 1.  The entries of T go
+    // first to the wavefronts with the least trailing work (wavefront 1, then 3 and 2 while
+    // wavefront 2's rows are updated, 2 and 3 after); the last pivot has no look-ahead,
+    // so there wavefront 0 joins and also handles that pivot's outputs.
+    const bool look = m > 0;
+    const int wv = tid >> 6, lane = tid & 63;
+    if (!look || wv > 0) {
+      const int pos = !look ? wv : (jb >= 4 ? wv - 1 : (wv == 1 ? 0 : (wv == 3 ? 1 : 2)));
+      const int slot = 64 * pos + lane, stride = look ? kThreads - 64 : kThreads;
+      switch (jb) {
+#define SGPU_TROWS(J) \
+  case J: diag_trows<J>(slot, stride, D); break;
+        SGPU_TROWS(1) SGPU_TROWS(2) SGPU_TROWS(3) SGPU_TROWS(4)
+        SGPU_TROWS(5) SGPU_TROWS(6) SGPU_TROWS(7)
+#undef SGPU_TROWS
       }
-      if (tid >= 192) {
-        // wavefront 3: this pivot's M_j and stores, column block jb - 1 of L (final since
-        // the previous pivot) and T's row block jb - 1 to memory
-        const int l3 = tid - 192;
-        const int gb = dn.c0 + (k0 + j0) / 6;
-        if (l3 < 36) {
-          // the pivot's inverse M = L^-T D^+ L^-1 (entry (a, b)) from the columns of L^-1
-          const int a = l3 / 6, b = l3 % 6;
+      // M_j = L^-T D^+ L^-1 (lane (a, b) of 36, from the columns of L^-1), L_jj and d_j
+      auto pivot_outputs = [&](int j) {
+        const int gb = dn.c0 + k0 / 6 + j;
+        if (lane < 36) {
+          const int a = lane / 6, b = lane % 6;
           double ua[6], ub[6];
 #pragma unroll
           for (int k = 0; k < 6; ++k) {
@@ -839,34 +882,35 @@ __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double
           for (int k = 1; k < 6; ++k)
 #pragma unroll
             for (int t = 0; t < k; ++t) {
-              ua[k] -= cL[6 * k + t] * ua[t];
-              ub[k] -= cL[6 * k + t] * ub[t];
+              ua[k] -= sL[j][6 * k + t] * ua[t];
+              ub[k] -= sL[j][6 * k + t] * ub[t];
             }
           double mm = 0.0;
 #pragma unroll
-          for (int k = 0; k < 6; ++k) mm += ua[k] * cdi[k] * ub[k];
-          f.piv_M[36 * int64_t(gb) + l3] = mm;
+          for (int k = 0; k < 6; ++k) mm += ua[k] * sdi[j][k] * ub[k];
+          f.piv_M[36 * int64_t(gb) + lane] = mm;
+          f.piv_L[36 * int64_t(gb) + lane] = sL[j][lane];
+        } else if (lane < 42) {
+          f.piv_d[6 * int64_t(gb) + lane - 36] = sL[j][lane];
         }
-        if (l3 < 42) {
-          if (l3 < 36) f.piv_L[36 * int64_t(gb) + l3] = cL[l3];
-          else f.piv_d[6 * int64_t(gb) + l3 - 36] = cL[l3];
+      };
+      if (jb > 0 && wv == (jb >= 4 ? 2 : 3)) {
+        pivot_outputs(jb - 1);
+        const int p0 = j0 - 6, nr = np - j0;
+        for (int e = lane; e < 6 * nr; e += 64) {
+          const int rr2 = e % nr, t = e / nr;
+          Lout[int64_t(k0 + p0 + t) * F + k0 + j0 + rr2] = D[j0 + rr2][p0 + t];
         }
-        if (jb > 0) {
-          const int p0 = j0 - 6, nr = np - j0;
-          for (int e = l3; e < 6 * nr; e += 64) {
-            const int rr2 = e % nr, t = e / nr;
-            Lout[int64_t(k0 + p0 + t) * F + k0 + j0 + rr2] = D[j0 + rr2][p0 + t];
-          }
-          for (int e = l3; e < 6 * nr; e += 64) {  // row-major copy
-            const int rr2 = e / 6, t = e % 6;
-            Lt[int64_t(k0 + j0 + rr2) * K + k0 + p0 + t] = D[j0 + rr2][p0 + t];
-          }
-          for (int e = l3; e < 6 * p0; e += 64) {
-            const int cp = e / 6, a = e % 6;
-            Tg[cp * kPW + p0 + a] = D[cp][p0 + a];
-          }
+        for (int e = lane; e < 6 * nr; e += 64) {  // row-major copy
+          const int rr2 = e / 6, t = e % 6;
+          Lt[int64_t(k0 + j0 + rr2) * K + k0 + p0 + t] = D[j0 + rr2][p0 + t];
+        }
+        for (int e = lane; e < 6 * p0; e += 64) {
+          const int cp = e / 6, a = e % 6;
+          Tg[cp * kPW + p0 + a] = D[cp][p0 + a];
         }
       }
+      if (!look && wv == 0) pivot_outputs(jb);
     }
   }
   __syncthreads();  // T's last row block is in D
@@ -1149,17 +1193,16 @@ __device__ __forceinline__ void diag_solve6(const double* L, const double* d, do
   }
 }
 
-// Node-local triangular solves: one thread per node row (K <= kTriMax, blockDim a
-// multiple of 64 covering K), 32-column solve panels that sit in one half of one
-// wavefront.  Per panel the owning wavefront finishes the panel's 32 values by
-// column-oriented substitution through lane shuffles (and applies them to its other
-// 32 rows), publishes them to LDS, and after one barrier every later wavefront applies
-// them to its rows.  Each thread's L entries for the next panel are loaded while the
-// current panel is processed.  The unit block diagonal of L is implicit (entries with
-// row block == column block are skipped); D^-1 is applied per 6x6 pivot at the end.
+// Node-local triangular solves, one thread per node row (K <= kTriMax, blockDim a
+// multiple of 64 covering K), panel by panel over the factor's kPW-column panels.  Per
+// panel (two barriers): the panel's rows publish their values; each applies the panel's
+// stored inverse (L_PP^-1, or L_PP^-T backward) to them, a dot product instead of a
+// substitution; then every row on the far side applies the panel's L entries to itself.
+// Each thread holds one kPW-wide operand row in registers: the inverse's row or column
+// while its row is in the panel, its L entries otherwise, loaded one panel ahead.  The
+// unit block diagonal of L is implicit; D^-1 is applied per 6x6 pivot at the end of the
+// forward solve.
 constexpr int kTriMax = 6 * 96;  // largest supernode (96 column blocks)
-constexpr int kSP = 32;
-
 
 __global__ void __launch_bounds__(kTriMax)
     k_mf_fwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y,
@@ -1167,40 +1210,56 @@ __global__ void __launch_bounds__(kTriMax)
   const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* A = f.lbuf + f.sn_loff[X];
-  __shared__ double yl[kTriMax];
-  const int r = threadIdx.x, wave = r >> 6, rb = r / 6;
+  const double* Tn = f.tinv + f.sn_poff[X];
+  __shared__ double yl[kTriMax], rl[kPW];
+  const int r = threadIdx.x;
   const bool own = r < K;
   // children's updates to this node's rows are in u_X
   double v = own ? y[6 * c0 + r] - f.usolve[f.sn_uoff[X] + r] : 0.0;
-  yl[r] = 0.0;
-  const int npan = (K + kSP - 1) / kSP;
-  double cur[kSP], nxt[kSP];
+  const int npan = (K + kPW - 1) / kPW;
+  for (int e = r; e < kTriMax; e += blockDim.x) yl[e] = 0.0;
+  // the operand row for panel p: column r - k0 of L_PP^-1 (row-major L_PP^-T holds it as
+  // a column) in the panel, this row's entries of the panel's columns below it
+  double cur[kPW];
+  auto load_panel = [&](int p) {
+    const int k0 = kPW * p, npp = min(kPW, K - k0);
+    const bool in_panel = r >= k0 && r < k0 + npp;
+    const double* Tg = Tn + int64_t(p) * kPW * kPW;
+    // wavefronts with a row in or below the panel: one load per element from a selected
+    // (always valid) address and no select on the value, so the loads stay in flight
+    // across the next barriers; the entries past the panel's width meet zeros (L_PP^-1 is
+    // zero there, yl past K)
+    if (__any(in_panel || (own && r >= k0 + npp))) {
 #pragma unroll
-  for (int t = 0; t < kSP; ++t)
-    cur[t] = load_or_zero(A + int64_t(min(t, K - 1)) * F, r, own && t < K && t / 6 < rb);
-  __syncthreads();
-  for (int p = 0; p < npan; ++p) {
-    const int k0 = kSP * p, pw = k0 >> 6;
-    if (p + 1 < npan) {
-      const int k1 = k0 + kSP;
-#pragma unroll
-      for (int t = 0; t < kSP; ++t) {
-        const int c = k1 + t;
-        nxt[t] = load_or_zero(A + int64_t(min(c, K - 1)) * F, r, own && c < K && c / 6 < rb);
+      for (int c = 0; c < kPW; ++c) {
+        const uintptr_t at = reinterpret_cast<uintptr_t>(Tg + c * kPW + (in_panel ? r - k0 : 0));
+        const uintptr_t aa =
+            reinterpret_cast<uintptr_t>(A + int64_t(k0 + min(c, npp - 1)) * F + (own ? r : 0));
+        cur[c] = *reinterpret_cast<const double*>(in_panel ? at : aa);
       }
     }
-    if (wave == pw) {  // wave-uniform: the panel's owning wavefront
+  };
+  load_panel(0);
+  for (int p = 0; p < npan; ++p) {
+    const int k0 = kPW * p, npp = min(kPW, K - k0);
+    const bool in_panel = r >= k0 && r < k0 + npp;
+    if (in_panel) rl[r - k0] = v;
+    __syncthreads();
+    if (in_panel) {
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int t = 0; t < kSP; ++t) v -= cur[t] * __shfl(v, (k0 + t) & 63, 64);
-      if (r >= k0 && r < k0 + kSP) yl[r] = v;
+      for (int c = 0; c < kPW; ++c) s[c & 3] += cur[c] * rl[min(c, npp - 1)];  // 0 past npp
+      v = (s[0] + s[1]) + (s[2] + s[3]);
+      yl[r] = v;
     }
     __syncthreads();
-    if (wave > pw) {
+    if (own && r >= k0 + npp) {
+      double s[4] = {v, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int t = 0; t < kSP; ++t) v -= cur[t] * yl[k0 + t];
+      for (int c = 0; c < kPW; ++c) s[c & 3] -= cur[c] * yl[k0 + c];  // yl is 0 past K
+      v = (s[0] + s[1]) + (s[2] + s[3]);
     }
-#pragma unroll
-    for (int t = 0; t < kSP; ++t) cur[t] = nxt[t];
+    if (p + 1 < npan) load_panel(p + 1);
   }
   if (own) ytmp[6 * c0 + r] = v;
   if (r < K / 6) {
@@ -1315,42 +1374,55 @@ __global__ void __launch_bounds__(kTriMax)
   const int X = nodes[blockIdx.x];
   const int K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   const double* Lt = f.ltbuf + f.sn_toff[X];
-  __shared__ double xl[kTriMax];
-  const int c = threadIdx.x, wave = c >> 6, cb = c / 6;
+  const double* Tn = f.tinv + f.sn_poff[X];
+  __shared__ double xl[kPW], xs[kTriMax];
+  const int c = threadIdx.x;
   const bool own = c < K;
   double x = own ? y[6 * c0 + c] : 0.0;
-  xl[c] = 0.0;
-  const int npan = (K + kSP - 1) / kSP;
-  double cur[kSP], nxt[kSP];
-  {
-    const int k0 = kSP * (npan - 1);
+  const int npan = (K + kPW - 1) / kPW;
+  for (int e = c; e < kTriMax; e += blockDim.x) xs[e] = 0.0;
+  // the operand row for panel p: row c - k0 of L_PP^-T in the panel, the panel rows'
+  // entries of this column (row-major copy of L) above it
+  double cur[kPW];
+  auto load_panel = [&](int p) {
+    const int k0 = kPW * p, npp = min(kPW, K - k0);
+    const bool in_panel = c >= k0 && c < k0 + npp;
+    const double* Tg = Tn + int64_t(p) * kPW * kPW;
+    // wavefronts with a column in or above the panel: one load per element from a
+    // selected (always valid) address and no select on the value, so the loads stay in
+    // flight across the next barriers; the entries past the panel's width meet zeros
+    // (L_PP^-T is zero there, xs past K)
+    if (__any(in_panel || c < k0)) {
 #pragma unroll
-    for (int t = 0; t < kSP; ++t) {
-      const int r = min(k0 + t, K - 1);
-      cur[t] = load_or_zero(Lt + int64_t(r) * K, c, own && k0 + t < K && (k0 + t) / 6 > cb);
+      for (int i = 0; i < kPW; ++i) {
+        const uintptr_t at = reinterpret_cast<uintptr_t>(Tg + (in_panel ? (c - k0) * kPW : 0) + i);
+        const uintptr_t al =
+            reinterpret_cast<uintptr_t>(Lt + int64_t(k0 + min(i, npp - 1)) * K + (own ? c : 0));
+        cur[i] = *reinterpret_cast<const double*>(in_panel ? at : al);
+      }
     }
-  }
-  __syncthreads();
+  };
+  load_panel(npan - 1);
   for (int p = npan - 1; p >= 0; --p) {
-    const int k0 = kSP * p, pw = k0 >> 6;
-    if (p > 0) {
-      const int k1 = k0 - kSP;
+    const int k0 = kPW * p, npp = min(kPW, K - k0);
+    const bool in_panel = c >= k0 && c < k0 + npp;
+    if (in_panel) xl[c - k0] = x;
+    __syncthreads();
+    if (in_panel) {
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int t = 0; t < kSP; ++t)
-        nxt[t] = load_or_zero(Lt + int64_t(k1 + t) * K, c, own && (k1 + t) / 6 > cb);
-    }
-    if (wave == pw) {
-#pragma unroll
-      for (int t = kSP - 1; t >= 0; --t) x -= cur[t] * __shfl(x, (k0 + t) & 63, 64);
-      if (c >= k0 && c < k0 + kSP) xl[c] = x;
+      for (int i = 0; i < kPW; ++i) s[i & 3] += cur[i] * xl[min(i, npp - 1)];  // 0 past npp
+      x = (s[0] + s[1]) + (s[2] + s[3]);
+      xs[c] = x;
     }
     __syncthreads();
-    if (wave < pw) {
+    if (c < k0) {
+      double s[4] = {x, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int t = 0; t < kSP; ++t) x -= cur[t] * xl[k0 + t];
+      for (int i = 0; i < kPW; ++i) s[i & 3] -= cur[i] * xs[k0 + i];  // xs is 0 past K
+      x = (s[0] + s[1]) + (s[2] + s[3]);
     }
-#pragma unroll
-    for (int t = 0; t < kSP; ++t) cur[t] = nxt[t];
+    if (p > 0) load_panel(p - 1);
   }
   if (own) y[6 * c0 + c] = x;
 }
