@@ -941,7 +941,7 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 // tiles of v_mfma_f64_16x16x4, three per wavefront.  W goes to the node's W scratch, L to
 // the factor and, for the node's own rows, to its row-major copy.
 __global__ void __launch_bounds__(kThreads)
-    k_mf_panel(DevFactor f, int panel, const int32_t* __restrict__ items) {
+    k_mf_panel(DevFactor f, int panel, const int32_t* __restrict__ items, double* __restrict__ z) {
   static_assert(kPanelRows == 64 && kPW == 48 && kThreads == 256, "4 x 3 tiles, 3 per wavefront");
   static_assert(kPW * kPanelRows % kThreads == 0 && kPW * kPW % kThreads == 0, "whole staging rounds");
   const int X = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
@@ -958,6 +958,8 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double sT[kPW][kPW + 1];         // L_PP^-T
   __shared__ double sW[kPanelRows][kPW + 1];
   __shared__ double pM[kPanelBlocks][36];
+  __shared__ double sb[kPW], sy[kThreads / 64][kPW];  // the panel's RHS; y_P per wavefront
+  double* bx = f.usolve + f.sn_uoff[X];  // the node's right-hand side (front row order)
   const int gb0 = f.sn_c0[X] + k0 / 6, nbp = np / 6;
   // every global load of the thread first, then the LDS stores
   constexpr int kNC = kPW * kPanelRows / kThreads, kNT = kPW * kPW / kThreads;
@@ -975,6 +977,7 @@ __global__ void __launch_bounds__(kThreads)
     const int e = tid + kThreads * it;
     vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
   }
+  const double vb = load_or_zero(bx, k0 + tid, tid < np);
 #pragma unroll
   for (int it = 0; it < kNC; ++it) {
     const int e = tid + kThreads * it;
@@ -990,7 +993,23 @@ __global__ void __launch_bounds__(kThreads)
     const int e = tid + kThreads * it;
     if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
   }
+  if (tid < kPW) sb[tid] = vb;
   __syncthreads();
+  // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront forms
+  // its own copy; lane c < kPW entry c), and z_P = D^-1 y_P to the solve vector
+  if (lane < kPW) {
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
+    sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+  }
+  if (chunk == 0 && wv == 0 && lane < np) {
+    const int jb = lane / 6, a = lane % 6;
+    double zz = 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[0][6 * jb + t];
+    z[6 * int64_t(f.sn_c0[X]) + k0 + lane] = zz;
+  }
   // Wavefront wv owns row tile wv (16 rows) and its three 16-column tiles; it reads only
   // its own rows of C, W and L after the barrier above, so it needs no further barrier.
   // W = C L_PP^-T: L_PP^-T is block upper triangular, so column tile ct stops at row
@@ -1034,6 +1053,17 @@ __global__ void __launch_bounds__(kThreads)
   for (int ct = 0; ct < 3; ++ct)
 #pragma unroll
     for (int q = 0; q < 4; ++q) sC[16 * ct + lr][rw + lk + 4 * q] = acc[ct][q];
+  // the right-hand side of the wavefront's 16 rows: b_r -= L_r y_P (four lanes per row,
+  // every fourth column, combined by two xor shuffles)
+  {
+    const int i = rw + (lane >> 2), part = lane & 3, r = rbase + i;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kPW / 4; ++u) acc += sC[part + 4 * u][i] * sy[wv][part + 4 * u];
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    if (part == 0 && r < F) bx[r] -= acc;
+  }
   // the wavefront's 16 rows to memory: W and L column-major, and for the node's own rows
   // the row-major copy of L
   for (int e = lane; e < 16 * kPW; e += 64) {
@@ -1147,9 +1177,19 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// Extend-add: scatter each finished node's update matrix into its parent's front.  The
-// launch's nodes are distinct children of distinct parents (one sibling rank at a time),
-// so the adds need no atomics and their order is fixed.
+// Each node's right-hand side in front row order: its own rows from the permuted RHS, its
+// update rows zero (they gather the node's and its descendants' eliminations).
+__global__ void __launch_bounds__(kThreads) k_mf_rhs_init(DevFactor f, const double* __restrict__ b) {
+  const int X = blockIdx.x;
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  double* bx = f.usolve + f.sn_uoff[X];
+  for (int r = threadIdx.x; r < F; r += blockDim.x) bx[r] = r < K ? b[6 * int64_t(c0) + r] : 0.0;
+}
+
+// Extend-add: scatter each finished node's update matrix, and the update rows of its
+// right-hand side, into its parent's front.  The launch's nodes are distinct children of
+// distinct parents (one sibling rank at a time), so the adds need no atomics and their
+// order is fixed.
 __global__ void __launch_bounds__(kThreads)
     k_mf_extend(DevFactor f, const int32_t* __restrict__ nodes) {
   const int X = nodes[blockIdx.y];
@@ -1160,6 +1200,11 @@ __global__ void __launch_bounds__(kThreads)
   const double* U = f.front + f.sn_off[X] + int64_t(K) * F + K;
   double* B = f.front + f.sn_off[P];
   const int32_t* up = f.sn_upos + f.sn_upos_ptr[X];
+  if (blockIdx.x == 0) {  // the update rows' right-hand side into the parent's
+    const double* bx = f.usolve + f.sn_uoff[X] + K;
+    double* bp = f.usolve + f.sn_uoff[P];
+    for (int r = threadIdx.x; r < m; r += blockDim.x) bp[6 * up[r / 6] + r % 6] += bx[r];
+  }
   for (int c = blockIdx.x; c < m; c += gridDim.x) {
     const int pc = 6 * up[c / 6] + c % 6;
     for (int r = c + threadIdx.x; r < m; r += blockDim.x) {
@@ -1171,160 +1216,18 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // ---- solves with the supernodal factor ----------------------------------------------------
-// Per level (children first): y_K <- L_KK^-1 y_K (node's own rows, one workgroup per
-// node), then the rows below: y_rows -= L_rect y_K (many workgroups, atomics into the
-// ancestors' entries), z_K = D^-1 y_K.  Backward (parents first): t_K = z_K - L_rect^T
-// x_rows (column tiles), then x_K = L_KK^-T t_K (one workgroup per node).
-
-__device__ __forceinline__ void diag_solve6(const double* L, const double* d, double v[6]) {
-#pragma unroll
-  for (int i = 0; i < 6; ++i) {
-    double s = v[i];
-    for (int k = 0; k < i; ++k) s -= L[6 * i + k] * v[k];
-    v[i] = s;
-  }
-#pragma unroll
-  for (int i = 0; i < 6; ++i) v[i] = d[i] == 0.0 ? 0.0 : v[i] / d[i];
-#pragma unroll
-  for (int i = 5; i >= 0; --i) {
-    double s = v[i];
-    for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * v[k];
-    v[i] = s;
-  }
-}
-
-// Node-local triangular solves, one thread per node row (K <= kTriMax, blockDim a
-// multiple of 64 covering K), panel by panel over the factor's kPW-column panels.  Per
-// panel (two barriers): the panel's rows publish their values; each applies the panel's
-// stored inverse (L_PP^-1, or L_PP^-T backward) to them, a dot product instead of a
-// substitution; then every row on the far side applies the panel's L entries to itself.
-// Each thread holds one kPW-wide operand row in registers: the inverse's row or column
-// while its row is in the panel, its L entries otherwise, loaded one panel ahead.  The
-// unit block diagonal of L is implicit; D^-1 is applied per 6x6 pivot at the end of the
-// forward solve.
+// The forward solve runs inside the factorization: each node carries its right-hand side
+// in front row order (k_mf_rhs_init), every panel step eliminates it with the panel's
+// L (k_mf_panel, which also writes z_P = D^-1 y_P for the panel's columns) and the
+// extend-add carries the update rows to the parent.  The backward solve, parents first:
+// t_K = z_K - L_rect^T x_rows (column tiles), then x_K = L_KK^-T t_K (one workgroup per
+// node, one thread per row, K <= kTriMax, blockDim a multiple of 64 covering K), panel by
+// panel from the last of the factor's kPW-column panels: the panel's rows publish their
+// values and apply the panel's stored L_PP^-T to them (a dot product instead of a
+// substitution), then every earlier row applies the panel's L entries to itself.  Each
+// thread holds one kPW-wide operand row in registers, loaded a panel ahead: the row of
+// L_PP^-T while its row is in the panel, the panel rows' L entries of its column before.
 constexpr int kTriMax = 6 * 96;  // largest supernode (96 column blocks)
-
-__global__ void __launch_bounds__(kTriMax)
-    k_mf_fwd_tri(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ y,
-                 double* __restrict__ ytmp) {
-  const int X = nodes[blockIdx.x];
-  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
-  const double* A = f.lbuf + f.sn_loff[X];
-  const double* Tn = f.tinv + f.sn_poff[X];
-  __shared__ double yl[kTriMax], rl[kPW];
-  const int r = threadIdx.x;
-  const bool own = r < K;
-  // children's updates to this node's rows are in u_X
-  double v = own ? y[6 * c0 + r] - f.usolve[f.sn_uoff[X] + r] : 0.0;
-  const int npan = (K + kPW - 1) / kPW;
-  for (int e = r; e < kTriMax; e += blockDim.x) yl[e] = 0.0;
-  // the operand row for panel p: column r - k0 of L_PP^-1 (row-major L_PP^-T holds it as
-  // a column) in the panel, this row's entries of the panel's columns below it
-  double cur[kPW];
-  auto load_panel = [&](int p) {
-    const int k0 = kPW * p, npp = min(kPW, K - k0);
-    const bool in_panel = r >= k0 && r < k0 + npp;
-    const double* Tg = Tn + int64_t(p) * kPW * kPW;
-    // wavefronts with a row in or below the panel: one load per element from a selected
-    // (always valid) address and no select on the value, so the loads stay in flight
-    // across the next barriers; the entries past the panel's width meet zeros (L_PP^-1 is
-    // zero there, yl past K)
-    if (__any(in_panel || (own && r >= k0 + npp))) {
-#pragma unroll
-      for (int c = 0; c < kPW; ++c) {
-        const uintptr_t at = reinterpret_cast<uintptr_t>(Tg + c * kPW + (in_panel ? r - k0 : 0));
-        const uintptr_t aa =
-            reinterpret_cast<uintptr_t>(A + int64_t(k0 + min(c, npp - 1)) * F + (own ? r : 0));
-        cur[c] = *reinterpret_cast<const double*>(in_panel ? at : aa);
-      }
-    }
-  };
-  load_panel(0);
-  for (int p = 0; p < npan; ++p) {
-    const int k0 = kPW * p, npp = min(kPW, K - k0);
-    const bool in_panel = r >= k0 && r < k0 + npp;
-    if (in_panel) rl[r - k0] = v;
-    __syncthreads();
-    if (in_panel) {
-      double s[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < kPW; ++c) s[c & 3] += cur[c] * rl[min(c, npp - 1)];  // 0 past npp
-      v = (s[0] + s[1]) + (s[2] + s[3]);
-      yl[r] = v;
-    }
-    __syncthreads();
-    if (own && r >= k0 + npp) {
-      double s[4] = {v, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int c = 0; c < kPW; ++c) s[c & 3] -= cur[c] * yl[k0 + c];  // yl is 0 past K
-      v = (s[0] + s[1]) + (s[2] + s[3]);
-    }
-    if (p + 1 < npan) load_panel(p + 1);
-  }
-  if (own) ytmp[6 * c0 + r] = v;
-  if (r < K / 6) {
-    double w[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) w[k] = yl[6 * r + k];
-    diag_solve6(f.piv_L + 36 * (c0 + r), f.piv_d + 6 * (c0 + r), w);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) y[6 * (c0 + r) + k] = w[k];
-  }
-}
-
-// rows below the node's own: u_X[rows] += L_rect y_K, completing the node's update to
-// its ancestors' rows (children's updates are already in u_X); items = (node, tile of
-// kSolveRows rows).  Each row's dot product over the K columns is split over the 4
-// wavefronts (columns g, g + 4, ...) and summed in a fixed order.
-__global__ void __launch_bounds__(kThreads)
-    k_mf_fwd_rect(DevFactor f, const int32_t* __restrict__ items, const double* __restrict__ ytmp) {
-  static_assert(kSolveRows == 64, "one row per lane");
-  const int X = items[2 * blockIdx.x], tile = items[2 * blockIdx.x + 1];
-  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
-  const double* A = f.lbuf + f.sn_loff[X];
-  double* ux = f.usolve + f.sn_uoff[X];
-  __shared__ double yk[kTriMax];
-  __shared__ double part[kThreads / 64][64];
-  stage_lds<kTriMax / kThreads + 1>(
-      K, [&](int e, bool ok) { return load_or_zero(ytmp, 6 * c0 + e, ok); },
-      [&](int e, double v) { yk[e] = v; });
-  __syncthreads();
-  const int q = threadIdx.x & 63, g = threadIdx.x >> 6;
-  const int r = K + kSolveRows * tile + q;
-  const bool in = r < F;
-  double s0 = 0.0, s1 = 0.0;
-  for (int cb = g; cb < K; cb += 32) {  // 8 columns (stride 4) per batch of loads
-    double av[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int c = cb + 4 * u;
-      av[u] = load_or_zero(A + int64_t(min(c, K - 1)) * F, r, in && c < K);
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u += 2) {
-      const int c = cb + 4 * u;
-      if (c < K) s0 += av[u] * yk[c];
-      if (c + 4 < K) s1 += av[u + 1] * yk[c + 4];
-    }
-  }
-  part[g][q] = s0 + s1;
-  __syncthreads();
-  if (g == 0 && in) ux[r] += (part[0][q] + part[1][q]) + (part[2][q] + part[3][q]);
-}
-
-// u_P[parent rows] += u_X[rows below K] for one sibling rank of a level (plain adds).
-__global__ void __launch_bounds__(kThreads)
-    k_mf_fwd_extend(DevFactor f, const int32_t* __restrict__ nodes) {
-  const int X = nodes[blockIdx.y];
-  const int P = f.sn_parent[X];
-  if (P < 0) return;
-  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], m = F - K;
-  const double* ux = f.usolve + f.sn_uoff[X] + K;
-  double* up_vec = f.usolve + f.sn_uoff[P];
-  const int32_t* up = f.sn_upos + f.sn_upos_ptr[X];
-  for (int r = blockIdx.x * kThreads + threadIdx.x; r < m; r += gridDim.x * kThreads)
-    up_vec[6 * up[r / 6] + r % 6] += ux[r];
-}
 
 // t_K = z_K - L_rect^T x_rows for kSolveCols node columns per workgroup (4 columns per
 // wavefront, accumulated side by side; lanes stride over the rows below the node);
@@ -1933,8 +1836,13 @@ static void launch_extend_groups(hipStream_t s, const DevFactor& f, int level) {
   }
 }
 
-void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
+void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const double* rhs_nat,
+                       double* y) {
   const Symbolic& S = *f.host;
+  // the right-hand side in elimination order, then into each node's front rows
+  const int g = blocks_for(6 * int64_t(f.nb), kThreads);
+  hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, rhs_nat, y);
+  hipLaunchKernelGGL(k_mf_rhs_init, dim3(S.nsn), dim3(kThreads), 0, s, f, y);
   int level = -1;
   for (const auto& L : S.launches) {
     if (L.level != level) {
@@ -1945,10 +1853,11 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc) {
       const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
       hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level], sc);
     }
-    if (L.tile_count > 0) {
-      const size_t li = size_t(&L - S.launches.data());
+    const size_t li = size_t(&L - S.launches.data());
+    if (L.item_count > 0)
       hipLaunchKernelGGL(k_mf_panel, dim3(L.item_count), dim3(kThreads), 0, s, f, L.panel,
-                         f.panel_items + 2 * int64_t(L.item_begin));
+                         f.panel_items + 2 * int64_t(L.item_begin), y);
+    if (L.tile_count > 0) {
       if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
       hipLaunchKernelGGL(k_mf_update, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
                          f.launch_tiles + 3 * int64_t(L.tile_begin), sc);
@@ -1964,28 +1873,10 @@ static int tri_threads(const Symbolic& S, int level) {
   return std::max(64, (K + 63) / 64 * 64);
 }
 
-void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, double* y,
-                      double* dx_cam) {
+void launch_rcs_solve(hipStream_t s, const DevFactor& f, double* y, double* dx_cam) {
   const Symbolic& S = *f.host;
   const int g = blocks_for(6 * int64_t(f.nb), kThreads);
-  hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, rhs_nat, y);
-  launch_fill(s, f.usolve, S.u_elems, 0.0);
   const int nlev = int(S.lvl_ptr.size()) - 1;
-  for (int l = 0; l < nlev; ++l) {
-    const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
-    const int tt = tri_threads(S, l);
-    hipLaunchKernelGGL(k_mf_fwd_tri, dim3(n), dim3(tt), 0, s, f, f.lvl_sn + S.lvl_ptr[l], y, f.ytmp);
-    const int nr = S.solve_rows_ptr[l + 1] - S.solve_rows_ptr[l];
-    if (nr > 0)
-      hipLaunchKernelGGL(k_mf_fwd_rect, dim3(nr), dim3(kThreads), 0, s, f,
-                         f.solve_rows + 2 * int64_t(S.solve_rows_ptr[l]), f.ytmp);
-    for (int q = S.ext_lvl_ptr[l]; q < S.ext_lvl_ptr[l + 1]; ++q) {
-      const int cnt = S.ext_grp_ptr[q + 1] - S.ext_grp_ptr[q];
-      if (cnt > 0)
-        hipLaunchKernelGGL(k_mf_fwd_extend, dim3(8, cnt), dim3(kThreads), 0, s, f,
-                           f.ext_nodes + S.ext_grp_ptr[q]);
-    }
-  }
   for (int l = nlev - 1; l >= 0; --l) {
     const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
     const int nc = S.solve_cols_ptr[l + 1] - S.solve_cols_ptr[l];

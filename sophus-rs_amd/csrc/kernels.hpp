@@ -71,12 +71,10 @@ struct DevFactor {
   const int32_t* sn_upos;
   const int32_t* lvl_sn;
   const int32_t* launch_tiles;
-  const int32_t* solve_rows;  // (node, 256-row tile) per level, rows below the node
   const int32_t* solve_cols;  // (node, 64-column tile) per level
-  double* ytmp;               // nb x 6 solve scratch
   const int32_t* ext_nodes;   // extend-add groups (see Symbolic)
-  const int64_t* sn_uoff;     // forward-solve update vector offsets
-  double* usolve;             // forward-solve update vectors (u_elems)
+  const int64_t* sn_uoff;     // right-hand side offsets (F entries per node)
+  double* usolve;             // each node's right-hand side in front row order (u_elems)
   hipEvent_t* step_events;    // optional: 2 per panel launch, recorded around k_mf_update
   const int32_t* panel_items;  // (node, kPanelRows-row chunk) per panel-row workgroup
   double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
@@ -124,11 +122,12 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
                         const double* intr, const double* cw, double nu, DevLin& w,
                         DevFactor& f, DevScalars* sc);
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v);
-// Full numeric factorization (all levels and panels) of the assembled fronts.
-void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc);
-// Solve with the factor: rhs in camera order -> dx in camera order (y: nb*6 scratch).
-void launch_rcs_solve(hipStream_t s, const DevFactor& f, const double* rhs_nat, double* y,
-                      double* dx_cam);
+// Full numeric factorization (all levels and panels) of the assembled fronts, with the
+// forward solve of rhs (camera order) folded in: y (nb*6) receives z = D^-1 L^-1 P rhs.
+void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const double* rhs_nat,
+                       double* y);
+// Backward solve with the factor: y (from launch_rcs_factor) -> dx in camera order.
+void launch_rcs_solve(hipStream_t s, const DevFactor& f, double* y, double* dx_cam);
 void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p,
                            const double* points, const double* intr, const double* cw,
                            const DevLin& w, double* points_new, DevScalars* sc);

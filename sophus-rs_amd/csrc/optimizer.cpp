@@ -289,9 +289,7 @@ struct sgpu_optimizer {
     df.sn_woff = arena.upload(S.sn_woff, stream);
     df.sn_toff = arena.upload(S.sn_toff, stream);
     df.sn_poff = arena.upload(S.sn_poff, stream);
-    df.solve_rows = arena.upload(S.solve_rows, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
-    df.ytmp = arena.alloc<double>(size_t(6) * S.nb);
     df.ext_nodes = arena.upload(S.ext_nodes, stream);
     df.sn_uoff = arena.upload(S.sn_uoff, stream);
     df.usolve = arena.alloc<double>(size_t(S.u_elems));
@@ -400,11 +398,11 @@ struct sgpu_optimizer {
       tic(K_TRIANGULAR);
     } else {
       tic(K_FACTOR);
-      launch_rcs_factor(stream, df, d_sc);
+      launch_rcs_factor(stream, df, d_sc, dl.rhs_nat, dl.x_perm);
       toc(K_FACTOR);
       SGPU_HIP(hipEventRecord(phase_ev[2], stream));
       tic(K_TRIANGULAR);
-      launch_rcs_solve(stream, df, dl.rhs_nat, dl.x_perm, dl.dx_cam);
+      launch_rcs_solve(stream, df, dl.x_perm, dl.dx_cam);
     }
     // the mixed path solves for centred camera tangents (see k_schur_slots)
     if (params.precision == SGPU_PRECISION_MIXED) launch_uncenter(stream, dp, dl.cw, dl.dx_cam);

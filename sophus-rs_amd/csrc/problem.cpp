@@ -636,7 +636,8 @@ void analyze_rcs(Symbolic& S) {
         L.flops += double(T) * double(T + 1) * np;
         L.bytes += 16.0 * double(T) * double(T + 1) / 2.0 + 16.0 * double(T) * np;
         L.panel_flops += double(T) * (np * np + 12.0 * np);
-        for (int32_t ch = 0; ch * kPanelRows < T; ++ch) {
+        // at least one item per node and panel: it also eliminates the panel's right-hand side
+        for (int32_t ch = 0; ch == 0 || ch * kPanelRows < T; ++ch) {
           S.panel_items.insert(S.panel_items.end(), {x, ch});
           ++L.item_count;
         }
@@ -651,9 +652,7 @@ void analyze_rcs(Symbolic& S) {
     }
   }
   // solve work items
-  S.solve_rows_ptr.assign(nlev + 1, 0);
   S.solve_cols_ptr.assign(nlev + 1, 0);
-  S.solve_rows.clear();
   S.solve_cols.clear();
   S.lvl_max_nc.assign(nlev, 0);
   for (int32_t l = 0; l < nlev; ++l) {
@@ -661,11 +660,9 @@ void analyze_rcs(Symbolic& S) {
       const int32_t x = S.lvl_sn[q];
       S.lvl_max_nc[l] = std::max(S.lvl_max_nc[l], S.sn_nc[x]);
       const int32_t m = 6 * (S.sn_nf[x] - S.sn_nc[x]), K = 6 * S.sn_nc[x];
-      for (int32_t t = 0; t * kSolveRows < m; ++t) S.solve_rows.insert(S.solve_rows.end(), {x, t});
       if (m > 0)
         for (int32_t t = 0; t * kSolveCols < K; ++t) S.solve_cols.insert(S.solve_cols.end(), {x, t});
     }
-    S.solve_rows_ptr[l + 1] = int32_t(S.solve_rows.size() / 2);
     S.solve_cols_ptr[l + 1] = int32_t(S.solve_cols.size() / 2);
   }
   // extend-add groups (children ranked by node index within their parent)

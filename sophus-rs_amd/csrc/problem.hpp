@@ -108,9 +108,8 @@ struct Symbolic {
   int64_t w_elems = 0;
   std::vector<int64_t> sn_poff;       // per node: its panels' L_PP^-T (kPW x kPW each)
   int64_t p_elems = 0;
-  // solve work items per level: (node, kSolveRows-row tile below K), (node,
-  // kSolveCols-column tile of K)
-  std::vector<int32_t> solve_rows_ptr, solve_rows, solve_cols_ptr, solve_cols;
+  // backward-solve work items per level: (node, kSolveCols-column tile of K)
+  std::vector<int32_t> solve_cols_ptr, solve_cols;
   std::vector<int32_t> lvl_max_nc;  // largest node (column blocks) per level
   // extend-add groups: per level, nodes grouped by their rank among their parent's
   // children, so one launch never has two children of one parent (plain adds, fixed
@@ -142,10 +141,9 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
 // multifrontal factorization.
 constexpr int32_t kPanelBlocks = 8;
 constexpr int32_t kTile = 64;
-// forward-solve row tile and backward-solve column tile of the rows below a node
-constexpr int32_t kSolveRows = 64;
-// rows per panel-row workgroup (4 lanes per row)
+// rows per panel-row workgroup
 constexpr int32_t kPanelRows = 64;
+// backward-solve column tile of the rows below a node
 constexpr int32_t kSolveCols = 16;
 
 }  // namespace sgpu

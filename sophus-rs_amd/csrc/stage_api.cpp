@@ -198,9 +198,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.sn_woff = bufs.up(S.sn_woff);
     df.sn_toff = bufs.up(S.sn_toff);
     df.sn_poff = bufs.up(S.sn_poff);
-    df.solve_rows = bufs.up(S.solve_rows);
     df.solve_cols = bufs.up(S.solve_cols);
-    df.ytmp = bufs.alloc<double>(size_t(6) * nb);
     df.ext_nodes = bufs.up(S.ext_nodes);
     df.sn_uoff = bufs.up(S.sn_uoff);
     df.usolve = bufs.alloc<double>(size_t(S.u_elems));
@@ -223,8 +221,8 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     launch_zero_scalars(nullptr, d_sc);
     launch_fill(nullptr, df.front, df.front_elems, 0.0);
     launch_scatter_s(nullptr, dp, int64_t(S.s_col.size()), d_blocks, df);
-    launch_rcs_factor(nullptr, df, d_sc);
-    launch_rcs_solve(nullptr, df, d_rhs, d_y, d_x);
+    launch_rcs_factor(nullptr, df, d_sc, d_rhs, d_y);
+    launch_rcs_solve(nullptr, df, d_y, d_x);
     SGPU_HIP(hipGetLastError());
     SGPU_HIP(hipDeviceSynchronize());
     DevScalars sc;
