@@ -424,13 +424,22 @@ __global__ void __launch_bounds__(kThreads)
       for (int k = 0; k < 9; ++k) vs[k] = U[k];
 #pragma unroll
       for (int e = 0; e < 3; ++e) vs[9 + e] = xl[e];
-      // own term: M - M H_ll^-1 M = M - U^T U;  t = M y - v
+      // own term: M - M H_ll^-1 M = M H_ll^-1 (H_ll - M), without the cancellation (M
+      // alone for a constant point);  t = M y - v
       double N[9];
+      if (p.pt_block[b] >= 0) {
+        const double* Hu = hll_g + 6 * b;
+        const double Hf[9] = {Hu[0], Hu[1], Hu[2], Hu[1], Hu[3], Hu[4], Hu[2], Hu[4], Hu[5]};
+        const double Hv[9] = {rc[3], rc[4], rc[5], rc[4], rc[6], rc[7], rc[5], rc[7], rc[8]};
+        double R[9], Z[9];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+        for (int e = 0; e < 9; ++e) R[e] = Hf[e] - Md[e];
+        mat3_mul_t(Md, Hv, Z);
+        mat3_mul_t(Z, R, N);
+      } else {
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-          N[3 * i + j] = Md[3 * i + j] - (U[i] * U[j] + U[3 + i] * U[3 + j] + U[6 + i] * U[6 + j]);
+        for (int e = 0; e < 9; ++e) N[e] = Md[e];
+      }
 #pragma unroll
       for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * rc[9] + Md[3 * r + 1] * rc[10] + Md[3 * r + 2] * rc[11] - v[r];
       add_lift(xl, xl, N, 1.0, [&](int e, double val) { own[e] = val; });
