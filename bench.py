@@ -6,10 +6,11 @@ Huber 1 px, camera 0 fixed, nu0 = 1), inputs resident in HBM before the timed re
 - value: observations per second of the plain FP64 GPU path (Schur + direct block LDLt),
   N_o / (seconds per LM iteration) over exactly --steps timed iterations after --warmup
   (tolerances disabled so every iteration does the full work).
-- roofline: the dominant kernel (k_mf_update, the trailing updates of the RCS
-  factorization: a rank-48 update of HBM-resident fronts, 6 flop/byte, so HBM-bound),
-  algorithmic bytes per launch from the symbolic factor / its average launch time from
-  HIP events on the optimizer's stream, against HBM peak.
+- roofline: the dominant kernel, one panel step of the RCS factorization (k_mf_step; the
+  steps with >= 640 tiles run as k_mf_panel + k_mf_update): the rank-48 trailing update
+  of HBM-resident fronts, 6 flop/byte, so HBM-bound; algorithmic bytes per step from the
+  symbolic factor / the step's average time from HIP events around its launches on the
+  optimizer's stream, against HBM peak.
 - time_to_tol: F* = the FP64 direct path's final cost (function tolerance 1e-10, at most
   100 iterations); each path's first iteration within 1e-6 of F*, its time and
   obs/s = N_o x iterations / time: FP64 direct, block-Jacobi PCG, learned PCG, mixed.
@@ -190,27 +191,27 @@ def main() -> int:
         opt.step()
     kt2 = opt.kernel_times()
     opt.close()
-    n_launch, ms_launch = kt2["mf_update"]
+    n_launch, ms_launch = kt2["mf_step"]
     avg_s = ms_launch / n_launch / 1e3
-    bytes_launch = st["mf_update_bytes"] / st["mf_update_launches"]
-    flops_launch = st["mf_update_flops"] / st["mf_update_launches"]
+    bytes_launch = st["mf_step_bytes"] / st["mf_step_launches"]
+    flops_launch = st["mf_step_flops"] / st["mf_step_launches"]
     achieved = bytes_launch / avg_s / 1e9
-    # HBM bytes per k_mf_update launch from the committed rocprofv3 --pmc summary of this
-    # workload (FETCH_SIZE and WRITE_SIZE runs, scaled by a 256 MiB copy's readings)
+    # HBM bytes per panel step from the committed rocprofv3 --pmc summary of this workload
+    # (FETCH_SIZE and WRITE_SIZE runs, scaled by a 256 MiB copy's readings)
     traffic = None
     tfile = ROOT / "profiles" / "pmc_traffic_ba_walk_2000cams.json"
     if args.config == 2 and tfile.exists():
-        kern = json.loads(tfile.read_text())["kernels"].get("k_mf_update")
-        traffic = kern["hbm_bytes_per_launch"] if kern else None
+        kern = json.loads(tfile.read_text()).get("panel_step")
+        traffic = kern["hbm_bytes_per_step"] if kern else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "k_mf_update",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "panel step (k_mf_step; k_mf_panel + k_mf_update when split)",
             "bytes_per_launch": bytes_launch, "flops_per_launch": flops_launch,
             "tflops": flops_launch / avg_s / 1e12, "avg_launch_us": avg_s * 1e6,
-            "launches_per_factorization": st["mf_update_launches"]}
+            "launches_per_factorization": st["mf_step_launches"]}
     log(f"[bench] roofline {roof}")
     kernels = {k: {"launches": c, "avg_ms": (ms / c if c else None)} for k, (c, ms) in ktimes.items()
                if c}
-    kernels["mf_update"] = {"launches": n_launch, "avg_ms": ms_launch / n_launch}
+    kernels["mf_step"] = {"launches": n_launch, "avg_ms": ms_launch / n_launch}
 
     # ---- time to tolerance per path -----------------------------------------------------------
     tol = None

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 3
+#define SGPU_ABI_VERSION 4
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -261,10 +261,11 @@ typedef struct sgpu_rcs_stats {
   int64_t num_supernodes;    /* supernodes (dense fronts) of the factorization */
   int64_t max_front_blocks;  /* largest front, in 6x6 blocks */
   int64_t front_bytes;       /* device bytes of all fronts */
-  double mf_update_flops;      /* algorithmic flops of the trailing-update launches (k_mf_update) */
-  double mf_update_bytes;      /* their algorithmic HBM bytes: read + write of the lower trailing
-                                  fronts and one read of the panel's L and W rows */
-  int64_t mf_update_launches;  /* trailing-update launches per factorization */
+  double mf_step_flops;      /* algorithmic flops of the panel-step launches' trailing updates
+                                (k_mf_step) */
+  double mf_step_bytes;      /* their algorithmic HBM bytes: read + write of the lower trailing
+                                fronts, one read of the panel columns below, L written */
+  int64_t mf_step_launches;  /* panel-step launches per factorization */
 } sgpu_rcs_stats;
 
 sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* stats);
@@ -284,7 +285,7 @@ void sgpu_optimizer_reset_kernel_times(sgpu_optimizer* opt);
 /* Event timing on the handle's stream: 1 = one event pair around each phase kernel
  * (point_prep, camera_rows, rcs_factor, rcs_solve, backsub, pose_update, cost);
  * 2 = also one pair around every panel-step launch of the direct factorization
- * ("mf_update"); 0 disables. */
+ * ("mf_step"); 0 disables. */
 void sgpu_optimizer_set_kernel_timing(sgpu_optimizer* opt, int32_t enable);
 
 void sgpu_optimizer_destroy(sgpu_optimizer* opt);

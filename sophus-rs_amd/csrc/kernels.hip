@@ -735,11 +735,12 @@ __device__ __forceinline__ void diag_trows(int slot, int stride, double (*D)[kPW
 // The per-node values a diagonal factorization needs, read at kernel entry (ahead of the
 // barriers, which would otherwise keep these loads from being hoisted).
 struct DiagNode {
-  int c0, K;                  // first column block, node columns
-  int64_t loff, toff, poff;   // L panel, row-major L_KK copy, L_PP^-T slots
+  int c0, K;                        // first column block, node columns
+  int64_t loff, toff, poff, uoff;   // L panel, row-major L_KK copy, L_PP^-T slots, RHS
 };
 __device__ __forceinline__ DiagNode diag_node(const DevFactor& f, int X) {
-  return DiagNode{f.sn_c0[X], 6 * f.sn_nc[X], f.sn_loff[X], f.sn_toff[X], f.sn_poff[X]};
+  return DiagNode{f.sn_c0[X],   6 * f.sn_nc[X], f.sn_loff[X],
+                  f.sn_toff[X], f.sn_poff[X],   f.sn_uoff[X]};
 }
 
 // Factor the np x np diagonal block of the panel at column k0 of one front in LDS (block
@@ -765,7 +766,7 @@ __device__ __forceinline__ DiagNode diag_node(const DevFactor& f, int X) {
 __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double* A, int F, int k0,
                                   int np, DevScalars* sc, double (*D)[kPW + 1],
                                   double (*Wd)[kPW + 1], double (*sL)[42], double (*sdi)[6],
-                                  bool loaded) {
+                                  bool loaded, double* __restrict__ z) {
   const int tid = threadIdx.x;
   if (!loaded) {
     stage_lds<(kPW * kPW + kThreads - 1) / kThreads>(
@@ -928,20 +929,283 @@ __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double
     const int cp = e / 6, a = e % 6;
     Tg[cp * kPW + pl + a] = D[cp][pl + a];
   }
+  if (k0 + np == F) {
+    // no rows below the panel (the last panel of a root), so no panel step follows: its
+    // right-hand side is final here; z_P = D^-1 L_PP^-1 b_P for the panel's columns
+    const double* bx = f.usolve + dn.uoff + k0;
+    double* yv = Wd[0];  // the block's W copy is done with
+    if (tid < np) {
+      double y = bx[tid];
+      for (int r = 0; r < 6 * (tid / 6); ++r) y += D[r][tid] * bx[r];  // T[c][r] = D[r][c]
+      yv[tid] = y;
+    }
+    __syncthreads();
+    if (tid < np / 6) {
+      const int j = tid;
+      double v[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) {
+        double t = yv[6 * j + i];
+        for (int k = 0; k < i; ++k) t -= sL[j][6 * i + k] * v[k];
+        v[i] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) v[i] *= sdi[j][i];
+#pragma unroll
+      for (int i = 5; i >= 0; --i) {
+        double t = v[i];
+        for (int k = i + 1; k < 6; ++k) t -= sL[j][6 * k + i] * v[k];
+        v[i] = t;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) z[6 * int64_t(dn.c0) + k0 + 6 * j + i] = v[i];
+    }
+  }
 }
 
 // First panel diagonal of every node of a level (after the level's fronts are assembled).
 __global__ void __launch_bounds__(kThreads)
-    k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc) {
+    k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc, double* __restrict__ z) {
   const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   factor_panel_diag(f, diag_node(f, X), f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL,
-                    sdi, false);
+                    sdi, false, z);
 }
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
+// One panel step for every node of a level, one workgroup per 64x64 tile (tr, tc) of the
+// lower trailing region below the panel:
+//   1. the tile's values, the panel columns of its row block tr and column block tc (C),
+//      L_PP^-T and the pivot inverses M_j are loaded (every global load first);
+//   2. W = C L_PP^-T for both blocks and L = W blockdiag(M_j) for block tr on the matrix
+//      cores (v_mfma_f64_16x16x4; each wavefront its own 16 rows of a 64-row block, so
+//      W and L need no barrier between them);
+//   3. A -= L W^T on the matrix cores: four wavefronts x 2 x 2 tiles of 16x16, (-W) L^T so
+//      that each lane's results run down 16 rows of the column-major front, accumulators
+//      seeded with the front's values;
+//   4. diagonal tiles (tr == tc) store their L rows (and the row-major copy of the node's
+//      own rows) and eliminate their rows of the node's right-hand side, b_r -= L_r y_P
+//      with y_P = L_PP^-1 b_P; tile (0, 0) also writes z_P = D^-1 y_P for the panel's
+//      columns, then factors the next panel's diagonal block (the top-left of its tile,
+//      handed over in LDS).
+// Each tile forms the L and W rows it needs itself (a row block is formed again by every
+// tile of its tile row and column), so a panel step is one launch with no panel-row pass
+// before it.
+__global__ void __launch_bounds__(kThreads)
+    k_mf_step(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc,
+              double* __restrict__ z) {
+  static_assert(kTile == 64 && kThreads == 256 && kPW == 48, "4 wavefronts, 16-row slices");
+  // consecutive tiles (one tile row shares its L rows) run on one XCD
+  const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int X = tiles[3 * bid], tr = tiles[3 * bid + 1], tc = tiles[3 * bid + 2];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
+  double* A = f.front + f.sn_off[X];
+  const DiagNode dn = diag_node(f, X);
+  const double* Tg = f.tinv + dn.poff + int64_t(panel) * kPW * kPW;
+  double* bx = f.usolve + dn.uoff;  // the node's right-hand side (front row order)
+  const int k0 = kPW * panel;
+  const int kend = min(K, k0 + kPW);
+  const int np = kend - k0;
+  const int r0 = kend + kTile * tr, c0 = kend + kTile * tc;
+  const bool dg = tr == tc;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lc = lane >> 4;
+  const int rb = 32 * (wv >> 1), cb = 32 * (wv & 1);  // the wavefront's quadrant
+  const int rw = 16 * wv;                              // the wavefront's 16-row slice
+  __shared__ double sA[kPW][kTile + 1], sB[kPW][kTile + 1];  // C^T, then L^T / W^T
+  __shared__ double sT[kPW][kPW + 1];                          // L_PP^-T
+  __shared__ double pM[kPanelBlocks][36];
+  __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
+  __shared__ double sb[kPW], sy[kThreads / 64][kPW];
+  // accumulator (mi, nj) component q is the tile element
+  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+  // it starts from the element's current value
+  mfma_d4 acc[2][2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+      }
+  {
+    const int gb0 = dn.c0 + k0 / 6, nbp = np / 6;
+    constexpr int kNC = kPW * kTile / kThreads, kNT = kPW * kPW / kThreads;
+    constexpr int kNM = (kPanelBlocks * 36 + kThreads - 1) / kThreads;
+    static_assert(kPW * kTile % kThreads == 0 && kPW * kPW % kThreads == 0, "whole rounds");
+    double va[kNC], vb[kNC], vt[kNT], vm[kNM];
+#pragma unroll
+    for (int it = 0; it < kNC; ++it) {
+      const int e = tid + kThreads * it, t = e / kTile, i = e % kTile;
+      va[it] = load_or_zero(A, int64_t(k0 + t) * F + r0 + i, t < np && r0 + i < F);
+      vb[it] = load_or_zero(A, int64_t(k0 + t) * F + c0 + i, !dg && t < np && c0 + i < F);
+    }
+#pragma unroll
+    for (int it = 0; it < kNT; ++it) vt[it] = Tg[tid + kThreads * it];
+#pragma unroll
+    for (int it = 0; it < kNM; ++it) {
+      const int e = tid + kThreads * it;
+      vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
+    }
+    const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
+#pragma unroll
+    for (int it = 0; it < kNC; ++it) {
+      const int e = tid + kThreads * it;
+      sA[e / kTile][e % kTile] = va[it];
+      sB[e / kTile][e % kTile] = vb[it];
+    }
+#pragma unroll
+    for (int it = 0; it < kNT; ++it) {
+      const int e = tid + kThreads * it;
+      sT[e / kPW][e % kPW] = vt[it];
+    }
+#pragma unroll
+    for (int it = 0; it < kNM; ++it) {
+      const int e = tid + kThreads * it;
+      if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
+    }
+    if (tid < kPW) sb[tid] = vbb;
+  }
+  __syncthreads();
+  // W = C L_PP^-T for the wavefront's slice of both row blocks; L_PP^-T is block upper
+  // triangular, so column tile ct stops at row kW[ct] (C and L_PP^-T are zero past np)
+  constexpr int kW[3] = {18, 36, 48};
+  mfma_d4 wa[3], wb[3];
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct) wa[ct] = wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < kPW; k4 += 4) {
+    const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+      if (k4 < kW[ct]) {
+        const double tv = sT[k4 + lc][16 * ct + lr];
+        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
+        wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
+      }
+  }
+  __syncthreads();  // every wavefront is done with C
+  // W^T of both blocks (a diagonal tile's column block is its row block)
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
+      sB[16 * ct + lr][rw + lc + 4 * q] = dg ? wa[ct][q] : wb[ct][q];
+    }
+  // L = W blockdiag(M_j) for the slice of row block tr: column tile ct meets the pivot
+  // blocks with rows [kL0[ct], kL1[ct]); the block-diagonal operand is read from the M_j
+  constexpr int kL0[3] = {0, 12, 30}, kL1[3] = {18, 36, 48};
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct) wa[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < kPW; k4 += 4) {
+    const int k = k4 + lc;
+    const double av = sA[k][rw + lr];
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+      if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
+        const int n = 16 * ct + lr;
+        const double mv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
+        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wa[ct], 0, 0, 0);
+      }
+  }
+  // L^T replaces the slice's W^T in sA
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
+  __syncthreads();  // the update reads every row of L^T and W^T
+#pragma unroll
+  for (int s = 0; s < kPW / 4; ++s) {
+    const int t = 4 * s + lc;  // this lane's k index
+    double a[2], b[2];
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
+  }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
+      }
+  if (dg) {
+    // the slice's L rows to the factor (and the row-major copy of the node's own rows)
+    double* Lo = f.lbuf + dn.loff;
+    double* Lt = f.ltbuf + dn.toff;
+    for (int e = lane; e < 16 * kPW; e += 64) {
+      const int i = rw + e % 16, c = e / 16, r = r0 + i;
+      if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = sA[c][i];
+    }
+    if (r0 + rw < K)
+      for (int e = lane; e < 16 * kPW; e += 64) {
+        const int c = e % kPW, i = rw + e / kPW, r = r0 + i;
+        if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sA[c][i];
+      }
+    // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront its
+    // own copy, lane c < kPW entry c), then b_r -= L_r y_P for the slice's rows (four
+    // lanes per row, every fourth column, two xor shuffles)
+    if (lane < kPW) {
+      double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
+      sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+    }
+    {
+      const int i = rw + (lane >> 2), part = lane & 3, r = r0 + i;
+      double s4 = 0.0;
+#pragma unroll
+      for (int u = 0; u < kPW / 4; ++u) s4 += sA[part + 4 * u][i] * sy[wv][part + 4 * u];
+      s4 += __shfl_xor(s4, 1, 64);
+      s4 += __shfl_xor(s4, 2, 64);
+      if (part == 0 && r < F) bx[r] -= s4;
+    }
+    if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
+      const int jb = lane / 6, a = lane % 6;
+      double zz = 0.0;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[0][6 * jb + t];
+      z[6 * int64_t(dn.c0) + k0 + lane] = zz;
+    }
+  }
+  if (tr == 0 && tc == 0 && kend < K) {
+    // the next panel's diagonal block is the top-left of this tile: hand the updated
+    // values to LDS directly
+    const int npn = min(K, kend + kPW) - kend;
+    double (*D)[kPW + 1] = as_square(sA);
+    __syncthreads();  // every read of sA / sB is done
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
+          if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+        }
+    __syncthreads();
+    factor_panel_diag(f, dn, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true, z);
+  }
+}
+
+// The split form of a panel step, for launches with many tiles (where forming a row
+// block's L and W once, instead of in every tile of its tile row and column, saves more
+// matrix-core time than the extra launch costs).
+//
 // Panel rows of one panel step for the level's nodes (items = (node, kPanelRows-row
 // chunk) below the panel's diagonal block), as two products instead of a substitution:
 //   W_r = C_r L_PP^-T            (the rows' values after the panel's own eliminations)
@@ -1097,7 +1361,8 @@ __global__ void __launch_bounds__(kThreads)
 // accumulators start from the tile's current values.  The tile (0, 0), which holds the
 // next panel's diagonal block, then factors that block.
 __global__ void __launch_bounds__(kThreads)
-    k_mf_update(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc) {
+    k_mf_update(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc,
+                double* __restrict__ z) {
   static_assert(kTile == 64 && kThreads == 256 && kPW % 4 == 0, "4 wavefronts of 32x32");
   // consecutive tiles (one tile row shares its L rows) run on one XCD
   const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -1182,7 +1447,7 @@ __global__ void __launch_bounds__(kThreads)
           if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
         }
     __syncthreads();
-    factor_panel_diag(f, dn, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true);
+    factor_panel_diag(f, dn, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true, z);
   }
 }
 
@@ -1860,18 +2125,22 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const 
       if (level >= 0) launch_extend_groups(s, f, level);
       level = L.level;
       const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
-      hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level], sc);
+      hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level],
+                         sc, y);
     }
     const size_t li = size_t(&L - S.launches.data());
-    if (L.item_count > 0)
+    if (L.tile_count == 0) continue;
+    if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
+    if (L.fused) {
+      hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
+                         f.launch_tiles + 3 * int64_t(L.tile_begin), sc, y);
+    } else {
       hipLaunchKernelGGL(k_mf_panel, dim3(L.item_count), dim3(kThreads), 0, s, f, L.panel,
                          f.panel_items + 2 * int64_t(L.item_begin), y);
-    if (L.tile_count > 0) {
-      if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
       hipLaunchKernelGGL(k_mf_update, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
-                         f.launch_tiles + 3 * int64_t(L.tile_begin), sc);
-      if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li + 1], s));
+                         f.launch_tiles + 3 * int64_t(L.tile_begin), sc, y);
     }
+    if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li + 1], s));
   }
 }
 

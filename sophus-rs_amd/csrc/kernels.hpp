@@ -75,7 +75,7 @@ struct DevFactor {
   const int32_t* ext_nodes;   // extend-add groups (see Symbolic)
   const int64_t* sn_uoff;     // right-hand side offsets (F entries per node)
   double* usolve;             // each node's right-hand side in front row order (u_elems)
-  hipEvent_t* step_events;    // optional: 2 per panel launch, recorded around k_mf_update
+  hipEvent_t* step_events;    // optional: 2 per panel step, recorded around its launches
   const int32_t* panel_items;  // (node, kPanelRows-row chunk) per panel-row workgroup
   double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
   const int64_t* sn_woff;

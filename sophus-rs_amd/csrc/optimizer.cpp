@@ -44,11 +44,11 @@ enum KernelId {
   K_BACKSUB,
   K_UPDATE,
   K_COST,
-  K_MF_UPDATE,  // k_mf_update launches, per-launch events (timing mode 2 only)
+  K_MF_STEP,  // k_mf_step launches, per-launch events (timing mode 2 only)
   K_COUNT
 };
 const char* kKernelNames[K_COUNT] = {"point_prep",  "camera_rows", "rcs_factor", "rcs_solve",
-                                     "backsub",     "pose_update", "cost",       "mf_update"};
+                                     "backsub",     "pose_update", "cost",       "mf_step"};
 
 double bits_to_double(unsigned long long b) {
   double d;
@@ -87,7 +87,7 @@ struct sgpu_optimizer {
   int64_t rejects = 0, steps = 0;
   bool terminated = false;
   int32_t last_termination = SGPU_TERM_NONE;
-  // timing: 0 off, 1 per phase, 2 also per k_mf_update launch
+  // timing: 0 off, 1 per phase, 2 also per k_mf_step launch
   int timing = 0;
   std::vector<hipEvent_t> step_ev;
   hipEvent_t ev[2 * K_COUNT] = {};
@@ -122,12 +122,12 @@ struct sgpu_optimizer {
         if (S.launches[li].tile_count == 0) continue;
         float ms = 0;
         SGPU_HIP(hipEventElapsedTime(&ms, step_ev[2 * li], step_ev[2 * li + 1]));
-        kernel_ms[K_MF_UPDATE] += ms;
-        kernel_count[K_MF_UPDATE] += 1;
+        kernel_ms[K_MF_STEP] += ms;
+        kernel_count[K_MF_STEP] += 1;
       }
     }
     for (int k = 0; k < K_COUNT; ++k) {
-      if (!used[k] || k == K_MF_UPDATE) continue;
+      if (!used[k] || k == K_MF_STEP) continue;
       float ms = 0;
       SGPU_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
       kernel_ms[k] += ms;
@@ -285,10 +285,10 @@ struct sgpu_optimizer {
     df.sn_upos = arena.upload(S.sn_upos, stream);
     df.lvl_sn = arena.upload(S.lvl_sn, stream);
     df.launch_tiles = arena.upload(S.launch_tiles, stream);
-    df.panel_items = arena.upload(S.panel_items, stream);
-    df.sn_woff = arena.upload(S.sn_woff, stream);
     df.sn_toff = arena.upload(S.sn_toff, stream);
     df.sn_poff = arena.upload(S.sn_poff, stream);
+    df.panel_items = arena.upload(S.panel_items, stream);
+    df.sn_woff = arena.upload(S.sn_woff, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
     df.ext_nodes = arena.upload(S.ext_nodes, stream);
     df.sn_uoff = arena.upload(S.sn_uoff, stream);
@@ -296,9 +296,9 @@ struct sgpu_optimizer {
     if (!iterative()) {
       df.front = arena.alloc<double>(size_t(S.front_elems));
       df.lbuf = arena.alloc<double>(size_t(S.l_elems));
-      df.wbuf = arena.alloc<double>(size_t(S.w_elems));
       df.ltbuf = arena.alloc<double>(size_t(S.t_elems));
       df.tinv = arena.alloc<double>(size_t(S.p_elems));
+      df.wbuf = arena.alloc<double>(size_t(S.w_elems));
     }
     df.piv_L = arena.alloc<double>(size_t(36) * S.nb);
     df.piv_d = arena.alloc<double>(size_t(6) * S.nb);
@@ -629,13 +629,13 @@ static void fill_stats(const HostProblem& P, const Symbolic& S, sgpu_rcs_stats* 
   st->num_supernodes = S.nsn;
   st->max_front_blocks = S.max_front;
   st->front_bytes = S.front_elems * 8;
-  st->mf_update_flops = 0.0;
-  st->mf_update_bytes = 0.0;
-  st->mf_update_launches = 0;
+  st->mf_step_flops = 0.0;
+  st->mf_step_bytes = 0.0;
+  st->mf_step_launches = 0;
   for (const auto& L : S.launches) {
-    st->mf_update_flops += L.flops;
-    st->mf_update_bytes += L.bytes;
-    st->mf_update_launches += L.tile_count > 0 ? 1 : 0;
+    st->mf_step_flops += L.flops;
+    st->mf_step_bytes += L.bytes;
+    st->mf_step_launches += L.tile_count > 0 ? 1 : 0;
   }
 }
 

@@ -611,7 +611,7 @@ void analyze_rcs(Symbolic& S) {
       maxpan = std::max(maxpan, (S.sn_nc[S.lvl_sn[q]] + kPanelBlocks - 1) / kPanelBlocks);
     for (int32_t pn = 0; pn < maxpan; ++pn) {
       Symbolic::PanelLaunch L{l,   pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0,
-                              int32_t(S.panel_items.size() / 2), 0, 0.0, 0.0, 0.0};
+                              int32_t(S.panel_items.size() / 2), 0, true, 0.0, 0.0};
       // every node's tile (0, 0) (which factors the next diagonal block) first, so the
       // critical path is dispatched first; then the other tiles, node-major, row-major
       for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
@@ -631,22 +631,23 @@ void analyze_rcs(Symbolic& S) {
         const int32_t T = 6 * S.sn_nf[x] - kend;
         const int32_t nt = (T + kTile - 1) / kTile;
         const double np = double(kend - 6 * std::min(S.sn_nc[x], pn * kPanelBlocks));
-        // lower trailing update T(T+1)/2 x np multiply-adds; the panel rows' triangular
-        // solve (np^2 / 2 multiply-adds per row) and D^-1 scaling (6 np per row)
+        // lower trailing update T(T+1)/2 x np multiply-adds; bytes: the lower trailing
+        // region read and written, the panel columns of the rows below read, their L written
         L.flops += double(T) * double(T + 1) * np;
         L.bytes += 16.0 * double(T) * double(T + 1) / 2.0 + 16.0 * double(T) * np;
-        L.panel_flops += double(T) * (np * np + 12.0 * np);
-        // at least one item per node and panel: it also eliminates the panel's right-hand side
-        for (int32_t ch = 0; ch == 0 || ch * kPanelRows < T; ++ch) {
-          S.panel_items.insert(S.panel_items.end(), {x, ch});
-          ++L.item_count;
-        }
+        for (int32_t ch = 0; ch * kPanelRows < T; ++ch) S.panel_items.insert(S.panel_items.end(), {x, ch});
         for (int32_t tr = 0; tr < nt; ++tr)
           for (int32_t tc = 0; tc <= tr; ++tc) {
             if (tr == 0 && tc == 0) continue;  // listed first
             S.launch_tiles.insert(S.launch_tiles.end(), {x, tr, tc});
             ++L.tile_count;
           }
+      }
+      L.item_count = int32_t(S.panel_items.size() / 2) - L.item_begin;
+      L.fused = L.tile_count < kSplitTiles;
+      if (L.fused) {  // the fused launch forms its panel rows itself
+        S.panel_items.resize(size_t(2) * L.item_begin);
+        L.item_count = 0;
       }
       S.launches.push_back(L);
     }

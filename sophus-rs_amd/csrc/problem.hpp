@@ -91,21 +91,22 @@ struct Symbolic {
   int64_t front_elems = 0, l_elems = 0;
   int32_t max_front = 0;                      // largest front (blocks)
   std::vector<int32_t> lvl_ptr, lvl_sn;
-  // schedule: nodes per level; per (level, panel) one panel-row launch over
-  // (node, 256-row chunk) items and one trailing-update launch over 64x64 tiles
+  // schedule: nodes per level; per (level, panel) the 64x64 tiles of every node's lower
+  // trailing region, either as one fused panel-step launch or (many tiles) as a panel-row
+  // launch over (node, kPanelRows-row chunk) items followed by a trailing-update launch
   struct PanelLaunch {
     int32_t level, panel, node_begin, node_count, tile_begin, tile_count, item_begin, item_count;
-    double flops;        // algorithmic FP64 flops of the trailing update
-    double bytes;        // its algorithmic HBM bytes (lower trailing RMW + L and W rows)
-    double panel_flops;  // algorithmic FP64 flops of the panel rows
+    bool fused;
+    double flops;  // algorithmic FP64 flops of the trailing update
+    double bytes;  // algorithmic HBM bytes: lower trailing RMW, panel columns read, L written
   };
   std::vector<PanelLaunch> launches;
   std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
-  std::vector<int32_t> panel_items;   // (node, kPanelRows-row chunk) pairs
+  std::vector<int32_t> panel_items;   // (node, kPanelRows-row chunk) pairs (split launches)
   std::vector<int64_t> sn_woff;       // W scratch offsets (6 nf x kPW per node)
+  int64_t w_elems = 0;
   std::vector<int64_t> sn_toff;       // row-major copies of each node's L_KK (K x K)
   int64_t t_elems = 0;
-  int64_t w_elems = 0;
   std::vector<int64_t> sn_poff;       // per node: its panels' L_PP^-T (kPW x kPW each)
   int64_t p_elems = 0;
   // backward-solve work items per level: (node, kSolveCols-column tile of K)
@@ -141,8 +142,10 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
 // multifrontal factorization.
 constexpr int32_t kPanelBlocks = 8;
 constexpr int32_t kTile = 64;
-// rows per panel-row workgroup
+// rows per panel-row workgroup; a panel step with at least kSplitTiles tiles is launched
+// split (panel rows, then the trailing update) instead of fused
 constexpr int32_t kPanelRows = 64;
+constexpr int32_t kSplitTiles = 640;
 // backward-solve column tile of the rows below a node
 constexpr int32_t kSolveCols = 16;
 

@@ -194,19 +194,19 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.sn_upos = bufs.up(S.sn_upos);
     df.lvl_sn = bufs.up(S.lvl_sn);
     df.launch_tiles = bufs.up(S.launch_tiles);
-    df.panel_items = bufs.up(S.panel_items);
-    df.sn_woff = bufs.up(S.sn_woff);
     df.sn_toff = bufs.up(S.sn_toff);
     df.sn_poff = bufs.up(S.sn_poff);
+    df.panel_items = bufs.up(S.panel_items);
+    df.sn_woff = bufs.up(S.sn_woff);
     df.solve_cols = bufs.up(S.solve_cols);
     df.ext_nodes = bufs.up(S.ext_nodes);
     df.sn_uoff = bufs.up(S.sn_uoff);
     df.usolve = bufs.alloc<double>(size_t(S.u_elems));
     df.front = bufs.alloc<double>(size_t(S.front_elems));
     df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
-    df.wbuf = bufs.alloc<double>(size_t(S.w_elems));
     df.ltbuf = bufs.alloc<double>(size_t(S.t_elems));
     df.tinv = bufs.alloc<double>(size_t(S.p_elems));
+    df.wbuf = bufs.alloc<double>(size_t(S.w_elems));
     df.piv_L = bufs.alloc<double>(size_t(36) * nb);
     df.piv_d = bufs.alloc<double>(size_t(6) * nb);
     df.piv_M = bufs.alloc<double>(size_t(36) * nb);

@@ -125,9 +125,9 @@ class _RcsStats(C.Structure):
         ("num_supernodes", C.c_int64),
         ("max_front_blocks", C.c_int64),
         ("front_bytes", C.c_int64),
-        ("mf_update_flops", C.c_double),
-        ("mf_update_bytes", C.c_double),
-        ("mf_update_launches", C.c_int64),
+        ("mf_step_flops", C.c_double),
+        ("mf_step_bytes", C.c_double),
+        ("mf_step_launches", C.c_int64),
     ]
 
 
@@ -225,7 +225,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 3  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 4  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -630,7 +630,7 @@ class Optimizer:
         return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
 
     def set_kernel_timing(self, mode: int) -> None:
-        """0 off, 1 per phase kernel, 2 also per trailing-update launch ("mf_update")."""
+        """0 off, 1 per phase kernel, 2 also per panel-step launch ("mf_step")."""
         library().sgpu_optimizer_set_kernel_timing(self._h, int(mode))
 
     def reset_kernel_times(self) -> None:
