@@ -1019,19 +1019,6 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double pM[kPanelBlocks][36];
   __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   __shared__ double sb[kPW], sy[kThreads / 64][kPW];
-  // accumulator (mi, nj) component q is the tile element
-  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
-  // it starts from the element's current value
-  mfma_d4 acc[2][2];
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
-      }
   {
     const int gb0 = dn.c0 + k0 / 6, nbp = np / 6;
     constexpr int kNC = kPW * kTile / kThreads, kNT = kPW * kPW / kThreads;
@@ -1071,6 +1058,19 @@ __global__ void __launch_bounds__(kThreads)
     if (tid < kPW) sb[tid] = vbb;
   }
   __syncthreads();
+  // accumulator (mi, nj) component q is the tile element
+  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+  // it starts from the element's current value, loaded while W and L are formed
+  mfma_d4 acc[2][2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+      }
   // W = C L_PP^-T for the wavefront's slice of both row blocks; L_PP^-T is block upper
   // triangular, so column tile ct stops at row kW[ct] (C and L_PP^-T are zero past np)
   constexpr int kW[3] = {18, 36, 48};
