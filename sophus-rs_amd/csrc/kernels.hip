@@ -2089,6 +2089,22 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
   }
 }
 
+// Zero the lower triangle (diagonal included) of every front: the assembly, extend-add and
+// factorization read and write only that part.  Workgroup (node, g) takes columns 4 g ..
+// 4 g + 3 of the node's front, one wavefront per column.
+__global__ void __launch_bounds__(kThreads) k_zero_fronts(DevFactor f) {
+  const int X = blockIdx.x, F = 6 * f.sn_nf[X];
+  const int c = 4 * blockIdx.y + (threadIdx.x >> 6);
+  if (c >= F) return;
+  double* col = f.front + f.sn_off[X] + int64_t(c) * F;
+  for (int r = c + (threadIdx.x & 63); r < F; r += 64) col[r] = 0.0;
+}
+
+void launch_zero_fronts(hipStream_t s, const DevFactor& f) {
+  const Symbolic& S = *f.host;
+  hipLaunchKernelGGL(k_zero_fronts, dim3(S.nsn, (6 * S.max_front + 3) / 4), dim3(kThreads), 0, s, f);
+}
+
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v) {
   if (n == 0) return;
   const int g = int(std::min<int64_t>((n + kThreads - 1) / kThreads, 4096));

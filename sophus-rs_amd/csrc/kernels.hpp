@@ -122,6 +122,8 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
                         const double* intr, const double* cw, double nu, DevLin& w,
                         DevFactor& f, DevScalars* sc);
 void launch_fill(hipStream_t s, double* ptr, int64_t n, double v);
+// Zero the lower triangle of every front (before the Schur assembly).
+void launch_zero_fronts(hipStream_t s, const DevFactor& f);
 // Full numeric factorization (all levels and panels) of the assembled fronts, with the
 // forward solve of rhs (camera order) folded in: y (nb*6) receives z = D^-1 L^-1 P rhs.
 void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const double* rhs_nat,

@@ -143,7 +143,7 @@ struct sgpu_optimizer {
   void linearize(const double* ps, const double* pts, double nu_lin, bool* used) {
     launch_zero_scalars(stream, d_sc);
     launch_cam_cache(stream, S.npose, ps, dl.cw);
-    if (!iterative()) launch_fill(stream, df.front, df.front_elems, 0.0);
+    if (!iterative()) launch_zero_fronts(stream, df);
     else if (!dpg.empty()) launch_fill(stream, dl.bsr, int64_t(36) * dp.nslot, 0.0);
     tic(K_LINEARIZE_POINTS);
     launch_point_prep(stream, params.precision, dp, ps, pts, intr, dl.cw, nu_lin, dl, d_sc);

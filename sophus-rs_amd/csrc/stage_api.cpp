@@ -219,7 +219,7 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     double* d_x = bufs.alloc<double>(size_t(6) * nb);
     DevScalars* d_sc = bufs.alloc<DevScalars>(1);
     launch_zero_scalars(nullptr, d_sc);
-    launch_fill(nullptr, df.front, df.front_elems, 0.0);
+    launch_zero_fronts(nullptr, df);
     launch_scatter_s(nullptr, dp, int64_t(S.s_col.size()), d_blocks, df);
     launch_rcs_factor(nullptr, df, d_sc, d_rhs, d_y);
     launch_rcs_solve(nullptr, df, d_y, d_x);
