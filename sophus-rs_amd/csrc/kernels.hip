@@ -1479,12 +1479,29 @@ __global__ void __launch_bounds__(kThreads)
     double* bp = f.usolve + f.sn_uoff[P];
     for (int r = threadIdx.x; r < m; r += blockDim.x) bp[6 * up[r / 6] + r % 6] += bx[r];
   }
+  // one child per parent per launch and distinct targets within a child: plain adds.
+  // Four rows per thread per pass, every load of the pass issued before the adds.
+  constexpr int kU = 4;
   for (int c = blockIdx.x; c < m; c += gridDim.x) {
     const int pc = 6 * up[c / 6] + c % 6;
-    for (int r = c + threadIdx.x; r < m; r += blockDim.x) {
-      const double v = U[int64_t(c) * F + r];
-      // one child per parent per launch and distinct targets within a child: plain add
-      if (v != 0.0) B[int64_t(pc) * FP + 6 * up[r / 6] + r % 6] += v;
+    double* Bc = B + int64_t(pc) * FP;
+    const double* Uc = U + int64_t(c) * F;
+    for (int r0 = c; r0 < m; r0 += kU * int(blockDim.x)) {
+      int64_t dst[kU];
+      double v[kU], b[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int r = min(r0 + int(threadIdx.x) + u * int(blockDim.x), m - 1);  // valid index
+        dst[u] = 6 * up[r / 6] + r % 6;
+        v[u] = Uc[r];
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u) b[u] = Bc[dst[u]];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int r = r0 + int(threadIdx.x) + u * int(blockDim.x);
+        if (r < m) Bc[dst[u]] = b[u] + v[u];
+      }
     }
   }
 }
@@ -2120,8 +2137,13 @@ static void launch_extend_groups(hipStream_t s, const DevFactor& f, int level) {
   const Symbolic& S = *f.host;
   for (int q = S.ext_lvl_ptr[level]; q < S.ext_lvl_ptr[level + 1]; ++q) {
     const int cnt = S.ext_grp_ptr[q + 1] - S.ext_grp_ptr[q];
+    int max_m = 1;  // one workgroup per update-matrix column
+    for (int e = S.ext_grp_ptr[q]; e < S.ext_grp_ptr[q + 1]; ++e) {
+      const int x = S.ext_nodes[e];
+      max_m = std::max(max_m, 6 * (S.sn_nf[x] - S.sn_nc[x]));
+    }
     if (cnt > 0)
-      hipLaunchKernelGGL(k_mf_extend, dim3(256, cnt), dim3(kThreads), 0, s, f,
+      hipLaunchKernelGGL(k_mf_extend, dim3(max_m, cnt), dim3(kThreads), 0, s, f,
                          f.ext_nodes + S.ext_grp_ptr[q]);
   }
 }
