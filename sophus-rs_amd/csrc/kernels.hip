@@ -195,10 +195,10 @@ __global__ void __launch_bounds__(kThreads)
     wave_atomic_max_abs(&sc->max_grad_bits, live && active ? double(g) : 0.0);
   }
   if (!live) return;
-  double* rec = prec + 12 * int64_t(b);
-  if (!active) {  // constant point: H_ll^-1 = 0 and y = 0 in its record
+  double* rec = prec + kPtRec * int64_t(b);
+  if (!active) {  // constant point: H_ll^-1, y and H_ll are 0 in its record
 #pragma unroll
-    for (int k = 0; k < 12; ++k) rec[k] = k < 3 ? x[k] : 0.0;
+    for (int k = 0; k < kPtRec; ++k) rec[k] = k < 3 ? x[k] : 0.0;
     return;
   }
   const T damp = T(nu * double(p.pt_nu[b]));
@@ -230,10 +230,10 @@ __global__ void __launch_bounds__(kThreads)
   double yv[3];
   for (int k = 0; k < 3; ++k)
     yv[k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) + inv[3 * k + 2] * double(s[2]);
-  const double r12[12] = {x[0],   x[1],   x[2],   inv[0], inv[1], inv[2],
-                          inv[4], inv[5], inv[8], yv[0],  yv[1],  yv[2]};
+  const double rv[kPtRec] = {x[0],   x[1],   x[2],   inv[0], inv[1], inv[2], inv[4], inv[5], inv[8],
+                             yv[0],  yv[1],  yv[2],  m11,    m12,    m13,    m22,    m23,    m33};
 #pragma unroll
-  for (int k = 0; k < 12; ++k) rec[k] = r12[k];
+  for (int k = 0; k < kPtRec; ++k) rec[k] = rv[k];
   hll_out[6 * b + 0] = m11;
   hll_out[6 * b + 1] = m12;
   hll_out[6 * b + 2] = m13;
@@ -391,10 +391,11 @@ __global__ void __launch_bounds__(kThreads)
     T* vs = sv + kObsRec * tid;
     double xd[3], xl[3], xc[3], Ad[6], Md[9], v[3], t[3];
     if constexpr (std::is_same<T, double>::value) {
-      // packed point record: x, H_ll^-1 upper, y (H_ll^-1 = y = 0 for a constant point)
-      double rc[12];
+      // packed point record: x, H_ll^-1 upper, y, H_ll upper (zeros but x for a constant
+      // point)
+      double rc[kPtRec];
 #pragma unroll
-      for (int k = 0; k < 12; ++k) rc[k] = prec[12 * int64_t(b) + k];
+      for (int k = 0; k < kPtRec; ++k) rc[k] = prec[kPtRec * int64_t(b) + k];
 #pragma unroll
       for (int e = 0; e < 3; ++e) xd[e] = xl[e] = rc[e];
       if (centered) {
@@ -428,7 +429,7 @@ __global__ void __launch_bounds__(kThreads)
       // alone for a constant point);  t = M y - v
       double N[9];
       if (p.pt_block[b] >= 0) {
-        const double* Hu = hll_g + 6 * b;
+        const double* Hu = rc + 12;
         const double Hf[9] = {Hu[0], Hu[1], Hu[2], Hu[1], Hu[3], Hu[4], Hu[2], Hu[4], Hu[5]};
         const double Hv[9] = {rc[3], rc[4], rc[5], rc[4], rc[6], rc[7], rc[5], rc[7], rc[8]};
         double R[9], Z[9];

@@ -309,7 +309,7 @@ struct sgpu_optimizer {
     dl.cw = arena.alloc<double>(size_t(12) * S.npose);
     dl.hinv = arena.alloc<double>(size_t(9) * S.npt);
     dl.hll = arena.alloc<double>(size_t(6) * S.npt);
-    dl.prec = arena.alloc<double>(size_t(12) * S.npt);
+    dl.prec = arena.alloc<double>(size_t(kPtRec) * S.npt);
     dl.pt_s = arena.alloc<double>(size_t(3) * S.npt);
     dl.pt_y = arena.alloc<double>(size_t(3) * S.npt);
     dl.rhs_nat = arena.alloc<double>(size_t(6) * S.nb);
