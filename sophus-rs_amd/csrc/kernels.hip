@@ -964,10 +964,11 @@ __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double
   }
 }
 
-// First panel diagonal of every node of a level (after the level's fronts are assembled).
+// First panel diagonal of each item's node (items as in k_mf_step, all first-diagonal
+// items: the leaves, whose fronts are assembled before the factorization starts).
 __global__ void __launch_bounds__(kThreads)
-    k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc, double* __restrict__ z) {
-  const int X = nodes[blockIdx.x];
+    k_mf_diag0(DevFactor f, const int32_t* __restrict__ items, DevScalars* sc, double* __restrict__ z) {
+  const int X = items[4 * blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   factor_panel_diag(f, diag_node(f, X), f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL,
@@ -976,8 +977,59 @@ __global__ void __launch_bounds__(kThreads)
 
 typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 
-// One panel step for every node of a level, one workgroup per 64x64 tile (tr, tc) of the
-// lower trailing region below the panel:
+// Position in the parent's front of row u of node X's update matrix (front row K + u):
+// update row block b is the parent's row block upos[b].
+__device__ __forceinline__ int parent_row(const DevFactor& f, int X, int u) {
+  return 6 * f.sn_upos[f.sn_upos_ptr[X] + u / 6] + u % 6;
+}
+
+// Extend-add of one lane's results of a trailing-update tile on node X's last panel step
+// (the tile lies in its update matrix, rows and columns >= K): accumulator (mi, nj)
+// component q is the front element (rl + 16 mi, cl + 16 nj + 4 q).  Each is added into
+// the parent P's front at the mapped row and column (update rows keep their order in the
+// parent, so the lower triangle maps to the lower triangle); every load first.
+__device__ __forceinline__ void extend_tile(const DevFactor& f, int X, int P, int K, int F,
+                                            int rl, int cl, const mfma_d4 (&acc)[2][2]) {
+  double* B = f.front + f.sn_off[P];
+  const int64_t FP = 6 * int64_t(f.sn_nf[P]);
+  int pr[2];
+  int64_t pc[2][4];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi) {
+    const int r = rl + 16 * mi;
+    pr[mi] = r < F ? parent_row(f, X, r - K) : 0;
+  }
+#pragma unroll
+  for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = cl + 16 * nj + 4 * q;
+      pc[nj][q] = c < F ? parent_row(f, X, c - K) * FP : 0;
+    }
+  double v[2][2][4];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = rl + 16 * mi, c = cl + 16 * nj + 4 * q;
+        v[mi][nj][q] = load_or_zero(B, pc[nj][q] + pr[mi], r < F && c < F && r >= c);
+      }
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = rl + 16 * mi, c = cl + 16 * nj + 4 * q;
+        if (r < F && c < F && r >= c) B[pc[nj][q] + pr[mi]] = v[mi][nj][q] + acc[mi][nj][q];
+      }
+}
+
+// One round of panel steps, one workgroup per item (node, panel, tr, tc): a 64x64 tile
+// (tr, tc) of the node's lower trailing region below the panel, or (tr = -1) the node's
+// first diagonal block, factored once every child has added its update matrix.  A tile:
 //   1. the tile's values, the panel columns of its row block tr and column block tc (C),
 //      L_PP^-T and the pivot inverses M_j are loaded (every global load first);
 //   2. W = C L_PP^-T for both blocks and L = W blockdiag(M_j) for block tr on the matrix
@@ -990,17 +1042,24 @@ typedef double mfma_d4 __attribute__((ext_vector_type(4)));
 //      own rows) and eliminate their rows of the node's right-hand side, b_r -= L_r y_P
 //      with y_P = L_PP^-1 b_P; tile (0, 0) also writes z_P = D^-1 y_P for the panel's
 //      columns, then factors the next panel's diagonal block (the top-left of its tile,
-//      handed over in LDS).
+//      handed over in LDS);
+//   5. on the node's last panel step the trailing region is its update matrix: the tile
+//      adds its results into the parent's front instead (extend-add; child update row
+//      block b is the parent's row block upos[b]), and diagonal tiles add their finished
+//      update rows of the right-hand side into the parent's.  No two children of one
+//      parent end in one round and a child's targets are distinct, so the adds are plain.
 // Each tile forms the L and W rows it needs itself (a row block is formed again by every
 // tile of its tile row and column), so a panel step is one launch with no panel-row pass
 // before it.
 __global__ void __launch_bounds__(kThreads)
-    k_mf_step(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc,
+    k_mf_step(DevFactor f, const int32_t* __restrict__ tiles, DevScalars* sc,
               double* __restrict__ z) {
   static_assert(kTile == 64 && kThreads == 256 && kPW == 48, "4 wavefronts, 16-row slices");
   // consecutive tiles (one tile row shares its L rows) run on one XCD
   const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int X = tiles[3 * bid], tr = tiles[3 * bid + 1], tc = tiles[3 * bid + 2];
+  const int X = tiles[4 * bid], panel = tiles[4 * bid + 1];
+  const int tr = tiles[4 * bid + 2], tc = tiles[4 * bid + 3];
+  const bool first = tr < 0;
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   double* A = f.front + f.sn_off[X];
   const DiagNode dn = diag_node(f, X);
@@ -1020,186 +1079,205 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double pM[kPanelBlocks][36];
   __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   __shared__ double sb[kPW], sy[kThreads / 64][kPW];
-  {
-    const int gb0 = dn.c0 + k0 / 6, nbp = np / 6;
-    constexpr int kNC = kPW * kTile / kThreads, kNT = kPW * kPW / kThreads;
-    constexpr int kNM = (kPanelBlocks * 36 + kThreads - 1) / kThreads;
-    static_assert(kPW * kTile % kThreads == 0 && kPW * kPW % kThreads == 0, "whole rounds");
-    double va[kNC], vb[kNC], vt[kNT], vm[kNM];
-#pragma unroll
-    for (int it = 0; it < kNC; ++it) {
-      const int e = tid + kThreads * it, t = e / kTile, i = e % kTile;
-      va[it] = load_or_zero(A, int64_t(k0 + t) * F + r0 + i, t < np && r0 + i < F);
-      vb[it] = load_or_zero(A, int64_t(k0 + t) * F + c0 + i, !dg && t < np && c0 + i < F);
-    }
-#pragma unroll
-    for (int it = 0; it < kNT; ++it) vt[it] = Tg[tid + kThreads * it];
-#pragma unroll
-    for (int it = 0; it < kNM; ++it) {
-      const int e = tid + kThreads * it;
-      vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
-    }
-    const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
-#pragma unroll
-    for (int it = 0; it < kNC; ++it) {
-      const int e = tid + kThreads * it;
-      sA[e / kTile][e % kTile] = va[it];
-      sB[e / kTile][e % kTile] = vb[it];
-    }
-#pragma unroll
-    for (int it = 0; it < kNT; ++it) {
-      const int e = tid + kThreads * it;
-      sT[e / kPW][e % kPW] = vt[it];
-    }
-#pragma unroll
-    for (int it = 0; it < kNM; ++it) {
-      const int e = tid + kThreads * it;
-      if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
-    }
-    if (tid < kPW) sb[tid] = vbb;
-  }
-  __syncthreads();
-  // accumulator (mi, nj) component q is the tile element
-  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
-  // it starts from the element's current value, loaded while W and L are formed
-  mfma_d4 acc[2][2];
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
-      }
-  // W = C L_PP^-T for the wavefront's slice of both row blocks; L_PP^-T is block upper
-  // triangular, so column tile ct stops at row kW[ct] (C and L_PP^-T are zero past np)
-  constexpr int kW[3] = {18, 36, 48};
-  mfma_d4 wa[3], wb[3];
-#pragma unroll
-  for (int ct = 0; ct < 3; ++ct) wa[ct] = wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int k4 = 0; k4 < kPW; k4 += 4) {
-    const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct)
-      if (k4 < kW[ct]) {
-        const double tv = sT[k4 + lc][16 * ct + lr];
-        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
-        wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
-      }
-  }
-  __syncthreads();  // every wavefront is done with C
-  // W^T of both blocks (a diagonal tile's column block is its row block)
-#pragma unroll
-  for (int ct = 0; ct < 3; ++ct)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
-      sB[16 * ct + lr][rw + lc + 4 * q] = dg ? wa[ct][q] : wb[ct][q];
-    }
-  // L = W blockdiag(M_j) for the slice of row block tr: column tile ct meets the pivot
-  // blocks with rows [kL0[ct], kL1[ct]); the block-diagonal operand is read from the M_j
-  constexpr int kL0[3] = {0, 12, 30}, kL1[3] = {18, 36, 48};
-#pragma unroll
-  for (int ct = 0; ct < 3; ++ct) wa[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int k4 = 0; k4 < kPW; k4 += 4) {
-    const int k = k4 + lc;
-    const double av = sA[k][rw + lr];
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct)
-      if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
-        const int n = 16 * ct + lr;
-        const double mv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
-        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wa[ct], 0, 0, 0);
-      }
-  }
-  // L^T replaces the slice's W^T in sA
-#pragma unroll
-  for (int ct = 0; ct < 3; ++ct)
-#pragma unroll
-    for (int q = 0; q < 4; ++q) sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
-  __syncthreads();  // the update reads every row of L^T and W^T
-#pragma unroll
-  for (int s = 0; s < kPW / 4; ++s) {
-    const int t = 4 * s + lc;  // this lane's k index
-    double a[2], b[2];
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-      for (int nj = 0; nj < 2; ++nj)
-        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
-  }
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-        if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
-      }
-  if (dg) {
-    // the slice's L rows to the factor (and the row-major copy of the node's own rows)
-    double* Lo = f.lbuf + dn.loff;
-    double* Lt = f.ltbuf + dn.toff;
-    for (int e = lane; e < 16 * kPW; e += 64) {
-      const int i = rw + e % 16, c = e / 16, r = r0 + i;
-      if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = sA[c][i];
-    }
-    if (r0 + rw < K)
-      for (int e = lane; e < 16 * kPW; e += 64) {
-        const int c = e % kPW, i = rw + e / kPW, r = r0 + i;
-        if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sA[c][i];
-      }
-    // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront its
-    // own copy, lane c < kPW entry c), then b_r -= L_r y_P for the slice's rows (four
-    // lanes per row, every fourth column, two xor shuffles)
-    if (lane < kPW) {
-      double a4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
-      sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
-    }
+  // the node's last panel step: its update matrix goes to the parent (-1: none)
+  const int P = kend == K ? f.sn_parent[X] : -1;
+  bool diag = first;  // this workgroup factors a diagonal block (after its tile, if any)
+  if (!first) {
     {
-      const int i = rw + (lane >> 2), part = lane & 3, r = r0 + i;
-      double s4 = 0.0;
+      const int gb0 = dn.c0 + k0 / 6, nbp = np / 6;
+      constexpr int kNC = kPW * kTile / kThreads, kNT = kPW * kPW / kThreads;
+      constexpr int kNM = (kPanelBlocks * 36 + kThreads - 1) / kThreads;
+      static_assert(kPW * kTile % kThreads == 0 && kPW * kPW % kThreads == 0, "whole rounds");
+      double va[kNC], vb[kNC], vt[kNT], vm[kNM];
 #pragma unroll
-      for (int u = 0; u < kPW / 4; ++u) s4 += sA[part + 4 * u][i] * sy[wv][part + 4 * u];
-      s4 += __shfl_xor(s4, 1, 64);
-      s4 += __shfl_xor(s4, 2, 64);
-      if (part == 0 && r < F) bx[r] -= s4;
-    }
-    if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
-      const int jb = lane / 6, a = lane % 6;
-      double zz = 0.0;
+      for (int it = 0; it < kNC; ++it) {
+        const int e = tid + kThreads * it, t = e / kTile, i = e % kTile;
+        va[it] = load_or_zero(A, int64_t(k0 + t) * F + r0 + i, t < np && r0 + i < F);
+        vb[it] = load_or_zero(A, int64_t(k0 + t) * F + c0 + i, !dg && t < np && c0 + i < F);
+      }
 #pragma unroll
-      for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[0][6 * jb + t];
-      z[6 * int64_t(dn.c0) + k0 + lane] = zz;
+      for (int it = 0; it < kNT; ++it) vt[it] = Tg[tid + kThreads * it];
+#pragma unroll
+      for (int it = 0; it < kNM; ++it) {
+        const int e = tid + kThreads * it;
+        vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
+      }
+      const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
+#pragma unroll
+      for (int it = 0; it < kNC; ++it) {
+        const int e = tid + kThreads * it;
+        sA[e / kTile][e % kTile] = va[it];
+        sB[e / kTile][e % kTile] = vb[it];
+      }
+#pragma unroll
+      for (int it = 0; it < kNT; ++it) {
+        const int e = tid + kThreads * it;
+        sT[e / kPW][e % kPW] = vt[it];
+      }
+#pragma unroll
+      for (int it = 0; it < kNM; ++it) {
+        const int e = tid + kThreads * it;
+        if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
+      }
+      if (tid < kPW) sb[tid] = vbb;
     }
-  }
-  if (tr == 0 && tc == 0 && kend < K) {
-    // the next panel's diagonal block is the top-left of this tile: hand the updated
-    // values to LDS directly
-    const int npn = min(K, kend + kPW) - kend;
-    double (*D)[kPW + 1] = as_square(sA);
-    __syncthreads();  // every read of sA / sB is done
+    __syncthreads();
+    // accumulator (mi, nj) component q is the tile element
+    //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+    // it starts from the element's current value, loaded while W and L are formed
+    mfma_d4 acc[2][2];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
       for (int nj = 0; nj < 2; ++nj)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
-          if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+          const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+          acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
         }
-    __syncthreads();
-    factor_panel_diag(f, dn, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true, z);
+    // W = C L_PP^-T for the wavefront's slice of both row blocks; L_PP^-T is block upper
+    // triangular, so column tile ct stops at row kW[ct] (C and L_PP^-T are zero past np)
+    constexpr int kW[3] = {18, 36, 48};
+    mfma_d4 wa[3], wb[3];
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct) wa[ct] = wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k4 = 0; k4 < kPW; k4 += 4) {
+      const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
+#pragma unroll
+      for (int ct = 0; ct < 3; ++ct)
+        if (k4 < kW[ct]) {
+          const double tv = sT[k4 + lc][16 * ct + lr];
+          wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
+          wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
+        }
+    }
+    __syncthreads();  // every wavefront is done with C
+    // W^T of both blocks (a diagonal tile's column block is its row block)
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
+        sB[16 * ct + lr][rw + lc + 4 * q] = dg ? wa[ct][q] : wb[ct][q];
+      }
+    // L = W blockdiag(M_j) for the slice of row block tr: column tile ct meets the pivot
+    // blocks with rows [kL0[ct], kL1[ct]); the block-diagonal operand is read from the M_j
+    constexpr int kL0[3] = {0, 12, 30}, kL1[3] = {18, 36, 48};
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct) wa[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k4 = 0; k4 < kPW; k4 += 4) {
+      const int k = k4 + lc;
+      const double av = sA[k][rw + lr];
+#pragma unroll
+      for (int ct = 0; ct < 3; ++ct)
+        if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
+          const int n = 16 * ct + lr;
+          const double mv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
+          wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wa[ct], 0, 0, 0);
+        }
+    }
+    // L^T replaces the slice's W^T in sA
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
+    __syncthreads();  // the update reads every row of L^T and W^T
+#pragma unroll
+    for (int s = 0; s < kPW / 4; ++s) {
+      const int t = 4 * s + lc;  // this lane's k index
+      double a[2], b[2];
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+          acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
+    }
+    if (P < 0) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+            if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
+          }
+    } else {
+      extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
+    }
+    if (dg) {
+      // the slice's L rows to the factor (and the row-major copy of the node's own rows)
+      double* Lo = f.lbuf + dn.loff;
+      double* Lt = f.ltbuf + dn.toff;
+      for (int e = lane; e < 16 * kPW; e += 64) {
+        const int i = rw + e % 16, c = e / 16, r = r0 + i;
+        if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = sA[c][i];
+      }
+      if (r0 + rw < K)
+        for (int e = lane; e < 16 * kPW; e += 64) {
+          const int c = e % kPW, i = rw + e / kPW, r = r0 + i;
+          if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sA[c][i];
+        }
+      // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront its
+      // own copy, lane c < kPW entry c), then b_r -= L_r y_P for the slice's rows (four
+      // lanes per row, every fourth column, two xor shuffles)
+      if (lane < kPW) {
+        double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
+        sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+      }
+      {
+        const int i = rw + (lane >> 2), part = lane & 3, r = r0 + i;
+        double s4 = 0.0;
+#pragma unroll
+        for (int u = 0; u < kPW / 4; ++u) s4 += sA[part + 4 * u][i] * sy[wv][part + 4 * u];
+        s4 += __shfl_xor(s4, 1, 64);
+        s4 += __shfl_xor(s4, 2, 64);
+        if (part == 0 && r < F) {
+          if (P < 0)
+            bx[r] -= s4;
+          else  // a finished update row: into the parent's right-hand side
+            f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += bx[r] - s4;
+        }
+      }
+      if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
+        const int jb = lane / 6, a = lane % 6;
+        double zz = 0.0;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[0][6 * jb + t];
+        z[6 * int64_t(dn.c0) + k0 + lane] = zz;
+      }
+    }
+    if (tr == 0 && tc == 0 && kend < K) {
+      // the next panel's diagonal block is the top-left of this tile: hand the updated
+      // values to LDS directly
+      const int npn = min(K, kend + kPW) - kend;
+      double (*D)[kPW + 1] = as_square(sA);
+      __syncthreads();  // every read of sA / sB is done
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
+            if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+          }
+      __syncthreads();
+      diag = true;
+    }
+  }
+  if (diag) {
+    const int dk0 = first ? 0 : kend;
+    factor_panel_diag(f, dn, A, F, dk0, min(K, dk0 + kPW) - dk0, sc, as_square(sA),
+                      as_square(sB), sL, sdi, !first, z);
   }
 }
 
@@ -1207,18 +1285,19 @@ __global__ void __launch_bounds__(kThreads)
 // block's L and W once, instead of in every tile of its tile row and column, saves more
 // matrix-core time than the extra launch costs).
 //
-// Panel rows of one panel step for the level's nodes (items = (node, kPanelRows-row
-// chunk) below the panel's diagonal block), as two products instead of a substitution:
+// Panel rows of one round of panel steps (items = (node, panel, kPanelRows-row chunk)
+// below the panel's diagonal block), as two products instead of a substitution:
 //   W_r = C_r L_PP^-T            (the rows' values after the panel's own eliminations)
 //   L_r,j = W_r,j M_j,  M_j = L_jj^-T D_j^-1 L_jj^-1
 // with L_PP^-T from the diagonal factorization.  Both run on the matrix cores as 4 x 3
 // tiles of v_mfma_f64_16x16x4, three per wavefront.  W goes to the node's W scratch, L to
 // the factor and, for the node's own rows, to its row-major copy.
 __global__ void __launch_bounds__(kThreads)
-    k_mf_panel(DevFactor f, int panel, const int32_t* __restrict__ items, double* __restrict__ z) {
+    k_mf_panel(DevFactor f, const int32_t* __restrict__ items, double* __restrict__ z) {
   static_assert(kPanelRows == 64 && kPW == 48 && kThreads == 256, "4 x 3 tiles, 3 per wavefront");
   static_assert(kPW * kPanelRows % kThreads == 0 && kPW * kPW % kThreads == 0, "whole staging rounds");
-  const int X = items[2 * blockIdx.x], chunk = items[2 * blockIdx.x + 1];
+  const int X = items[3 * blockIdx.x], panel = items[3 * blockIdx.x + 1];
+  const int chunk = items[3 * blockIdx.x + 2];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   const double* A = f.front + f.sn_off[X];
   double* Lo = f.lbuf + f.sn_loff[X];
@@ -1336,7 +1415,12 @@ __global__ void __launch_bounds__(kThreads)
     for (int u = 0; u < kPW / 4; ++u) acc += sC[part + 4 * u][i] * sy[wv][part + 4 * u];
     acc += __shfl_xor(acc, 1, 64);
     acc += __shfl_xor(acc, 2, 64);
-    if (part == 0 && r < F) bx[r] -= acc;
+    if (part == 0 && r < F) {
+      if (kend < K || f.sn_parent[X] < 0)
+        bx[r] -= acc;
+      else  // the node's last step: a finished update row, into the parent's right-hand side
+        f.usolve[f.sn_uoff[f.sn_parent[X]] + parent_row(f, X, r - K)] += bx[r] - acc;
+    }
   }
   // the wavefront's 16 rows to memory: W and L column-major, and for the node's own rows
   // the row-major copy of L
@@ -1354,20 +1438,24 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-// Trailing update of one panel step, one workgroup per 64x64 tile of the lower trailing
-// region: A[r, c] -= L_r W_c^T on the matrix cores.  The tile's L rows and W columns are
-// staged in LDS (zero past the panel's width); each wavefront owns a 32x32 quadrant as
-// 2 x 2 tiles of v_mfma_f64_16x16x4, computing (-W) L^T so that each lane's results run
-// down 16 consecutive rows of the column-major front (coalesced read-modify-write).  The
-// accumulators start from the tile's current values.  The tile (0, 0), which holds the
-// next panel's diagonal block, then factors that block.
+// Trailing updates of one round of split panel steps, one workgroup per item as in
+// k_mf_step (a 64x64 tile of a node's lower trailing region, or a first diagonal block):
+// A[r, c] -= L_r W_c^T on the matrix cores.  The tile's L rows and W columns are staged in
+// LDS (zero past the panel's width); each wavefront owns a 32x32 quadrant as 2 x 2 tiles
+// of v_mfma_f64_16x16x4, computing (-W) L^T so that each lane's results run down 16
+// consecutive rows of the column-major front (coalesced read-modify-write).  The
+// accumulators start from the tile's current values.  On the node's last step the results
+// go into the parent's front (extend-add).  The tile (0, 0), which holds the next panel's
+// diagonal block, then factors that block.
 __global__ void __launch_bounds__(kThreads)
-    k_mf_update(DevFactor f, int panel, const int32_t* __restrict__ tiles, DevScalars* sc,
+    k_mf_update(DevFactor f, const int32_t* __restrict__ tiles, DevScalars* sc,
                 double* __restrict__ z) {
   static_assert(kTile == 64 && kThreads == 256 && kPW % 4 == 0, "4 wavefronts of 32x32");
   // consecutive tiles (one tile row shares its L rows) run on one XCD
   const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
-  const int X = tiles[3 * bid], tr = tiles[3 * bid + 1], tc = tiles[3 * bid + 2];
+  const int X = tiles[4 * bid], panel = tiles[4 * bid + 1];
+  const int tr = tiles[4 * bid + 2], tc = tiles[4 * bid + 3];
+  const bool first = tr < 0;
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   double* A = f.front + f.sn_off[X];
   const double* Lo = f.lbuf + f.sn_loff[X];
@@ -1382,73 +1470,87 @@ __global__ void __launch_bounds__(kThreads)
   const int rb = 32 * (wv >> 1), cb = 32 * (wv & 1);  // the wavefront's quadrant
   __shared__ double sA[kPW][kTile + 1], sB[kPW][kTile + 1];
   __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
-  // accumulator (mi, nj) component q is the tile element
-  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
-  // it starts from the element's current value (loaded first)
-  mfma_d4 acc[2][2];
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
-      }
-  constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
-  stage_lds2<kStage>(
-      kPW * kTile,
-      [&](int e, bool ok) {
-        const int t = e / kTile, r = r0 + e % kTile;
-        return load_or_zero(Lo, int64_t(k0 + t) * F + r, ok && t < np && r < F);
-      },
-      [&](int e, double v) { sA[e / kTile][e % kTile] = v; },
-      [&](int e, bool ok) {
-        const int t = e / kTile, c = c0 + e % kTile;
-        return load_or_zero(Wo, int64_t(t) * F + c, ok && t < np && c < F);
-      },
-      [&](int e, double v) { sB[e / kTile][e % kTile] = v; });
-  __syncthreads();
-#pragma unroll
-  for (int s = 0; s < kPW / 4; ++s) {
-    const int t = 4 * s + lc;  // this lane's k index
-    double a[2], b[2];
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
-#pragma unroll
-    for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-      for (int nj = 0; nj < 2; ++nj)
-        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
-  }
-#pragma unroll
-  for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-    for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-        if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
-      }
-  if (tr == 0 && tc == 0 && kend < K) {
-    // the next panel's diagonal block is the top-left of this tile: hand the updated
-    // values to LDS directly
-    const int npn = min(K, kend + kPW) - kend;
-    double (*D)[kPW + 1] = as_square(sA);
-    __syncthreads();  // every MFMA operand read of sA / sB is done
+  // the node's last panel step: its update matrix goes to the parent (-1: none)
+  const int P = kend == K ? f.sn_parent[X] : -1;
+  bool diag = first;  // this workgroup factors a diagonal block (after its tile, if any)
+  if (!first) {
+    // accumulator (mi, nj) component q is the tile element
+    //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+    // it starts from the element's current value (loaded first)
+    mfma_d4 acc[2][2];
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
       for (int nj = 0; nj < 2; ++nj)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
-          if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+          const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+          acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
         }
+    constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
+    stage_lds2<kStage>(
+        kPW * kTile,
+        [&](int e, bool ok) {
+          const int t = e / kTile, r = r0 + e % kTile;
+          return load_or_zero(Lo, int64_t(k0 + t) * F + r, ok && t < np && r < F);
+        },
+        [&](int e, double v) { sA[e / kTile][e % kTile] = v; },
+        [&](int e, bool ok) {
+          const int t = e / kTile, c = c0 + e % kTile;
+          return load_or_zero(Wo, int64_t(t) * F + c, ok && t < np && c < F);
+        },
+        [&](int e, double v) { sB[e / kTile][e % kTile] = v; });
     __syncthreads();
-    factor_panel_diag(f, dn, A, F, kend, npn, sc, D, as_square(sB), sL, sdi, true, z);
+#pragma unroll
+    for (int s = 0; s < kPW / 4; ++s) {
+      const int t = 4 * s + lc;  // this lane's k index
+      double a[2], b[2];
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+          acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
+    }
+    if (P < 0) {
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+            if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
+          }
+    } else {
+      extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
+    }
+    if (tr == 0 && tc == 0 && kend < K) {
+      // the next panel's diagonal block is the top-left of this tile: hand the updated
+      // values to LDS directly
+      const int npn = min(K, kend + kPW) - kend;
+      double (*D)[kPW + 1] = as_square(sA);
+      __syncthreads();  // every MFMA operand read of sA / sB is done
+#pragma unroll
+      for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+        for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
+            if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+          }
+      __syncthreads();
+      diag = true;
+    }
+  }
+  if (diag) {
+    const int dk0 = first ? 0 : kend;
+    factor_panel_diag(f, dn, A, F, dk0, min(K, dk0 + kPW) - dk0, sc, as_square(sA),
+                      as_square(sB), sL, sdi, !first, z);
   }
 }
 
@@ -1459,52 +1561,6 @@ __global__ void __launch_bounds__(kThreads) k_mf_rhs_init(DevFactor f, const dou
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
   double* bx = f.usolve + f.sn_uoff[X];
   for (int r = threadIdx.x; r < F; r += blockDim.x) bx[r] = r < K ? b[6 * int64_t(c0) + r] : 0.0;
-}
-
-// Extend-add: scatter each finished node's update matrix, and the update rows of its
-// right-hand side, into its parent's front.  The launch's nodes are distinct children of
-// distinct parents (one sibling rank at a time), so the adds need no atomics and their
-// order is fixed.
-__global__ void __launch_bounds__(kThreads)
-    k_mf_extend(DevFactor f, const int32_t* __restrict__ nodes) {
-  const int X = nodes[blockIdx.y];
-  const int P = f.sn_parent[X];
-  if (P < 0) return;
-  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], m = F - K;
-  const int FP = 6 * f.sn_nf[P];
-  const double* U = f.front + f.sn_off[X] + int64_t(K) * F + K;
-  double* B = f.front + f.sn_off[P];
-  const int32_t* up = f.sn_upos + f.sn_upos_ptr[X];
-  if (blockIdx.x == 0) {  // the update rows' right-hand side into the parent's
-    const double* bx = f.usolve + f.sn_uoff[X] + K;
-    double* bp = f.usolve + f.sn_uoff[P];
-    for (int r = threadIdx.x; r < m; r += blockDim.x) bp[6 * up[r / 6] + r % 6] += bx[r];
-  }
-  // one child per parent per launch and distinct targets within a child: plain adds.
-  // Four rows per thread per pass, every load of the pass issued before the adds.
-  constexpr int kU = 4;
-  for (int c = blockIdx.x; c < m; c += gridDim.x) {
-    const int pc = 6 * up[c / 6] + c % 6;
-    double* Bc = B + int64_t(pc) * FP;
-    const double* Uc = U + int64_t(c) * F;
-    for (int r0 = c; r0 < m; r0 += kU * int(blockDim.x)) {
-      int64_t dst[kU];
-      double v[kU], b[kU];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int r = min(r0 + int(threadIdx.x) + u * int(blockDim.x), m - 1);  // valid index
-        dst[u] = 6 * up[r / 6] + r % 6;
-        v[u] = Uc[r];
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u) b[u] = Bc[dst[u]];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int r = r0 + int(threadIdx.x) + u * int(blockDim.x);
-        if (r < m) Bc[dst[u]] = b[u] + v[u];
-      }
-    }
-  }
 }
 
 // ---- solves with the supernodal factor ----------------------------------------------------
@@ -2133,22 +2189,6 @@ void launch_zero_scalars(hipStream_t s, DevScalars* sc) {
   hipLaunchKernelGGL(k_zero_scalars, dim3(1), dim3(64), 0, s, sc);
 }
 
-// extend-add of one level's nodes, one sibling rank per launch
-static void launch_extend_groups(hipStream_t s, const DevFactor& f, int level) {
-  const Symbolic& S = *f.host;
-  for (int q = S.ext_lvl_ptr[level]; q < S.ext_lvl_ptr[level + 1]; ++q) {
-    const int cnt = S.ext_grp_ptr[q + 1] - S.ext_grp_ptr[q];
-    int max_m = 1;  // one workgroup per update-matrix column
-    for (int e = S.ext_grp_ptr[q]; e < S.ext_grp_ptr[q + 1]; ++e) {
-      const int x = S.ext_nodes[e];
-      max_m = std::max(max_m, 6 * (S.sn_nf[x] - S.sn_nc[x]));
-    }
-    if (cnt > 0)
-      hipLaunchKernelGGL(k_mf_extend, dim3(max_m, cnt), dim3(kThreads), 0, s, f,
-                         f.ext_nodes + S.ext_grp_ptr[q]);
-  }
-}
-
 void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const double* rhs_nat,
                        double* y) {
   const Symbolic& S = *f.host;
@@ -2156,28 +2196,21 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const 
   const int g = blocks_for(6 * int64_t(f.nb), kThreads);
   hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, rhs_nat, y);
   hipLaunchKernelGGL(k_mf_rhs_init, dim3(S.nsn), dim3(kThreads), 0, s, f, y);
-  int level = -1;
-  for (const auto& L : S.launches) {
-    if (L.level != level) {
-      // extend-add of the previous level's nodes into their parents, then the first
-      // panel diagonal of this level's nodes
-      if (level >= 0) launch_extend_groups(s, f, level);
-      level = L.level;
-      const int n = S.lvl_ptr[level + 1] - S.lvl_ptr[level];
-      hipLaunchKernelGGL(k_mf_diag0, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[level],
-                         sc, y);
+  for (size_t li = 0; li < S.launches.size(); ++li) {
+    const auto& L = S.launches[li];
+    const int32_t* items = f.launch_tiles + 4 * int64_t(L.tile_begin);
+    if (L.tile_count == L.first_count) {  // first diagonal blocks only
+      if (L.tile_count > 0)
+        hipLaunchKernelGGL(k_mf_diag0, dim3(L.tile_count), dim3(kThreads), 0, s, f, items, sc, y);
+      continue;
     }
-    const size_t li = size_t(&L - S.launches.data());
-    if (L.tile_count == 0) continue;
     if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
     if (L.fused) {
-      hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
-                         f.launch_tiles + 3 * int64_t(L.tile_begin), sc, y);
+      hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, items, sc, y);
     } else {
-      hipLaunchKernelGGL(k_mf_panel, dim3(L.item_count), dim3(kThreads), 0, s, f, L.panel,
-                         f.panel_items + 2 * int64_t(L.item_begin), y);
-      hipLaunchKernelGGL(k_mf_update, dim3(L.tile_count), dim3(kThreads), 0, s, f, L.panel,
-                         f.launch_tiles + 3 * int64_t(L.tile_begin), sc, y);
+      hipLaunchKernelGGL(k_mf_panel, dim3(L.item_count), dim3(kThreads), 0, s, f,
+                         f.panel_items + 3 * int64_t(L.item_begin), y);
+      hipLaunchKernelGGL(k_mf_update, dim3(L.tile_count), dim3(kThreads), 0, s, f, items, sc, y);
     }
     if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li + 1], s));
   }

@@ -70,13 +70,12 @@ struct DevFactor {
   const int32_t* sn_upos_ptr;
   const int32_t* sn_upos;
   const int32_t* lvl_sn;
-  const int32_t* launch_tiles;
+  const int32_t* launch_tiles;  // round work items (node, panel, tile_row, tile_col)
   const int32_t* solve_cols;  // (node, 64-column tile) per level
-  const int32_t* ext_nodes;   // extend-add groups (see Symbolic)
   const int64_t* sn_uoff;     // right-hand side offsets (F entries per node)
   double* usolve;             // each node's right-hand side in front row order (u_elems)
-  hipEvent_t* step_events;    // optional: 2 per panel step, recorded around its launches
-  const int32_t* panel_items;  // (node, kPanelRows-row chunk) per panel-row workgroup
+  hipEvent_t* step_events;    // optional: 2 per round of panel steps, around its launches
+  const int32_t* panel_items;  // (node, panel, kPanelRows-row chunk) per panel-row workgroup
   double* wbuf;                // per node F x kPW: unscaled W of the current panel (ld F)
   const int64_t* sn_woff;
   double* tinv;                // per panel kPW x kPW row-major: L_PP^-T (unit block diagonal)

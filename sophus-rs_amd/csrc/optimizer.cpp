@@ -119,7 +119,7 @@ struct sgpu_optimizer {
     if (!timing) return;
     if (timing == 2 && used[K_FACTOR] && df.step_events) {
       for (size_t li = 0; li < S.launches.size(); ++li) {
-        if (S.launches[li].tile_count == 0) continue;
+        if (S.launches[li].tile_count == S.launches[li].first_count) continue;  // no step
         float ms = 0;
         SGPU_HIP(hipEventElapsedTime(&ms, step_ev[2 * li], step_ev[2 * li + 1]));
         kernel_ms[K_MF_STEP] += ms;
@@ -290,7 +290,6 @@ struct sgpu_optimizer {
     df.panel_items = arena.upload(S.panel_items, stream);
     df.sn_woff = arena.upload(S.sn_woff, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
-    df.ext_nodes = arena.upload(S.ext_nodes, stream);
     df.sn_uoff = arena.upload(S.sn_uoff, stream);
     df.usolve = arena.alloc<double>(size_t(S.u_elems));
     if (!iterative()) {
@@ -635,7 +634,7 @@ static void fill_stats(const HostProblem& P, const Symbolic& S, sgpu_rcs_stats* 
   for (const auto& L : S.launches) {
     st->mf_step_flops += L.flops;
     st->mf_step_bytes += L.bytes;
-    st->mf_step_launches += L.tile_count > 0 ? 1 : 0;
+    st->mf_step_launches += L.tile_count > L.first_count ? 1 : 0;
   }
 }
 

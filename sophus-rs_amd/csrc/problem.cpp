@@ -605,48 +605,81 @@ void analyze_rcs(Symbolic& S) {
     S.sn_toff[x] = S.t_elems;
     S.t_elems += int64_t(36) * S.sn_nc[x] * S.sn_nc[x];
   }
-  for (int32_t l = 0; l < nlev; ++l) {
-    int32_t maxpan = 0;
-    for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q)
-      maxpan = std::max(maxpan, (S.sn_nc[S.lvl_sn[q]] + kPanelBlocks - 1) / kPanelBlocks);
-    for (int32_t pn = 0; pn < maxpan; ++pn) {
-      Symbolic::PanelLaunch L{l,   pn, 0, 0, int32_t(S.launch_tiles.size() / 3), 0,
-                              int32_t(S.panel_items.size() / 2), 0, true, 0.0, 0.0};
-      // every node's tile (0, 0) (which factors the next diagonal block) first, so the
-      // critical path is dispatched first; then the other tiles, node-major, row-major
-      for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
-        const int32_t x = S.lvl_sn[q];
-        const int32_t npan = (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks;
-        const int32_t kend = 6 * std::min(S.sn_nc[x], (pn + 1) * kPanelBlocks);
-        if (pn < npan && 6 * S.sn_nf[x] > kend) {
-          S.launch_tiles.insert(S.launch_tiles.end(), {x, 0, 0});
-          ++L.tile_count;
-        }
+  // round schedule: a node's first diagonal block in the round after its children's last
+  // steps (round 0 for leaves), then one panel step per round; children of one parent end
+  // in distinct rounds (the earliest possible, in the order they could end)
+  {
+    auto npanels = [&](int32_t x) { return (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks; };
+    // panel steps: every panel with rows below it (all but a root's last)
+    auto nsteps = [&](int32_t x) { return npanels(x) - (S.sn_nf[x] > S.sn_nc[x] ? 0 : 1); };
+    std::vector<std::vector<int32_t>> kids(S.nsn);
+    for (int32_t x = 0; x < S.nsn; ++x)
+      if (S.sn_parent[x] >= 0) kids[S.sn_parent[x]].push_back(x);
+    std::vector<int32_t> first(S.nsn, 0);
+    S.sn_last_round.assign(S.nsn, -1);
+    for (int32_t x = 0; x < S.nsn; ++x) {  // children have smaller indices than parents
+      std::vector<std::pair<int32_t, int32_t>> want;
+      for (const int32_t c : kids[x]) {
+        if (nsteps(c) == 0) throw std::runtime_error("supernode with a parent but no update rows");
+        want.push_back({first[c] + nsteps(c), c});
       }
-      for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) {
-        const int32_t x = S.lvl_sn[q];
-        const int32_t npan = (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks;
-        if (pn >= npan) continue;
-        const int32_t kend = 6 * std::min(S.sn_nc[x], (pn + 1) * kPanelBlocks);
+      std::sort(want.begin(), want.end());
+      int32_t r = -1;
+      for (const auto& [w, c] : want) S.sn_last_round[c] = r = std::max(w, r + 1);
+      first[x] = r + 1;
+      if (S.sn_parent[x] < 0 && nsteps(x) > 0) S.sn_last_round[x] = first[x] + nsteps(x);
+    }
+    // priority within a round: the rounds from a node's first to its root's end
+    std::vector<int32_t> prio(S.nsn, 0);
+    for (int32_t x = S.nsn - 1; x >= 0; --x)
+      prio[x] = nsteps(x) + 1 + (S.sn_parent[x] >= 0 ? prio[S.sn_parent[x]] : 0);
+    int32_t nround = 0;
+    for (int32_t x = 0; x < S.nsn; ++x) nround = std::max({nround, first[x] + 1, S.sn_last_round[x] + 1});
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> steps(nround);  // (node, panel)
+    std::vector<std::vector<int32_t>> firsts(nround);
+    for (int32_t x = 0; x < S.nsn; ++x) {
+      firsts[first[x]].push_back(x);
+      const int32_t ns = nsteps(x);
+      for (int32_t p = 0; p < ns; ++p)
+        steps[p == ns - 1 ? S.sn_last_round[x] : first[x] + 1 + p].push_back({x, p});
+    }
+    auto by_prio = [&](int32_t a, int32_t b) { return prio[a] != prio[b] ? prio[a] > prio[b] : a < b; };
+    for (int32_t r = 0; r < nround; ++r) {
+      auto& st = steps[r];
+      std::sort(st.begin(), st.end(), [&](const auto& a, const auto& b) { return by_prio(a.first, b.first); });
+      std::sort(firsts[r].begin(), firsts[r].end(), by_prio);
+      Symbolic::PanelLaunch L{int32_t(S.launch_tiles.size() / 4), 0, 0,
+                              int32_t(S.panel_items.size() / 3), 0, true, 0.0, 0.0};
+      auto push = [&](int32_t x, int32_t p, int32_t tr, int32_t tc) {
+        S.launch_tiles.insert(S.launch_tiles.end(), {x, p, tr, tc});
+        ++L.tile_count;
+      };
+      // the critical work first: tile (0, 0) of every step that factors a next diagonal
+      // block, then the first diagonal blocks, then the other tiles
+      for (const auto& [x, p] : st)
+        if (p + 1 < npanels(x)) push(x, p, 0, 0);
+      for (const int32_t x : firsts[r]) {
+        push(x, 0, -1, 0);
+        ++L.first_count;
+      }
+      for (const auto& [x, p] : st) {
+        const int32_t kend = 6 * std::min(S.sn_nc[x], (p + 1) * kPanelBlocks);
         const int32_t T = 6 * S.sn_nf[x] - kend;
         const int32_t nt = (T + kTile - 1) / kTile;
-        const double np = double(kend - 6 * std::min(S.sn_nc[x], pn * kPanelBlocks));
+        const double np = double(kend - 6 * p * kPanelBlocks);
         // lower trailing update T(T+1)/2 x np multiply-adds; bytes: the lower trailing
         // region read and written, the panel columns of the rows below read, their L written
         L.flops += double(T) * double(T + 1) * np;
         L.bytes += 16.0 * double(T) * double(T + 1) / 2.0 + 16.0 * double(T) * np;
-        for (int32_t ch = 0; ch * kPanelRows < T; ++ch) S.panel_items.insert(S.panel_items.end(), {x, ch});
+        for (int32_t ch = 0; ch * kPanelRows < T; ++ch) S.panel_items.insert(S.panel_items.end(), {x, p, ch});
         for (int32_t tr = 0; tr < nt; ++tr)
-          for (int32_t tc = 0; tc <= tr; ++tc) {
-            if (tr == 0 && tc == 0) continue;  // listed first
-            S.launch_tiles.insert(S.launch_tiles.end(), {x, tr, tc});
-            ++L.tile_count;
-          }
+          for (int32_t tc = 0; tc <= tr; ++tc)
+            if (tr > 0 || p + 1 >= npanels(x)) push(x, p, tr, tc);
       }
-      L.item_count = int32_t(S.panel_items.size() / 2) - L.item_begin;
-      L.fused = L.tile_count < kSplitTiles;
+      L.item_count = int32_t(S.panel_items.size() / 3) - L.item_begin;
+      L.fused = L.tile_count - L.first_count < kSplitTiles;
       if (L.fused) {  // the fused launch forms its panel rows itself
-        S.panel_items.resize(size_t(2) * L.item_begin);
+        S.panel_items.resize(size_t(3) * L.item_begin);
         L.item_count = 0;
       }
       S.launches.push_back(L);
@@ -666,24 +699,7 @@ void analyze_rcs(Symbolic& S) {
     }
     S.solve_cols_ptr[l + 1] = int32_t(S.solve_cols.size() / 2);
   }
-  // extend-add groups (children ranked by node index within their parent)
   {
-    std::vector<int32_t> rank(S.nsn, -1), seen(S.nsn, 0);
-    for (int32_t x = 0; x < S.nsn; ++x)
-      if (S.sn_parent[x] >= 0) rank[x] = seen[S.sn_parent[x]]++;
-    S.ext_lvl_ptr.assign(nlev + 1, 0);
-    S.ext_grp_ptr.assign(1, 0);
-    S.ext_nodes.clear();
-    for (int32_t l = 0; l < nlev; ++l) {
-      int32_t maxr = -1;
-      for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q) maxr = std::max(maxr, rank[S.lvl_sn[q]]);
-      for (int32_t r = 0; r <= maxr; ++r) {
-        for (int32_t q = S.lvl_ptr[l]; q < S.lvl_ptr[l + 1]; ++q)
-          if (rank[S.lvl_sn[q]] == r) S.ext_nodes.push_back(S.lvl_sn[q]);
-        S.ext_grp_ptr.push_back(int32_t(S.ext_nodes.size()));
-      }
-      S.ext_lvl_ptr[l + 1] = int32_t(S.ext_grp_ptr.size()) - 1;
-    }
     S.sn_uoff.assign(S.nsn, 0);
     S.u_elems = 0;
     for (int32_t x = 0; x < S.nsn; ++x) {

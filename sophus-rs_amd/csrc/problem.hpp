@@ -91,18 +91,28 @@ struct Symbolic {
   int64_t front_elems = 0, l_elems = 0;
   int32_t max_front = 0;                      // largest front (blocks)
   std::vector<int32_t> lvl_ptr, lvl_sn;
-  // schedule: nodes per level; per (level, panel) the 64x64 tiles of every node's lower
-  // trailing region, either as one fused panel-step launch or (many tiles) as a panel-row
-  // launch over (node, kPanelRows-row chunk) items followed by a trailing-update launch
+  // Schedule of the numeric factorization in rounds, one launch (or a panel-row launch and
+  // a trailing-update launch) per round.  A node's first diagonal block is factored in a
+  // round after every child's last step; its panel steps then run in the rounds that
+  // follow, one per round, so nodes of different tree levels share a launch and the
+  // number of rounds follows the elimination tree's critical path.  A child's last step
+  // adds its update matrix (and its update rows of the right-hand side) into the parent's
+  // front; children of one parent finish in distinct rounds, so those adds are plain and
+  // their order is fixed.
   struct PanelLaunch {
-    int32_t level, panel, node_begin, node_count, tile_begin, tile_count, item_begin, item_count;
+    int32_t tile_begin, tile_count;  // work items in launch_tiles
+    int32_t first_count;             // of which first-diagonal items (tile_row -1)
+    int32_t item_begin, item_count;  // split rounds: panel-row items
     bool fused;
-    double flops;  // algorithmic FP64 flops of the trailing update
+    double flops;  // algorithmic FP64 flops of the trailing updates
     double bytes;  // algorithmic HBM bytes: lower trailing RMW, panel columns read, L written
   };
   std::vector<PanelLaunch> launches;
-  std::vector<int32_t> launch_tiles;  // (node, tile_row, tile_col) triples
-  std::vector<int32_t> panel_items;   // (node, kPanelRows-row chunk) pairs (split launches)
+  // 4 per work item: (node, panel, tile_row, tile_col) for a 64x64 tile of the lower
+  // trailing region below the node's panel; (node, 0, -1, 0) for a first diagonal block
+  std::vector<int32_t> launch_tiles;
+  std::vector<int32_t> panel_items;   // (node, panel, kPanelRows-row chunk) (split rounds)
+  std::vector<int32_t> sn_last_round;  // round of each node's last panel step (-1: none)
   std::vector<int64_t> sn_woff;       // W scratch offsets (6 nf x kPW per node)
   int64_t w_elems = 0;
   std::vector<int64_t> sn_toff;       // row-major copies of each node's L_KK (K x K)
@@ -112,10 +122,6 @@ struct Symbolic {
   // backward-solve work items per level: (node, kSolveCols-column tile of K)
   std::vector<int32_t> solve_cols_ptr, solve_cols;
   std::vector<int32_t> lvl_max_nc;  // largest node (column blocks) per level
-  // extend-add groups: per level, nodes grouped by their rank among their parent's
-  // children, so one launch never has two children of one parent (plain adds, fixed
-  // order).  ext_lvl_ptr[l] .. ext_lvl_ptr[l+1] index groups; ext_grp_ptr into ext_nodes.
-  std::vector<int32_t> ext_lvl_ptr, ext_grp_ptr, ext_nodes;
   // forward-solve update vectors: node x owns 6 * sn_nf[x] entries at sn_uoff[x]
   std::vector<int64_t> sn_uoff;
   int64_t u_elems = 0;
@@ -142,8 +148,8 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
 // multifrontal factorization.
 constexpr int32_t kPanelBlocks = 8;
 constexpr int32_t kTile = 64;
-// rows per panel-row workgroup; a panel step with at least kSplitTiles tiles is launched
-// split (panel rows, then the trailing update) instead of fused
+// rows per panel-row workgroup; a round with at least kSplitTiles tiles is launched split
+// (panel rows, then the trailing updates) instead of fused
 constexpr int32_t kPanelRows = 64;
 constexpr int32_t kSplitTiles = 640;
 // backward-solve column tile of the rows below a node

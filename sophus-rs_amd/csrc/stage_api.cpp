@@ -199,7 +199,6 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.panel_items = bufs.up(S.panel_items);
     df.sn_woff = bufs.up(S.sn_woff);
     df.solve_cols = bufs.up(S.solve_cols);
-    df.ext_nodes = bufs.up(S.ext_nodes);
     df.sn_uoff = bufs.up(S.sn_uoff);
     df.usolve = bufs.alloc<double>(size_t(S.u_elems));
     df.front = bufs.alloc<double>(size_t(S.front_elems));
