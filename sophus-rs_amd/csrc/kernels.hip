@@ -964,11 +964,11 @@ __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double
   }
 }
 
-// First panel diagonal of each item's node (items as in k_mf_step, all first-diagonal
-// items: the leaves, whose fronts are assembled before the factorization starts).
+// First panel diagonal of the leaves (whose fronts are assembled before the factorization
+// starts; every other node's is factored by the last step of its last child).
 __global__ void __launch_bounds__(kThreads)
-    k_mf_diag0(DevFactor f, const int32_t* __restrict__ items, DevScalars* sc, double* __restrict__ z) {
-  const int X = items[4 * blockIdx.x];
+    k_mf_diag0(DevFactor f, const int32_t* __restrict__ nodes, DevScalars* sc, double* __restrict__ z) {
+  const int X = nodes[blockIdx.x];
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   __shared__ double D[kPW][kPW + 1], Wd[kPW][kPW + 1], sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   factor_panel_diag(f, diag_node(f, X), f.front + f.sn_off[X], F, 0, min(K, kPW), sc, D, Wd, sL,
@@ -1027,9 +1027,8 @@ __device__ __forceinline__ void extend_tile(const DevFactor& f, int X, int P, in
       }
 }
 
-// One round of panel steps, one workgroup per item (node, panel, tr, tc): a 64x64 tile
-// (tr, tc) of the node's lower trailing region below the panel, or (tr = -1) the node's
-// first diagonal block, factored once every child has added its update matrix.  A tile:
+// One round of panel steps, one workgroup per item (node, panel, tr, tc): the 64x64 tile
+// (tr, tc) of the node's lower trailing region below the panel:
 //   1. the tile's values, the panel columns of its row block tr and column block tc (C),
 //      L_PP^-T and the pivot inverses M_j are loaded (every global load first);
 //   2. W = C L_PP^-T for both blocks and L = W blockdiag(M_j) for block tr on the matrix
@@ -1047,7 +1046,8 @@ __device__ __forceinline__ void extend_tile(const DevFactor& f, int X, int P, in
 //      adds its results into the parent's front instead (extend-add; child update row
 //      block b is the parent's row block upos[b]), and diagonal tiles add their finished
 //      update rows of the right-hand side into the parent's.  No two children of one
-//      parent end in one round and a child's targets are distinct, so the adds are plain.
+//      parent end in one round and a child's targets are distinct, so the adds are plain;
+//      the last child's tile (0, 0) then factors the parent's first diagonal block.
 // Each tile forms the L and W rows it needs itself (a row block is formed again by every
 // tile of its tile row and column), so a panel step is one launch with no panel-row pass
 // before it.
@@ -1059,7 +1059,6 @@ __global__ void __launch_bounds__(kThreads)
   const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
   const int X = tiles[4 * bid], panel = tiles[4 * bid + 1];
   const int tr = tiles[4 * bid + 2], tc = tiles[4 * bid + 3];
-  const bool first = tr < 0;
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   double* A = f.front + f.sn_off[X];
   const DiagNode dn = diag_node(f, X);
@@ -1081,51 +1080,128 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double sb[kPW], sy[kThreads / 64][kPW];
   // the node's last panel step: its update matrix goes to the parent (-1: none)
   const int P = kend == K ? f.sn_parent[X] : -1;
-  bool diag = first;  // this workgroup factors a diagonal block (after its tile, if any)
-  if (!first) {
-    {
-      const int gb0 = dn.c0 + k0 / 6, nbp = np / 6;
-      constexpr int kNC = kPW * kTile / kThreads, kNT = kPW * kPW / kThreads;
-      constexpr int kNM = (kPanelBlocks * 36 + kThreads - 1) / kThreads;
-      static_assert(kPW * kTile % kThreads == 0 && kPW * kPW % kThreads == 0, "whole rounds");
-      double va[kNC], vb[kNC], vt[kNT], vm[kNM];
+  // the last child of its parent (in round order) factors the parent's first diagonal
+  // block after its last step's tile (0, 0): update rows keep their order in the parent, so
+  // that block gets the children's update only from their tiles (0, 0), and the other
+  // children ended in earlier rounds
+  const bool hand = P >= 0 && tr == 0 && tc == 0 && f.sn_hand[X];
+  bool diag = false;  // this workgroup factors the node's next diagonal block
+  {
+    const int gb0 = dn.c0 + k0 / 6, nbp = np / 6;
+    constexpr int kNC = kPW * kTile / kThreads, kNT = kPW * kPW / kThreads;
+    constexpr int kNM = (kPanelBlocks * 36 + kThreads - 1) / kThreads;
+    static_assert(kPW * kTile % kThreads == 0 && kPW * kPW % kThreads == 0, "whole rounds");
+    double va[kNC], vb[kNC], vt[kNT], vm[kNM];
 #pragma unroll
-      for (int it = 0; it < kNC; ++it) {
-        const int e = tid + kThreads * it, t = e / kTile, i = e % kTile;
-        va[it] = load_or_zero(A, int64_t(k0 + t) * F + r0 + i, t < np && r0 + i < F);
-        vb[it] = load_or_zero(A, int64_t(k0 + t) * F + c0 + i, !dg && t < np && c0 + i < F);
-      }
-#pragma unroll
-      for (int it = 0; it < kNT; ++it) vt[it] = Tg[tid + kThreads * it];
-#pragma unroll
-      for (int it = 0; it < kNM; ++it) {
-        const int e = tid + kThreads * it;
-        vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
-      }
-      const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
-#pragma unroll
-      for (int it = 0; it < kNC; ++it) {
-        const int e = tid + kThreads * it;
-        sA[e / kTile][e % kTile] = va[it];
-        sB[e / kTile][e % kTile] = vb[it];
-      }
-#pragma unroll
-      for (int it = 0; it < kNT; ++it) {
-        const int e = tid + kThreads * it;
-        sT[e / kPW][e % kPW] = vt[it];
-      }
-#pragma unroll
-      for (int it = 0; it < kNM; ++it) {
-        const int e = tid + kThreads * it;
-        if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
-      }
-      if (tid < kPW) sb[tid] = vbb;
+    for (int it = 0; it < kNC; ++it) {
+      const int e = tid + kThreads * it, t = e / kTile, i = e % kTile;
+      va[it] = load_or_zero(A, int64_t(k0 + t) * F + r0 + i, t < np && r0 + i < F);
+      vb[it] = load_or_zero(A, int64_t(k0 + t) * F + c0 + i, !dg && t < np && c0 + i < F);
     }
-    __syncthreads();
-    // accumulator (mi, nj) component q is the tile element
-    //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
-    // it starts from the element's current value, loaded while W and L are formed
-    mfma_d4 acc[2][2];
+#pragma unroll
+    for (int it = 0; it < kNT; ++it) vt[it] = Tg[tid + kThreads * it];
+#pragma unroll
+    for (int it = 0; it < kNM; ++it) {
+      const int e = tid + kThreads * it;
+      vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
+    }
+    const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
+#pragma unroll
+    for (int it = 0; it < kNC; ++it) {
+      const int e = tid + kThreads * it;
+      sA[e / kTile][e % kTile] = va[it];
+      sB[e / kTile][e % kTile] = vb[it];
+    }
+#pragma unroll
+    for (int it = 0; it < kNT; ++it) {
+      const int e = tid + kThreads * it;
+      sT[e / kPW][e % kPW] = vt[it];
+    }
+#pragma unroll
+    for (int it = 0; it < kNM; ++it) {
+      const int e = tid + kThreads * it;
+      if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
+    }
+    if (tid < kPW) sb[tid] = vbb;
+  }
+  __syncthreads();
+  // accumulator (mi, nj) component q is the tile element
+  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+  // it starts from the element's current value, loaded while W and L are formed
+  mfma_d4 acc[2][2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+      }
+  // W = C L_PP^-T for the wavefront's slice of both row blocks; L_PP^-T is block upper
+  // triangular, so column tile ct stops at row kW[ct] (C and L_PP^-T are zero past np)
+  constexpr int kW[3] = {18, 36, 48};
+  mfma_d4 wa[3], wb[3];
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct) wa[ct] = wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < kPW; k4 += 4) {
+    const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+      if (k4 < kW[ct]) {
+        const double tv = sT[k4 + lc][16 * ct + lr];
+        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
+        wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
+      }
+  }
+  __syncthreads();  // every wavefront is done with C
+  // W^T of both blocks (a diagonal tile's column block is its row block)
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
+      sB[16 * ct + lr][rw + lc + 4 * q] = dg ? wa[ct][q] : wb[ct][q];
+    }
+  // L = W blockdiag(M_j) for the slice of row block tr: column tile ct meets the pivot
+  // blocks with rows [kL0[ct], kL1[ct]); the block-diagonal operand is read from the M_j
+  constexpr int kL0[3] = {0, 12, 30}, kL1[3] = {18, 36, 48};
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct) wa[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k4 = 0; k4 < kPW; k4 += 4) {
+    const int k = k4 + lc;
+    const double av = sA[k][rw + lr];
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct)
+      if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
+        const int n = 16 * ct + lr;
+        const double mv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
+        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wa[ct], 0, 0, 0);
+      }
+  }
+  // L^T replaces the slice's W^T in sA
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
+  __syncthreads();  // the update reads every row of L^T and W^T
+#pragma unroll
+  for (int s = 0; s < kPW / 4; ++s) {
+    const int t = 4 * s + lc;  // this lane's k index
+    double a[2], b[2];
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
+  }
+  if (P < 0) {
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -1133,151 +1209,81 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-          acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+          if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
         }
-    // W = C L_PP^-T for the wavefront's slice of both row blocks; L_PP^-T is block upper
-    // triangular, so column tile ct stops at row kW[ct] (C and L_PP^-T are zero past np)
-    constexpr int kW[3] = {18, 36, 48};
-    mfma_d4 wa[3], wb[3];
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct) wa[ct] = wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k4 = 0; k4 < kPW; k4 += 4) {
-      const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
-#pragma unroll
-      for (int ct = 0; ct < 3; ++ct)
-        if (k4 < kW[ct]) {
-          const double tv = sT[k4 + lc][16 * ct + lr];
-          wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
-          wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
-        }
+  } else {
+    extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
+  }
+  if (dg) {
+    // the slice's L rows to the factor (and the row-major copy of the node's own rows)
+    double* Lo = f.lbuf + dn.loff;
+    double* Lt = f.ltbuf + dn.toff;
+    for (int e = lane; e < 16 * kPW; e += 64) {
+      const int i = rw + e % 16, c = e / 16, r = r0 + i;
+      if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = sA[c][i];
     }
-    __syncthreads();  // every wavefront is done with C
-    // W^T of both blocks (a diagonal tile's column block is its row block)
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
-        sB[16 * ct + lr][rw + lc + 4 * q] = dg ? wa[ct][q] : wb[ct][q];
-      }
-    // L = W blockdiag(M_j) for the slice of row block tr: column tile ct meets the pivot
-    // blocks with rows [kL0[ct], kL1[ct]); the block-diagonal operand is read from the M_j
-    constexpr int kL0[3] = {0, 12, 30}, kL1[3] = {18, 36, 48};
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct) wa[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k4 = 0; k4 < kPW; k4 += 4) {
-      const int k = k4 + lc;
-      const double av = sA[k][rw + lr];
-#pragma unroll
-      for (int ct = 0; ct < 3; ++ct)
-        if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
-          const int n = 16 * ct + lr;
-          const double mv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
-          wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wa[ct], 0, 0, 0);
-        }
-    }
-    // L^T replaces the slice's W^T in sA
-#pragma unroll
-    for (int ct = 0; ct < 3; ++ct)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) sA[16 * ct + lr][rw + lc + 4 * q] = wa[ct][q];
-    __syncthreads();  // the update reads every row of L^T and W^T
-#pragma unroll
-    for (int s = 0; s < kPW / 4; ++s) {
-      const int t = 4 * s + lc;  // this lane's k index
-      double a[2], b[2];
-#pragma unroll
-      for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj)
-          acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
-    }
-    if (P < 0) {
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-            if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
-          }
-    } else {
-      extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
-    }
-    if (dg) {
-      // the slice's L rows to the factor (and the row-major copy of the node's own rows)
-      double* Lo = f.lbuf + dn.loff;
-      double* Lt = f.ltbuf + dn.toff;
+    if (r0 + rw < K)
       for (int e = lane; e < 16 * kPW; e += 64) {
-        const int i = rw + e % 16, c = e / 16, r = r0 + i;
-        if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = sA[c][i];
+        const int c = e % kPW, i = rw + e / kPW, r = r0 + i;
+        if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sA[c][i];
       }
-      if (r0 + rw < K)
-        for (int e = lane; e < 16 * kPW; e += 64) {
-          const int c = e % kPW, i = rw + e / kPW, r = r0 + i;
-          if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sA[c][i];
-        }
-      // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront its
-      // own copy, lane c < kPW entry c), then b_r -= L_r y_P for the slice's rows (four
-      // lanes per row, every fourth column, two xor shuffles)
-      if (lane < kPW) {
-        double a4[4] = {0.0, 0.0, 0.0, 0.0};
+    // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront its
+    // own copy, lane c < kPW entry c), then b_r -= L_r y_P for the slice's rows (four
+    // lanes per row, every fourth column, two xor shuffles)
+    if (lane < kPW) {
+      double a4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
-        sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
-      }
-      {
-        const int i = rw + (lane >> 2), part = lane & 3, r = r0 + i;
-        double s4 = 0.0;
+      for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
+      sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+    }
+    {
+      const int i = rw + (lane >> 2), part = lane & 3, r = r0 + i;
+      double s4 = 0.0;
 #pragma unroll
-        for (int u = 0; u < kPW / 4; ++u) s4 += sA[part + 4 * u][i] * sy[wv][part + 4 * u];
-        s4 += __shfl_xor(s4, 1, 64);
-        s4 += __shfl_xor(s4, 2, 64);
-        if (part == 0 && r < F) {
-          if (P < 0)
-            bx[r] -= s4;
-          else  // a finished update row: into the parent's right-hand side
-            f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += bx[r] - s4;
-        }
-      }
-      if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
-        const int jb = lane / 6, a = lane % 6;
-        double zz = 0.0;
-#pragma unroll
-        for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[0][6 * jb + t];
-        z[6 * int64_t(dn.c0) + k0 + lane] = zz;
+      for (int u = 0; u < kPW / 4; ++u) s4 += sA[part + 4 * u][i] * sy[wv][part + 4 * u];
+      s4 += __shfl_xor(s4, 1, 64);
+      s4 += __shfl_xor(s4, 2, 64);
+      if (part == 0 && r < F) {
+        if (P < 0)
+          bx[r] -= s4;
+        else  // a finished update row: into the parent's right-hand side
+          f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += bx[r] - s4;
       }
     }
-    if (tr == 0 && tc == 0 && kend < K) {
-      // the next panel's diagonal block is the top-left of this tile: hand the updated
-      // values to LDS directly
-      const int npn = min(K, kend + kPW) - kend;
-      double (*D)[kPW + 1] = as_square(sA);
-      __syncthreads();  // every read of sA / sB is done
+    if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
+      const int jb = lane / 6, a = lane % 6;
+      double zz = 0.0;
 #pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
-            if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
-          }
-      __syncthreads();
-      diag = true;
+      for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[0][6 * jb + t];
+      z[6 * int64_t(dn.c0) + k0 + lane] = zz;
     }
   }
+  if (tr == 0 && tc == 0 && kend < K) {
+    // the next panel's diagonal block is the top-left of this tile: hand the updated
+    // values to LDS directly
+    const int npn = min(K, kend + kPW) - kend;
+    double (*D)[kPW + 1] = as_square(sA);
+    __syncthreads();  // every read of sA / sB is done
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
+          if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+        }
+    __syncthreads();
+    diag = true;
+  }
   if (diag) {
-    const int dk0 = first ? 0 : kend;
-    factor_panel_diag(f, dn, A, F, dk0, min(K, dk0 + kPW) - dk0, sc, as_square(sA),
-                      as_square(sB), sL, sdi, !first, z);
+    factor_panel_diag(f, dn, A, F, kend, min(K, kend + kPW) - kend, sc, as_square(sA),
+                      as_square(sB), sL, sdi, true, z);
+  } else if (hand) {
+    __syncthreads();  // the tile's adds into the parent's front are done
+    const int KP = 6 * f.sn_nc[P];
+    factor_panel_diag(f, diag_node(f, P), f.front + f.sn_off[P], 6 * f.sn_nf[P], 0, min(KP, kPW),
+                      sc, as_square(sA), as_square(sB), sL, sdi, false, z);
   }
 }
 
@@ -1438,15 +1444,16 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
-// Trailing updates of one round of split panel steps, one workgroup per item as in
-// k_mf_step (a 64x64 tile of a node's lower trailing region, or a first diagonal block):
-// A[r, c] -= L_r W_c^T on the matrix cores.  The tile's L rows and W columns are staged in
+// Trailing updates of one round of split panel steps, one workgroup per 64x64 tile of a
+// node's lower trailing region (items as in k_mf_step): A[r, c] -= L_r W_c^T on the
+// matrix cores.  The tile's L rows and W columns are staged in
 // LDS (zero past the panel's width); each wavefront owns a 32x32 quadrant as 2 x 2 tiles
 // of v_mfma_f64_16x16x4, computing (-W) L^T so that each lane's results run down 16
 // consecutive rows of the column-major front (coalesced read-modify-write).  The
 // accumulators start from the tile's current values.  On the node's last step the results
 // go into the parent's front (extend-add).  The tile (0, 0), which holds the next panel's
-// diagonal block, then factors that block.
+// diagonal block, then factors that block (on the last child's last step: the parent's
+// first one).
 __global__ void __launch_bounds__(kThreads)
     k_mf_update(DevFactor f, const int32_t* __restrict__ tiles, DevScalars* sc,
                 double* __restrict__ z) {
@@ -1455,7 +1462,6 @@ __global__ void __launch_bounds__(kThreads)
   const int bid = xcd_contiguous(blockIdx.x, gridDim.x);
   const int X = tiles[4 * bid], panel = tiles[4 * bid + 1];
   const int tr = tiles[4 * bid + 2], tc = tiles[4 * bid + 3];
-  const bool first = tr < 0;
   const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X];
   double* A = f.front + f.sn_off[X];
   const double* Lo = f.lbuf + f.sn_loff[X];
@@ -1472,12 +1478,54 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   // the node's last panel step: its update matrix goes to the parent (-1: none)
   const int P = kend == K ? f.sn_parent[X] : -1;
-  bool diag = first;  // this workgroup factors a diagonal block (after its tile, if any)
-  if (!first) {
-    // accumulator (mi, nj) component q is the tile element
-    //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
-    // it starts from the element's current value (loaded first)
-    mfma_d4 acc[2][2];
+  // the last child of its parent (in round order) factors the parent's first diagonal
+  // block after its last step's tile (0, 0): update rows keep their order in the parent, so
+  // that block gets the children's update only from their tiles (0, 0), and the other
+  // children ended in earlier rounds
+  const bool hand = P >= 0 && tr == 0 && tc == 0 && f.sn_hand[X];
+  bool diag = false;  // this workgroup factors the node's next diagonal block
+  // accumulator (mi, nj) component q is the tile element
+  //   row rb + 16 mi + lr, column cb + 16 nj + lc + 4 q;
+  // it starts from the element's current value (loaded first)
+  mfma_d4 acc[2][2];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
+        acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+      }
+  constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
+  stage_lds2<kStage>(
+      kPW * kTile,
+      [&](int e, bool ok) {
+        const int t = e / kTile, r = r0 + e % kTile;
+        return load_or_zero(Lo, int64_t(k0 + t) * F + r, ok && t < np && r < F);
+      },
+      [&](int e, double v) { sA[e / kTile][e % kTile] = v; },
+      [&](int e, bool ok) {
+        const int t = e / kTile, c = c0 + e % kTile;
+        return load_or_zero(Wo, int64_t(t) * F + c, ok && t < np && c < F);
+      },
+      [&](int e, double v) { sB[e / kTile][e % kTile] = v; });
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < kPW / 4; ++s) {
+    const int t = 4 * s + lc;  // this lane's k index
+    double a[2], b[2];
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+        acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
+  }
+  if (P < 0) {
 #pragma unroll
     for (int mi = 0; mi < 2; ++mi)
 #pragma unroll
@@ -1485,72 +1533,37 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-          acc[mi][nj][q] = load_or_zero(A, int64_t(c) * F + r, r < F && c < F && r >= c);
+          if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
         }
-    constexpr int kStage = (kPW * kTile + kThreads - 1) / kThreads;
-    stage_lds2<kStage>(
-        kPW * kTile,
-        [&](int e, bool ok) {
-          const int t = e / kTile, r = r0 + e % kTile;
-          return load_or_zero(Lo, int64_t(k0 + t) * F + r, ok && t < np && r < F);
-        },
-        [&](int e, double v) { sA[e / kTile][e % kTile] = v; },
-        [&](int e, bool ok) {
-          const int t = e / kTile, c = c0 + e % kTile;
-          return load_or_zero(Wo, int64_t(t) * F + c, ok && t < np && c < F);
-        },
-        [&](int e, double v) { sB[e / kTile][e % kTile] = v; });
+  } else {
+    extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
+  }
+  if (tr == 0 && tc == 0 && kend < K) {
+    // the next panel's diagonal block is the top-left of this tile: hand the updated
+    // values to LDS directly
+    const int npn = min(K, kend + kPW) - kend;
+    double (*D)[kPW + 1] = as_square(sA);
+    __syncthreads();  // every MFMA operand read of sA / sB is done
+#pragma unroll
+    for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+      for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
+          if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
+        }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kPW / 4; ++s) {
-      const int t = 4 * s + lc;  // this lane's k index
-      double a[2], b[2];
-#pragma unroll
-      for (int nj = 0; nj < 2; ++nj) a[nj] = -sB[t][cb + 16 * nj + lr];  // A operand: -W (16 x 4)
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi) b[mi] = sA[t][rb + 16 * mi + lr];   // B operand: L^T (4 x 16)
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj)
-          acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[nj], b[mi], acc[mi][nj], 0, 0, 0);
-    }
-    if (P < 0) {
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
-            if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
-          }
-    } else {
-      extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
-    }
-    if (tr == 0 && tc == 0 && kend < K) {
-      // the next panel's diagonal block is the top-left of this tile: hand the updated
-      // values to LDS directly
-      const int npn = min(K, kend + kPW) - kend;
-      double (*D)[kPW + 1] = as_square(sA);
-      __syncthreads();  // every MFMA operand read of sA / sB is done
-#pragma unroll
-      for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-        for (int nj = 0; nj < 2; ++nj)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int rl = rb + 16 * mi + lr, cl = cb + 16 * nj + lc + 4 * q;
-            if (rl < npn && cl < npn) D[rl][cl] = rl >= cl ? acc[mi][nj][q] : 0.0;
-          }
-      __syncthreads();
-      diag = true;
-    }
+    diag = true;
   }
   if (diag) {
-    const int dk0 = first ? 0 : kend;
-    factor_panel_diag(f, dn, A, F, dk0, min(K, dk0 + kPW) - dk0, sc, as_square(sA),
-                      as_square(sB), sL, sdi, !first, z);
+    factor_panel_diag(f, dn, A, F, kend, min(K, kend + kPW) - kend, sc, as_square(sA),
+                      as_square(sB), sL, sdi, true, z);
+  } else if (hand) {
+    __syncthreads();  // the tile's adds into the parent's front are done
+    const int KP = 6 * f.sn_nc[P];
+    factor_panel_diag(f, diag_node(f, P), f.front + f.sn_off[P], 6 * f.sn_nf[P], 0, min(KP, kPW),
+                      sc, as_square(sA), as_square(sB), sL, sdi, false, z);
   }
 }
 
@@ -2196,14 +2209,13 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const 
   const int g = blocks_for(6 * int64_t(f.nb), kThreads);
   hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, rhs_nat, y);
   hipLaunchKernelGGL(k_mf_rhs_init, dim3(S.nsn), dim3(kThreads), 0, s, f, y);
+  // the leaves' first diagonal blocks; every other node's is factored in a round
+  if (!S.leaves.empty())
+    hipLaunchKernelGGL(k_mf_diag0, dim3(S.leaves.size()), dim3(kThreads), 0, s, f, f.leaves, sc, y);
   for (size_t li = 0; li < S.launches.size(); ++li) {
     const auto& L = S.launches[li];
+    if (L.tile_count == 0) continue;
     const int32_t* items = f.launch_tiles + 4 * int64_t(L.tile_begin);
-    if (L.tile_count == L.first_count) {  // first diagonal blocks only
-      if (L.tile_count > 0)
-        hipLaunchKernelGGL(k_mf_diag0, dim3(L.tile_count), dim3(kThreads), 0, s, f, items, sc, y);
-      continue;
-    }
     if (f.step_events) SGPU_HIP(hipEventRecord(f.step_events[2 * li], s));
     if (L.fused) {
       hipLaunchKernelGGL(k_mf_step, dim3(L.tile_count), dim3(kThreads), 0, s, f, items, sc, y);

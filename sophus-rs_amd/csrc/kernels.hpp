@@ -71,6 +71,8 @@ struct DevFactor {
   const int32_t* sn_upos;
   const int32_t* lvl_sn;
   const int32_t* launch_tiles;  // round work items (node, panel, tile_row, tile_col)
+  const int32_t* leaves;        // nodes without children
+  const int32_t* sn_hand;       // 1: the node's last step factors its parent's first block
   const int32_t* solve_cols;  // (node, 64-column tile) per level
   const int64_t* sn_uoff;     // right-hand side offsets (F entries per node)
   double* usolve;             // each node's right-hand side in front row order (u_elems)

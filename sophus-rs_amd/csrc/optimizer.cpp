@@ -119,7 +119,7 @@ struct sgpu_optimizer {
     if (!timing) return;
     if (timing == 2 && used[K_FACTOR] && df.step_events) {
       for (size_t li = 0; li < S.launches.size(); ++li) {
-        if (S.launches[li].tile_count == S.launches[li].first_count) continue;  // no step
+        if (S.launches[li].tile_count == 0) continue;
         float ms = 0;
         SGPU_HIP(hipEventElapsedTime(&ms, step_ev[2 * li], step_ev[2 * li + 1]));
         kernel_ms[K_MF_STEP] += ms;
@@ -285,6 +285,8 @@ struct sgpu_optimizer {
     df.sn_upos = arena.upload(S.sn_upos, stream);
     df.lvl_sn = arena.upload(S.lvl_sn, stream);
     df.launch_tiles = arena.upload(S.launch_tiles, stream);
+    df.leaves = arena.upload(S.leaves, stream);
+    df.sn_hand = arena.upload(S.sn_hand, stream);
     df.sn_toff = arena.upload(S.sn_toff, stream);
     df.sn_poff = arena.upload(S.sn_poff, stream);
     df.panel_items = arena.upload(S.panel_items, stream);
@@ -634,7 +636,7 @@ static void fill_stats(const HostProblem& P, const Symbolic& S, sgpu_rcs_stats* 
   for (const auto& L : S.launches) {
     st->mf_step_flops += L.flops;
     st->mf_step_bytes += L.bytes;
-    st->mf_step_launches += L.tile_count > L.first_count ? 1 : 0;
+    st->mf_step_launches += L.tile_count > 0 ? 1 : 0;
   }
 }
 

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <limits>
 #include <numeric>
 #include <queue>
 
@@ -605,63 +606,86 @@ void analyze_rcs(Symbolic& S) {
     S.sn_toff[x] = S.t_elems;
     S.t_elems += int64_t(36) * S.sn_nc[x] * S.sn_nc[x];
   }
-  // round schedule: a node's first diagonal block in the round after its children's last
-  // steps (round 0 for leaves), then one panel step per round; children of one parent end
-  // in distinct rounds (the earliest possible, in the order they could end)
+  // round schedule (list scheduling, every step as early as it can run): a node's panel
+  // steps run one per round, a leaf's from round 0, a parent's from the round after its
+  // last child's last step (which also factors the parent's first diagonal block);
+  // children of one parent end in distinct rounds, taken in order of the rounds left on
+  // their path to the root (which is also the order of the work within a round).  Holding
+  // back steps off the critical path to even out the rounds gained nothing here: the
+  // rounds with the most tiles hold the large separators, which are all near-critical.
   {
     auto npanels = [&](int32_t x) { return (S.sn_nc[x] + kPanelBlocks - 1) / kPanelBlocks; };
     // panel steps: every panel with rows below it (all but a root's last)
     auto nsteps = [&](int32_t x) { return npanels(x) - (S.sn_nf[x] > S.sn_nc[x] ? 0 : 1); };
-    std::vector<std::vector<int32_t>> kids(S.nsn);
+    std::vector<int32_t> pending(S.nsn, 0);
     for (int32_t x = 0; x < S.nsn; ++x)
-      if (S.sn_parent[x] >= 0) kids[S.sn_parent[x]].push_back(x);
-    std::vector<int32_t> first(S.nsn, 0);
-    S.sn_last_round.assign(S.nsn, -1);
-    for (int32_t x = 0; x < S.nsn; ++x) {  // children have smaller indices than parents
-      std::vector<std::pair<int32_t, int32_t>> want;
-      for (const int32_t c : kids[x]) {
-        if (nsteps(c) == 0) throw std::runtime_error("supernode with a parent but no update rows");
-        want.push_back({first[c] + nsteps(c), c});
+      if (S.sn_parent[x] >= 0) {
+        if (nsteps(x) == 0) throw std::runtime_error("supernode with a parent but no update rows");
+        ++pending[S.sn_parent[x]];
       }
-      std::sort(want.begin(), want.end());
-      int32_t r = -1;
-      for (const auto& [w, c] : want) S.sn_last_round[c] = r = std::max(w, r + 1);
-      first[x] = r + 1;
-      if (S.sn_parent[x] < 0 && nsteps(x) > 0) S.sn_last_round[x] = first[x] + nsteps(x);
-    }
-    // priority within a round: the rounds from a node's first to its root's end
+    // rounds from a node's first step to its root's last
     std::vector<int32_t> prio(S.nsn, 0);
-    for (int32_t x = S.nsn - 1; x >= 0; --x)
-      prio[x] = nsteps(x) + 1 + (S.sn_parent[x] >= 0 ? prio[S.sn_parent[x]] : 0);
-    int32_t nround = 0;
-    for (int32_t x = 0; x < S.nsn; ++x) nround = std::max({nround, first[x] + 1, S.sn_last_round[x] + 1});
-    std::vector<std::vector<std::pair<int32_t, int32_t>>> steps(nround);  // (node, panel)
-    std::vector<std::vector<int32_t>> firsts(nround);
+    for (int32_t x = S.nsn - 1; x >= 0; --x)  // parents have larger indices
+      prio[x] = nsteps(x) + (S.sn_parent[x] >= 0 ? prio[S.sn_parent[x]] : 0);
+    std::vector<int32_t> next(S.nsn, 0), ready(S.nsn, std::numeric_limits<int32_t>::max());
+    std::vector<int32_t> sibling_end(S.nsn, -1);  // per parent: the last round a child ended in
+    S.sn_last_round.assign(S.nsn, -1);
+    S.sn_hand.assign(S.nsn, 0);
+    S.leaves.clear();
+    int64_t remaining = 0;
     for (int32_t x = 0; x < S.nsn; ++x) {
-      firsts[first[x]].push_back(x);
-      const int32_t ns = nsteps(x);
-      for (int32_t p = 0; p < ns; ++p)
-        steps[p == ns - 1 ? S.sn_last_round[x] : first[x] + 1 + p].push_back({x, p});
+      if (pending[x] == 0) {
+        S.leaves.push_back(x);
+        ready[x] = 0;
+      }
+      remaining += nsteps(x);
     }
-    auto by_prio = [&](int32_t a, int32_t b) { return prio[a] != prio[b] ? prio[a] > prio[b] : a < b; };
+    auto rem = [&](int32_t x) {  // rounds left on x's path, this step included
+      return nsteps(x) - next[x] + (S.sn_parent[x] >= 0 ? prio[S.sn_parent[x]] : 0);
+    };
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> steps;  // (node, panel) per round
+    std::vector<int32_t> cand;
+    for (int32_t r = 0; remaining > 0; ++r) {
+      cand.clear();
+      for (int32_t x = 0; x < S.nsn; ++x)
+        if (ready[x] <= r && next[x] < nsteps(x)) cand.push_back(x);
+      std::sort(cand.begin(), cand.end(),
+                [&](int32_t a, int32_t b) { return rem(a) != rem(b) ? rem(a) > rem(b) : a < b; });
+      auto& st = steps.emplace_back();
+      for (const int32_t x : cand) {
+        const int32_t p = next[x], P = S.sn_parent[x];
+        const bool last = p + 1 == nsteps(x);
+        if (last && P >= 0 && sibling_end[P] == r) continue;  // a sibling ends in this round
+        st.push_back({x, p});
+        ++next[x];
+        ready[x] = r + 1;
+        --remaining;
+        if (last) {
+          S.sn_last_round[x] = r;
+          if (P >= 0) {
+            sibling_end[P] = r;
+            if (--pending[P] == 0) {
+              ready[P] = r + 1;
+              S.sn_hand[x] = 1;
+            }
+          }
+        }
+      }
+    }
+    const int32_t nround = int32_t(steps.size());
     for (int32_t r = 0; r < nround; ++r) {
-      auto& st = steps[r];
-      std::sort(st.begin(), st.end(), [&](const auto& a, const auto& b) { return by_prio(a.first, b.first); });
-      std::sort(firsts[r].begin(), firsts[r].end(), by_prio);
-      Symbolic::PanelLaunch L{int32_t(S.launch_tiles.size() / 4), 0, 0,
+      const auto& st = steps[r];  // in order of priority
+      Symbolic::PanelLaunch L{int32_t(S.launch_tiles.size() / 4), 0,
                               int32_t(S.panel_items.size() / 3), 0, true, 0.0, 0.0};
       auto push = [&](int32_t x, int32_t p, int32_t tr, int32_t tc) {
         S.launch_tiles.insert(S.launch_tiles.end(), {x, p, tr, tc});
         ++L.tile_count;
       };
-      // the critical work first: tile (0, 0) of every step that factors a next diagonal
-      // block, then the first diagonal blocks, then the other tiles
+      // the critical work first: tile (0, 0) of every step that factors a diagonal block
+      // (the node's next, or its parent's first), then the other tiles
+      auto chain = [&](int32_t x, int32_t p) { return p + 1 < npanels(x) || S.sn_hand[x]; };
       for (const auto& [x, p] : st)
-        if (p + 1 < npanels(x)) push(x, p, 0, 0);
-      for (const int32_t x : firsts[r]) {
-        push(x, 0, -1, 0);
-        ++L.first_count;
-      }
+        if (chain(x, p)) push(x, p, 0, 0);
       for (const auto& [x, p] : st) {
         const int32_t kend = 6 * std::min(S.sn_nc[x], (p + 1) * kPanelBlocks);
         const int32_t T = 6 * S.sn_nf[x] - kend;
@@ -674,10 +698,10 @@ void analyze_rcs(Symbolic& S) {
         for (int32_t ch = 0; ch * kPanelRows < T; ++ch) S.panel_items.insert(S.panel_items.end(), {x, p, ch});
         for (int32_t tr = 0; tr < nt; ++tr)
           for (int32_t tc = 0; tc <= tr; ++tc)
-            if (tr > 0 || p + 1 >= npanels(x)) push(x, p, tr, tc);
+            if (tr > 0 || !chain(x, p)) push(x, p, tr, tc);
       }
       L.item_count = int32_t(S.panel_items.size() / 3) - L.item_begin;
-      L.fused = L.tile_count - L.first_count < kSplitTiles;
+      L.fused = L.tile_count < kSplitTiles;
       if (L.fused) {  // the fused launch forms its panel rows itself
         S.panel_items.resize(size_t(3) * L.item_begin);
         L.item_count = 0;

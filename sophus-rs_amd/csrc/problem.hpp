@@ -92,26 +92,29 @@ struct Symbolic {
   int32_t max_front = 0;                      // largest front (blocks)
   std::vector<int32_t> lvl_ptr, lvl_sn;
   // Schedule of the numeric factorization in rounds, one launch (or a panel-row launch and
-  // a trailing-update launch) per round.  A node's first diagonal block is factored in a
-  // round after every child's last step; its panel steps then run in the rounds that
-  // follow, one per round, so nodes of different tree levels share a launch and the
-  // number of rounds follows the elimination tree's critical path.  A child's last step
-  // adds its update matrix (and its update rows of the right-hand side) into the parent's
-  // front; children of one parent finish in distinct rounds, so those adds are plain and
-  // their order is fixed.
+  // a trailing-update launch) per round, each holding one panel step of every node that
+  // is ready, so nodes of different tree levels share a launch and the number of rounds
+  // follows the elimination tree's critical path.  The leaves' first diagonal blocks are
+  // factored before the first round.  A child's last step adds its update matrix (and its
+  // update rows of the right-hand side) into the parent's front; children of one parent
+  // end in distinct rounds, so those adds are plain and their order is fixed.  The last
+  // of them also factors the parent's first diagonal block (in its tile (0, 0), the only
+  // tile whose update reaches that block), and the parent's steps follow from the next
+  // round.
   struct PanelLaunch {
-    int32_t tile_begin, tile_count;  // work items in launch_tiles
-    int32_t first_count;             // of which first-diagonal items (tile_row -1)
+    int32_t tile_begin, tile_count;  // tiles in launch_tiles
     int32_t item_begin, item_count;  // split rounds: panel-row items
     bool fused;
     double flops;  // algorithmic FP64 flops of the trailing updates
     double bytes;  // algorithmic HBM bytes: lower trailing RMW, panel columns read, L written
   };
   std::vector<PanelLaunch> launches;
-  // 4 per work item: (node, panel, tile_row, tile_col) for a 64x64 tile of the lower
-  // trailing region below the node's panel; (node, 0, -1, 0) for a first diagonal block
+  // 4 per tile: (node, panel, tile_row, tile_col), a 64x64 tile of the lower trailing
+  // region below the node's panel
   std::vector<int32_t> launch_tiles;
-  std::vector<int32_t> panel_items;   // (node, panel, kPanelRows-row chunk) (split rounds)
+  std::vector<int32_t> panel_items;    // (node, panel, kPanelRows-row chunk) (split rounds)
+  std::vector<int32_t> leaves;         // nodes without children
+  std::vector<int32_t> sn_hand;        // 1: the node's last step factors its parent's first block
   std::vector<int32_t> sn_last_round;  // round of each node's last panel step (-1: none)
   std::vector<int64_t> sn_woff;       // W scratch offsets (6 nf x kPW per node)
   int64_t w_elems = 0;

@@ -194,6 +194,8 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     df.sn_upos = bufs.up(S.sn_upos);
     df.lvl_sn = bufs.up(S.lvl_sn);
     df.launch_tiles = bufs.up(S.launch_tiles);
+    df.leaves = bufs.up(S.leaves);
+    df.sn_hand = bufs.up(S.sn_hand);
     df.sn_toff = bufs.up(S.sn_toff);
     df.sn_poff = bufs.up(S.sn_poff);
     df.panel_items = bufs.up(S.panel_items);
