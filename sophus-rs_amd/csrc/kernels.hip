@@ -1078,6 +1078,7 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double pM[kPanelBlocks][36];
   __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   __shared__ double sb[kPW], sy[kThreads / 64][kPW];
+  __shared__ double sbt[kTile];  // a diagonal tile's rows of the right-hand side
   // the node's last panel step: its update matrix goes to the parent (-1: none)
   const int P = kend == K ? f.sn_parent[X] : -1;
   // the last child of its parent (in round order) factors the parent's first diagonal
@@ -1106,6 +1107,8 @@ __global__ void __launch_bounds__(kThreads)
       vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
     }
     const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
+    const double vbt = load_or_zero(bx, r0 + tid - kPW, dg && tid >= kPW && tid < kPW + kTile &&
+                                                            r0 + tid - kPW < F);
 #pragma unroll
     for (int it = 0; it < kNC; ++it) {
       const int e = tid + kThreads * it;
@@ -1123,6 +1126,7 @@ __global__ void __launch_bounds__(kThreads)
       if (e < kPanelBlocks * 36) pM[e / 36][e % 36] = vm[it];
     }
     if (tid < kPW) sb[tid] = vbb;
+    if (tid >= kPW && tid < kPW + kTile) sbt[tid - kPW] = vbt;
   }
   __syncthreads();
   // accumulator (mi, nj) component q is the tile element
@@ -1144,16 +1148,27 @@ __global__ void __launch_bounds__(kThreads)
   mfma_d4 wa[3], wb[3];
 #pragma unroll
   for (int ct = 0; ct < 3; ++ct) wa[ct] = wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+  if (dg) {  // the column block is the row block
 #pragma unroll
-  for (int k4 = 0; k4 < kPW; k4 += 4) {
-    const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
+    for (int k4 = 0; k4 < kPW; k4 += 4) {
+      const double av = sA[k4 + lc][rw + lr];
 #pragma unroll
-    for (int ct = 0; ct < 3; ++ct)
-      if (k4 < kW[ct]) {
-        const double tv = sT[k4 + lc][16 * ct + lr];
-        wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
-        wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
-      }
+      for (int ct = 0; ct < 3; ++ct)
+        if (k4 < kW[ct])
+          wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sT[k4 + lc][16 * ct + lr], wa[ct], 0, 0, 0);
+    }
+  } else {
+#pragma unroll
+    for (int k4 = 0; k4 < kPW; k4 += 4) {
+      const double av = sA[k4 + lc][rw + lr], bv = sB[k4 + lc][rw + lr];
+#pragma unroll
+      for (int ct = 0; ct < 3; ++ct)
+        if (k4 < kW[ct]) {
+          const double tv = sT[k4 + lc][16 * ct + lr];
+          wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, tv, wa[ct], 0, 0, 0);
+          wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, tv, wb[ct], 0, 0, 0);
+        }
+    }
   }
   __syncthreads();  // every wavefront is done with C
   // W^T of both blocks (a diagonal tile's column block is its row block)
@@ -1245,9 +1260,9 @@ __global__ void __launch_bounds__(kThreads)
       s4 += __shfl_xor(s4, 2, 64);
       if (part == 0 && r < F) {
         if (P < 0)
-          bx[r] -= s4;
+          bx[r] = sbt[i] - s4;
         else  // a finished update row: into the parent's right-hand side
-          f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += bx[r] - s4;
+          f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += sbt[i] - s4;
       }
     }
     if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
