@@ -1027,6 +1027,70 @@ __device__ __forceinline__ void extend_tile(const DevFactor& f, int X, int P, in
       }
 }
 
+// Outputs of one wavefront's 16 rows r0.. of a panel step's rows below the diagonal block,
+// from their L in registers (component q of L4[ct] is row r0 + lc + 4 q, column
+// 16 ct + lr of the panel, lane = 16 lc + lr): the L rows to the factor (and the row-major
+// copy for the node's own rows), the rows' right-hand side b_r -= L_r y_P with
+// y_P = L_PP^-1 b_P (sb: b_P, bt: the rows' b, sy: the wavefront's copy of y_P), into the
+// parent's right-hand side on the node's last step (P >= 0); with zp, z_P = D^-1 y_P for
+// the panel's columns.
+__device__ __forceinline__ void panel_rows_out(const DevFactor& f, const DiagNode& dn, int P, int X,
+                                               int F, int K, int k0, int np, int r0,
+                                               const mfma_d4 (&L4)[3], const double (*sT)[kPW + 1],
+                                               const double* sb, const double (*pM)[36],
+                                               const double* bt, double* sy, double* __restrict__ z,
+                                               int lane, bool zp) {
+  const int lr = lane & 15, lc = lane >> 4;
+  if (lane < kPW) {
+    double a4[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = 0; t < kPW; ++t) a4[t & 3] += sT[t][lane] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
+    sy[lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
+  }
+  double* Lo = f.lbuf + dn.loff;
+  double* Lt = f.ltbuf + dn.toff;
+#pragma unroll
+  for (int ct = 0; ct < 3; ++ct) {
+    const int c = 16 * ct + lr;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int r = r0 + lc + 4 * q;
+      if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = L4[ct][q];
+      if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = L4[ct][q];
+    }
+  }
+  double part[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double acc = 0.0;
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct) acc += L4[ct][q] * sy[16 * ct + lr];  // L is 0 past np
+    acc += __shfl_xor(acc, 1, 64);
+    acc += __shfl_xor(acc, 2, 64);
+    acc += __shfl_xor(acc, 4, 64);
+    acc += __shfl_xor(acc, 8, 64);
+    part[q] = acc;
+  }
+  if (lr == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = lc + 4 * q, r = r0 + i;
+      if (r < F) {
+        if (P < 0)
+          f.usolve[dn.uoff + r] = bt[i] - part[q];
+        else  // a finished update row: into the parent's right-hand side
+          f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += bt[i] - part[q];
+      }
+    }
+  if (zp && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
+    const int jb = lane / 6, a = lane % 6;
+    double zz = 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) zz += pM[jb][6 * a + t] * sy[6 * jb + t];
+    z[6 * int64_t(dn.c0) + k0 + lane] = zz;
+  }
+}
+
 // One round of panel steps, one workgroup per item (node, panel, tr, tc): the 64x64 tile
 // (tr, tc) of the node's lower trailing region below the panel:
 //   1. the tile's values, the panel columns of its row block tr and column block tc (C),
@@ -1037,11 +1101,13 @@ __device__ __forceinline__ void extend_tile(const DevFactor& f, int X, int P, in
 //   3. A -= L W^T on the matrix cores: four wavefronts x 2 x 2 tiles of 16x16, (-W) L^T so
 //      that each lane's results run down 16 rows of the column-major front, accumulators
 //      seeded with the front's values;
-//   4. diagonal tiles (tr == tc) store their L rows (and the row-major copy of the node's
-//      own rows) and eliminate their rows of the node's right-hand side, b_r -= L_r y_P
-//      with y_P = L_PP^-1 b_P; tile (0, 0) also writes z_P = D^-1 y_P for the panel's
-//      columns, then factors the next panel's diagonal block (the top-left of its tile,
-//      handed over in LDS);
+//   4. each row block's output tile stores the block's L rows (and the row-major copy of
+//      the node's own rows) and eliminates its rows of the node's right-hand side,
+//      b_r -= L_r y_P with y_P = L_PP^-1 b_P; the output tile is the diagonal tile (tr, tr),
+//      but for row block 0 it is tile (1, 0) when the step has one (it forms the block's L
+//      from its column block's W), and it also writes z_P = D^-1 y_P for the panel's
+//      columns.  Tile (0, 0) factors the next panel's diagonal block (the top-left of its
+//      tile, handed over in LDS);
 //   5. on the node's last panel step the trailing region is its update matrix: the tile
 //      adds its results into the parent's front instead (extend-add; child update row
 //      block b is the parent's row block upos[b]), and diagonal tiles add their finished
@@ -1069,6 +1135,14 @@ __global__ void __launch_bounds__(kThreads)
   const int np = kend - k0;
   const int r0 = kend + kTile * tr, c0 = kend + kTile * tc;
   const bool dg = tr == tc;
+  // The L rows and right-hand side of row block tr are output by the diagonal tile (tr, tr),
+  // except row block 0's: tile (1, 0), when the step has it, forms that block's L from its
+  // column block's W and outputs it, so that tile (0, 0) goes from its update straight to
+  // the next diagonal block.
+  const int nt = (F - kend + kTile - 1) / kTile;  // tile rows of the step
+  const bool own0 = tr == 1 && tc == 0;
+  const bool out = own0 || (dg && !(tr == 0 && nt > 1));
+  const int ro = own0 ? c0 : r0;  // the output row block's first row
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lr = lane & 15, lc = lane >> 4;
   const int rb = 32 * (wv >> 1), cb = 32 * (wv & 1);  // the wavefront's quadrant
@@ -1078,7 +1152,7 @@ __global__ void __launch_bounds__(kThreads)
   __shared__ double pM[kPanelBlocks][36];
   __shared__ double sL[kPanelBlocks][42], sdi[kPanelBlocks][6];
   __shared__ double sb[kPW], sy[kThreads / 64][kPW];
-  __shared__ double sbt[kTile];  // a diagonal tile's rows of the right-hand side
+  __shared__ double sbt[kTile];  // the output row block's rows of the right-hand side
   // the node's last panel step: its update matrix goes to the parent (-1: none)
   const int P = kend == K ? f.sn_parent[X] : -1;
   // the last child of its parent (in round order) factors the parent's first diagonal
@@ -1106,9 +1180,9 @@ __global__ void __launch_bounds__(kThreads)
       const int e = tid + kThreads * it;
       vm[it] = load_or_zero(f.piv_M, int64_t(36) * gb0 + e, e < kPanelBlocks * 36 && e / 36 < nbp);
     }
-    const double vbb = load_or_zero(bx, k0 + tid, dg && tid < np);
-    const double vbt = load_or_zero(bx, r0 + tid - kPW, dg && tid >= kPW && tid < kPW + kTile &&
-                                                            r0 + tid - kPW < F);
+    const double vbb = load_or_zero(bx, k0 + tid, out && tid < np);
+    const double vbt = load_or_zero(bx, ro + tid - kPW, out && tid >= kPW && tid < kPW + kTile &&
+                                                            ro + tid - kPW < F);
 #pragma unroll
     for (int it = 0; it < kNC; ++it) {
       const int e = tid + kThreads * it;
@@ -1196,6 +1270,26 @@ __global__ void __launch_bounds__(kThreads)
         wa[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wa[ct], 0, 0, 0);
       }
   }
+  if (own0) {
+    // row block 0's outputs, from L of the block (the column block's W times the M_j) in
+    // registers: component q of wb[ct] is L row rw + lc + 4 q, column 16 ct + lr
+#pragma unroll
+    for (int ct = 0; ct < 3; ++ct) wb[ct] = mfma_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k4 = 0; k4 < kPW; k4 += 4) {
+      const int k = k4 + lc;
+      const double av = sB[k][rw + lr];
+#pragma unroll
+      for (int ct = 0; ct < 3; ++ct)
+        if (k4 + 4 > kL0[ct] && k4 < kL1[ct]) {
+          const int n = 16 * ct + lr;
+          const double mv = k / 6 == n / 6 ? pM[k / 6][6 * (k % 6) + n % 6] : 0.0;
+          wb[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, mv, wb[ct], 0, 0, 0);
+        }
+    }
+    panel_rows_out(f, dn, P, X, F, K, k0, np, ro + rw, wb, sT, sb, pM, sbt + rw, sy[wv], z,
+                   lane, wv == 0);
+  }
   // L^T replaces the slice's W^T in sA
 #pragma unroll
   for (int ct = 0; ct < 3; ++ct)
@@ -1229,18 +1323,19 @@ __global__ void __launch_bounds__(kThreads)
   } else {
     extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
   }
-  if (dg) {
+  if (out && !own0) {
     // the slice's L rows to the factor (and the row-major copy of the node's own rows)
+    double (*Lr)[kTile + 1] = sA;  // L^T of the row block
     double* Lo = f.lbuf + dn.loff;
     double* Lt = f.ltbuf + dn.toff;
     for (int e = lane; e < 16 * kPW; e += 64) {
-      const int i = rw + e % 16, c = e / 16, r = r0 + i;
-      if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = sA[c][i];
+      const int i = rw + e % 16, c = e / 16, r = ro + i;
+      if (c < np && r < F) Lo[int64_t(k0 + c) * F + r] = Lr[c][i];
     }
-    if (r0 + rw < K)
+    if (ro + rw < K)
       for (int e = lane; e < 16 * kPW; e += 64) {
-        const int c = e % kPW, i = rw + e / kPW, r = r0 + i;
-        if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = sA[c][i];
+        const int c = e % kPW, i = rw + e / kPW, r = ro + i;
+        if (c < np && r < K) Lt[int64_t(r) * K + k0 + c] = Lr[c][i];
       }
     // forward elimination of the right-hand side: y_P = L_PP^-1 b_P (each wavefront its
     // own copy, lane c < kPW entry c), then b_r -= L_r y_P for the slice's rows (four
@@ -1252,10 +1347,10 @@ __global__ void __launch_bounds__(kThreads)
       sy[wv][lane] = (a4[0] + a4[1]) + (a4[2] + a4[3]);
     }
     {
-      const int i = rw + (lane >> 2), part = lane & 3, r = r0 + i;
+      const int i = rw + (lane >> 2), part = lane & 3, r = ro + i;
       double s4 = 0.0;
 #pragma unroll
-      for (int u = 0; u < kPW / 4; ++u) s4 += sA[part + 4 * u][i] * sy[wv][part + 4 * u];
+      for (int u = 0; u < kPW / 4; ++u) s4 += Lr[part + 4 * u][i] * sy[wv][part + 4 * u];
       s4 += __shfl_xor(s4, 1, 64);
       s4 += __shfl_xor(s4, 2, 64);
       if (part == 0 && r < F) {
@@ -1265,7 +1360,7 @@ __global__ void __launch_bounds__(kThreads)
           f.usolve[f.sn_uoff[P] + parent_row(f, X, r - K)] += sbt[i] - s4;
       }
     }
-    if (tr == 0 && wv == 0 && lane < np) {  // z_P = D^-1 y_P (blockwise M_j)
+    if (ro == kend && wv == 0 && lane < np) {  // row block 0: z_P = D^-1 y_P (blockwise M_j)
       const int jb = lane / 6, a = lane % 6;
       double zz = 0.0;
 #pragma unroll
