@@ -1027,6 +1027,70 @@ __device__ __forceinline__ void extend_tile(const DevFactor& f, int X, int P, in
       }
 }
 
+// extend_tile for the tile (0, 0) of a parent's last child on its last step, which then
+// factors the parent's first diagonal block (np x np): besides the adds into the parent's
+// front, the block's values after them are built in LDS (D2, which the caller's update no
+// longer reads) from the block's values before them, loaded with the adds' own loads, so
+// the factorization does not wait for the adds to reach memory and load them again.
+// Called by all threads; ends with the block complete in D2.
+__device__ __forceinline__ void extend_tile_hand(const DevFactor& f, int X, int P, int K, int F,
+                                                 int rl, int cl, const mfma_d4 (&acc)[2][2],
+                                                 double (*D2)[kPW + 1], int np) {
+  double* B = f.front + f.sn_off[P];
+  const int64_t FP = 6 * int64_t(f.sn_nf[P]);
+  int pr[2], pc[2][4];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi) {
+    const int r = rl + 16 * mi;
+    pr[mi] = r < F ? parent_row(f, X, r - K) : 0;
+  }
+#pragma unroll
+  for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = cl + 16 * nj + 4 * q;
+      pc[nj][q] = c < F ? parent_row(f, X, c - K) : 0;
+    }
+  double v[2][2][4];
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = rl + 16 * mi, c = cl + 16 * nj + 4 * q;
+        v[mi][nj][q] = load_or_zero(B, pc[nj][q] * FP + pr[mi], r < F && c < F && r >= c);
+      }
+  constexpr int kNB = (kPW * kPW + kThreads - 1) / kThreads;
+  double bv[kNB];
+#pragma unroll
+  for (int it = 0; it < kNB; ++it) {
+    const int e = threadIdx.x + kThreads * it, r = e % np, c = e / np;
+    bv[it] = load_or_zero(B, int64_t(c) * FP + r, e < np * np && r >= c);
+  }
+  __syncthreads();  // every wavefront is done with D2's memory
+#pragma unroll
+  for (int it = 0; it < kNB; ++it) {
+    const int e = threadIdx.x + kThreads * it;
+    if (e < np * np) D2[e % np][e / np] = bv[it];
+  }
+  __syncthreads();  // the block's values before the adds are in place
+#pragma unroll
+  for (int mi = 0; mi < 2; ++mi)
+#pragma unroll
+    for (int nj = 0; nj < 2; ++nj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int r = rl + 16 * mi, c = cl + 16 * nj + 4 * q;
+        if (r < F && c < F && r >= c) {
+          const double nv = v[mi][nj][q] + acc[mi][nj][q];
+          B[pc[nj][q] * FP + pr[mi]] = nv;
+          if (pr[mi] < np && pc[nj][q] < np) D2[pr[mi]][pc[nj][q]] = nv;
+        }
+      }
+  __syncthreads();
+}
+
 // Outputs of one wavefront's 16 rows r0.. of a panel step's rows below the diagonal block,
 // from their L in registers (component q of L4[ct] is row r0 + lc + 4 q, column
 // 16 ct + lr of the panel, lane = 16 lc + lr): the L rows to the factor (and the row-major
@@ -1320,8 +1384,11 @@ __global__ void __launch_bounds__(kThreads)
           const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
           if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
         }
-  } else {
+  } else if (!hand) {
     extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
+  } else {
+    extend_tile_hand(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc, as_square(sB),
+                     min(6 * f.sn_nc[P], kPW));
   }
   if (out && !own0) {
     // the slice's L rows to the factor (and the row-major copy of the node's own rows)
@@ -1390,10 +1457,9 @@ __global__ void __launch_bounds__(kThreads)
     factor_panel_diag(f, dn, A, F, kend, min(K, kend + kPW) - kend, sc, as_square(sA),
                       as_square(sB), sL, sdi, true, z);
   } else if (hand) {
-    __syncthreads();  // the tile's adds into the parent's front are done
-    const int KP = 6 * f.sn_nc[P];
-    factor_panel_diag(f, diag_node(f, P), f.front + f.sn_off[P], 6 * f.sn_nf[P], 0, min(KP, kPW),
-                      sc, as_square(sA), as_square(sB), sL, sdi, false, z);
+    __syncthreads();  // the work above that reads sA is done
+    factor_panel_diag(f, diag_node(f, P), f.front + f.sn_off[P], 6 * f.sn_nf[P], 0,
+                      min(6 * f.sn_nc[P], kPW), sc, as_square(sB), as_square(sA), sL, sdi, true, z);
   }
 }
 
@@ -1563,8 +1629,8 @@ __global__ void __launch_bounds__(kThreads)
 // accumulators start from the tile's current values.  On the node's last step the results
 // go into the parent's front (extend-add).  The tile (0, 0), which holds the next panel's
 // diagonal block, then factors that block (on the last child's last step: the parent's
-// first one).
-__global__ void __launch_bounds__(kThreads)
+// first one).  Two workgroups per CU (the tiles are many, and short).
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
     k_mf_update(DevFactor f, const int32_t* __restrict__ tiles, DevScalars* sc,
                 double* __restrict__ z) {
   static_assert(kTile == 64 && kThreads == 256 && kPW % 4 == 0, "4 wavefronts of 32x32");
@@ -1645,8 +1711,11 @@ __global__ void __launch_bounds__(kThreads)
           const int r = r0 + rb + 16 * mi + lr, c = c0 + cb + 16 * nj + lc + 4 * q;
           if (r < F && c < F && r >= c) A[int64_t(c) * F + r] = acc[mi][nj][q];
         }
-  } else {
+  } else if (!hand) {
     extend_tile(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc);
+  } else {
+    extend_tile_hand(f, X, P, K, F, r0 + rb + lr, c0 + cb + lc, acc, as_square(sB),
+                     min(6 * f.sn_nc[P], kPW));
   }
   if (tr == 0 && tc == 0 && kend < K) {
     // the next panel's diagonal block is the top-left of this tile: hand the updated
@@ -1670,10 +1739,9 @@ __global__ void __launch_bounds__(kThreads)
     factor_panel_diag(f, dn, A, F, kend, min(K, kend + kPW) - kend, sc, as_square(sA),
                       as_square(sB), sL, sdi, true, z);
   } else if (hand) {
-    __syncthreads();  // the tile's adds into the parent's front are done
-    const int KP = 6 * f.sn_nc[P];
-    factor_panel_diag(f, diag_node(f, P), f.front + f.sn_off[P], 6 * f.sn_nf[P], 0, min(KP, kPW),
-                      sc, as_square(sA), as_square(sB), sL, sdi, false, z);
+    __syncthreads();  // the work above that reads sA is done
+    factor_panel_diag(f, diag_node(f, P), f.front + f.sn_off[P], 6 * f.sn_nf[P], 0,
+                      min(6 * f.sn_nc[P], kPW), sc, as_square(sB), as_square(sA), sL, sdi, true, z);
   }
 }
 
