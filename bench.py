@@ -6,11 +6,12 @@ Huber 1 px, camera 0 fixed, nu0 = 1), inputs resident in HBM before the timed re
 - value: observations per second of the plain FP64 GPU path (Schur + direct block LDLt),
   N_o / (seconds per LM iteration) over exactly --steps timed iterations after --warmup
   (tolerances disabled so every iteration does the full work).
-- roofline: the dominant kernel, one panel step of the RCS factorization (k_mf_step; the
-  steps with >= 640 tiles run as k_mf_panel + k_mf_update): the rank-48 trailing update
-  of HBM-resident fronts, 6 flop/byte, so HBM-bound; algorithmic bytes per step from the
-  symbolic factor / the step's average time from HIP events around its launches on the
-  optimizer's stream, against HBM peak.
+- roofline: the dominant kernel, one round of panel steps of the RCS factorization (one
+  k_mf_step launch holding a panel step of every ready supernode; rounds with >= 640
+  tiles run as k_mf_panel + k_mf_update): rank-48 trailing updates of HBM-resident
+  fronts, 6 flop/byte, so HBM-bound; algorithmic bytes per round from the symbolic factor
+  / the round's average time from HIP events around its launches on the optimizer's
+  stream, against HBM peak.
 - time_to_tol: F* = the FP64 direct path's final cost (function tolerance 1e-10, at most
   100 iterations); each path's first iteration within 1e-6 of F*, its time and
   obs/s = N_o x iterations / time: FP64 direct, block-Jacobi PCG, learned PCG, mixed.
@@ -196,15 +197,15 @@ def main() -> int:
     bytes_launch = st["mf_step_bytes"] / st["mf_step_launches"]
     flops_launch = st["mf_step_flops"] / st["mf_step_launches"]
     achieved = bytes_launch / avg_s / 1e9
-    # HBM bytes per panel step from the committed rocprofv3 --pmc summary of this workload
+    # HBM bytes per round from the committed rocprofv3 --pmc summary of this workload
     # (FETCH_SIZE and WRITE_SIZE runs, scaled by a 256 MiB copy's readings)
     traffic = None
     tfile = ROOT / "profiles" / "pmc_traffic_ba_walk_2000cams.json"
     if args.config == 2 and tfile.exists():
-        kern = json.loads(tfile.read_text()).get("panel_step")
-        traffic = kern["hbm_bytes_per_step"] if kern else None
+        kern = json.loads(tfile.read_text()).get("round")
+        traffic = kern["hbm_bytes_per_round"] if kern else None
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "panel step (k_mf_step; k_mf_panel + k_mf_update when split)",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "round of panel steps (k_mf_step; k_mf_panel + k_mf_update when split)",
             "bytes_per_launch": bytes_launch, "flops_per_launch": flops_launch,
             "tflops": flops_launch / avg_s / 1e12, "avg_launch_us": avg_s * 1e6,
             "launches_per_factorization": st["mf_step_launches"]}

@@ -261,11 +261,12 @@ typedef struct sgpu_rcs_stats {
   int64_t num_supernodes;    /* supernodes (dense fronts) of the factorization */
   int64_t max_front_blocks;  /* largest front, in 6x6 blocks */
   int64_t front_bytes;       /* device bytes of all fronts */
-  double mf_step_flops;      /* algorithmic flops of the panel-step launches' trailing updates
-                                (k_mf_step) */
+  double mf_step_flops;      /* algorithmic flops of the trailing updates of all rounds of panel
+                                steps (k_mf_step launches, one panel step of every ready
+                                supernode each) */
   double mf_step_bytes;      /* their algorithmic HBM bytes: read + write of the lower trailing
                                 fronts, one read of the panel columns below, L written */
-  int64_t mf_step_launches;  /* panel-step launches per factorization */
+  int64_t mf_step_launches;  /* rounds of panel steps per factorization */
 } sgpu_rcs_stats;
 
 sgpu_status sgpu_optimizer_rcs_stats(const sgpu_optimizer* opt, sgpu_rcs_stats* stats);
