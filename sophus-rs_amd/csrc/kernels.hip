@@ -164,9 +164,8 @@ __device__ __forceinline__ void obs_residual(const double* __restrict__ in,
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_point_prep(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
-                 const double* __restrict__ cw, double nu, double* __restrict__ hinv_out,
-                 double* __restrict__ hll_out, double* __restrict__ s_out,
-                 double* __restrict__ y_out, double* __restrict__ prec, DevScalars* sc) {
+                 const double* __restrict__ cw, double nu, double* __restrict__ s_out,
+                 double* __restrict__ prec, DevScalars* sc) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = gid < p.npt;
   const int b = live ? gid : p.npt - 1;  // trailing lanes stay for the wavefront max
@@ -226,7 +225,6 @@ __global__ void __launch_bounds__(kThreads)
     inv[7] = (m12 * m31 - m32 * m11) * rdet;
     inv[8] = (m11 * m22 - m21 * m12) * rdet;
   }
-  for (int k = 0; k < 9; ++k) hinv_out[9 * b + k] = inv[k];
   double yv[3];
   for (int k = 0; k < 3; ++k)
     yv[k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) + inv[3 * k + 2] * double(s[2]);
@@ -234,18 +232,7 @@ __global__ void __launch_bounds__(kThreads)
                              yv[0],  yv[1],  yv[2],  m11,    m12,    m13,    m22,    m23,    m33};
 #pragma unroll
   for (int k = 0; k < kPtRec; ++k) rec[k] = rv[k];
-  hll_out[6 * b + 0] = m11;
-  hll_out[6 * b + 1] = m12;
-  hll_out[6 * b + 2] = m13;
-  hll_out[6 * b + 3] = m22;
-  hll_out[6 * b + 4] = m23;
-  hll_out[6 * b + 5] = m33;
-  for (int k = 0; k < 3; ++k) {
-    s_out[3 * b + k] = double(s[k]);
-    y_out[3 * b + k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) +
-                       inv[3 * k + 2] * double(s[2]);
-
-  }
+  for (int k = 0; k < 3; ++k) s_out[3 * b + k] = double(s[k]);
 }
 
 // P_i^T N P_j for P = [hat(-x) | I] and a 3x3 N:
@@ -369,9 +356,8 @@ __device__ __forceinline__ int wave_sum_scatter(double (&v)[N], int lane) {
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_cam_obs(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
-              const double* __restrict__ cw, const double* __restrict__ hinv_g,
-              const double* __restrict__ hll_g, const double* __restrict__ y_g,
-              const double* __restrict__ prec, int centered, T* __restrict__ vobs,
+              const double* __restrict__ cw, const double* __restrict__ prec, int centered,
+              T* __restrict__ vobs,
               double* __restrict__ own_part) {
   constexpr int kOwn = 48;  // 36 block entries, 6 reduced rhs, 6 gradient
   __shared__ T sv[kThreads * kObsRec];
@@ -460,13 +446,15 @@ __global__ void __launch_bounds__(kThreads)
       obs_residual<double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
       // record: U_o row-major, then the lift point (zero for a constant point: it has no
       // cross pairs)
+      // the point's packed record: H_ll^-1 upper at 3, y at 9, damped H_ll upper at 12
+      const double* rb = prec + kPtRec * int64_t(b);
       if (active) {
-        const double* h = hinv_g + 9 * b;
+        const double* h = rb + 3;
         const double l00 = sqrt(fmax(h[0], 0.0)), i00 = l00 > 0.0 ? 1.0 / l00 : 0.0;
-        const double l10 = h[3] * i00, l20 = h[6] * i00;
-        const double l11 = sqrt(fmax(h[4] - l10 * l10, 0.0)), i11 = l11 > 0.0 ? 1.0 / l11 : 0.0;
-        const double l21 = (h[7] - l20 * l10) * i11;
-        const double l22 = sqrt(fmax(h[8] - l20 * l20 - l21 * l21, 0.0));
+        const double l10 = h[1] * i00, l20 = h[2] * i00;
+        const double l11 = sqrt(fmax(h[3] - l10 * l10, 0.0)), i11 = l11 > 0.0 ? 1.0 / l11 : 0.0;
+        const double l21 = (h[4] - l20 * l10) * i11;
+        const double l22 = sqrt(fmax(h[5] - l20 * l20 - l21 * l21, 0.0));
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
           vs[j] = T(l00 * Md[j] + l10 * Md[3 + j] + l20 * Md[6 + j]);
@@ -486,16 +474,18 @@ __global__ void __launch_bounds__(kThreads)
       for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
       gram(Ai, Mi);
       if (active) {
-        const double* yb = y_g + 3 * b;
+        const double* yb = rb + 9;
 #pragma unroll
         for (int r = 0; r < 3; ++r) t[r] = Md[3 * r] * yb[0] + Md[3 * r + 1] * yb[1] + Md[3 * r + 2] * yb[2] - v[r];
-        const double* H = hll_g + 6 * b;
+        const double* H = rb + 12;
         const double Hf[9] = {H[0], H[1], H[2], H[1], H[3], H[4], H[2], H[4], H[5]};
+        const double* h = rb + 3;
+        const double Hv[9] = {h[0], h[1], h[2], h[1], h[3], h[4], h[2], h[4], h[5]};
         T R[9], Hi[9], Z[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) {
           R[e] = T(Hf[e] - Md[e]);
-          Hi[e] = T(hinv_g[9 * b + e]);
+          Hi[e] = T(Hv[e]);
         }
         mat3_mul_t(Mi, Hi, Z);
         mat3_mul_t(Z, R, N);
@@ -1919,7 +1909,7 @@ __device__ void block_reduce2(double a, double b, double* partial) {
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
     k_backsub(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
-              const double* __restrict__ cw, const double* __restrict__ hinv,
+              const double* __restrict__ cw, const double* __restrict__ prec,
               const double* __restrict__ pt_s, const double* __restrict__ dx_cam,
               double* __restrict__ points_new, double* __restrict__ partial, DevScalars* sc) {
   double dd = 0, dg = 0, mx = 0;
@@ -1949,7 +1939,9 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
       for (int k = 0; k < 3; ++k) acc[k] += M[3 * k] * q0 + M[3 * k + 1] * q1 + M[3 * k + 2] * q2;
     }
-    const double* Hi = hinv + 9 * b;
+    // H_ll^-1 from the point's packed record (upper triangle 00, 01, 02, 11, 12, 22)
+    const double* hu = prec + kPtRec * int64_t(b) + 3;
+    const double Hi[9] = {hu[0], hu[1], hu[2], hu[1], hu[3], hu[4], hu[2], hu[4], hu[5]};
     for (int k = 0; k < 3; ++k) {
       const double dk = Hi[3 * k] * double(acc[0]) + Hi[3 * k + 1] * double(acc[1]) +
                         Hi[3 * k + 2] * double(acc[2]);
@@ -2323,11 +2315,11 @@ void launch_point_prep(hipStream_t s, int precision, const DevProblem& p, const 
   if (p.npt == 0) return;
   const dim3 g(blocks_for(p.npt, kThreads)), b(kThreads);
   if (precision == 0)
-    hipLaunchKernelGGL(k_point_prep<double>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.hll,
-                       w.pt_s, w.pt_y, w.prec, sc);
+    hipLaunchKernelGGL(k_point_prep<double>, g, b, 0, s, p, points, intr, cw, nu, w.pt_s, w.prec,
+                       sc);
   else
-    hipLaunchKernelGGL(k_point_prep<float>, g, b, 0, s, p, points, intr, cw, nu, w.hinv, w.hll,
-                       w.pt_s, w.pt_y, w.prec, sc);
+    hipLaunchKernelGGL(k_point_prep<float>, g, b, 0, s, p, points, intr, cw, nu, w.pt_s, w.prec,
+                       sc);
 }
 
 void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const double* points,
@@ -2340,15 +2332,15 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
   if (precision == 0) {
     double* v = static_cast<double*>(w.vobs);
     if (p.n_own_chunks > 0)
-      hipLaunchKernelGGL(k_cam_obs<double>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.hinv,
-                         w.hll, w.pt_y, w.prec, 0, v, w.own_part);
+      hipLaunchKernelGGL(k_cam_obs<double>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
+                         0, v, w.own_part);
     hipLaunchKernelGGL(k_schur_blocks<double>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 0,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   } else {
     float* v = static_cast<float*>(w.vobs);
     if (p.n_own_chunks > 0)
-      hipLaunchKernelGGL(k_cam_obs<float>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.hinv,
-                         w.hll, w.pt_y, w.prec, 1, v, w.own_part);
+      hipLaunchKernelGGL(k_cam_obs<float>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
+                         1, v, w.own_part);
     hipLaunchKernelGGL(k_schur_blocks<float>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 1,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   }
@@ -2442,10 +2434,10 @@ void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p, co
   const int g = std::max(1, std::min(blocks_for(p.npt, kThreads), kCostBlocks));
   if (precision == 0)
     hipLaunchKernelGGL(k_backsub<double>, dim3(g), dim3(kThreads), 0, s, p, points, intr, cw,
-                       w.hinv, w.pt_s, w.dx_cam, points_new, w.partial, sc);
+                       w.prec, w.pt_s, w.dx_cam, points_new, w.partial, sc);
   else
     hipLaunchKernelGGL(k_backsub<float>, dim3(g), dim3(kThreads), 0, s, p, points, intr, cw,
-                       w.hinv, w.pt_s, w.dx_cam, points_new, w.partial, sc);
+                       w.prec, w.pt_s, w.dx_cam, points_new, w.partial, sc);
 }
 
 void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses, const DevLin& w,

@@ -98,12 +98,9 @@ constexpr int kPtRec = 18;  // doubles per packed point record (DevLin::prec)
 // Linearization workspace.
 struct DevLin {
   double* cw;         // npose x 12: camera_from_world rotation (row-major) + translation
-  double* hinv;       // npt x 9
-  double* hll;        // npt x 6: damped H_ll (00, 01, 02, 11, 12, 22)
   double* prec;       // npt x kPtRec packed point record: x (3), H_ll^-1 upper (00, 01, 02, 11,
                       // 12, 22), y (3), damped H_ll upper (6); zeros but x for a constant point
   double* pt_s;       // npt x 3: neg-gradient of the point (sum of A^T r~)
-  double* pt_y;       // npt x 3: hinv * s
   double* rhs_nat;    // nb x 6: reduced rhs in camera order
   double* grad_cam;   // nb x 6: neg-gradient of the cameras
   double* x_perm;     // nb x 6 solve buffer in permuted order

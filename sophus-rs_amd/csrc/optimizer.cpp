@@ -308,11 +308,8 @@ struct sgpu_optimizer {
     df.host = &S;
     // workspace
     dl.cw = arena.alloc<double>(size_t(12) * S.npose);
-    dl.hinv = arena.alloc<double>(size_t(9) * S.npt);
-    dl.hll = arena.alloc<double>(size_t(6) * S.npt);
     dl.prec = arena.alloc<double>(size_t(kPtRec) * S.npt);
     dl.pt_s = arena.alloc<double>(size_t(3) * S.npt);
-    dl.pt_y = arena.alloc<double>(size_t(3) * S.npt);
     dl.rhs_nat = arena.alloc<double>(size_t(6) * S.nb);
     dl.grad_cam = arena.alloc<double>(size_t(6) * S.nb);
     dl.x_perm = arena.alloc<double>(size_t(6) * S.nb);
