@@ -529,22 +529,24 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
-// acc -= P_i^T (U_i^T U_j) P_j (6x6 row-major) from two observation records
-template <typename T>
+// acc -= P_i^T (U_i^T U_j) P_j (6x6 row-major) from two observation records.  With one lift
+// point (the records' points are the same point, not camera-centred), only U is read from
+// the second record.
+template <bool kOneLift, typename T>
 __device__ __forceinline__ void sub_pair(const T* __restrict__ ri, const T* __restrict__ rj,
                                          double acc[36]) {
+  constexpr int kB = kOneLift ? 9 : kObsRec;
   T a[kObsRec], b[kObsRec];
 #pragma unroll
-  for (int e = 0; e < kObsRec; ++e) {
-    a[e] = ri[e];
-    b[e] = rj[e];
-  }
+  for (int e = 0; e < kObsRec; ++e) a[e] = ri[e];
+#pragma unroll
+  for (int e = 0; e < kB; ++e) b[e] = rj[e];
   T N[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
 #pragma unroll
     for (int q = 0; q < 3; ++q) N[3 * r + q] = a[r] * b[q] + a[3 + r] * b[3 + q] + a[6 + r] * b[6 + q];
-  add_lift(a + 9, b + 9, N, T(-1), [&](int e, T val) { acc[e] += double(val); });
+  add_lift(a + 9, (kOneLift ? a : b) + 9, N, T(-1), [&](int e, T val) { acc[e] += double(val); });
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; this remaps them so each XCD
@@ -559,7 +561,7 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
 // FP64 accumulation, a fixed-order wavefront reduction; on the diagonal the camera's
 // chunk sums, the damping and the rhs / gradient rows.
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
     k_schur_blocks(DevProblem p, const T* __restrict__ vobs, const double* __restrict__ own_part,
                    const double* __restrict__ cw, double nu, int centered,
                    double* __restrict__ front, double* __restrict__ rhs_nat,
@@ -573,10 +575,19 @@ __global__ void __launch_bounds__(kThreads)
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  // a pair's two records hold the same point, so without centring one lift point serves both
+  if (centered) {
 #pragma unroll 2
-  for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
-    const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
-    sub_pair(vobs + kObsRec * int64_t(i), vobs + kObsRec * int64_t(j), acc);
+    for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
+      const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
+      sub_pair<false>(vobs + kObsRec * int64_t(i), vobs + kObsRec * int64_t(j), acc);
+    }
+  } else {
+#pragma unroll 2
+    for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
+      const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
+      sub_pair<true>(vobs + kObsRec * int64_t(i), vobs + kObsRec * int64_t(j), acc);
+    }
   }
   // 36 = 4 * 9: after two halvings lanes 16 m .. 16 m + 15 hold totals 9 m .. 9 m + 8
   wave_sum_scatter(acc, lane);
