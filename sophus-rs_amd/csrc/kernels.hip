@@ -560,10 +560,10 @@ __device__ __forceinline__ int xcd_contiguous(int bid, int n) {
 // One wavefront per S block: lanes stride over the block's cross pairs (point order),
 // FP64 accumulation, a fixed-order wavefront reduction; on the diagonal the camera's
 // chunk sums, the damping and the rhs / gradient rows.
-template <typename T>
+template <typename T, bool kCentered>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
     k_schur_blocks(DevProblem p, const T* __restrict__ vobs, const double* __restrict__ own_part,
-                   const double* __restrict__ cw, double nu, int centered,
+                   const double* __restrict__ cw, double nu,
                    double* __restrict__ front, double* __restrict__ rhs_nat,
                    double* __restrict__ grad_cam, double* __restrict__ bsr) {
   const int lane = threadIdx.x & 63;
@@ -576,7 +576,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
   // a pair's two records hold the same point, so without centring one lift point serves both
-  if (centered) {
+  if constexpr (kCentered) {
 #pragma unroll 2
     for (int64_t q = p.pair_ptr[k] + lane; q < p.pair_ptr[k + 1]; q += 64) {
       const int i = p.pairs[2 * q], j = p.pairs[2 * q + 1];
@@ -605,7 +605,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
   }
   if (lane < 36) {
     const int a = lane / 6, bb = lane % 6;
-    if (diag && centered) {
+    if (diag && kCentered) {
       double c[3];
       cam_center(cw + 12 * p.cam_member[c1], c);
       mine += nu * double(p.cam_nu[c1]) * centered_gram(c, a, bb);
@@ -2345,14 +2345,14 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
     if (p.n_own_chunks > 0)
       hipLaunchKernelGGL(k_cam_obs<double>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
                          0, v, w.own_part);
-    hipLaunchKernelGGL(k_schur_blocks<double>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 0,
+    hipLaunchKernelGGL((k_schur_blocks<double, false>), gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   } else {
     float* v = static_cast<float*>(w.vobs);
     if (p.n_own_chunks > 0)
       hipLaunchKernelGGL(k_cam_obs<float>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
                          1, v, w.own_part);
-    hipLaunchKernelGGL(k_schur_blocks<float>, gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu, 1,
+    hipLaunchKernelGGL((k_schur_blocks<float, true>), gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   }
 }
