@@ -70,10 +70,10 @@ struct DevFactor {
   const int32_t* sn_upos_ptr;
   const int32_t* sn_upos;
   const int32_t* lvl_sn;
-  const int32_t* launch_tiles;  // round work items (node, panel, tile_row, tile_col)
+  const int32_t* launch_tiles;  // tiles of the rounds: (node, panel, tile_row, tile_col)
   const int32_t* leaves;        // nodes without children
   const int32_t* sn_hand;       // 1: the node's last step factors its parent's first block
-  const int32_t* solve_cols;  // (node, 64-column tile) per level
+  const int32_t* solve_cols;  // backward solve: (node, kSolveCols-column tile) per level
   const int64_t* sn_uoff;     // right-hand side offsets (F entries per node)
   double* usolve;             // each node's right-hand side in front row order (u_elems)
   hipEvent_t* step_events;    // optional: 2 per round of panel steps, around its launches

@@ -586,8 +586,8 @@ void analyze_rcs(Symbolic& S) {
     std::vector<int32_t> fill(S.lvl_ptr.begin(), S.lvl_ptr.end() - 1);
     for (int32_t x = 0; x < S.nsn; ++x) S.lvl_sn[fill[S.sn_level[x]]++] = x;
   }
-  // launch schedule: per level, per panel: the nodes with that panel and the lower
-  // triangle of trailing tiles
+  // per-node scratch offsets (W of a panel, the panel inverses, the row-major L_KK copy),
+  // then the round schedule
   S.launches.clear();
   S.launch_tiles.clear();
   S.panel_items.clear();
