@@ -122,6 +122,18 @@ static void check(int layout, int ncam, long npt, uint64_t seed) {
       CHECK(S.sn_parent[x] > x, "parent index below child");
       kids[S.sn_parent[x]].push_back(x);
     }
+  // a child's update rows keep their order in the parent (strictly increasing positions), so
+  // the parent's first kPW rows get the update only from the child's first update rows
+  // (its tile (0, 0)), which the hand-off relies on
+  for (int x = 0; x < nsn; ++x) {
+    const int m = S.sn_nf[x] - S.sn_nc[x];
+    CHECK(S.sn_upos_ptr[x + 1] - S.sn_upos_ptr[x] == m, "node %d: %d update positions for %d rows", x,
+          S.sn_upos_ptr[x + 1] - S.sn_upos_ptr[x], m);
+    for (int b = 1; b < m; ++b)
+      CHECK(S.sn_upos[S.sn_upos_ptr[x] + b] > S.sn_upos[S.sn_upos_ptr[x] + b - 1], "node %d: update rows reorder", x);
+    if (m > 0 && S.sn_parent[x] >= 0)
+      CHECK(S.sn_upos[S.sn_upos_ptr[x] + m - 1] < S.sn_nf[S.sn_parent[x]], "node %d: position past the parent", x);
+  }
   std::set<int> leaves(S.leaves.begin(), S.leaves.end());
   for (int x = 0; x < nsn; ++x) {
     CHECK(leaves.count(x) == (kids[x].empty() ? 1u : 0u), "leaf list wrong at %d", x);

@@ -6,7 +6,7 @@ a parent steps only after every child's last step; children of one parent end in
 rounds (their extend-adds are plain stores); exactly one child per parent, the last to end,
 factors the parent's first diagonal block; each step's tiles are the lower triangle of its
 trailing region; split rounds list every panel-row chunk; the tiles that factor a diagonal
-block come first in their round.
+block come first in their round; a child's update rows keep their order in the parent.
 """
 import shutil
 import subprocess
