@@ -122,6 +122,33 @@ def test_block_ldlt_kernel_zero_pivot_rule(sg, orc):
     assert _rel(x, x_or) < 1e-6
 
 
+@pytest.mark.parametrize("nb,hub", [(24, 1), (40, 2), (70, 9)])
+def test_block_ldlt_kernel_arrow_many_children(sg, orc, nb, hub):
+    # arrow pattern: every leading block couples only to the last `hub` blocks, so the
+    # elimination tree is one separator with many leaf children; the children end in
+    # distinct rounds (their adds into the separator's front are plain stores) and the last
+    # one factors the separator's first diagonal block (with hub <= 8 also its whole
+    # solve, z written by that hand-off)
+    rng = np.random.default_rng(100 + nb)
+    n = 6 * nb
+    A = np.zeros((n, n))
+    for i in range(nb - hub):
+        B = rng.normal(size=(6, 6))
+        A[6 * i:6 * i + 6, 6 * i:6 * i + 6] += B @ B.T
+        for j in range(nb - hub, nb):
+            C = rng.normal(size=(6, 6))
+            A[6 * j:6 * j + 6, 6 * i:6 * i + 6] += C
+            A[6 * i:6 * i + 6, 6 * j:6 * j + 6] += C.T
+    H = rng.normal(size=(6 * hub, 6 * hub))
+    A[6 * (nb - hub):, 6 * (nb - hub):] += H @ H.T
+    A += np.eye(n) * (np.abs(A).sum(axis=1).max() + 1.0)
+    b = rng.normal(size=n)
+    x = sg.block_ldlt_solve(A, b)
+    assert _rel(x, np.linalg.solve(A, b)) < 1e-10
+    x_or = orc.block_ldlt_solve_dense(A, [6] * nb, b)
+    assert _rel(x, x_or) < 1e-10
+
+
 def _run_gpu(sg, problem, params, steps=None):
     opt = sg.Optimizer(problem.variables(), problem.cost_fns(1.0), params)
     trace = []
