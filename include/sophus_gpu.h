@@ -277,6 +277,22 @@ sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
                          const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                          sgpu_rcs_stats* stats);
 
+/* Elimination order of the reduced camera system (no reference counterpart as an entry
+ * point; it replaces the AMD permutation that BlockSparseLdlt computes internally,
+ * crates/sophus_solver/src/ldlt.rs:49-67): perm[new] = old over the free camera blocks,
+ * numbered in the variables' layout order.  Host-only, like sgpu_analyze.  *num_blocks
+ * receives the block count; perm (may be NULL) needs that capacity. */
+sgpu_status sgpu_analyze_rcs_order(const sgpu_var_family* families, int64_t num_families,
+                                   const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                   int64_t* perm, int64_t capacity, int64_t* num_blocks);
+/* The same order for an optimizer handle. */
+sgpu_status sgpu_optimizer_rcs_order(const sgpu_optimizer* opt, int64_t* perm, int64_t capacity,
+                                     int64_t* num_blocks);
+/* Pivots d of the last direct RCS factorization (BlockDiagonalLdltSystem's D_j =
+ * L_jj diag(d) L_jj^T, block_diag_ldlt.rs:80-117): 6 per block in elimination order,
+ * zero where the PSD rule dropped a pivot (dense_ldlt.rs:164-256).  d needs 6 x blocks. */
+sgpu_status sgpu_optimizer_rcs_pivots(sgpu_optimizer* opt, double* d, int64_t capacity);
+
 /* Per-kernel device timing recorded with HIP events on the handle's stream since the
  * last reset: names[i] (static strings), count[i] launches, total_ms[i].
  * Returns the number of kernels written (<= capacity). */

@@ -23,7 +23,9 @@
 //     contribute exact zeros to every sum.
 //   - The RCS block ordering is a deterministic minimum-degree ordering written
 //     here; the reference uses faer's AMD (crates/sophus_solver/src/ldlt.rs:49-67),
-//     which is not available.  Orderings change rounding only.
+//     which is not available.  Orderings change rounding only.  A caller may set the
+//     ordering instead (oracle_set_rcs_order), e.g. to the one the GPU library uses, so
+//     that both eliminate the blocks in the same order.
 //   - Sums run sequentially in term order (the reference's rayon sums are
 //     order-unspecified).
 //   - Bunch-Kaufman fallback (block_diag_ldlt.rs:97-114) is not restated: it only
@@ -554,6 +556,18 @@ std::vector<int64_t> min_degree_order(int64_t nb, const std::vector<std::set<int
   return perm;  // perm[new] = old
 }
 
+// Ordering override: perm[new] = old over the RCS blocks; empty = minimum degree.
+std::vector<int64_t> g_rcs_order;
+
+// Pivot statistics of every block LDLt factorization since the last reset: pivots set to
+// zero by the PSD rule, negative pivots, and the smallest |d| / max(1, max |d|) over the
+// kept pivots of a diagonal block (how close the factorization came to the zero rule).
+struct FactorStat {
+  int64_t zero, neg;
+  double min_ratio;
+};
+std::vector<FactorStat> g_factor_stats;
+
 struct BlockLdltFactor {
   int64_t nb = 0;
   std::vector<int64_t> perm, dim, start;  // perm[new] = old; dim/start in new order
@@ -573,7 +587,10 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
         adj[r].insert(A.col[k]);
         adj[A.col[k]].insert(r);
       }
-  F.perm = min_degree_order(nb, adj);
+  if (int64_t(g_rcs_order.size()) == nb)
+    F.perm = g_rcs_order;
+  else
+    F.perm = min_degree_order(nb, adj);
   std::vector<int64_t> inv(nb);
   for (int64_t i = 0; i < nb; ++i) inv[F.perm[i]] = i;
   F.dim.resize(nb);
@@ -642,6 +659,7 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
   F.Ljj.assign(nb, {});
   F.d.assign(nb, {});
   std::vector<std::map<int64_t, std::vector<double>>> Lmap(nb);  // per column
+  FactorStat stat{0, 0, 1.0};
   for (int64_t j = 0; j < nb; ++j) {
     const int64_t dj = F.dim[j];
     std::map<int64_t, std::vector<double>> C;  // row -> block (di x dj)
@@ -696,6 +714,15 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
     F.d[j].assign(dj, 0.0);
     if (!ldlt_decompose(int(dj), C[j].data(), F.Ljj[j].data(), F.d[j].data(), tol_rel))
       return false;
+    {
+      double mx = 1.0;
+      for (double v : F.d[j]) mx = std::max(mx, std::fabs(v));
+      for (double v : F.d[j]) {
+        if (v == 0.0) ++stat.zero;
+        else stat.min_ratio = std::min(stat.min_ratio, std::fabs(v) / mx);
+        if (v < 0.0) ++stat.neg;
+      }
+    }
     for (auto& [i, Cb] : C) {
       if (i <= j) continue;
       ldlt_right_solve(int(F.dim[i]), int(dj), F.Ljj[j].data(), F.d[j].data(), Cb.data());
@@ -707,6 +734,7 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
       F.lrows[j].push_back(i);
       F.lval[j].push_back(b);
     }
+  g_factor_stats.push_back(stat);
   return true;
 }
 
@@ -1127,6 +1155,33 @@ double g_step_seconds = 0.0;  // wall time spent inside step() by the last optim
 extern "C" {
 
 const char* oracle_last_error(void) { return g_error.c_str(); }
+
+// Sets the RCS block ordering used by every later factorization (perm[new] = old over the
+// free blocks, in the layout order of the variables); n = 0 restores minimum degree.
+// Returns 1 if perm is not a permutation of 0..n-1.
+int32_t oracle_set_rcs_order(const int64_t* perm, int64_t n) {
+  std::vector<char> seen(size_t(std::max<int64_t>(n, 0)), 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (perm[i] < 0 || perm[i] >= n || seen[perm[i]]) return 1;
+    seen[perm[i]] = 1;
+  }
+  g_rcs_order.assign(perm, perm + n);
+  return 0;
+}
+
+// Pivot statistics of the block LDLt factorizations since the last reset (one entry per
+// factorization, oldest first); returns how many were written (<= capacity).
+int64_t oracle_factor_stats(int64_t* zero, int64_t* neg, double* min_ratio, int64_t capacity) {
+  const int64_t n = std::min<int64_t>(capacity, int64_t(g_factor_stats.size()));
+  for (int64_t i = 0; i < n; ++i) {
+    if (zero) zero[i] = g_factor_stats[i].zero;
+    if (neg) neg[i] = g_factor_stats[i].neg;
+    if (min_ratio) min_ratio[i] = g_factor_stats[i].min_ratio;
+  }
+  return n;
+}
+
+void oracle_reset_factor_stats(void) { g_factor_stats.clear(); }
 
 double oracle_last_step_seconds(void) { return g_step_seconds; }
 

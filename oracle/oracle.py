@@ -112,6 +112,12 @@ def lib() -> C.CDLL:
         L.oracle_huber_weight.argtypes = [C.c_double, C.c_double]
         L.oracle_inverse3.restype = C.c_int32
         L.oracle_inverse3.argtypes = [DP, DP]
+        L.oracle_set_rcs_order.restype = C.c_int32
+        L.oracle_set_rcs_order.argtypes = [I64P, C.c_int64]
+        L.oracle_factor_stats.restype = C.c_int64
+        L.oracle_factor_stats.argtypes = [I64P, I64P, DP, C.c_int64]
+        L.oracle_reset_factor_stats.restype = None
+        L.oracle_reset_factor_stats.argtypes = []
         _lib = L
     return _lib
 
@@ -196,6 +202,30 @@ def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trac
     solution = dict(initial_cost=sol.initial_cost, final_cost=sol.final_cost, num_steps=sol.num_steps,
                     termination=sol.termination, final_max_abs_neg_grad=sol.final_max_abs_neg_grad)
     return solution, {f[0]: o for f, o in zip(families, outs)}, recs
+
+
+def set_rcs_order(perm=None) -> None:
+    """Eliminate the RCS blocks in the given order (perm[new] = old free block) in every later
+    factorization; None restores the oracle's minimum-degree order."""
+    if perm is None:
+        lib().oracle_set_rcs_order(None, 0)
+        return
+    p = np.ascontiguousarray(perm, dtype=np.int64)
+    if lib().oracle_set_rcs_order(p.ctypes.data_as(I64P), p.size) != 0:
+        raise ValueError("not a permutation")
+
+
+def factor_stats(reset: bool = True) -> List[dict]:
+    """Pivot statistics of the block LDLt factorizations since the last reset: zeroed and
+    negative pivots, and the smallest kept |d| / max(1, max |d|) of a diagonal block."""
+    cap = 100000
+    z = np.zeros(cap, dtype=np.int64)
+    n_ = np.zeros(cap, dtype=np.int64)
+    r = np.zeros(cap)
+    k = lib().oracle_factor_stats(z.ctypes.data_as(I64P), n_.ctypes.data_as(I64P), _p(r), cap)
+    if reset:
+        lib().oracle_reset_factor_stats()
+    return [{"zero": int(z[i]), "neg": int(n_[i]), "min_ratio": float(r[i])} for i in range(k)]
 
 
 def last_step_seconds() -> float:

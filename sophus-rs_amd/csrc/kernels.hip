@@ -115,16 +115,22 @@ __global__ void k_cam_cache(int npose, const double* __restrict__ poses, double*
 
 __device__ __forceinline__ void to_camera(const double* __restrict__ cw, const double x[3],
                                           double xc[3]) {
+#pragma clang fp contract(off)
   xc[0] = cw[0] * x[0] + cw[1] * x[1] + cw[2] * x[2] + cw[9];
   xc[1] = cw[3] * x[0] + cw[4] * x[1] + cw[5] * x[2] + cw[10];
   xc[2] = cw[6] * x[0] + cw[7] * x[1] + cw[8] * x[2] + cw[11];
 }
 
 // A = diag(fx, fy) * dx_proj_x(x_c) * R_cw  (perspective.rs:54-72, affine.rs:77-85).
+// Contraction is off here and in gram(): the point pass, the camera pass and the
+// back-substitution each evaluate the same observation's A and M = A^T A, and the mixed
+// path relies on those values being bitwise equal in all three (H_ll - M_o cancels
+// exactly only then).
 template <typename T>
 __device__ __forceinline__ void obs_jacobian(const double* __restrict__ cw,
                                              const double* __restrict__ in, const double xc[3],
                                              T A[6]) {
+#pragma clang fp contract(off)
   const T z = T(xc[2]);
   const T iz = T(1) / z;
   const T zz = z * z;
@@ -140,6 +146,7 @@ __device__ __forceinline__ void obs_jacobian(const double* __restrict__ cw,
 // M = A^T A (symmetric 3x3, stored full row-major)
 template <typename T>
 __device__ __forceinline__ void gram(const T A[6], T M[9]) {
+#pragma clang fp contract(off)
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -171,8 +178,11 @@ __global__ void __launch_bounds__(kThreads)
   const int b = live ? gid : p.npt - 1;  // trailing lanes stay for the wavefront max
   const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
   const bool active = p.pt_block[b] >= 0;
-  T H[9] = {T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-  T s[3] = {T(0), T(0), T(0)};
+  // H_ll and s are summed in FP64 in both precisions (of FP32 terms in the mixed path), so
+  // that H_ll - M_o in the camera pass is exactly the other terms' sum plus the damping and
+  // each point's Schur contribution stays positive semi-definite however small nu gets
+  double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double s[3] = {0, 0, 0};
   for (int o = p.pt_ptr[b]; live && active && o < p.pt_ptr[b + 1]; ++o) {
     const double* c = cw + 12 * p.obs_pose[o];
     const double* in = intr + 4 * p.obs_intr[o];
@@ -183,15 +193,15 @@ __global__ void __launch_bounds__(kThreads)
     obs_residual<T>(in, p.obs_uv + 2 * o, xc, p.robust, p.delta, A, v);
     gram(A, M);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) H[k] += M[k];
+    for (int k = 0; k < 9; ++k) H[k] += double(M[k]);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) s[k] += v[k];
+    for (int k = 0; k < 3; ++k) s[k] += double(v[k]);
   }
   {
-    T g = T(0);
+    double g = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) g = fmax(g, fabs(s[k]));
-    wave_atomic_max_abs(&sc->max_grad_bits, live && active ? double(g) : 0.0);
+    wave_atomic_max_abs(&sc->max_grad_bits, live && active ? g : 0.0);
   }
   if (!live) return;
   double* rec = prec + kPtRec * int64_t(b);
@@ -200,7 +210,7 @@ __global__ void __launch_bounds__(kThreads)
     for (int k = 0; k < kPtRec; ++k) rec[k] = k < 3 ? x[k] : 0.0;
     return;
   }
-  const T damp = T(nu * double(p.pt_nu[b]));
+  const double damp = nu * double(p.pt_nu[b]);
   H[0] += damp;
   H[4] += damp;
   H[8] += damp;
@@ -226,13 +236,12 @@ __global__ void __launch_bounds__(kThreads)
     inv[8] = (m11 * m22 - m21 * m12) * rdet;
   }
   double yv[3];
-  for (int k = 0; k < 3; ++k)
-    yv[k] = inv[3 * k] * double(s[0]) + inv[3 * k + 1] * double(s[1]) + inv[3 * k + 2] * double(s[2]);
+  for (int k = 0; k < 3; ++k) yv[k] = inv[3 * k] * s[0] + inv[3 * k + 1] * s[1] + inv[3 * k + 2] * s[2];
   const double rv[kPtRec] = {x[0],   x[1],   x[2],   inv[0], inv[1], inv[2], inv[4], inv[5], inv[8],
                              yv[0],  yv[1],  yv[2],  m11,    m12,    m13,    m22,    m23,    m33};
 #pragma unroll
   for (int k = 0; k < kPtRec; ++k) rec[k] = rv[k];
-  for (int k = 0; k < 3; ++k) s_out[3 * b + k] = double(s[k]);
+  for (int k = 0; k < 3; ++k) s_out[3 * b + k] = s[k];
 }
 
 // P_i^T N P_j for P = [hat(-x) | I] and a 3x3 N:
@@ -442,8 +451,16 @@ __global__ void __launch_bounds__(kThreads)
       }
       to_camera(c, xd, xc);
       obs_jacobian<double>(c, ini, xc, Ad);
-      gram(Ad, Md);
       obs_residual<double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
+      // M_o as the point pass and the back-substitution form it (FP32 Jacobian and Gram),
+      // held in FP64: H_ll sums exactly these terms, so R = H_ll - M_o below is the other
+      // terms plus the damping, and U, the reduced rhs and the own term all describe one
+      // matrix
+      T Ai[6], Mi[9];
+      obs_jacobian<T>(c, ini, xc, Ai);
+      gram(Ai, Mi);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Md[e] = double(Mi[e]);
       // record: U_o row-major, then the lift point (zero for a constant point: it has no
       // cross pairs)
       // the point's packed record: H_ll^-1 upper at 3, y at 9, damped H_ll upper at 12
@@ -469,10 +486,7 @@ __global__ void __launch_bounds__(kThreads)
       // own (o, o) term in T with the FP64 difference R = H_ll - M_o:
       // N = M_o H_ll^-1 R (M_o alone for a constant point)
       const T x[3] = {T(xl[0]), T(xl[1]), T(xl[2])};
-      T Ai[6], Mi[9], N[9];
-#pragma unroll
-      for (int e = 0; e < 6; ++e) Ai[e] = T(Ad[e]);
-      gram(Ai, Mi);
+      T N[9];
       if (active) {
         const double* yb = rb + 9;
 #pragma unroll

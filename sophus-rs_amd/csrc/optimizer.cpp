@@ -655,6 +655,42 @@ sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
   });
 }
 
+static void copy_order(const Symbolic& S, int64_t* perm, int64_t capacity, int64_t* num_blocks) {
+  if (num_blocks) *num_blocks = S.nb;
+  if (!perm) return;
+  if (capacity < S.nb) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
+  for (int32_t i = 0; i < S.nb; ++i) perm[i] = S.perm[i];
+}
+
+sgpu_status sgpu_analyze_rcs_order(const sgpu_var_family* families, int64_t num_families,
+                                   const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                   int64_t* perm, int64_t capacity, int64_t* num_blocks) {
+  return guarded([&] {
+    const HostProblem P = build_host_problem(families, num_families, cost_fns, num_cost_fns);
+    copy_order(analyze(P), perm, capacity, num_blocks);
+  });
+}
+
+sgpu_status sgpu_optimizer_rcs_order(const sgpu_optimizer* opt, int64_t* perm, int64_t capacity,
+                                     int64_t* num_blocks) {
+  return guarded([&] {
+    if (!opt) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null optimizer");
+    copy_order(opt->S, perm, capacity, num_blocks);
+  });
+}
+
+sgpu_status sgpu_optimizer_rcs_pivots(sgpu_optimizer* opt, double* d, int64_t capacity) {
+  return guarded([&] {
+    if (!opt || !d) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    if (opt->iterative()) throw Error(SGPU_ERR_UNSUPPORTED, "the PCG solvers have no pivots");
+    const int64_t need = int64_t(6) * opt->S.nb;
+    if (capacity < need) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
+    SGPU_HIP(hipSetDevice(opt->device));
+    SGPU_HIP(hipMemcpyAsync(d, opt->df.piv_d, need * 8, hipMemcpyDeviceToHost, opt->stream));
+    SGPU_HIP(hipStreamSynchronize(opt->stream));
+  });
+}
+
 int32_t sgpu_optimizer_kernel_times(const sgpu_optimizer* opt, const char** names, int64_t* counts,
                                     double* total_ms, int32_t capacity) {
   if (!opt) return 0;

@@ -213,15 +213,6 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
   out.reserve(n);
   std::vector<int32_t> mark(n, -1), level(n, -1);
   int32_t stamp = 0;
-  // explicit stack of node sets to avoid deep recursion
-  struct Task {
-    std::vector<int32_t> nodes;
-    std::vector<int32_t> sep;  // appended after both halves (pending separator)
-    int stage;
-  };
-  // Recursive lambda via std::function-free approach
-  std::vector<int32_t> all(n);
-  std::iota(all.begin(), all.end(), 0);
   auto bfs = [&](const std::vector<int32_t>& nodes, int32_t tag, int32_t src,
                  std::vector<int32_t>& orderv) {
     // nodes carry mark == tag; returns BFS order, sets level[]
@@ -256,7 +247,6 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
     bfs(nodes, tag, nodes[0], orderv);
     if (orderv.size() < nodes.size()) {  // disconnected: split into components
       std::vector<std::vector<int32_t>> comps;
-      std::vector<char> seen(0);
       std::vector<int32_t> rest = nodes;
       while (!rest.empty()) {
         const int32_t t2 = ++stamp;
@@ -306,6 +296,8 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
     std::sort(sep.begin(), sep.end());
     out.insert(out.end(), sep.begin(), sep.end());
   };
+  std::vector<int32_t> all(n);
+  std::iota(all.begin(), all.end(), 0);
   rec(all);
   return out;
 }

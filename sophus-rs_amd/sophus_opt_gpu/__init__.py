@@ -198,6 +198,10 @@ _SIGNATURES = {
     "sgpu_optimizer_rcs_stats": (C.c_int, [C.c_void_p, C.POINTER(_RcsStats)]),
     "sgpu_analyze": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
                                C.POINTER(_RcsStats)]),
+    "sgpu_analyze_rcs_order": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                                         _I64P, C.c_int64, _I64P]),
+    "sgpu_optimizer_rcs_order": (C.c_int, [C.c_void_p, _I64P, C.c_int64, _I64P]),
+    "sgpu_optimizer_rcs_pivots": (C.c_int, [C.c_void_p, _DP, C.c_int64]),
     "sgpu_optimizer_kernel_times": (C.c_int32, [C.c_void_p, C.POINTER(C.c_char_p), _I64P, _DP, C.c_int32]),
     "sgpu_optimizer_reset_kernel_times": (None, [C.c_void_p]),
     "sgpu_optimizer_set_kernel_timing": (None, [C.c_void_p, C.c_int32]),
@@ -629,6 +633,21 @@ class Optimizer:
         _check(library().sgpu_optimizer_rcs_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
 
+    def rcs_order(self) -> np.ndarray:
+        """Elimination order of the reduced camera system, perm[new] = old free block."""
+        n = C.c_int64()
+        _check(library().sgpu_optimizer_rcs_order(self._h, None, 0, C.byref(n)))
+        out = np.empty(n.value, dtype=np.int64)
+        _check(library().sgpu_optimizer_rcs_order(self._h, out.ctypes.data_as(_I64P), out.size, C.byref(n)))
+        return out
+
+    def rcs_pivots(self) -> np.ndarray:
+        """Pivots d of the last direct factorization, (blocks x 6) in elimination order."""
+        n = self.rcs_stats()["num_free_blocks"]
+        out = np.empty((n, 6))
+        _check(library().sgpu_optimizer_rcs_pivots(self._h, out.ctypes.data_as(_DP), out.size))
+        return out
+
     def set_kernel_timing(self, mode: int) -> None:
         """0 off, 1 per phase kernel, 2 also per panel-step launch ("mf_step")."""
         library().sgpu_optimizer_set_kernel_timing(self._h, int(mode))
@@ -657,6 +676,18 @@ def analyze(variables: VarFamilies, cost_fns: Sequence[CostFn]) -> dict:
     s = _RcsStats()
     _check(library().sgpu_analyze(m.fams, len(m.names), m.costs, len(cost_fns), C.byref(s)))
     return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
+
+
+def analyze_rcs_order(variables: VarFamilies, cost_fns: Sequence[CostFn]) -> np.ndarray:
+    """Host-only: the elimination order the GPU factorization uses, perm[new] = old free block."""
+    m = _Marshal(variables, cost_fns)
+    n = C.c_int64()
+    _check(library().sgpu_analyze_rcs_order(m.fams, len(m.names), m.costs, len(cost_fns), None, 0,
+                                            C.byref(n)))
+    out = np.empty(n.value, dtype=np.int64)
+    _check(library().sgpu_analyze_rcs_order(m.fams, len(m.names), m.costs, len(cost_fns),
+                                            out.ctypes.data_as(_I64P), out.size, C.byref(n)))
+    return out
 
 
 def optimize_nlls(variables: VarFamilies, cost_fns: Sequence[CostFn],
