@@ -343,6 +343,21 @@ sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
                                   const int64_t* row_idx, const double* values,
                                   const double* rhs, double* x);
 
+/* Allocation-free variant of sgpu_block_ldlt_solve.  It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * results and errors, on the current device, but it calls no hipMalloc/hipFree: every
+ * device buffer (pattern, fronts, factor, work vectors) is carved from one static array of
+ * SGPU_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the process ends.  A
+ * system whose buffers do not fit returns SGPU_ERR_OUT_OF_MEMORY (sgpu_last_error states
+ * the bytes needed and the limit) before touching the device.  Calls on one GPU share the
+ * array: they are serialised within one process; calls that overlap on one GPU from
+ * several processes race. */
+#define SGPU_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+sgpu_status sgpu_block_ldlt_solve_noalloc(int64_t num_blocks, const int64_t* col_ptr,
+                                          const int64_t* row_idx, const double* values,
+                                          const double* rhs, double* x);
+
 /* Kernel (d'), no reference counterpart: block-Jacobi preconditioned conjugate
  * gradients on the same input, FP64, x0 = 0, stopping at |r| <= relative_tolerance |b|
  * or max_iterations.  *iterations receives the iterations taken (may be NULL).

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "api_internal.hpp"
+#include "block_solve.hpp"
 #include "kernels.hpp"
 #include "learned.hpp"
 #include "pcg.hpp"
@@ -47,10 +48,14 @@ struct DeviceBufs {
     return static_cast<T*>(p);
   }
   template <typename T>
-  T* up(const std::vector<T>& v) {
-    T* d = alloc<T>(v.size());
-    if (!v.empty()) SGPU_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  T* up(const T* host, size_t n) {
+    T* d = alloc<T>(n);
+    if (n) SGPU_HIP(hipMemcpy(d, host, n * sizeof(T), hipMemcpyHostToDevice));
     return d;
+  }
+  template <typename T>
+  T* up(const std::vector<T>& v) {
+    return up<T>(v.data(), v.size());
   }
 };
 
@@ -64,7 +69,7 @@ void require_device() {
 namespace sgpu {
 // Parses lower blocks in CSC-by-block-column (see sgpu_block_ldlt_solve) into the slot
 // pattern (rows of lower blocks) and row-major blocks in slot order.
-static void parse_lower_blocks(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,
+void parse_lower_blocks(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,
                                const double* values, const double* rhs, const double* x,
                                Symbolic& S, std::vector<double>& blocks) {
   if (num_blocks <= 0 || !col_ptr || !row_idx || !values || !rhs || !x)
@@ -169,67 +174,11 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     Symbolic S;
     std::vector<double> blocks;
     parse_lower_blocks(num_blocks, col_ptr, row_idx, values, rhs, x, S, blocks);
-    const int32_t nb = S.nb;
     analyze_rcs(S);
     require_device();
-    // S blocks in slot order (row-major 6x6), scattered into the fronts on the device
     DeviceBufs bufs;
-    DevProblem dp{};
-    dp.nb = nb;
-    dp.s_off = bufs.up(S.s_off);
-    dp.s_ld = bufs.up(S.s_ld);
-    DevFactor df{};
-    df.nb = nb;
-    df.nsn = S.nsn;
-    df.perm = bufs.up(S.perm);
-    df.sn_c0 = bufs.up(S.sn_c0);
-    df.sn_nc = bufs.up(S.sn_nc);
-    df.sn_nf = bufs.up(S.sn_nf);
-    df.sn_off = bufs.up(S.sn_off);
-    df.sn_loff = bufs.up(S.sn_loff);
-    df.sn_rows_ptr = bufs.up(S.sn_rows_ptr);
-    df.sn_rows = bufs.up(S.sn_rows);
-    df.sn_parent = bufs.up(S.sn_parent);
-    df.sn_upos_ptr = bufs.up(S.sn_upos_ptr);
-    df.sn_upos = bufs.up(S.sn_upos);
-    df.lvl_sn = bufs.up(S.lvl_sn);
-    df.launch_tiles = bufs.up(S.launch_tiles);
-    df.leaves = bufs.up(S.leaves);
-    df.sn_hand = bufs.up(S.sn_hand);
-    df.sn_toff = bufs.up(S.sn_toff);
-    df.sn_poff = bufs.up(S.sn_poff);
-    df.panel_items = bufs.up(S.panel_items);
-    df.sn_woff = bufs.up(S.sn_woff);
-    df.solve_cols = bufs.up(S.solve_cols);
-    df.sn_uoff = bufs.up(S.sn_uoff);
-    df.usolve = bufs.alloc<double>(size_t(S.u_elems));
-    df.front = bufs.alloc<double>(size_t(S.front_elems));
-    df.lbuf = bufs.alloc<double>(size_t(S.l_elems));
-    df.ltbuf = bufs.alloc<double>(size_t(S.t_elems));
-    df.tinv = bufs.alloc<double>(size_t(S.p_elems));
-    df.wbuf = bufs.alloc<double>(size_t(S.w_elems));
-    df.piv_L = bufs.alloc<double>(size_t(36) * nb);
-    df.piv_d = bufs.alloc<double>(size_t(6) * nb);
-    df.piv_M = bufs.alloc<double>(size_t(36) * nb);
-    df.front_elems = S.front_elems;
-    df.host = &S;
-    double* d_blocks = bufs.up(blocks);
-    std::vector<double> b(rhs, rhs + 6 * num_blocks);
-    double* d_rhs = bufs.up(b);
-    double* d_y = bufs.alloc<double>(size_t(6) * nb);
-    double* d_x = bufs.alloc<double>(size_t(6) * nb);
-    DevScalars* d_sc = bufs.alloc<DevScalars>(1);
-    launch_zero_scalars(nullptr, d_sc);
-    launch_zero_fronts(nullptr, df);
-    launch_scatter_s(nullptr, dp, int64_t(S.s_col.size()), d_blocks, df);
-    launch_rcs_factor(nullptr, df, d_sc, d_rhs, d_y);
-    launch_rcs_solve(nullptr, df, d_y, d_x);
-    SGPU_HIP(hipGetLastError());
-    SGPU_HIP(hipDeviceSynchronize());
-    DevScalars sc;
-    SGPU_HIP(hipMemcpy(&sc, d_sc, sizeof(sc), hipMemcpyDeviceToHost));
-    if (sc.bad_pivot) throw Error(SGPU_ERR_NON_FINITE, "non-finite pivot");
-    SGPU_HIP(hipMemcpy(x, d_x, size_t(6) * nb * 8, hipMemcpyDeviceToHost));
+    BlockSolveBufs b = bind_block_solve(S, blocks, rhs, bufs);
+    run_block_solve(S, b, x);
   });
 }
 

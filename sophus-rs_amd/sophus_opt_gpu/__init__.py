@@ -210,6 +210,7 @@ _SIGNATURES = {
                                           _DP, _DP, _DP, _DP, _DP]),
     "sgpu_eval_pose_graph3_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
+    "sgpu_block_ldlt_solve_noalloc": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_block_pcg_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP, C.c_double, C.c_int64,
                                        _I64P]),
     "sgpu_precond_features": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _FP]),
@@ -759,14 +760,19 @@ def _lower_blocks(A_dense: np.ndarray):
             np.ascontiguousarray(np.stack(vals), dtype=np.float64))
 
 
-def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray) -> np.ndarray:
-    """Kernel (d): block-sparse LDLt solve of a symmetric matrix with 6x6 blocks on the GPU."""
+NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024  # SGPU_NOALLOC_SCRATCH_BYTES
+
+
+def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray, noalloc: bool = False) -> np.ndarray:
+    """Kernel (d): block-sparse LDLt solve of a symmetric matrix with 6x6 blocks on the GPU.
+    noalloc=True calls the allocation-free variant (device buffers from the library's static
+    scratch array of NOALLOC_SCRATCH_BYTES per GPU)."""
     nb, col_ptr, rows, vals = _lower_blocks(A_dense)
     rhs = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
     x = np.empty(6 * nb)
-    _check(library().sgpu_block_ldlt_solve(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P),
-                                           vals.ctypes.data_as(_DP), rhs.ctypes.data_as(_DP),
-                                           x.ctypes.data_as(_DP)))
+    fn = library().sgpu_block_ldlt_solve_noalloc if noalloc else library().sgpu_block_ldlt_solve
+    _check(fn(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P), vals.ctypes.data_as(_DP),
+              rhs.ctypes.data_as(_DP), x.ctypes.data_as(_DP)))
     return x
 
 
