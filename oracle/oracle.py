@@ -17,6 +17,9 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
+# variant -> library: "" is the oracle; "fma" its rounding twin (same source, FMA contraction
+# on), used to measure how far the oracle itself moves under a rounding-only change
+VARIANTS = {"": LIB, "fma": HERE / "liboracle_fma.so"}
 
 
 def build() -> Path:
@@ -60,15 +63,15 @@ class StepRecordC(C.Structure):
 
 DP = C.POINTER(C.c_double)
 I64P = C.POINTER(C.c_int64)
-_lib = None
+_libs: Dict[str, C.CDLL] = {}
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not LIB.exists():
+def lib(variant: str = "") -> C.CDLL:
+    if variant not in _libs:
+        path = VARIANTS[variant]
+        if not path.exists():
             build()
-        L = C.CDLL(str(LIB))
+        L = C.CDLL(str(path))
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_last_step_seconds.restype = C.c_double
         L.oracle_optimize_nlls.restype = C.c_int
@@ -118,8 +121,8 @@ def lib() -> C.CDLL:
         L.oracle_factor_stats.argtypes = [I64P, I64P, DP, C.c_int64]
         L.oracle_reset_factor_stats.restype = None
         L.oracle_reset_factor_stats.argtypes = []
-        _lib = L
-    return _lib
+        _libs[variant] = L
+    return _libs[variant]
 
 
 def _p(a: np.ndarray):
@@ -183,8 +186,10 @@ def default_params(**kw) -> OptParamsC:
     return p
 
 
-def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trace: int = 1000):
-    """Returns (solution dict, {name: final members}, trace list) or raises with the status."""
+def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trace: int = 1000,
+                  variant: str = ""):
+    """Returns (solution dict, {name: final members}, trace list) or raises with the status.
+    variant "fma" runs the oracle's rounding twin."""
     m = Marshal(families, costs)
     params = params or default_params()
     outs = [np.empty_like(np.asarray(f[3], dtype=np.float64)) for f in families]
@@ -192,8 +197,8 @@ def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trac
     sol = SolutionC()
     trace = (StepRecordC * max_trace)()
     n = C.c_int64()
-    st = lib().oracle_optimize_nlls(m.fams, m.n_fam, m.costs, m.n_cost, C.byref(params), ptrs, C.byref(sol),
-                                    trace, max_trace, C.byref(n))
+    st = lib(variant).oracle_optimize_nlls(m.fams, m.n_fam, m.costs, m.n_cost, C.byref(params), ptrs,
+                                           C.byref(sol), trace, max_trace, C.byref(n))
     if st != 0:
         return st, None, None
     recs = [dict(merit=t.merit, lm_damping=t.lm_damping, new_merit=t.new_merit, gain_ratio=t.gain_ratio,
@@ -204,27 +209,27 @@ def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trac
     return solution, {f[0]: o for f, o in zip(families, outs)}, recs
 
 
-def set_rcs_order(perm=None) -> None:
+def set_rcs_order(perm=None, variant: str = "") -> None:
     """Eliminate the RCS blocks in the given order (perm[new] = old free block) in every later
     factorization; None restores the oracle's minimum-degree order."""
     if perm is None:
-        lib().oracle_set_rcs_order(None, 0)
+        lib(variant).oracle_set_rcs_order(None, 0)
         return
     p = np.ascontiguousarray(perm, dtype=np.int64)
-    if lib().oracle_set_rcs_order(p.ctypes.data_as(I64P), p.size) != 0:
+    if lib(variant).oracle_set_rcs_order(p.ctypes.data_as(I64P), p.size) != 0:
         raise ValueError("not a permutation")
 
 
-def factor_stats(reset: bool = True) -> List[dict]:
+def factor_stats(reset: bool = True, variant: str = "") -> List[dict]:
     """Pivot statistics of the block LDLt factorizations since the last reset: zeroed and
     negative pivots, and the smallest kept |d| / max(1, max |d|) of a diagonal block."""
     cap = 100000
     z = np.zeros(cap, dtype=np.int64)
     n_ = np.zeros(cap, dtype=np.int64)
     r = np.zeros(cap)
-    k = lib().oracle_factor_stats(z.ctypes.data_as(I64P), n_.ctypes.data_as(I64P), _p(r), cap)
+    k = lib(variant).oracle_factor_stats(z.ctypes.data_as(I64P), n_.ctypes.data_as(I64P), _p(r), cap)
     if reset:
-        lib().oracle_reset_factor_stats()
+        lib(variant).oracle_reset_factor_stats()
     return [{"zero": int(z[i]), "neg": int(n_[i]), "min_ratio": float(r[i])} for i in range(k)]
 
 
