@@ -247,6 +247,33 @@ sgpu_status sgpu_optimizer_eval_cost_at(sgpu_optimizer* opt, const double* const
  * nu = 0 unless skip_final_hessian. */
 sgpu_status sgpu_optimizer_build_solution(sgpu_optimizer* opt, sgpu_solution* solution);
 
+/* Block (row, col) of the inverse of the damped Hessian of the handle's final
+ * linearization, by family name and member index: SchurFactor::inverse_block
+ * (crates/sophus_solver/src/ldlt/schur_ldlt.rs:814-958) as DampedHessian::
+ * inverse_block_with_damping reaches it (crates/sophus_opt/src/nlls/damped_hessian.rs:98).
+ * The matrix is H + nu N at the current variables with build_solution's nu (0, or the
+ * current nu with skip_final_hessian); it is linearized and factored on the device at the
+ * first query after a step and reused by later queries.  free x free: S^+ (the min-norm
+ * pseudo-inverse where S is singular, e.g. along a gauge direction); free x marg:
+ * -(S^+ H_fm H_mm^-1); marg x free: its transpose; marg x marg:
+ * H_mm^-1 delta + H_mm^-1 H_mf S^+ H_fm H_mm^-1.  out receives the block row-major
+ * (6 or 3 rows by 6 or 3 columns).  Conditioned families and constant members have no block
+ * (SGPU_ERR_INVALID_ARGUMENT; the reference returns None).  Direct solver only. */
+sgpu_status sgpu_optimizer_inverse_block(sgpu_optimizer* opt, const char* row_family,
+                                         int64_t row_member, const char* col_family,
+                                         int64_t col_member, double* out, int64_t capacity);
+
+/* OptimizationSolution::covariance_block (crates/sophus_opt/src/nlls/optimizer_types.rs:
+ * 335-373): a block of the min-norm inverse of the undamped final Hessian H.  Where H has
+ * full rank this is H^-1, which the Schur formulas of sgpu_optimizer_inverse_block give
+ * exactly (at nu = 0).  A rank-deficient H (an unfixed gauge: a non-positive pivot of S, or
+ * a pivot below 1e-9 of the largest) returns SGPU_ERR_UNSUPPORTED:
+ * the reference then takes the min-norm inverse of the whole of H, which differs from the
+ * Schur-complement blocks.  Requires skip_final_hessian = 0. */
+sgpu_status sgpu_optimizer_covariance_block(sgpu_optimizer* opt, const char* row_family,
+                                            int64_t row_member, const char* col_family,
+                                            int64_t col_member, double* out, int64_t capacity);
+
 /* Structure of the reduced camera system of this problem. */
 typedef struct sgpu_rcs_stats {
   int64_t num_free_blocks;   /* RCS block rows (6x6 blocks) */

@@ -1884,6 +1884,91 @@ __global__ void __launch_bounds__(kTriMax)
   if (own) y[6 * c0 + c] = x;
 }
 
+// Forward solve of one level's nodes (one workgroup each) for a right-hand side already in
+// the nodes' front row order: first the children's finished update rows are added into the
+// node's (children one after another, so the adds are plain), then panel by panel y_P =
+// L_PP^-1 b_P (the stored L_PP^-T), z_P = blockdiag(M_j) y_P and b_r -= L_r y_P for every
+// row below the panel, the same eliminations the factorization applies to the step's own
+// right-hand side.
+__global__ void __launch_bounds__(kThreads)
+    k_mf_fwd(DevFactor f, const int32_t* __restrict__ nodes, double* __restrict__ z) {
+  const int X = nodes[blockIdx.x];
+  const int F = 6 * f.sn_nf[X], K = 6 * f.sn_nc[X], c0 = f.sn_c0[X];
+  double* b = f.usolve + f.sn_uoff[X];
+  const double* L = f.lbuf + f.sn_loff[X];
+  const double* Tn = f.tinv + f.sn_poff[X];
+  const int tid = threadIdx.x;
+  __shared__ double sb[kPW], sy[kPW];
+  for (int ci = f.sn_child_ptr[X]; ci < f.sn_child_ptr[X + 1]; ++ci) {
+    const int C = f.sn_child[ci];
+    const int FC = 6 * f.sn_nf[C], KC = 6 * f.sn_nc[C];
+    const double* bc = f.usolve + f.sn_uoff[C];
+    for (int u = tid; u < FC - KC; u += blockDim.x) b[parent_row(f, C, u)] += bc[KC + u];
+    __syncthreads();
+  }
+  const int npan = (K + kPW - 1) / kPW;
+  for (int p = 0; p < npan; ++p) {
+    const int k0 = kPW * p, np = min(kPW, K - k0);
+    const double* Tg = Tn + int64_t(p) * kPW * kPW;
+    if (tid < np) sb[tid] = b[k0 + tid];
+    __syncthreads();
+    if (tid < np) {
+      double y = 0.0;
+      for (int t = 0; t < np; ++t) y += Tg[t * kPW + tid] * sb[t];  // L_PP^-T (t, c) = L_PP^-1 (c, t)
+      sy[tid] = y;
+    }
+    __syncthreads();
+    if (tid < np) {
+      const int jb = tid / 6, a = tid % 6;
+      const double* M = f.piv_M + 36 * int64_t(f.sn_c0[X] + k0 / 6 + jb);
+      double zz = 0.0;
+#pragma unroll
+      for (int t = 0; t < 6; ++t) zz += M[6 * a + t] * sy[6 * jb + t];
+      z[6 * int64_t(c0) + k0 + tid] = zz;
+    }
+    for (int r = k0 + np + tid; r < F; r += blockDim.x) {
+      double acc = 0.0;
+      for (int t = 0; t < np; ++t) acc += L[int64_t(k0 + t) * F + r] * sy[t];
+      b[r] -= acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Coupling blocks of one point with its cameras (covariance queries): one thread per
+// observation.  H_{c,b} = (A P)^T (-A) = -P^T M with P = [hat(-x) | I], M = A^T A.
+__global__ void k_point_coupling(DevProblem p, const double* __restrict__ points,
+                                 const double* __restrict__ intr, const double* __restrict__ cw,
+                                 const double* __restrict__ prec, int b, double* __restrict__ out,
+                                 int32_t* __restrict__ cam, double* __restrict__ hinv) {
+  if (threadIdx.x == 0) {
+    const double* h = prec + kPtRec * int64_t(b) + 3;
+    const double hv[9] = {h[0], h[1], h[2], h[1], h[3], h[4], h[2], h[4], h[5]};
+    for (int e = 0; e < 9; ++e) hinv[e] = hv[e];
+  }
+  for (int q = threadIdx.x; p.pt_ptr[b] + q < p.pt_ptr[b + 1]; q += blockDim.x) {
+    const int o = p.pt_ptr[b] + q;
+    const int pm = p.obs_pose[o];
+    cam[q] = p.pose_block[pm];
+    const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
+    const double* c = cw + 12 * pm;
+    double xc[3], A[6], M[9];
+    to_camera(c, x, xc);
+    obs_jacobian<double>(c, intr + 4 * p.obs_intr[o], xc, A);
+    gram(A, M);
+    // P^T M: rows 0-2 hat(-x)^T M = hat(x) M, rows 3-5 M
+    double* H = out + 18 * q;
+    for (int j = 0; j < 3; ++j) {
+      H[0 * 3 + j] = -(-x[2] * M[3 + j] + x[1] * M[6 + j]);
+      H[1 * 3 + j] = -(x[2] * M[j] - x[0] * M[6 + j]);
+      H[2 * 3 + j] = -(-x[1] * M[j] + x[0] * M[3 + j]);
+      H[9 + j] = -M[j];
+      H[12 + j] = -M[3 + j];
+      H[15 + j] = -M[6 + j];
+    }
+  }
+}
+
 __global__ void k_permute_in(int nb, const int32_t* __restrict__ perm, const double* __restrict__ src,
                              double* __restrict__ dst) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2444,6 +2529,27 @@ void launch_rcs_solve(hipStream_t s, const DevFactor& f, double* y, double* dx_c
                        f.lvl_sn + S.lvl_ptr[l], y);
   }
   hipLaunchKernelGGL(k_permute_out, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, y, dx_cam);
+}
+
+void launch_rcs_forward(hipStream_t s, const DevFactor& f, const double* b_nat, double* b_perm,
+                        double* z) {
+  const Symbolic& S = *f.host;
+  if (!f.sn_child_ptr) throw Error(SGPU_ERR_UNSUPPORTED, "forward solve without child lists");
+  const int g = blocks_for(6 * int64_t(f.nb), kThreads);
+  hipLaunchKernelGGL(k_permute_in, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, b_nat, b_perm);
+  hipLaunchKernelGGL(k_mf_rhs_init, dim3(S.nsn), dim3(kThreads), 0, s, f, b_perm);
+  const int nlev = int(S.lvl_ptr.size()) - 1;
+  for (int l = 0; l < nlev; ++l) {
+    const int n = S.lvl_ptr[l + 1] - S.lvl_ptr[l];
+    hipLaunchKernelGGL(k_mf_fwd, dim3(n), dim3(kThreads), 0, s, f, f.lvl_sn + S.lvl_ptr[l], z);
+  }
+}
+
+void launch_point_coupling(hipStream_t s, const DevProblem& p, const double* points,
+                           const double* intr, const double* cw, const double* prec, int b,
+                           double* out, int32_t* cam, double* hinv) {
+  hipLaunchKernelGGL(k_point_coupling, dim3(1), dim3(64), 0, s, p, points, intr, cw, prec, b, out,
+                     cam, hinv);
 }
 
 void launch_scatter_s(hipStream_t s, const DevProblem& p, int64_t nslots, const double* blocks,

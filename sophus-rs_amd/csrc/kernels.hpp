@@ -91,6 +91,8 @@ struct DevFactor {
   double* piv_M;   // nb x 36: (L_jj D_j L_jj^T)^+ = L_jj^-T D_j^-1 L_jj^-1 (0 rows for zero pivots)
   int64_t front_elems;
   const Symbolic* host;  // launch schedule
+  const int32_t* sn_child_ptr;  // children of each node (standalone forward solve; may be null
+  const int32_t* sn_child;      // where that solve is not used)
 };
 
 constexpr int kPtRec = 18;  // doubles per packed point record (DevLin::prec)
@@ -130,6 +132,18 @@ void launch_rcs_factor(hipStream_t s, const DevFactor& f, DevScalars* sc, const 
                        double* y);
 // Backward solve with the factor: y (from launch_rcs_factor) -> dx in camera order.
 void launch_rcs_solve(hipStream_t s, const DevFactor& f, double* y, double* dx_cam);
+// Standalone forward solve with a factored RCS: z = D^+ L^-1 P b for a right-hand side b in
+// camera order (node right-hand sides in f.usolve are overwritten); z in elimination order,
+// ready for launch_rcs_solve.
+void launch_rcs_forward(hipStream_t s, const DevFactor& f, const double* b_nat, double* b_perm,
+                        double* z);
+// H_{c,b} = J_cam^T J_point (6 x 3 row-major) of every observation of point b with a free
+// camera, at the point's current value: out[18 q + ..] for its q-th observation, cam[q] its
+// free block (-1: a constant camera); hinv (9, row-major) receives the point's H_ll^-1 from
+// its packed record.  nq observations of the point.
+void launch_point_coupling(hipStream_t s, const DevProblem& p, const double* points,
+                           const double* intr, const double* cw, const double* prec, int b,
+                           double* out, int32_t* cam, double* hinv);
 void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p,
                            const double* points, const double* intr, const double* cw,
                            const DevLin& w, double* points_new, DevScalars* sc);

@@ -94,6 +94,15 @@ struct sgpu_optimizer {
   double kernel_ms[K_COUNT] = {};
   int64_t kernel_count[K_COUNT] = {};
   hipEvent_t phase_ev[5] = {};
+  // Inverse-block queries: the final linearization (build_solution's nu) factored, S's null
+  // vectors (from the zeroed pivots, orthonormal, camera order) and the query buffers.
+  bool cov_ready = false;
+  std::vector<std::vector<double>> cov_null;
+  double *cov_b = nullptr, *cov_bperm = nullptr, *cov_z = nullptr, *cov_x = nullptr;
+  double *cov_coup = nullptr, *cov_hinv = nullptr;
+  int32_t* cov_cam = nullptr;
+  int32_t cov_max_obs = 0;
+  double cov_pivot_ratio = 0.0;  // smallest pivot / largest pivot of the factored S (<= 0: none)
 
   ~sgpu_optimizer() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -141,6 +150,7 @@ struct sgpu_optimizer {
   const Family& point_fam() const { return P.fams[P.f_point]; }
 
   void linearize(const double* ps, const double* pts, double nu_lin, bool* used) {
+    cov_ready = false;  // the fronts and point records are rewritten
     launch_zero_scalars(stream, d_sc);
     launch_cam_cache(stream, S.npose, ps, dl.cw);
     if (!iterative()) launch_zero_fronts(stream, df);
@@ -293,6 +303,8 @@ struct sgpu_optimizer {
     df.sn_woff = arena.upload(S.sn_woff, stream);
     df.solve_cols = arena.upload(S.solve_cols, stream);
     df.sn_uoff = arena.upload(S.sn_uoff, stream);
+    df.sn_child_ptr = arena.upload(S.sn_child_ptr, stream);
+    df.sn_child = arena.upload(S.sn_child, stream);
     df.usolve = arena.alloc<double>(size_t(S.u_elems));
     if (!iterative()) {
       df.front = arena.alloc<double>(size_t(S.front_elems));
@@ -508,6 +520,204 @@ struct sgpu_optimizer {
     }
   }
 
+  // ---- inverse blocks (SchurFactor::inverse_block, schur_ldlt.rs:814-958) ----------------
+  // The matrix is the damped Hessian of build_solution's final linearization at the current
+  // variables (nu = 0, or the current nu with skip_final_hessian), as DampedHessian holds it
+  // (damped_hessian.rs:98).  S is factored with the direct solver's kernels, S = L B L^T with
+  // B = blockdiag(L_jj diag(d) L_jj^T).  Like the reference's min-norm LDLt
+  // (block_sparse_min_norm_ldlt.rs:63-88), only positive pivots span the range: with S' = S
+  // minus the non-positive pivots' terms, S'^+ b = P S'^- P b where S'^- = L^-T M' L^-1,
+  // M'_j = L_jj^-T diag(d+)^-1 L_jj^-1 (non-positive pivots give 0), and P projects off the
+  // null space of S', spanned by n = L^-T (L_jj^-T e_c) for each non-positive pivot c of
+  // block j (S' n = 0).  This is the Moore-Penrose inverse the reference forms from E and G.
+  void ensure_cov() {
+    if (cov_ready) return;
+    if (iterative()) throw Error(SGPU_ERR_UNSUPPORTED, "inverse blocks need the direct RCS solver");
+    const int64_t n = 6 * int64_t(S.nb);
+    if (!cov_b) {
+      cov_b = arena.alloc<double>(size_t(n));
+      cov_bperm = arena.alloc<double>(size_t(n));
+      cov_z = arena.alloc<double>(size_t(n));
+      cov_x = arena.alloc<double>(size_t(n));
+      for (int32_t b = 0; b < S.npt; ++b) cov_max_obs = std::max(cov_max_obs, S.pt_ptr[b + 1] - S.pt_ptr[b]);
+      cov_coup = arena.alloc<double>(size_t(18) * std::max(cov_max_obs, 1));
+      cov_cam = arena.alloc<int32_t>(size_t(std::max(cov_max_obs, 1)));
+      cov_hinv = arena.alloc<double>(9);
+    }
+    bool used[K_COUNT] = {};
+    linearize(poses[cur], points[cur], params.skip_final_hessian ? nu : 0.0, used);
+    launch_rcs_factor(stream, df, d_sc, dl.rhs_nat, dl.x_perm);
+    SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
+    std::vector<double> d(n), Lp(size_t(6) * n);
+    SGPU_HIP(hipMemcpyAsync(d.data(), df.piv_d, n * 8, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipMemcpyAsync(Lp.data(), df.piv_L, size_t(6) * n * 8, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipStreamSynchronize(stream));
+    if (h_sc->singular_marg) throw Error(SGPU_ERR_NON_FINITE, "singular marginal block in the final Hessian");
+    if (h_sc->bad_pivot) throw Error(SGPU_ERR_NON_FINITE, "non-finite pivot in the final Hessian");
+    cov_null.clear();
+    {
+      double lo = INFINITY, hi = 0.0;
+      for (double x : d) {
+        lo = std::min(lo, x);
+        hi = std::max(hi, x);
+      }
+      cov_pivot_ratio = hi > 0.0 ? lo / hi : 0.0;
+    }
+    std::vector<double> e(n), v(n);
+    for (int64_t gb = 0; gb < S.nb; ++gb) {
+      bool any = false;
+      for (int c = 0; c < 6; ++c) any = any || !(d[6 * gb + c] > 0.0);
+      if (!any) continue;
+      // Li = L_jj^-1 (unit lower, row-major) by forward substitution
+      const double* Lj = &Lp[36 * gb];
+      double Li[36];
+      for (int c = 0; c < 6; ++c)
+        for (int r = 0; r < 6; ++r) {
+          double x = r == c ? 1.0 : 0.0;
+          for (int t = c; t < r; ++t) x -= Lj[6 * r + t] * Li[6 * t + c];
+          Li[6 * r + c] = r < c ? 0.0 : x;
+        }
+      double Mj[36];  // L_jj^-T diag(1/d+) L_jj^-1
+      for (int a = 0; a < 6; ++a)
+        for (int b = 0; b < 6; ++b) {
+          double acc = 0.0;
+          for (int k = 0; k < 6; ++k) {
+            const double dk = d[6 * gb + k];
+            if (dk > 0.0) acc += Li[6 * k + a] * (1.0 / dk) * Li[6 * k + b];
+          }
+          Mj[6 * a + b] = acc;
+        }
+      SGPU_HIP(hipMemcpyAsync(df.piv_M + 36 * gb, Mj, 36 * 8, hipMemcpyHostToDevice, stream));
+      for (int c = 0; c < 6; ++c) {
+        if (d[6 * gb + c] > 0.0) continue;
+        std::fill(e.begin(), e.end(), 0.0);
+        for (int r = 0; r < 6; ++r) e[6 * gb + r] = Li[6 * c + r];  // L_jj^-T e_c
+        SGPU_HIP(hipMemcpyAsync(cov_z, e.data(), n * 8, hipMemcpyHostToDevice, stream));
+        launch_rcs_solve(stream, df, cov_z, cov_x);
+        SGPU_HIP(hipMemcpyAsync(v.data(), cov_x, n * 8, hipMemcpyDeviceToHost, stream));
+        SGPU_HIP(hipStreamSynchronize(stream));
+        for (const auto& q : cov_null) {  // Gram-Schmidt against the earlier ones
+          double dot = 0.0;
+          for (int64_t i = 0; i < n; ++i) dot += q[i] * v[i];
+          for (int64_t i = 0; i < n; ++i) v[i] -= dot * q[i];
+        }
+        double nrm = 0.0;
+        for (double x : v) nrm += x * x;
+        nrm = std::sqrt(nrm);
+        if (nrm > 0.0) {
+          for (double& x : v) x /= nrm;
+          cov_null.push_back(v);
+        }
+      }
+    }
+    SGPU_HIP(hipStreamSynchronize(stream));
+    cov_ready = true;
+  }
+
+  void project(std::vector<double>& v) const {
+    for (const auto& q : cov_null) {
+      double dot = 0.0;
+      for (size_t i = 0; i < v.size(); ++i) dot += q[i] * v[i];
+      for (size_t i = 0; i < v.size(); ++i) v[i] -= dot * q[i];
+    }
+  }
+
+  // x = S^+ b, both in camera (layout) order
+  std::vector<double> solve_pinv(std::vector<double> b) {
+    const int64_t n = 6 * int64_t(S.nb);
+    project(b);
+    SGPU_HIP(hipMemcpyAsync(cov_b, b.data(), n * 8, hipMemcpyHostToDevice, stream));
+    launch_rcs_forward(stream, df, cov_b, cov_bperm, cov_z);
+    launch_rcs_solve(stream, df, cov_z, cov_x);
+    std::vector<double> x(n);
+    SGPU_HIP(hipMemcpyAsync(x.data(), cov_x, n * 8, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipStreamSynchronize(stream));
+    project(x);
+    return x;
+  }
+
+  // H_{f, b}: three columns (camera order) of the point's coupling with the free cameras,
+  // and its (damped) H_ll^-1 (row-major)
+  void coupling(int32_t b, std::vector<std::vector<double>>& cols, double hinv[9]) {
+    const int32_t nq = S.pt_ptr[b + 1] - S.pt_ptr[b];
+    launch_point_coupling(stream, dp, points[cur], intr, dl.cw, dl.prec, b, cov_coup, cov_cam, cov_hinv);
+    std::vector<double> H(size_t(18) * std::max(nq, 1));
+    std::vector<int32_t> cams(std::max(nq, 1));
+    SGPU_HIP(hipMemcpyAsync(H.data(), cov_coup, size_t(18) * nq * 8, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipMemcpyAsync(cams.data(), cov_cam, size_t(nq) * 4, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipMemcpyAsync(hinv, cov_hinv, 9 * 8, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipStreamSynchronize(stream));
+    cols.assign(3, std::vector<double>(6 * size_t(S.nb), 0.0));
+    for (int32_t q = 0; q < nq; ++q) {
+      if (cams[q] < 0) continue;
+      for (int r = 0; r < 6; ++r)
+        for (int k = 0; k < 3; ++k) cols[k][6 * size_t(cams[q]) + r] += H[18 * q + 3 * r + k];
+    }
+  }
+
+  // (row, col) = (block, is_free); out row-major
+  void inverse_block(int32_t rb, bool rfree, int32_t cb, bool cfree, double* out) {
+    ensure_cov();
+    const int64_t n = 6 * int64_t(S.nb);
+    if (rfree && cfree) {  // S^+[row, col]
+      for (int q = 0; q < 6; ++q) {
+        std::vector<double> e(n, 0.0);
+        e[6 * int64_t(cb) + q] = 1.0;
+        const std::vector<double> x = solve_pinv(e);
+        for (int r = 0; r < 6; ++r) out[6 * r + q] = x[6 * int64_t(rb) + r];
+      }
+      return;
+    }
+    if (rfree != cfree) {  // -(S^+ H_fm H_mm^-1)[free, marg], or its transpose
+      const int32_t f = rfree ? rb : cb, m = rfree ? cb : rb;
+      std::vector<std::vector<double>> cols;
+      double hi[9];
+      coupling(m, cols, hi);
+      double T[18];
+      for (int q = 0; q < 3; ++q) {
+        const std::vector<double> x = solve_pinv(cols[q]);
+        for (int r = 0; r < 6; ++r) T[3 * r + q] = x[6 * int64_t(f) + r];
+      }
+      for (int r = 0; r < 6; ++r)
+        for (int q = 0; q < 3; ++q) {
+          double acc = 0.0;
+          for (int t = 0; t < 3; ++t) acc += T[3 * r + t] * hi[3 * t + q];
+          if (rfree)
+            out[3 * r + q] = -acc;
+          else
+            out[6 * q + r] = -acc;
+        }
+      return;
+    }
+    // H_mm[i]^-1 delta_ij + H_mm[i]^-1 (H_mf_i S^+ H_fm_j) H_mm[j]^-1
+    std::vector<std::vector<double>> ci, cj;
+    double hi[9], hj[9];
+    coupling(rb, ci, hi);
+    coupling(cb, cj, hj);
+    double core[9];
+    for (int q = 0; q < 3; ++q) {
+      const std::vector<double> x = solve_pinv(cj[q]);
+      for (int p = 0; p < 3; ++p) {
+        double acc = 0.0;
+        for (int64_t i = 0; i < n; ++i) acc += ci[p][i] * x[i];
+        core[3 * p + q] = acc;
+      }
+    }
+    double t1[9];
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int t = 0; t < 3; ++t) acc += hi[3 * a + t] * core[3 * t + c];
+        t1[3 * a + c] = acc;
+      }
+    for (int a = 0; a < 3; ++a)
+      for (int c = 0; c < 3; ++c) {
+        double acc = 0.0;
+        for (int t = 0; t < 3; ++t) acc += t1[3 * a + t] * hj[3 * t + c];
+        out[3 * a + c] = acc + (rb == cb ? hi[3 * a + c] : 0.0);
+      }
+  }
+
   double final_max_grad(double nu_lin) {
     bool used[K_COUNT] = {};
     linearize(poses[cur], points[cur], nu_lin, used);
@@ -653,6 +863,60 @@ sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
     const Symbolic S = analyze(P);
     fill_stats(P, S, st);
   });
+}
+
+sgpu_status sgpu_optimizer_inverse_block(sgpu_optimizer* opt, const char* row_family,
+                                         int64_t row_member, const char* col_family,
+                                         int64_t col_member, double* out, int64_t capacity) {
+  return guarded([&] {
+    if (!opt || !row_family || !col_family || !out) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null argument");
+    auto resolve = [&](const char* name, int64_t member, int32_t& block, bool& is_free) {
+      for (size_t i = 0; i < opt->P.fams.size(); ++i) {
+        const Family& f = opt->P.fams[i];
+        if (f.name != name) continue;
+        if (member < 0 || member >= f.n) throw Error(SGPU_ERR_INDEX_OUT_OF_RANGE, "member index out of range");
+        if (f.kind == SGPU_VAR_CONDITIONED || f.block[member] < 0)
+          throw Error(SGPU_ERR_INVALID_ARGUMENT,
+                      std::string("member of '") + name + "' is conditioned or constant: it has no block");
+        is_free = f.kind == SGPU_VAR_FREE;
+        block = is_free ? f.block[member] : int32_t(member);
+        return;
+      }
+      throw Error(SGPU_ERR_UNKNOWN_FAMILY, std::string("unknown family '") + name + "'");
+    };
+    int32_t rb = 0, cb = 0;
+    bool rf = false, cf = false;
+    resolve(row_family, row_member, rb, rf);
+    resolve(col_family, col_member, cb, cf);
+    const int64_t need = int64_t(rf ? 6 : 3) * (cf ? 6 : 3);
+    if (capacity < need) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
+    SGPU_HIP(hipSetDevice(opt->device));
+    opt->inverse_block(rb, rf, cb, cf, out);
+  });
+}
+
+sgpu_status sgpu_optimizer_covariance_block(sgpu_optimizer* opt, const char* row_family,
+                                            int64_t row_member, const char* col_family,
+                                            int64_t col_member, double* out, int64_t capacity) {
+  if (!opt) return set_error(SGPU_ERR_INVALID_ARGUMENT, "null optimizer");
+  if (opt->params.skip_final_hessian)
+    return set_error(SGPU_ERR_UNSUPPORTED,
+                     "covariance blocks are of the undamped final Hessian (build_solution without "
+                     "skip_final_hessian)");
+  const sgpu_status st = sgpu_optimizer_inverse_block(opt, row_family, row_member, col_family,
+                                                      col_member, out, capacity);
+  if (st != SGPU_OK) return st;
+  // rank-deficient (a non-positive pivot) or numerically singular (a pivot below 1e-9 of the
+  // largest, e.g. a gauge direction left at rounding level with either sign)
+  if (!opt->cov_null.empty() || opt->cov_pivot_ratio < 1e-9)
+    return set_error(SGPU_ERR_UNSUPPORTED,
+                     "the final Hessian is rank-deficient (" + std::to_string(opt->cov_null.size()) +
+                         " non-positive pivots of its Schur complement, smallest / largest pivot " +
+                         std::to_string(opt->cov_pivot_ratio) +
+                         ", e.g. an unfixed gauge): its min-norm inverse is not the "
+                         "Schur-complement inverse block; use sgpu_optimizer_inverse_block for "
+                         "the latter");
+  return SGPU_OK;
 }
 
 static void copy_order(const Symbolic& S, int64_t* perm, int64_t capacity, int64_t* num_blocks) {

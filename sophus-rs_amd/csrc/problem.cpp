@@ -740,6 +740,17 @@ void analyze_rcs(Symbolic& S) {
       S.s_off[k] = S.sn_off[x] + 6 * int64_t(pos_in_front[col]) * F + 6 * int64_t(rp);
       S.s_ld[k] = int32_t(F) * 2 + (pi < pj ? 1 : 0);
     }
+  // children of each supernode, in node order (the standalone forward solve gathers them)
+  S.sn_child_ptr.assign(S.nsn + 1, 0);
+  for (int32_t x = 0; x < S.nsn; ++x)
+    if (S.sn_parent[x] >= 0) ++S.sn_child_ptr[S.sn_parent[x] + 1];
+  for (int32_t x = 0; x < S.nsn; ++x) S.sn_child_ptr[x + 1] += S.sn_child_ptr[x];
+  S.sn_child.assign(S.sn_child_ptr[S.nsn], 0);
+  {
+    std::vector<int32_t> fill(S.sn_child_ptr.begin(), S.sn_child_ptr.end() - 1);
+    for (int32_t x = 0; x < S.nsn; ++x)
+      if (S.sn_parent[x] >= 0) S.sn_child[fill[S.sn_parent[x]]++] = x;
+  }
 }
 
 }  // namespace sgpu

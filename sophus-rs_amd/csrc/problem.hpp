@@ -91,6 +91,7 @@ struct Symbolic {
   int64_t front_elems = 0, l_elems = 0;
   int32_t max_front = 0;                      // largest front (blocks)
   std::vector<int32_t> lvl_ptr, lvl_sn;
+  std::vector<int32_t> sn_child_ptr, sn_child;  // children of each supernode (nsn + 1, nsn - roots)
   // Schedule of the numeric factorization in rounds, one launch (or a panel-row launch and
   // a trailing-update launch) per round, each holding one panel step of every node that
   // is ready, so nodes of different tree levels share a launch and the number of rounds

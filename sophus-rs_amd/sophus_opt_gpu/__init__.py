@@ -200,6 +200,10 @@ _SIGNATURES = {
                                C.POINTER(_RcsStats)]),
     "sgpu_analyze_rcs_order": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
                                          _I64P, C.c_int64, _I64P]),
+    "sgpu_optimizer_inverse_block": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_char_p, C.c_int64, _DP,
+                                               C.c_int64]),
+    "sgpu_optimizer_covariance_block": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_char_p, C.c_int64,
+                                                  _DP, C.c_int64]),
     "sgpu_optimizer_rcs_order": (C.c_int, [C.c_void_p, _I64P, C.c_int64, _I64P]),
     "sgpu_optimizer_rcs_pivots": (C.c_int, [C.c_void_p, _DP, C.c_int64]),
     "sgpu_optimizer_kernel_times": (C.c_int32, [C.c_void_p, C.POINTER(C.c_char_p), _I64P, _DP, C.c_int32]),
@@ -633,6 +637,26 @@ class Optimizer:
         s = _RcsStats()
         _check(library().sgpu_optimizer_rcs_stats(self._h, C.byref(s)))
         return {k: getattr(s, k) for k, _ in _RcsStats._fields_}
+
+    def inverse_block(self, row_family: str, row_member: int, col_family: str, col_member: int) -> np.ndarray:
+        """Block of the inverse of the final linearization's damped Hessian
+        (DampedHessian::inverse_block_with_damping / SchurFactor::inverse_block)."""
+        dims = {n: (6 if f.type == VarType.Isometry3 else 3) for n, f in self._variables.collection.items()}
+        out = np.empty((dims[row_family], dims[col_family]))
+        _check(library().sgpu_optimizer_inverse_block(self._h, row_family.encode(), int(row_member),
+                                                      col_family.encode(), int(col_member),
+                                                      out.ctypes.data_as(_DP), out.size))
+        return out
+
+    def covariance_block(self, row_family: str, row_member: int, col_family: str, col_member: int) -> np.ndarray:
+        """OptimizationSolution::covariance_block: block of H^-1 of the undamped final Hessian
+        (full-rank H; a rank-deficient one raises Unsupported)."""
+        dims = {n: (6 if f.type == VarType.Isometry3 else 3) for n, f in self._variables.collection.items()}
+        out = np.empty((dims[row_family], dims[col_family]))
+        _check(library().sgpu_optimizer_covariance_block(self._h, row_family.encode(), int(row_member),
+                                                         col_family.encode(), int(col_member),
+                                                         out.ctypes.data_as(_DP), out.size))
+        return out
 
     def rcs_order(self) -> np.ndarray:
         """Elimination order of the reduced camera system, perm[new] = old free block."""
