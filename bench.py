@@ -19,7 +19,8 @@ Huber 1 px, camera 0 fixed, nu0 = 1), inputs resident in HBM before the timed re
   LDLt, block-Jacobi PCG and learned PCG; headline = t(FP64 direct) / t(learned + FP32).
 - phase_bandwidth: achieved GB/s of the assembly phases from their algorithmic bytes.
 - workloads: the 10-camera ring and the 500-camera walk (FP64 direct, GPU and CPU oracle).
-- cpu_baseline: one LM iteration of the FP64 oracle on this host, one core.
+- cpu_baseline: one LM iteration of the FP64 oracle on this host at threads = nproc (the
+  process's cores, capped by OMP_NUM_THREADS), and on one core.
 
 Prints ONE JSON line (stdout).  Progress goes to stderr.
 """
@@ -48,22 +49,46 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_threads() -> int:
+    """Host cores for the CPU baseline: this process's CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (a shared host's share of cores)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
+def oracle_step_seconds(oracle, problem, threads: int, iterations: int = 1) -> float:
+    """Wall time inside the LM steps of `iterations` FP64-oracle iterations at `threads`."""
+    oracle.set_threads(threads)
+    try:
+        oracle.optimize_nlls(oracle.ba_families(problem), oracle.ba_costs(problem),
+                             oracle.default_params(num_iterations=iterations, initial_lm_damping=1.0))
+        return oracle.last_step_seconds()
+    finally:
+        oracle.set_threads(1)
+
+
 def cpu_baseline(problem, label: str) -> dict:
-    """One LM iteration of the FP64 oracle (single-threaded C++ restatement of the
-    reference path) on the same problem, on this host's cores."""
+    """One LM iteration of the FP64 oracle (the C++ restatement of the reference path) on the
+    same problem, on this host: at `threads = nproc` (the reported value) and on one core.
+    The oracle's threaded loops give bitwise the same result as its sequential ones."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
 
-    params = oracle.default_params(num_iterations=1, initial_lm_damping=1.0)
-    oracle.optimize_nlls(oracle.ba_families(problem), oracle.ba_costs(problem), params)
-    step_s = oracle.last_step_seconds()
+    nthreads = host_threads()
+    t_n = oracle_step_seconds(oracle, problem, nthreads)
+    t_1 = oracle_step_seconds(oracle, problem, 1)
     return {
-        "value": problem.num_obs / step_s,
+        "value": problem.num_obs / t_n,
         "unit": "obs/s",
-        "cores": 1,
+        "cores": nthreads,
         "kind": "port",
         "sample": f"one LM iteration (linearize, Schur, RCS LDLt, update, merit) of the FP64 oracle "
-                  f"on the full {label} problem: {step_s:.2f} s",
+                  f"on the full {label} problem: {t_n:.2f} s on {nthreads} threads, "
+                  f"{t_1:.2f} s on 1",
+        "single_thread": {"value": problem.num_obs / t_1, "unit": "obs/s", "cores": 1,
+                          "seconds": t_1},
+        "seconds": t_n,
     }
 
 
@@ -241,14 +266,13 @@ def small_workload_leg(sg, index: int, cpu: bool) -> dict:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle
         it_cpu = len(hist) if index == 0 else 1
-        ct = []
-        for rep in range(4 if index == 0 else 1):
-            oracle.optimize_nlls(oracle.ba_families(p), oracle.ba_costs(p),
-                                 oracle.default_params(num_iterations=it_cpu, initial_lm_damping=1.0))
-            if rep or index != 0:
-                ct.append(oracle.last_step_seconds())
-        out["cpu_oracle"] = {"threads": 1, "iterations": it_cpu, "ms": 1e3 * float(np.median(ct)),
-                             "obs_per_s": p.num_obs * it_cpu / float(np.median(ct))}
+        out["cpu_oracle"] = []
+        for threads in sorted({host_threads(), 1}, reverse=True):
+            ct = [oracle_step_seconds(oracle, p, threads, it_cpu) for _ in range(4 if index == 0 else 1)]
+            ct = ct[1:] if index == 0 else ct
+            out["cpu_oracle"].append({"threads": threads, "iterations": it_cpu,
+                                      "ms": 1e3 * float(np.median(ct)),
+                                      "obs_per_s": p.num_obs * it_cpu / float(np.median(ct))})
     return out
 
 

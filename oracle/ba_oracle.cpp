@@ -37,6 +37,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <omp.h>
 #include <set>
 #include <string>
 #include <vector>
@@ -51,6 +52,25 @@ thread_local std::string g_error;
 sgpu_status fail(sgpu_status s, const std::string& msg) {
   g_error = msg;
   return s;
+}
+
+// Host threads for the per-term and per-point loops (oracle_set_threads; 1 by default).
+// The parallel loops keep every output's accumulation order: per-term and per-point values
+// are formed independently, each row of H, S and the right-hand side is written by one
+// thread only (row % threads), which visits that row's contributions in the sequential
+// order, and scalar sums are added in term order.  Results are bitwise the same for any
+// thread count (the reference's rayon paths, cost_system.rs:119-160 and
+// schur_ldlt.rs:282-360, instead re-associate the sums).
+int g_threads = 1;
+
+template <class Fn>
+void run_threads(Fn&& fn) {
+  if (g_threads <= 1) {
+    fn(0, 1);
+    return;
+  }
+#pragma omp parallel num_threads(g_threads)
+  fn(omp_get_thread_num(), omp_get_num_threads());
 }
 
 int var_dof(int type) { return type == SGPU_VAR_ISOMETRY3 ? 6 : type == SGPU_VAR_VEC3 ? 3 : 4; }
@@ -276,13 +296,21 @@ void eval_term(const Problem& P, const Vars& V, const CostFn& c, int64_t t, bool
 
 // calc_merit / calc_total_cost (optimizer_helpers.rs:309-321, cost_fn.rs:403-433).
 double merit(const Problem& P, const Vars& V) {
+  constexpr int64_t kChunk = 1 << 16;
+  std::vector<double> cost(kChunk);
   double total = 0;
   for (const auto& c : P.costs) {
     double s = 0;
-    EvalTerm e;
-    for (int64_t t = 0; t < c.num_terms(); ++t) {
-      eval_term(P, V, c, t, false, e);
-      s += e.cost;
+    const int64_t nt = c.num_terms();
+    for (int64_t t0 = 0; t0 < nt; t0 += kChunk) {
+      const int64_t n = std::min(kChunk, nt - t0);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+      for (int64_t k = 0; k < n; ++k) {
+        EvalTerm e;
+        eval_term(P, V, c, t0 + k, false, e);
+        cost[k] = e.cost;
+      }
+      for (int64_t k = 0; k < n; ++k) s += cost[k];
     }
     total += s;
   }
@@ -382,64 +410,85 @@ void symbolic(const Problem& P, Linearization& L) {
 // cost_system.eval + populate_upper_triangular_normal_equation (sequential path):
 // terms are evaluated, reduced over their reduction range, and accumulated with
 // nu * I added per reduced term to each active diagonal block.
+// Sum of the H blocks and gradients of the terms [t0, t1) of one reduction range.
+struct RangeSums {
+  double Hs[3][3][36];
+  double gs[3][6];
+};
+
+void reduce_range(const Problem& P, const Vars& V, const CostFn& c, int64_t t0, int64_t t1,
+                  const int dof[3], int R, RangeSums& out) {
+  std::memset(&out, 0, sizeof(out));
+  EvalTerm e;
+  for (int64_t t = t0; t < t1; ++t) {
+    eval_term(P, V, c, t, true, e);
+    for (int a = 0; a < c.arity; ++a) {
+      if (!e.has[a]) continue;
+      for (int i = 0; i < dof[a]; ++i) {
+        double acc = 0;
+        for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * e.r[k];
+        out.gs[a][i] += acc;
+      }
+      for (int b = 0; b < c.arity; ++b) {
+        if (!e.has[b]) continue;
+        for (int i = 0; i < dof[a]; ++i)
+          for (int j = 0; j < dof[b]; ++j) {
+            double acc = 0;
+            for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * e.J[b][k * dof[b] + j];
+            out.Hs[a][b][i * dof[b] + j] += acc;
+          }
+      }
+    }
+  }
+}
+
 void linearize(const Problem& P, const Vars& V, double nu, Linearization& L) {
   std::fill(L.H.val.begin(), L.H.val.end(), 0.0);
   L.neg_grad.assign(V.num_free_scalars + V.num_marg_scalars, 0.0);
+  constexpr int64_t kChunk = 1 << 10;
+  std::vector<RangeSums> sums(kChunk);
   for (const auto& c : P.costs) {
     const int64_t nt = c.num_terms();
-    EvalTerm e;
     const int R = c.kind == SGPU_COST_PINHOLE_REPROJECTION ? 2 : 6;
     int dof[3] = {0, 0, 0};
     for (int a = 0; a < c.arity; ++a) dof[a] = V.fam[c.fam[a]].dof;
-    for (size_t rr = 0; rr < c.range_start.size(); ++rr) {
-      const int64_t t0 = c.range_start[rr];
-      const int64_t t1 = rr + 1 < c.range_start.size() ? c.range_start[rr + 1] : nt;
-      // reduced term: sum of H blocks and gradients over [t0, t1)
-      double Hs[3][3][36] = {};
-      double gs[3][6] = {};
-      for (int64_t t = t0; t < t1; ++t) {
-        eval_term(P, V, c, t, true, e);
-        for (int a = 0; a < c.arity; ++a) {
-          if (!e.has[a]) continue;
-          for (int i = 0; i < dof[a]; ++i) {
-            double acc = 0;
-            for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * e.r[k];
-            gs[a][i] += acc;
+    const int64_t nr = int64_t(c.range_start.size());
+    auto range_end = [&](int64_t rr) { return rr + 1 < nr ? c.range_start[rr + 1] : nt; };
+    for (int64_t r0 = 0; r0 < nr; r0 += kChunk) {
+      const int64_t n = std::min(kChunk, nr - r0);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+      for (int64_t k = 0; k < n; ++k)
+        reduce_range(P, V, c, c.range_start[r0 + k], range_end(r0 + k), dof, R, sums[k]);
+      // accumulate in range order; H row ba and its gradient rows belong to one thread
+      run_threads([&](int tid, int T) {
+        for (int64_t k = 0; k < n; ++k) {
+          const RangeSums& s = sums[k];
+          const int64_t* ix = &c.idx[c.range_start[r0 + k] * c.arity];
+          for (int a = 0; a < c.arity; ++a) {
+            const Family& fa = V.fam[c.fam[a]];
+            if (fa.kind == SGPU_VAR_CONDITIONED) continue;
+            const int64_t sa = fa.scalar_start[ix[a]];
+            if (sa < 0) continue;
+            const int64_t ba = fa.block_id[ix[a]];
+            if (ba % T != tid) continue;
+            for (int i = 0; i < dof[a]; ++i) L.neg_grad[sa + i] -= s.gs[a][i];
+            add_block(L.H, ba, ba, s.Hs[a][a]);
+            if (nu != 0.0) {
+              double ni[36] = {};
+              for (int i = 0; i < dof[a]; ++i) ni[i * dof[a] + i] = nu;
+              add_block(L.H, ba, ba, ni);
+            }
+            for (int b = 0; b < c.arity; ++b) {
+              if (b == a) continue;
+              const Family& fb = V.fam[c.fam[b]];
+              if (fb.kind == SGPU_VAR_CONDITIONED) continue;
+              const int64_t sb = fb.scalar_start[ix[b]];
+              if (sb < 0 || sb > sa) continue;
+              add_block(L.H, ba, fb.block_id[ix[b]], s.Hs[a][b]);
+            }
           }
-          for (int b = 0; b < c.arity; ++b) {
-            if (!e.has[b]) continue;
-            for (int i = 0; i < dof[a]; ++i)
-              for (int j = 0; j < dof[b]; ++j) {
-                double acc = 0;
-                for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * e.J[b][k * dof[b] + j];
-                Hs[a][b][i * dof[b] + j] += acc;
-              }
-          }
         }
-      }
-      const int64_t* ix = &c.idx[t0 * c.arity];
-      for (int a = 0; a < c.arity; ++a) {
-        const Family& fa = V.fam[c.fam[a]];
-        if (fa.kind == SGPU_VAR_CONDITIONED) continue;
-        const int64_t sa = fa.scalar_start[ix[a]];
-        if (sa < 0) continue;
-        const int64_t ba = fa.block_id[ix[a]];
-        for (int i = 0; i < dof[a]; ++i) L.neg_grad[sa + i] -= gs[a][i];
-        add_block(L.H, ba, ba, Hs[a][a]);
-        if (nu != 0.0) {
-          double ni[36] = {};
-          for (int i = 0; i < dof[a]; ++i) ni[i * dof[a] + i] = nu;
-          add_block(L.H, ba, ba, ni);
-        }
-        for (int b = 0; b < c.arity; ++b) {
-          if (b == a) continue;
-          const Family& fb = V.fam[c.fam[b]];
-          if (fb.kind == SGPU_VAR_CONDITIONED) continue;
-          const int64_t sb = fb.scalar_start[ix[b]];
-          if (sb < 0 || sb > sa) continue;
-          add_block(L.H, ba, fb.block_id[ix[b]], Hs[a][b]);
-        }
-      }
+      });
     }
   }
 }
@@ -659,6 +708,7 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
   F.Ljj.assign(nb, {});
   F.d.assign(nb, {});
   std::vector<std::map<int64_t, std::vector<double>>> Lmap(nb);  // per column
+  std::vector<int64_t> slot_of(nb, -1);  // row -> update target of the current column
   FactorStat stat{0, 0, 1.0};
   for (int64_t j = 0; j < nb; ++j) {
     const int64_t dj = F.dim[j];
@@ -669,13 +719,26 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
       for (size_t t = 0; t < e.second.size(); ++t) b[t] += e.second[t];
     }
     if (!C.count(j)) C[j].assign(dj * dj, 0.0);
-    for (int64_t k : reach(j)) {  // apply_to_col_k_in_reach (:614-720)
+    // apply_to_col_k_in_reach (:614-720): every column k in the reach with L(j, k) != 0
+    // updates C(j) and each C(i), i > j, with L(i, k) != 0, in reach order.  Each target
+    // block's updates are listed in that order and the blocks are updated in parallel.
+    struct Upd {
+      int64_t q;                       // index into kcol / Qj
+      const std::vector<double>* Lik;  // L(i, k) (null for the diagonal target)
+      int64_t slot;                    // target block
+    };
+    std::vector<Upd> upd;
+    std::vector<int64_t> kcol;            // listed columns k, in reach order
+    std::vector<std::vector<double>> Qj;  // per listed k: Qjk = L(j, k) L_kk (dj x dk)
+    std::vector<std::vector<double>*> targets{&C[j]};
+    std::vector<int64_t> target_row{j};
+    slot_of[j] = 0;
+    for (int64_t k : reach(j)) {
       auto it = Lmap[k].find(j);
       if (it == Lmap[k].end()) continue;
       const int64_t dk = F.dim[k];
       const std::vector<double>& Ljk = it->second;  // dj x dk
       const std::vector<double>& Lkk = F.Ljj[k];
-      const std::vector<double>& dkv = F.d[k];
       std::vector<double> Qjk(dj * dk, 0.0);
       for (int64_t a = 0; a < dj; ++a)
         for (int64_t b = 0; b < dk; ++b) {
@@ -683,30 +746,63 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
           for (int64_t t = 0; t < dk; ++t) acc += Ljk[a * dk + t] * Lkk[t * dk + b];
           Qjk[a * dk + b] = acc;
         }
-      auto sub = [&](std::vector<double>& Cb, const std::vector<double>& Q, int64_t di) {
-        // C -= Q diag(d) Qjk^T (sub_mat_diag_mat_t_inplace, kernel.rs:169-195)
-        for (int64_t t = 0; t < dk; ++t) {
-          const double dt = dkv[t];
-          for (int64_t c2 = 0; c2 < dj; ++c2) {
-            const double bj = dt * Qjk[c2 * dk + t];
-            for (int64_t r2 = 0; r2 < di; ++r2) Cb[r2 * dj + c2] -= Q[r2 * dk + t] * bj;
-          }
-        }
-      };
-      sub(C[j], Qjk, dj);
+      const int64_t q = int64_t(kcol.size());
+      kcol.push_back(k);
+      Qj.push_back(std::move(Qjk));
+      upd.push_back({q, nullptr, 0});
       for (auto& [i, Lik] : Lmap[k]) {
         if (i <= j) continue;
-        const int64_t di = F.dim[i];
-        std::vector<double> Qik(di * dk, 0.0);
-        for (int64_t a = 0; a < di; ++a)
-          for (int64_t b = 0; b < dk; ++b) {
-            double acc = 0;
-            for (int64_t t = 0; t < dk; ++t) acc += Lik[a * dk + t] * Lkk[t * dk + b];
-            Qik[a * dk + b] = acc;
+        if (slot_of[i] < 0) {
+          auto& Cb = C[i];
+          if (Cb.empty()) Cb.assign(F.dim[i] * dj, 0.0);
+          slot_of[i] = int64_t(targets.size());
+          targets.push_back(&Cb);
+          target_row.push_back(i);
+        }
+        upd.push_back({q, &Lik, slot_of[i]});
+      }
+    }
+    // group the updates by target block (counting sort keeps reach order in each group)
+    const int64_t nt = int64_t(targets.size());
+    std::vector<int64_t> gptr(nt + 1, 0), gidx(upd.size());
+    for (const Upd& u : upd) ++gptr[u.slot + 1];
+    for (int64_t t = 0; t < nt; ++t) gptr[t + 1] += gptr[t];
+    {
+      std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
+      for (size_t u = 0; u < upd.size(); ++u) gidx[fill[upd[u].slot]++] = int64_t(u);
+    }
+    for (int64_t r : target_row) slot_of[r] = -1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(g_threads) if (g_threads > 1)
+    for (int64_t t = 0; t < nt; ++t) {
+      std::vector<double>& Cb = *targets[t];
+      const int64_t di = int64_t(Cb.size()) / dj;
+      std::vector<double> Qik;
+      for (int64_t g = gptr[t]; g < gptr[t + 1]; ++g) {
+        const Upd& u = upd[gidx[g]];
+        const int64_t k = kcol[u.q], dk = F.dim[k];
+        const std::vector<double>& dkv = F.d[k];
+        const std::vector<double>& Qjk = Qj[u.q];
+        const std::vector<double>* Q = &Qjk;
+        if (u.Lik) {
+          const std::vector<double>& Lik = *u.Lik;
+          const std::vector<double>& Lkk = F.Ljj[k];
+          Qik.assign(di * dk, 0.0);
+          for (int64_t a = 0; a < di; ++a)
+            for (int64_t b = 0; b < dk; ++b) {
+              double acc = 0;
+              for (int64_t s2 = 0; s2 < dk; ++s2) acc += Lik[a * dk + s2] * Lkk[s2 * dk + b];
+              Qik[a * dk + b] = acc;
+            }
+          Q = &Qik;
+        }
+        // C -= Q diag(d) Qjk^T (sub_mat_diag_mat_t_inplace, kernel.rs:169-195)
+        for (int64_t s2 = 0; s2 < dk; ++s2) {
+          const double dt = dkv[s2];
+          for (int64_t c2 = 0; c2 < dj; ++c2) {
+            const double bj = dt * Qjk[c2 * dk + s2];
+            for (int64_t r2 = 0; r2 < di; ++r2) Cb[r2 * dj + c2] -= (*Q)[r2 * dk + s2] * bj;
           }
-        auto& Cb = C[i];
-        if (Cb.empty()) Cb.assign(di * dj, 0.0);
-        sub(Cb, Qik, di);
+        }
       }
     }
     // append_to_ldlt (:722-747)
@@ -723,11 +819,14 @@ bool block_ldlt_factorize(const BlockLower& A, BlockLdltFactor& F, double tol_re
         if (v < 0.0) ++stat.neg;
       }
     }
-    for (auto& [i, Cb] : C) {
-      if (i <= j) continue;
-      ldlt_right_solve(int(F.dim[i]), int(dj), F.Ljj[j].data(), F.d[j].data(), Cb.data());
-      Lmap[j][i] = Cb;
-    }
+    std::vector<std::pair<int64_t, std::vector<double>*>> below;
+    for (auto& [i, Cb] : C)
+      if (i > j) below.push_back({i, &Cb});
+#pragma omp parallel for schedule(dynamic, 8) num_threads(g_threads) if (g_threads > 1)
+    for (int64_t e = 0; e < int64_t(below.size()); ++e)
+      ldlt_right_solve(int(F.dim[below[e].first]), int(dj), F.Ljj[j].data(), F.d[j].data(),
+                       below[e].second->data());
+    for (auto& [i, Cb] : below) Lmap[j][i] = std::move(*Cb);
   }
   for (int64_t j = 0; j < nb; ++j)
     for (auto& [i, b] : Lmap[j]) {
@@ -839,16 +938,20 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
   const int64_t nfs = nfb == 0 ? 0 : H.start[nfb - 1] + H.dim[nfb - 1];
   // S pattern: H_ff pattern plus every (f_i, f_j) pair sharing a marg block.
   std::vector<std::set<int64_t>> srows(nfb);
-  for (int64_t r = 0; r < nfb; ++r)
-    for (int64_t k = H.row_ptr[r]; k < H.row_ptr[r + 1]; ++k) srows[r].insert(H.col[k]);
-  for (int64_t r = 0; r < nfb; ++r) srows[r].insert(r);
-  for (int64_t b = nfb; b < nb; ++b) {
-    std::vector<int64_t> fs;
-    for (int64_t k = H.row_ptr[b]; k < H.row_ptr[b + 1]; ++k)
-      if (H.col[k] < nfb) fs.push_back(H.col[k]);
-    for (size_t i = 0; i < fs.size(); ++i)
-      for (size_t j = 0; j <= i; ++j) srows[fs[i]].insert(fs[j]);
-  }
+  run_threads([&](int tid, int T) {
+    for (int64_t r = tid; r < nfb; r += T) {
+      for (int64_t k = H.row_ptr[r]; k < H.row_ptr[r + 1]; ++k) srows[r].insert(H.col[k]);
+      srows[r].insert(r);
+    }
+    for (int64_t b = nfb; b < nb; ++b) {
+      // free columns of marg row b (ascending); row fs[i] of S belongs to one thread
+      const int64_t k0 = H.row_ptr[b], k1 = H.row_ptr[b + 1];
+      for (int64_t ki = k0; ki < k1 && H.col[ki] < nfb; ++ki) {
+        if (H.col[ki] % T != tid) continue;
+        for (int64_t kj = k0; kj <= ki; ++kj) srows[H.col[ki]].insert(H.col[kj]);
+      }
+    }
+  });
   BlockLower S;
   S.nb = nfb;
   S.dim.assign(H.dim.begin(), H.dim.begin() + nfb);
@@ -859,58 +962,85 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
     for (int64_t k = H.row_ptr[r]; k < H.row_ptr[r + 1]; ++k) add_block(S, r, H.col[k], H.block(k));
   std::vector<double> rhs(g.begin(), g.begin() + nfs);
   std::vector<std::vector<double>> hinv(nb - nfb);
-  // forward pass, sequential (schur_forward_pass_sequential, :241-278)
-  for (int64_t b = nfb; b < nb; ++b) {
-    const int64_t dm = H.dim[b];
-    int64_t kd = H.find(b, b);
-    std::vector<double> hmm(dm * dm, 0.0);
-    if (kd >= 0) std::copy(H.block(kd), H.block(kd) + dm * dm, hmm.begin());
-    std::vector<double>& hi = hinv[b - nfb];
-    hi.assign(dm * dm, 0.0);
-    if (!inverse_dense(int(dm), hmm.data(), hi.data())) return false;
-    std::vector<double> hig(dm, 0.0);
-    for (int64_t i = 0; i < dm; ++i) {
-      double acc = 0;
-      for (int64_t k = 0; k < dm; ++k) acc += hi[i * dm + k] * g[H.start[b] + k];
-      hig[i] = acc;
+  // forward pass (schur_forward_pass_sequential, :241-278), over chunks of marg blocks:
+  // each block's H_mm^-1, H_mm^-1 g_m and H_mm^-1 H_mf are formed in parallel, then
+  // applied to S and the rhs in block order (apply_contrib_to_sparse, :138-182).
+  constexpr int64_t kChunk = 1 << 10;
+  std::vector<int64_t> off(kChunk + 1);  // per block: start of [hig | w_0 | w_1 ...]
+  std::vector<double> work;
+  std::vector<char> singular(kChunk);
+  for (int64_t b0 = nfb; b0 < nb; b0 += kChunk) {
+    const int64_t n = std::min(kChunk, nb - b0);
+    off[0] = 0;
+    for (int64_t q = 0; q < n; ++q) {
+      const int64_t b = b0 + q, dm = H.dim[b];
+      int64_t len = dm;
+      for (int64_t k = H.row_ptr[b]; k < H.row_ptr[b + 1] && H.col[k] < nfb; ++k)
+        len += dm * H.dim[H.col[k]];
+      off[q + 1] = off[q] + len;
     }
-    std::vector<int64_t> fi;
-    std::vector<const double*> hmf;
-    std::vector<std::vector<double>> w;
-    for (int64_t k = H.row_ptr[b]; k < H.row_ptr[b + 1]; ++k) {
-      const int64_t f = H.col[k];
-      if (f >= nfb) continue;
-      const int64_t df = H.dim[f];
-      fi.push_back(f);
-      hmf.push_back(H.block(k));  // dm x df
-      std::vector<double> wb(dm * df, 0.0);
-      for (int64_t i = 0; i < dm; ++i)
-        for (int64_t j = 0; j < df; ++j) {
-          double acc = 0;
-          for (int64_t t = 0; t < dm; ++t) acc += hi[i * dm + t] * H.block(k)[t * df + j];
-          wb[i * df + j] = acc;
-        }
-      w.push_back(std::move(wb));
-    }
-    for (size_t i = 0; i < fi.size(); ++i) {  // apply_contrib_to_sparse (:138-182)
-      const int64_t di = H.dim[fi[i]];
-      for (int64_t c = 0; c < di; ++c) {
+    if (int64_t(work.size()) < off[n]) work.resize(off[n]);
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
+    for (int64_t q = 0; q < n; ++q) {
+      const int64_t b = b0 + q, dm = H.dim[b];
+      int64_t kd = H.find(b, b);
+      double hmm[36] = {};
+      if (kd >= 0) std::copy(H.block(kd), H.block(kd) + dm * dm, hmm);
+      std::vector<double>& hi = hinv[b - nfb];
+      hi.assign(dm * dm, 0.0);
+      singular[q] = !inverse_dense(int(dm), hmm, hi.data());
+      if (singular[q]) continue;
+      double* hig = &work[off[q]];
+      for (int64_t i = 0; i < dm; ++i) {
         double acc = 0;
-        for (int64_t t = 0; t < dm; ++t) acc += hmf[i][t * di + c] * hig[t];
-        rhs[H.start[fi[i]] + c] -= acc;
+        for (int64_t k = 0; k < dm; ++k) acc += hi[i * dm + k] * g[H.start[b] + k];
+        hig[i] = acc;
       }
-      for (size_t j = 0; j <= i; ++j) {
-        const int64_t dj = H.dim[fi[j]];
-        std::vector<double> blk(di * dj);
-        for (int64_t r = 0; r < di; ++r)
-          for (int64_t c = 0; c < dj; ++c) {
+      double* wb = hig + dm;
+      for (int64_t k = H.row_ptr[b]; k < H.row_ptr[b + 1] && H.col[k] < nfb; ++k) {
+        const int64_t df = H.dim[H.col[k]];
+        for (int64_t i = 0; i < dm; ++i)
+          for (int64_t j = 0; j < df; ++j) {
             double acc = 0;
-            for (int64_t t = 0; t < dm; ++t) acc += hmf[i][t * di + r] * w[j][t * dj + c];
-            blk[r * dj + c] = -acc;
+            for (int64_t t = 0; t < dm; ++t) acc += hi[i * dm + t] * H.block(k)[t * df + j];
+            wb[i * df + j] = acc;
           }
-        add_block(S, fi[i], fi[j], blk.data());
+        wb += dm * df;
       }
     }
+    for (int64_t q = 0; q < n; ++q)
+      if (singular[q]) return false;
+    run_threads([&](int tid, int T) {
+      for (int64_t q = 0; q < n; ++q) {
+        const int64_t b = b0 + q, dm = H.dim[b];
+        const double* hig = &work[off[q]];
+        const int64_t k0 = H.row_ptr[b];
+        for (int64_t ki = k0; ki < H.row_ptr[b + 1] && H.col[ki] < nfb; ++ki) {
+          const int64_t fi = H.col[ki], di = H.dim[fi];
+          const double* hmf_i = H.block(ki);  // dm x di
+          if (fi % T == tid) {
+            for (int64_t c = 0; c < di; ++c) {
+              double acc = 0;
+              for (int64_t t = 0; t < dm; ++t) acc += hmf_i[t * di + c] * hig[t];
+              rhs[H.start[fi] + c] -= acc;
+            }
+            const double* wj = hig + dm;
+            for (int64_t kj = k0; kj <= ki; ++kj) {
+              const int64_t fj = H.col[kj], dj = H.dim[fj];
+              double blk[36];
+              for (int64_t r = 0; r < di; ++r)
+                for (int64_t c = 0; c < dj; ++c) {
+                  double acc = 0;
+                  for (int64_t t = 0; t < dm; ++t) acc += hmf_i[t * di + r] * wj[t * dj + c];
+                  blk[r * dj + c] = -acc;
+                }
+              add_block(S, fi, fj, blk);
+              wj += dm * dj;
+            }
+          }
+        }
+      }
+    });
   }
   BlockLdltFactor F;
   if (!block_ldlt_factorize(S, F, tol_rel)) return false;
@@ -919,9 +1049,11 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
   dx.assign(g.size(), 0.0);
   std::copy(dxf.begin(), dxf.end(), dx.begin());
   // back-substitution (:785-806)
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
   for (int64_t b = nfb; b < nb; ++b) {
     const int64_t dm = H.dim[b];
-    std::vector<double> mv(g.begin() + H.start[b], g.begin() + H.start[b] + dm);
+    double mv[6];
+    std::copy(g.begin() + H.start[b], g.begin() + H.start[b] + dm, mv);
     for (int64_t k = H.row_ptr[b]; k < H.row_ptr[b + 1]; ++k) {
       const int64_t f = H.col[k];
       if (f >= nfb) continue;
@@ -1167,6 +1299,14 @@ int32_t oracle_set_rcs_order(const int64_t* perm, int64_t n) {
   }
   g_rcs_order.assign(perm, perm + n);
   return 0;
+}
+
+// Host threads for the oracle's per-term and per-point loops (n < 1 -> 1); results do not
+// depend on it.  Returns the previous value.
+int32_t oracle_set_threads(int32_t n) {
+  const int32_t prev = g_threads;
+  g_threads = std::max<int32_t>(1, n);
+  return prev;
 }
 
 // Pivot statistics of the block LDLt factorizations since the last reset (one entry per

@@ -121,6 +121,8 @@ def lib(variant: str = "") -> C.CDLL:
         L.oracle_factor_stats.argtypes = [I64P, I64P, DP, C.c_int64]
         L.oracle_reset_factor_stats.restype = None
         L.oracle_reset_factor_stats.argtypes = []
+        L.oracle_set_threads.restype = C.c_int32
+        L.oracle_set_threads.argtypes = [C.c_int32]
         _libs[variant] = L
     return _libs[variant]
 
@@ -218,6 +220,12 @@ def set_rcs_order(perm=None, variant: str = "") -> None:
     p = np.ascontiguousarray(perm, dtype=np.int64)
     if lib(variant).oracle_set_rcs_order(p.ctypes.data_as(I64P), p.size) != 0:
         raise ValueError("not a permutation")
+
+
+def set_threads(n: int, variant: str = "") -> int:
+    """Host threads for the oracle's per-term and per-point loops; results are bitwise the
+    same for any count.  Returns the previous count."""
+    return int(lib(variant).oracle_set_threads(int(n)))
 
 
 def factor_stats(reset: bool = True, variant: str = "") -> List[dict]:
