@@ -211,7 +211,7 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
                                        int32_t leaf_size) {
   std::vector<int32_t> out;
   out.reserve(n);
-  std::vector<int32_t> mark(n, -1), level(n, -1);
+  std::vector<int32_t> mark(n, -1), level(n, -1), loc(n, -1);
   int32_t stamp = 0;
   auto bfs = [&](const std::vector<int32_t>& nodes, int32_t tag, int32_t src,
                  std::vector<int32_t>& orderv) {
@@ -271,25 +271,111 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
       out.insert(out.end(), nodes.begin(), nodes.end());
       return;
     }
+    const int32_t N = int32_t(nodes.size());
     std::vector<int32_t> cnt(maxlev + 1, 0);
     for (int32_t v : nodes) ++cnt[level[v]];
-    int32_t acc = 0, m = 1;
-    for (int32_t l = 0; l <= maxlev; ++l) {
-      acc += cnt[l];
-      if (2 * acc >= int32_t(nodes.size())) {
-        m = l;
-        break;
+    std::vector<std::vector<int32_t>> by_level(maxlev + 1);
+    for (int32_t v : orderv) by_level[level[v]].push_back(v);
+    // Candidate splits between BFS levels k and k + 1 (edges join equal or adjacent
+    // levels only) whose lower side holds 30-70% of the nodes.  The separator of a split is
+    // a minimum vertex cover of the edges between the two levels, which by Konig's theorem
+    // follows from a maximum matching of that bipartite graph; it is never larger than
+    // either level set.  The smallest such separator wins (the first on ties).
+    std::vector<int32_t> best_sep;
+    int32_t best_k = -1, acc = 0;
+    for (int32_t k = 0; k < maxlev; ++k) {
+      acc += cnt[k];
+      if (10 * acc < 3 * N || 10 * acc > 7 * N) continue;
+      // bipartite graph: level-k nodes (left) -> level-(k+1) neighbours (right)
+      std::vector<int32_t>& Lk = by_level[k];
+      std::vector<int32_t>& Rk = by_level[k + 1];
+      for (size_t i = 0; i < Rk.size(); ++i) loc[Rk[i]] = int32_t(i);
+      std::vector<int32_t> eptr(Lk.size() + 1, 0), eadj;
+      for (size_t i = 0; i < Lk.size(); ++i) {
+        for (int32_t u : adj[Lk[i]])
+          if (mark[u] == tag && level[u] == k + 1) eadj.push_back(loc[u]);
+        eptr[i + 1] = int32_t(eadj.size());
+      }
+      // maximum matching by augmenting paths (Kuhn)
+      std::vector<int32_t> match_l(Lk.size(), -1), match_r(Rk.size(), -1), seen(Rk.size(), -1);
+      std::function<bool(int32_t, int32_t)> augment = [&](int32_t a, int32_t round) {
+        for (int32_t e = eptr[a]; e < eptr[a + 1]; ++e) {
+          const int32_t b = eadj[e];
+          if (seen[b] == round) continue;
+          seen[b] = round;
+          if (match_r[b] < 0 || augment(match_r[b], round)) {
+            match_l[a] = b;
+            match_r[b] = a;
+            return true;
+          }
+        }
+        return false;
+      };
+      for (int32_t a = 0; a < int32_t(Lk.size()); ++a) augment(a, a);
+      // Konig: alternating search from the unmatched left nodes; the cover is the left
+      // nodes it does not reach that have edges, plus the right nodes it reaches
+      std::vector<char> vis_l(Lk.size(), 0), vis_r(Rk.size(), 0);
+      std::vector<int32_t> stack;
+      for (int32_t a = 0; a < int32_t(Lk.size()); ++a)
+        if (match_l[a] < 0) {
+          vis_l[a] = 1;
+          stack.push_back(a);
+        }
+      while (!stack.empty()) {
+        const int32_t a = stack.back();
+        stack.pop_back();
+        for (int32_t e = eptr[a]; e < eptr[a + 1]; ++e) {
+          const int32_t b = eadj[e];
+          if (vis_r[b]) continue;
+          vis_r[b] = 1;
+          const int32_t a2 = match_r[b];
+          if (a2 >= 0 && !vis_l[a2]) {
+            vis_l[a2] = 1;
+            stack.push_back(a2);
+          }
+        }
+      }
+      std::vector<int32_t> sep;
+      int32_t sep_l = 0, sep_r = 0;
+      for (int32_t a = 0; a < int32_t(Lk.size()); ++a)
+        if (!vis_l[a] && eptr[a + 1] > eptr[a]) {
+          sep.push_back(Lk[a]);
+          ++sep_l;
+        }
+      for (int32_t b = 0; b < int32_t(Rk.size()); ++b)
+        if (vis_r[b]) {
+          sep.push_back(Rk[b]);
+          ++sep_r;
+        }
+      if (acc - sep_l == 0 || N - acc - sep_r == 0) continue;
+      if (best_k < 0 || sep.size() < best_sep.size()) {
+        best_k = k;
+        best_sep.swap(sep);
       }
     }
-    m = std::max(1, std::min(m, maxlev - 1));
     std::vector<int32_t> left, right, sep;
-    for (int32_t v : nodes) {
-      if (level[v] < m)
-        left.push_back(v);
-      else if (level[v] > m)
-        right.push_back(v);
-      else
-        sep.push_back(v);
+    if (best_k < 0) {  // no candidate: the median level set
+      int32_t m = 1;
+      acc = 0;
+      for (int32_t l = 0; l <= maxlev; ++l) {
+        acc += cnt[l];
+        if (2 * acc >= N) {
+          m = l;
+          break;
+        }
+      }
+      m = std::max(1, std::min(m, maxlev - 1));
+      for (int32_t v : nodes)
+        (level[v] < m ? left : level[v] > m ? right : sep).push_back(v);
+    } else {
+      for (int32_t v : best_sep) loc[v] = -2;
+      for (int32_t v : nodes) {
+        if (loc[v] == -2)
+          sep.push_back(v);
+        else
+          (level[v] <= best_k ? left : right).push_back(v);
+      }
+      for (int32_t v : best_sep) loc[v] = -1;
     }
     rec(left);
     rec(right);

@@ -1,4 +1,5 @@
 """Shared test setup: import paths, the `gpu` marker and common fixtures."""
+import os
 import sys
 from pathlib import Path
 
@@ -23,9 +24,18 @@ def sg():
     return sophus_opt_gpu
 
 
+def _host_threads() -> int:
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
 @pytest.fixture(scope="session")
 def orc():
+    """The oracle on this host's cores: its threaded loops give bitwise the result of its
+    sequential ones (test_threaded_oracle_is_bitwise_sequential)."""
     import oracle
 
-    oracle.lib()
+    for variant in oracle.VARIANTS:
+        oracle.set_threads(min(16, _host_threads()), variant=variant)
     return oracle

@@ -215,12 +215,12 @@ def test_threaded_oracle_is_bitwise_sequential(sg, orc):
     p = sg.generate_ba(sg.BaSpec(layout=1, num_cameras=60, num_points=3000, geometric_p=0.15, seed=7))
     runs = []
     for threads in (1, 4):
-        orc.set_threads(threads)
+        prev = orc.set_threads(threads)
         try:
             runs.append(orc.optimize_nlls(orc.ba_families(p), orc.ba_costs(p),
                                           orc.default_params(num_iterations=5, initial_lm_damping=1.0)))
         finally:
-            orc.set_threads(1)
+            orc.set_threads(prev)
     (s1, v1, t1), (s4, v4, t4) = runs
     assert [(r["merit"], r["lm_damping"], r["accepted"]) for r in t1] == \
         [(r["merit"], r["lm_damping"], r["accepted"]) for r in t4]
