@@ -16,7 +16,8 @@ Huber 1 px, camera 0 fixed, nu0 = 1), inputs resident in HBM before the timed re
   termination, no iteration cap in reach), checked against the lowest end of any path and
   FP64 restarted from it; each path's first iteration within 1e-6 of F*, its time and
   obs/s = N_o x iterations / time: FP64 and FP32-assembly (mixed) paths with the direct
-  LDLt, block-Jacobi PCG and learned PCG; headline = t(FP64 direct) / t(learned + FP32).
+  LDLt, block-Jacobi PCG and learned PCG, and the FP64 PCG paths with a forcing-sequence CG
+  tolerance; headline = t(FP64 direct) / t(learned + FP32).
 - phase_bandwidth: achieved GB/s of the assembly phases from their algorithmic bytes.
 - workloads: the 10-camera ring and the 500-camera walk (FP64 direct, GPU and CPU oracle).
 - cpu_baseline: one LM iteration of the FP64 oracle on this host at threads = nproc (the
@@ -151,6 +152,14 @@ def time_to_tol_leg(sg, problem, n_obs) -> dict:
         "mixed_pcg_jacobi": dict(precision=sg.Precision.Mixed, solver=sg.Solver.SchurPcgJacobi, **pcg),
         "mixed_pcg_learned": dict(precision=sg.Precision.Mixed, solver=sg.Solver.SchurPcgLearned,
                                   precond_weights=weights, **pcg),
+        # forcing-sequence CG tolerance: 0.1 sqrt(|g| / |g0|), floored at 1e-8
+        "f64_pcg_jacobi_forcing": dict(solver=sg.Solver.SchurPcgJacobi, pcg_forcing=0.1, **pcg),
+        "f64_pcg_learned_forcing": dict(solver=sg.Solver.SchurPcgLearned, precond_weights=weights,
+                                        pcg_forcing=0.1, **pcg),
+        "mixed_pcg_jacobi_forcing": dict(precision=sg.Precision.Mixed, solver=sg.Solver.SchurPcgJacobi,
+                                         pcg_forcing=0.1, **pcg),
+        "mixed_pcg_learned_forcing": dict(precision=sg.Precision.Mixed, solver=sg.Solver.SchurPcgLearned,
+                                          precond_weights=weights, pcg_forcing=0.1, **pcg),
     }
     if weights is None:
         paths = {k: v for k, v in paths.items() if "learned" not in k}
@@ -176,22 +185,33 @@ def time_to_tol_leg(sg, problem, n_obs) -> dict:
     base = tol["paths"]["f64_ldlt"]
     for k, v in tol["paths"].items():
         v["speedup_vs_f64_ldlt"] = (base["seconds"] / v["seconds"]) if v["reached"] else None
-    num = tol["paths"].get("mixed_pcg_learned")
+    # headline: learned PCG + FP32 assembly, with the forcing-sequence CG tolerance (the
+    # metric fixes the cost tolerance, not the CG one); the fixed-tolerance run beside it
+    name = "mixed_pcg_learned_forcing"
+    num = tol["paths"].get(name)
+    definition = f"t(f64_ldlt to F*(1+1e-6)) / t({name} to F*(1+1e-6))"
     if num is None:
         tol["headline"] = {"value": None, "reason": "no trained weights (precond_weights.f32)"}
     elif num["reached"]:
-        tol["headline"] = {"value": base["seconds"] / num["seconds"],
-                           "definition": "t(f64_ldlt to F*(1+1e-6)) / t(mixed_pcg_learned to F*(1+1e-6))",
-                           "target": 3.0}
+        tol["headline"] = {"value": base["seconds"] / num["seconds"], "definition": definition, "target": 3.0}
     else:
         tol["headline"] = {
-            "value": None, "target": 3.0,
-            "definition": "t(f64_ldlt to F*(1+1e-6)) / t(mixed_pcg_learned to F*(1+1e-6))",
-            "reason": (f"learned+FP32 (mixed_pcg_learned) ends at {num['final_rel_to_f_star']:+.3g} relative "
+            "value": None, "target": 3.0, "definition": definition,
+            "reason": (f"learned+FP32 ({name}) ends at {num['final_rel_to_f_star']:+.3g} relative "
                        f"to F* ({num['termination']} after {num['iterations_run']} iterations, "
                        f"{num['rejected_steps']} rejected) and never reaches F*(1+1e-6); per iteration it "
                        f"costs {num['ms_per_iteration']:.2f} ms against {base['ms_per_iteration']:.2f} ms "
                        f"for FP64 direct")}
+    fixed = tol["paths"].get("mixed_pcg_learned")
+    if fixed is not None:
+        tol["headline"]["fixed_cg_tolerance"] = {
+            "path": "mixed_pcg_learned", "reached": fixed["reached"],
+            "value": (base["seconds"] / fixed["seconds"]) if fixed["reached"] else None}
+    fp64 = tol["paths"].get("f64_pcg_learned_forcing")
+    if fp64 is not None:
+        tol["headline"]["fp64_assembly"] = {
+            "path": "f64_pcg_learned_forcing", "reached": fp64["reached"],
+            "value": (base["seconds"] / fp64["seconds"]) if fp64["reached"] else None}
     return tol
 
 

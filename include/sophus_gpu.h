@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 4
+#define SGPU_ABI_VERSION 5
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -165,6 +165,13 @@ typedef struct sgpu_opt_params {
   /* SCHUR_PCG_LEARNED only: SGPU_PRECOND_WEIGHTS FP32 weights of the block-Jacobi
    * correction (copied at create; see sgpu_precond_train).  Default NULL. */
   const float* precond_weights;
+  /* PCG solvers: forcing-sequence CG tolerance (an inexact-Newton choice; no reference
+   * counterpart).  0 (default): every LM step solves to pcg_relative_tolerance.  eta > 0:
+   * the step's relative CG tolerance is max(pcg_relative_tolerance, eta * min(1,
+   * sqrt(|g|_inf / |g0|_inf))), g the gradient of the last linearization whose scalars the
+   * host has read (the previous step's), g0 the first one's: loose while the gradient is
+   * large, tightening as it falls. */
+  double pcg_forcing;
 } sgpu_opt_params;
 
 /* StepInfo (optimizer_types.rs:67-83) plus per-step diagnostics. */
@@ -436,14 +443,20 @@ typedef struct sgpu_precond_train_params {
   int32_t device;
   double learning_rate;  /* Adam */
   uint64_t seed;
+  int32_t batch_snapshots;   /* snapshots per Adam step (0: 1) */
+  int32_t holdout_problems;  /* further problems (the seeds after the training ones) whose
+                                snapshots are held out for the validation loss */
 } sgpu_precond_train_params;
 
-/* Trains the correction on the GPU (hand-written forward, backward and Adam kernels)
- * from RCS snapshots the solver generates on seeded problems.  weights_inout holds
- * the initial weights and receives the trained ones; loss_history (epochs + 1 entries)
- * receives the mean loss before training and after each epoch. */
+/* Trains the correction on the GPU (hand-written forward, backward and Adam kernels; the
+ * layer products and weight gradients on the matrix cores) from RCS snapshots the solver
+ * generates on seeded problems.  Each snapshot is kept as its camera blocks' features and
+ * `probes` fixed probe pairs (w, u~); an Adam step takes batch_snapshots snapshots in a
+ * shuffled order.  weights_inout holds the initial weights and receives the trained ones;
+ * loss_history and val_history (epochs + 1 entries each, may be NULL) receive the mean
+ * training and held-out loss before training and after each epoch. */
 sgpu_status sgpu_precond_train(const sgpu_precond_train_params* params, float* weights_inout,
-                               double* loss_history, int64_t* num_snapshots);
+                               double* loss_history, double* val_history, int64_t* num_snapshots);
 
 /* ---- batched small windows (no reference counterpart) --------------------------
  * Many independent bundle-adjustment windows solved together: per window one shared

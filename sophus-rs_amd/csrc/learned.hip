@@ -156,52 +156,93 @@ __global__ void __launch_bounds__(kT) k_features(DevPcg g, float* __restrict__ f
 }
 
 // ---- MLP -----------------------------------------------------------------------------
-// Y[n x M] = act(X[n x K] W^T + b), one thread per output; FP32 accumulation.
-template <bool kBf16, bool kTanh>
+// Every layer product runs on the matrix cores, one wavefront per 16 x 16 tile of the
+// output (the workgroup's four wavefronts take four neighbouring column tiles):
+//   C[i][j] = epi(sum_k A(i, k) B(k, j)),  A(i, k) = A[i sai + k sak],  B(k, j) = B[k sbk + j sbj]
+// with the strides choosing the transposes: the forward layers act(X W^T + b), the hidden
+// backward products (dA_next W) (1 - H^2), and the weight gradients dA^T X (the long
+// reduction over the rows of the batch; the bias gradient against a stride-0 vector of ones).
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+typedef short mfma_s4 __attribute__((ext_vector_type(4)));
+enum GemmEpi { kEpiPlain = 0, kEpiTanh = 1, kEpiDtanh = 2 };
+
+// FP32 on v_mfma_f32_16x16x4f32: A lane l holds A(i0 + l % 16, k + l / 16), B lane l holds
+// B(k + l / 16, j0 + l % 16); accumulator component q is C(i0 + 4 (l / 16) + q, j0 + l % 16).
+template <int kEpi>
 __global__ void __launch_bounds__(kT)
-    k_dense(int64_t n, int K, int M, const float* __restrict__ X, const float* __restrict__ W32,
-            const uint16_t* __restrict__ W16, int off_w, int off_b, float* __restrict__ Y) {
-  const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
-  if (t >= n * M) return;
-  const int64_t b = t / M;
-  const int m = int(t - b * M);
-  const float* x = X + b * K;
-  float acc = kBf16 ? bf16_to_f32(W16[off_b + m]) : W32[off_b + m];
-  for (int k = 0; k < K; ++k) {
-    const float w = kBf16 ? bf16_to_f32(W16[off_w + m * K + k]) : W32[off_w + m * K + k];
-    acc += w * x[k];
+    k_gemm_f32(int64_t M, int N, int64_t K, const float* __restrict__ A, int64_t sai, int64_t sak,
+               const float* __restrict__ B, int64_t sbk, int64_t sbj, const float* __restrict__ bias,
+               const float* __restrict__ H, float* __restrict__ C, int64_t ldc) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lc = lane >> 4;
+  const int64_t i0 = int64_t(blockIdx.x) * 16;
+  const int j0 = (int(blockIdx.y) * 4 + wv) * 16;
+  if (j0 >= N) return;  // whole wavefront; no barriers here
+  const bool iok = i0 + lr < M, jok = j0 + lr < N;
+  const float* Ai = A + (iok ? i0 + lr : 0) * sai;
+  const float* Bj = B + int64_t(jok ? j0 + lr : 0) * sbj;
+  mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll 4
+  for (int64_t k0 = 0; k0 < K; k0 += 4) {
+    const int64_t k = k0 + lc;
+    const float a = iok && k < K ? Ai[k * sak] : 0.0f;
+    const float b = jok && k < K ? Bj[k * sbk] : 0.0f;
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
   }
-  Y[t] = kTanh ? tanhf(acc) : acc;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = i0 + 4 * lc + q;
+    const int j = j0 + lr;
+    if (i >= M || j >= N) continue;
+    float v = acc[q];
+    if (kEpi != kEpiDtanh && bias) v += bias[j];
+    if (kEpi == kEpiTanh) v = tanhf(v);
+    if (kEpi == kEpiDtanh) {
+      const float h = H[i * ldc + j];
+      v *= 1.0f - h * h;
+    }
+    C[i * ldc + j] = v;
+  }
 }
 
-// dA[b][m] = (sum_i dAn[b][i] Wn[i][m]) (1 - H[b][m]^2)
-__global__ void __launch_bounds__(kT)
-    k_back_hidden(int64_t n, int Mn, int M, const float* __restrict__ dAn, const float* __restrict__ W32,
-                  int off_wn, const float* __restrict__ H, float* __restrict__ dA) {
-  const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
-  if (t >= n * M) return;
-  const int64_t b = t / M;
-  const int m = int(t - b * M);
-  float s = 0.0f;
-  for (int i = 0; i < Mn; ++i) s += dAn[b * Mn + i] * W32[off_wn + i * M + m];
-  const float h = H[t];
-  dA[t] = s * (1.0f - h * h);
+__device__ __forceinline__ short f32_to_bf16(float x) {
+  const uint32_t u = __float_as_uint(x);
+  return short((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);  // round to nearest even
 }
 
-// grad W[m][k] = sum_b dA[b][m] X[b][k]  (k == K: the bias), fixed order over b.
+// Inference layer Y = act(X W^T + b) with bf16 weights (W: N x K row-major, b: N) on
+// v_mfma_f32_16x16x16bf16_1k, the activations rounded to bf16 as they are loaded, FP32
+// accumulation: A lane l holds X(i0 + l % 16, k + 4 (l / 16) .. + 3), B lane l holds
+// W(j0 + l % 16, k + 4 (l / 16) .. + 3); accumulator component q is Y(i0 + 4 (l / 16) + q,
+// j0 + l % 16).
+template <bool kTanh>
 __global__ void __launch_bounds__(kT)
-    k_weight_grad(int64_t n, int M, int K, const float* __restrict__ dA, const float* __restrict__ X,
-                  int off_w, int off_b, float* __restrict__ grad) {
-  const int t = blockIdx.x * kT + threadIdx.x;
-  if (t >= M * (K + 1)) return;
-  const int m = t / (K + 1), k = t - m * (K + 1);
-  float s = 0.0f;
-  if (k < K) {
-    for (int64_t b = 0; b < n; ++b) s += dA[b * M + m] * X[b * K + k];
-    grad[off_w + m * K + k] = s;
-  } else {
-    for (int64_t b = 0; b < n; ++b) s += dA[b * M + m];
-    grad[off_b + m] = s;
+    k_dense_bf16(int64_t M, int N, int K, const float* __restrict__ X, const uint16_t* __restrict__ W16,
+                 int off_w, int off_b, float* __restrict__ Y) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, lr = lane & 15, lc = lane >> 4;
+  const int64_t i0 = int64_t(blockIdx.x) * 16;
+  const int j0 = (int(blockIdx.y) * 4 + wv) * 16;
+  if (j0 >= N) return;
+  const bool iok = i0 + lr < M, jok = j0 + lr < N;
+  const float* x = X + (iok ? i0 + lr : 0) * K;
+  const uint16_t* w = W16 + off_w + (jok ? j0 + lr : 0) * K;
+  mfma_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    mfma_s4 a, b;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int k = k0 + 4 * lc + e;
+      a[e] = iok && k < K ? f32_to_bf16(x[k]) : short(0);
+      b[e] = jok && k < K ? short(w[k]) : short(0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t i = i0 + 4 * lc + q;
+    const int j = j0 + lr;
+    if (i >= M || j >= N) continue;
+    const float v = acc[q] + bf16_to_f32(W16[off_b + j]);
+    Y[i * N + j] = kTanh ? tanhf(v) : v;
   }
 }
 
@@ -385,6 +426,31 @@ __global__ void __launch_bounds__(kT)
 
 inline unsigned grid_for(int64_t n) { return unsigned((n + kT - 1) / kT); }
 
+__global__ void __launch_bounds__(kT) k_fill_f32(int64_t n, float* __restrict__ p, float v) {
+  const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Row rowmap[t] of the training data set to row t of the batch buffers (the probe arrays
+// are probe-major: probe q of row r at 6 (q cap + r)).
+__global__ void __launch_bounds__(kT)
+    k_gather_rows(int64_t n, const int64_t* __restrict__ rowmap, int64_t rows_cap, int probes,
+                  const float* __restrict__ feat, const int32_t* __restrict__ ok,
+                  const double* __restrict__ w, const double* __restrict__ ut, float* __restrict__ feat_out,
+                  int32_t* __restrict__ ok_out, double* __restrict__ w_out, double* __restrict__ ut_out,
+                  int64_t out_cap) {
+  const int64_t t = int64_t(blockIdx.x) * kT + threadIdx.x;
+  if (t >= n) return;
+  const int64_t r = rowmap[t];
+  for (int k = 0; k < kPcFeat; ++k) feat_out[int64_t(kPcFeat) * t + k] = feat[int64_t(kPcFeat) * r + k];
+  ok_out[t] = ok[r];
+  for (int q = 0; q < probes; ++q)
+    for (int k = 0; k < 6; ++k) {
+      w_out[6 * (q * out_cap + t) + k] = w[6 * (q * rows_cap + r) + k];
+      ut_out[6 * (q * out_cap + t) + k] = ut[6 * (q * rows_cap + r) + k];
+    }
+}
+
 }  // namespace
 
 std::vector<uint16_t> weights_to_bf16(const float* w, int64_t n) {
@@ -406,23 +472,35 @@ void launch_precond_features(hipStream_t s, const DevPcg& g, float* feat) {
   hipLaunchKernelGGL(k_features, dim3((g.nb + kRows - 1) / kRows), dim3(kT), 0, s, g, feat);
 }
 
+static dim3 gemm_grid(int64_t M, int N) {
+  return dim3(unsigned((M + 15) / 16), unsigned((N + 63) / 64));
+}
+
+// act(X W^T + b): X n x K row-major, W (at off_w) M x K row-major, b (at off_b).
+template <int kEpi>
+static void dense_f32(hipStream_t s, int64_t n, int K, int M, const float* X, const float* w32, int off_w,
+                      int off_b, float* Y) {
+  hipLaunchKernelGGL((k_gemm_f32<kEpi>), gemm_grid(n, M), dim3(kT), 0, s, n, M, int64_t(K), X, int64_t(K),
+                     int64_t(1), w32 + off_w, int64_t(1), int64_t(K), w32 + off_b, nullptr, Y, int64_t(M));
+}
+
+template <bool kTanh>
+static void dense_bf16(hipStream_t s, int64_t n, int K, int M, const float* X, const uint16_t* w16,
+                       int off_w, int off_b, float* Y) {
+  hipLaunchKernelGGL((k_dense_bf16<kTanh>), gemm_grid(n, M), dim3(kT), 0, s, n, M, K, X, w16, off_w, off_b, Y);
+}
+
 void launch_mlp_forward(hipStream_t s, int64_t n, const float* feat, const float* w32,
                         const uint16_t* wbf16, float* h1, float* h2, float* out) {
   if (n <= 0) return;
   if (wbf16) {
-    hipLaunchKernelGGL((k_dense<true, true>), dim3(grid_for(n * kPcHid)), dim3(kT), 0, s, n, kPcFeat,
-                       kPcHid, feat, nullptr, wbf16, kPcOffW1, kPcOffB1, h1);
-    hipLaunchKernelGGL((k_dense<true, true>), dim3(grid_for(n * kPcHid)), dim3(kT), 0, s, n, kPcHid,
-                       kPcHid, h1, nullptr, wbf16, kPcOffW2, kPcOffB2, h2);
-    hipLaunchKernelGGL((k_dense<true, false>), dim3(grid_for(n * kPcOut)), dim3(kT), 0, s, n, kPcHid,
-                       kPcOut, h2, nullptr, wbf16, kPcOffW3, kPcOffB3, out);
+    dense_bf16<true>(s, n, kPcFeat, kPcHid, feat, wbf16, kPcOffW1, kPcOffB1, h1);
+    dense_bf16<true>(s, n, kPcHid, kPcHid, h1, wbf16, kPcOffW2, kPcOffB2, h2);
+    dense_bf16<false>(s, n, kPcHid, kPcOut, h2, wbf16, kPcOffW3, kPcOffB3, out);
   } else {
-    hipLaunchKernelGGL((k_dense<false, true>), dim3(grid_for(n * kPcHid)), dim3(kT), 0, s, n, kPcFeat,
-                       kPcHid, feat, w32, nullptr, kPcOffW1, kPcOffB1, h1);
-    hipLaunchKernelGGL((k_dense<false, true>), dim3(grid_for(n * kPcHid)), dim3(kT), 0, s, n, kPcHid,
-                       kPcHid, h1, w32, nullptr, kPcOffW2, kPcOffB2, h2);
-    hipLaunchKernelGGL((k_dense<false, false>), dim3(grid_for(n * kPcOut)), dim3(kT), 0, s, n, kPcHid,
-                       kPcOut, h2, w32, nullptr, kPcOffW3, kPcOffB3, out);
+    dense_f32<kEpiTanh>(s, n, kPcFeat, kPcHid, feat, w32, kPcOffW1, kPcOffB1, h1);
+    dense_f32<kEpiTanh>(s, n, kPcHid, kPcHid, h1, w32, kPcOffW2, kPcOffB2, h2);
+    dense_f32<kEpiPlain>(s, n, kPcHid, kPcOut, h2, w32, kPcOffW3, kPcOffB3, out);
   }
 }
 
@@ -437,6 +515,17 @@ void launch_learned_preconditioner(hipStream_t s, const DevPcg& g, const DevLear
 }
 
 // ---- training pieces -----------------------------------------------------------------------
+
+void launch_fill_f32(hipStream_t s, float* p, int64_t n, float v) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n)), dim3(kT), 0, s, n, p, v);
+}
+void launch_gather_rows(hipStream_t s, int64_t n, const int64_t* rowmap, int64_t rows_cap, int probes,
+                        const float* feat, const int32_t* ok, const double* w, const double* ut,
+                        float* feat_out, int32_t* ok_out, double* w_out, double* ut_out, int64_t out_cap) {
+  if (n > 0)
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for(n)), dim3(kT), 0, s, n, rowmap, rows_cap, probes, feat,
+                       ok, w, ut, feat_out, ok_out, w_out, ut_out, out_cap);
+}
 
 void launch_whiten(hipStream_t s, const DevPcg& g, double* Bm, int32_t* ok) {
   hipLaunchKernelGGL(k_whiten, dim3(grid_for(g.nb)), dim3(kT), 0, s, g, Bm, ok);
@@ -455,19 +544,34 @@ void launch_loss_grad(hipStream_t s, int64_t n, const int32_t* ok, const double*
   hipLaunchKernelGGL(k_loss_grad, dim3(grid_for(n)), dim3(kT), 0, s, n, ok, ut, w, out, dO, loss,
                      accumulate);
 }
+// grad W (M x K at off_w) = dA^T X and grad b (at off_b) = dA^T 1: the reduction over the n
+// rows of the batch runs down the matrix-core k dimension.
+static void weight_grad(hipStream_t s, int64_t n, int M, int K, const float* dA, const float* X,
+                        const float* ones, int off_w, int off_b, float* grad) {
+  hipLaunchKernelGGL((k_gemm_f32<kEpiPlain>), gemm_grid(M, K), dim3(kT), 0, s, int64_t(M), K, n, dA,
+                     int64_t(1), int64_t(M), X, int64_t(K), int64_t(1), nullptr, nullptr, grad + off_w,
+                     int64_t(K));
+  hipLaunchKernelGGL((k_gemm_f32<kEpiPlain>), gemm_grid(M, 1), dim3(kT), 0, s, int64_t(M), 1, n, dA,
+                     int64_t(1), int64_t(M), ones, int64_t(1), int64_t(0), nullptr, nullptr, grad + off_b,
+                     int64_t(1));
+}
+
+// dA (n x M) = (dAn (n x Mn) Wn (Mn x M at off_wn)) (1 - H^2)
+static void back_hidden(hipStream_t s, int64_t n, int Mn, int M, const float* dAn, const float* w32,
+                        int off_wn, const float* H, float* dA) {
+  hipLaunchKernelGGL((k_gemm_f32<kEpiDtanh>), gemm_grid(n, M), dim3(kT), 0, s, n, M, int64_t(Mn), dAn,
+                     int64_t(Mn), int64_t(1), w32 + off_wn, int64_t(M), int64_t(1), nullptr, H, dA,
+                     int64_t(M));
+}
+
 void launch_mlp_backward(hipStream_t s, int64_t n, const float* feat, const float* w32, const float* h1,
-                         const float* h2, const float* dO, float* dA2, float* dA1, float* grad) {
-  // layer 3 (linear): grad W3, b3 from dO and h2
-  hipLaunchKernelGGL(k_weight_grad, dim3(grid_for(kPcOut * (kPcHid + 1))), dim3(kT), 0, s, n, kPcOut,
-                     kPcHid, dO, h2, kPcOffW3, kPcOffB3, grad);
-  hipLaunchKernelGGL(k_back_hidden, dim3(grid_for(n * kPcHid)), dim3(kT), 0, s, n, kPcOut, kPcHid, dO,
-                     w32, kPcOffW3, h2, dA2);
-  hipLaunchKernelGGL(k_weight_grad, dim3(grid_for(kPcHid * (kPcHid + 1))), dim3(kT), 0, s, n, kPcHid,
-                     kPcHid, dA2, h1, kPcOffW2, kPcOffB2, grad);
-  hipLaunchKernelGGL(k_back_hidden, dim3(grid_for(n * kPcHid)), dim3(kT), 0, s, n, kPcHid, kPcHid, dA2,
-                     w32, kPcOffW2, h1, dA1);
-  hipLaunchKernelGGL(k_weight_grad, dim3(grid_for(kPcHid * (kPcFeat + 1))), dim3(kT), 0, s, n, kPcHid,
-                     kPcFeat, dA1, feat, kPcOffW1, kPcOffB1, grad);
+                         const float* h2, const float* dO, const float* ones, float* dA2, float* dA1,
+                         float* grad) {
+  weight_grad(s, n, kPcOut, kPcHid, dO, h2, ones, kPcOffW3, kPcOffB3, grad);
+  back_hidden(s, n, kPcOut, kPcHid, dO, w32, kPcOffW3, h2, dA2);
+  weight_grad(s, n, kPcHid, kPcHid, dA2, h1, ones, kPcOffW2, kPcOffB2, grad);
+  back_hidden(s, n, kPcHid, kPcHid, dA2, w32, kPcOffW2, h1, dA1);
+  weight_grad(s, n, kPcHid, kPcFeat, dA1, feat, ones, kPcOffW1, kPcOffB1, grad);
 }
 void launch_adam(hipStream_t s, float* w, const float* g, float* m, float* v, float lr, int step,
                  float scale) {
@@ -533,6 +637,8 @@ void precond_loss_grad_host(int64_t n, const float* features, const double* diag
   float* dA1 = a.alloc<float>(size_t(kPcHid) * n);
   float* g = a.alloc<float>(kPcWeights);
   double* l = a.alloc<double>(size_t(n));
+  float* ones = a.alloc<float>(size_t(n));
+  launch_fill_f32(nullptr, ones, n, 1.0f);
   SGPU_HIP(hipMemcpy(dF, features, size_t(kPcFeat) * n * 4, hipMemcpyHostToDevice));
   SGPU_HIP(hipMemcpy(dD, diag_blocks, size_t(36) * n * 8, hipMemcpyHostToDevice));
   SGPU_HIP(hipMemcpy(dU, u, size_t(6) * n * 8, hipMemcpyHostToDevice));
@@ -542,7 +648,7 @@ void precond_loss_grad_host(int64_t n, const float* features, const double* diag
   launch_utilde(nullptr, n, dB, dU, dUt);
   launch_mlp_forward(nullptr, n, dF, w32, nullptr, h1, h2, out);
   launch_loss_grad(nullptr, n, ok, dUt, dW, out, dO, l, 0);
-  launch_mlp_backward(nullptr, n, dF, w32, h1, h2, dO, dA2, dA1, g);
+  launch_mlp_backward(nullptr, n, dF, w32, h1, h2, dO, ones, dA2, dA1, g);
   SGPU_HIP(hipGetLastError());
   std::vector<double> hl(static_cast<size_t>(n));
   SGPU_HIP(hipMemcpy(hl.data(), l, size_t(n) * 8, hipMemcpyDeviceToHost));

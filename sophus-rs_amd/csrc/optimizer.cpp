@@ -85,6 +85,7 @@ struct sgpu_optimizer {
   DevLearned learned{};        // learned block-Jacobi correction (PCG_LEARNED only)
   double merit = 0, nu = 0;
   int64_t rejects = 0, steps = 0;
+  double grad0 = -1.0, grad_last = -1.0;  // max |g| of the first and the last linearization read
   bool terminated = false;
   int32_t last_termination = SGPU_TERM_NONE;
   // timing: 0 off, 1 per phase, 2 also per k_mf_step launch
@@ -198,6 +199,8 @@ struct sgpu_optimizer {
       throw Error(SGPU_ERR_UNSUPPORTED, "solver not available");
     if (iterative() && !(params.pcg_relative_tolerance > 0.0 && params.pcg_max_iterations > 0))
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "PCG needs a positive tolerance and iteration cap");
+    if (!(params.pcg_forcing >= 0.0 && params.pcg_forcing < 1.0))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "pcg_forcing must be in [0, 1)");
     if (params.solver == SGPU_SOLVER_SCHUR_PCG_LEARNED && !params.precond_weights)
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "the learned preconditioner needs precond_weights");
     const float* precond_weights = params.precond_weights;
@@ -385,6 +388,16 @@ struct sgpu_optimizer {
     merit = compute_merit(poses[0], points[0]);
   }
 
+  // The CG tolerance of this step: pcg_relative_tolerance, or with pcg_forcing = eta > 0
+  // the forcing term max(tol, eta min(1, sqrt(|g| / |g0|))) from the gradients the host has
+  // read (the previous step's linearization and the first one).
+  double pcg_tolerance() const {
+    const double tol = params.pcg_relative_tolerance;
+    if (!(params.pcg_forcing > 0.0)) return tol;
+    const double ratio = grad0 > 0.0 && grad_last >= 0.0 ? std::sqrt(grad_last / grad0) : 1.0;
+    return std::max(tol, params.pcg_forcing * std::min(1.0, ratio));
+  }
+
   void step(sgpu_step_info* info) {
     if (terminated) throw Error(SGPU_ERR_TERMINATED, "optimizer already terminated");
     const auto t0 = std::chrono::steady_clock::now();
@@ -394,13 +407,13 @@ struct sgpu_optimizer {
     linearize(poses[cur], points[cur], nu, used);
     SGPU_HIP(hipEventRecord(phase_ev[1], stream));
     if (iterative()) {
-      // block-Jacobi PCG on the resident S, one cooperative launch; its iteration count
+      // block-Jacobi PCG on the resident S (two launches per CG iteration); its iteration count
       // and breakdown flag travel back with the step scalars
       tic(K_FACTOR);
       launch_pcg_setup(stream, pcg);
       if (params.solver == SGPU_SOLVER_SCHUR_PCG_LEARNED)
         launch_learned_preconditioner(stream, pcg, learned);
-      launch_pcg(stream, pcg, dl.rhs_nat, params.pcg_relative_tolerance,
+      launch_pcg(stream, pcg, dl.rhs_nat, pcg_tolerance(),
                  int32_t(std::min<int64_t>(params.pcg_max_iterations, INT32_MAX)),
                  &d_sc->pcg_iterations, &d_sc->bad_pivot, h_pcg);
       toc(K_FACTOR);
@@ -434,6 +447,8 @@ struct sgpu_optimizer {
     used[K_FACTOR] = used[K_TRIANGULAR] = used[K_BACKSUB] = used[K_UPDATE] = true;
     collect(used);
     const DevScalars sc = *h_sc;
+    if (grad0 < 0.0) grad0 = bits_to_double(sc.max_grad_bits);
+    grad_last = bits_to_double(sc.max_grad_bits);
     // LM control, step_nlls_inner order
     int reason = SGPU_TERM_NONE;
     int accepted = 0;

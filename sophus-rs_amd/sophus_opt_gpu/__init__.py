@@ -76,6 +76,7 @@ class _OptParams(C.Structure):
         ("device", C.c_int32),
         ("reserved0", C.c_int32),
         ("precond_weights", C.POINTER(C.c_float)),
+        ("pcg_forcing", C.c_double),
     ]
 
 
@@ -167,7 +168,7 @@ class _PrecondTrainParams(C.Structure):
     _fields_ = [("num_problems", C.c_int32), ("num_cameras", C.c_int32), ("num_points", C.c_int64),
                 ("geometric_p", C.c_double), ("lm_iterations", C.c_int32), ("epochs", C.c_int32),
                 ("probes", C.c_int32), ("device", C.c_int32), ("learning_rate", C.c_double),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("batch_snapshots", C.c_int32), ("holdout_problems", C.c_int32)]
 
 
 _DP = C.POINTER(C.c_double)
@@ -221,7 +222,7 @@ _SIGNATURES = {
     "sgpu_precond_forward": (C.c_int, [C.c_int64, _FP, _FP, C.c_int32, _FP]),
     "sgpu_precond_blocks": (C.c_int, [C.c_int64, _DP, _FP, _DP]),
     "sgpu_precond_loss_grad": (C.c_int, [C.c_int64, _FP, _DP, _DP, _DP, _FP, _DP, _FP]),
-    "sgpu_precond_train": (C.c_int, [C.POINTER(_PrecondTrainParams), _FP, _DP, _I64P]),
+    "sgpu_precond_train": (C.c_int, [C.POINTER(_PrecondTrainParams), _FP, _DP, _DP, _I64P]),
     "sgpu_batch_create": (C.c_int, [C.POINTER(_BatchDesc), C.POINTER(_OptParams), C.POINTER(C.c_void_p)]),
     "sgpu_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
     "sgpu_batch_reset": (C.c_int, [C.c_void_p]),
@@ -234,7 +235,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 4  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 5  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -480,6 +481,7 @@ class OptParams:
     pcg_max_iterations: int = 500
     device: int = 0
     precond_weights: Optional[np.ndarray] = None  # SchurPcgLearned: PRECOND_WEIGHTS float32
+    pcg_forcing: float = 0.0  # PCG: forcing-sequence CG tolerance (0: off)
 
     def _c(self) -> _OptParams:
         p = _OptParams()
@@ -497,6 +499,7 @@ class OptParams:
         p.pcg_relative_tolerance = float(self.pcg_relative_tolerance)
         p.pcg_max_iterations = int(self.pcg_max_iterations)
         p.device = int(self.device)
+        p.pcg_forcing = float(self.pcg_forcing)
         if self.precond_weights is not None:
             w = np.ascontiguousarray(self.precond_weights, dtype=np.float32).reshape(-1)
             if w.size != PRECOND_WEIGHTS:
@@ -895,6 +898,8 @@ class PrecondTrainParams:
     device: int = 0
     learning_rate: float = 1e-3
     seed: int = 1000
+    batch_snapshots: int = 8
+    holdout_problems: int = 2
 
     def _c(self) -> _PrecondTrainParams:
         c = _PrecondTrainParams()
@@ -904,14 +909,16 @@ class PrecondTrainParams:
 
 
 def train_preconditioner(params: PrecondTrainParams, weights: Optional[np.ndarray] = None):
-    """Train on the GPU; returns (weights, loss history (epochs + 1), snapshots used)."""
+    """Train on the GPU; returns (weights, training loss history (epochs + 1), held-out loss
+    history (epochs + 1), snapshots generated)."""
     w = (init_precond_weights(params.seed) if weights is None
          else np.array(weights, dtype=np.float32).reshape(-1).copy())
     hist = np.zeros(params.epochs + 1)
+    val = np.zeros(params.epochs + 1)
     n = C.c_int64()
     _check(library().sgpu_precond_train(C.byref(params._c()), w.ctypes.data_as(_FP), hist.ctypes.data_as(_DP),
-                                        C.byref(n)))
-    return w, hist, n.value
+                                        val.ctypes.data_as(_DP), C.byref(n)))
+    return w, hist, val, n.value
 
 
 # ---- batched small windows -----------------------------------------------------------------
