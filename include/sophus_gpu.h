@@ -19,7 +19,8 @@
  * Conventions (identical to the reference):
  *   - SE(3) members are 7 doubles (qw, qx, qy, qz, tx, ty, tz) = world_from_camera;
  *     tangent order [omega; nu] (rotation first); update T <- exp(delta) * T.
- *   - Vec3 members are 3 doubles; pinhole members are 4 doubles (fx, fy, cx, cy).
+ *   - Vec3 members are 3 doubles; pinhole members are 4 doubles (fx, fy, cx, cy); EUCM
+ *     members are 6 doubles (fx, fy, cx, cy, alpha, beta).
  *   - Family partitions: Free families in lexicographic name order, then
  *     Marginalized families; Conditioned families and const members get no blocks.
  *   - The cost has no 1/2 factor: cost = sum rho(|r|) (Huber) or |r|^2.
@@ -36,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 5
+#define SGPU_ABI_VERSION 6
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -75,7 +76,10 @@ typedef enum sgpu_var_kind {
 typedef enum sgpu_var_type {
   SGPU_VAR_ISOMETRY3 = 0, /* Isometry3F64: 7 params, 6 dof */
   SGPU_VAR_VEC3 = 1,      /* VecF64<3>:   3 params, 3 dof */
-  SGPU_VAR_PINHOLE = 2    /* PinholeCameraF64: 4 params, 4 dof (additive) */
+  SGPU_VAR_PINHOLE = 2,   /* PinholeCameraF64: 4 params, 4 dof (additive) */
+  /* EnhancedUnifiedCameraF64: 6 params (fx, fy, cx, cy, alpha, beta), 6 dof (additive;
+   * perspective_camera.rs:76-89, distortions/enhanced_unified.rs) */
+  SGPU_VAR_EUCM = 3
 } sgpu_var_type;
 
 /* Cost term kinds (closed set instead of HasResidualFn trait objects). */
@@ -85,7 +89,12 @@ typedef enum sgpu_cost_kind {
   SGPU_COST_PINHOLE_REPROJECTION = 0,
   /* PoseGraph3CostTerm: args (world_from_pose_m, world_from_pose_n);
    * measurement pose_m_from_pose_n[7].  pose_graph3.rs:33-79 */
-  SGPU_COST_POSE_GRAPH3 = 1
+  SGPU_COST_POSE_GRAPH3 = 1,
+  /* EucmCameraReprojectionCostTerm: args (intrinsics, world_from_camera, point);
+   * measurement uv[2].  example_problems/ba_problem.rs:45-110.  The intrinsics family may
+   * be Conditioned, or Free with one active member (one calibration shared by the
+   * observations: its block couples to every camera, so it is ordered last in the RCS) */
+  SGPU_COST_EUCM_REPROJECTION = 2
 } sgpu_cost_kind;
 
 typedef enum sgpu_robust_kernel {
@@ -274,7 +283,7 @@ sgpu_status sgpu_optimizer_inverse_block(sgpu_optimizer* opt, const char* row_fa
  * 335-373): a block of the min-norm inverse of the undamped final Hessian H.  Where H has
  * full rank this is H^-1, which the Schur formulas of sgpu_optimizer_inverse_block give
  * exactly (at nu = 0).  A rank-deficient H (an unfixed gauge: a non-positive pivot of S, or
- * a pivot below 1e-9 of the largest) returns SGPU_ERR_UNSUPPORTED:
+ * a pivot below 1e-9 of S's diagonal entry in its column) returns SGPU_ERR_UNSUPPORTED:
  * the reference then takes the min-norm inverse of the whole of H, which differs from the
  * Schur-complement blocks.  Requires skip_final_hessian = 0. */
 sgpu_status sgpu_optimizer_covariance_block(sgpu_optimizer* opt, const char* row_family,
@@ -359,6 +368,19 @@ sgpu_status sgpu_eval_pinhole_terms(int32_t precision, int64_t num_terms,
                                     double huber_delta, double* residual_out,
                                     double* jac_pose_out, double* jac_point_out,
                                     double* cost_out, double* weight_out);
+
+/* Kernel (a) for EUCM reprojection terms, FP64: EucmCameraReprojectionCostTerm::eval + make
+ * (example_problems/ba_problem.rs:68-110, evaluated_term.rs:286-327).  Per term t:
+ * residual_out[2t..], jac_intrinsics_out[12t..] (2x6 row-major, d/d(fx, fy, cx, cy, alpha,
+ * beta)), jac_pose_out[12t..], jac_point_out[6t..], cost_out[t], weight_out[t].  Any output
+ * pointer may be NULL. */
+sgpu_status sgpu_eval_eucm_terms(int64_t num_terms, const double* intrinsics /*6 per term*/,
+                                 const double* world_from_camera /*7 per term*/,
+                                 const double* point_in_world /*3 per term*/,
+                                 const double* uv /*2 per term*/, int32_t robust_kernel,
+                                 double huber_delta, double* residual_out,
+                                 double* jac_intrinsics_out, double* jac_pose_out,
+                                 double* jac_point_out, double* cost_out, double* weight_out);
 
 /* Kernel (a'): SE(3) between factor (pose_graph3.rs:33-79) on the GPU (FP64).
  * Per term: residual_out[6t..], jac_m_out[36t..], jac_n_out[36t..] (6x6 row-major). */
@@ -534,6 +556,17 @@ sgpu_status sgpu_generate_ba(const sgpu_ba_spec* spec, sgpu_ba_sizes* sizes,
                              double* poses_init, double* points_gt /*3 x pts*/,
                              double* points_init, int64_t* obs_idx /*3 x obs: cam,pose,pt*/,
                              double* obs_uv /*2 x obs*/);
+
+/* The reference's own BaProblem geometry (example_problems/ba_problem.rs:154-345) with this
+ * library's RNG: num_cameras on a ring of radius 8 looking into an open box of num_points
+ * candidate points, one shared EUCM camera; points seen fewer than twice are dropped.
+ * obs_idx rows are (intrinsics 0, pose, point).  Poses 0 and 1 are unperturbed (the gauge
+ * the reference holds constant).  Two-call protocol as above. */
+sgpu_status sgpu_generate_ba_eucm(int32_t num_cameras, int64_t num_points, uint64_t seed,
+                                  sgpu_ba_sizes* sizes, double* intrinsics_gt /*6*/,
+                                  double* intrinsics_init /*6*/, double* poses_gt,
+                                  double* poses_init, double* points_gt, double* points_init,
+                                  int64_t* obs_idx, double* obs_uv);
 
 #ifdef __cplusplus
 } /* extern "C" */

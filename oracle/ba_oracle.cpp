@@ -73,9 +73,14 @@ void run_threads(Fn&& fn) {
   fn(omp_get_thread_num(), omp_get_num_threads());
 }
 
-int var_dof(int type) { return type == SGPU_VAR_ISOMETRY3 ? 6 : type == SGPU_VAR_VEC3 ? 3 : 4; }
+int var_dof(int type) {
+  return type == SGPU_VAR_ISOMETRY3 || type == SGPU_VAR_EUCM ? 6 : type == SGPU_VAR_VEC3 ? 3 : 4;
+}
 int var_params(int type) {
-  return type == SGPU_VAR_ISOMETRY3 ? 7 : type == SGPU_VAR_VEC3 ? 3 : 4;
+  return type == SGPU_VAR_ISOMETRY3 ? 7 : type == SGPU_VAR_VEC3 ? 3 : type == SGPU_VAR_EUCM ? 6 : 4;
+}
+bool is_reprojection(int kind) {
+  return kind == SGPU_COST_PINHOLE_REPROJECTION || kind == SGPU_COST_EUCM_REPROJECTION;
 }
 
 // ---- robust kernel: crates/sophus_opt/src/robust_kernel.rs:13-35 -----------------------
@@ -176,7 +181,7 @@ struct CostFn {
   std::vector<int64_t> idx;   // sorted terms, arity per term
   std::vector<double> meas;   // 2 or 7 per term
   std::vector<int64_t> range_start;  // reduction ranges (sorted)
-  int mlen() const { return kind == SGPU_COST_PINHOLE_REPROJECTION ? 2 : 7; }
+  int mlen() const { return is_reprojection(kind) ? 2 : 7; }
   int64_t num_terms() const { return int64_t(idx.size()) / arity; }
 };
 
@@ -216,6 +221,66 @@ void eval_pinhole(const double* intr, const double* pose, const double* xw, cons
     for (int k = 0; k < 8; ++k) Jc[k] = -v[k];
   }
   if (d_pose) {  // A * [hat(-x_w) | I]  (affine_group_template.rs:505-510, rotation3.rs:575-577)
+    const double nx[3] = {-xw[0], -xw[1], -xw[2]};
+    double H[9];
+    hat(nx, H);
+    for (int i = 0; i < 2; ++i) {
+      for (int j = 0; j < 3; ++j)
+        Jp[6 * i + j] = A[3 * i] * H[j] + A[3 * i + 1] * H[3 + j] + A[3 * i + 2] * H[6 + j];
+      for (int j = 0; j < 3; ++j) Jp[6 * i + 3 + j] = A[3 * i + j];
+    }
+  }
+  if (d_pt)
+    for (int k = 0; k < 6; ++k) Jx[k] = -A[k];
+}
+
+// EucmCameraReprojectionCostTerm::eval (example_problems/ba_problem.rs:68-110), with the
+// analytic derivatives of crates/sophus_sensor/src/distortions/enhanced_unified.rs:125-215:
+//   d_pi_r_cw = dx_distort_x(proj) * dx_proj_x(x_c) * R_cw          (2 x 3)
+//   J_intr = -dx_distort_params(proj)                                (2 x 6)
+//   J_pose = d_pi_r_cw * dx_exp_x_times_point_at_0(x_w) = d_pi_r_cw [hat(-x_w) | I]
+//   J_point = -d_pi_r_cw
+void eval_eucm(const double* p, const double* pose, const double* xw, const double* uv,
+               bool d_intr, bool d_pose, bool d_pt, double r[2], double Jc[12], double Jp[12],
+               double Jx[6]) {
+  double cw[7], xc[3];
+  se3_inverse(pose, cw);
+  se3_transform(cw, xw, xc);
+  const double x = xc[0] / xc[2], y = xc[1] / xc[2];
+  const double fx = p[0], fy = p[1], alpha = p[4], beta = p[5];
+  const double r2 = x * x + y * y;
+  const double rho2 = beta * r2 + 1.0;
+  const double rho = std::sqrt(rho2);
+  const double norm = alpha * rho + (1.0 - alpha);
+  const double mx = x / norm, my = y / norm;
+  r[0] = uv[0] - (fx * mx + p[2]);
+  r[1] = uv[1] - (fy * my + p[3]);
+  const double nn = norm * norm;
+  // dx_distort_x (enhanced_unified.rs:125-162)
+  const double drho_dx = (2.0 * beta * x) / (2.0 * rho), drho_dy = (2.0 * beta * y) / (2.0 * rho);
+  const double dnorm_dx = alpha * drho_dx, dnorm_dy = alpha * drho_dy;
+  const double Dd[4] = {fx * ((norm - x * dnorm_dx) / nn), fx * (-x * dnorm_dy / nn),
+                        fy * (-y * dnorm_dx / nn), fy * ((norm - y * dnorm_dy) / nn)};
+  // dx_proj_x (perspective.rs:54-72)
+  const double iz = 1.0 / xc[2], zz = xc[2] * xc[2];
+  const double Dp[6] = {iz, 0.0, -xc[0] / zz, 0.0, iz, -xc[1] / zz};
+  double DD[6];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 3; ++j) DD[3 * i + j] = Dd[2 * i] * Dp[j] + Dd[2 * i + 1] * Dp[3 + j];
+  double R[9];
+  so3_matrix(cw, R);
+  double A[6];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 3; ++j)
+      A[3 * i + j] = DD[3 * i] * R[j] + DD[3 * i + 1] * R[3 + j] + DD[3 * i + 2] * R[6 + j];
+  if (d_intr) {  // -dx_distort_params (enhanced_unified.rs:164-215)
+    const double drho_dbeta = r2 / (2.0 * rho);
+    const double dnorm_dalpha = rho - 1.0, dnorm_dbeta = alpha * drho_dbeta;
+    const double v[12] = {mx,  0.0, 1.0, 0.0, fx * (-x * dnorm_dalpha / nn), fx * (-x * dnorm_dbeta / nn),
+                          0.0, my,  0.0, 1.0, fy * (-y * dnorm_dalpha / nn), fy * (-y * dnorm_dbeta / nn)};
+    for (int k = 0; k < 12; ++k) Jc[k] = -v[k];
+  }
+  if (d_pose) {
     const double nx[3] = {-xw[0], -xw[1], -xw[2]};
     double H[9];
     hat(nx, H);
@@ -275,6 +340,12 @@ void eval_term(const Problem& P, const Vars& V, const CostFn& c, int64_t t, bool
     const double* pose = &V.fam[c.fam[1]].members[ix[1] * 7];
     const double* xw = &V.fam[c.fam[2]].members[ix[2] * 3];
     eval_pinhole(intr, pose, xw, ms, act[0], act[1], act[2], r, e.J[0], e.J[1], e.J[2]);
+  } else if (c.kind == SGPU_COST_EUCM_REPROJECTION) {
+    e.R = 2;
+    const double* intr = &V.fam[c.fam[0]].members[ix[0] * 6];
+    const double* pose = &V.fam[c.fam[1]].members[ix[1] * 7];
+    const double* xw = &V.fam[c.fam[2]].members[ix[2] * 3];
+    eval_eucm(intr, pose, xw, ms, act[0], act[1], act[2], r, e.J[0], e.J[1], e.J[2]);
   } else {
     e.R = 6;
     const double* m = &V.fam[c.fam[0]].members[ix[0] * 7];
@@ -449,7 +520,7 @@ void linearize(const Problem& P, const Vars& V, double nu, Linearization& L) {
   std::vector<RangeSums> sums(kChunk);
   for (const auto& c : P.costs) {
     const int64_t nt = c.num_terms();
-    const int R = c.kind == SGPU_COST_PINHOLE_REPROJECTION ? 2 : 6;
+    const int R = is_reprojection(c.kind) ? 2 : 6;
     int dof[3] = {0, 0, 0};
     for (int a = 0; a < c.arity; ++a) dof[a] = V.fam[c.fam[a]].dof;
     const int64_t nr = int64_t(c.range_start.size());
@@ -1080,7 +1151,7 @@ sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_co
   if (!fams || nf <= 0) return fail(SGPU_ERR_INVALID_ARGUMENT, "no variable families");
   for (int64_t i = 0; i < nf; ++i) {
     const auto& s = fams[i];
-    if (!s.name || s.type < 0 || s.type > 2 || s.kind < 0 || s.kind > 2 || s.num_members < 0 ||
+    if (!s.name || s.type < 0 || s.type > 3 || s.kind < 0 || s.kind > 2 || s.num_members < 0 ||
         (s.num_members > 0 && !s.members))
       return fail(SGPU_ERR_INVALID_ARGUMENT, "bad variable family");
     Family f;
@@ -1111,7 +1182,7 @@ sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_co
                 "variable");
   for (int64_t i = 0; i < nc; ++i) {
     const auto& s = cfs[i];
-    const int arity = s.kind == SGPU_COST_PINHOLE_REPROJECTION ? 3 : 2;
+    const int arity = is_reprojection(s.kind) ? 3 : 2;
     int marg = 0;
     for (int a = 0; a < arity; ++a) {
       if (!s.family_names[a]) return fail(SGPU_ERR_INVALID_ARGUMENT, "missing family name");
@@ -1124,15 +1195,16 @@ sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_co
   }
   for (int64_t i = 0; i < nc; ++i) {
     const auto& s = cfs[i];
-    if (s.kind != SGPU_COST_PINHOLE_REPROJECTION && s.kind != SGPU_COST_POSE_GRAPH3)
+    if (!is_reprojection(s.kind) && s.kind != SGPU_COST_POSE_GRAPH3)
       return fail(SGPU_ERR_INVALID_ARGUMENT, "unknown cost kind");
     CostFn c;
     c.kind = s.kind;
     c.robust = s.robust_kernel;
     c.delta = s.huber_delta;
-    c.arity = s.kind == SGPU_COST_PINHOLE_REPROJECTION ? 3 : 2;
-    const int want[2][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
-                            {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1}};
+    c.arity = is_reprojection(s.kind) ? 3 : 2;
+    const int want[3][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+                            {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1},
+                            {SGPU_VAR_EUCM, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3}};
     for (int a = 0; a < c.arity; ++a) {
       c.fam[a] = P.vars.find(s.family_names[a]);
       if (c.fam[a] < 0) return fail(SGPU_ERR_UNKNOWN_FAMILY, "unknown family");
@@ -1564,6 +1636,61 @@ void oracle_pinhole_dual_jacobians(int64_t n, const double* intr, const double* 
       for (int k = 0; k < 3; ++k) jpt[6 * t + 3 * i + k] = r[i].d[6 + k];
     }
   }
+}
+
+// Per-term evaluation of EUCM reprojection terms (analytic Jacobians, FP64).
+void oracle_eval_eucm_terms(int64_t n, const double* intr, const double* pose, const double* pt,
+                            const double* uv, int32_t robust, double delta, double* res,
+                            double* jintr, double* jpose, double* jpt, double* cost,
+                            double* weight) {
+  for (int64_t t = 0; t < n; ++t) {
+    double r[2], Jc[12], Jp[12], Jx[6];
+    eval_eucm(intr + 6 * t, pose + 7 * t, pt + 3 * t, uv + 2 * t, true, true, true, r, Jc, Jp, Jx);
+    const double nrm = std::sqrt(r[0] * r[0] + r[1] * r[1]);
+    const double w = robust == SGPU_ROBUST_HUBER ? robust_weight(robust, delta, nrm) : 1.0;
+    const double r0 = robust == SGPU_ROBUST_HUBER ? w * r[0] : r[0];
+    const double r1 = robust == SGPU_ROBUST_HUBER ? w * r[1] : r[1];
+    if (res) {
+      res[2 * t] = r[0];
+      res[2 * t + 1] = r[1];
+    }
+    if (jintr) std::memcpy(jintr + 12 * t, Jc, sizeof(Jc));
+    if (jpose) std::memcpy(jpose + 12 * t, Jp, sizeof(Jp));
+    if (jpt) std::memcpy(jpt + 6 * t, Jx, sizeof(Jx));
+    if (cost) cost[t] = r0 * r0 + r1 * r1;
+    if (weight) weight[t] = w;
+  }
+}
+
+// Forward-mode dual-number Jacobians of the generic EUCM residual (ba_problem.rs:58-66)
+// with respect to the intrinsics, the left-multiplicative pose tangent at 0 and the point.
+void oracle_eucm_dual_jacobians(int64_t n, const double* intr, const double* pose,
+                                const double* pt, const double* uv, double* jintr, double* jpose,
+                                double* jpt) {
+  using D = Dual<15>;
+  for (int64_t t = 0; t < n; ++t) {
+    D in[6], tangent[6], xw[3], T[7], Td[7], E[7], r[2];
+    for (int k = 0; k < 6; ++k) in[k] = D::var(intr[6 * t + k], k);
+    for (int k = 0; k < 6; ++k) tangent[k] = D::var(0.0, 6 + k);
+    for (int k = 0; k < 3; ++k) xw[k] = D::var(pt[3 * t + k], 12 + k);
+    for (int k = 0; k < 7; ++k) T[k] = D(pose[7 * t + k]);
+    se3_exp(tangent, E);
+    se3_mul(E, T, Td);
+    eucm_residual(in, Td, xw, uv + 2 * t, r);
+    for (int i = 0; i < 2; ++i) {
+      for (int k = 0; k < 6; ++k) jintr[12 * t + 6 * i + k] = r[i].d[k];
+      for (int k = 0; k < 6; ++k) jpose[12 * t + 6 * i + k] = r[i].d[6 + k];
+      for (int k = 0; k < 3; ++k) jpt[6 * t + 3 * i + k] = r[i].d[12 + k];
+    }
+  }
+}
+
+// EUCM distort / undistort of n points on the z = 1 plane (enhanced_unified.rs:60-124).
+void oracle_eucm_distort(int64_t n, const double* intr, const double* xy, double* uv) {
+  for (int64_t t = 0; t < n; ++t) eucm_distort(intr + 6 * t, xy + 2 * t, uv + 2 * t);
+}
+void oracle_eucm_undistort(int64_t n, const double* intr, const double* uv, double* xy) {
+  for (int64_t t = 0; t < n; ++t) eucm_undistort(intr + 6 * t, uv + 2 * t, xy + 2 * t);
 }
 
 void oracle_eval_pose_graph3_terms(int64_t n, const double* m, const double* nn,

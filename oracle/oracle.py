@@ -100,6 +100,14 @@ def lib(variant: str = "") -> C.CDLL:
                                                 DP, DP, DP, DP, DP]
         L.oracle_pinhole_dual_jacobians.restype = None
         L.oracle_pinhole_dual_jacobians.argtypes = [C.c_int64, DP, DP, DP, DP, DP, DP]
+        L.oracle_eval_eucm_terms.restype = None
+        L.oracle_eval_eucm_terms.argtypes = [C.c_int64, DP, DP, DP, DP, C.c_int32, C.c_double,
+                                             DP, DP, DP, DP, DP, DP]
+        L.oracle_eucm_dual_jacobians.restype = None
+        L.oracle_eucm_dual_jacobians.argtypes = [C.c_int64, DP, DP, DP, DP, DP, DP, DP]
+        for name in ("oracle_eucm_distort", "oracle_eucm_undistort"):
+            getattr(L, name).restype = None
+            getattr(L, name).argtypes = [C.c_int64, DP, DP, DP]
         L.oracle_eval_pose_graph3_terms.restype = None
         L.oracle_eval_pose_graph3_terms.argtypes = [C.c_int64, DP, DP, DP, DP, DP, DP]
         L.oracle_pose_graph3_dual_jacobians.restype = None
@@ -336,6 +344,49 @@ def pinhole_dual_jacobians(intr, poses, points, uv):
     jp, jx = np.zeros((n, 2, 6)), np.zeros((n, 2, 3))
     lib().oracle_pinhole_dual_jacobians(n, _p(intr), _p(poses), _p(points), _p(uv), _p(jp), _p(jx))
     return jp, jx
+
+
+def eval_eucm_terms(intr, poses, points, uv, huber_delta=None) -> Dict[str, np.ndarray]:
+    """EUCM reprojection terms: residual, J_intr (2x6), J_pose (2x6), J_point (2x3), cost, weight."""
+    intr = np.ascontiguousarray(intr, dtype=np.float64).reshape(-1, 6)
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 7)
+    points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    n = uv.shape[0]
+    out = dict(residual=np.zeros((n, 2)), jac_intr=np.zeros((n, 2, 6)), jac_pose=np.zeros((n, 2, 6)),
+               jac_point=np.zeros((n, 2, 3)), cost=np.zeros(n), weight=np.zeros(n))
+    lib().oracle_eval_eucm_terms(n, _p(intr), _p(poses), _p(points), _p(uv), 0 if huber_delta is None else 1,
+                                 0.0 if huber_delta is None else huber_delta,
+                                 *[_p(out[k]) for k in ("residual", "jac_intr", "jac_pose", "jac_point", "cost",
+                                                        "weight")])
+    return out
+
+
+def eucm_dual_jacobians(intr, poses, points, uv):
+    intr = np.ascontiguousarray(intr, dtype=np.float64).reshape(-1, 6)
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 7)
+    points = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    n = uv.shape[0]
+    jc, jp, jx = np.zeros((n, 2, 6)), np.zeros((n, 2, 6)), np.zeros((n, 2, 3))
+    lib().oracle_eucm_dual_jacobians(n, _p(intr), _p(poses), _p(points), _p(uv), _p(jc), _p(jp), _p(jx))
+    return jc, jp, jx
+
+
+def eucm_distort(intr, xy):
+    intr = np.ascontiguousarray(intr, dtype=np.float64).reshape(-1, 6)
+    xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros_like(xy)
+    lib().oracle_eucm_distort(xy.shape[0], _p(intr), _p(xy), _p(out))
+    return out
+
+
+def eucm_undistort(intr, uv):
+    intr = np.ascontiguousarray(intr, dtype=np.float64).reshape(-1, 6)
+    uv = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros_like(uv)
+    lib().oracle_eucm_undistort(uv.shape[0], _p(intr), _p(uv), _p(out))
+    return out
 
 
 def eval_pose_graph3_terms(m, n_, mn):

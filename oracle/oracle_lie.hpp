@@ -368,6 +368,45 @@ void pinhole_residual(const S intr[4], const S pose[7], const S xw[3], const dou
   r[1] = S(uv[1]) - (py * intr[1] + intr[3]);
 }
 
+// EnhancedUnifiedDistortionImpl::distort, crates/sophus_sensor/src/distortions/
+// enhanced_unified.rs:60-95: (x, y) on the z = 1 plane, rho = sqrt(beta r^2 + 1),
+// norm = alpha rho + 1 - alpha, pixel = (fx x / norm + cx, fy y / norm + cy).
+template <typename S>
+void eucm_distort(const S p[6], const S xy[2], S uv[2]) {
+  const S r2 = xy[0] * xy[0] + xy[1] * xy[1];
+  const S rho = sqrt(p[5] * r2 + S(1.0));
+  const S norm = p[4] * rho + (S(1.0) - p[4]);
+  uv[0] = p[0] * (xy[0] / norm) + p[2];
+  uv[1] = p[1] * (xy[1] / norm) + p[3];
+}
+
+// EnhancedUnifiedDistortionImpl::undistort, enhanced_unified.rs:97-124.
+template <typename S>
+void eucm_undistort(const S p[6], const S uv[2], S xy[2]) {
+  const S mx = (uv[0] - p[2]) / p[0], my = (uv[1] - p[3]) / p[1];
+  const S r2 = mx * mx + my * my;
+  const S gamma = S(1.0) - p[4];
+  const S num = S(1.0) - p[4] * p[4] * p[5] * r2;
+  const S den = p[4] * sqrt(S(1.0) - (p[4] - gamma) * p[5] * r2) + gamma;
+  const S k = num / den;
+  xy[0] = mx / k;
+  xy[1] = my / k;
+}
+
+// EucmCameraReprojectionCostTerm::residual, example_problems/ba_problem.rs:58-66:
+// uv - distort(params, proj(world_from_camera^-1 * x_w)) (camera.rs:94-99).
+template <typename S>
+void eucm_residual(const S intr[6], const S pose[7], const S xw[3], const double uv[2], S r[2]) {
+  S cw[7], xc[3], xy[2], px[2];
+  se3_inverse(pose, cw);
+  se3_transform(cw, xw, xc);
+  xy[0] = xc[0] / xc[2];
+  xy[1] = xc[1] / xc[2];
+  eucm_distort(intr, xy, px);
+  r[0] = S(uv[0]) - px[0];
+  r[1] = S(uv[1]) - px[1];
+}
+
 // PoseGraph3CostTerm::residual, pose_graph3.rs:33-39
 template <typename S>
 void pose_graph3_residual(const S m[7], const S n[7], const S mn[7], S r[6]) {

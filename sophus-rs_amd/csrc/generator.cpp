@@ -326,3 +326,153 @@ extern "C" sgpu_status sgpu_generate_ba(const sgpu_ba_spec* spec, sgpu_ba_sizes*
   if (obs_idx) std::memcpy(obs_idx, p.idx.data(), p.idx.size() * sizeof(int64_t));
   return SGPU_OK;
 }
+
+// ---- EUCM ring around an open box ------------------------------------------------------------
+// The reference's own BaProblem (crates/sophus_opt/src/example_problems/ba_problem.rs:154-345)
+// with this file's RNG in place of ChaCha12: num_cameras on a ring of radius 8 in the x-z
+// plane at height 0, each looking at (0, 0, 10); points on the five closed faces of the box
+// x in [-3, 3], y in [-2, 2], z in [8, 12] (point k on face k % 5); one shared EUCM camera
+// (300, 300, 320, 240, 0.5, 1.0) for a 640 x 480 image; a camera sees a point within 8 m
+// (any distance below 20 cameras) in front of it (z > 0.1) and inside the image; pixel noise
+// 0.15 U(-0.5, 0.5); points seen fewer than twice are dropped.  Initial values: intrinsics
+// (297, 297, 321, 241, 0.495, 1.0), poses 2.. perturbed by exp(d) T with d uniform in
+// +-(0.04, 0.04, 0.02, 0.03, 0.03, 0.01) (poses 0 and 1 are the gauge), points by
+// U(-0.2, 0.2) per coordinate.
+namespace sgpu {
+namespace {
+
+struct EucmProblem {
+  double intr_gt[6] = {300.0, 300.0, 320.0, 240.0, 0.5, 1.0};
+  double intr_init[6] = {297.0, 297.0, 321.0, 241.0, 0.495, 1.0};
+  std::vector<double> poses_gt, poses_init, points_gt, points_init, uv;
+  std::vector<int64_t> idx;
+};
+
+void eucm_pixel(const double p[6], const double xc[3], double uv[2]) {
+  const double x = xc[0] / xc[2], y = xc[1] / xc[2];
+  const double rho = std::sqrt(p[5] * (x * x + y * y) + 1.0);
+  const double norm = p[4] * rho + (1.0 - p[4]);
+  uv[0] = p[0] * (x / norm) + p[2];
+  uv[1] = p[1] * (y / norm) + p[3];
+}
+
+void gen_eucm_ring_box(int32_t ncam, int64_t npt_all, uint64_t seed, EucmProblem& p) {
+  constexpr double kRadius = 8.0, kCenterZ = 10.0;
+  Rng rng(seed);
+  p.poses_gt.resize(7 * size_t(ncam));
+  for (int32_t j = 0; j < ncam; ++j) {
+    const double th = 2.0 * M_PI * double(j) / double(ncam);
+    double* T = &p.poses_gt[7 * size_t(j)];
+    T[0] = std::cos(-0.5 * th);  // rotation about y by -theta
+    T[1] = 0.0;
+    T[2] = std::sin(-0.5 * th);
+    T[3] = 0.0;
+    T[4] = kRadius * std::sin(th);
+    T[5] = 0.0;
+    T[6] = kCenterZ - kRadius * std::cos(th);
+  }
+  const double bw = 3.0, bh = 2.0, zn = 8.0, zf = 12.0;
+  std::vector<double> pts(3 * size_t(npt_all));
+  for (int64_t k = 0; k < npt_all; ++k) {
+    double* x = &pts[3 * size_t(k)];
+    switch (k % 5) {
+      case 0:
+        x[0] = (rng.uniform() - 0.5) * 2.0 * bw;
+        x[1] = (rng.uniform() - 0.5) * 2.0 * bh;
+        x[2] = zf;
+        break;
+      case 1:
+      case 2:
+        x[0] = k % 5 == 1 ? -bw : bw;
+        x[1] = (rng.uniform() - 0.5) * 2.0 * bh;
+        x[2] = zn + rng.uniform() * (zf - zn);
+        break;
+      default:
+        x[0] = (rng.uniform() - 0.5) * 2.0 * bw;
+        x[1] = k % 5 == 3 ? bh : -bh;
+        x[2] = zn + rng.uniform() * (zf - zn);
+        break;
+    }
+  }
+  const double max_dist = ncam >= 20 ? kRadius : INFINITY;
+  std::vector<int64_t> raw_idx;
+  std::vector<double> raw_uv;
+  std::vector<int32_t> count(npt_all, 0);
+  for (int32_t j = 0; j < ncam; ++j) {
+    const double* T = &p.poses_gt[7 * size_t(j)];
+    double cw[7], R[9];
+    se3_inverse(T, cw);
+    so3_matrix(cw, R);
+    for (int64_t k = 0; k < npt_all; ++k) {
+      const double* x = &pts[3 * size_t(k)];
+      const double dx = T[4] - x[0], dy = T[5] - x[1], dz = T[6] - x[2];
+      if (std::sqrt(dx * dx + dy * dy + dz * dz) > max_dist) continue;
+      double xc[3];
+      mat3_vec(R, x, xc);
+      for (int e = 0; e < 3; ++e) xc[e] += cw[4 + e];
+      if (xc[2] <= 0.1) continue;
+      double uv[2];
+      eucm_pixel(p.intr_gt, xc, uv);
+      if (uv[0] < 0.0 || uv[0] >= kWidth || uv[1] < 0.0 || uv[1] >= kHeight) continue;
+      const double n0 = 0.15 * (rng.uniform() - 0.5), n1 = 0.15 * (rng.uniform() - 0.5);
+      raw_idx.insert(raw_idx.end(), {0, j, k});
+      raw_uv.insert(raw_uv.end(), {uv[0] + n0, uv[1] + n1});
+      ++count[k];
+    }
+  }
+  std::vector<int64_t> remap(npt_all, -1);
+  int64_t npt = 0;
+  for (int64_t k = 0; k < npt_all; ++k)
+    if (count[k] >= 2) {
+      remap[k] = npt++;
+      p.points_gt.insert(p.points_gt.end(), &pts[3 * size_t(k)], &pts[3 * size_t(k)] + 3);
+    }
+  for (size_t o = 0; o < raw_uv.size() / 2; ++o) {
+    const int64_t nk = remap[raw_idx[3 * o + 2]];
+    if (nk < 0) continue;
+    p.idx.insert(p.idx.end(), {0, raw_idx[3 * o + 1], nk});
+    p.uv.insert(p.uv.end(), {raw_uv[2 * o], raw_uv[2 * o + 1]});
+  }
+  p.poses_init = p.poses_gt;
+  const double amp[6] = {0.08, 0.08, 0.04, 0.06, 0.06, 0.02};
+  for (int32_t j = 2; j < ncam; ++j) {
+    double d[6], e[7];
+    for (int k = 0; k < 6; ++k) d[k] = (rng.uniform() - 0.5) * amp[k];
+    se3_exp(d, e);
+    se3_mul(e, &p.poses_gt[7 * size_t(j)], &p.poses_init[7 * size_t(j)]);
+  }
+  p.points_init = p.points_gt;
+  for (double& v : p.points_init) v += (rng.uniform() - 0.5) * 0.4;
+}
+
+}  // namespace
+}  // namespace sgpu
+
+extern "C" sgpu_status sgpu_generate_ba_eucm(int32_t num_cameras, int64_t num_points, uint64_t seed,
+                                             sgpu_ba_sizes* sizes, double* intrinsics_gt,
+                                             double* intrinsics_init, double* poses_gt,
+                                             double* poses_init, double* points_gt,
+                                             double* points_init, int64_t* obs_idx,
+                                             double* obs_uv) {
+  using namespace sgpu;
+  return guarded([&] {
+    if (!sizes || num_cameras < 2 || num_points < 1 || num_points > (int64_t(1) << 31))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad EUCM problem size");
+    EucmProblem p;
+    gen_eucm_ring_box(num_cameras, num_points, seed, p);
+    sizes->num_cameras = num_cameras;
+    sizes->num_points = int64_t(p.points_gt.size() / 3);
+    sizes->num_obs = int64_t(p.uv.size() / 2);
+    if (!intrinsics_gt) return;
+    if (!intrinsics_init || !poses_gt || !poses_init || !points_gt || !points_init || !obs_idx || !obs_uv)
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "output arrays required");
+    std::memcpy(intrinsics_gt, p.intr_gt, sizeof(p.intr_gt));
+    std::memcpy(intrinsics_init, p.intr_init, sizeof(p.intr_init));
+    std::memcpy(poses_gt, p.poses_gt.data(), p.poses_gt.size() * 8);
+    std::memcpy(poses_init, p.poses_init.data(), p.poses_init.size() * 8);
+    std::memcpy(points_gt, p.points_gt.data(), p.points_gt.size() * 8);
+    std::memcpy(points_init, p.points_init.data(), p.points_init.size() * 8);
+    std::memcpy(obs_idx, p.idx.data(), p.idx.size() * 8);
+    std::memcpy(obs_uv, p.uv.data(), p.uv.size() * 8);
+  });
+}

@@ -167,8 +167,120 @@ __device__ __forceinline__ void obs_residual(const double* __restrict__ in,
   for (int j = 0; j < 3; ++j) v[j] = A[j] * rw0 + A[3 + j] * rw1;
 }
 
+// ---- camera models ----------------------------------------------------------------------
+// The reprojection kernels are templated on the camera model: the pinhole (above) or the
+// enhanced unified camera model (EUCM).  Only A_o = d_pi R_cw and the predicted pixel depend
+// on it; everything downstream is the same 3-space algebra.
+constexpr int kPinhole = 0, kEucm = 1;
+template <int kModel>
+__host__ __device__ constexpr int intr_stride() {
+  return kModel == kEucm ? 6 : 4;
+}
+
+// EUCM on the perspective projection (enhanced_unified.rs:60-215, perspective.rs:30-72,
+// ba_problem.rs:76-101): the pixel of x_c, D = dx_distort_x(proj) dx_proj_x(x_c) (2x3
+// row-major) and, with E non-null, dx_distort_params(proj) (2x6 row-major; J_intr = -E).
+// Contraction is off for the reason given at obs_jacobian.
+__device__ __forceinline__ void eucm_project(const double* __restrict__ in, const double xc[3],
+                                             double px[2], double D[6], double* E) {
+#pragma clang fp contract(off)
+  const double x = xc[0] / xc[2], y = xc[1] / xc[2];
+  const double fx = in[0], fy = in[1], alpha = in[4], beta = in[5];
+  const double r2 = x * x + y * y;
+  const double rho = sqrt(beta * r2 + 1.0);
+  const double norm = alpha * rho + (1.0 - alpha);
+  const double mx = x / norm, my = y / norm;
+  px[0] = fx * mx + in[2];
+  px[1] = fy * my + in[3];
+  const double nn = norm * norm;
+  const double dnx = alpha * ((2.0 * beta * x) / (2.0 * rho));
+  const double dny = alpha * ((2.0 * beta * y) / (2.0 * rho));
+  const double d00 = fx * ((norm - x * dnx) / nn), d01 = fx * (-x * dny / nn);
+  const double d10 = fy * (-y * dnx / nn), d11 = fy * ((norm - y * dny) / nn);
+  const double iz = 1.0 / xc[2], zz = xc[2] * xc[2];
+  const double p02 = -xc[0] / zz, p12 = -xc[1] / zz;
+  D[0] = d00 * iz;
+  D[1] = d01 * iz;
+  D[2] = d00 * p02 + d01 * p12;
+  D[3] = d10 * iz;
+  D[4] = d11 * iz;
+  D[5] = d10 * p02 + d11 * p12;
+  if (E) {
+    const double dna = rho - 1.0, dnb = alpha * (r2 / (2.0 * rho));
+    E[0] = mx;
+    E[1] = 0.0;
+    E[2] = 1.0;
+    E[3] = 0.0;
+    E[4] = fx * (-x * dna / nn);
+    E[5] = fx * (-x * dnb / nn);
+    E[6] = 0.0;
+    E[7] = my;
+    E[8] = 0.0;
+    E[9] = 1.0;
+    E[10] = fy * (-y * dna / nn);
+    E[11] = fy * (-y * dnb / nn);
+  }
+}
+
+// A = D R_cw (2x3) from the projection derivative D and the cached rotation.
+__device__ __forceinline__ void lift_rotation(const double* __restrict__ cw, const double D[6],
+                                              double A[6]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      A[3 * i + j] = D[3 * i] * cw[j] + D[3 * i + 1] * cw[3 + j] + D[3 * i + 2] * cw[6 + j];
+}
+
+template <int kModel, typename T>
+__device__ __forceinline__ void model_jacobian(const double* __restrict__ cw,
+                                               const double* __restrict__ in, const double xc[3],
+                                               T A[6]) {
+  if constexpr (kModel == kPinhole) {
+    obs_jacobian<T>(cw, in, xc, A);
+  } else {
+    double px[2], D[6], Ad[6];
+    eucm_project(in, xc, px, D, nullptr);
+    lift_rotation(cw, D, Ad);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) A[k] = T(Ad[k]);
+  }
+}
+
+// predicted pixel of x_c
+template <int kModel>
+__device__ __forceinline__ void model_pixel(const double* __restrict__ in, const double xc[3],
+                                            double px[2]) {
+  if constexpr (kModel == kPinhole) {
+    px[0] = xc[0] / xc[2] * in[0] + in[2];
+    px[1] = xc[1] / xc[2] * in[1] + in[3];
+  } else {
+    double D[6];
+    eucm_project(in, xc, px, D, nullptr);
+  }
+}
+
+// residual r = uv - pi(x_c) in FP64 and v = A^T (w r)
+template <int kModel, typename T>
+__device__ __forceinline__ void model_residual(const double* __restrict__ in,
+                                               const double* __restrict__ uv, const double xc[3],
+                                               int robust, double delta, const T A[6], T v[3]) {
+  if constexpr (kModel == kPinhole) {
+    obs_residual<T>(in, uv, xc, robust, delta, A, v);
+  } else {
+    double px[2];
+    model_pixel<kModel>(in, xc, px);
+    const double r0 = uv[0] - px[0], r1 = uv[1] - px[1];
+    const double w = robust_weight(robust, delta, sqrt(r0 * r0 + r1 * r1));
+    const T rw0 = T(robust == 1 ? w * r0 : r0), rw1 = T(robust == 1 ? w * r1 : r1);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[j] = A[j] * rw0 + A[3 + j] * rw1;
+  }
+}
+
 // ---- kernel: per-point H_ll, its inverse and the point gradient --------------------------
-template <typename T>
+template <typename T, int kModel>
 __global__ void __launch_bounds__(kThreads)
     k_point_prep(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
                  const double* __restrict__ cw, double nu, double* __restrict__ s_out,
@@ -185,12 +297,12 @@ __global__ void __launch_bounds__(kThreads)
   double s[3] = {0, 0, 0};
   for (int o = p.pt_ptr[b]; live && active && o < p.pt_ptr[b + 1]; ++o) {
     const double* c = cw + 12 * p.obs_pose[o];
-    const double* in = intr + 4 * p.obs_intr[o];
+    const double* in = intr + intr_stride<kModel>() * p.obs_intr[o];
     double xc[3];
     to_camera(c, x, xc);
     T A[6], v[3], M[9];
-    obs_jacobian<T>(c, in, xc, A);
-    obs_residual<T>(in, p.obs_uv + 2 * o, xc, p.robust, p.delta, A, v);
+    model_jacobian<kModel, T>(c, in, xc, A);
+    model_residual<kModel, T>(in, p.obs_uv + 2 * o, xc, p.robust, p.delta, A, v);
     gram(A, M);
 #pragma unroll
     for (int k = 0; k < 9; ++k) H[k] += double(M[k]);
@@ -362,7 +474,7 @@ __device__ __forceinline__ int wave_sum_scatter(double (&v)[N], int lane) {
   return off;
 }
 
-template <typename T>
+template <typename T, int kModel>
 __global__ void __launch_bounds__(kThreads)
     k_cam_obs(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
               const double* __restrict__ cw, const double* __restrict__ prec, int centered,
@@ -381,7 +493,7 @@ __global__ void __launch_bounds__(kThreads)
     // camera-major observation fields; one camera per chunk
     const int b = p.cm_point[q];
     const double* c = cw + 12 * p.cam_member[cam];
-    const double* ini = intr + 4 * p.cm_intr[q];
+    const double* ini = intr + intr_stride<kModel>() * p.cm_intr[q];
     const double* uvq = p.cm_uv + 2 * int64_t(q);
     T* vs = sv + kObsRec * tid;
     double xd[3], xl[3], xc[3], Ad[6], Md[9], v[3], t[3];
@@ -400,9 +512,9 @@ __global__ void __launch_bounds__(kThreads)
         for (int e = 0; e < 3; ++e) xl[e] -= cc[e];
       }
       to_camera(c, xd, xc);
-      obs_jacobian<double>(c, ini, xc, Ad);
+      model_jacobian<kModel, double>(c, ini, xc, Ad);
       gram(Ad, Md);
-      obs_residual<double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
+      model_residual<kModel, double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
       // U = L^T M with H_ll^-1 = L L^T (zero columns past a non-positive pivot)
       const double l00 = sqrt(fmax(rc[3], 0.0)), i00 = l00 > 0.0 ? 1.0 / l00 : 0.0;
       const double l10 = rc[4] * i00, l20 = rc[5] * i00;
@@ -450,14 +562,14 @@ __global__ void __launch_bounds__(kThreads)
         for (int e = 0; e < 3; ++e) xl[e] -= cc[e];
       }
       to_camera(c, xd, xc);
-      obs_jacobian<double>(c, ini, xc, Ad);
-      obs_residual<double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
+      model_jacobian<kModel, double>(c, ini, xc, Ad);
+      model_residual<kModel, double>(ini, uvq, xc, p.robust, p.delta, Ad, v);
       // M_o as the point pass and the back-substitution form it (FP32 Jacobian and Gram),
       // held in FP64: H_ll sums exactly these terms, so R = H_ll - M_o below is the other
       // terms plus the damping, and U, the reduced rhs and the own term all describe one
       // matrix
       T Ai[6], Mi[9];
-      obs_jacobian<T>(c, ini, xc, Ai);
+      model_jacobian<kModel, T>(c, ini, xc, Ai);
       gram(Ai, Mi);
 #pragma unroll
       for (int e = 0; e < 9; ++e) Md[e] = double(Mi[e]);
@@ -1937,6 +2049,7 @@ __global__ void __launch_bounds__(kThreads)
 
 // Coupling blocks of one point with its cameras (covariance queries): one thread per
 // observation.  H_{c,b} = (A P)^T (-A) = -P^T M with P = [hat(-x) | I], M = A^T A.
+template <int kModel>
 __global__ void k_point_coupling(DevProblem p, const double* __restrict__ points,
                                  const double* __restrict__ intr, const double* __restrict__ cw,
                                  const double* __restrict__ prec, int b, double* __restrict__ out,
@@ -1954,7 +2067,7 @@ __global__ void k_point_coupling(DevProblem p, const double* __restrict__ points
     const double* c = cw + 12 * pm;
     double xc[3], A[6], M[9];
     to_camera(c, x, xc);
-    obs_jacobian<double>(c, intr + 4 * p.obs_intr[o], xc, A);
+    model_jacobian<kModel, double>(c, intr + intr_stride<kModel>() * p.obs_intr[o], xc, A);
     gram(A, M);
     // P^T M: rows 0-2 hat(-x)^T M = hat(x) M, rows 3-5 M
     double* H = out + 18 * q;
@@ -2016,12 +2129,14 @@ __device__ void block_reduce2(double a, double b, double* partial) {
 }
 
 // ---- back-substitution of the points and their update ------------------------------------
-template <typename T>
+// kFreeIntr: free shared intrinsics, dx_b -= V_b^T dx_intr (H_ll^-1 H_lc dx_intr, H_lc = W_b^T)
+template <typename T, int kModel, bool kFreeIntr>
 __global__ void __launch_bounds__(kThreads)
     k_backsub(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
               const double* __restrict__ cw, const double* __restrict__ prec,
               const double* __restrict__ pt_s, const double* __restrict__ dx_cam,
-              double* __restrict__ points_new, double* __restrict__ partial, DevScalars* sc) {
+              const double* __restrict__ pt_vw, double* __restrict__ points_new,
+              double* __restrict__ partial, DevScalars* sc) {
   double dd = 0, dg = 0, mx = 0;
   for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < p.npt; b += gridDim.x * blockDim.x) {
     const double xd[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
@@ -2038,7 +2153,7 @@ __global__ void __launch_bounds__(kThreads)
       double xc[3];
       to_camera(cwo, xd, xc);
       T A[6], M[9];
-      obs_jacobian<T>(cwo, intr + 4 * p.obs_intr[o], xc, A);
+      model_jacobian<kModel, T>(cwo, intr + intr_stride<kModel>() * p.obs_intr[o], xc, A);
       gram(A, M);
       const double* dx = dx_cam + 6 * c;
       // P dx = omega x point + nu, in FP64 (omega x point and nu nearly cancel far from
@@ -2053,8 +2168,13 @@ __global__ void __launch_bounds__(kThreads)
     const double* hu = prec + kPtRec * int64_t(b) + 3;
     const double Hi[9] = {hu[0], hu[1], hu[2], hu[1], hu[3], hu[4], hu[2], hu[4], hu[5]};
     for (int k = 0; k < 3; ++k) {
-      const double dk = Hi[3 * k] * double(acc[0]) + Hi[3 * k + 1] * double(acc[1]) +
-                        Hi[3 * k + 2] * double(acc[2]);
+      double dk = Hi[3 * k] * double(acc[0]) + Hi[3 * k + 1] * double(acc[1]) +
+                  Hi[3 * k + 2] * double(acc[2]);
+      if constexpr (kFreeIntr) {
+        const double* V = pt_vw + 36 * int64_t(b);
+        const double* dxi = dx_cam + 6 * p.intr_block;
+        for (int a = 0; a < 6; ++a) dk -= V[3 * a + k] * dxi[a];
+      }
       points_new[3 * b + k] = xd[k] + dk;
       dd += dk * dk;
       dg += dk * pt_s[3 * b + k];
@@ -2065,14 +2185,225 @@ __global__ void __launch_bounds__(kThreads)
   block_reduce2(dd, dg, partial);
 }
 
-// ---- pose update T <- exp(delta) T ----------------------------------------------------------
+// ---- free shared intrinsics (EUCM) -----------------------------------------------------------
+// One active intrinsics member c shared by the observations adds the free block c to the RCS.
+// With E_o = dx_distort_params (J_intr = -E_o), J_point = -A_o and J_pose = A_o P_b:
+//   H_cc = sum E^T E,  H_cl = W_b = sum_o E_o^T A_o (6x3),  H_cp = -E^T A P,
+//   neg_grad_c = sum E^T r~.
+// With V_b = W_b H_ll^-1 the Schur complement's intrinsics rows are
+//   S_cc = H_cc + nu n_c I - sum_b V_b W_b^T,
+//   S_cp = sum_{o of camera p} (V_b M_o - E_o^T A_o) P_b,
+//   rhs_c = neg_grad_c - sum_b V_b s_b,
+// and the back-substitution subtracts V_b^T dx_c from each point's step.  Terms whose
+// intrinsics member is not the free one are conditioned for it.
+constexpr int kIntrPart = 33;  // per workgroup: S_cc upper (21), rhs_c (6), neg_grad_c (6)
+
+// deterministic workgroup sum of N per-thread values: wavefront butterfly, then the
+// wavefronts in order; thread e < N of the workgroup receives total e in out
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N], double (*red)[N], double* out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    double x = v[e];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+    if (lane == 0) red[wid][e] = x;
+  }
+  __syncthreads();
+  if (int(threadIdx.x) < N) {
+    double t = 0.0;
+    for (int w = 0; w < int(blockDim.x >> 6); ++w) t += red[w][threadIdx.x];
+    *out = t;
+  }
+}
+
+// Per point: W_b and V_b (pt_vw: V row-major 6x3, then W), and the point's share of S_cc,
+// rhs_c and neg_grad_c summed over the workgroup into intr_part[kIntrPart * block + e].
 __global__ void __launch_bounds__(kThreads)
-    k_update_poses(DevProblem p, const double* __restrict__ poses, const double* __restrict__ dx_cam,
-                   const double* __restrict__ grad_cam, double* __restrict__ poses_new,
-                   double* __restrict__ partial, DevScalars* sc) {
+    k_intr_points(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
+                  const double* __restrict__ cw, const double* __restrict__ prec,
+                  const double* __restrict__ pt_s, double* __restrict__ pt_vw,
+                  double* __restrict__ intr_part) {
+  __shared__ double red[kThreads / 64][kIntrPart];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[kIntrPart];
+#pragma unroll
+  for (int e = 0; e < kIntrPart; ++e) acc[e] = 0.0;
+  if (b < p.npt) {
+    const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
+    const double* in = intr + 6 * int64_t(p.intr_member);
+    double W[18];
+#pragma unroll
+    for (int e = 0; e < 18; ++e) W[e] = 0.0;
+    for (int o = p.pt_ptr[b]; o < p.pt_ptr[b + 1]; ++o) {
+      if (p.obs_intr[o] != p.intr_member) continue;
+      const double* c = cw + 12 * p.obs_pose[o];
+      double xc[3], px[2], D[6], E[12], A[6];
+      to_camera(c, x, xc);
+      eucm_project(in, xc, px, D, E);
+      lift_rotation(c, D, A);
+      const double r0 = p.obs_uv[2 * o] - px[0], r1 = p.obs_uv[2 * o + 1] - px[1];
+      const double w = robust_weight(p.robust, p.delta, sqrt(r0 * r0 + r1 * r1));
+      const double rw0 = p.robust == 1 ? w * r0 : r0, rw1 = p.robust == 1 ? w * r1 : r1;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) W[3 * a + j] += E[a] * A[j] + E[6 + a] * A[3 + j];
+        const double g = E[a] * rw0 + E[6 + a] * rw1;
+        acc[21 + a] += g;
+        acc[27 + a] += g;
+      }
+      int u = 0;
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int bb = a; bb < 6; ++bb) acc[u++] += E[a] * E[bb] + E[6 + a] * E[6 + bb];
+    }
+    double* out = pt_vw + 36 * int64_t(b);
+    if (p.pt_block[b] >= 0) {
+      const double* h = prec + kPtRec * int64_t(b) + 3;
+      const double Hi[9] = {h[0], h[1], h[2], h[1], h[3], h[4], h[2], h[4], h[5]};
+      double V[18];
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          V[3 * a + j] = W[3 * a] * Hi[j] + W[3 * a + 1] * Hi[3 + j] + W[3 * a + 2] * Hi[6 + j];
+      int u = 0;
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int bb = a; bb < 6; ++bb)
+          acc[u++] -= V[3 * a] * W[3 * bb] + V[3 * a + 1] * W[3 * bb + 1] + V[3 * a + 2] * W[3 * bb + 2];
+      const double sb[3] = {pt_s[3 * b], pt_s[3 * b + 1], pt_s[3 * b + 2]};
+#pragma unroll
+      for (int a = 0; a < 6; ++a) acc[21 + a] -= V[3 * a] * sb[0] + V[3 * a + 1] * sb[1] + V[3 * a + 2] * sb[2];
+#pragma unroll
+      for (int e = 0; e < 18; ++e) {
+        out[e] = V[e];
+        out[18 + e] = W[e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 36; ++e) out[e] = 0.0;
+    }
+  }
+  double total = 0.0;
+  block_sum_n<kIntrPart>(acc, red, &total);
+  if (int(threadIdx.x) < kIntrPart) intr_part[kIntrPart * int64_t(blockIdx.x) + threadIdx.x] = total;
+}
+
+// One workgroup per free pose camera: its S block (intrinsics, camera), stored into the slot
+// (k_schur_blocks leaves it zero).
+__global__ void __launch_bounds__(kThreads)
+    k_intr_cams(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
+                const double* __restrict__ cw, const double* __restrict__ pt_vw,
+                double* __restrict__ front, double* __restrict__ bsr) {
+  __shared__ double red[kThreads / 64][36];
+  const int cam = blockIdx.x;
+  const int slot = p.intr_slot[cam];
+  if (slot < 0) return;  // the whole workgroup leaves together
+  double acc[36];
+#pragma unroll
+  for (int e = 0; e < 36; ++e) acc[e] = 0.0;
+  const double* c = cw + 12 * p.cam_member[cam];
+  const double* in = intr + 6 * int64_t(p.intr_member);
+  for (int q = p.cam_ptr[cam] + threadIdx.x; q < p.cam_ptr[cam + 1]; q += blockDim.x) {
+    const int o = p.cam_obs[q];
+    if (p.obs_intr[o] != p.intr_member) continue;
+    const int b = p.obs_point[o];
+    const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
+    double xc[3], px[2], D[6], E[12], A[6];
+    to_camera(c, x, xc);
+    eucm_project(in, xc, px, D, E);
+    lift_rotation(c, D, A);
+    double G[18];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G[3 * a + j] = -(E[a] * A[j] + E[6 + a] * A[3 + j]);
+    if (p.pt_block[b] >= 0) {
+      const double* V = pt_vw + 36 * int64_t(b);
+      double M[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[3 * i + j] = A[i] * A[j] + A[3 + i] * A[3 + j];
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          G[3 * a + j] += V[3 * a] * M[j] + V[3 * a + 1] * M[3 + j] + V[3 * a + 2] * M[6 + j];
+    }
+    // G P = [G hat(-x) | G] = [-(G hat(x)) | G]
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      const double g0 = G[3 * a], g1 = G[3 * a + 1], g2 = G[3 * a + 2];
+      acc[6 * a + 0] -= g1 * x[2] - g2 * x[1];
+      acc[6 * a + 1] -= g2 * x[0] - g0 * x[2];
+      acc[6 * a + 2] -= g0 * x[1] - g1 * x[0];
+      acc[6 * a + 3] += g0;
+      acc[6 * a + 4] += g1;
+      acc[6 * a + 5] += g2;
+    }
+  }
+  double total = 0.0;
+  block_sum_n<36>(acc, red, &total);
+  if (threadIdx.x < 36) {
+    const int a = threadIdx.x / 6, bb = threadIdx.x % 6;
+    if (bsr) {
+      bsr[36 * int64_t(slot) + threadIdx.x] = total;
+    } else {
+      const int ldc = p.s_ld[slot], ld = ldc >> 1;
+      front[p.s_off[slot] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = total;
+    }
+  }
+}
+
+// Sums k_intr_points' partials (fixed order) into the intrinsics' diagonal S block (added to
+// the damping k_schur_blocks stored there), rhs_c and neg_grad_c.
+__global__ void __launch_bounds__(kThreads)
+    k_intr_finish(DevProblem p, const double* __restrict__ intr_part, int nblk,
+                  double* __restrict__ front, double* __restrict__ bsr, double* __restrict__ rhs_nat,
+                  double* __restrict__ grad_cam) {
+  __shared__ double tot[kIntrPart];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int e = wid; e < kIntrPart; e += nw) {
+    double v = 0.0;
+    for (int i = lane; i < nblk; i += 64) v += intr_part[kIntrPart * int64_t(i) + e];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) tot[e] = v;
+  }
+  __syncthreads();
+  const int t = threadIdx.x, c = p.intr_block;
+  if (t < 36) {
+    const int a = t / 6, bb = t % 6, lo = min(a, bb), hi = max(a, bb);
+    const double v = tot[lo * 6 - lo * (lo - 1) / 2 + (hi - lo)];
+    const int64_t k = p.intr_diag_slot;
+    if (bsr) {
+      bsr[36 * k + t] += v;
+    } else {
+      const int ldc = p.s_ld[k], ld = ldc >> 1;
+      front[p.s_off[k] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] += v;
+    }
+  } else if (t < 42) {
+    rhs_nat[6 * c + t - 36] = tot[21 + t - 36];
+  } else if (t < 48) {
+    grad_cam[6 * c + t - 42] = tot[27 + t - 42];
+  }
+}
+
+// ---- pose update T <- exp(delta) T ----------------------------------------------------------
+// Pose blocks 0 .. npb - 1 (the free intrinsics' block, if any, follows them).
+__global__ void __launch_bounds__(kThreads)
+    k_update_poses(DevProblem p, int npb, const double* __restrict__ poses,
+                   const double* __restrict__ dx_cam, const double* __restrict__ grad_cam,
+                   double* __restrict__ poses_new, double* __restrict__ partial, DevScalars* sc) {
   double dd = 0, dg = 0;
   double mx = 0;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < p.nb; c += gridDim.x * blockDim.x) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < npb; c += gridDim.x * blockDim.x) {
     const int pm = p.cam_member[c];
     double t[6], T0[7], T1[7];
     for (int k = 0; k < 6; ++k) {
@@ -2089,7 +2420,36 @@ __global__ void __launch_bounds__(kThreads)
   block_reduce2(dd, dg, partial);
 }
 
+// Free intrinsics: additive update (perspective_camera.rs:85-88) into intr_new, which holds a
+// copy of intr; the block's share of delta.delta, delta.neg_grad and max |delta| goes to
+// partial[0..1] (one wavefront).
+__global__ void k_update_intrinsics(DevProblem p, const double* __restrict__ dx_cam,
+                                    const double* __restrict__ grad_cam,
+                                    const double* __restrict__ intr, double* __restrict__ intr_new,
+                                    double* __restrict__ partial, DevScalars* sc) {
+  const int k = threadIdx.x, c = p.intr_block;
+  double dd = 0, dg = 0, mx = 0;
+  if (k < 6) {
+    const double t = dx_cam[6 * c + k];
+    const int64_t i = int64_t(6) * p.intr_member + k;
+    intr_new[i] = intr[i] + t;
+    dd = t * t;
+    dg = t * grad_cam[6 * c + k];
+    mx = fabs(t);
+  }
+  wave_atomic_max_abs(&sc->max_delta_bits, mx);
+  for (int off = 32; off > 0; off >>= 1) {
+    dd += __shfl_down(dd, off, 64);
+    dg += __shfl_down(dg, off, 64);
+  }
+  if (k == 0) {
+    partial[0] = dd;
+    partial[1] = dg;
+  }
+}
+
 // ---- merit: sum over terms of r~^T r~ ------------------------------------------------------
+template <int kModel>
 __global__ void __launch_bounds__(kThreads)
     k_cost(DevProblem p, const int32_t* __restrict__ obs_point, const double* __restrict__ poses,
            const double* __restrict__ points, const double* __restrict__ intr,
@@ -2097,7 +2457,7 @@ __global__ void __launch_bounds__(kThreads)
   double acc = 0;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < p.nobs; o += gridDim.x * blockDim.x) {
     const double* T = poses + 7 * p.obs_pose[o];
-    const double* in = intr + 4 * p.obs_intr[o];
+    const double* in = intr + intr_stride<kModel>() * p.obs_intr[o];
     const int b = obs_point[o];
     double inv[7], R[9], xc[3];
     se3_inverse(T, inv);
@@ -2107,9 +2467,10 @@ __global__ void __launch_bounds__(kThreads)
     xc[0] += inv[4];
     xc[1] += inv[5];
     xc[2] += inv[6];
-    const double px = xc[0] / xc[2], py = xc[1] / xc[2];
-    const double r0 = p.obs_uv[2 * o] - (px * in[0] + in[2]);
-    const double r1 = p.obs_uv[2 * o + 1] - (py * in[1] + in[3]);
+    double pxl[2];
+    model_pixel<kModel>(in, xc, pxl);
+    const double r0 = p.obs_uv[2 * o] - pxl[0];
+    const double r1 = p.obs_uv[2 * o + 1] - pxl[1];
     if (p.robust == 1) {
       const double w = robust_weight(1, p.delta, sqrt(r0 * r0 + r1 * r1));
       const double a = w * r0, b2 = w * r1;
@@ -2393,6 +2754,49 @@ __global__ void k_eval_pinhole(int64_t n, const double* __restrict__ intr,
   if (weight) weight[t] = w;
 }
 
+__global__ void k_eval_eucm(int64_t n, const double* __restrict__ intr,
+                            const double* __restrict__ pose, const double* __restrict__ pt,
+                            const double* __restrict__ uv, int robust, double delta, double* res,
+                            double* jintr, double* jpose, double* jpt, double* cost,
+                            double* weight) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  double inv[7], R[9], cw[12];
+  se3_inverse(pose + 7 * t, inv);
+  so3_matrix(inv, R);
+  for (int k = 0; k < 9; ++k) cw[k] = R[k];
+  cw[9] = inv[4];
+  cw[10] = inv[5];
+  cw[11] = inv[6];
+  const double x[3] = {pt[3 * t], pt[3 * t + 1], pt[3 * t + 2]};
+  double xc[3], px[2], D[6], E[12], A[6];
+  to_camera(cw, x, xc);
+  eucm_project(intr + 6 * t, xc, px, D, E);
+  lift_rotation(cw, D, A);
+  const double r0 = uv[2 * t] - px[0], r1 = uv[2 * t + 1] - px[1];
+  const double w = robust_weight(robust, delta, sqrt(r0 * r0 + r1 * r1));
+  if (res) {
+    res[2 * t] = r0;
+    res[2 * t + 1] = r1;
+  }
+  if (jintr)
+    for (int k = 0; k < 12; ++k) jintr[12 * t + k] = -E[k];
+  if (jpose) {
+    for (int i = 0; i < 2; ++i) {
+      const double* a = A + 3 * i;
+      jpose[12 * t + 6 * i + 0] = a[1] * (-x[2]) - a[2] * (-x[1]);
+      jpose[12 * t + 6 * i + 1] = -a[0] * (-x[2]) + a[2] * (-x[0]);
+      jpose[12 * t + 6 * i + 2] = a[0] * (-x[1]) - a[1] * (-x[0]);
+      for (int j = 0; j < 3; ++j) jpose[12 * t + 6 * i + 3 + j] = a[j];
+    }
+  }
+  if (jpt)
+    for (int k = 0; k < 6; ++k) jpt[6 * t + k] = -A[k];
+  const double a0 = robust == 1 ? w * r0 : r0, a1 = robust == 1 ? w * r1 : r1;
+  if (cost) cost[t] = a0 * a0 + a1 * a1;
+  if (weight) weight[t] = w;
+}
+
 __global__ void k_eval_pose_graph(int64_t n, const double* __restrict__ m,
                                   const double* __restrict__ nn, const double* __restrict__ mn,
                                   double* res, double* jm, double* jn) {
@@ -2424,12 +2828,13 @@ void launch_point_prep(hipStream_t s, int precision, const DevProblem& p, const 
   (void)poses;
   if (p.npt == 0) return;
   const dim3 g(blocks_for(p.npt, kThreads)), b(kThreads);
+  const bool eucm = p.cam_model == kEucm;
   if (precision == 0)
-    hipLaunchKernelGGL(k_point_prep<double>, g, b, 0, s, p, points, intr, cw, nu, w.pt_s, w.prec,
-                       sc);
+    hipLaunchKernelGGL((eucm ? k_point_prep<double, kEucm> : k_point_prep<double, kPinhole>), g, b,
+                       0, s, p, points, intr, cw, nu, w.pt_s, w.prec, sc);
   else
-    hipLaunchKernelGGL(k_point_prep<float>, g, b, 0, s, p, points, intr, cw, nu, w.pt_s, w.prec,
-                       sc);
+    hipLaunchKernelGGL((eucm ? k_point_prep<float, kEucm> : k_point_prep<float, kPinhole>), g, b, 0,
+                       s, p, points, intr, cw, nu, w.pt_s, w.prec, sc);
 }
 
 void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const double* points,
@@ -2439,18 +2844,19 @@ void launch_camera_rows(hipStream_t s, int precision, const DevProblem& p, const
   if (p.nslot == 0) return;
   const int wpb = kThreads / 64;
   const dim3 go(unsigned(p.n_own_chunks)), gs(unsigned((p.nslot + wpb - 1) / wpb));
+  const bool eucm = p.cam_model == kEucm;
   if (precision == 0) {
     double* v = static_cast<double*>(w.vobs);
     if (p.n_own_chunks > 0)
-      hipLaunchKernelGGL(k_cam_obs<double>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
-                         0, v, w.own_part);
+      hipLaunchKernelGGL((eucm ? k_cam_obs<double, kEucm> : k_cam_obs<double, kPinhole>), go,
+                         dim3(kThreads), 0, s, p, points, intr, cw, w.prec, 0, v, w.own_part);
     hipLaunchKernelGGL((k_schur_blocks<double, false>), gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   } else {
     float* v = static_cast<float*>(w.vobs);
     if (p.n_own_chunks > 0)
-      hipLaunchKernelGGL(k_cam_obs<float>, go, dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
-                         1, v, w.own_part);
+      hipLaunchKernelGGL((eucm ? k_cam_obs<float, kEucm> : k_cam_obs<float, kPinhole>), go,
+                         dim3(kThreads), 0, s, p, points, intr, cw, w.prec, 1, v, w.own_part);
     hipLaunchKernelGGL((k_schur_blocks<float, true>), gs, dim3(kThreads), 0, s, p, v, w.own_part, cw, nu,
                        f.front, w.rhs_nat, w.grad_cam, w.bsr);
   }
@@ -2548,8 +2954,40 @@ void launch_rcs_forward(hipStream_t s, const DevFactor& f, const double* b_nat, 
 void launch_point_coupling(hipStream_t s, const DevProblem& p, const double* points,
                            const double* intr, const double* cw, const double* prec, int b,
                            double* out, int32_t* cam, double* hinv) {
-  hipLaunchKernelGGL(k_point_coupling, dim3(1), dim3(64), 0, s, p, points, intr, cw, prec, b, out,
-                     cam, hinv);
+  hipLaunchKernelGGL((p.cam_model == kEucm ? k_point_coupling<kEucm> : k_point_coupling<kPinhole>),
+                     dim3(1), dim3(64), 0, s, p, points, intr, cw, prec, b, out, cam, hinv);
+}
+
+void launch_intrinsics_rows(hipStream_t s, const DevProblem& p, const double* points,
+                            const double* intr, const double* cw, DevLin& w, DevFactor& f) {
+  if (p.intr_block < 0) return;
+  const int gpt = std::max(1, blocks_for(p.npt, kThreads));
+  hipLaunchKernelGGL(k_intr_points, dim3(gpt), dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
+                     w.pt_s, w.pt_vw, w.intr_part);
+  if (p.intr_block > 0)
+    hipLaunchKernelGGL(k_intr_cams, dim3(p.intr_block), dim3(kThreads), 0, s, p, points, intr, cw,
+                       w.pt_vw, f.front, w.bsr);
+  hipLaunchKernelGGL(k_intr_finish, dim3(1), dim3(kThreads), 0, s, p, w.intr_part, gpt, f.front,
+                     w.bsr, w.rhs_nat, w.grad_cam);
+}
+
+int intrinsics_partials(int npt) { return kIntrPart * std::max(1, blocks_for(npt, kThreads)); }
+
+// Diagonal of the assembled S (camera order) from the diagonal slots of the fronts: the
+// diagonal block of row c is its last slot (columns ascending, c the largest).
+__global__ void k_gather_s_diag(DevProblem p, const double* __restrict__ front, double* __restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 6 * p.nb) return;
+  const int c = t / 6, a = t % 6;
+  const int k = p.s_ptr[c + 1] - 1;
+  const int ld = p.s_ld[k] >> 1;
+  out[t] = front[p.s_off[k] + int64_t(a) * ld + a];
+}
+
+void launch_gather_s_diag(hipStream_t s, const DevProblem& p, const double* front, double* out) {
+  if (p.nb == 0) return;
+  hipLaunchKernelGGL(k_gather_s_diag, dim3(blocks_for(6 * int64_t(p.nb), kThreads)), dim3(kThreads), 0, s,
+                     p, front, out);
 }
 
 void launch_scatter_s(hipStream_t s, const DevProblem& p, int64_t nslots, const double* blocks,
@@ -2563,25 +3001,44 @@ void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p, co
                            const double* intr, const double* cw, const DevLin& w,
                            double* points_new, DevScalars* sc) {
   const int g = std::max(1, std::min(blocks_for(p.npt, kThreads), kCostBlocks));
-  if (precision == 0)
-    hipLaunchKernelGGL(k_backsub<double>, dim3(g), dim3(kThreads), 0, s, p, points, intr, cw,
-                       w.prec, w.pt_s, w.dx_cam, points_new, w.partial, sc);
-  else
-    hipLaunchKernelGGL(k_backsub<float>, dim3(g), dim3(kThreads), 0, s, p, points, intr, cw,
-                       w.prec, w.pt_s, w.dx_cam, points_new, w.partial, sc);
+  auto kernel = k_backsub<double, kPinhole, false>;
+  if (precision == 0 && p.intr_block >= 0)
+    kernel = k_backsub<double, kEucm, true>;  // free intrinsics: FP64 EUCM only
+  else if (precision == 0)
+    kernel = p.cam_model == kEucm ? k_backsub<double, kEucm, false> : k_backsub<double, kPinhole, false>;
+  if (precision == 0) {
+    hipLaunchKernelGGL(kernel, dim3(g), dim3(kThreads), 0, s, p, points, intr, cw, w.prec, w.pt_s,
+                       w.dx_cam, w.pt_vw, points_new, w.partial, sc);
+  } else {
+    auto kf = p.cam_model == kEucm ? k_backsub<float, kEucm, false> : k_backsub<float, kPinhole, false>;
+    hipLaunchKernelGGL(kf, dim3(g), dim3(kThreads), 0, s, p, points, intr, cw, w.prec, w.pt_s,
+                       w.dx_cam, w.pt_vw, points_new, w.partial, sc);
+  }
 }
 
-void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses, const DevLin& w,
-                         double* poses_new, DevScalars* sc) {
+// Partials of the step sums: 2 per back-substitution workgroup, 2 per pose-update workgroup,
+// then 2 for the free intrinsics' block.
+static int pose_update_blocks(const DevProblem& p) {
+  const int npb = p.intr_block >= 0 ? p.intr_block : p.nb;
+  return std::max(1, std::min(blocks_for(npb, kThreads), kCostBlocks));
+}
+
+void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses,
+                         const double* intr, const DevLin& w, double* poses_new, double* intr_new,
+                         DevScalars* sc) {
   const int gpt = std::max(1, std::min(blocks_for(p.npt, kThreads), kCostBlocks));
-  const int g = std::max(1, std::min(blocks_for(p.nb, kThreads), kCostBlocks));
-  hipLaunchKernelGGL(k_update_poses, dim3(g), dim3(kThreads), 0, s, p, poses, w.dx_cam,
+  const int g = pose_update_blocks(p);
+  const int npb = p.intr_block >= 0 ? p.intr_block : p.nb;
+  hipLaunchKernelGGL(k_update_poses, dim3(g), dim3(kThreads), 0, s, p, npb, poses, w.dx_cam,
                      w.grad_cam, poses_new, w.partial + 2 * gpt, sc);
+  if (p.intr_block >= 0)
+    hipLaunchKernelGGL(k_update_intrinsics, dim3(1), dim3(64), 0, s, p, w.dx_cam, w.grad_cam, intr,
+                       intr_new, w.partial + 2 * gpt + 2 * g, sc);
 }
 
-void launch_finalize_step(hipStream_t s, const DevLin& w, int nb, int npt, DevScalars* sc) {
-  const int gpt = std::max(1, std::min(blocks_for(npt, kThreads), kCostBlocks));
-  const int gpo = std::max(1, std::min(blocks_for(nb, kThreads), kCostBlocks));
+void launch_finalize_step(hipStream_t s, const DevProblem& p, const DevLin& w, DevScalars* sc) {
+  const int gpt = std::max(1, std::min(blocks_for(p.npt, kThreads), kCostBlocks));
+  const int gpo = pose_update_blocks(p) + (p.intr_block >= 0 ? 1 : 0);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, w.partial, gpt, gpo, 0, nullptr, 0, sc,
                      0);
 }
@@ -2589,8 +3046,8 @@ void launch_finalize_step(hipStream_t s, const DevLin& w, int nb, int npt, DevSc
 void launch_cost(hipStream_t s, const DevProblem& p, const double* poses, const double* points,
                  const double* intr, double* partial, int32_t partial_cap, DevScalars* sc) {
   (void)partial_cap;
-  hipLaunchKernelGGL(k_cost, dim3(kCostBlocks), dim3(kThreads), 0, s, p, p.obs_point, poses,
-                     points, intr, partial);
+  hipLaunchKernelGGL((p.cam_model == kEucm ? k_cost<kEucm> : k_cost<kPinhole>), dim3(kCostBlocks),
+                     dim3(kThreads), 0, s, p, p.obs_point, poses, points, intr, partial);
   (void)sc;
 }
 
@@ -2636,6 +3093,14 @@ void launch_eval_pinhole(hipStream_t s, int precision, int64_t n, const double* 
   else
     hipLaunchKernelGGL(k_eval_pinhole<float>, dim3(g), dim3(kThreads), 0, s, n, intr, pose, pt, uv,
                        robust, delta, res, jpose, jpt, cost, weight);
+}
+
+void launch_eval_eucm(hipStream_t s, int64_t n, const double* intr, const double* pose,
+                      const double* pt, const double* uv, int robust, double delta, double* res,
+                      double* jintr, double* jpose, double* jpt, double* cost, double* weight) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_eval_eucm, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, n, intr,
+                     pose, pt, uv, robust, delta, res, jintr, jpose, jpt, cost, weight);
 }
 
 void launch_eval_pose_graph(hipStream_t s, int64_t n, const double* m, const double* nn,

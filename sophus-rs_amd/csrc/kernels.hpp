@@ -53,6 +53,13 @@ struct DevProblem {
   const int32_t* s_ld;
   int robust;
   double delta;
+  int32_t cam_model;     // 0 pinhole, 1 EUCM
+  // free shared intrinsics: their free block (after the poses; -1 when conditioned), their
+  // member, the S slot of (intrinsics, pose block) per pose block (-1: none) and of
+  // (intrinsics, intrinsics)
+  int32_t intr_block, intr_member;
+  const int32_t* intr_slot;
+  int64_t intr_diag_slot;
 };
 
 // RCS supernodal multifrontal factorization on the device.
@@ -113,6 +120,10 @@ struct DevLin {
   double* own_part;   // 48 per own-term chunk: the chunk's (o, o) block, rhs and gradient sums
   double* partial;    // reduction partials
   int32_t partial_cap;
+  // free shared intrinsics only (else null): per point V_b = W_b H_ll^-1 (6x3 row-major) then
+  // W_b = H_cl (6x3); per point-pass workgroup the partial sums of the intrinsics rows
+  double* pt_vw;
+  double* intr_part;
 };
 
 void launch_cam_cache(hipStream_t s, int npose, const double* poses, double* cw);
@@ -147,11 +158,21 @@ void launch_point_coupling(hipStream_t s, const DevProblem& p, const double* poi
 void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p,
                            const double* points, const double* intr, const double* cw,
                            const DevLin& w, double* points_new, DevScalars* sc);
+// Pose update T <- exp(delta) T into poses_new, and the free intrinsics' additive update into
+// intr_new (which holds a copy of intr).
 void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses,
-                         const DevLin& w, double* poses_new, DevScalars* sc);
+                         const double* intr, const DevLin& w, double* poses_new, double* intr_new,
+                         DevScalars* sc);
+// Free shared intrinsics (p.intr_block >= 0): V_b and W_b per point, the S blocks
+// (intrinsics, pose) and (intrinsics, intrinsics), the intrinsics' reduced rhs and
+// neg-gradient.  After launch_camera_rows (which leaves those slots zero but the damping).
+void launch_intrinsics_rows(hipStream_t s, const DevProblem& p, const double* points,
+                            const double* intr, const double* cw, DevLin& w, DevFactor& f);
+// doubles of DevLin::intr_part for npt points
+int intrinsics_partials(int npt);
 void launch_cost(hipStream_t s, const DevProblem& p, const double* poses, const double* points,
                  const double* intr, double* partial, int32_t partial_cap, DevScalars* sc);
-void launch_finalize_step(hipStream_t s, const DevLin& w, int nb, int npt, DevScalars* sc);
+void launch_finalize_step(hipStream_t s, const DevProblem& p, const DevLin& w, DevScalars* sc);
 void launch_finalize_cost(hipStream_t s, const double* partial, int nblocks, DevScalars* sc);
 void launch_max_grad(hipStream_t s, const double* grad_cam, int nb, DevScalars* sc);
 void launch_zero_scalars(hipStream_t s, DevScalars* sc);
@@ -172,6 +193,8 @@ void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const Dev
                                  const double* centered_cw);
 // Converts centred camera steps back: delta = G delta' (nu += c x omega).
 void launch_uncenter(hipStream_t s, const DevProblem& p, const double* cw, double* dx_cam);
+// Diagonal of the assembled S (6 nb, camera order) from the fronts, before the factorization.
+void launch_gather_s_diag(hipStream_t s, const DevProblem& p, const double* front, double* out);
 // Scatter S blocks given per slot (36 doubles, row-major, camera order) into the fronts.
 void launch_scatter_s(hipStream_t s, const DevProblem& p, int64_t nslots, const double* blocks,
                       DevFactor& f);
@@ -183,6 +206,9 @@ void launch_eval_pinhole(hipStream_t s, int precision, int64_t n, const double* 
                          const double* pose, const double* pt, const double* uv, int robust,
                          double delta, double* res, double* jpose, double* jpt, double* cost,
                          double* weight);
+void launch_eval_eucm(hipStream_t s, int64_t n, const double* intr, const double* pose,
+                      const double* pt, const double* uv, int robust, double delta, double* res,
+                      double* jintr, double* jpose, double* jpt, double* cost, double* weight);
 void launch_eval_pose_graph(hipStream_t s, int64_t n, const double* m, const double* nn,
                             const double* mn, double* res, double* jm, double* jn);
 

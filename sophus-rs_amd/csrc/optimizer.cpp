@@ -76,7 +76,8 @@ struct sgpu_optimizer {
   double* poses[2] = {nullptr, nullptr};
   double* points[2] = {nullptr, nullptr};
   int cur = 0;
-  double* intr = nullptr;
+  // intrinsics members (both copies equal unless the intrinsics are free)
+  double* intr[2] = {nullptr, nullptr};
   double* cost_partial = nullptr;
   DevScalars* d_sc = nullptr;
   DevScalars* h_sc = nullptr;  // pinned
@@ -103,7 +104,11 @@ struct sgpu_optimizer {
   double *cov_coup = nullptr, *cov_hinv = nullptr;
   int32_t* cov_cam = nullptr;
   int32_t cov_max_obs = 0;
-  double cov_pivot_ratio = 0.0;  // smallest pivot / largest pivot of the factored S (<= 0: none)
+  // smallest kept pivot of the factored S relative to the diagonal entry of S in its column
+  // (scale-invariant: a column nearly in the span of the ones eliminated before it has a
+  // pivot far below its diagonal entry)
+  double cov_pivot_ratio = 0.0;
+  double* cov_sdiag = nullptr;
 
   ~sgpu_optimizer() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -150,17 +155,19 @@ struct sgpu_optimizer {
   const Family& pose_fam() const { return P.fams[P.f_pose]; }
   const Family& point_fam() const { return P.fams[P.f_point]; }
 
-  void linearize(const double* ps, const double* pts, double nu_lin, bool* used) {
+  void linearize(const double* ps, const double* pts, const double* in, double nu_lin,
+                 bool* used) {
     cov_ready = false;  // the fronts and point records are rewritten
     launch_zero_scalars(stream, d_sc);
     launch_cam_cache(stream, S.npose, ps, dl.cw);
     if (!iterative()) launch_zero_fronts(stream, df);
     else if (!dpg.empty()) launch_fill(stream, dl.bsr, int64_t(36) * dp.nslot, 0.0);
     tic(K_LINEARIZE_POINTS);
-    launch_point_prep(stream, params.precision, dp, ps, pts, intr, dl.cw, nu_lin, dl, d_sc);
+    launch_point_prep(stream, params.precision, dp, ps, pts, in, dl.cw, nu_lin, dl, d_sc);
     toc(K_LINEARIZE_POINTS);
     tic(K_CAMERA_ROWS);
-    launch_camera_rows(stream, params.precision, dp, pts, intr, dl.cw, nu_lin, dl, df, d_sc);
+    launch_camera_rows(stream, params.precision, dp, pts, in, dl.cw, nu_lin, dl, df, d_sc);
+    launch_intrinsics_rows(stream, dp, pts, in, dl.cw, dl, df);
     toc(K_CAMERA_ROWS);
     const double* centered = params.precision == SGPU_PRECISION_MIXED ? dl.cw : nullptr;
     for (auto& g : dpg) launch_pose_graph_linearize(stream, g, dp, ps, nu_lin, dl, df, centered);
@@ -168,9 +175,9 @@ struct sgpu_optimizer {
     used[K_LINEARIZE_POINTS] = used[K_CAMERA_ROWS] = true;
   }
 
-  void cost_of(const double* ps, const double* pts, bool* used) {
+  void cost_of(const double* ps, const double* pts, const double* in, bool* used) {
     tic(K_COST);
-    launch_cost(stream, dp, ps, pts, intr, cost_partial, 2 * kCostBlocks, d_sc);
+    launch_cost(stream, dp, ps, pts, in, cost_partial, 2 * kCostBlocks, d_sc);
     toc(K_COST);
     int nblk = kCostBlocks;
     for (auto& g : dpg) {
@@ -181,10 +188,10 @@ struct sgpu_optimizer {
     if (used) used[K_COST] = true;
   }
 
-  double compute_merit(const double* ps, const double* pts) {
+  double compute_merit(const double* ps, const double* pts, const double* in) {
     bool used[K_COUNT] = {};
     launch_zero_scalars(stream, d_sc);
-    cost_of(ps, pts, used);
+    cost_of(ps, pts, in, used);
     SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipStreamSynchronize(stream));
     collect(used);
@@ -208,6 +215,10 @@ struct sgpu_optimizer {
     if (params.precision != SGPU_PRECISION_F64 && params.precision != SGPU_PRECISION_MIXED)
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown precision");
     P = build_host_problem(families, nf, cfs, nc);
+    if (P.intr_block >= 0 && params.precision != SGPU_PRECISION_F64)
+      throw Error(SGPU_ERR_UNSUPPORTED,
+                  "free intrinsics are implemented in the FP64 precision (the mixed path's centred "
+                  "camera tangents do not cover the intrinsics block)");
     S = analyze(P);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -229,7 +240,7 @@ struct sgpu_optimizer {
     }
     SGPU_HIP(hipMemcpy(poses[0], pf.members.data(), pf.members.size() * 8, hipMemcpyHostToDevice));
     SGPU_HIP(hipMemcpy(points[0], xf.members.data(), xf.members.size() * 8, hipMemcpyHostToDevice));
-    intr = arena.upload(kf.members, stream);
+    for (int i = 0; i < 2; ++i) intr[i] = arena.upload(kf.members, stream);
     // problem arrays
     std::vector<int32_t> pt_block(xf.block.begin(), xf.block.end());
     dp.nb = S.nb;
@@ -282,6 +293,11 @@ struct sgpu_optimizer {
     dp.s_ld = arena.upload(S.s_ld, stream);
     dp.robust = P.robust;
     dp.delta = P.delta;
+    dp.cam_model = P.cam_model;
+    dp.intr_block = P.intr_block;
+    dp.intr_member = P.intr_member;
+    dp.intr_slot = P.intr_block >= 0 ? arena.upload(S.intr_slot, stream) : nullptr;
+    dp.intr_diag_slot = S.intr_diag_slot;
     // factor structure
     df.nb = S.nb;
     df.nsn = S.nsn;
@@ -365,7 +381,13 @@ struct sgpu_optimizer {
         learned.out = arena.alloc<float>(size_t(kPcOut) * S.nb);
       }
     }
-    dl.partial_cap = 4 * kCostBlocks;
+    dl.pt_vw = nullptr;
+    dl.intr_part = nullptr;
+    if (P.intr_block >= 0) {
+      dl.pt_vw = arena.alloc<double>(size_t(36) * std::max<int32_t>(S.npt, 1));
+      dl.intr_part = arena.alloc<double>(size_t(intrinsics_partials(S.npt)));
+    }
+    dl.partial_cap = 4 * kCostBlocks + 2;
     dl.partial = arena.alloc<double>(dl.partial_cap);
     cost_partial = arena.alloc<double>(size_t(2) * kCostBlocks * (1 + P.pg.size()));
     d_sc = arena.alloc<DevScalars>(1);
@@ -385,7 +407,7 @@ struct sgpu_optimizer {
       dpg.push_back(q);
     }
     nu = params.initial_lm_damping;
-    merit = compute_merit(poses[0], points[0]);
+    merit = compute_merit(poses[0], points[0], intr[0]);
   }
 
   // The CG tolerance of this step: pcg_relative_tolerance, or with pcg_forcing = eta > 0
@@ -404,7 +426,7 @@ struct sgpu_optimizer {
     bool used[K_COUNT] = {};
     const int nxt = 1 - cur;
     SGPU_HIP(hipEventRecord(phase_ev[0], stream));
-    linearize(poses[cur], points[cur], nu, used);
+    linearize(poses[cur], points[cur], intr[cur], nu, used);
     SGPU_HIP(hipEventRecord(phase_ev[1], stream));
     if (iterative()) {
       // block-Jacobi PCG on the resident S (two launches per CG iteration); its iteration count
@@ -431,16 +453,20 @@ struct sgpu_optimizer {
     if (params.precision == SGPU_PRECISION_MIXED) launch_uncenter(stream, dp, dl.cw, dl.dx_cam);
     toc(K_TRIANGULAR);
     tic(K_BACKSUB);
-    launch_backsub_points(stream, params.precision, dp, points[cur], intr, dl.cw, dl, points[nxt], d_sc);
+    launch_backsub_points(stream, params.precision, dp, points[cur], intr[cur], dl.cw, dl,
+                          points[nxt], d_sc);
     toc(K_BACKSUB);
     SGPU_HIP(hipMemcpyAsync(poses[nxt], poses[cur], pose_fam().members.size() * 8,
                             hipMemcpyDeviceToDevice, stream));
+    if (P.intr_block >= 0)
+      SGPU_HIP(hipMemcpyAsync(intr[nxt], intr[cur], P.fams[P.f_intr].members.size() * 8,
+                              hipMemcpyDeviceToDevice, stream));
     tic(K_UPDATE);
-    launch_update_poses(stream, dp, poses[cur], dl, poses[nxt], d_sc);
+    launch_update_poses(stream, dp, poses[cur], intr[cur], dl, poses[nxt], intr[nxt], d_sc);
     toc(K_UPDATE);
-    launch_finalize_step(stream, dl, S.nb, S.npt, d_sc);
+    launch_finalize_step(stream, dp, dl, d_sc);
     SGPU_HIP(hipEventRecord(phase_ev[3], stream));
-    cost_of(poses[nxt], points[nxt], used);
+    cost_of(poses[nxt], points[nxt], intr[nxt], used);
     SGPU_HIP(hipEventRecord(phase_ev[4], stream));
     SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipStreamSynchronize(stream));
@@ -526,8 +552,11 @@ struct sgpu_optimizer {
     const Family& f = P.fams[fam_sorted];
     const int64_t need = int64_t(f.members.size());
     if (capacity < need) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
-    if (fam_sorted == P.f_pose || fam_sorted == P.f_point) {
-      const double* src = fam_sorted == P.f_pose ? poses[cur] : points[cur];
+    if (fam_sorted == P.f_pose || fam_sorted == P.f_point ||
+        (fam_sorted == P.f_intr && P.intr_block >= 0)) {
+      const double* src = fam_sorted == P.f_pose    ? poses[cur]
+                          : fam_sorted == P.f_point ? points[cur]
+                                                    : intr[cur];
       SGPU_HIP(hipMemcpyAsync(out, src, need * 8, hipMemcpyDeviceToHost, stream));
       SGPU_HIP(hipStreamSynchronize(stream));
     } else {
@@ -558,25 +587,29 @@ struct sgpu_optimizer {
       cov_coup = arena.alloc<double>(size_t(18) * std::max(cov_max_obs, 1));
       cov_cam = arena.alloc<int32_t>(size_t(std::max(cov_max_obs, 1)));
       cov_hinv = arena.alloc<double>(9);
+      cov_sdiag = arena.alloc<double>(size_t(n));
     }
     bool used[K_COUNT] = {};
-    linearize(poses[cur], points[cur], params.skip_final_hessian ? nu : 0.0, used);
+    linearize(poses[cur], points[cur], intr[cur], params.skip_final_hessian ? nu : 0.0, used);
+    launch_gather_s_diag(stream, dp, df.front, cov_sdiag);
     launch_rcs_factor(stream, df, d_sc, dl.rhs_nat, dl.x_perm);
     SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
-    std::vector<double> d(n), Lp(size_t(6) * n);
+    std::vector<double> d(n), Lp(size_t(6) * n), sdiag(n);
     SGPU_HIP(hipMemcpyAsync(d.data(), df.piv_d, n * 8, hipMemcpyDeviceToHost, stream));
+    SGPU_HIP(hipMemcpyAsync(sdiag.data(), cov_sdiag, n * 8, hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipMemcpyAsync(Lp.data(), df.piv_L, size_t(6) * n * 8, hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipStreamSynchronize(stream));
     if (h_sc->singular_marg) throw Error(SGPU_ERR_NON_FINITE, "singular marginal block in the final Hessian");
     if (h_sc->bad_pivot) throw Error(SGPU_ERR_NON_FINITE, "non-finite pivot in the final Hessian");
     cov_null.clear();
     {
-      double lo = INFINITY, hi = 0.0;
-      for (double x : d) {
-        lo = std::min(lo, x);
-        hi = std::max(hi, x);
+      // pivots are in elimination order: pivot block i is camera block perm[i]
+      double lo = INFINITY;
+      for (int64_t i = 0; i < n; ++i) {
+        const double sd = sdiag[6 * size_t(S.perm[i / 6]) + i % 6];
+        lo = std::min(lo, sd > 0.0 ? d[i] / sd : 0.0);
       }
-      cov_pivot_ratio = hi > 0.0 ? lo / hi : 0.0;
+      cov_pivot_ratio = std::isfinite(lo) ? lo : 0.0;
     }
     std::vector<double> e(n), v(n);
     for (int64_t gb = 0; gb < S.nb; ++gb) {
@@ -655,12 +688,17 @@ struct sgpu_optimizer {
   // and its (damped) H_ll^-1 (row-major)
   void coupling(int32_t b, std::vector<std::vector<double>>& cols, double hinv[9]) {
     const int32_t nq = S.pt_ptr[b + 1] - S.pt_ptr[b];
-    launch_point_coupling(stream, dp, points[cur], intr, dl.cw, dl.prec, b, cov_coup, cov_cam, cov_hinv);
+    launch_point_coupling(stream, dp, points[cur], intr[cur], dl.cw, dl.prec, b, cov_coup, cov_cam,
+                          cov_hinv);
     std::vector<double> H(size_t(18) * std::max(nq, 1));
     std::vector<int32_t> cams(std::max(nq, 1));
+    double Wb[18] = {};
     SGPU_HIP(hipMemcpyAsync(H.data(), cov_coup, size_t(18) * nq * 8, hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipMemcpyAsync(cams.data(), cov_cam, size_t(nq) * 4, hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipMemcpyAsync(hinv, cov_hinv, 9 * 8, hipMemcpyDeviceToHost, stream));
+    // free intrinsics: their coupling H_cl = W_b from the intrinsics point pass
+    if (P.intr_block >= 0)
+      SGPU_HIP(hipMemcpyAsync(Wb, dl.pt_vw + 36 * size_t(b) + 18, 18 * 8, hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipStreamSynchronize(stream));
     cols.assign(3, std::vector<double>(6 * size_t(S.nb), 0.0));
     for (int32_t q = 0; q < nq; ++q) {
@@ -668,6 +706,9 @@ struct sgpu_optimizer {
       for (int r = 0; r < 6; ++r)
         for (int k = 0; k < 3; ++k) cols[k][6 * size_t(cams[q]) + r] += H[18 * q + 3 * r + k];
     }
+    if (P.intr_block >= 0)
+      for (int r = 0; r < 6; ++r)
+        for (int k = 0; k < 3; ++k) cols[k][6 * size_t(P.intr_block) + r] += Wb[3 * r + k];
   }
 
   // (row, col) = (block, is_free); out row-major
@@ -735,7 +776,7 @@ struct sgpu_optimizer {
 
   double final_max_grad(double nu_lin) {
     bool used[K_COUNT] = {};
-    linearize(poses[cur], points[cur], nu_lin, used);
+    linearize(poses[cur], points[cur], intr[cur], nu_lin, used);
     SGPU_HIP(hipMemcpyAsync(h_sc, d_sc, sizeof(DevScalars), hipMemcpyDeviceToHost, stream));
     SGPU_HIP(hipStreamSynchronize(stream));
     collect(used);
@@ -818,11 +859,14 @@ sgpu_status sgpu_optimizer_eval_cost_at(sgpu_optimizer* opt, const double* const
         SGPU_HIP(hipMemcpyAsync(opt->poses[nxt], members[i], f.members.size() * 8, hipMemcpyHostToDevice, opt->stream));
       else if (s == opt->P.f_point)
         SGPU_HIP(hipMemcpyAsync(opt->points[nxt], members[i], f.members.size() * 8, hipMemcpyHostToDevice, opt->stream));
+      else if (s == opt->P.f_intr && opt->P.intr_block >= 0)
+        SGPU_HIP(hipMemcpyAsync(opt->intr[nxt], members[i], f.members.size() * 8, hipMemcpyHostToDevice, opt->stream));
       else if (s == opt->P.f_intr &&
                std::memcmp(members[i], f.members.data(), f.members.size() * 8) != 0)
         throw Error(SGPU_ERR_UNSUPPORTED, "conditioned intrinsics differ from the handle's");
     }
-    *cost = opt->compute_merit(opt->poses[nxt], opt->points[nxt]);
+    *cost = opt->compute_merit(opt->poses[nxt], opt->points[nxt],
+                               opt->P.intr_block >= 0 ? opt->intr[nxt] : opt->intr[opt->cur]);
   });
 }
 
@@ -894,7 +938,7 @@ sgpu_status sgpu_optimizer_inverse_block(sgpu_optimizer* opt, const char* row_fa
           throw Error(SGPU_ERR_INVALID_ARGUMENT,
                       std::string("member of '") + name + "' is conditioned or constant: it has no block");
         is_free = f.kind == SGPU_VAR_FREE;
-        block = is_free ? f.block[member] : int32_t(member);
+        block = !is_free ? int32_t(member) : int(i) == opt->P.f_intr ? opt->P.intr_block : f.block[member];
         return;
       }
       throw Error(SGPU_ERR_UNKNOWN_FAMILY, std::string("unknown family '") + name + "'");
@@ -922,11 +966,12 @@ sgpu_status sgpu_optimizer_covariance_block(sgpu_optimizer* opt, const char* row
                                                       col_member, out, capacity);
   if (st != SGPU_OK) return st;
   // rank-deficient (a non-positive pivot) or numerically singular (a pivot below 1e-9 of the
-  // largest, e.g. a gauge direction left at rounding level with either sign)
+  // diagonal entry of S in its column, e.g. a gauge direction left at rounding level with
+  // either sign)
   if (!opt->cov_null.empty() || opt->cov_pivot_ratio < 1e-9)
     return set_error(SGPU_ERR_UNSUPPORTED,
                      "the final Hessian is rank-deficient (" + std::to_string(opt->cov_null.size()) +
-                         " non-positive pivots of its Schur complement, smallest / largest pivot " +
+                         " non-positive pivots of its Schur complement, smallest pivot / diagonal " +
                          std::to_string(opt->cov_pivot_ratio) +
                          ", e.g. an unfixed gauge): its min-norm inverse is not the "
                          "Schur-complement inverse block; use sgpu_optimizer_inverse_block for "
@@ -934,11 +979,13 @@ sgpu_status sgpu_optimizer_covariance_block(sgpu_optimizer* opt, const char* row
   return SGPU_OK;
 }
 
-static void copy_order(const Symbolic& S, int64_t* perm, int64_t capacity, int64_t* num_blocks) {
+// The elimination order in the reference's free-block numbering (var_families.rs:60-113).
+static void copy_order(const HostProblem& P, const Symbolic& S, int64_t* perm, int64_t capacity,
+                       int64_t* num_blocks) {
   if (num_blocks) *num_blocks = S.nb;
   if (!perm) return;
   if (capacity < S.nb) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
-  for (int32_t i = 0; i < S.nb; ++i) perm[i] = S.perm[i];
+  for (int32_t i = 0; i < S.nb; ++i) perm[i] = P.ref_block[S.perm[i]];
 }
 
 sgpu_status sgpu_analyze_rcs_order(const sgpu_var_family* families, int64_t num_families,
@@ -946,7 +993,7 @@ sgpu_status sgpu_analyze_rcs_order(const sgpu_var_family* families, int64_t num_
                                    int64_t* perm, int64_t capacity, int64_t* num_blocks) {
   return guarded([&] {
     const HostProblem P = build_host_problem(families, num_families, cost_fns, num_cost_fns);
-    copy_order(analyze(P), perm, capacity, num_blocks);
+    copy_order(P, analyze(P), perm, capacity, num_blocks);
   });
 }
 
@@ -954,7 +1001,7 @@ sgpu_status sgpu_optimizer_rcs_order(const sgpu_optimizer* opt, int64_t* perm, i
                                      int64_t* num_blocks) {
   return guarded([&] {
     if (!opt) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null optimizer");
-    copy_order(opt->S, perm, capacity, num_blocks);
+    copy_order(opt->P, opt->S, perm, capacity, num_blocks);
   });
 }
 

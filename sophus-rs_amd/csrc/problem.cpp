@@ -11,9 +11,14 @@
 namespace sgpu {
 namespace {
 
-int var_dof(int type) { return type == SGPU_VAR_ISOMETRY3 ? 6 : type == SGPU_VAR_VEC3 ? 3 : 4; }
+int var_dof(int type) {
+  return type == SGPU_VAR_ISOMETRY3 || type == SGPU_VAR_EUCM ? 6 : type == SGPU_VAR_VEC3 ? 3 : 4;
+}
 int var_params(int type) {
-  return type == SGPU_VAR_ISOMETRY3 ? 7 : type == SGPU_VAR_VEC3 ? 3 : 4;
+  return type == SGPU_VAR_ISOMETRY3 ? 7 : type == SGPU_VAR_VEC3 ? 3 : type == SGPU_VAR_EUCM ? 6 : 4;
+}
+bool is_reprojection(int kind) {
+  return kind == SGPU_COST_PINHOLE_REPROJECTION || kind == SGPU_COST_EUCM_REPROJECTION;
 }
 
 [[noreturn]] void raise(sgpu_status s, const std::string& m) { throw Error(s, m); }
@@ -28,7 +33,7 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
   std::vector<Family> in(num_families);
   for (int64_t i = 0; i < num_families; ++i) {
     const sgpu_var_family& s = families[i];
-    if (!s.name || s.type < 0 || s.type > 2 || s.kind < 0 || s.kind > 2 || s.num_members < 0 ||
+    if (!s.name || s.type < 0 || s.type > 3 || s.kind < 0 || s.kind > 2 || s.num_members < 0 ||
         (s.num_members > 0 && !s.members) || (s.num_const_ids > 0 && !s.const_ids))
       raise(SGPU_ERR_INVALID_ARGUMENT, "bad variable family descriptor");
     if (s.num_members > INT32_MAX / 8) raise(SGPU_ERR_UNSUPPORTED, "family too large");
@@ -80,7 +85,7 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
           "Schur solver requires at least one active free and one active marginalized variable");
   for (int64_t i = 0; i < num_cost_fns; ++i) {
     const sgpu_cost_fn& c = cost_fns[i];
-    const int arity = c.kind == SGPU_COST_PINHOLE_REPROJECTION ? 3 : 2;
+    const int arity = is_reprojection(c.kind) ? 3 : 2;
     int marg = 0;
     for (int a = 0; a < arity; ++a) {
       if (!c.family_names[a]) raise(SGPU_ERR_INVALID_ARGUMENT, "missing cost family name");
@@ -93,13 +98,14 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
   }
   for (int64_t i = 0; i < num_cost_fns; ++i) {
     const sgpu_cost_fn& c = cost_fns[i];
-    if (c.kind != SGPU_COST_PINHOLE_REPROJECTION && c.kind != SGPU_COST_POSE_GRAPH3)
+    if (!is_reprojection(c.kind) && c.kind != SGPU_COST_POSE_GRAPH3)
       raise(SGPU_ERR_INVALID_ARGUMENT, "unknown cost kind");
     if (c.robust_kernel != SGPU_ROBUST_NONE && c.robust_kernel != SGPU_ROBUST_HUBER)
       raise(SGPU_ERR_INVALID_ARGUMENT, "unknown robust kernel");
-    const int arity = c.kind == SGPU_COST_PINHOLE_REPROJECTION ? 3 : 2;
-    const int want[2][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
-                            {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1}};
+    const int arity = is_reprojection(c.kind) ? 3 : 2;
+    const int want[3][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+                            {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1},
+                            {SGPU_VAR_EUCM, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3}};
     int fi[3] = {-1, -1, -1};
     for (int a = 0; a < arity; ++a) {
       fi[a] = find(c.family_names[a]);
@@ -118,33 +124,50 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
         if (v < 0 || v >= P.fams[fi[a]].n)
           raise(SGPU_ERR_INDEX_OUT_OF_RANGE, "entity index out of range");
       }
-    // The GPU path implements the BA layout: SE(3) poses Free, points Marginalized,
-    // pinhole intrinsics Conditioned.
-    if (c.kind == SGPU_COST_PINHOLE_REPROJECTION) {
-      if (P.has_reproj)
-        raise(SGPU_ERR_UNSUPPORTED, "more than one pinhole reprojection cost function");
-      if (P.fams[fi[0]].kind != SGPU_VAR_CONDITIONED || P.fams[fi[1]].kind != SGPU_VAR_FREE ||
+    // The GPU path implements the BA layout: SE(3) poses Free, points Marginalized, and
+    // intrinsics Conditioned, or (EUCM) Free with one active member.
+    if (is_reprojection(c.kind)) {
+      if (P.has_reproj) raise(SGPU_ERR_UNSUPPORTED, "more than one reprojection cost function");
+      const Family& kf = P.fams[fi[0]];
+      if (kf.kind == SGPU_VAR_MARGINALIZED || P.fams[fi[1]].kind != SGPU_VAR_FREE ||
           P.fams[fi[2]].kind != SGPU_VAR_MARGINALIZED)
         raise(SGPU_ERR_UNSUPPORTED,
-              "reprojection terms need Conditioned intrinsics, Free poses and Marginalized "
-              "points");
+              "reprojection terms need Conditioned or Free intrinsics, Free poses and "
+              "Marginalized points");
+      if (kf.kind == SGPU_VAR_FREE && c.kind == SGPU_COST_PINHOLE_REPROJECTION && kf.num_active > 0)
+        raise(SGPU_ERR_UNSUPPORTED,
+              "free pinhole intrinsics are not implemented (a 4-dof block in the RCS of 6x6 "
+              "blocks); free EUCM intrinsics are");
+      if (kf.kind == SGPU_VAR_FREE && kf.num_active > 1)
+        raise(SGPU_ERR_UNSUPPORTED,
+              "more than one active member of a free intrinsics family (one shared calibration "
+              "block is implemented)");
       if ((P.f_pose >= 0 && P.f_pose != fi[1]))
         raise(SGPU_ERR_UNSUPPORTED, "cost functions use different pose families");
       P.has_reproj = true;
       P.f_intr = fi[0];
       P.f_pose = fi[1];
       P.f_point = fi[2];
+      P.cam_model = c.kind == SGPU_COST_EUCM_REPROJECTION ? 1 : 0;
+      P.intr_params = kf.nparams;
+      P.intr_kind = kf.kind;
+      if (kf.kind == SGPU_VAR_FREE)
+        for (int64_t i = 0; i < kf.n; ++i)
+          if (kf.block[i] >= 0) P.intr_member = int32_t(i);
       P.robust = c.robust_kernel;
       P.delta = c.huber_delta;
-      // CostFn::sort: stable sort by (point, pose, intrinsics)
+      // CostFn::sort (compare_idx.rs:38-128): stable sort by the marginalized argument,
+      // then the free ones in argument order, then the conditioned one
+      const bool intr_free = kf.kind == SGPU_VAR_FREE;
       const int64_t nt = c.num_terms;
       std::vector<int32_t> ord(nt);
       std::iota(ord.begin(), ord.end(), 0);
       const int64_t* E = c.entity_indices;
+      const int k1 = intr_free ? 0 : 1, k2 = intr_free ? 1 : 0;
       std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
         if (E[3 * x + 2] != E[3 * y + 2]) return E[3 * x + 2] < E[3 * y + 2];
-        if (E[3 * x + 1] != E[3 * y + 1]) return E[3 * x + 1] < E[3 * y + 1];
-        return E[3 * x] < E[3 * y];
+        if (E[3 * x + k1] != E[3 * y + k1]) return E[3 * x + k1] < E[3 * y + k1];
+        return E[3 * x + k2] < E[3 * y + k2];
       });
       P.obs_pose.resize(nt);
       P.obs_point.resize(nt);
@@ -187,21 +210,35 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
       P.pg.push_back(std::move(g));
     }
   }
-  // every Free / Marginalized family must be the pose / point family of this layout
+  // every Free / Marginalized family must be the pose (or intrinsics) / point family of this
+  // layout
   for (size_t i = 0; i < P.fams.size(); ++i) {
     const Family& f = P.fams[i];
-    if (f.kind == SGPU_VAR_FREE && int(i) != P.f_pose && f.num_active > 0)
+    if (f.kind == SGPU_VAR_FREE && int(i) != P.f_pose && int(i) != P.f_intr && f.num_active > 0)
       raise(SGPU_ERR_UNSUPPORTED, "free family '" + f.name + "' is not the pose family");
     if (f.kind == SGPU_VAR_MARGINALIZED && int(i) != P.f_point && f.num_active > 0)
       raise(SGPU_ERR_UNSUPPORTED, "marginalized family '" + f.name + "' is not the point family");
   }
   if (P.f_point < 0) raise(SGPU_ERR_UNSUPPORTED, "no reprojection cost function");
-  P.num_free_blocks = P.fams[P.f_pose].num_active;
+  P.num_pose_blocks = P.fams[P.f_pose].num_active;
+  P.intr_block = P.intr_member >= 0 ? int32_t(P.num_pose_blocks) : -1;
+  P.num_free_blocks = P.num_pose_blocks + (P.intr_member >= 0 ? 1 : 0);
   P.num_marg_blocks = P.fams[P.f_point].num_active;
   const Family& fp = P.fams[P.f_pose];
-  P.cam_member.resize(P.num_free_blocks);
+  P.cam_member.assign(P.num_free_blocks, -1);
   for (int64_t i = 0; i < fp.n; ++i)
     if (fp.block[i] >= 0) P.cam_member[fp.block[i]] = int32_t(i);
+  // the reference's free-block numbering: Free families in name order (fams is sorted)
+  P.ref_block.assign(P.num_free_blocks, -1);
+  int32_t next = 0;
+  for (size_t i = 0; i < P.fams.size(); ++i) {
+    if (P.fams[i].kind != SGPU_VAR_FREE) continue;
+    if (int(i) == P.f_pose) {
+      for (int32_t b = 0; b < int32_t(P.num_pose_blocks); ++b) P.ref_block[b] = next++;
+    } else if (int(i) == P.f_intr && P.intr_block >= 0) {
+      P.ref_block[P.intr_block] = next++;
+    }
+  }
   return P;
 }
 
@@ -405,13 +442,17 @@ Symbolic analyze(const HostProblem& P) {
   for (int32_t b = 0; b < S.npt; ++b) S.pt_ptr[b + 1] += S.pt_ptr[b];
   S.pt_nu.assign(S.npt, 0);
   S.cam_nu.assign(S.nb, 0);
+  // a reduced term is a run of terms equal in every non-conditioned argument
+  const bool intr_key = P.intr_kind != SGPU_VAR_CONDITIONED;
   for (int32_t o = 0; o < S.nobs; ++o) {
     const bool first = o == 0 || P.obs_point[o] != P.obs_point[o - 1] ||
-                       P.obs_pose[o] != P.obs_pose[o - 1];
+                       P.obs_pose[o] != P.obs_pose[o - 1] ||
+                       (intr_key && P.obs_intr[o] != P.obs_intr[o - 1]);
     if (!first) continue;
     ++S.pt_nu[P.obs_point[o]];
     const int32_t c = poses.block[P.obs_pose[o]];
     if (c >= 0) ++S.cam_nu[c];
+    if (P.intr_block >= 0 && P.obs_intr[o] == P.intr_member) ++S.cam_nu[P.intr_block];
   }
   // camera-major lists
   S.cam_ptr.assign(S.nb + 1, 0);
@@ -446,6 +487,12 @@ Symbolic analyze(const HostProblem& P) {
     for (size_t t = 0; t < g.m.size(); ++t) {
       const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
       if (a >= 0 && b >= 0) rows[std::max(a, b)].push_back(std::min(a, b));
+    }
+  // free intrinsics: the last row, coupled to every pose observed through them
+  if (P.intr_block >= 0)
+    for (int32_t o = 0; o < S.nobs; ++o) {
+      const int32_t c = poses.block[P.obs_pose[o]];
+      if (c >= 0 && P.obs_intr[o] == P.intr_member) rows[P.intr_block].push_back(c);
     }
   S.s_ptr.assign(S.nb + 1, 0);
   for (int32_t c = 0; c < S.nb; ++c) {
@@ -504,6 +551,17 @@ Symbolic analyze(const HostProblem& P) {
     }
   }
   S.num_schur_pairs = int64_t(S.pairs.size() / 2) + S.cam_ptr[S.nb];
+  if (P.intr_block >= 0) {
+    const int32_t r = P.intr_block;
+    S.intr_slot.assign(S.nb, -1);
+    for (int32_t k = S.s_ptr[r]; k < S.s_ptr[r + 1]; ++k) {
+      if (S.s_col[k] == r)
+        S.intr_diag_slot = k;
+      else
+        S.intr_slot[S.s_col[k]] = k;
+    }
+    S.n_dense_tail = 1;
+  }
   analyze_rcs(S);
   // pose-graph destinations (S slots)
   auto s_slot = [&](int32_t r, int32_t c) -> int32_t {
@@ -529,15 +587,17 @@ void analyze_rcs(Symbolic& S) {
   S.max_slots_per_row = 0;
   for (int32_t c = 0; c < S.nb; ++c)
     S.max_slots_per_row = std::max(S.max_slots_per_row, S.s_ptr[c + 1] - S.s_ptr[c]);
-  // ordering
-  std::vector<std::vector<int32_t>> adj(S.nb);
-  for (int32_t c = 0; c < S.nb; ++c)
+  // ordering: nested dissection of the blocks before the dense tail, which follows them
+  const int32_t nd = S.nb - S.n_dense_tail;
+  std::vector<std::vector<int32_t>> adj(nd);
+  for (int32_t c = 0; c < nd; ++c)
     for (int32_t k = S.s_ptr[c]; k < S.s_ptr[c + 1]; ++k)
       if (S.s_col[k] != c) {
         adj[c].push_back(S.s_col[k]);
         adj[S.s_col[k]].push_back(c);
       }
-  S.perm = nested_dissection(S.nb, adj, 24);
+  S.perm = nested_dissection(nd, adj, 24);
+  for (int32_t c = nd; c < S.nb; ++c) S.perm.push_back(c);
   S.perm_inv.assign(S.nb, 0);
   for (int32_t i = 0; i < S.nb; ++i) S.perm_inv[S.perm[i]] = i;
   // permuted lower pattern: acol[j] = rows i > j of A(:, j); upper[i] = columns j < i

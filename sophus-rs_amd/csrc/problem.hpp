@@ -34,14 +34,26 @@ struct HostProblem {
   std::vector<int> sorted_of_input;  // caller family index -> sorted index
   int f_pose = -1, f_point = -1, f_intr = -1;
   bool has_reproj = false;
+  // camera model of the reprojection terms (0 pinhole, 1 EUCM) and its parameter count
+  int cam_model = 0, intr_params = 4;
+  // kind of the intrinsics family; with a Free family, its one active member (a calibration
+  // shared by the observations) is the free block intr_block = num_pose_blocks, after the
+  // pose blocks
+  int intr_kind = SGPU_VAR_CONDITIONED;
+  int32_t intr_member = -1, intr_block = -1;
+  int64_t num_pose_blocks = 0;
   int robust = 0;
   double delta = 0;
-  // reprojection terms sorted by (point, pose, intrinsics)
+  // reprojection terms sorted by CostFn::sort's key: (point, pose, intrinsics), or (point,
+  // intrinsics, pose) when the intrinsics family is Free
   std::vector<int32_t> obs_pose, obs_point, obs_intr;
   std::vector<double> obs_uv;
   std::vector<PoseGraphFn> pg;
   int64_t num_free_blocks = 0, num_marg_blocks = 0;
-  std::vector<int32_t> cam_member;  // free block -> pose member
+  std::vector<int32_t> cam_member;  // free block -> pose member (-1: the intrinsics block)
+  // free block -> its index in the reference's layout (Free families in name order, active
+  // members in member order; var_families.rs:60-113)
+  std::vector<int32_t> ref_block;
 };
 
 // Builds and validates; throws sgpu::Error with the reference's error variant.
@@ -52,12 +64,19 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
 constexpr int32_t kOwnChunk = 256;  // camera-major observations per own-term workgroup
 
 struct Symbolic {
-  int32_t nb = 0;  // free camera blocks
+  int32_t nb = 0;  // free blocks of the RCS: the poses', then the intrinsics' (if free)
+  // trailing blocks coupled to (nearly) every other block: kept out of the nested
+  // dissection and eliminated last
+  int32_t n_dense_tail = 0;
   int32_t npt = 0, nobs = 0, npose = 0;
   // point-major observations
   std::vector<int32_t> pt_ptr;      // npt + 1
   std::vector<int32_t> pt_nu;       // reduced reprojection terms per point member
-  std::vector<int32_t> cam_nu;      // reduced reprojection terms per free block
+  std::vector<int32_t> cam_nu;      // reduced terms per free block (the intrinsics' included)
+  // free intrinsics: per pose block, the S slot of (intrinsics, pose) or -1; the slot of
+  // (intrinsics, intrinsics)
+  std::vector<int32_t> intr_slot;
+  int32_t intr_diag_slot = -1;
   std::vector<int32_t> pose_block;  // pose member -> free block or -1
   // camera-major observation lists (free cameras only)
   std::vector<int32_t> cam_ptr, cam_obs;

@@ -143,6 +143,39 @@ extern "C" sgpu_status sgpu_eval_pinhole_terms(int32_t precision, int64_t n, con
   });
 }
 
+extern "C" sgpu_status sgpu_eval_eucm_terms(int64_t n, const double* intr, const double* pose,
+                                            const double* pt, const double* uv, int32_t robust,
+                                            double delta, double* res, double* jintr,
+                                            double* jpose, double* jpt, double* cost,
+                                            double* weight) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!intr || !pose || !pt || !uv)))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (n == 0) return;
+    require_device();
+    Buf bi, bp, bx, bu, br, bq, bj, bk, bc, bw;
+    const double* di = upload(bi, intr, 6 * n);
+    const double* dpo = upload(bp, pose, 7 * n);
+    const double* dx = upload(bx, pt, 3 * n);
+    const double* du = upload(bu, uv, 2 * n);
+    double* dr = br.get<double>(2 * n);
+    double* dq = bq.get<double>(12 * n);
+    double* dj = bj.get<double>(12 * n);
+    double* dk = bk.get<double>(6 * n);
+    double* dc = bc.get<double>(n);
+    double* dw = bw.get<double>(n);
+    launch_eval_eucm(nullptr, n, di, dpo, dx, du, robust, delta, dr, dq, dj, dk, dc, dw);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    if (res) SGPU_HIP(hipMemcpy(res, dr, 2 * n * 8, hipMemcpyDeviceToHost));
+    if (jintr) SGPU_HIP(hipMemcpy(jintr, dq, 12 * n * 8, hipMemcpyDeviceToHost));
+    if (jpose) SGPU_HIP(hipMemcpy(jpose, dj, 12 * n * 8, hipMemcpyDeviceToHost));
+    if (jpt) SGPU_HIP(hipMemcpy(jpt, dk, 6 * n * 8, hipMemcpyDeviceToHost));
+    if (cost) SGPU_HIP(hipMemcpy(cost, dc, n * 8, hipMemcpyDeviceToHost));
+    if (weight) SGPU_HIP(hipMemcpy(weight, dw, n * 8, hipMemcpyDeviceToHost));
+  });
+}
+
 extern "C" sgpu_status sgpu_eval_pose_graph3_terms(int64_t n, const double* m, const double* nn,
                                                    const double* mn, double* res, double* jm,
                                                    double* jn) {

@@ -213,6 +213,8 @@ _SIGNATURES = {
     "sgpu_optimizer_destroy": (None, [C.c_void_p]),
     "sgpu_eval_pinhole_terms": (C.c_int, [C.c_int32, C.c_int64, _DP, _DP, _DP, _DP, C.c_int32, C.c_double,
                                           _DP, _DP, _DP, _DP, _DP]),
+    "sgpu_eval_eucm_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, C.c_int32, C.c_double,
+                                       _DP, _DP, _DP, _DP, _DP, _DP]),
     "sgpu_eval_pose_graph3_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve_noalloc": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
@@ -232,10 +234,12 @@ _SIGNATURES = {
     "sgpu_batch_destroy": (None, [C.c_void_p]),
     "sgpu_generate_ba": (C.c_int, [C.POINTER(_BaSpec), C.POINTER(_BaSizes), _DP, _DP, _DP, _DP, _DP,
                                    _I64P, _DP]),
+    "sgpu_generate_ba_eucm": (C.c_int, [C.c_int32, C.c_int64, C.c_uint64, C.POINTER(_BaSizes), _DP, _DP,
+                                        _DP, _DP, _DP, _DP, _I64P, _DP]),
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 5  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 6  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -345,9 +349,10 @@ class VarType(enum.IntEnum):
     Isometry3 = 0
     Vec3 = 1
     Pinhole = 2
+    Eucm = 3  # EnhancedUnifiedCameraF64: (fx, fy, cx, cy, alpha, beta)
 
 
-_PARAMS = {VarType.Isometry3: 7, VarType.Vec3: 3, VarType.Pinhole: 4}
+_PARAMS = {VarType.Isometry3: 7, VarType.Vec3: 3, VarType.Pinhole: 4, VarType.Eucm: 6}
 
 
 @dataclass
@@ -408,6 +413,7 @@ class RobustKernel:
 class CostKind(enum.IntEnum):
     PinholeReprojection = 0
     PoseGraph3 = 1
+    EucmReprojection = 2
 
 
 @dataclass
@@ -425,8 +431,9 @@ class CostFn:
     robust_kernel: Optional[RobustKernel] = None
 
     def __post_init__(self):
-        arity = 3 if self.kind == CostKind.PinholeReprojection else 2
-        mlen = 2 if self.kind == CostKind.PinholeReprojection else 7
+        reproj = self.kind in (CostKind.PinholeReprojection, CostKind.EucmReprojection)
+        arity = 3 if reproj else 2
+        mlen = 2 if reproj else 7
         self.entity_indices = np.ascontiguousarray(self.entity_indices, dtype=np.int64).reshape(-1, arity)
         self.measurements = np.ascontiguousarray(self.measurements, dtype=np.float64).reshape(-1, mlen)
         if len(self.family_names) != arity:
@@ -436,6 +443,11 @@ class CostFn:
     def pinhole_reprojection(family_names, entity_indices, uv, robust_kernel=None) -> "CostFn":
         """PinholeCameraReprojectionCostTerm terms: (intrinsics, world_from_camera, point)."""
         return CostFn(CostKind.PinholeReprojection, family_names, entity_indices, uv, robust_kernel)
+
+    @staticmethod
+    def eucm_reprojection(family_names, entity_indices, uv, robust_kernel=None) -> "CostFn":
+        """EucmCameraReprojectionCostTerm terms: (intrinsics, world_from_camera, point)."""
+        return CostFn(CostKind.EucmReprojection, family_names, entity_indices, uv, robust_kernel)
 
     @staticmethod
     def pose_graph3(family_names, entity_indices, pose_m_from_pose_n, robust_kernel=None) -> "CostFn":
@@ -644,7 +656,7 @@ class Optimizer:
     def inverse_block(self, row_family: str, row_member: int, col_family: str, col_member: int) -> np.ndarray:
         """Block of the inverse of the final linearization's damped Hessian
         (DampedHessian::inverse_block_with_damping / SchurFactor::inverse_block)."""
-        dims = {n: (6 if f.type == VarType.Isometry3 else 3) for n, f in self._variables.collection.items()}
+        dims = {n: _PARAMS[f.type] - (f.type == VarType.Isometry3) for n, f in self._variables.collection.items()}
         out = np.empty((dims[row_family], dims[col_family]))
         _check(library().sgpu_optimizer_inverse_block(self._h, row_family.encode(), int(row_member),
                                                       col_family.encode(), int(col_member),
@@ -654,7 +666,7 @@ class Optimizer:
     def covariance_block(self, row_family: str, row_member: int, col_family: str, col_member: int) -> np.ndarray:
         """OptimizationSolution::covariance_block: block of H^-1 of the undamped final Hessian
         (full-rank H; a rank-deficient one raises Unsupported)."""
-        dims = {n: (6 if f.type == VarType.Isometry3 else 3) for n, f in self._variables.collection.items()}
+        dims = {n: _PARAMS[f.type] - (f.type == VarType.Isometry3) for n, f in self._variables.collection.items()}
         out = np.empty((dims[row_family], dims[col_family]))
         _check(library().sgpu_optimizer_covariance_block(self._h, row_family.encode(), int(row_member),
                                                          col_family.encode(), int(col_member),
@@ -751,6 +763,24 @@ def eval_pinhole_terms(intrinsics, poses, points, uv, robust_kernel: Optional[Ro
         int(precision), n, intr.ctypes.data_as(_DP), T.ctypes.data_as(_DP), x.ctypes.data_as(_DP),
         z.ctypes.data_as(_DP), 1 if robust_kernel else 0, robust_kernel.delta if robust_kernel else 0.0,
         *[out[k].ctypes.data_as(_DP) for k in ("residual", "jac_pose", "jac_point", "cost", "weight")]))
+    return out
+
+
+def eval_eucm_terms(intrinsics, poses, points, uv, robust_kernel: Optional[RobustKernel] = None) -> dict:
+    """EUCM reprojection terms on the GPU (FP64): residual, J_intr (2x6), J_pose (2x6),
+    J_point (2x3), cost, weight."""
+    intr = np.ascontiguousarray(intrinsics, dtype=np.float64).reshape(-1, 6)
+    T = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 7)
+    x = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    z = np.ascontiguousarray(uv, dtype=np.float64).reshape(-1, 2)
+    n = z.shape[0]
+    out = {"residual": np.empty((n, 2)), "jac_intr": np.empty((n, 2, 6)), "jac_pose": np.empty((n, 2, 6)),
+           "jac_point": np.empty((n, 2, 3)), "cost": np.empty(n), "weight": np.empty(n)}
+    _check(library().sgpu_eval_eucm_terms(
+        n, intr.ctypes.data_as(_DP), T.ctypes.data_as(_DP), x.ctypes.data_as(_DP), z.ctypes.data_as(_DP),
+        1 if robust_kernel else 0, robust_kernel.delta if robust_kernel else 0.0,
+        *[out[k].ctypes.data_as(_DP) for k in ("residual", "jac_intr", "jac_pose", "jac_point", "cost",
+                                                "weight")]))
     return out
 
 
@@ -1103,6 +1133,68 @@ def generate_ba(spec: BaSpec) -> BaProblem:
                                 pi.ctypes.data_as(_DP), xg.ctypes.data_as(_DP), xi.ctypes.data_as(_DP),
                                 idx.ctypes.data_as(_I64P), uv.ctypes.data_as(_DP)))
     return BaProblem(intr.reshape(1, 4), pg, pi, xg, xi, idx, uv)
+
+
+@dataclass
+class EucmBaProblem:
+    """The reference's BaProblem (example_problems/ba_problem.rs:131-412): one shared EUCM camera,
+    a ring of poses (0 and 1 held constant), points marginalized."""
+
+    intrinsics_gt: np.ndarray
+    intrinsics_init: np.ndarray
+    poses_gt: np.ndarray
+    poses_init: np.ndarray
+    points_gt: np.ndarray
+    points_init: np.ndarray
+    obs_idx: np.ndarray
+    obs_uv: np.ndarray
+
+    @property
+    def num_obs(self) -> int:
+        return self.obs_uv.shape[0]
+
+    def variables(self, cams_kind: VarKind = VarKind.Free, intrinsics=None, poses=None,
+                  points=None) -> VarFamilies:
+        """BaProblem::build_variables: 'cams' (the EUCM camera; from the ground truth when
+        Conditioned), 'poses' (Free, 0 and 1 constant), 'points' (Marginalized)."""
+        if intrinsics is None:
+            intrinsics = self.intrinsics_gt if cams_kind == VarKind.Conditioned else self.intrinsics_init
+        return (VarBuilder()
+                .add_family("cams", VarFamily(cams_kind, VarType.Eucm, intrinsics))
+                .add_family("poses", VarFamily(VarKind.Free, VarType.Isometry3,
+                                               self.poses_init if poses is None else poses, [0, 1]))
+                .add_family("points", VarFamily(VarKind.Marginalized, VarType.Vec3,
+                                                self.points_init if points is None else points))
+                .build())
+
+    def cost_fns(self, huber_delta: Optional[float] = None) -> List[CostFn]:
+        rk = RobustKernel.huber(huber_delta) if huber_delta is not None else None
+        return [CostFn.eucm_reprojection(["cams", "poses", "points"], self.obs_idx, self.obs_uv, rk)]
+
+    def gt_errors(self, variables: VarFamilies):
+        """BaProblem::gt_errors: RMS pose translation error and RMS point error."""
+        P = variables.get_members("poses")
+        X = variables.get_members("points")
+        pose_rms = float(np.sqrt(np.mean(np.sum((P[:, 4:] - self.poses_gt[:, 4:]) ** 2, axis=1))))
+        pt_rms = float(np.sqrt(np.mean(np.sum((X - self.points_gt) ** 2, axis=1))))
+        return pose_rms, pt_rms
+
+
+def generate_ba_eucm(num_cameras: int, num_points: int, seed: int = 0) -> EucmBaProblem:
+    """The reference BaProblem's geometry with the library's seeded RNG (host code)."""
+    lib = library()
+    sizes = _BaSizes()
+    _check(lib.sgpu_generate_ba_eucm(num_cameras, num_points, seed, C.byref(sizes), *([None] * 8)))
+    nc, npt, no = sizes.num_cameras, sizes.num_points, sizes.num_obs
+    ig, ii = np.empty(6), np.empty(6)
+    pg, pi = np.empty((nc, 7)), np.empty((nc, 7))
+    xg, xi = np.empty((npt, 3)), np.empty((npt, 3))
+    idx = np.empty((no, 3), dtype=np.int64)
+    uv = np.empty((no, 2))
+    _check(lib.sgpu_generate_ba_eucm(num_cameras, num_points, seed, C.byref(sizes),
+                                     *[a.ctypes.data_as(_DP) for a in (ig, ii, pg, pi, xg, xi)],
+                                     idx.ctypes.data_as(_I64P), uv.ctypes.data_as(_DP)))
+    return EucmBaProblem(ig.reshape(1, 6), ii.reshape(1, 6), pg, pi, xg, xi, idx, uv)
 
 
 def config_spec(index: int, seed: int = 0) -> BaSpec:
