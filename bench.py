@@ -58,11 +58,12 @@ def host_threads() -> int:
     return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
 
 
-def oracle_step_seconds(oracle, problem, threads: int, iterations: int = 1) -> float:
+def oracle_step_seconds(oracle, problem, threads: int, iterations: int = 1, families=None,
+                        costs=None) -> float:
     """Wall time inside the LM steps of `iterations` FP64-oracle iterations at `threads`."""
     oracle.set_threads(threads)
     try:
-        oracle.optimize_nlls(oracle.ba_families(problem), oracle.ba_costs(problem),
+        oracle.optimize_nlls(families or oracle.ba_families(problem), costs or oracle.ba_costs(problem),
                              oracle.default_params(num_iterations=iterations, initial_lm_damping=1.0))
         return oracle.last_step_seconds()
     finally:
@@ -296,6 +297,61 @@ def small_workload_leg(sg, index: int, cpu: bool) -> dict:
     return out
 
 
+def eucm_leg(sg, cpu: bool) -> dict:
+    """The reference's own BaProblem (one shared EUCM camera, a ring of cameras around an open
+    box, poses 0 and 1 constant) on the FP64 direct path, the harness of its CPU bench
+    (ba_bench.rs:213-251): 20 LM iterations from nu = 1, the camera Free and Conditioned, one
+    warm-up and the median of 3 solves.  10 cameras / 500 points is the reference bench's size;
+    120 cameras / 12k candidate points (605k observations, 50 per point) shows the free
+    calibration block at scale.  The CPU
+    oracle runs the same 20 iterations (the larger problem: one) at nproc and on one core."""
+    out = {}
+    for nc, npt in ((10, 500), (120, 12000)):
+        p = sg.generate_ba_eucm(nc, npt, seed=0)
+        for kind in (sg.VarKind.Free, sg.VarKind.Conditioned):
+            times, hist = [], None
+            for rep in range(4):
+                params = sg.OptParams(num_iterations=20, initial_lm_damping=1.0)
+                opt = sg.Optimizer(p.variables(kind), p.cost_fns(), params)
+                h = []
+                t0 = time.perf_counter()
+                while True:
+                    info = opt.step()
+                    h.append(info.merit)
+                    if info.termination is not None:
+                        break
+                dt = time.perf_counter() - t0
+                v = opt.variables()
+                opt.close()
+                if rep:
+                    times.append(dt)
+                hist = h
+            pose_rms, pt_rms = p.gt_errors(v)
+            key = f"eucm_ring_{nc}cams_{p.points_gt.shape[0]}pts_{'free' if kind == sg.VarKind.Free else 'const'}_cams"
+            row = {"num_obs": p.num_obs, "iterations": len(hist), "final_cost": hist[-1],
+                   "gpu_ms_per_solve": 1e3 * float(np.median(times)),
+                   "gpu_ms_per_iteration": 1e3 * float(np.median(times)) / len(hist),
+                   "gpu_obs_per_s": p.num_obs * len(hist) / float(np.median(times)),
+                   "pose_rms": pose_rms, "point_rms": pt_rms}
+            if cpu:
+                sys.path.insert(0, str(ROOT / "oracle"))
+                import oracle
+                ck = 0 if kind == sg.VarKind.Free else 1
+                fams = [("cams", 3, ck, p.intrinsics_init if ck == 0 else p.intrinsics_gt, []),
+                        ("poses", 0, 0, p.poses_init, [0, 1]), ("points", 1, 2, p.points_init, [])]
+                costs = [(2, ["cams", "poses", "points"], p.obs_idx, p.obs_uv, None)]
+                it_cpu = 20 if nc == 10 else 1
+                row["cpu_oracle"] = []
+                for threads in sorted({host_threads(), 1}, reverse=True):
+                    ct = [oracle_step_seconds(oracle, p, threads, it_cpu, fams, costs)
+                          for _ in range(4 if nc == 10 else 1)]
+                    ct = ct[1:] if nc == 10 else ct
+                    row["cpu_oracle"].append({"threads": threads, "iterations": it_cpu,
+                                              "ms_per_iteration": 1e3 * float(np.median(ct)) / it_cpu})
+            out[key] = row
+    return out
+
+
 def batch_leg(sg, n_windows: int, iters: int = 20) -> dict:
     """The batched workload: n_windows independent ring windows (5-30 cameras, 100-2000
     points, 2-8 cameras per point, seeded by window id), 20 LM iterations each, FP64 and
@@ -346,6 +402,7 @@ def main() -> int:
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-window leg")
     ap.add_argument("--no-small", action="store_true", help="skip the 10- and 500-camera workloads")
+    ap.add_argument("--no-eucm", action="store_true", help="skip the EUCM BaProblem leg")
     ap.add_argument("--batch-windows", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -437,6 +494,12 @@ def main() -> int:
             small[f"config{idx}"] = small_workload_leg(sg, idx, cpu=not args.no_cpu)
             log(f"[bench] workload {idx}: {small[f'config{idx}']}")
 
+    # ---- the reference's BaProblem: EUCM camera, free shared intrinsics -----------------------
+    eucm = None
+    if not args.no_eucm and args.config == 2:
+        eucm = eucm_leg(sg, cpu=not args.no_cpu)
+        log(f"[bench] eucm {json.dumps(eucm)}")
+
     # ---- batched small windows: problems/s and per-window latency -----------------------------
     batch = None
     if not args.no_batch:
@@ -472,6 +535,7 @@ def main() -> int:
         "phase_bandwidth": bandwidth,
         "time_to_tol": tol,
         "workloads": small,
+        "eucm_ba_problem": eucm,
         "batch": batch,
         "rcs": st,
     }

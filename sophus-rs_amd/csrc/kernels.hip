@@ -2220,73 +2220,98 @@ __device__ __forceinline__ void block_sum_n(double (&v)[N], double (*red)[N], do
 
 // Per point: W_b and V_b (pt_vw: V row-major 6x3, then W), and the point's share of S_cc,
 // rhs_c and neg_grad_c summed over the workgroup into intr_part[kIntrPart * block + e].
+// G lanes per point (G divides 64) stride over its observations; their sums are combined by
+// a butterfly over the G lanes, after which every lane of the group holds the same totals
+// (each addition's operands are the same pair on both lanes), and the group's first lane
+// finishes the point.
+template <int G>
 __global__ void __launch_bounds__(kThreads)
     k_intr_points(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
                   const double* __restrict__ cw, const double* __restrict__ prec,
                   const double* __restrict__ pt_s, double* __restrict__ pt_vw,
                   double* __restrict__ intr_part) {
   __shared__ double red[kThreads / 64][kIntrPart];
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int kSum = 18 + 21 + 6;  // W, sum E^T E (upper), sum E^T r~
+  const int groups = blockDim.x / G, sub = threadIdx.x % G;
   double acc[kIntrPart];
 #pragma unroll
   for (int e = 0; e < kIntrPart; ++e) acc[e] = 0.0;
-  if (b < p.npt) {
-    const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
-    const double* in = intr + 6 * int64_t(p.intr_member);
-    double W[18];
+  // the loop bound is uniform over the workgroup, so every lane of a group shuffles together
+  for (int b0 = blockIdx.x * groups; b0 < p.npt; b0 += gridDim.x * groups) {
+    const int b = b0 + int(threadIdx.x) / G;
+    const bool live = b < p.npt;
+    double t[kSum];
 #pragma unroll
-    for (int e = 0; e < 18; ++e) W[e] = 0.0;
-    for (int o = p.pt_ptr[b]; o < p.pt_ptr[b + 1]; ++o) {
-      if (p.obs_intr[o] != p.intr_member) continue;
-      const double* c = cw + 12 * p.obs_pose[o];
-      double xc[3], px[2], D[6], E[12], A[6];
-      to_camera(c, x, xc);
-      eucm_project(in, xc, px, D, E);
-      lift_rotation(c, D, A);
-      const double r0 = p.obs_uv[2 * o] - px[0], r1 = p.obs_uv[2 * o + 1] - px[1];
-      const double w = robust_weight(p.robust, p.delta, sqrt(r0 * r0 + r1 * r1));
-      const double rw0 = p.robust == 1 ? w * r0 : r0, rw1 = p.robust == 1 ? w * r1 : r1;
+    for (int e = 0; e < kSum; ++e) t[e] = 0.0;
+    double x[3] = {0.0, 0.0, 0.0};
+    if (live) {
+      x[0] = points[3 * b];
+      x[1] = points[3 * b + 1];
+      x[2] = points[3 * b + 2];
+      const double* in = intr + 6 * int64_t(p.intr_member);
+      for (int o = p.pt_ptr[b] + sub; o < p.pt_ptr[b + 1]; o += G) {
+        if (p.obs_intr[o] != p.intr_member) continue;
+        const double* c = cw + 12 * p.obs_pose[o];
+        double xc[3], px[2], D[6], E[12], A[6];
+        to_camera(c, x, xc);
+        eucm_project(in, xc, px, D, E);
+        lift_rotation(c, D, A);
+        const double r0 = p.obs_uv[2 * o] - px[0], r1 = p.obs_uv[2 * o + 1] - px[1];
+        const double w = robust_weight(p.robust, p.delta, sqrt(r0 * r0 + r1 * r1));
+        const double rw0 = p.robust == 1 ? w * r0 : r0, rw1 = p.robust == 1 ? w * r1 : r1;
+        int u = 18;
+#pragma unroll
+        for (int a = 0; a < 6; ++a) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) t[3 * a + j] += E[a] * A[j] + E[6 + a] * A[3 + j];
+#pragma unroll
+          for (int bb = a; bb < 6; ++bb) t[u++] += E[a] * E[bb] + E[6 + a] * E[6 + bb];
+          t[39 + a] += E[a] * rw0 + E[6 + a] * rw1;
+        }
+      }
+    }
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1)
+#pragma unroll
+      for (int e = 0; e < kSum; ++e) t[e] += __shfl_xor(t[e], off, 64);
+    if (live && sub == 0) {
+      const double* W = t;
+#pragma unroll
+      for (int e = 0; e < 21; ++e) acc[e] += t[18 + e];
 #pragma unroll
       for (int a = 0; a < 6; ++a) {
-#pragma unroll
-        for (int j = 0; j < 3; ++j) W[3 * a + j] += E[a] * A[j] + E[6 + a] * A[3 + j];
-        const double g = E[a] * rw0 + E[6 + a] * rw1;
-        acc[21 + a] += g;
-        acc[27 + a] += g;
+        acc[21 + a] += t[39 + a];
+        acc[27 + a] += t[39 + a];
       }
-      int u = 0;
+      double* out = pt_vw + 36 * int64_t(b);
+      if (p.pt_block[b] >= 0) {
+        const double* h = prec + kPtRec * int64_t(b) + 3;
+        const double Hi[9] = {h[0], h[1], h[2], h[1], h[3], h[4], h[2], h[4], h[5]};
+        double V[18];
 #pragma unroll
-      for (int a = 0; a < 6; ++a)
+        for (int a = 0; a < 6; ++a)
 #pragma unroll
-        for (int bb = a; bb < 6; ++bb) acc[u++] += E[a] * E[bb] + E[6 + a] * E[6 + bb];
-    }
-    double* out = pt_vw + 36 * int64_t(b);
-    if (p.pt_block[b] >= 0) {
-      const double* h = prec + kPtRec * int64_t(b) + 3;
-      const double Hi[9] = {h[0], h[1], h[2], h[1], h[3], h[4], h[2], h[4], h[5]};
-      double V[18];
+          for (int j = 0; j < 3; ++j)
+            V[3 * a + j] = W[3 * a] * Hi[j] + W[3 * a + 1] * Hi[3 + j] + W[3 * a + 2] * Hi[6 + j];
+        int u = 0;
 #pragma unroll
-      for (int a = 0; a < 6; ++a)
+        for (int a = 0; a < 6; ++a)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-          V[3 * a + j] = W[3 * a] * Hi[j] + W[3 * a + 1] * Hi[3 + j] + W[3 * a + 2] * Hi[6 + j];
-      int u = 0;
+          for (int bb = a; bb < 6; ++bb)
+            acc[u++] -= V[3 * a] * W[3 * bb] + V[3 * a + 1] * W[3 * bb + 1] + V[3 * a + 2] * W[3 * bb + 2];
+        const double sb[3] = {pt_s[3 * b], pt_s[3 * b + 1], pt_s[3 * b + 2]};
 #pragma unroll
-      for (int a = 0; a < 6; ++a)
+        for (int a = 0; a < 6; ++a)
+          acc[21 + a] -= V[3 * a] * sb[0] + V[3 * a + 1] * sb[1] + V[3 * a + 2] * sb[2];
 #pragma unroll
-        for (int bb = a; bb < 6; ++bb)
-          acc[u++] -= V[3 * a] * W[3 * bb] + V[3 * a + 1] * W[3 * bb + 1] + V[3 * a + 2] * W[3 * bb + 2];
-      const double sb[3] = {pt_s[3 * b], pt_s[3 * b + 1], pt_s[3 * b + 2]};
+        for (int e = 0; e < 18; ++e) {
+          out[e] = V[e];
+          out[18 + e] = W[e];
+        }
+      } else {
 #pragma unroll
-      for (int a = 0; a < 6; ++a) acc[21 + a] -= V[3 * a] * sb[0] + V[3 * a + 1] * sb[1] + V[3 * a + 2] * sb[2];
-#pragma unroll
-      for (int e = 0; e < 18; ++e) {
-        out[e] = V[e];
-        out[18 + e] = W[e];
+        for (int e = 0; e < 36; ++e) out[e] = 0.0;
       }
-    } else {
-#pragma unroll
-      for (int e = 0; e < 36; ++e) out[e] = 0.0;
     }
   }
   double total = 0.0;
@@ -2294,24 +2319,24 @@ __global__ void __launch_bounds__(kThreads)
   if (int(threadIdx.x) < kIntrPart) intr_part[kIntrPart * int64_t(blockIdx.x) + threadIdx.x] = total;
 }
 
-// One workgroup per free pose camera: its S block (intrinsics, camera), stored into the slot
-// (k_schur_blocks leaves it zero).
-__global__ void __launch_bounds__(kThreads)
-    k_intr_cams(DevProblem p, const double* __restrict__ points, const double* __restrict__ intr,
-                const double* __restrict__ cw, const double* __restrict__ pt_vw,
-                double* __restrict__ front, double* __restrict__ bsr) {
-  __shared__ double red[kThreads / 64][36];
-  const int cam = blockIdx.x;
-  const int slot = p.intr_slot[cam];
-  if (slot < 0) return;  // the whole workgroup leaves together
+// The S block (intrinsics, camera) per chunk of at most kOwnChunk camera-major observations
+// (the own-term chunks), one wavefront per chunk, lanes striding over its observations:
+// chunk sums into cpart[36 * chunk + e] (a butterfly leaves every lane with the totals).
+__global__ void __launch_bounds__(64)
+    k_intr_cam_chunks(DevProblem p, const double* __restrict__ points,
+                      const double* __restrict__ intr, const double* __restrict__ cw,
+                      const double* __restrict__ pt_vw, double* __restrict__ cpart) {
+  const int ch = blockIdx.x, lane = threadIdx.x;
+  const int cam = p.own_chunks[3 * ch], q0 = p.own_chunks[3 * ch + 1], q1 = p.own_chunks[3 * ch + 2];
   double acc[36];
 #pragma unroll
   for (int e = 0; e < 36; ++e) acc[e] = 0.0;
-  const double* c = cw + 12 * p.cam_member[cam];
-  const double* in = intr + 6 * int64_t(p.intr_member);
-  for (int q = p.cam_ptr[cam] + threadIdx.x; q < p.cam_ptr[cam + 1]; q += blockDim.x) {
+  const bool any = p.intr_slot[cam] >= 0;
+  for (int q = q0 + lane; any && q < q1; q += 64) {
     const int o = p.cam_obs[q];
     if (p.obs_intr[o] != p.intr_member) continue;
+    const double* c = cw + 12 * p.cam_member[cam];
+    const double* in = intr + 6 * int64_t(p.intr_member);
     const int b = p.obs_point[o];
     const double x[3] = {points[3 * b], points[3 * b + 1], points[3 * b + 2]};
     double xc[3], px[2], D[6], E[12], A[6];
@@ -2348,16 +2373,34 @@ __global__ void __launch_bounds__(kThreads)
       acc[6 * a + 5] += g2;
     }
   }
-  double total = 0.0;
-  block_sum_n<36>(acc, red, &total);
-  if (threadIdx.x < 36) {
-    const int a = threadIdx.x / 6, bb = threadIdx.x % 6;
-    if (bsr) {
-      bsr[36 * int64_t(slot) + threadIdx.x] = total;
-    } else {
-      const int ldc = p.s_ld[slot], ld = ldc >> 1;
-      front[p.s_off[slot] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = total;
-    }
+  double mine = 0.0;
+#pragma unroll
+  for (int e = 0; e < 36; ++e) {
+    double v = acc[e];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) v += __shfl_xor(v, off, 64);
+    if (lane == e) mine = v;
+  }
+  if (lane < 36) cpart[36 * int64_t(ch) + lane] = mine;
+}
+
+// One wavefront per pose block: its chunks' sums in chunk order, stored into the slot
+// (intrinsics, camera) (k_schur_blocks leaves it zero).
+__global__ void __launch_bounds__(kThreads)
+    k_intr_cam_store(DevProblem p, int npb, const double* __restrict__ cpart,
+                     double* __restrict__ front, double* __restrict__ bsr) {
+  const int cam = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (cam >= npb || lane >= 36) return;
+  const int slot = p.intr_slot[cam];
+  if (slot < 0) return;
+  double v = 0.0;
+  for (int ch = p.cam_chunk_ptr[cam]; ch < p.cam_chunk_ptr[cam + 1]; ++ch) v += cpart[36 * int64_t(ch) + lane];
+  const int a = lane / 6, bb = lane % 6;
+  if (bsr) {
+    bsr[36 * int64_t(slot) + lane] = v;
+  } else {
+    const int ldc = p.s_ld[slot], ld = ldc >> 1;
+    front[p.s_off[slot] + ((ldc & 1) ? a * ld + bb : bb * ld + a)] = v;
   }
 }
 
@@ -2958,20 +3001,41 @@ void launch_point_coupling(hipStream_t s, const DevProblem& p, const double* poi
                      dim3(1), dim3(64), 0, s, p, points, intr, cw, prec, b, out, cam, hinv);
 }
 
+// Lanes per point of k_intr_points from the mean track length.
+static int intr_group(const DevProblem& p) {
+  const int64_t mean = p.npt > 0 ? int64_t(p.nobs) / p.npt : 1;
+  return mean >= 24 ? 16 : mean >= 6 ? 4 : 1;
+}
+
+// at most 256 workgroups, each striding over groups of points (fewer partials to sum)
+static int intr_point_blocks(const DevProblem& p) {
+  return std::max(1, std::min(blocks_for(int64_t(p.npt) * intr_group(p), kThreads), 256));
+}
+
 void launch_intrinsics_rows(hipStream_t s, const DevProblem& p, const double* points,
                             const double* intr, const double* cw, DevLin& w, DevFactor& f) {
   if (p.intr_block < 0) return;
-  const int gpt = std::max(1, blocks_for(p.npt, kThreads));
-  hipLaunchKernelGGL(k_intr_points, dim3(gpt), dim3(kThreads), 0, s, p, points, intr, cw, w.prec,
-                     w.pt_s, w.pt_vw, w.intr_part);
-  if (p.intr_block > 0)
-    hipLaunchKernelGGL(k_intr_cams, dim3(p.intr_block), dim3(kThreads), 0, s, p, points, intr, cw,
-                       w.pt_vw, f.front, w.bsr);
+  const int gpt = intr_point_blocks(p);
+  switch (intr_group(p)) {
+#define SGPU_INTR_POINTS(G)                                                                   \
+  case G:                                                                                     \
+    hipLaunchKernelGGL(k_intr_points<G>, dim3(gpt), dim3(kThreads), 0, s, p, points, intr, cw, \
+                       w.prec, w.pt_s, w.pt_vw, w.intr_part);                                 \
+    break;
+    SGPU_INTR_POINTS(1) SGPU_INTR_POINTS(4) SGPU_INTR_POINTS(16)
+#undef SGPU_INTR_POINTS
+  }
+  if (p.n_own_chunks > 0) {
+    hipLaunchKernelGGL(k_intr_cam_chunks, dim3(p.n_own_chunks), dim3(64), 0, s, p, points, intr,
+                       cw, w.pt_vw, w.intr_cpart);
+    hipLaunchKernelGGL(k_intr_cam_store, dim3(blocks_for(int64_t(p.intr_block) * 64, kThreads)),
+                       dim3(kThreads), 0, s, p, p.intr_block, w.intr_cpart, f.front, w.bsr);
+  }
   hipLaunchKernelGGL(k_intr_finish, dim3(1), dim3(kThreads), 0, s, p, w.intr_part, gpt, f.front,
                      w.bsr, w.rhs_nat, w.grad_cam);
 }
 
-int intrinsics_partials(int npt) { return kIntrPart * std::max(1, blocks_for(npt, kThreads)); }
+int intrinsics_partials(const DevProblem& p) { return kIntrPart * intr_point_blocks(p); }
 
 // Diagonal of the assembled S (camera order) from the diagonal slots of the fronts: the
 // diagonal block of row c is its last slot (columns ascending, c the largest).

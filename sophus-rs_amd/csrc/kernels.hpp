@@ -124,6 +124,7 @@ struct DevLin {
   // W_b = H_cl (6x3); per point-pass workgroup the partial sums of the intrinsics rows
   double* pt_vw;
   double* intr_part;
+  double* intr_cpart;  // 36 per own-term chunk: the chunk's share of its (intrinsics, camera) block
 };
 
 void launch_cam_cache(hipStream_t s, int npose, const double* poses, double* cw);
@@ -168,8 +169,8 @@ void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses
 // neg-gradient.  After launch_camera_rows (which leaves those slots zero but the damping).
 void launch_intrinsics_rows(hipStream_t s, const DevProblem& p, const double* points,
                             const double* intr, const double* cw, DevLin& w, DevFactor& f);
-// doubles of DevLin::intr_part for npt points
-int intrinsics_partials(int npt);
+// doubles of DevLin::intr_part for the problem
+int intrinsics_partials(const DevProblem& p);
 void launch_cost(hipStream_t s, const DevProblem& p, const double* poses, const double* points,
                  const double* intr, double* partial, int32_t partial_cap, DevScalars* sc);
 void launch_finalize_step(hipStream_t s, const DevProblem& p, const DevLin& w, DevScalars* sc);

@@ -383,9 +383,11 @@ struct sgpu_optimizer {
     }
     dl.pt_vw = nullptr;
     dl.intr_part = nullptr;
+    dl.intr_cpart = nullptr;
     if (P.intr_block >= 0) {
       dl.pt_vw = arena.alloc<double>(size_t(36) * std::max<int32_t>(S.npt, 1));
-      dl.intr_part = arena.alloc<double>(size_t(intrinsics_partials(S.npt)));
+      dl.intr_part = arena.alloc<double>(size_t(intrinsics_partials(dp)));
+      dl.intr_cpart = arena.alloc<double>(size_t(36) * std::max<size_t>(S.own_chunks.size() / 3, 1));
     }
     dl.partial_cap = 4 * kCostBlocks + 2;
     dl.partial = arena.alloc<double>(dl.partial_cap);

@@ -179,3 +179,40 @@ def test_eucm_free_intrinsics_covariance_blocks_match_dense_inverse(sg, orc):
         print(f"{rf}[{rm}] x {cf}[{cm}]: rel {rel:.3g}")
         assert rel < 1e-8, (rf, rm, cf, cm, rel)
     opt.close()
+
+
+def test_ba_problem_inverse_blocks_match_dense_pseudo_inverse(sg, orc):
+    """The reference's schur_inverse_block_matches_dense_pseudo_inverse
+    (crates/sophus_opt/src/nlls/damped_hessian.rs:610-700): BaProblem::new(4, 20) at its
+    initial variables, the damped Hessian at nu = 1 (skip_final_hessian keeps that nu), and
+    every free x free block (the shared camera and the free poses), every marginal diagonal
+    block and every free x marginal block against the dense pseudo-inverse of H + nu N.
+    The reference's bar is 1e-6 absolute; here 1e-9 relative to the block's largest entry."""
+    p = sg.generate_ba_eucm(4, 20, seed=0)
+    opt = sg.Optimizer(p.variables(), p.cost_fns(),
+                       sg.OptParams(num_iterations=1, initial_lm_damping=1.0, skip_final_hessian=1))
+    H, _ = orc.linearize_dense(_oracle_fams(p), _oracle_costs(p), 1.0)
+    Hinv = np.linalg.pinv(H, rcond=1e-14)
+    free = [("cams", 0)] + [("poses", m) for m in range(2, p.poses_init.shape[0])]
+    marg = [("points", m) for m in range(p.points_init.shape[0])]
+    start = {}
+    off = 0
+    for name, m in free:
+        start[(name, m)] = (off, 6)
+        off += 6
+    for name, m in marg:
+        start[(name, m)] = (off, 3)
+        off += 3
+    assert off == H.shape[0]
+    pairs = [(a, b) for a in free for b in free] + [(a, a) for a in marg] + \
+            [(a, b) for a in free for b in marg] + [(b, a) for a in free for b in marg[::5]]
+    worst = 0.0
+    for (rf, rm), (cf, cm) in pairs:
+        blk = opt.inverse_block(rf, rm, cf, cm)
+        (r0, dr), (c0, dc) = start[(rf, rm)], start[(cf, cm)]
+        ref = Hinv[r0:r0 + dr, c0:c0 + dc]
+        assert np.max(np.abs(blk - ref)) < 1e-6
+        worst = max(worst, np.max(np.abs(blk - ref)) / np.max(np.abs(ref)))
+    print(f"{len(pairs)} blocks, worst relative {worst:.3g}")
+    assert worst < 1e-9
+    opt.close()
