@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 6
+#define SGPU_ABI_VERSION 7
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -62,7 +62,9 @@ typedef enum sgpu_status {
   /* a non-finite residual or pivot (LdltDecompositionError::NonFinitePivot) */
   SGPU_ERR_NON_FINITE = 11,
   /* the handle has already terminated */
-  SGPU_ERR_TERMINATED = 12
+  SGPU_ERR_TERMINATED = 12,
+  /* NllsError::SchurConstraintOnMarginalizedFamily (optimizer.rs:664-678) */
+  SGPU_ERR_SCHUR_CONSTRAINT_ON_MARGINALIZED_FAMILY = 13
 } sgpu_status;
 
 /* VarKind (crates/sophus_opt/src/variables.rs:17-24) */
@@ -96,6 +98,13 @@ typedef enum sgpu_cost_kind {
    * observations: its block couples to every camera, so it is ordered last in the RCS) */
   SGPU_COST_EUCM_REPROJECTION = 2
 } sgpu_cost_kind;
+
+/* Equality-constraint kinds (closed set instead of HasEqConstraintResidualFn objects). */
+typedef enum sgpu_eq_kind {
+  /* TranslationNormConstraint: c(T) = |t(T)| - r on one SE(3) argument, residual dim 1,
+   * one parameter r per term.  example_problems/ba_scale_constraint.rs:44-99 */
+  SGPU_EQ_TRANSLATION_NORM = 0
+} sgpu_eq_kind;
 
 typedef enum sgpu_robust_kernel {
   SGPU_ROBUST_NONE = 0,
@@ -154,6 +163,21 @@ typedef struct sgpu_cost_fn {
   const int64_t* entity_indices;
   const double* measurements;
 } sgpu_cost_fn;
+
+/* EqConstraintFn::new_boxed((), EqConstraints::new(family_names, terms))
+ * (crates/sophus_opt/src/nlls/constraint/eq_constraint_fn.rs:61-140).  All terms of one set
+ * share one constraint block and one Lagrange multiplier block, as in the reference's
+ * EqSystem (one partition of one block per set, eq_system.rs:33-37, 207-262): their
+ * residuals and Jacobians add.  entity_indices: num_terms x arity; params: num_terms x
+ * (1: the radius r). */
+typedef struct sgpu_eq_fn {
+  int32_t kind; /* sgpu_eq_kind */
+  int32_t reserved0;
+  const char* family_names[1];
+  int64_t num_terms;
+  const int64_t* entity_indices;
+  const double* params;
+} sgpu_eq_fn;
 
 /* OptParams (optimizer_types.rs:161-189) + build-only fields. */
 typedef struct sgpu_opt_params {
@@ -242,6 +266,31 @@ sgpu_status sgpu_optimize_nlls(const sgpu_var_family* families, int64_t num_fami
 sgpu_status sgpu_optimizer_create(const sgpu_var_family* families, int64_t num_families,
                                   const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                                   const sgpu_opt_params* params, sgpu_optimizer** out);
+
+/* optimize_nlls(variables, cost_fns, eq_fns, params) (nlls.rs:73-80) and Optimizer::new with
+ * equality constraints (optimizer.rs:71-120): the constrained LM step of step_nlls_inner
+ * (optimizer_step.rs:366-443) with the range-space KKT solve of SchurFactor
+ * (schur_ldlt.rs:450-480, 674-730, 762-772): G^T lambda in the free rows of the right-hand
+ * side, -c in the constraint rows (eq_system.rs:162-262); Y = S^-1 G^T, M = G Y,
+ * dlambda = M^-1 (G u + c), dx_f = u - Y dlambda with u = S^-1 rhs; lambda <- lambda - dlambda
+ * with an accepted step (update_lambdas, eq_system.rs:149-159); the gain ratio over the whole
+ * step, multiplier rows included; no gradient-tolerance test (optimizer.rs:527-535).  The
+ * merit is the cost.  Constraint sets must name the pose family (Free; a Marginalized one is
+ * SGPU_ERR_SCHUR_CONSTRAINT_ON_MARGINALIZED_FAMILY, optimizer.rs:664-678); the direct solver
+ * in FP64 only; at most 16 sets.  A singular M rejects the step (SingularKktConstraint). */
+sgpu_status sgpu_optimizer_create_eq(const sgpu_var_family* families, int64_t num_families,
+                                     const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                     const sgpu_eq_fn* eq_fns, int64_t num_eq_fns,
+                                     const sgpu_opt_params* params, sgpu_optimizer** out);
+sgpu_status sgpu_optimize_nlls_eq(const sgpu_var_family* families, int64_t num_families,
+                                  const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                  const sgpu_eq_fn* eq_fns, int64_t num_eq_fns,
+                                  const sgpu_opt_params* params, double* const* out_members,
+                                  sgpu_solution* solution);
+/* The multipliers (EqSystem::lambda) of the current variables and the constraint residuals
+ * c at them, one per non-empty set in the order given. */
+sgpu_status sgpu_optimizer_eq_state(sgpu_optimizer* opt, double* lambda, double* residual,
+                                    int64_t capacity);
 
 /* Optimizer::step (optimizer.rs:345-377 + optimizer_step.rs:366-443). */
 sgpu_status sgpu_optimizer_step(sgpu_optimizer* opt, sgpu_step_info* info);

@@ -320,10 +320,68 @@ void eval_pose_graph(const double* m, const double* n, const double* mn, double 
   }
 }
 
+// An equality-constraint set (EqConstraintFn, eq_constraint_fn.rs:61-140): terms of one kind;
+// all of them share one constraint block (EqSystem::new gives each set one partition of one
+// block, eq_system.rs:33-37, and populate_lower_triangular_kkt_mat adds every term into it,
+// :207-262).
+struct EqFn {
+  int kind = SGPU_EQ_TRANSLATION_NORM;
+  int fam = -1;
+  std::vector<int64_t> idx;       // sorted terms (arity 1)
+  std::vector<double> radius;     // one parameter per term
+};
+
 struct Problem {
   Vars vars;
   std::vector<CostFn> costs;
+  std::vector<EqFn> eqs;
 };
+
+// TranslationNormConstraint (ba_scale_constraint.rs:44-99): c = |t(T)| - r, and its Jacobian
+// by forward-mode duals of c(exp(x) T) at x = 0 (the reference's own derivative).
+void eval_translation_norm(const double* pose, double r, double* c, double J[6]) {
+  using D = Dual<6>;
+  D x[6], E[7], T[7], Tx[7];
+  for (int k = 0; k < 6; ++k) x[k] = D::var(0.0, k);
+  for (int k = 0; k < 7; ++k) T[k] = D(pose[k]);
+  se3_exp(x, E);
+  se3_mul(E, T, Tx);
+  const D n = sqrt(Tx[4] * Tx[4] + Tx[5] * Tx[5] + Tx[6] * Tx[6]) - D(r);
+  *c = n.re;
+  for (int k = 0; k < 6; ++k) J[k] = n.d[k];
+}
+
+// The KKT rows of the equality constraints at V (EqSystem, eq_system.rs:162-262): G (one
+// dense row per set over the free scalars), -c per set, and G^T lambda added to the free
+// rows of the neg-gradient.
+struct Kkt {
+  int64_t nc = 0;
+  std::vector<double> G;      // nc x nfs row-major
+  std::vector<double> rhs_c;  // -c
+};
+
+void eval_kkt(const Problem& P, const Vars& V, const std::vector<double>& lambda, Kkt& K,
+              std::vector<double>* neg_grad) {
+  const int64_t nfs = V.num_free_scalars;
+  K.nc = int64_t(P.eqs.size());
+  K.G.assign(size_t(K.nc * nfs), 0.0);
+  K.rhs_c.assign(size_t(K.nc), 0.0);
+  for (int64_t k = 0; k < K.nc; ++k) {
+    const EqFn& e = P.eqs[k];
+    const Family& f = V.fam[e.fam];
+    for (size_t t = 0; t < e.idx.size(); ++t) {
+      double c, J[6];
+      eval_translation_norm(&f.members[7 * e.idx[t]], e.radius[t], &c, J);
+      K.rhs_c[k] -= c;
+      const int64_t s0 = f.kind == SGPU_VAR_CONDITIONED ? -1 : f.scalar_start[e.idx[t]];
+      if (s0 < 0) continue;
+      for (int i = 0; i < 6; ++i) {
+        K.G[size_t(k * nfs + s0 + i)] += J[i];
+        if (neg_grad) (*neg_grad)[s0 + i] += J[i] * lambda[k];
+      }
+    }
+  }
+}
 
 // Evaluate one term.  `derivs` false = all args Conditioned (calc_total_cost).
 void eval_term(const Problem& P, const Vars& V, const CostFn& c, int64_t t, bool derivs,
@@ -1004,7 +1062,7 @@ bool inverse_dense(int n, const double* m, double* out) {
 // H: all active blocks; the first nfb blocks are free.  g: full rhs (neg_grad).
 // Returns false on SingularMargBlock or a factorization failure.
 bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
-                 std::vector<double>& dx, double tol_rel = 1e-12) {
+                 std::vector<double>& dx, double tol_rel = 1e-12, const Kkt* kkt = nullptr) {
   const int64_t nb = H.nb;
   const int64_t nfs = nfb == 0 ? 0 : H.start[nfb - 1] + H.dim[nfb - 1];
   // S pattern: H_ff pattern plus every (f_i, f_j) pair sharing a marg block.
@@ -1116,9 +1174,42 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
   BlockLdltFactor F;
   if (!block_ldlt_factorize(S, F, tol_rel)) return false;
   std::vector<double> dxf = rhs;
+  std::vector<double> dlam;
+  if (kkt && kkt->nc > 0) {
+    // range-space KKT (schur_ldlt.rs:450-480, 674-730, 762-772): Y = S^-1 G^T, M = G Y,
+    // u = S^-1 rhs, dlam = M^-1 (G u - rhs_c), dx_f = S^-1 (rhs - G^T dlam)
+    const int64_t nc = kkt->nc;
+    std::vector<std::vector<double>> Y(nc);
+    for (int64_t k = 0; k < nc; ++k) {
+      Y[k].assign(kkt->G.begin() + k * nfs, kkt->G.begin() + (k + 1) * nfs);
+      block_ldlt_solve(F, S, Y[k]);
+    }
+    std::vector<double> M(size_t(nc * nc)), Mi(size_t(nc * nc));
+    for (int64_t a = 0; a < nc; ++a)
+      for (int64_t b = 0; b < nc; ++b) {
+        double acc = 0;
+        for (int64_t i = 0; i < nfs; ++i) acc += kkt->G[size_t(a * nfs + i)] * Y[b][i];
+        M[size_t(a * nc + b)] = acc;
+      }
+    if (!inverse_dense(int(nc), M.data(), Mi.data())) return false;  // SingularKktConstraint
+    std::vector<double> u = rhs;
+    block_ldlt_solve(F, S, u);
+    std::vector<double> w(nc);
+    for (int64_t a = 0; a < nc; ++a) {
+      double acc = 0;
+      for (int64_t i = 0; i < nfs; ++i) acc += kkt->G[size_t(a * nfs + i)] * u[i];
+      w[a] = acc - kkt->rhs_c[a];
+    }
+    dlam.assign(nc, 0.0);
+    for (int64_t a = 0; a < nc; ++a)
+      for (int64_t b = 0; b < nc; ++b) dlam[a] += Mi[size_t(a * nc + b)] * w[b];
+    for (int64_t i = 0; i < nfs; ++i)
+      for (int64_t a = 0; a < nc; ++a) dxf[i] -= kkt->G[size_t(a * nfs + i)] * dlam[a];
+  }
   block_ldlt_solve(F, S, dxf);
-  dx.assign(g.size(), 0.0);
+  dx.assign(g.size() + dlam.size(), 0.0);
   std::copy(dxf.begin(), dxf.end(), dx.begin());
+  std::copy(dlam.begin(), dlam.end(), dx.begin() + g.size());
   // back-substitution (:785-806)
 #pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1)
   for (int64_t b = nfb; b < nb; ++b) {
@@ -1147,7 +1238,8 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
 
 // ---- problem construction / validation ----------------------------------------------------
 sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_cost_fn* cfs,
-                          int64_t nc, Problem& P) {
+                          int64_t nc, Problem& P, const sgpu_eq_fn* eqs = nullptr,
+                          int64_t neq = 0) {
   if (!fams || nf <= 0) return fail(SGPU_ERR_INVALID_ARGUMENT, "no variable families");
   for (int64_t i = 0; i < nf; ++i) {
     const auto& s = fams[i];
@@ -1256,6 +1348,37 @@ sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_co
     P.costs.push_back(std::move(c));
   }
   if (P.costs.empty()) return fail(SGPU_ERR_INVALID_ARGUMENT, "no cost functions");
+  // equality-constraint sets (EqSystem::new, eq_system.rs:24-48; validate_solver_config,
+  // optimizer.rs:649-683)
+  for (int64_t i = 0; i < neq; ++i) {
+    const sgpu_eq_fn& q = eqs[i];
+    if (q.kind != SGPU_EQ_TRANSLATION_NORM || !q.family_names[0] || q.num_terms < 0 ||
+        (q.num_terms > 0 && (!q.entity_indices || !q.params)))
+      return fail(SGPU_ERR_INVALID_ARGUMENT, "bad equality-constraint set");
+    if (q.num_terms == 0) continue;  // residual_dim 0 sets add nothing
+    EqFn e;
+    e.fam = P.vars.find(q.family_names[0]);
+    if (e.fam < 0) return fail(SGPU_ERR_UNKNOWN_FAMILY, "unknown constraint family");
+    const Family& f = P.vars.fam[e.fam];
+    if (f.kind == SGPU_VAR_MARGINALIZED)
+      return fail(SGPU_ERR_SCHUR_CONSTRAINT_ON_MARGINALIZED_FAMILY,
+                  "equality constraint on a marginalized family");
+    if (f.type != SGPU_VAR_ISOMETRY3) return fail(SGPU_ERR_FAMILY_TYPE, "constraint family type");
+    std::vector<int64_t> order(q.num_terms);
+    for (int64_t t = 0; t < q.num_terms; ++t) {
+      if (q.entity_indices[t] < 0 || q.entity_indices[t] >= f.n)
+        return fail(SGPU_ERR_INDEX_OUT_OF_RANGE, "constraint entity index out of range");
+      order[t] = t;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+      return q.entity_indices[a] < q.entity_indices[b];
+    });
+    for (int64_t t : order) {
+      e.idx.push_back(q.entity_indices[t]);
+      e.radius.push_back(q.params[t]);
+    }
+    P.eqs.push_back(std::move(e));
+  }
   return SGPU_OK;
 }
 
@@ -1271,6 +1394,7 @@ struct Optimizer {
   int64_t rejects = 0, steps = 0;
   bool terminated = false;
   Linearization L;
+  std::vector<double> lambda;  // Lagrange multipliers, one per constraint set (zero at start)
 };
 
 // step_nlls_inner (optimizer_step.rs:366-443) + Optimizer::step (optimizer.rs:345-377)
@@ -1283,15 +1407,25 @@ int step(Optimizer& O, StepRecord& rec) {
   rec.max_abs_delta = NAN;
   int reason = SGPU_TERM_NONE;
   linearize(P, V, O.nu, O.L);
+  // equality constraints: G^T lambda in the free rows, -c below (eq_system.rs:207-262)
+  Kkt K;
+  std::vector<double> full_grad;
+  if (!P.eqs.empty()) {
+    O.lambda.resize(P.eqs.size(), 0.0);
+    eval_kkt(P, V, O.lambda, K, &O.L.neg_grad);
+  }
+  full_grad = O.L.neg_grad;
+  full_grad.insert(full_grad.end(), K.rhs_c.begin(), K.rhs_c.end());
   double maxg = 0;
   for (double v : O.L.neg_grad) maxg = std::max(maxg, std::fabs(v));
   bool done = false;
-  if (maxg < O.params.gradient_tolerance) {
+  // check_gradient_tolerance_unless_eq (optimizer.rs:527-535)
+  if (P.eqs.empty() && maxg < O.params.gradient_tolerance) {
     reason = SGPU_TERM_GRADIENT_TOLERANCE;
     done = true;
   }
   std::vector<double> dx;
-  if (!done && !schur_solve(O.L.H, V.num_free_blocks, O.L.neg_grad, dx)) {
+  if (!done && !schur_solve(O.L.H, V.num_free_blocks, O.L.neg_grad, dx, 1e-12, P.eqs.empty() ? nullptr : &K)) {
     O.nu *= 2.0;  // handle_solve_failure -> reject_step
     ++O.rejects;
     if (O.rejects >= O.params.max_consecutive_rejects) reason = SGPU_TERM_MAX_CONSECUTIVE_REJECTS;
@@ -1299,6 +1433,9 @@ int step(Optimizer& O, StepRecord& rec) {
   }
   if (!done) {
     Vars U = update(V, dx);
+    // update_lambdas (eq_system.rs:149-159): lambda - delta's multiplier rows
+    std::vector<double> lam_new = O.lambda;
+    for (size_t k = 0; k < lam_new.size(); ++k) lam_new[k] -= dx[O.L.neg_grad.size() + k];
     const double nm = merit(P, U);
     rec.new_merit = nm;
     double max_step = 0;
@@ -1307,16 +1444,18 @@ int step(Optimizer& O, StepRecord& rec) {
     if (max_step < O.params.parameter_tolerance) {
       if (nm < O.merit_v) {
         P.vars = U;
+        O.lambda = lam_new;
         O.merit_v = nm;
         rec.accepted = 1;
       }
       reason = SGPU_TERM_PARAMETER_TOLERANCE;
     } else {
-      // compute_gain_ratio (optimizer_helpers.rs:159-173)
+      // compute_gain_ratio (optimizer_helpers.rs:159-173), over the whole step and the whole
+      // right-hand side (multiplier rows included)
       double dd = 0, dg = 0;
       for (size_t i = 0; i < dx.size(); ++i) {
         dd += dx[i] * dx[i];
-        dg += dx[i] * O.L.neg_grad[i];
+        dg += dx[i] * full_grad[i];
       }
       const double pred = O.nu * dd + dg;
       const double actual = O.merit_v - nm;
@@ -1329,6 +1468,7 @@ int step(Optimizer& O, StepRecord& rec) {
         const double rel = (O.merit_v - nm) / (std::fabs(O.merit_v) + DBL_MIN);
         if (rel < O.params.function_tolerance) reason = SGPU_TERM_FUNCTION_TOLERANCE;
         P.vars = U;
+        O.lambda = lam_new;
         O.merit_v = nm;
         rec.accepted = 1;
       } else {
@@ -1404,16 +1544,35 @@ typedef struct oracle_step_record {
 
 // optimize_nlls (nlls.rs:73-104): create, step until termination, build_solution.
 // trace (may be NULL) receives up to trace_capacity step records.
+sgpu_status oracle_optimize_nlls_eq(const sgpu_var_family* fams, int64_t nf,
+                                    const sgpu_cost_fn* cfs, int64_t nc, const sgpu_eq_fn* eqs,
+                                    int64_t neq, const sgpu_opt_params* params,
+                                    double* const* out_members, sgpu_solution* sol,
+                                    oracle_step_record* trace, int64_t trace_capacity,
+                                    int64_t* trace_len, double* lambda_out);
+
 sgpu_status oracle_optimize_nlls(const sgpu_var_family* fams, int64_t nf, const sgpu_cost_fn* cfs,
                                  int64_t nc, const sgpu_opt_params* params,
                                  double* const* out_members, sgpu_solution* sol,
                                  oracle_step_record* trace, int64_t trace_capacity,
                                  int64_t* trace_len) {
+  return oracle_optimize_nlls_eq(fams, nf, cfs, nc, nullptr, 0, params, out_members, sol, trace,
+                                 trace_capacity, trace_len, nullptr);
+}
+
+// optimize_nlls with equality-constraint sets (the reference's eq_fns argument); lambda_out
+// (may be NULL) receives the final multipliers, one per non-empty set.
+sgpu_status oracle_optimize_nlls_eq(const sgpu_var_family* fams, int64_t nf,
+                                    const sgpu_cost_fn* cfs, int64_t nc, const sgpu_eq_fn* eqs,
+                                    int64_t neq, const sgpu_opt_params* params,
+                                    double* const* out_members, sgpu_solution* sol,
+                                    oracle_step_record* trace, int64_t trace_capacity,
+                                    int64_t* trace_len, double* lambda_out) {
   g_error.clear();
   if (!params || !sol) return fail(SGPU_ERR_INVALID_ARGUMENT, "params and solution required");
   Optimizer O;
   O.params = *params;
-  sgpu_status st = build_problem(fams, nf, cfs, nc, O.P);
+  sgpu_status st = build_problem(fams, nf, cfs, nc, O.P, eqs, neq);
   if (st != SGPU_OK) return st;
   symbolic(O.P, O.L);
   O.nu = params->initial_lm_damping;
@@ -1451,7 +1610,34 @@ sgpu_status oracle_optimize_nlls(const sgpu_var_family* fams, int64_t nf, const 
       std::memcpy(out_members[i], f.members.data(), f.members.size() * sizeof(double));
     }
   }
+  if (lambda_out)
+    for (size_t k = 0; k < O.P.eqs.size(); ++k) lambda_out[k] = k < O.lambda.size() ? O.lambda[k] : 0.0;
   return SGPU_OK;
+}
+
+// One equality-constrained linear step at the given values (damping nu, multipliers lambda):
+// G (nc x nfs) and -c of the constraint rows, the neg-gradient with G^T lambda added, and the
+// range-space Schur solve's [dx_f; dx_m; dlambda] (status 1 if the solve failed).
+int32_t oracle_kkt_step(const sgpu_var_family* fams, int64_t nf, const sgpu_cost_fn* cfs,
+                        int64_t nc, const sgpu_eq_fn* eqs, int64_t neq, double nu,
+                        const double* lambda, double* G, double* rhs_c, double* neg_grad,
+                        double* dx) {
+  g_error.clear();
+  Problem P;
+  if (build_problem(fams, nf, cfs, nc, P, eqs, neq) != SGPU_OK) return 2;
+  Linearization L;
+  symbolic(P, L);
+  linearize(P, P.vars, nu, L);
+  std::vector<double> lam(lambda, lambda + P.eqs.size());
+  Kkt K;
+  eval_kkt(P, P.vars, lam, K, &L.neg_grad);
+  std::copy(K.G.begin(), K.G.end(), G);
+  std::copy(K.rhs_c.begin(), K.rhs_c.end(), rhs_c);
+  std::copy(L.neg_grad.begin(), L.neg_grad.end(), neg_grad);
+  std::vector<double> d;
+  if (!schur_solve(L.H, P.vars.num_free_blocks, L.neg_grad, d, 1e-12, &K)) return 1;
+  std::copy(d.begin(), d.end(), dx);
+  return 0;
 }
 
 // Linearization at the given values with damping nu, as a dense matrix over the

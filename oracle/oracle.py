@@ -49,6 +49,12 @@ class OptParamsC(C.Structure):
                 ("precond_weights", C.c_void_p)]
 
 
+class EqFnC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved0", C.c_int32), ("family_names", C.c_char_p * 1),
+                ("num_terms", C.c_int64), ("entity_indices", C.POINTER(C.c_int64)),
+                ("params", C.POINTER(C.c_double))]
+
+
 class SolutionC(C.Structure):
     _fields_ = [("initial_cost", C.c_double), ("final_cost", C.c_double), ("num_steps", C.c_int64),
                 ("termination", C.c_int32), ("reserved0", C.c_int32),
@@ -74,6 +80,13 @@ def lib(variant: str = "") -> C.CDLL:
         L = C.CDLL(str(path))
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_last_step_seconds.restype = C.c_double
+        L.oracle_optimize_nlls_eq.restype = C.c_int
+        L.oracle_optimize_nlls_eq.argtypes = [C.POINTER(VarFamilyC), C.c_int64, C.POINTER(CostFnC), C.c_int64,
+                                              C.POINTER(EqFnC), C.c_int64, C.POINTER(OptParamsC), C.POINTER(DP),
+                                              C.POINTER(SolutionC), C.POINTER(StepRecordC), C.c_int64, I64P, DP]
+        L.oracle_kkt_step.restype = C.c_int32
+        L.oracle_kkt_step.argtypes = [C.POINTER(VarFamilyC), C.c_int64, C.POINTER(CostFnC), C.c_int64,
+                                      C.POINTER(EqFnC), C.c_int64, C.c_double, DP, DP, DP, DP, DP]
         L.oracle_optimize_nlls.restype = C.c_int
         L.oracle_optimize_nlls.argtypes = [C.POINTER(VarFamilyC), C.c_int64, C.POINTER(CostFnC), C.c_int64,
                                            C.POINTER(OptParamsC), C.POINTER(DP), C.POINTER(SolutionC),
@@ -197,9 +210,11 @@ def default_params(**kw) -> OptParamsC:
 
 
 def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trace: int = 1000,
-                  variant: str = ""):
+                  variant: str = "", eqs=None):
     """Returns (solution dict, {name: final members}, trace list) or raises with the status.
-    variant "fma" runs the oracle's rounding twin."""
+    variant "fma" runs the oracle's rounding twin.  eqs: equality-constraint sets
+    [(kind, family_name, entity_indices, params)]; the solution then holds the final
+    multipliers under "lambda"."""
     m = Marshal(families, costs)
     params = params or default_params()
     outs = [np.empty_like(np.asarray(f[3], dtype=np.float64)) for f in families]
@@ -207,16 +222,59 @@ def optimize_nlls(families, costs, params: Optional[OptParamsC] = None, max_trac
     sol = SolutionC()
     trace = (StepRecordC * max_trace)()
     n = C.c_int64()
-    st = lib(variant).oracle_optimize_nlls(m.fams, m.n_fam, m.costs, m.n_cost, C.byref(params), ptrs,
-                                           C.byref(sol), trace, max_trace, C.byref(n))
+    eqs = eqs or []
+    eq_arr = (EqFnC * max(1, len(eqs)))()
+    keep = []
+    for i, (kind, name, idx, prm) in enumerate(eqs):
+        ix = np.ascontiguousarray(idx, dtype=np.int64).reshape(-1)
+        pr = np.ascontiguousarray(prm, dtype=np.float64).reshape(-1)
+        nm = name.encode()
+        keep += [ix, pr, nm]
+        eq_arr[i].kind, eq_arr[i].num_terms = int(kind), ix.size
+        eq_arr[i].family_names[0] = nm
+        eq_arr[i].entity_indices, eq_arr[i].params = ix.ctypes.data_as(I64P), _p(pr)
+    lam = np.zeros(max(1, len(eqs)))
+    st = lib(variant).oracle_optimize_nlls_eq(m.fams, m.n_fam, m.costs, m.n_cost, eq_arr, len(eqs),
+                                              C.byref(params), ptrs, C.byref(sol), trace, max_trace,
+                                              C.byref(n), _p(lam))
     if st != 0:
         return st, None, None
     recs = [dict(merit=t.merit, lm_damping=t.lm_damping, new_merit=t.new_merit, gain_ratio=t.gain_ratio,
                  max_abs_delta=t.max_abs_delta, accepted=bool(t.accepted), termination=t.termination)
             for t in trace[: min(n.value, max_trace)]]
     solution = dict(initial_cost=sol.initial_cost, final_cost=sol.final_cost, num_steps=sol.num_steps,
-                    termination=sol.termination, final_max_abs_neg_grad=sol.final_max_abs_neg_grad)
+                    termination=sol.termination, final_max_abs_neg_grad=sol.final_max_abs_neg_grad,
+                    **({"lambda": lam[:len(eqs)].copy()} if eqs else {}))
     return solution, {f[0]: o for f, o in zip(families, outs)}, recs
+
+
+def _eq_structs(eqs):
+    arr = (EqFnC * max(1, len(eqs)))()
+    keep = []
+    for i, (kind, name, idx, prm) in enumerate(eqs):
+        ix = np.ascontiguousarray(idx, dtype=np.int64).reshape(-1)
+        pr = np.ascontiguousarray(prm, dtype=np.float64).reshape(-1)
+        nm = name.encode()
+        keep += [ix, pr, nm]
+        arr[i].kind, arr[i].num_terms = int(kind), ix.size
+        arr[i].family_names[0] = nm
+        arr[i].entity_indices, arr[i].params = ix.ctypes.data_as(I64P), _p(pr)
+    return arr, keep
+
+
+def kkt_step(families, costs, eqs, nu: float, lam, num_free_scalars: int, dim: int):
+    """One equality-constrained linear step: (G, -c, neg_grad + G^T lambda, [dx; dlambda])."""
+    m = Marshal(families, costs)
+    arr, keep = _eq_structs(eqs)
+    nc = len(eqs)
+    lam = np.ascontiguousarray(lam, dtype=np.float64).reshape(-1)
+    G, rc = np.zeros((nc, num_free_scalars)), np.zeros(nc)
+    g, dx = np.zeros(dim), np.zeros(dim + nc)
+    st = lib().oracle_kkt_step(m.fams, m.n_fam, m.costs, m.n_cost, arr, nc, nu, _p(lam), _p(G), _p(rc),
+                               _p(g), _p(dx))
+    if st != 0:
+        raise RuntimeError(f"oracle_kkt_step status {st}")
+    return G, rc, g, dx
 
 
 def set_rcs_order(perm=None, variant: str = "") -> None:

@@ -2491,6 +2491,131 @@ __global__ void k_update_intrinsics(DevProblem p, const double* __restrict__ dx_
   }
 }
 
+// ---- equality constraints: range-space KKT -------------------------------------------------
+// TranslationNormConstraint (ba_scale_constraint.rs:44-99): c = |t| - r; with T <- exp(x) T,
+// dt/dx = [-hat(t) | I] at x = 0, so dc/dx = [t^T (-hat(t)) | t^T] / |t| = [0 | t^T / |t|].
+// One workgroup: each term's G and c, each set's c summed over its terms in order, and (with
+// add_rhs) G^T lambda added to the free rows, one thread in term order.
+__global__ void k_eq_eval(DevProblem p, DevEq q, const double* __restrict__ poses,
+                          const double* __restrict__ lambda, double* __restrict__ rhs_nat,
+                          double* __restrict__ grad_cam, int add_rhs) {
+  for (int t = threadIdx.x; t < q.nterm; t += blockDim.x) {
+    const double* T = poses + 7 * int64_t(q.term_member[t]);
+    const double n = sqrt(T[4] * T[4] + T[5] * T[5] + T[6] * T[6]);
+    double* g = q.G + 7 * int64_t(t);
+    g[0] = g[1] = g[2] = 0.0;
+    g[3] = T[4] / n;
+    g[4] = T[5] / n;
+    g[5] = T[6] / n;
+    g[6] = n - q.radius[t];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < q.nset; k += blockDim.x) {
+    double c = 0.0;
+    for (int t = q.set_ptr[k]; t < q.set_ptr[k + 1]; ++t) c += q.G[7 * int64_t(t) + 6];
+    q.c[k] = c;
+  }
+  if (add_rhs && threadIdx.x == 0)
+    for (int k = 0; k < q.nset; ++k)
+      for (int t = q.set_ptr[k]; t < q.set_ptr[k + 1]; ++t) {
+        const int blk = p.pose_block[q.term_member[t]];
+        if (blk < 0) continue;
+        for (int i = 0; i < 6; ++i) {
+          const double v = q.G[7 * int64_t(t) + i] * lambda[k];
+          rhs_nat[6 * int64_t(blk) + i] += v;
+          grad_cam[6 * int64_t(blk) + i] += v;
+        }
+      }
+}
+
+// b = G_k^T in camera order (zeros but the set's pose blocks), one workgroup.
+__global__ void k_eq_column(DevProblem p, DevEq q, int k) {
+  for (int64_t e = threadIdx.x; e < 6 * int64_t(p.nb); e += blockDim.x) q.b[e] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int t = q.set_ptr[k]; t < q.set_ptr[k + 1]; ++t) {
+      const int blk = p.pose_block[q.term_member[t]];
+      if (blk < 0) continue;
+      for (int i = 0; i < 6; ++i) q.b[6 * int64_t(blk) + i] += q.G[7 * int64_t(t) + i];
+    }
+}
+
+// G_k . v over the set's pose blocks (v in camera order)
+__device__ double eq_dot(const DevProblem& p, const DevEq& q, int k, const double* v) {
+  double acc = 0.0;
+  for (int t = q.set_ptr[k]; t < q.set_ptr[k + 1]; ++t) {
+    const int blk = p.pose_block[q.term_member[t]];
+    if (blk < 0) continue;
+    for (int i = 0; i < 6; ++i) acc += q.G[7 * int64_t(t) + i] * v[6 * int64_t(blk) + i];
+  }
+  return acc;
+}
+
+// One workgroup: M = G Y, its inverse (Gauss-Jordan with partial pivoting; a zero or
+// non-finite pivot is the reference's SingularKktConstraint), dlambda = M^-1 (G u + c),
+// dx = u - Y dlambda, lambda_next, and the multiplier rows of the step sums into partial[0..1].
+__global__ void __launch_bounds__(kThreads)
+    k_eq_kkt(DevProblem p, DevEq q, const double* __restrict__ lambda,
+             double* __restrict__ lambda_next, double* __restrict__ dx_cam,
+             double* __restrict__ partial, DevScalars* sc) {
+  __shared__ double M[kMaxEqSets][2 * kMaxEqSets], w[kMaxEqSets], dl[kMaxEqSets];
+  __shared__ int singular;
+  const int n = q.nset, tid = threadIdx.x;
+  const int64_t nf = 6 * int64_t(p.nb);
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int a = e / n, b = e % n;
+    M[a][b] = eq_dot(p, q, a, q.Y + nf * b);
+    M[a][n + b] = a == b ? 1.0 : 0.0;
+  }
+  for (int a = tid; a < n; a += blockDim.x) w[a] = eq_dot(p, q, a, dx_cam) + q.c[a];
+  if (tid == 0) singular = 0;
+  __syncthreads();
+  if (tid == 0) {
+    for (int c = 0; c < n && !singular; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < n; ++r)
+        if (fabs(M[r][c]) > fabs(M[piv][c])) piv = r;
+      const double pv = M[piv][c];
+      if (pv == 0.0 || !isfinite(pv)) {
+        singular = 1;
+        break;
+      }
+      for (int k = 0; k < 2 * n; ++k) {
+        const double tmp = M[c][k];
+        M[c][k] = M[piv][k];
+        M[piv][k] = tmp;
+      }
+      for (int k = 0; k < 2 * n; ++k) M[c][k] /= pv;
+      for (int r = 0; r < n; ++r) {
+        if (r == c) continue;
+        const double fct = M[r][c];
+        for (int k = 0; k < 2 * n; ++k) M[r][k] -= fct * M[c][k];
+      }
+    }
+    double dd = 0.0, dg = 0.0, mx = 0.0;
+    for (int a = 0; a < n; ++a) {
+      double v = 0.0;
+      if (!singular)
+        for (int b = 0; b < n; ++b) v += M[a][n + b] * w[b];
+      dl[a] = v;
+      dd += v * v;
+      dg += v * -q.c[a];
+      mx = fmax(mx, fabs(v));
+    }
+    if (singular) sc->bad_pivot = 1;
+    partial[0] = dd;
+    partial[1] = dg;
+    if (mx > 0.0) atomicMax(&sc->max_delta_bits, (unsigned long long)__double_as_longlong(mx));
+  }
+  __syncthreads();
+  for (int64_t e = tid; e < nf; e += blockDim.x) {
+    double v = dx_cam[e];
+    for (int b = 0; b < n; ++b) v -= q.Y[nf * b + e] * dl[b];
+    dx_cam[e] = v;
+  }
+  for (int a = tid; a < n; a += blockDim.x) lambda_next[a] = lambda[a] - dl[a];
+}
+
 // ---- merit: sum over terms of r~^T r~ ------------------------------------------------------
 template <int kModel>
 __global__ void __launch_bounds__(kThreads)
@@ -3081,7 +3206,7 @@ void launch_backsub_points(hipStream_t s, int precision, const DevProblem& p, co
 }
 
 // Partials of the step sums: 2 per back-substitution workgroup, 2 per pose-update workgroup,
-// then 2 for the free intrinsics' block.
+// then 2 for the free intrinsics' block and 2 for the equality constraints' multipliers.
 static int pose_update_blocks(const DevProblem& p) {
   const int npb = p.intr_block >= 0 ? p.intr_block : p.nb;
   return std::max(1, std::min(blocks_for(npb, kThreads), kCostBlocks));
@@ -3100,9 +3225,31 @@ void launch_update_poses(hipStream_t s, const DevProblem& p, const double* poses
                        intr_new, w.partial + 2 * gpt + 2 * g, sc);
 }
 
+void launch_eq_eval(hipStream_t s, const DevProblem& p, const DevEq& q, const double* poses,
+                    const double* lambda, DevLin& w, bool add_rhs) {
+  if (q.nset == 0) return;
+  hipLaunchKernelGGL(k_eq_eval, dim3(1), dim3(64), 0, s, p, q, poses, lambda, w.rhs_nat, w.grad_cam,
+                     add_rhs ? 1 : 0);
+}
+
+void launch_eq_solve(hipStream_t s, const DevProblem& p, const DevEq& q, const DevFactor& f,
+                     const double* lambda, double* lambda_next, DevLin& w, DevScalars* sc) {
+  if (q.nset == 0) return;
+  const int64_t nf = 6 * int64_t(p.nb);
+  for (int k = 0; k < q.nset; ++k) {
+    hipLaunchKernelGGL(k_eq_column, dim3(1), dim3(kThreads), 0, s, p, q, k);
+    launch_rcs_forward(s, f, q.b, q.bperm, q.z);
+    launch_rcs_solve(s, f, q.z, q.Y + nf * k);
+  }
+  const int gpt = std::max(1, std::min(blocks_for(p.npt, kThreads), kCostBlocks));
+  double* part = w.partial + 2 * gpt + 2 * (pose_update_blocks(p) + (p.intr_block >= 0 ? 1 : 0));
+  hipLaunchKernelGGL(k_eq_kkt, dim3(1), dim3(kThreads), 0, s, p, q, lambda, lambda_next, w.dx_cam,
+                     part, sc);
+}
+
 void launch_finalize_step(hipStream_t s, const DevProblem& p, const DevLin& w, DevScalars* sc) {
   const int gpt = std::max(1, std::min(blocks_for(p.npt, kThreads), kCostBlocks));
-  const int gpo = pose_update_blocks(p) + (p.intr_block >= 0 ? 1 : 0);
+  const int gpo = pose_update_blocks(p) + (p.intr_block >= 0 ? 1 : 0) + (p.n_eq > 0 ? 1 : 0);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, w.partial, gpt, gpo, 0, nullptr, 0, sc,
                      0);
 }

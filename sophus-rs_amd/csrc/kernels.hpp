@@ -60,6 +60,23 @@ struct DevProblem {
   int32_t intr_block, intr_member;
   const int32_t* intr_slot;
   int64_t intr_diag_slot;
+  int32_t n_eq;  // equality-constraint sets (their KKT block adds one step-sum partial)
+};
+
+// Equality-constraint sets on the device: translation-norm terms on poses, the terms of a set
+// contiguous and the sets in order.
+struct DevEq {
+  int32_t nset, nterm;
+  const int32_t* term_member;  // pose member per term
+  const int32_t* set_ptr;      // nset + 1
+  const double* radius;        // per term
+  double* G;                   // 7 per term: dc/d(pose tangent) (6), then c
+  double* c;                   // per set: the sum of its terms' residuals
+  double* Y;                   // nset x 6 nb: S^-1 G_k^T (camera order)
+  double* b;                   // 6 nb: G_k^T (camera order)
+  double* bperm;               // 6 nb
+  double* z;                   // 6 nb
+  double* lambda[2];           // multipliers per set (indexed like the variables' buffers)
 };
 
 // RCS supernodal multifrontal factorization on the device.
@@ -171,6 +188,17 @@ void launch_intrinsics_rows(hipStream_t s, const DevProblem& p, const double* po
                             const double* intr, const double* cw, DevLin& w, DevFactor& f);
 // doubles of DevLin::intr_part for the problem
 int intrinsics_partials(const DevProblem& p);
+// Equality constraints (range-space KKT, schur_ldlt.rs:450-480, 674-730, 762-772):
+// launch_eq_eval evaluates c and G at `poses` and, with add_rhs, adds G^T lambda to the free
+// rows of rhs_nat and grad_cam (eq_system.rs:207-262); launch_eq_solve, after the RCS factor
+// and the backward solve of u = S^-1 rhs into dx_cam, forms Y = S^-1 G^T (one standalone
+// forward and backward solve per set), M = G Y, dlambda = M^-1 (G u + c), dx_cam = u - Y
+// dlambda, lambda_next = lambda - dlambda, and the step sums' multiplier rows (|dlambda|^2,
+// -c.dlambda, max |dlambda|).  A singular M sets the bad-pivot flag (the step is rejected).
+void launch_eq_eval(hipStream_t s, const DevProblem& p, const DevEq& q, const double* poses,
+                    const double* lambda, DevLin& w, bool add_rhs);
+void launch_eq_solve(hipStream_t s, const DevProblem& p, const DevEq& q, const DevFactor& f,
+                     const double* lambda, double* lambda_next, DevLin& w, DevScalars* sc);
 void launch_cost(hipStream_t s, const DevProblem& p, const double* poses, const double* points,
                  const double* intr, double* partial, int32_t partial_cap, DevScalars* sc);
 void launch_finalize_step(hipStream_t s, const DevProblem& p, const DevLin& w, DevScalars* sc);

@@ -84,6 +84,7 @@ struct sgpu_optimizer {
   DevPcg pcg{};                // iterative RCS solver state (PCG solvers only)
   PcgState* h_pcg = nullptr;   // pinned poll buffer
   DevLearned learned{};        // learned block-Jacobi correction (PCG_LEARNED only)
+  DevEq deq{};                 // equality-constraint sets (none: nset = 0)
   double merit = 0, nu = 0;
   int64_t rejects = 0, steps = 0;
   double grad0 = -1.0, grad_last = -1.0;  // max |g| of the first and the last linearization read
@@ -171,6 +172,8 @@ struct sgpu_optimizer {
     toc(K_CAMERA_ROWS);
     const double* centered = params.precision == SGPU_PRECISION_MIXED ? dl.cw : nullptr;
     for (auto& g : dpg) launch_pose_graph_linearize(stream, g, dp, ps, nu_lin, dl, df, centered);
+    // the multipliers of the variables' buffer (the variables linearized are the current ones)
+    launch_eq_eval(stream, dp, deq, ps, deq.lambda[cur], dl, true);
     launch_max_grad(stream, dl.grad_cam, S.nb, d_sc);
     used[K_LINEARIZE_POINTS] = used[K_CAMERA_ROWS] = true;
   }
@@ -199,7 +202,7 @@ struct sgpu_optimizer {
   }
 
   void init(const sgpu_var_family* families, int64_t nf, const sgpu_cost_fn* cfs, int64_t nc,
-            const sgpu_opt_params* prm) {
+            const sgpu_eq_fn* eqs, int64_t neq, const sgpu_opt_params* prm) {
     params = *prm;
     if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT &&
         params.solver != SGPU_SOLVER_SCHUR_PCG_JACOBI && params.solver != SGPU_SOLVER_SCHUR_PCG_LEARNED)
@@ -214,7 +217,11 @@ struct sgpu_optimizer {
     params.precond_weights = nullptr;  // copied below; the caller's array is not kept
     if (params.precision != SGPU_PRECISION_F64 && params.precision != SGPU_PRECISION_MIXED)
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown precision");
-    P = build_host_problem(families, nf, cfs, nc);
+    P = build_host_problem(families, nf, cfs, nc, eqs, neq);
+    if (!P.eqs.empty() && (iterative() || params.precision != SGPU_PRECISION_F64))
+      throw Error(SGPU_ERR_UNSUPPORTED,
+                  "equality constraints are implemented for the direct RCS solver in FP64 (the "
+                  "range-space KKT solve needs S^-1 G^T from the factorization)");
     if (P.intr_block >= 0 && params.precision != SGPU_PRECISION_F64)
       throw Error(SGPU_ERR_UNSUPPORTED,
                   "free intrinsics are implemented in the FP64 precision (the mixed path's centred "
@@ -298,6 +305,30 @@ struct sgpu_optimizer {
     dp.intr_member = P.intr_member;
     dp.intr_slot = P.intr_block >= 0 ? arena.upload(S.intr_slot, stream) : nullptr;
     dp.intr_diag_slot = S.intr_diag_slot;
+    dp.n_eq = int32_t(P.eqs.size());
+    if (!P.eqs.empty()) {
+      std::vector<int32_t> member, set_ptr{0};
+      std::vector<double> radius;
+      for (const auto& e : P.eqs) {
+        member.insert(member.end(), e.member.begin(), e.member.end());
+        radius.insert(radius.end(), e.radius.begin(), e.radius.end());
+        set_ptr.push_back(int32_t(member.size()));
+      }
+      deq.nset = int32_t(P.eqs.size());
+      deq.nterm = int32_t(member.size());
+      deq.term_member = arena.upload(member, stream);
+      deq.set_ptr = arena.upload(set_ptr, stream);
+      deq.radius = arena.upload(radius, stream);
+      deq.G = arena.alloc<double>(size_t(7) * deq.nterm);
+      deq.c = arena.alloc<double>(size_t(deq.nset));
+      deq.Y = arena.alloc<double>(size_t(6) * S.nb * deq.nset);
+      deq.b = arena.alloc<double>(size_t(6) * S.nb);
+      deq.bperm = arena.alloc<double>(size_t(6) * S.nb);
+      deq.z = arena.alloc<double>(size_t(6) * S.nb);
+      // EqSystem::new starts the multipliers at zero (eq_system.rs:37)
+      const std::vector<double> zero(size_t(deq.nset), 0.0);
+      for (int i = 0; i < 2; ++i) deq.lambda[i] = arena.upload(zero, stream);
+    }
     // factor structure
     df.nb = S.nb;
     df.nsn = S.nsn;
@@ -389,7 +420,7 @@ struct sgpu_optimizer {
       dl.intr_part = arena.alloc<double>(size_t(intrinsics_partials(dp)));
       dl.intr_cpart = arena.alloc<double>(size_t(36) * std::max<size_t>(S.own_chunks.size() / 3, 1));
     }
-    dl.partial_cap = 4 * kCostBlocks + 2;
+    dl.partial_cap = 4 * kCostBlocks + 4;
     dl.partial = arena.alloc<double>(dl.partial_cap);
     cost_partial = arena.alloc<double>(size_t(2) * kCostBlocks * (1 + P.pg.size()));
     d_sc = arena.alloc<DevScalars>(1);
@@ -450,6 +481,8 @@ struct sgpu_optimizer {
       SGPU_HIP(hipEventRecord(phase_ev[2], stream));
       tic(K_TRIANGULAR);
       launch_rcs_solve(stream, df, dl.x_perm, dl.dx_cam);
+      // equality constraints: the range-space KKT correction of dx and the multipliers' step
+      launch_eq_solve(stream, dp, deq, df, deq.lambda[cur], deq.lambda[nxt], dl, d_sc);
     }
     // the mixed path solves for centred camera tangents (see k_schur_slots)
     if (params.precision == SGPU_PRECISION_MIXED) launch_uncenter(stream, dp, dl.cw, dl.dx_cam);
@@ -483,7 +516,8 @@ struct sgpu_optimizer {
     double new_merit = NAN, rho = NAN;
     const double max_grad = bits_to_double(sc.max_grad_bits);
     const double max_step = bits_to_double(sc.max_delta_bits);
-    if (max_grad < params.gradient_tolerance) {
+    // check_gradient_tolerance_unless_eq (optimizer.rs:527-535)
+    if (P.eqs.empty() && max_grad < params.gradient_tolerance) {
       reason = SGPU_TERM_GRADIENT_TOLERANCE;
     } else if (sc.singular_marg || sc.bad_pivot) {
       nu *= 2.0;  // handle_solve_failure -> reject_step
@@ -579,6 +613,8 @@ struct sgpu_optimizer {
   void ensure_cov() {
     if (cov_ready) return;
     if (iterative()) throw Error(SGPU_ERR_UNSUPPORTED, "inverse blocks need the direct RCS solver");
+    if (!P.eqs.empty())
+      throw Error(SGPU_ERR_UNSUPPORTED, "inverse blocks of an equality-constrained problem");
     const int64_t n = 6 * int64_t(S.nb);
     if (!cov_b) {
       cov_b = arena.alloc<double>(size_t(n));
@@ -813,12 +849,37 @@ void sgpu_opt_params_default(sgpu_opt_params* p) {
 sgpu_status sgpu_optimizer_create(const sgpu_var_family* families, int64_t num_families,
                                   const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                                   const sgpu_opt_params* params, sgpu_optimizer** out) {
+  return sgpu_optimizer_create_eq(families, num_families, cost_fns, num_cost_fns, nullptr, 0,
+                                  params, out);
+}
+
+sgpu_status sgpu_optimizer_create_eq(const sgpu_var_family* families, int64_t num_families,
+                                     const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                     const sgpu_eq_fn* eq_fns, int64_t num_eq_fns,
+                                     const sgpu_opt_params* params, sgpu_optimizer** out) {
   return guarded([&] {
     if (!out || !params) throw Error(SGPU_ERR_INVALID_ARGUMENT, "params and out are required");
     *out = nullptr;
     auto opt = std::make_unique<sgpu_optimizer>();
-    opt->init(families, num_families, cost_fns, num_cost_fns, params);
+    opt->init(families, num_families, cost_fns, num_cost_fns, eq_fns, num_eq_fns, params);
     *out = opt.release();
+  });
+}
+
+sgpu_status sgpu_optimizer_eq_state(sgpu_optimizer* opt, double* lambda, double* residual,
+                                    int64_t capacity) {
+  return guarded([&] {
+    if (!opt) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null optimizer");
+    const int64_t n = int64_t(opt->P.eqs.size());
+    if (capacity < n) throw Error(SGPU_ERR_INVALID_ARGUMENT, "output capacity too small");
+    if (n == 0) return;
+    SGPU_HIP(hipSetDevice(opt->device));
+    launch_eq_eval(opt->stream, opt->dp, opt->deq, opt->poses[opt->cur], opt->deq.lambda[opt->cur],
+                   opt->dl, false);
+    if (lambda)
+      SGPU_HIP(hipMemcpyAsync(lambda, opt->deq.lambda[opt->cur], n * 8, hipMemcpyDeviceToHost, opt->stream));
+    if (residual) SGPU_HIP(hipMemcpyAsync(residual, opt->deq.c, n * 8, hipMemcpyDeviceToHost, opt->stream));
+    SGPU_HIP(hipStreamSynchronize(opt->stream));
   });
 }
 
@@ -1061,10 +1122,19 @@ sgpu_status sgpu_optimize_nlls(const sgpu_var_family* families, int64_t num_fami
                                const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                                const sgpu_opt_params* params, double* const* out_members,
                                sgpu_solution* solution) {
+  return sgpu_optimize_nlls_eq(families, num_families, cost_fns, num_cost_fns, nullptr, 0, params,
+                               out_members, solution);
+}
+
+sgpu_status sgpu_optimize_nlls_eq(const sgpu_var_family* families, int64_t num_families,
+                                  const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                  const sgpu_eq_fn* eq_fns, int64_t num_eq_fns,
+                                  const sgpu_opt_params* params, double* const* out_members,
+                                  sgpu_solution* solution) {
   return guarded([&] {
     if (!params || !solution) throw Error(SGPU_ERR_INVALID_ARGUMENT, "params and solution required");
     auto opt = std::make_unique<sgpu_optimizer>();
-    opt->init(families, num_families, cost_fns, num_cost_fns, params);
+    opt->init(families, num_families, cost_fns, num_cost_fns, eq_fns, num_eq_fns, params);
     solution->initial_cost = opt->merit;
     while (!opt->terminated) opt->step(nullptr);
     solution->final_cost = opt->merit;

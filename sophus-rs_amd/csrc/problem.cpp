@@ -26,7 +26,8 @@ bool is_reprojection(int kind) {
 }  // namespace
 
 HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_families,
-                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns) {
+                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                               const sgpu_eq_fn* eq_fns, int64_t num_eq_fns) {
   HostProblem P;
   if (!families || num_families <= 0) raise(SGPU_ERR_INVALID_ARGUMENT, "no variable families");
   if (!cost_fns || num_cost_fns <= 0) raise(SGPU_ERR_INVALID_ARGUMENT, "no cost functions");
@@ -220,6 +221,47 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
       raise(SGPU_ERR_UNSUPPORTED, "marginalized family '" + f.name + "' is not the point family");
   }
   if (P.f_point < 0) raise(SGPU_ERR_UNSUPPORTED, "no reprojection cost function");
+  // equality-constraint sets (EqSystem::new, eq_system.rs:24-48; the Schur check of
+  // validate_solver_config, optimizer.rs:649-683)
+  if (num_eq_fns < 0 || (num_eq_fns > 0 && !eq_fns))
+    raise(SGPU_ERR_INVALID_ARGUMENT, "bad equality-constraint sets");
+  for (int64_t i = 0; i < num_eq_fns; ++i) {
+    const sgpu_eq_fn& q = eq_fns[i];
+    if (q.kind != SGPU_EQ_TRANSLATION_NORM || !q.family_names[0] || q.num_terms < 0 ||
+        (q.num_terms > 0 && (!q.entity_indices || !q.params)))
+      raise(SGPU_ERR_INVALID_ARGUMENT, "bad equality-constraint set");
+    if (q.num_terms == 0) continue;  // a set with no terms has no residual rows
+    const int fi = find(q.family_names[0]);
+    if (fi < 0)
+      raise(SGPU_ERR_UNKNOWN_FAMILY, std::string("unknown family '") + q.family_names[0] + "'");
+    const Family& f = P.fams[fi];
+    // validate_solver_config's Schur check comes first (optimizer.rs:664-678)
+    if (f.kind == SGPU_VAR_MARGINALIZED)
+      raise(SGPU_ERR_SCHUR_CONSTRAINT_ON_MARGINALIZED_FAMILY,
+            "equality constraint on the marginalized family '" + f.name + "'");
+    if (f.type != SGPU_VAR_ISOMETRY3)
+      raise(SGPU_ERR_FAMILY_TYPE, "family '" + f.name + "' has the wrong variable type");
+    if (fi != P.f_pose)
+      raise(SGPU_ERR_UNSUPPORTED, "equality constraints on a family other than the pose family");
+    if (q.num_terms > INT32_MAX / 8) raise(SGPU_ERR_UNSUPPORTED, "too many constraint terms");
+    std::vector<int32_t> ord(q.num_terms);
+    for (int64_t t = 0; t < q.num_terms; ++t) {
+      if (q.entity_indices[t] < 0 || q.entity_indices[t] >= f.n)
+        raise(SGPU_ERR_INDEX_OUT_OF_RANGE, "constraint entity index out of range");
+      ord[t] = int32_t(t);
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) {
+      return q.entity_indices[a] < q.entity_indices[b];
+    });
+    EqSet e;
+    for (int32_t t : ord) {
+      e.member.push_back(int32_t(q.entity_indices[t]));
+      e.radius.push_back(q.params[t]);
+    }
+    P.eqs.push_back(std::move(e));
+  }
+  if (int64_t(P.eqs.size()) > kMaxEqSets)
+    raise(SGPU_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxEqSets) + " equality-constraint sets");
   P.num_pose_blocks = P.fams[P.f_pose].num_active;
   P.intr_block = P.intr_member >= 0 ? int32_t(P.num_pose_blocks) : -1;
   P.num_free_blocks = P.num_pose_blocks + (P.intr_member >= 0 ? 1 : 0);

@@ -19,6 +19,13 @@ struct Family {
   int64_t num_active = 0;
 };
 
+// An equality-constraint set (EqConstraintFn): translation-norm terms on pose members, sorted
+// by member; all terms share one constraint row and multiplier (eq_system.rs:33-37,207-262).
+struct EqSet {
+  std::vector<int32_t> member;
+  std::vector<double> radius;
+};
+
 struct PoseGraphFn {
   int robust = 0;
   double delta = 0;
@@ -49,6 +56,7 @@ struct HostProblem {
   std::vector<int32_t> obs_pose, obs_point, obs_intr;
   std::vector<double> obs_uv;
   std::vector<PoseGraphFn> pg;
+  std::vector<EqSet> eqs;  // non-empty equality-constraint sets on the pose family
   int64_t num_free_blocks = 0, num_marg_blocks = 0;
   std::vector<int32_t> cam_member;  // free block -> pose member (-1: the intrinsics block)
   // free block -> its index in the reference's layout (Free families in name order, active
@@ -58,7 +66,10 @@ struct HostProblem {
 
 // Builds and validates; throws sgpu::Error with the reference's error variant.
 HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_families,
-                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns);
+                               const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                               const sgpu_eq_fn* eq_fns = nullptr, int64_t num_eq_fns = 0);
+// Most equality-constraint sets one handle takes (the KKT block is solved densely per step).
+constexpr int32_t kMaxEqSets = 16;
 
 // Symbolic structure of one problem (computed once per solve).
 constexpr int32_t kOwnChunk = 256;  // camera-major observations per own-term workgroup

@@ -59,6 +59,17 @@ class _CostFn(C.Structure):
     ]
 
 
+class _EqFn(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("family_names", C.c_char_p * 1),
+        ("num_terms", C.c_int64),
+        ("entity_indices", C.POINTER(C.c_int64)),
+        ("params", C.POINTER(C.c_double)),
+    ]
+
+
 class _OptParams(C.Structure):
     _fields_ = [
         ("num_iterations", C.c_int64),
@@ -190,6 +201,13 @@ _SIGNATURES = {
                                      C.POINTER(_OptParams), C.POINTER(_DP), C.POINTER(_Solution)]),
     "sgpu_optimizer_create": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
                                         C.POINTER(_OptParams), C.POINTER(C.c_void_p)]),
+    "sgpu_optimize_nlls_eq": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                                        C.POINTER(_EqFn), C.c_int64, C.POINTER(_OptParams), C.POINTER(_DP),
+                                        C.POINTER(_Solution)]),
+    "sgpu_optimizer_create_eq": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                                           C.POINTER(_EqFn), C.c_int64, C.POINTER(_OptParams),
+                                           C.POINTER(C.c_void_p)]),
+    "sgpu_optimizer_eq_state": (C.c_int, [C.c_void_p, _DP, _DP, C.c_int64]),
     "sgpu_optimizer_step": (C.c_int, [C.c_void_p, C.POINTER(_StepInfo)]),
     "sgpu_optimizer_get_members": (C.c_int, [C.c_void_p, C.c_char_p, _DP, C.c_int64]),
     "sgpu_optimizer_merit": (C.c_double, [C.c_void_p]),
@@ -239,7 +257,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 6  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 7  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -279,6 +297,10 @@ class NllsError(RuntimeError):
 
 
 class InvalidArgument(NllsError):
+    pass
+
+
+class SchurConstraintOnMarginalizedFamily(NllsError):
     pass
 
 
@@ -327,6 +349,7 @@ _ERRORS = {
     10: DeviceError,
     11: NllsError,
     12: Terminated,
+    13: SchurConstraintOnMarginalizedFamily,
 }
 
 
@@ -551,10 +574,36 @@ class OptimizationSolution:
     final_max_abs_neg_grad: float
 
 
+class EqKind(enum.IntEnum):
+    TranslationNorm = 0
+
+
+@dataclass
+class EqConstraintFn:
+    """EqConstraintFn::new_boxed((), EqConstraints::new(family_names, terms))
+    (eq_constraint_fn.rs:61-140).  All terms of one set share one constraint row and one
+    multiplier (eq_system.rs:33-37,207-262)."""
+
+    kind: EqKind
+    family_name: str
+    entity_indices: np.ndarray
+    params: np.ndarray
+
+    def __post_init__(self):
+        self.entity_indices = np.ascontiguousarray(self.entity_indices, dtype=np.int64).reshape(-1)
+        self.params = np.ascontiguousarray(self.params, dtype=np.float64).reshape(-1)
+
+    @staticmethod
+    def translation_norm(family_name: str, entity_indices, radii) -> "EqConstraintFn":
+        """TranslationNormConstraint terms |t(T)| = r (ba_scale_constraint.rs:44-99)."""
+        return EqConstraintFn(EqKind.TranslationNorm, family_name, entity_indices, radii)
+
+
 class _Marshal:
     """Keeps the ctypes views of one problem alive for the duration of a call."""
 
-    def __init__(self, variables: VarFamilies, cost_fns: Sequence[CostFn]):
+    def __init__(self, variables: VarFamilies, cost_fns: Sequence[CostFn],
+                 eq_fns: Optional[Sequence[EqConstraintFn]] = None):
         self.names = list(variables.collection)
         self.keep = []
         fams = (_VarFamily * len(self.names))()
@@ -583,6 +632,17 @@ class _Marshal:
             costs[i].measurements = c.measurements.ctypes.data_as(_DP)
             self.keep += [c.entity_indices, c.measurements]
         self.fams, self.costs = fams, costs
+        eq_fns = list(eq_fns or [])
+        self.n_eq = len(eq_fns)
+        self.eqs = (_EqFn * max(1, len(eq_fns)))()
+        for i, e in enumerate(eq_fns):
+            nm = e.family_name.encode()
+            self.keep += [nm, e.entity_indices, e.params]
+            self.eqs[i].kind = int(e.kind)
+            self.eqs[i].family_names[0] = nm
+            self.eqs[i].num_terms = e.entity_indices.size
+            self.eqs[i].entity_indices = e.entity_indices.ctypes.data_as(_I64P)
+            self.eqs[i].params = e.params.ctypes.data_as(_DP)
 
 
 def _termination(v: int) -> Optional[TerminationReason]:
@@ -592,14 +652,24 @@ def _termination(v: int) -> Optional[TerminationReason]:
 class Optimizer:
     """Stateful LM optimizer on the GPU (Optimizer::new / step / build_solution)."""
 
-    def __init__(self, variables: VarFamilies, cost_fns: Sequence[CostFn], params: Optional[OptParams] = None):
+    def __init__(self, variables: VarFamilies, cost_fns: Sequence[CostFn], params: Optional[OptParams] = None,
+                 eq_fns: Optional[Sequence[EqConstraintFn]] = None):
         self.params = params or OptParams()
         self._variables = variables
-        self._m = _Marshal(variables, cost_fns)
+        self._m = _Marshal(variables, cost_fns, eq_fns)
         h = C.c_void_p()
-        _check(library().sgpu_optimizer_create(self._m.fams, len(self._m.names), self._m.costs,
-                                               len(cost_fns), C.byref(self.params._c()), C.byref(h)))
+        _check(library().sgpu_optimizer_create_eq(self._m.fams, len(self._m.names), self._m.costs,
+                                                  len(cost_fns), self._m.eqs, self._m.n_eq,
+                                                  C.byref(self.params._c()), C.byref(h)))
         self._h = h
+
+    def eq_state(self):
+        """(multipliers, constraint residuals c) at the current variables, one per set."""
+        n = self._m.n_eq
+        lam, res = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        _check(library().sgpu_optimizer_eq_state(self._h, lam.ctypes.data_as(_DP), res.ctypes.data_as(_DP),
+                                                 lam.size))
+        return lam[:n], res[:n]
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -731,15 +801,16 @@ def analyze_rcs_order(variables: VarFamilies, cost_fns: Sequence[CostFn]) -> np.
 
 
 def optimize_nlls(variables: VarFamilies, cost_fns: Sequence[CostFn],
-                  params: Optional[OptParams] = None) -> OptimizationSolution:
-    """optimize_nlls(variables, cost_fns, vec![], params) on the GPU (nlls.rs:73-80)."""
+                  params: Optional[OptParams] = None,
+                  eq_fns: Optional[Sequence[EqConstraintFn]] = None) -> OptimizationSolution:
+    """optimize_nlls(variables, cost_fns, eq_fns, params) on the GPU (nlls.rs:73-80)."""
     params = params or OptParams()
-    m = _Marshal(variables, cost_fns)
+    m = _Marshal(variables, cost_fns, eq_fns)
     outs = [np.empty_like(variables.collection[n].members) for n in m.names]
     ptrs = (_DP * len(outs))(*[o.ctypes.data_as(_DP) for o in outs])
     sol = _Solution()
-    _check(library().sgpu_optimize_nlls(m.fams, len(m.names), m.costs, len(cost_fns),
-                                        C.byref(params._c()), ptrs, C.byref(sol)))
+    _check(library().sgpu_optimize_nlls_eq(m.fams, len(m.names), m.costs, len(cost_fns), m.eqs, m.n_eq,
+                                           C.byref(params._c()), ptrs, C.byref(sol)))
     fams = {n: VarFamily(variables.collection[n].kind, variables.collection[n].type, o,
                          variables.collection[n].const_ids) for n, o in zip(m.names, outs)}
     return OptimizationSolution(VarFamilies(fams), sol.initial_cost, sol.final_cost, sol.num_steps,
