@@ -59,12 +59,13 @@ def host_threads() -> int:
 
 
 def oracle_step_seconds(oracle, problem, threads: int, iterations: int = 1, families=None,
-                        costs=None) -> float:
+                        costs=None, eqs=None) -> float:
     """Wall time inside the LM steps of `iterations` FP64-oracle iterations at `threads`."""
     oracle.set_threads(threads)
     try:
         oracle.optimize_nlls(families or oracle.ba_families(problem), costs or oracle.ba_costs(problem),
-                             oracle.default_params(num_iterations=iterations, initial_lm_damping=1.0))
+                             oracle.default_params(num_iterations=iterations, initial_lm_damping=1.0),
+                             eqs=eqs)
         return oracle.last_step_seconds()
     finally:
         oracle.set_threads(1)
@@ -303,8 +304,10 @@ def eucm_leg(sg, cpu: bool) -> dict:
     (ba_bench.rs:213-251): 20 LM iterations from nu = 1, the camera Free and Conditioned, one
     warm-up and the median of 3 solves.  10 cameras / 500 points is the reference bench's size;
     120 cameras / 12k candidate points (605k observations, 50 per point) shows the free
-    calibration block at scale.  The CPU
-    oracle runs the same 20 iterations (the larger problem: one) at nproc and on one core."""
+    calibration block at scale.  The CPU oracle runs the same 20 iterations (the larger
+    problem: one) at nproc and on one core.  Each size also runs the reference's
+    BaScaleConstraintProblem (pose 0 fixed, |t(pose 1)| = r, the range-space KKT solve; the
+    reference bench's "BA+scale" rows)."""
     out = {}
     for nc, npt in ((10, 500), (120, 12000)):
         p = sg.generate_ba_eucm(nc, npt, seed=0)
@@ -349,6 +352,45 @@ def eucm_leg(sg, cpu: bool) -> dict:
                     row["cpu_oracle"].append({"threads": threads, "iterations": it_cpu,
                                               "ms_per_iteration": 1e3 * float(np.median(ct)) / it_cpu})
             out[key] = row
+        r = float(np.linalg.norm(p.poses_gt[1, 4:]))
+        v = p.variables(sg.VarKind.Free, const_poses=[0])
+        eqs = [sg.EqConstraintFn.translation_norm("poses", [1], [r])]
+        times = []
+        for rep in range(4):
+            opt = sg.Optimizer(v, p.cost_fns(), sg.OptParams(num_iterations=20, initial_lm_damping=1.0), eqs)
+            h = []
+            t0 = time.perf_counter()
+            while True:
+                info = opt.step()
+                h.append(info.merit)
+                if info.termination is not None:
+                    break
+            dt = time.perf_counter() - t0
+            fv = opt.variables()
+            _, res = opt.eq_state()
+            opt.close()
+            if rep:
+                times.append(dt)
+        pose_rms, pt_rms = p.gt_errors(fv)
+        row = {"num_obs": p.num_obs, "iterations": len(h), "final_cost": h[-1],
+               "gpu_ms_per_solve": 1e3 * float(np.median(times)),
+               "gpu_ms_per_iteration": 1e3 * float(np.median(times)) / len(h),
+               "constraint_residual": float(res[0]), "pose_rms": pose_rms, "point_rms": pt_rms}
+        if cpu:
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import oracle
+            fams = [("cams", 3, 0, p.intrinsics_init, []), ("poses", 0, 0, p.poses_init, [0]),
+                    ("points", 1, 2, p.points_init, [])]
+            costs = [(2, ["cams", "poses", "points"], p.obs_idx, p.obs_uv, None)]
+            it_cpu = 20 if nc == 10 else 1
+            row["cpu_oracle"] = []
+            for threads in sorted({host_threads(), 1}, reverse=True):
+                ct = [oracle_step_seconds(oracle, p, threads, it_cpu, fams, costs, [(0, "poses", [1], [r])])
+                      for _ in range(4 if nc == 10 else 1)]
+                ct = ct[1:] if nc == 10 else ct
+                row["cpu_oracle"].append({"threads": threads, "iterations": it_cpu,
+                                          "ms_per_iteration": 1e3 * float(np.median(ct)) / it_cpu})
+        out[f"eucm_ring_{nc}cams_{p.points_gt.shape[0]}pts_scale_constraint"] = row
     return out
 
 

@@ -1225,15 +1225,17 @@ class EucmBaProblem:
         return self.obs_uv.shape[0]
 
     def variables(self, cams_kind: VarKind = VarKind.Free, intrinsics=None, poses=None,
-                  points=None) -> VarFamilies:
+                  points=None, const_poses=(0, 1)) -> VarFamilies:
         """BaProblem::build_variables: 'cams' (the EUCM camera; from the ground truth when
-        Conditioned), 'poses' (Free, 0 and 1 constant), 'points' (Marginalized)."""
+        Conditioned), 'poses' (Free, 0 and 1 constant; BaScaleConstraintProblem holds only 0,
+        ba_scale_constraint.rs:124-155), 'points' (Marginalized)."""
         if intrinsics is None:
             intrinsics = self.intrinsics_gt if cams_kind == VarKind.Conditioned else self.intrinsics_init
         return (VarBuilder()
                 .add_family("cams", VarFamily(cams_kind, VarType.Eucm, intrinsics))
                 .add_family("poses", VarFamily(VarKind.Free, VarType.Isometry3,
-                                               self.poses_init if poses is None else poses, [0, 1]))
+                                               self.poses_init if poses is None else poses,
+                                               list(const_poses)))
                 .add_family("points", VarFamily(VarKind.Marginalized, VarType.Vec3,
                                                 self.points_init if points is None else points))
                 .build())
