@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 7
+#define SGPU_ABI_VERSION 8
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -194,7 +194,9 @@ typedef struct sgpu_opt_params {
   double pcg_relative_tolerance; /* default 1e-8 (PCG solvers only) */
   int64_t pcg_max_iterations;    /* default 500 */
   int32_t device;                /* HIP device ordinal, default 0 */
-  int32_t reserved0;
+  /* 0 (default): the term sort, the S pattern and the pair lists are built on the device;
+   * 1: by the host analysis (the same arrays) */
+  int32_t host_symbolic;
   /* SCHUR_PCG_LEARNED only: SGPU_PRECOND_WEIGHTS FP32 weights of the block-Jacobi
    * correction (copied at create; see sgpu_precond_train).  Default NULL. */
   const float* precond_weights;
@@ -377,6 +379,16 @@ sgpu_status sgpu_analyze(const sgpu_var_family* families, int64_t num_families,
 sgpu_status sgpu_analyze_rcs_order(const sgpu_var_family* families, int64_t num_families,
                                    const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                                    int64_t* perm, int64_t capacity, int64_t* num_blocks);
+/* The symbolic analysis both ways on `device`: the term sort, the S pattern and the pair
+ * lists by the host (what sgpu_analyze runs) and on the device (what an optimizer runs
+ * unless host_symbolic is set), compared element by element over every array the numeric
+ * phase reads (the sorted terms, the pattern, the pair lists, the camera-major lists, the
+ * ordering and the supernodal structure).  No reference counterpart: a self-check of the
+ * device pipeline.  out[0] = mismatched elements, out[1] = Schur pairs, out[2] = S blocks;
+ * ms[0] / ms[1] = host / device wall time of the analysis (NULL to skip). */
+sgpu_status sgpu_symbolic_check(const sgpu_var_family* families, int64_t num_families,
+                                const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                int32_t device, int64_t out[3], double ms[2]);
 /* The same order for an optimizer handle. */
 sgpu_status sgpu_optimizer_rcs_order(const sgpu_optimizer* opt, int64_t* perm, int64_t capacity,
                                      int64_t* num_blocks);

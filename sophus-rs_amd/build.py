@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libsophus_gpu.so"
 
-HIP_SOURCES = ["kernels.hip", "pcg.hip", "learned.hip", "batch.hip", "noalloc.hip"]
+HIP_SOURCES = ["kernels.hip", "pcg.hip", "learned.hip", "batch.hip", "noalloc.hip", "symbolic.hip"]
 HOST_SOURCES = ["problem.cpp", "optimizer.cpp", "stage_api.cpp", "generator.cpp", "train.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-result"]

@@ -21,6 +21,7 @@
 #include "optimizer_internal.hpp"
 #include "pcg.hpp"
 #include "problem.hpp"
+#include "symbolic.hpp"
 
 namespace sgpu {
 
@@ -217,7 +218,8 @@ struct sgpu_optimizer {
     params.precond_weights = nullptr;  // copied below; the caller's array is not kept
     if (params.precision != SGPU_PRECISION_F64 && params.precision != SGPU_PRECISION_MIXED)
       throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown precision");
-    P = build_host_problem(families, nf, cfs, nc, eqs, neq);
+    // validation (and, with host_symbolic, the term sort) on the host, before any device call
+    P = build_host_problem_sorted(families, nf, cfs, nc, eqs, neq, params.host_symbolic != 0);
     if (!P.eqs.empty() && (iterative() || params.precision != SGPU_PRECISION_F64))
       throw Error(SGPU_ERR_UNSUPPORTED,
                   "equality constraints are implemented for the direct RCS solver in FP64 (the "
@@ -226,7 +228,6 @@ struct sgpu_optimizer {
       throw Error(SGPU_ERR_UNSUPPORTED,
                   "free intrinsics are implemented in the FP64 precision (the mixed path's centred "
                   "camera tangents do not cover the intrinsics block)");
-    S = analyze(P);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
       throw Error(SGPU_ERR_NO_DEVICE, "no HIP device");
@@ -234,6 +235,15 @@ struct sgpu_optimizer {
     if (device < 0 || device >= ndev) throw Error(SGPU_ERR_NO_DEVICE, "device ordinal out of range");
     SGPU_HIP(hipSetDevice(device));
     SGPU_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // symbolic analysis: the term sort, the S pattern and the pair lists on the device (the
+    // pairs stay there), or all of it on the host
+    int32_t* d_pairs = nullptr;
+    if (params.host_symbolic) {
+      S = analyze(P);
+    } else {
+      sort_terms_on_device(P, stream);
+      S = analyze_device(P, stream, arena, &d_pairs);
+    }
     SGPU_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_sc), sizeof(DevScalars), hipHostMallocDefault));
     for (auto& e : ev) SGPU_HIP(hipEventCreate(&e));
     for (auto& e : phase_ev) SGPU_HIP(hipEventCreate(&e));
@@ -291,7 +301,7 @@ struct sgpu_optimizer {
       dp.s_row = arena.upload(srow, stream);
     }
     dp.pair_ptr = arena.upload(S.pair_ptr, stream);
-    dp.pairs = arena.upload(S.pairs, stream);
+    dp.pairs = d_pairs ? d_pairs : arena.upload(S.pairs, stream);
     dp.own_chunks = arena.upload(S.own_chunks, stream);
     dp.cam_chunk_ptr = arena.upload(S.cam_chunk_ptr, stream);
     dp.n_own_chunks = int32_t(S.own_chunks.size() / 3);

@@ -28,6 +28,14 @@ bool is_reprojection(int kind) {
 HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_families,
                                const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                                const sgpu_eq_fn* eq_fns, int64_t num_eq_fns) {
+  return build_host_problem_sorted(families, num_families, cost_fns, num_cost_fns, eq_fns,
+                                   num_eq_fns, true);
+}
+
+HostProblem build_host_problem_sorted(const sgpu_var_family* families, int64_t num_families,
+                                      const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                      const sgpu_eq_fn* eq_fns, int64_t num_eq_fns,
+                                      bool sort_terms) {
   HostProblem P;
   if (!families || num_families <= 0) raise(SGPU_ERR_INVALID_ARGUMENT, "no variable families");
   if (!cost_fns || num_cost_fns <= 0) raise(SGPU_ERR_INVALID_ARGUMENT, "no cost functions");
@@ -165,11 +173,13 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
       std::iota(ord.begin(), ord.end(), 0);
       const int64_t* E = c.entity_indices;
       const int k1 = intr_free ? 0 : 1, k2 = intr_free ? 1 : 0;
-      std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
-        if (E[3 * x + 2] != E[3 * y + 2]) return E[3 * x + 2] < E[3 * y + 2];
-        if (E[3 * x + k1] != E[3 * y + k1]) return E[3 * x + k1] < E[3 * y + k1];
-        return E[3 * x + k2] < E[3 * y + k2];
-      });
+      P.terms_sorted = sort_terms;
+      if (sort_terms)
+        std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) {
+          if (E[3 * x + 2] != E[3 * y + 2]) return E[3 * x + 2] < E[3 * y + 2];
+          if (E[3 * x + k1] != E[3 * y + k1]) return E[3 * x + k1] < E[3 * y + k1];
+          return E[3 * x + k2] < E[3 * y + k2];
+        });
       P.obs_pose.resize(nt);
       P.obs_point.resize(nt);
       P.obs_intr.resize(nt);
@@ -262,6 +272,7 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
   }
   if (int64_t(P.eqs.size()) > kMaxEqSets)
     raise(SGPU_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxEqSets) + " equality-constraint sets");
+  if (!P.has_reproj) P.terms_sorted = true;
   P.num_pose_blocks = P.fams[P.f_pose].num_active;
   P.intr_block = P.intr_member >= 0 ? int32_t(P.num_pose_blocks) : -1;
   P.num_free_blocks = P.num_pose_blocks + (P.intr_member >= 0 ? 1 : 0);
@@ -282,6 +293,25 @@ HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_fami
     }
   }
   return P;
+}
+
+void apply_term_order(HostProblem& P, const std::vector<int32_t>& ord) {
+  const size_t n = ord.size();
+  std::vector<int32_t> pose(n), point(n), intr(n);
+  std::vector<double> uv(2 * n);
+  for (size_t t = 0; t < n; ++t) {
+    const int32_t o = ord[t];
+    pose[t] = P.obs_pose[o];
+    point[t] = P.obs_point[o];
+    intr[t] = P.obs_intr[o];
+    uv[2 * t] = P.obs_uv[2 * size_t(o)];
+    uv[2 * t + 1] = P.obs_uv[2 * size_t(o) + 1];
+  }
+  P.obs_pose.swap(pose);
+  P.obs_point.swap(point);
+  P.obs_intr.swap(intr);
+  P.obs_uv.swap(uv);
+  P.terms_sorted = true;
 }
 
 // ---- nested dissection ----------------------------------------------------------------
@@ -469,8 +499,8 @@ std::vector<int32_t> nested_dissection(int32_t n, const std::vector<std::vector<
 
 // ---- symbolic analysis -------------------------------------------------------------------
 
-Symbolic analyze(const HostProblem& P) {
-  Symbolic S;
+void analyze_points(const HostProblem& P, Symbolic& S) {
+  if (!P.terms_sorted) throw Error(SGPU_ERR_INVALID_ARGUMENT, "terms not sorted");
   const Family& poses = P.fams[P.f_pose];
   const Family& points = P.fams[P.f_point];
   S.nb = int32_t(P.num_free_blocks);
@@ -511,9 +541,45 @@ Symbolic analyze(const HostProblem& P) {
       if (c >= 0) S.cam_obs[fill[c]++] = o;
     }
   }
-  // S pattern: diagonal, co-observed pairs of active points, pose-graph pairs
+}
+
+std::vector<std::pair<int32_t, int32_t>> pattern_extras(const HostProblem& P, const Symbolic& S) {
+  const Family& poses = P.fams[P.f_pose];
+  std::vector<std::pair<int32_t, int32_t>> out;
+  for (int32_t c = 0; c < S.nb; ++c) out.emplace_back(c, c);
+  for (const auto& g : P.pg)
+    for (size_t t = 0; t < g.m.size(); ++t) {
+      const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
+      if (a >= 0 && b >= 0) out.emplace_back(std::max(a, b), std::min(a, b));
+    }
+  // free intrinsics: the last row, coupled to every pose observed through them
+  if (P.intr_block >= 0)
+    for (int32_t o = 0; o < S.nobs; ++o) {
+      const int32_t c = poses.block[P.obs_pose[o]];
+      if (c >= 0 && P.obs_intr[o] == P.intr_member) out.emplace_back(P.intr_block, c);
+    }
+  return out;
+}
+
+void analyze_chunks(Symbolic& S) {
+  S.cam_chunk_ptr.assign(S.nb + 1, 0);
+  S.own_chunks.clear();
+  for (int32_t c = 0; c < S.nb; ++c) {
+    for (int32_t q = S.cam_ptr[c]; q < S.cam_ptr[c + 1]; q += kOwnChunk)
+      S.own_chunks.insert(S.own_chunks.end(), {c, q, std::min(q + kOwnChunk, S.cam_ptr[c + 1])});
+    S.cam_chunk_ptr[c + 1] = int32_t(S.own_chunks.size() / 3);
+  }
+}
+
+Symbolic analyze(const HostProblem& P) {
+  Symbolic S;
+  analyze_points(P, S);
+  const Family& poses = P.fams[P.f_pose];
+  const Family& points = P.fams[P.f_point];
+  // S pattern: diagonal, co-observed pairs of active points, pose-graph pairs, the free
+  // intrinsics' row
   std::vector<std::vector<int32_t>> rows(S.nb);
-  for (int32_t c = 0; c < S.nb; ++c) rows[c].push_back(c);
+  for (const auto& rc : pattern_extras(P, S)) rows[rc.first].push_back(rc.second);
   std::vector<int32_t> cams;
   for (int32_t b = 0; b < S.npt; ++b) {
     if (points.block[b] < 0) continue;
@@ -523,19 +589,9 @@ Symbolic analyze(const HostProblem& P) {
       if (c >= 0 && (cams.empty() || cams.back() != c)) cams.push_back(c);
     }
     for (size_t i = 0; i < cams.size(); ++i)
-      for (size_t j = 0; j < i; ++j) rows[cams[i]].push_back(cams[j]);
+      for (size_t j = 0; j < i; ++j)
+        rows[std::max(cams[i], cams[j])].push_back(std::min(cams[i], cams[j]));
   }
-  for (const auto& g : P.pg)
-    for (size_t t = 0; t < g.m.size(); ++t) {
-      const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
-      if (a >= 0 && b >= 0) rows[std::max(a, b)].push_back(std::min(a, b));
-    }
-  // free intrinsics: the last row, coupled to every pose observed through them
-  if (P.intr_block >= 0)
-    for (int32_t o = 0; o < S.nobs; ++o) {
-      const int32_t c = poses.block[P.obs_pose[o]];
-      if (c >= 0 && P.obs_intr[o] == P.intr_member) rows[P.intr_block].push_back(c);
-    }
   S.s_ptr.assign(S.nb + 1, 0);
   for (int32_t c = 0; c < S.nb; ++c) {
     auto& r = rows[c];
@@ -550,13 +606,7 @@ Symbolic analyze(const HostProblem& P) {
   // camera are summed per chunk of its camera-major list
   std::vector<int32_t> cam_pos(S.nobs, -1);
   for (int32_t q = 0; q < int32_t(S.cam_obs.size()); ++q) cam_pos[S.cam_obs[q]] = q;
-  S.cam_chunk_ptr.assign(S.nb + 1, 0);
-  S.own_chunks.clear();
-  for (int32_t c = 0; c < S.nb; ++c) {
-    for (int32_t q = S.cam_ptr[c]; q < S.cam_ptr[c + 1]; q += kOwnChunk)
-      S.own_chunks.insert(S.own_chunks.end(), {c, q, std::min(q + kOwnChunk, S.cam_ptr[c + 1])});
-    S.cam_chunk_ptr[c + 1] = int32_t(S.own_chunks.size() / 3);
-  }
+  analyze_chunks(S);
   {
     const int64_t nslot = int64_t(S.s_col.size());
     auto slot_of = [&](int32_t r, int32_t c) -> int64_t {
@@ -593,6 +643,12 @@ Symbolic analyze(const HostProblem& P) {
     }
   }
   S.num_schur_pairs = int64_t(S.pairs.size() / 2) + S.cam_ptr[S.nb];
+  analyze_finish(P, S);
+  return S;
+}
+
+void analyze_finish(const HostProblem& P, Symbolic& S) {
+  const Family& poses = P.fams[P.f_pose];
   if (P.intr_block >= 0) {
     const int32_t r = P.intr_block;
     S.intr_slot.assign(S.nb, -1);
@@ -622,7 +678,6 @@ Symbolic analyze(const HostProblem& P) {
     }
     S.pg_dest.push_back(std::move(d));
   }
-  return S;
 }
 
 void analyze_rcs(Symbolic& S) {

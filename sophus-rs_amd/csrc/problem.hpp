@@ -57,6 +57,7 @@ struct HostProblem {
   std::vector<double> obs_uv;
   std::vector<PoseGraphFn> pg;
   std::vector<EqSet> eqs;  // non-empty equality-constraint sets on the pose family
+  bool terms_sorted = true;  // false: the reprojection terms are still in the caller's order
   int64_t num_free_blocks = 0, num_marg_blocks = 0;
   std::vector<int32_t> cam_member;  // free block -> pose member (-1: the intrinsics block)
   // free block -> its index in the reference's layout (Free families in name order, active
@@ -68,6 +69,13 @@ struct HostProblem {
 HostProblem build_host_problem(const sgpu_var_family* families, int64_t num_families,
                                const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
                                const sgpu_eq_fn* eq_fns = nullptr, int64_t num_eq_fns = 0);
+// build_host_problem with the term sort optional (sort_terms = false keeps the caller's order
+// for a sort elsewhere, applied with apply_term_order: ord[sorted] = caller's index).
+HostProblem build_host_problem_sorted(const sgpu_var_family* families, int64_t num_families,
+                                      const sgpu_cost_fn* cost_fns, int64_t num_cost_fns,
+                                      const sgpu_eq_fn* eq_fns, int64_t num_eq_fns,
+                                      bool sort_terms);
+void apply_term_order(HostProblem& P, const std::vector<int32_t>& ord);
 // Most equality-constraint sets one handle takes (the KKT block is solved densely per step).
 constexpr int32_t kMaxEqSets = 16;
 
@@ -168,6 +176,15 @@ struct Symbolic {
 };
 
 Symbolic analyze(const HostProblem& P);
+// The steps of analyze(): point-major arrays, reduced-term counts and camera-major lists; the
+// S pattern entries that do not come from shared points (diagonal, pose-graph pairs, the free
+// intrinsics' row) as (row, column) with column <= row; the own-term chunks; and, once the
+// pattern and the pairs are in S, the intrinsics' slots, the factor structure (analyze_rcs)
+// and the pose-graph destinations.
+void analyze_points(const HostProblem& P, Symbolic& S);
+std::vector<std::pair<int32_t, int32_t>> pattern_extras(const HostProblem& P, const Symbolic& S);
+void analyze_chunks(Symbolic& S);
+void analyze_finish(const HostProblem& P, Symbolic& S);
 
 // Ordering, elimination tree, supernodes, fronts and the launch schedule of the RCS
 // whose lower pattern is S.nb, S.s_ptr, S.s_col (camera order).

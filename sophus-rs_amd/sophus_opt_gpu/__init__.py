@@ -85,7 +85,7 @@ class _OptParams(C.Structure):
         ("pcg_relative_tolerance", C.c_double),
         ("pcg_max_iterations", C.c_int64),
         ("device", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("host_symbolic", C.c_int32),
         ("precond_weights", C.POINTER(C.c_float)),
         ("pcg_forcing", C.c_double),
     ]
@@ -219,6 +219,8 @@ _SIGNATURES = {
                                C.POINTER(_RcsStats)]),
     "sgpu_analyze_rcs_order": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
                                          _I64P, C.c_int64, _I64P]),
+    "sgpu_symbolic_check": (C.c_int, [C.POINTER(_VarFamily), C.c_int64, C.POINTER(_CostFn), C.c_int64,
+                                      C.c_int32, _I64P, _DP]),
     "sgpu_optimizer_inverse_block": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_char_p, C.c_int64, _DP,
                                                C.c_int64]),
     "sgpu_optimizer_covariance_block": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.c_char_p, C.c_int64,
@@ -257,7 +259,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 7  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 8  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -517,6 +519,7 @@ class OptParams:
     device: int = 0
     precond_weights: Optional[np.ndarray] = None  # SchurPcgLearned: PRECOND_WEIGHTS float32
     pcg_forcing: float = 0.0  # PCG: forcing-sequence CG tolerance (0: off)
+    host_symbolic: bool = False  # build the term sort, S pattern and pair lists on the host
 
     def _c(self) -> _OptParams:
         p = _OptParams()
@@ -535,6 +538,7 @@ class OptParams:
         p.pcg_max_iterations = int(self.pcg_max_iterations)
         p.device = int(self.device)
         p.pcg_forcing = float(self.pcg_forcing)
+        p.host_symbolic = int(bool(self.host_symbolic))
         if self.precond_weights is not None:
             w = np.ascontiguousarray(self.precond_weights, dtype=np.float32).reshape(-1)
             if w.size != PRECOND_WEIGHTS:
@@ -798,6 +802,18 @@ def analyze_rcs_order(variables: VarFamilies, cost_fns: Sequence[CostFn]) -> np.
     _check(library().sgpu_analyze_rcs_order(m.fams, len(m.names), m.costs, len(cost_fns),
                                             out.ctypes.data_as(_I64P), out.size, C.byref(n)))
     return out
+
+
+def symbolic_check(variables: VarFamilies, cost_fns: Sequence[CostFn], device: int = 0) -> dict:
+    """The host and the device symbolic analysis compared element by element (needs a GPU):
+    mismatches, Schur pairs, S blocks and the two wall times in ms."""
+    m = _Marshal(variables, cost_fns)
+    out = np.zeros(3, dtype=np.int64)
+    ms = np.zeros(2, dtype=np.float64)
+    _check(library().sgpu_symbolic_check(m.fams, len(m.names), m.costs, len(cost_fns), int(device),
+                                         out.ctypes.data_as(_I64P), ms.ctypes.data_as(_DP)))
+    return {"mismatches": int(out[0]), "schur_pairs": int(out[1]), "s_blocks": int(out[2]),
+            "host_ms": float(ms[0]), "device_ms": float(ms[1])}
 
 
 def optimize_nlls(variables: VarFamilies, cost_fns: Sequence[CostFn],
