@@ -59,12 +59,13 @@ def host_threads() -> int:
 
 
 def oracle_step_seconds(oracle, problem, threads: int, iterations: int = 1, families=None,
-                        costs=None, eqs=None) -> float:
+                        costs=None, eqs=None, solver: int = 0) -> float:
     """Wall time inside the LM steps of `iterations` FP64-oracle iterations at `threads`."""
     oracle.set_threads(threads)
     try:
         oracle.optimize_nlls(families or oracle.ba_families(problem), costs or oracle.ba_costs(problem),
-                             oracle.default_params(num_iterations=iterations, initial_lm_damping=1.0),
+                             oracle.default_params(num_iterations=iterations, initial_lm_damping=1.0,
+                                                   solver=solver),
                              eqs=eqs)
         return oracle.last_step_seconds()
     finally:
@@ -257,14 +258,16 @@ def phase_bandwidth(st, ktimes, n_obs, n_pts, n_cams) -> dict:
 
 def small_workload_leg(sg, index: int, cpu: bool) -> dict:
     """A smaller workload on the FP64 direct path: GPU and (optionally) CPU-oracle time per
-    solve (configs 0: 20 LM iterations; 1: to the reference's termination, at most 300), one
-    warm-up then the median of 3 (the reference bench's protocol, ba_bench.rs:47-65)."""
+    solve (configs 0: 20 LM iterations with SchurSparseLdlt, the RCS solver that workload
+    names; 1: SchurBlockSparseLdlt to the reference's termination, at most 300), one warm-up
+    then the median of 3 (the reference bench's protocol, ba_bench.rs:47-65)."""
     p = sg.generate_ba(sg.config_spec(index))
     iters = 20 if index == 0 else 300
+    solver = sg.Solver.SchurSparseLdlt if index == 0 else sg.Solver.SchurBlockSparseLdlt
     times, hist, term = [], None, None
     for rep in range(4):
         t = time.perf_counter()
-        params = sg.OptParams(num_iterations=iters, initial_lm_damping=1.0)
+        params = sg.OptParams(num_iterations=iters, initial_lm_damping=1.0, solver=solver)
         opt = sg.Optimizer(p.variables(), p.cost_fns(1.0), params)
         t_create = time.perf_counter() - t
         h = []
@@ -279,7 +282,8 @@ def small_workload_leg(sg, index: int, cpu: bool) -> dict:
             times.append(h[-1][0])
         hist, term = h, info.termination.name
     out = {"workload": f"ba_{'ring' if index == 0 else 'walk'}_{p.poses_init.shape[0]}cams_"
-                       f"{p.points_init.shape[0]}pts", "num_obs": p.num_obs, "iterations": len(hist),
+                       f"{p.points_init.shape[0]}pts", "solver": solver.name, "num_obs": p.num_obs,
+           "iterations": len(hist),
            "termination": term, "final_cost": hist[-1][1],
            "gpu_ms_per_solve": 1e3 * float(np.median(times)),
            "gpu_obs_per_s": p.num_obs * len(hist) / float(np.median(times)),
@@ -290,7 +294,8 @@ def small_workload_leg(sg, index: int, cpu: bool) -> dict:
         it_cpu = len(hist) if index == 0 else 1
         out["cpu_oracle"] = []
         for threads in sorted({host_threads(), 1}, reverse=True):
-            ct = [oracle_step_seconds(oracle, p, threads, it_cpu) for _ in range(4 if index == 0 else 1)]
+            ct = [oracle_step_seconds(oracle, p, threads, it_cpu, solver=int(solver))
+                  for _ in range(4 if index == 0 else 1)]
             ct = ct[1:] if index == 0 else ct
             out["cpu_oracle"].append({"threads": threads, "iterations": it_cpu,
                                       "ms": 1e3 * float(np.median(ct)),

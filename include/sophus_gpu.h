@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 8
+#define SGPU_ABI_VERSION 9
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -119,7 +119,11 @@ typedef enum sgpu_solver {
   /* Schur complement + block-Jacobi preconditioned CG on the RCS */
   SGPU_SOLVER_SCHUR_PCG_JACOBI = 1,
   /* Schur complement + PCG with the learned (MLP-corrected) block-Jacobi */
-  SGPU_SOLVER_SCHUR_PCG_LEARNED = 2
+  SGPU_SOLVER_SCHUR_PCG_LEARNED = 2,
+  /* SchurSparseLdlt: Schur complement + scalar sparse LDLt of the RCS (SparseLdlt,
+   * sparse_ldlt.rs:571-650): the same factorization, in the same block order, with the
+   * scalar pivot rule |d_jj| <= max(|d_jj|, 1) * 1e-12 -> 0 (sparse_ldlt.rs:399-445) */
+  SGPU_SOLVER_SCHUR_SPARSE_LDLT = 3
 } sgpu_solver;
 
 typedef enum sgpu_precision {
@@ -459,6 +463,13 @@ sgpu_status sgpu_eval_pose_graph3_terms(int64_t num_terms, const double* world_f
 sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
                                   const int64_t* row_idx, const double* values,
                                   const double* rhs, double* x);
+/* The same input solved as SparseLdlt::factorize + solve_inplace (sparse_ldlt.rs:237-330,
+ * 571-650): the scalar LDLt in the block order, with its per-pivot rule (a pivot with
+ * |d_jj| <= max(|d_jj|, 1) * 1e-12 is zero and its column of L is empty).  The reference's
+ * solve divides by a zero pivot; that component of x is NaN here. */
+sgpu_status sgpu_sparse_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
+                                   const int64_t* row_idx, const double* values,
+                                   const double* rhs, double* x);
 
 /* Allocation-free variant of sgpu_block_ldlt_solve.  It has no counterpart in the
  * reference; it is for callers that cannot allocate device memory (code under HIP graph

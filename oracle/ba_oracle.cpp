@@ -15,6 +15,7 @@
 //   SchurFactor::factorize / solve  crates/sophus_solver/src/ldlt/schur_ldlt.rs:82-182,536-812
 //   BlockSparseLdlt             crates/sophus_solver/src/ldlt/block_sparse_ldlt.rs:96-190,400-747
 //   ldlt_decompose_inplace      crates/sophus_solver/src/ldlt/dense_ldlt.rs:164-256
+//   SparseLdlt (SchurSparseLdlt)  crates/sophus_solver/src/ldlt/sparse_ldlt.rs:121-445,571-650
 //   EliminationTree             crates/sophus_solver/src/ldlt/elimination_tree.rs:20-105
 //
 // Documented differences (rounding-level only):
@@ -1058,11 +1059,137 @@ bool inverse_dense(int n, const double* m, double* out) {
   return true;
 }
 
+// ---- scalar sparse LDLt (sparse_ldlt.rs) ------------------------------------------------
+// SparseLdlt::factorize_impl (:571-650): the block order (faer's AMD on the block graph
+// there, :97-115; here the order block_ldlt_factorize uses) expanded to scalars
+// (scalar_perm_from_block_perm, :121-160), then the left-looking scalar LDLt of the
+// LdltWorkspace (:335-445): column j is loaded from the permuted lower matrix, every
+// earlier column k with L(j, k) != 0 subtracts (L(j, k) d_k) L(i, k) for its rows i >= j,
+// and the pivot is tested on its own, |d_jj| <= max(|d_jj|, 1) tol -> 0 with an empty
+// column, else L(i, j) = c_i / d_jj (append_to_ldlt, :399-445).  The reference visits the
+// columns k in the elimination-tree reach order; here they go in increasing k, which
+// changes only the order of the subtractions.
+struct ScalarLdltFactor {
+  int64_t n = 0;
+  std::vector<int64_t> perm;                                   // scalar perm[new] = old
+  std::vector<std::vector<std::pair<int64_t, double>>> col;    // L(i, j), i > j ascending
+  std::vector<double> d;
+};
+
+bool sparse_ldlt_factorize(const BlockLower& A, ScalarLdltFactor& F, double tol_rel) {
+  const int64_t nb = A.nb;
+  std::vector<int64_t> bperm;
+  {
+    std::vector<std::set<int64_t>> adj(nb);
+    for (int64_t r = 0; r < nb; ++r)
+      for (int64_t k = A.row_ptr[r]; k < A.row_ptr[r + 1]; ++k)
+        if (A.col[k] != r) {
+          adj[r].insert(A.col[k]);
+          adj[A.col[k]].insert(r);
+        }
+    bperm = int64_t(g_rcs_order.size()) == nb ? g_rcs_order : min_degree_order(nb, adj);
+  }
+  const int64_t n = nb == 0 ? 0 : A.start[nb - 1] + A.dim[nb - 1];
+  F.n = n;
+  F.perm.clear();
+  for (int64_t b : bperm)
+    for (int64_t t = 0; t < A.dim[b]; ++t) F.perm.push_back(A.start[b] + t);
+  std::vector<int64_t> inv(n);
+  for (int64_t i = 0; i < n; ++i) inv[F.perm[i]] = i;
+  // permuted lower matrix by column (permute_sparse_lower, :163-205): acol[j] = (i >= j, a)
+  std::vector<std::vector<std::pair<int64_t, double>>> acol(n);
+  for (int64_t r = 0; r < nb; ++r)
+    for (int64_t k = A.row_ptr[r]; k < A.row_ptr[r + 1]; ++k) {
+      const int64_t c = A.col[k], dr = A.dim[r], dc = A.dim[c];
+      const double* B = A.block(k);
+      for (int64_t i = 0; i < dr; ++i)
+        for (int64_t j = 0; j < dc; ++j) {
+          if (r == c && j > i) continue;  // the lower triangle of a diagonal block
+          int64_t pi = inv[A.start[r] + i], pj = inv[A.start[c] + j];
+          if (pi < pj) std::swap(pi, pj);
+          acol[pj].push_back({pi, B[i * dc + j]});
+        }
+    }
+  F.col.assign(n, {});
+  F.d.assign(n, 0.0);
+  std::vector<std::vector<std::pair<int64_t, double>>> lrow(n);  // (k, L(i, k)) per row i
+  std::vector<double> c(n, 0.0);
+  std::vector<char> touched(n, 0);
+  std::vector<int64_t> rows;
+  for (int64_t j = 0; j < n; ++j) {
+    rows.clear();
+    for (const auto& [i, v] : acol[j])  // load_column (:344-357)
+      if (!touched[i]) {
+        c[i] = v;
+        touched[i] = 1;
+        rows.push_back(i);
+      }
+    for (const auto& [k, ljk] : lrow[j]) {  // apply_to_col_k_in_reach (:359-397)
+      const double ljk_dk = ljk * F.d[k];
+      for (const auto& [i, lik] : F.col[k]) {
+        if (i < j) continue;
+        const double delta = ljk_dk * lik;
+        if (!touched[i]) {
+          c[i] = -delta;
+          touched[i] = 1;
+          rows.push_back(i);
+        } else {
+          c[i] -= delta;
+        }
+      }
+    }
+    const double djj_raw = touched[j] ? c[j] : 0.0;
+    if (!std::isfinite(djj_raw)) return false;  // NonFinitePivot
+    const double tau = std::max(std::fabs(djj_raw), 1.0) * tol_rel;
+    const double djj = std::fabs(djj_raw) <= tau ? 0.0 : djj_raw;
+    F.d[j] = djj;
+    if (djj != 0.0) {
+      std::sort(rows.begin(), rows.end());
+      for (int64_t i : rows)
+        if (i > j) {
+          const double lij = c[i] / djj;
+          if (lij != 0.0) {
+            F.col[j].push_back({i, lij});
+            lrow[i].push_back({j, lij});
+          }
+        }
+    }
+    for (int64_t i : rows) {
+      touched[i] = 0;
+      c[i] = 0.0;
+    }
+  }
+  FactorStat stat{0, 0, DBL_MAX};
+  for (double v : F.d) {
+    if (v == 0.0) ++stat.zero;
+    if (v < 0.0) ++stat.neg;
+  }
+  g_factor_stats.push_back(stat);
+  return true;
+}
+
+// SparseLdltFactor::solve_inplace (:252-330): L y = P b, z = y / d (a zero pivot divides by
+// zero there), L^T x = z, x <- P^T x.
+void sparse_ldlt_solve(const ScalarLdltFactor& F, std::vector<double>& x) {
+  const int64_t n = F.n;
+  std::vector<double> b(n);
+  for (int64_t i = 0; i < n; ++i) b[i] = x[F.perm[i]];
+  for (int64_t j = 0; j < n; ++j)
+    for (const auto& [i, l] : F.col[j]) b[i] -= l * b[j];
+  for (int64_t i = 0; i < n; ++i) b[i] /= F.d[i];
+  for (int64_t j = n - 1; j >= 0; --j)
+    for (const auto& [i, l] : F.col[j]) b[j] -= l * b[i];
+  for (int64_t i = 0; i < n; ++i) x[F.perm[i]] = b[i];
+}
+
 // ---- Schur complement solve (schur_ldlt.rs:82-182, :536-812) -------------------------------
 // H: all active blocks; the first nfb blocks are free.  g: full rhs (neg_grad).
 // Returns false on SingularMargBlock or a factorization failure.
+// scalar_rule: the RCS is factored by SparseLdlt (SchurSparseLdlt) instead of
+// BlockSparseLdlt (SchurBlockSparseLdlt).
 bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
-                 std::vector<double>& dx, double tol_rel = 1e-12, const Kkt* kkt = nullptr) {
+                 std::vector<double>& dx, double tol_rel = 1e-12, const Kkt* kkt = nullptr,
+                 bool scalar_rule = false) {
   const int64_t nb = H.nb;
   const int64_t nfs = nfb == 0 ? 0 : H.start[nfb - 1] + H.dim[nfb - 1];
   // S pattern: H_ff pattern plus every (f_i, f_j) pair sharing a marg block.
@@ -1172,7 +1299,15 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
     });
   }
   BlockLdltFactor F;
-  if (!block_ldlt_factorize(S, F, tol_rel)) return false;
+  ScalarLdltFactor Fs;
+  if (scalar_rule ? !sparse_ldlt_factorize(S, Fs, tol_rel) : !block_ldlt_factorize(S, F, tol_rel))
+    return false;
+  auto solve_s = [&](std::vector<double>& v) {
+    if (scalar_rule)
+      sparse_ldlt_solve(Fs, v);
+    else
+      block_ldlt_solve(F, S, v);
+  };
   std::vector<double> dxf = rhs;
   std::vector<double> dlam;
   if (kkt && kkt->nc > 0) {
@@ -1182,7 +1317,7 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
     std::vector<std::vector<double>> Y(nc);
     for (int64_t k = 0; k < nc; ++k) {
       Y[k].assign(kkt->G.begin() + k * nfs, kkt->G.begin() + (k + 1) * nfs);
-      block_ldlt_solve(F, S, Y[k]);
+      solve_s(Y[k]);
     }
     std::vector<double> M(size_t(nc * nc)), Mi(size_t(nc * nc));
     for (int64_t a = 0; a < nc; ++a)
@@ -1193,7 +1328,7 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
       }
     if (!inverse_dense(int(nc), M.data(), Mi.data())) return false;  // SingularKktConstraint
     std::vector<double> u = rhs;
-    block_ldlt_solve(F, S, u);
+    solve_s(u);
     std::vector<double> w(nc);
     for (int64_t a = 0; a < nc; ++a) {
       double acc = 0;
@@ -1206,7 +1341,7 @@ bool schur_solve(const BlockLower& H, int64_t nfb, const std::vector<double>& g,
     for (int64_t i = 0; i < nfs; ++i)
       for (int64_t a = 0; a < nc; ++a) dxf[i] -= kkt->G[size_t(a * nfs + i)] * dlam[a];
   }
-  block_ldlt_solve(F, S, dxf);
+  solve_s(dxf);
   dx.assign(g.size() + dlam.size(), 0.0);
   std::copy(dxf.begin(), dxf.end(), dx.begin());
   std::copy(dlam.begin(), dlam.end(), dx.begin() + g.size());
@@ -1425,7 +1560,8 @@ int step(Optimizer& O, StepRecord& rec) {
     done = true;
   }
   std::vector<double> dx;
-  if (!done && !schur_solve(O.L.H, V.num_free_blocks, O.L.neg_grad, dx, 1e-12, P.eqs.empty() ? nullptr : &K)) {
+  if (!done && !schur_solve(O.L.H, V.num_free_blocks, O.L.neg_grad, dx, 1e-12, P.eqs.empty() ? nullptr : &K,
+                            O.params.solver == SGPU_SOLVER_SCHUR_SPARSE_LDLT)) {
     O.nu *= 2.0;  // handle_solve_failure -> reject_step
     ++O.rejects;
     if (O.rejects >= O.params.max_consecutive_rejects) reason = SGPU_TERM_MAX_CONSECUTIVE_REJECTS;
@@ -1725,8 +1861,9 @@ void oracle_schur_solve_dense(int64_t nb, const int64_t* dims, int64_t nfb, cons
 
 // Block-sparse LDLt solve (factorize_impl + solve_inplace) of a symmetric matrix given
 // dense (n x n) with the block partition dims[nb]; structurally zero blocks are skipped.
-int32_t oracle_block_ldlt_solve_dense(int64_t nb, const int64_t* dims, const double* Ad,
-                                      const double* b, double* x) {
+// scalar_rule: SparseLdlt's factorize + solve_inplace on the same input instead.
+static int32_t direct_solve_dense(int64_t nb, const int64_t* dims, const double* Ad,
+                                  const double* b, double* x, bool scalar_rule) {
   BlockLower A;
   A.nb = nb;
   A.dim.assign(dims, dims + nb);
@@ -1753,12 +1890,28 @@ int32_t oracle_block_ldlt_solve_dense(int64_t nb, const int64_t* dims, const dou
         for (int64_t j = 0; j < dims[c]; ++j)
           A.block(k)[i * dims[c] + j] = Ad[(A.start[r] + i) * n + A.start[c] + j];
     }
-  BlockLdltFactor F;
-  if (!block_ldlt_factorize(A, F, 1e-12)) return 1;
   std::vector<double> v(b, b + n);
-  block_ldlt_solve(F, A, v);
+  if (scalar_rule) {
+    ScalarLdltFactor F;
+    if (!sparse_ldlt_factorize(A, F, 1e-12)) return 1;
+    sparse_ldlt_solve(F, v);
+  } else {
+    BlockLdltFactor F;
+    if (!block_ldlt_factorize(A, F, 1e-12)) return 1;
+    block_ldlt_solve(F, A, v);
+  }
   std::copy(v.begin(), v.end(), x);
   return 0;
+}
+
+int32_t oracle_block_ldlt_solve_dense(int64_t nb, const int64_t* dims, const double* Ad,
+                                      const double* b, double* x) {
+  return direct_solve_dense(nb, dims, Ad, b, x, false);
+}
+
+int32_t oracle_sparse_ldlt_solve_dense(int64_t nb, const int64_t* dims, const double* Ad,
+                                       const double* b, double* x) {
+  return direct_solve_dense(nb, dims, Ad, b, x, true);
 }
 
 // ldlt_decompose_inplace on a dense n x n matrix (row-major).  definiteness: 0 PD,

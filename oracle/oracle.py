@@ -103,6 +103,8 @@ def lib(variant: str = "") -> C.CDLL:
         L.oracle_schur_solve_dense.argtypes = [C.c_int64, I64P, C.c_int64, DP, DP, DP, C.POINTER(C.c_int32)]
         L.oracle_block_ldlt_solve_dense.restype = C.c_int32
         L.oracle_block_ldlt_solve_dense.argtypes = [C.c_int64, I64P, DP, DP, DP]
+        L.oracle_sparse_ldlt_solve_dense.restype = C.c_int32
+        L.oracle_sparse_ldlt_solve_dense.argtypes = [C.c_int64, I64P, DP, DP, DP]
         L.oracle_dense_ldlt.restype = C.c_int32
         L.oracle_dense_ldlt.argtypes = [C.c_int32, DP, C.c_double, DP, DP, C.POINTER(C.c_int32),
                                         C.POINTER(C.c_int32), DP]
@@ -354,6 +356,20 @@ def block_ldlt_solve_dense(A: np.ndarray, dims: Sequence[int], b: np.ndarray) ->
     x = np.zeros(A.shape[0])
     if lib().oracle_block_ldlt_solve_dense(len(dims), d.ctypes.data_as(I64P), _p(A), _p(b), _p(x)):
         return None
+    return x
+
+
+def sparse_ldlt_solve_dense(A: np.ndarray, dims: Sequence[int], b: np.ndarray) -> Optional[np.ndarray]:
+    """SparseLdlt (sparse_ldlt.rs:571-650, 252-330) on A given dense with the block partition
+    dims: the scalar LDLt in the block order, each pivot tested on its own; a zero pivot makes
+    the solve divide by zero."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    d = np.ascontiguousarray(dims, dtype=np.int64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(A.shape[0])
+    with np.errstate(all="ignore"):
+        if lib().oracle_sparse_ldlt_solve_dense(len(dims), d.ctypes.data_as(I64P), _p(A), _p(b), _p(x)):
+            return None
     return x
 
 

@@ -18,6 +18,13 @@ __device__ __forceinline__ double recip_nr(double x) {
 // The off-diagonal quotients use each pivot's reciprocal (formed once); dinv, if given,
 // receives those reciprocals (0 for a zero pivot).  Row j keeps w_q = l_jq d_q, so each
 // inner term is one multiply-add.
+//
+// kScalarRule: the pivot rule of the scalar SparseLdlt instead (append_to_ldlt,
+// sparse_ldlt.rs:399-445): each pivot is tested on its own, |d_jj| <= max(|d_jj|, 1) tol,
+// and a kept pivot's column of L is never zeroed later.  Eliminating a 6x6 block column by
+// column is the scalar factorization of those six columns, so with this rule the block
+// factorization is SparseLdlt's in the same (block) order.
+template <bool kScalarRule = false>
 __device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv = nullptr) {
   double a[36], rd[6], w[36];
   for (int k = 0; k < 36; ++k) a[k] = A[k];
@@ -27,14 +34,18 @@ __device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv
     for (int k = 0; k < j; ++k) {
       double s = a[6 * j + k];
       for (int q = 0; q < k; ++q) s -= w[6 * j + q] * a[6 * k + q];
-      const double zt = fmax(max_abs_pivot, 1.0) * tol;
-      a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s * rd[k];
+      if constexpr (kScalarRule) {
+        a[6 * j + k] = d[k] == 0.0 ? 0.0 : s * rd[k];
+      } else {
+        const double zt = fmax(max_abs_pivot, 1.0) * tol;
+        a[6 * j + k] = fabs(d[k]) <= zt ? 0.0 : s * rd[k];
+      }
       w[6 * j + k] = a[6 * j + k] * d[k];
     }
     double djj = a[6 * j + j];
     for (int q = 0; q < j; ++q) djj -= w[6 * j + q] * a[6 * j + q];
     if (!isfinite(djj)) return false;
-    const double zt = fmax(max_abs_pivot, 1.0) * tol;
+    const double zt = (kScalarRule ? fmax(fabs(djj), 1.0) : fmax(max_abs_pivot, 1.0)) * tol;
     if (fabs(djj) <= zt) {
       d[j] = 0.0;
       rd[j] = 0.0;

@@ -921,7 +921,8 @@ __device__ void factor_panel_diag(const DevFactor& f, const DiagNode& dn, double
   // thread 0 factors the 6x6 block a6 into slot s
   auto factor_pivot = [&](const double* a6, int s) {
     double L[36], d[6], di[6];
-    if (!ldlt6(a6, L, d, di)) {
+    const bool ok = f.scalar_pivots ? ldlt6<true>(a6, L, d, di) : ldlt6<false>(a6, L, d, di);
+    if (!ok) {
       atomicOr(&sc->bad_pivot, 1);
       for (int k = 0; k < 36; ++k) L[k] = (k % 7 == 0) ? 1.0 : 0.0;
       for (int k = 0; k < 6; ++k) d[k] = di[k] = 0.0;
@@ -2094,6 +2095,21 @@ __global__ void k_permute_out(int nb, const int32_t* __restrict__ perm, const do
   if (e < 6 * nb) dst[6 * perm[e / 6] + e % 6] = src[e];
 }
 
+// k_permute_out for SparseLdlt's pivot rule.  Its solve divides by every pivot
+// (sparse_ldlt.rs:283-286), so a zero pivot's component is y / 0 there; it is written as
+// NaN here.  That is the reference's result for a pivot of a block nothing couples to (a
+// free pose no term touches: y = 0, and 0 / 0 is NaN, while its empty rows and columns of L
+// keep it out of every other component).  Where a coupled variable's pivot is zero, the
+// reference's y / 0 may be +-inf and its back-substitution carries it into the components
+// coupled to it; those stay finite here.
+__global__ void k_permute_out_scalar_rule(int nb, const int32_t* __restrict__ perm,
+                                          const double* __restrict__ piv_d,
+                                          const double* __restrict__ src, double* __restrict__ dst) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < 6 * nb)
+    dst[6 * perm[e / 6] + e % 6] = piv_d[e] == 0.0 ? __builtin_nan("") : src[e];
+}
+
 __global__ void k_scatter_s(DevProblem p, int64_t nslots, const double* __restrict__ blocks,
                             double* __restrict__ front) {
   const int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -3102,7 +3118,11 @@ void launch_rcs_solve(hipStream_t s, const DevFactor& f, double* y, double* dx_c
     hipLaunchKernelGGL(k_mf_bwd_tri, dim3(n), dim3(tri_threads(S, l)), 0, s, f,
                        f.lvl_sn + S.lvl_ptr[l], y);
   }
-  hipLaunchKernelGGL(k_permute_out, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, y, dx_cam);
+  if (f.scalar_pivots)
+    hipLaunchKernelGGL(k_permute_out_scalar_rule, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, f.piv_d,
+                       y, dx_cam);
+  else
+    hipLaunchKernelGGL(k_permute_out, dim3(g), dim3(kThreads), 0, s, f.nb, f.perm, y, dx_cam);
 }
 
 void launch_rcs_forward(hipStream_t s, const DevFactor& f, const double* b_nat, double* b_perm,

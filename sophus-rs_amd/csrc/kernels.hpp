@@ -117,6 +117,9 @@ struct DevFactor {
   const Symbolic* host;  // launch schedule
   const int32_t* sn_child_ptr;  // children of each node (standalone forward solve; may be null
   const int32_t* sn_child;      // where that solve is not used)
+  // pivot rule: 0 the 6x6 block rule of BlockSparseLdlt (dense_ldlt.rs:164-256), 1 the
+  // scalar rule of SparseLdlt (sparse_ldlt.rs:399-445)
+  int32_t scalar_pivots;
 };
 
 constexpr int kPtRec = 18;  // doubles per packed point record (DevLin::prec)

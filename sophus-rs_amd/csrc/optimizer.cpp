@@ -152,7 +152,9 @@ struct sgpu_optimizer {
     }
   }
 
-  bool iterative() const { return params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT; }
+  bool iterative() const {
+    return params.solver == SGPU_SOLVER_SCHUR_PCG_JACOBI || params.solver == SGPU_SOLVER_SCHUR_PCG_LEARNED;
+  }
 
   const Family& pose_fam() const { return P.fams[P.f_pose]; }
   const Family& point_fam() const { return P.fams[P.f_point]; }
@@ -205,7 +207,7 @@ struct sgpu_optimizer {
   void init(const sgpu_var_family* families, int64_t nf, const sgpu_cost_fn* cfs, int64_t nc,
             const sgpu_eq_fn* eqs, int64_t neq, const sgpu_opt_params* prm) {
     params = *prm;
-    if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT &&
+    if (params.solver != SGPU_SOLVER_SCHUR_BLOCK_SPARSE_LDLT && params.solver != SGPU_SOLVER_SCHUR_SPARSE_LDLT &&
         params.solver != SGPU_SOLVER_SCHUR_PCG_JACOBI && params.solver != SGPU_SOLVER_SCHUR_PCG_LEARNED)
       throw Error(SGPU_ERR_UNSUPPORTED, "solver not available");
     if (iterative() && !(params.pcg_relative_tolerance > 0.0 && params.pcg_max_iterations > 0))
@@ -341,6 +343,7 @@ struct sgpu_optimizer {
     }
     // factor structure
     df.nb = S.nb;
+    df.scalar_pivots = params.solver == SGPU_SOLVER_SCHUR_SPARSE_LDLT ? 1 : 0;
     df.nsn = S.nsn;
     df.perm = arena.upload(S.perm, stream);
     df.sn_c0 = arena.upload(S.sn_c0, stream);

@@ -200,9 +200,10 @@ extern "C" sgpu_status sgpu_eval_pose_graph3_terms(int64_t n, const double* m, c
   });
 }
 
-extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
-                                             const int64_t* row_idx, const double* values,
-                                             const double* rhs, double* x) {
+namespace {
+
+sgpu_status direct_block_solve(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,
+                               const double* values, const double* rhs, double* x, bool scalar_rule) {
   return guarded([&] {
     Symbolic S;
     std::vector<double> blocks;
@@ -211,8 +212,23 @@ extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* 
     require_device();
     DeviceBufs bufs;
     BlockSolveBufs b = bind_block_solve(S, blocks, rhs, bufs);
+    b.df.scalar_pivots = scalar_rule ? 1 : 0;
     run_block_solve(S, b, x);
   });
+}
+
+}  // namespace
+
+extern "C" sgpu_status sgpu_block_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
+                                             const int64_t* row_idx, const double* values,
+                                             const double* rhs, double* x) {
+  return direct_block_solve(num_blocks, col_ptr, row_idx, values, rhs, x, false);
+}
+
+extern "C" sgpu_status sgpu_sparse_ldlt_solve(int64_t num_blocks, const int64_t* col_ptr,
+                                              const int64_t* row_idx, const double* values,
+                                              const double* rhs, double* x) {
+  return direct_block_solve(num_blocks, col_ptr, row_idx, values, rhs, x, true);
 }
 
 sgpu_status sgpu_block_pcg_solve(int64_t num_blocks, const int64_t* col_ptr, const int64_t* row_idx,

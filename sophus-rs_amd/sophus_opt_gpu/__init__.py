@@ -238,6 +238,7 @@ _SIGNATURES = {
     "sgpu_eval_pose_graph3_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve_noalloc": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
+    "sgpu_sparse_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_block_pcg_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP, C.c_double, C.c_int64,
                                        _I64P]),
     "sgpu_precond_features": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _FP]),
@@ -259,7 +260,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 8  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 9  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -484,6 +485,7 @@ class Solver(enum.IntEnum):
     SchurBlockSparseLdlt = 0
     SchurPcgJacobi = 1
     SchurPcgLearned = 2
+    SchurSparseLdlt = 3  # the same factorization with SparseLdlt's scalar pivot rule
 
 
 class Precision(enum.IntEnum):
@@ -917,6 +919,18 @@ def block_ldlt_solve(A_dense: np.ndarray, b: np.ndarray, noalloc: bool = False) 
     fn = library().sgpu_block_ldlt_solve_noalloc if noalloc else library().sgpu_block_ldlt_solve
     _check(fn(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P), vals.ctypes.data_as(_DP),
               rhs.ctypes.data_as(_DP), x.ctypes.data_as(_DP)))
+    return x
+
+
+def sparse_ldlt_solve(A_dense: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """SparseLdlt::factorize + solve_inplace (the scalar pivot rule) of a symmetric matrix with
+    6x6 blocks on the GPU; a zero pivot's component is NaN (the reference divides by it)."""
+    nb, col_ptr, rows, vals = _lower_blocks(A_dense)
+    rhs = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+    x = np.empty(6 * nb)
+    _check(library().sgpu_sparse_ldlt_solve(nb, col_ptr.ctypes.data_as(_I64P), rows.ctypes.data_as(_I64P),
+                                            vals.ctypes.data_as(_DP), rhs.ctypes.data_as(_DP),
+                                            x.ctypes.data_as(_DP)))
     return x
 
 
