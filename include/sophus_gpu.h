@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 9
+#define SGPU_ABI_VERSION 10
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -584,6 +584,15 @@ sgpu_status sgpu_batch_create(const sgpu_batch_desc* desc, const sgpu_opt_params
  * per iteration; use_graph = 1 replays them from a captured hipGraph). */
 sgpu_status sgpu_batch_run(sgpu_batch* batch, int32_t iterations, int32_t use_graph,
                            double* seconds);
+/* The same in one persistent launch: each workgroup (as many as are resident) takes the
+ * next window from a queue, largest first, and runs up to `iterations` LM iterations of it
+ * until it terminates, then takes the next; use_graph = 1 replays the launch from a
+ * captured hipGraph.  Results are those of sgpu_batch_run with the same iterations. */
+sgpu_status sgpu_batch_run_persistent(sgpu_batch* batch, int32_t iterations, int32_t use_graph,
+                                      double* seconds);
+/* Per window, from the device wall clock of the last persistent run: when a workgroup took
+ * it and when its run ended, in ms after the first window was taken (num_problems each). */
+sgpu_status sgpu_batch_latencies(const sgpu_batch* batch, double* start_ms, double* end_ms);
 /* Restarts every window from its initial values. */
 sgpu_status sgpu_batch_reset(sgpu_batch* batch);
 /* Final variables (same layout as the description), costs, steps and termination
@@ -591,9 +600,13 @@ sgpu_status sgpu_batch_reset(sgpu_batch* batch);
 sgpu_status sgpu_batch_results(sgpu_batch* batch, double* poses_out, double* points_out,
                                double* initial_cost, double* final_cost, int32_t* num_steps,
                                int32_t* termination);
-/* The autotuner's choice, its three timings (ms) and the LDS bytes per workgroup. */
+/* The autotuner's choices: the per-iteration kernel's workgroup size and its three timings
+ * (ms, one LM iteration of the batch at 128, 256 and 512 threads), the LDS bytes per
+ * workgroup, the persistent kernel's size, its three timings (one persistent launch of one
+ * iteration per window) and its grid (the resident workgroups).  Any pointer may be NULL. */
 sgpu_status sgpu_batch_info(const sgpu_batch* batch, int32_t* threads, double* tune_ms3,
-                            int64_t* lds_bytes);
+                            int64_t* lds_bytes, int32_t* persistent_threads,
+                            double* persistent_tune_ms3, int32_t* persistent_grid);
 void sgpu_batch_destroy(sgpu_batch* batch);
 
 /* ---- synthetic problems ---------------------------------------------------------

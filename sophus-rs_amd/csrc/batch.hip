@@ -201,13 +201,12 @@ __device__ void d_solve(const double* S, int k, double v[6]) {
     for (int j = i + 1; j < 6; ++j) v[i] -= S[pk(o + j, o + i)] * v[j];
 }
 
+// One LM iteration of unfinished window p by the whole workgroup (lds: the dynamic LDS, red
+// and flags (0: singular marginal block, 1: bad pivot) its static scratch).  Thread 0 takes
+// the LM decision and writes the window's state; the other threads return before it, so a
+// caller that goes on synchronizes first.
 template <typename T, int NT>
-__global__ void __launch_bounds__(NT) k_batch_step(BatchDev d) {
-  extern __shared__ double lds[];
-  __shared__ double red[NT / 64];
-  __shared__ int flags[4];  // 0: singular marginal block, 1: bad pivot
-  const int p = d.order[blockIdx.x];
-  if (d.done[p]) return;  // uniform per workgroup
+__device__ void batch_iteration(const BatchDev& d, int p, double* lds, double* red, int* flags) {
   const int c0 = d.cam0[p], nc = d.cam0[p + 1] - c0;
   const int q0 = d.pt0[p], np = d.pt0[p + 1] - q0;
   const int o0 = d.obs0[p], no = d.obs0[p + 1] - o0;
@@ -513,6 +512,60 @@ __global__ void __launch_bounds__(NT) k_batch_step(BatchDev d) {
   }
 }
 
+// One LM iteration of every unfinished window: one workgroup per window, largest first.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_batch_step(BatchDev d) {
+  extern __shared__ double lds[];
+  __shared__ double red[NT / 64];
+  __shared__ int flags[4];
+  const int p = d.order[blockIdx.x];
+  if (d.done[p]) return;  // uniform per workgroup
+  batch_iteration<T, NT>(d, p, lds, red, flags);
+}
+
+// Every window's LM run in one launch: each workgroup takes the next window (largest first)
+// from a queue counter and runs up to `iterations` LM iterations of it until it terminates,
+// then takes the next; the grid is sized to the resident workgroups.  Nothing waits on
+// another workgroup, and every loop is bounded (windows, iterations), so the grid always
+// drains.  t_start / t_end: the device wall clock when a window was taken and when its run
+// ended.
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT)
+    k_batch_persistent(BatchDev d, int32_t iterations, int32_t* __restrict__ queue,
+                       unsigned long long* __restrict__ t_start, unsigned long long* __restrict__ t_end) {
+  extern __shared__ double lds[];
+  __shared__ double red[NT / 64];
+  __shared__ int flags[4];
+  __shared__ int s_slot, s_done;
+  for (;;) {
+    if (threadIdx.x == 0) s_slot = atomicAdd(queue, 1);
+    __syncthreads();
+    const int slot = s_slot;
+    if (slot >= d.nprob) return;  // uniform per workgroup
+    const int p = d.order[slot];
+    if (threadIdx.x == 0) {
+      t_start[p] = (unsigned long long)wall_clock64();
+      s_done = d.done[p];
+    }
+    __syncthreads();
+    for (int it = 0; it < iterations && !s_done; ++it) {
+      // p is uniform, but read from LDS the compiler cannot know it: made a scalar so that
+      // the addresses derived from it stay in scalar registers (in vector registers the body
+      // takes 284 of them against 165, with spills at 512 threads)
+      batch_iteration<T, NT>(d, __builtin_amdgcn_readfirstlane(p), lds, red, flags);
+      __syncthreads();
+      if (threadIdx.x == 0) s_done = d.done[p];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) t_end[p] = (unsigned long long)wall_clock64();
+    __syncthreads();
+  }
+}
+
+__global__ void k_batch_zero_queue(int32_t* queue) {
+  if (threadIdx.x == 0) *queue = 0;
+}
+
 // State from the initial values and the initial merit.
 template <int NT>
 __global__ void __launch_bounds__(NT) k_batch_init(BatchDev d) {
@@ -578,11 +631,39 @@ struct sgpu_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_iters = 0;
+  // the persistent launch: queue counter, per-window wall-clock stamps, its own graph
+  int32_t* queue = nullptr;
+  unsigned long long* t_start = nullptr;
+  unsigned long long* t_end = nullptr;
+  int persistent_grid = 0;     // resident workgroups of the persistent kernel's size
+  int persistent_threads = 256;  // the persistent kernel's workgroup size (its own autotuner)
+  double ptune_ms[3] = {0, 0, 0};
+  double clock_khz = 0.0;   // device wall-clock rate
+  hipGraph_t pgraph = nullptr;
+  hipGraphExec_t pgraph_exec = nullptr;
+  int pgraph_iters = -1;
 
   ~sgpu_batch() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (pgraph_exec) (void)hipGraphExecDestroy(pgraph_exec);
+    if (pgraph) (void)hipGraphDestroy(pgraph);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void launch_persistent(int iterations) {
+    hipLaunchKernelGGL(k_batch_zero_queue, dim3(1), dim3(64), 0, stream, queue);
+    const dim3 g(persistent_grid);
+    const int nt = persistent_threads;
+    if (precision == SGPU_PRECISION_MIXED) {
+      if (nt == 128) hipLaunchKernelGGL((k_batch_persistent<float, 128>), g, dim3(128), lds_bytes, stream, d, iterations, queue, t_start, t_end);
+      else if (nt == 512) hipLaunchKernelGGL((k_batch_persistent<float, 512>), g, dim3(512), lds_bytes, stream, d, iterations, queue, t_start, t_end);
+      else hipLaunchKernelGGL((k_batch_persistent<float, 256>), g, dim3(256), lds_bytes, stream, d, iterations, queue, t_start, t_end);
+    } else {
+      if (nt == 128) hipLaunchKernelGGL((k_batch_persistent<double, 128>), g, dim3(128), lds_bytes, stream, d, iterations, queue, t_start, t_end);
+      else if (nt == 512) hipLaunchKernelGGL((k_batch_persistent<double, 512>), g, dim3(512), lds_bytes, stream, d, iterations, queue, t_start, t_end);
+      else hipLaunchKernelGGL((k_batch_persistent<double, 256>), g, dim3(256), lds_bytes, stream, d, iterations, queue, t_start, t_end);
+    }
   }
 
   void launch_init() {
@@ -684,7 +765,7 @@ sgpu_status sgpu_batch_create(const sgpu_batch_desc* desc, const sgpu_opt_params
     const size_t lds_doubles = size_t(pk(n_max, 0)) + 6 * size_t(n_max) + 3 * size_t(n_max) +
                                15 * size_t(kBatchMaxCams);
     const size_t lds_bytes = lds_doubles * 8;
-    if (lds_bytes + 8 * (nt_max / 64) + 16 > 160 * 1024)
+    if (lds_bytes + 8 * (nt_max / 64) + 16 + 8 > 160 * 1024)  // + red, flags, queue slot
       throw Error(SGPU_ERR_UNSUPPORTED, "window too large for the LDS-resident solve");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) throw Error(SGPU_ERR_NO_DEVICE, "no HIP device");
@@ -749,6 +830,19 @@ sgpu_status sgpu_batch_create(const sgpu_batch_desc* desc, const sgpu_opt_params
       set_lds(k_batch_step<float, 128>, lds_bytes);
       set_lds(k_batch_step<float, 256>, lds_bytes);
       set_lds(k_batch_step<float, 512>, lds_bytes);
+      set_lds(k_batch_persistent<double, 128>, lds_bytes);
+      set_lds(k_batch_persistent<double, 256>, lds_bytes);
+      set_lds(k_batch_persistent<double, 512>, lds_bytes);
+      set_lds(k_batch_persistent<float, 128>, lds_bytes);
+      set_lds(k_batch_persistent<float, 256>, lds_bytes);
+      set_lds(k_batch_persistent<float, 512>, lds_bytes);
+      b->queue = A.alloc<int32_t>(1);
+      b->t_start = A.alloc<unsigned long long>(P);
+      b->t_end = A.alloc<unsigned long long>(P);
+      int khz = 0;
+      SGPU_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, params->device));
+      if (khz <= 0) throw Error(SGPU_ERR_HIP, "device wall-clock rate unavailable");
+      b->clock_khz = double(khz);
       // autotuner: one LM iteration of the whole batch per candidate workgroup size, from
       // the same initial state; the fastest is kept
       hipEvent_t e0, e1;
@@ -777,6 +871,57 @@ sgpu_status sgpu_batch_create(const sgpu_batch_desc* desc, const sgpu_opt_params
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
+      // the persistent kernel keeps more registers live than the per-iteration one, so it
+      // has its own choice: for each size, the grid of every resident workgroup and one
+      // timed persistent launch of one LM iteration per window; the fastest is kept
+      {
+        int cus = 0;
+        SGPU_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, params->device));
+        int grids[3] = {0, 0, 0};
+        for (int i = 0; i < 3; ++i) {
+          const int nt = cand[i];
+          const void* fn = nullptr;
+          if (b->precision == SGPU_PRECISION_MIXED)
+            fn = nt == 128 ? reinterpret_cast<const void*>(k_batch_persistent<float, 128>)
+                 : nt == 512 ? reinterpret_cast<const void*>(k_batch_persistent<float, 512>)
+                             : reinterpret_cast<const void*>(k_batch_persistent<float, 256>);
+          else
+            fn = nt == 128 ? reinterpret_cast<const void*>(k_batch_persistent<double, 128>)
+                 : nt == 512 ? reinterpret_cast<const void*>(k_batch_persistent<double, 512>)
+                             : reinterpret_cast<const void*>(k_batch_persistent<double, 256>);
+          int per_cu = 0;
+          SGPU_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nt, lds_bytes));
+          grids[i] = int(std::max<int64_t>(1, std::min<int64_t>(P, int64_t(std::max(per_cu, 1)) * cus)));
+        }
+        hipEvent_t p0, p1;
+        SGPU_HIP(hipEventCreate(&p0));
+        SGPU_HIP(hipEventCreate(&p1));
+        float pbest = 1e30f;
+        for (int rep = 0; rep < 2; ++rep)  // the first pass is untimed
+          for (int i = 0; i < 3; ++i) {
+            b->persistent_threads = cand[i];
+            b->persistent_grid = grids[i];
+            b->launch_init();
+            SGPU_HIP(hipEventRecord(p0, b->stream));
+            b->launch_persistent(1);
+            SGPU_HIP(hipEventRecord(p1, b->stream));
+            SGPU_HIP(hipGetLastError());
+            SGPU_HIP(hipEventSynchronize(p1));
+            float ms = 0;
+            SGPU_HIP(hipEventElapsedTime(&ms, p0, p1));
+            if (rep == 1) b->ptune_ms[i] = ms;
+          }
+        int pick = 0;
+        for (int i = 0; i < 3; ++i)
+          if (b->ptune_ms[i] < pbest) {
+            pbest = float(b->ptune_ms[i]);
+            pick = i;
+          }
+        b->persistent_threads = cand[pick];
+        b->persistent_grid = grids[pick];
+        (void)hipEventDestroy(p0);
+        (void)hipEventDestroy(p1);
+      }
       b->launch_init();
       SGPU_HIP(hipStreamSynchronize(b->stream));
     } catch (...) {
@@ -811,6 +956,49 @@ sgpu_status sgpu_batch_run(sgpu_batch* b, int32_t iterations, int32_t use_graph,
     SGPU_HIP(hipStreamSynchronize(b->stream));
     if (seconds)
       *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+sgpu_status sgpu_batch_run_persistent(sgpu_batch* b, int32_t iterations, int32_t use_graph,
+                                      double* seconds) {
+  return guarded([&] {
+    if (!b || iterations < 0) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    const auto t0 = std::chrono::steady_clock::now();
+    if (use_graph) {
+      if (!b->pgraph_exec || b->pgraph_iters != iterations) {
+        if (b->pgraph_exec) (void)hipGraphExecDestroy(b->pgraph_exec);
+        if (b->pgraph) (void)hipGraphDestroy(b->pgraph);
+        b->pgraph_exec = nullptr;
+        b->pgraph = nullptr;
+        SGPU_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        b->launch_persistent(iterations);
+        SGPU_HIP(hipStreamEndCapture(b->stream, &b->pgraph));
+        SGPU_HIP(hipGraphInstantiate(&b->pgraph_exec, b->pgraph, nullptr, nullptr, 0));
+        b->pgraph_iters = iterations;
+      }
+      SGPU_HIP(hipGraphLaunch(b->pgraph_exec, b->stream));
+    } else {
+      b->launch_persistent(iterations);
+    }
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipStreamSynchronize(b->stream));
+    if (seconds)
+      *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+sgpu_status sgpu_batch_latencies(const sgpu_batch* b, double* start_ms, double* end_ms) {
+  return guarded([&] {
+    if (!b || !start_ms || !end_ms) throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    const size_t P = size_t(b->d.nprob);
+    std::vector<unsigned long long> ts(P), te(P);
+    SGPU_HIP(hipMemcpy(ts.data(), b->t_start, P * 8, hipMemcpyDeviceToHost));
+    SGPU_HIP(hipMemcpy(te.data(), b->t_end, P * 8, hipMemcpyDeviceToHost));
+    const unsigned long long t0 = *std::min_element(ts.begin(), ts.end());
+    for (size_t p = 0; p < P; ++p) {
+      start_ms[p] = double(ts[p] - t0) / b->clock_khz;
+      end_ms[p] = double(te[p] - t0) / b->clock_khz;
+    }
   });
 }
 
@@ -855,10 +1043,16 @@ sgpu_status sgpu_batch_results(sgpu_batch* b, double* poses_out, double* points_
   });
 }
 
-sgpu_status sgpu_batch_info(const sgpu_batch* b, int32_t* threads, double* tune_ms3, int64_t* lds_bytes) {
+sgpu_status sgpu_batch_info(const sgpu_batch* b, int32_t* threads, double* tune_ms3, int64_t* lds_bytes,
+                            int32_t* persistent_threads, double* persistent_tune_ms3,
+                            int32_t* persistent_grid) {
   return guarded([&] {
     if (!b) throw Error(SGPU_ERR_INVALID_ARGUMENT, "null batch");
     if (threads) *threads = b->threads;
+    if (persistent_threads) *persistent_threads = b->persistent_threads;
+    if (persistent_tune_ms3)
+      for (int i = 0; i < 3; ++i) persistent_tune_ms3[i] = b->ptune_ms[i];
+    if (persistent_grid) *persistent_grid = b->persistent_grid;
     if (tune_ms3)
       for (int i = 0; i < 3; ++i) tune_ms3[i] = b->tune_ms[i];
     if (lds_bytes) *lds_bytes = int64_t(b->lds_bytes);

@@ -248,10 +248,13 @@ _SIGNATURES = {
     "sgpu_precond_train": (C.c_int, [C.POINTER(_PrecondTrainParams), _FP, _DP, _DP, _I64P]),
     "sgpu_batch_create": (C.c_int, [C.POINTER(_BatchDesc), C.POINTER(_OptParams), C.POINTER(C.c_void_p)]),
     "sgpu_batch_run": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
+    "sgpu_batch_run_persistent": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _DP]),
+    "sgpu_batch_latencies": (C.c_int, [C.c_void_p, _DP, _DP]),
     "sgpu_batch_reset": (C.c_int, [C.c_void_p]),
     "sgpu_batch_results": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP, C.POINTER(C.c_int32),
                                      C.POINTER(C.c_int32)]),
-    "sgpu_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _DP, _I64P]),
+    "sgpu_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _DP, _I64P, C.POINTER(C.c_int32), _DP,
+                                  C.POINTER(C.c_int32)]),
     "sgpu_batch_destroy": (None, [C.c_void_p]),
     "sgpu_generate_ba": (C.c_int, [C.POINTER(_BaSpec), C.POINTER(_BaSizes), _DP, _DP, _DP, _DP, _DP,
                                    _I64P, _DP]),
@@ -260,7 +263,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 9  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 10  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -1104,6 +1107,20 @@ class Batch:
         _check(library().sgpu_batch_run(self._h, int(iterations), int(bool(graph)), C.byref(secs)))
         return secs.value
 
+    def run_persistent(self, iterations: int, graph: bool = True) -> float:
+        """Every window's LM run (up to `iterations` iterations) in one persistent launch."""
+        secs = C.c_double()
+        _check(library().sgpu_batch_run_persistent(self._h, int(iterations), int(bool(graph)), C.byref(secs)))
+        return secs.value
+
+    def latencies(self) -> dict:
+        """Per window, from the device clock of the last persistent run: when it was taken
+        and when its run ended (ms after the first window was taken)."""
+        P = self.num_problems
+        t0, t1 = np.empty(P), np.empty(P)
+        _check(library().sgpu_batch_latencies(self._h, t0.ctypes.data_as(_DP), t1.ctypes.data_as(_DP)))
+        return {"start_ms": t0, "end_ms": t1}
+
     def reset(self) -> None:
         _check(library().sgpu_batch_reset(self._h))
 
@@ -1132,8 +1149,13 @@ class Batch:
         t = C.c_int32()
         ms = (C.c_double * 3)()
         lds = C.c_int64()
-        _check(library().sgpu_batch_info(self._h, C.byref(t), ms, C.byref(lds)))
-        return {"threads": t.value, "tune_ms": {128: ms[0], 256: ms[1], 512: ms[2]}, "lds_bytes": lds.value}
+        pt = C.c_int32()
+        pms = (C.c_double * 3)()
+        grid = C.c_int32()
+        _check(library().sgpu_batch_info(self._h, C.byref(t), ms, C.byref(lds), C.byref(pt), pms, C.byref(grid)))
+        return {"threads": t.value, "tune_ms": {128: ms[0], 256: ms[1], 512: ms[2]}, "lds_bytes": lds.value,
+                "persistent_threads": pt.value, "persistent_tune_ms": {128: pms[0], 256: pms[1], 512: pms[2]},
+                "persistent_grid": grid.value}
 
     def window_slice(self, p: int) -> dict:
         return {"poses": slice(self.cam_ptr[p], self.cam_ptr[p + 1]),
