@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SGPU_ABI_VERSION 10
+#define SGPU_ABI_VERSION 11
 
 /* Status codes.  One code per NllsError variant the NLLS path can raise
  * (crates/sophus_opt/src/nlls/optimizer_types.rs:422-504) plus device errors. */
@@ -96,7 +96,12 @@ typedef enum sgpu_cost_kind {
    * measurement uv[2].  example_problems/ba_problem.rs:45-110.  The intrinsics family may
    * be Conditioned, or Free with one active member (one calibration shared by the
    * observations: its block couples to every camera, so it is ordered last in the RCS) */
-  SGPU_COST_EUCM_REPROJECTION = 2
+  SGPU_COST_EUCM_REPROJECTION = 2,
+  /* Isometry3PriorCostTerm: arg (world_from_pose) of the pose family; measurement the prior
+   * mean [7] then the precision W [36, row-major]: r = log(T E^-1), cost r~^T W r~ with
+   * r~ = w(|r|) r (functor_library/costs/isometry3_prior.rs:46-88, make with a precision,
+   * cost/evaluated_term.rs:286-327) */
+  SGPU_COST_ISOMETRY3_PRIOR = 3
 } sgpu_cost_kind;
 
 /* Equality-constraint kinds (closed set instead of HasEqConstraintResidualFn objects). */
@@ -446,6 +451,15 @@ sgpu_status sgpu_eval_eucm_terms(int64_t num_terms, const double* intrinsics /*6
                                  double huber_delta, double* residual_out,
                                  double* jac_intrinsics_out, double* jac_pose_out,
                                  double* jac_point_out, double* cost_out, double* weight_out);
+
+/* Isometry3PriorCostTerm (isometry3_prior.rs:46-88) on the GPU (FP64): per term the residual
+ * log(T E^-1) (residual_out[6t..]), its Jacobian by forward-mode duals of log(exp(x) T E^-1)
+ * at x = 0 (jac_out[36t..], row-major) and the cost r~^T W r~ (cost_out[t]; W 36 per term,
+ * robust_kernel / huber_delta as in sgpu_cost_fn). */
+sgpu_status sgpu_eval_isometry3_prior_terms(int64_t num_terms, const double* world_from_pose,
+                                            const double* prior_mean, const double* precision,
+                                            int32_t robust_kernel, double huber_delta,
+                                            double* residual_out, double* jac_out, double* cost_out);
 
 /* Kernel (a'): SE(3) between factor (pose_graph3.rs:33-79) on the GPU (FP64).
  * Per term: residual_out[6t..], jac_m_out[36t..], jac_n_out[36t..] (6x6 row-major). */

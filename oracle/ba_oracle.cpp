@@ -16,6 +16,7 @@
 //   BlockSparseLdlt             crates/sophus_solver/src/ldlt/block_sparse_ldlt.rs:96-190,400-747
 //   ldlt_decompose_inplace      crates/sophus_solver/src/ldlt/dense_ldlt.rs:164-256
 //   SparseLdlt (SchurSparseLdlt)  crates/sophus_solver/src/ldlt/sparse_ldlt.rs:121-445,571-650
+//   Isometry3PriorCostTerm      crates/sophus_opt/src/nlls/functor_library/costs/isometry3_prior.rs:46-88
 //   EliminationTree             crates/sophus_solver/src/ldlt/elimination_tree.rs:20-105
 //
 // Documented differences (rounding-level only):
@@ -182,7 +183,7 @@ struct CostFn {
   std::vector<int64_t> idx;   // sorted terms, arity per term
   std::vector<double> meas;   // 2 or 7 per term
   std::vector<int64_t> range_start;  // reduction ranges (sorted)
-  int mlen() const { return is_reprojection(kind) ? 2 : 7; }
+  int mlen() const { return is_reprojection(kind) ? 2 : kind == SGPU_COST_ISOMETRY3_PRIOR ? 43 : 7; }
   int64_t num_terms() const { return int64_t(idx.size()) / arity; }
 };
 
@@ -193,6 +194,7 @@ struct EvalTerm {
   double cost = 0;
   double J[3][6 * 6];   // per arg, R x dof
   bool has[3] = {false, false, false};
+  const double* W = nullptr;  // precision (R x R row-major) or none
 };
 
 // PinholeCameraReprojectionCostTerm::eval (pinhole_reprojection.rs:60-95).
@@ -321,6 +323,28 @@ void eval_pose_graph(const double* m, const double* n, const double* mn, double 
   }
 }
 
+// Isometry3PriorCostTerm::eval (isometry3_prior.rs:46-88): r = log(T E^-1) and its Jacobian by
+// forward-mode duals of log(exp(x) T E^-1) at x = 0, the reference's own derivative.
+void eval_isometry3_prior(const double* pose, const double* mean, double r[6], double J[36]) {
+  using D = Dual<6>;
+  D x[6], E[7], T[7], Ei[7], Tx[7], TE[7], g[6];
+  for (int k = 0; k < 6; ++k) x[k] = D::var(0.0, k);
+  for (int k = 0; k < 7; ++k) {
+    T[k] = D(pose[k]);
+    Ei[k] = D(mean[k]);
+  }
+  se3_inverse(Ei, E);
+  D ex[7];
+  se3_exp(x, ex);
+  se3_mul(ex, T, Tx);
+  se3_mul(Tx, E, TE);
+  se3_log(TE, g);
+  for (int i = 0; i < 6; ++i) {
+    r[i] = g[i].re;
+    for (int k = 0; k < 6; ++k) J[6 * i + k] = g[i].d[k];
+  }
+}
+
 // An equality-constraint set (EqConstraintFn, eq_constraint_fn.rs:61-140): terms of one kind;
 // all of them share one constraint block (EqSystem::new gives each set one partition of one
 // block, eq_system.rs:33-37, and populate_lower_triangular_kkt_mat adds every term into it,
@@ -405,6 +429,9 @@ void eval_term(const Problem& P, const Vars& V, const CostFn& c, int64_t t, bool
     const double* pose = &V.fam[c.fam[1]].members[ix[1] * 7];
     const double* xw = &V.fam[c.fam[2]].members[ix[2] * 3];
     eval_eucm(intr, pose, xw, ms, act[0], act[1], act[2], r, e.J[0], e.J[1], e.J[2]);
+  } else if (c.kind == SGPU_COST_ISOMETRY3_PRIOR) {
+    e.R = 6;
+    eval_isometry3_prior(&V.fam[c.fam[0]].members[ix[0] * 7], ms, r, e.J[0]);
   } else {
     e.R = 6;
     const double* m = &V.fam[c.fam[0]].members[ix[0] * 7];
@@ -412,14 +439,23 @@ void eval_term(const Problem& P, const Vars& V, const CostFn& c, int64_t t, bool
     eval_pose_graph(m, n, ms, r, e.J[0], e.J[1]);
   }
   for (int a = 0; a < 3; ++a) e.has[a] = act[a];
+  // make (evaluated_term.rs:286-327): r~ = w(|r|) r; with a precision W the gradient is
+  // J^T W r~, the Hessian J^T W J and the cost r~^T W r~
+  e.W = c.kind == SGPU_COST_ISOMETRY3_PRIOR ? ms + 7 : nullptr;
   double nrm2 = 0;
   for (int i = 0; i < e.R; ++i) nrm2 += r[i] * r[i];
   const double w = c.robust == SGPU_ROBUST_HUBER ? robust_weight(c.robust, c.delta, std::sqrt(nrm2))
                                                  : 1.0;
+  for (int i = 0; i < e.R; ++i) e.r[i] = c.robust == SGPU_ROBUST_HUBER ? w * r[i] : r[i];
   double cost = 0;
-  for (int i = 0; i < e.R; ++i) {
-    e.r[i] = c.robust == SGPU_ROBUST_HUBER ? w * r[i] : r[i];
-    cost += e.r[i] * e.r[i];
+  if (e.W) {
+    for (int i = 0; i < e.R; ++i) {
+      double lr = 0;
+      for (int k = 0; k < e.R; ++k) lr += e.W[e.R * i + k] * e.r[k];
+      cost += e.r[i] * lr;
+    }
+  } else {
+    for (int i = 0; i < e.R; ++i) cost += e.r[i] * e.r[i];
   }
   e.cost = cost;
 }
@@ -552,11 +588,32 @@ void reduce_range(const Problem& P, const Vars& V, const CostFn& c, int64_t t0, 
   EvalTerm e;
   for (int64_t t = t0; t < t1; ++t) {
     eval_term(P, V, c, t, true, e);
+    // with a precision: lambda r~ = W r~ and W J_b in place of r~ and J_b
+    double lr[6], WJ[3][36];
+    for (int k = 0; k < R; ++k) {
+      lr[k] = e.r[k];
+      if (e.W) {
+        lr[k] = 0;
+        for (int q = 0; q < R; ++q) lr[k] += e.W[R * k + q] * e.r[q];
+      }
+    }
+    for (int b = 0; b < c.arity; ++b) {
+      if (!e.has[b]) continue;
+      for (int k = 0; k < R; ++k)
+        for (int j = 0; j < dof[b]; ++j) {
+          double acc = e.J[b][k * dof[b] + j];
+          if (e.W) {
+            acc = 0;
+            for (int q = 0; q < R; ++q) acc += e.W[R * k + q] * e.J[b][q * dof[b] + j];
+          }
+          WJ[b][k * dof[b] + j] = acc;
+        }
+    }
     for (int a = 0; a < c.arity; ++a) {
       if (!e.has[a]) continue;
       for (int i = 0; i < dof[a]; ++i) {
         double acc = 0;
-        for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * e.r[k];
+        for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * lr[k];
         out.gs[a][i] += acc;
       }
       for (int b = 0; b < c.arity; ++b) {
@@ -564,7 +621,7 @@ void reduce_range(const Problem& P, const Vars& V, const CostFn& c, int64_t t0, 
         for (int i = 0; i < dof[a]; ++i)
           for (int j = 0; j < dof[b]; ++j) {
             double acc = 0;
-            for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * e.J[b][k * dof[b] + j];
+            for (int k = 0; k < R; ++k) acc += e.J[a][k * dof[a] + i] * WJ[b][k * dof[b] + j];
             out.Hs[a][b][i * dof[b] + j] += acc;
           }
       }
@@ -1409,7 +1466,7 @@ sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_co
                 "variable");
   for (int64_t i = 0; i < nc; ++i) {
     const auto& s = cfs[i];
-    const int arity = is_reprojection(s.kind) ? 3 : 2;
+    const int arity = is_reprojection(s.kind) ? 3 : s.kind == SGPU_COST_ISOMETRY3_PRIOR ? 1 : 2;
     int marg = 0;
     for (int a = 0; a < arity; ++a) {
       if (!s.family_names[a]) return fail(SGPU_ERR_INVALID_ARGUMENT, "missing family name");
@@ -1422,16 +1479,17 @@ sgpu_status build_problem(const sgpu_var_family* fams, int64_t nf, const sgpu_co
   }
   for (int64_t i = 0; i < nc; ++i) {
     const auto& s = cfs[i];
-    if (!is_reprojection(s.kind) && s.kind != SGPU_COST_POSE_GRAPH3)
+    if (!is_reprojection(s.kind) && s.kind != SGPU_COST_POSE_GRAPH3 && s.kind != SGPU_COST_ISOMETRY3_PRIOR)
       return fail(SGPU_ERR_INVALID_ARGUMENT, "unknown cost kind");
     CostFn c;
     c.kind = s.kind;
     c.robust = s.robust_kernel;
     c.delta = s.huber_delta;
-    c.arity = is_reprojection(s.kind) ? 3 : 2;
-    const int want[3][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+    c.arity = is_reprojection(s.kind) ? 3 : s.kind == SGPU_COST_ISOMETRY3_PRIOR ? 1 : 2;
+    const int want[4][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
                             {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1},
-                            {SGPU_VAR_EUCM, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3}};
+                            {SGPU_VAR_EUCM, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+                            {SGPU_VAR_ISOMETRY3, -1, -1}};
     for (int a = 0; a < c.arity; ++a) {
       c.fam[a] = P.vars.find(s.family_names[a]);
       if (c.fam[a] < 0) return fail(SGPU_ERR_UNKNOWN_FAMILY, "unknown family");
@@ -2039,6 +2097,32 @@ void oracle_eval_pose_graph3_terms(int64_t n, const double* m, const double* nn,
 }
 
 // Dual-number Jacobians of the pose-graph residual w.r.t. left perturbations of m and n.
+// Isometry3PriorCostTerm terms: the residual (6 per term), the Jacobian (36, row-major) by
+// duals and, with the precision W (36 per term) and Huber delta (robust 1), the cost.
+void oracle_eval_isometry3_prior_terms(int64_t n, const double* poses, const double* means,
+                                       const double* precisions, int32_t robust, double delta,
+                                       double* res, double* jac, double* cost) {
+  for (int64_t t = 0; t < n; ++t) {
+    double r[6];
+    eval_isometry3_prior(poses + 7 * t, means + 7 * t, r, jac + 36 * t);
+    double nrm2 = 0;
+    for (int k = 0; k < 6; ++k) nrm2 += r[k] * r[k];
+    const double w = robust == SGPU_ROBUST_HUBER ? robust_weight(robust, delta, std::sqrt(nrm2)) : 1.0;
+    double rw[6], c = 0;
+    for (int k = 0; k < 6; ++k) {
+      res[6 * t + k] = r[k];
+      rw[k] = robust == SGPU_ROBUST_HUBER ? w * r[k] : r[k];
+    }
+    const double* W = precisions + 36 * t;
+    for (int i = 0; i < 6; ++i) {
+      double lr = 0;
+      for (int k = 0; k < 6; ++k) lr += W[6 * i + k] * rw[k];
+      c += rw[i] * lr;
+    }
+    cost[t] = c;
+  }
+}
+
 void oracle_pose_graph3_dual_jacobians(int64_t n, const double* m, const double* nn,
                                        const double* mn, double* jm, double* jn) {
   using D = Dual<12>;

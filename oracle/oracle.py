@@ -125,6 +125,9 @@ def lib(variant: str = "") -> C.CDLL:
             getattr(L, name).argtypes = [C.c_int64, DP, DP, DP]
         L.oracle_eval_pose_graph3_terms.restype = None
         L.oracle_eval_pose_graph3_terms.argtypes = [C.c_int64, DP, DP, DP, DP, DP, DP]
+        L.oracle_eval_isometry3_prior_terms.restype = None
+        L.oracle_eval_isometry3_prior_terms.argtypes = [C.c_int64, DP, DP, DP, C.c_int32, C.c_double,
+                                                        DP, DP, DP]
         L.oracle_pose_graph3_dual_jacobians.restype = None
         L.oracle_pose_graph3_dual_jacobians.argtypes = [C.c_int64, DP, DP, DP, DP, DP]
         for name in ("oracle_so3_exp", "oracle_so3_log", "oracle_so3_matrix", "oracle_se3_exp",
@@ -471,6 +474,20 @@ def eval_pose_graph3_terms(m, n_, mn):
     r, jm, jn = np.zeros((k, 6)), np.zeros((k, 6, 6)), np.zeros((k, 6, 6))
     lib().oracle_eval_pose_graph3_terms(k, _p(m), _p(n_), _p(mn), _p(r), _p(jm), _p(jn))
     return r, jm, jn
+
+
+def eval_isometry3_prior_terms(poses, means, precisions, huber_delta=None):
+    """Isometry3PriorCostTerm (isometry3_prior.rs:46-88): residual log(T E^-1), its Jacobian
+    by duals of log(exp(x) T E^-1) at 0, and the cost r~^T W r~."""
+    poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 7)
+    means = np.ascontiguousarray(means, dtype=np.float64).reshape(-1, 7)
+    W = np.ascontiguousarray(precisions, dtype=np.float64).reshape(-1, 6, 6)
+    k = poses.shape[0]
+    r, J, c = np.zeros((k, 6)), np.zeros((k, 6, 6)), np.zeros(k)
+    lib().oracle_eval_isometry3_prior_terms(k, _p(poses), _p(means), _p(W), 0 if huber_delta is None else 1,
+                                            0.0 if huber_delta is None else float(huber_delta), _p(r), _p(J),
+                                            _p(c))
+    return r, J, c
 
 
 def pose_graph3_dual_jacobians(m, n_, mn):

@@ -30,6 +30,8 @@ __device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv
   for (int k = 0; k < 36; ++k) a[k] = A[k];
   double max_abs_pivot = 1.0;
   const double tol = 1e-12;
+  bool finite = true;
+#pragma unroll
   for (int j = 0; j < 6; ++j) {
     for (int k = 0; k < j; ++k) {
       double s = a[6 * j + k];
@@ -44,18 +46,18 @@ __device__ inline bool ldlt6(const double* A, double* L, double* d, double* dinv
     }
     double djj = a[6 * j + j];
     for (int q = 0; q < j; ++q) djj -= w[6 * j + q] * a[6 * j + q];
-    if (!isfinite(djj)) return false;
+    // branch-free: the pivot's tests are selects and a non-finite pivot is reported after
+    // the loop (its outputs are discarded then), which keeps the dependent chain short
+    finite = finite && isfinite(djj);
     const double zt = (kScalarRule ? fmax(fabs(djj), 1.0) : fmax(max_abs_pivot, 1.0)) * tol;
-    if (fabs(djj) <= zt) {
-      d[j] = 0.0;
-      rd[j] = 0.0;
-    } else {
-      d[j] = djj;
-      rd[j] = recip_nr(djj);
-      max_abs_pivot = fmax(max_abs_pivot, fabs(djj));
-    }
+    const bool zero = fabs(djj) <= zt;
+    const double r = recip_nr(djj);
+    d[j] = zero ? 0.0 : djj;
+    rd[j] = zero ? 0.0 : r;
+    max_abs_pivot = zero ? max_abs_pivot : fmax(max_abs_pivot, fabs(djj));
     a[6 * j + j] = 1.0;
   }
+  if (!finite) return false;
   if (dinv)
     for (int k = 0; k < 6; ++k) dinv[k] = rd[k];
   for (int r = 0; r < 6; ++r)

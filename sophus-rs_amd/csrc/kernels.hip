@@ -20,6 +20,7 @@
 #include "block6.hpp"
 #include "kernels.hpp"
 #include "lie.hpp"
+#include "dual.hpp"
 
 namespace sgpu {
 
@@ -2747,6 +2748,62 @@ __device__ void pose_graph_eval(const double* m, const double* n, const double* 
     }
 }
 
+// Helpers of the pose terms' linearization.  Right-multiplication of a 6x6 row-major block by
+// the centring matrix G(c) = [[I, 0], [hat(c), I]]: (X G)[:, 0:3] = X[:, 0:3] + X[:, 3:6] hat(c),
+// (X G)[:, 3:6] = X[:, 3:6].
+__device__ __forceinline__ void pose_right_g(const double* X, const double c[3], double* Y) {
+  for (int r = 0; r < 6; ++r) {
+    const double* x = X + 6 * r;
+    Y[6 * r + 0] = x[0] + (x[4] * c[2] - x[5] * c[1]);
+    Y[6 * r + 1] = x[1] + (-x[3] * c[2] + x[5] * c[0]);
+    Y[6 * r + 2] = x[2] + (x[3] * c[1] - x[4] * c[0]);
+    for (int k = 3; k < 6; ++k) Y[6 * r + k] = x[k];
+  }
+}
+// G(ca)^T X G(cb) (X row-major 6x6)
+__device__ __forceinline__ void pose_sandwich(const double* X, const double ca[3], const double cb[3],
+                                              double* Y) {
+  double XG[36], Xt[36], T1[36];
+  pose_right_g(X, cb, XG);
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) Xt[6 * j + i] = XG[6 * i + j];
+  pose_right_g(Xt, ca, T1);  // (X G_b)^T G_a = (G_a^T X G_b)^T
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) Y[6 * i + j] = T1[6 * j + i];
+}
+// Adds element (a, b) of a row-m / column-n block into S slot `slot` (its front position, or
+// the row-major slot-order blocks of the iterative solvers); `flip` when the slot holds the
+// transpose.
+__device__ __forceinline__ void pose_add_block(const DevProblem& p, double* C, double* bsr, int slot,
+                                               const double* X, int flip) {
+  if (slot < 0) return;
+  if (bsr) {
+    for (int e = 0; e < 36; ++e) {
+      const int a = e / 6, b = e % 6;
+      const double v = flip ? X[6 * b + a] : X[e];
+      if (v != 0.0) atomic_add_d(bsr + 36 * int64_t(slot) + e, v);
+    }
+    return;
+  }
+  double* B = C + p.s_off[slot];
+  const int ldc = p.s_ld[slot], ld = ldc >> 1;
+  for (int e = 0; e < 36; ++e) {
+    const int a = e / 6, b = e % 6;
+    const double v = flip ? X[6 * b + a] : X[e];
+    if (v != 0.0) atomic_add_d(B + ((ldc & 1) ? a * ld + b : b * ld + a), v);
+  }
+}
+// rhs += sign G(c)^T v = sign [v_w + v_v x c; v_v], grad_cam += sign v (block cb)
+__device__ __forceinline__ void pose_add_rhs(double* rhs_nat, double* grad_cam, int cb, double sign,
+                                             const double* v, const double c[3]) {
+  const double r[6] = {v[0] + (v[4] * c[2] - v[5] * c[1]), v[1] + (v[5] * c[0] - v[3] * c[2]),
+                       v[2] + (v[3] * c[1] - v[4] * c[0]), v[3], v[4], v[5]};
+  for (int k = 0; k < 6; ++k) {
+    atomic_add_d(rhs_nat + 6 * cb + k, sign * r[k]);
+    atomic_add_d(grad_cam + 6 * cb + k, sign * v[k]);
+  }
+}
+
 // With `centered` (cw != nullptr), the blocks and rhs are those of the centred camera
 // tangents: G_m^T H_mn G_n and G_m^T b_m (see k_schur_slots).
 __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const double* __restrict__ poses,
@@ -2776,80 +2833,121 @@ __global__ void k_pose_graph_linearize(DevPoseGraph g, DevProblem p, const doubl
   const int cm = p.pose_block[g.m[t]], cn = p.pose_block[g.nn[t]];
   const int* dst = g.dest + 4 * t;
   const double damp = g.first[t] ? nu : 0.0;
-  // H_mm = H_nn = D^T D (+ nu I), H_mn = J_m^T J_n = -D^T D (symmetric, so the stored
-  // orientation of the (max, min) block does not matter)
   // camera centres (zero when not centred: G = I)
   double cmc[3] = {0, 0, 0}, cnc[3] = {0, 0, 0};
   if (cw) {
     cam_center(cw + 12 * g.m[t], cmc);
     cam_center(cw + 12 * g.nn[t], cnc);
   }
-  // right-multiply a 6x6 row-major block by G(c) = [[I, 0], [hat(c), I]]:
-  // (X G)[:, 0:3] = X[:, 0:3] + X[:, 3:6] hat(c), (X G)[:, 3:6] = X[:, 3:6]
-  auto right_g = [](const double* X, const double c[3], double* Y) {
-    for (int r = 0; r < 6; ++r) {
-      const double* x = X + 6 * r;
-      Y[6 * r + 0] = x[0] + (x[4] * c[2] - x[5] * c[1]);
-      Y[6 * r + 1] = x[1] + (-x[3] * c[2] + x[5] * c[0]);
-      Y[6 * r + 2] = x[2] + (x[3] * c[1] - x[4] * c[0]);
-      for (int k = 3; k < 6; ++k) Y[6 * r + k] = x[k];
-    }
-  };
-  // G(ca)^T X G(cb) (X row-major 6x6)
-  auto sandwich = [&](const double* X, const double ca[3], const double cb[3], double* Y) {
-    double XG[36], Xt[36], T1[36];
-    right_g(X, cb, XG);
-    for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) Xt[6 * j + i] = XG[6 * i + j];
-    right_g(Xt, ca, T1);  // (X G_b)^T G_a = (G_a^T X G_b)^T
-    for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) Y[6 * i + j] = T1[6 * j + i];
-  };
-  // stores element (a, b) of the row-m / column-n block; `flip` when the slot holds its
-  // transpose
-  auto add = [&](int slot, const double* X, int flip) {
-    if (slot < 0) return;
-    if (bsr) {
-      for (int e = 0; e < 36; ++e) {
-        const int a = e / 6, b = e % 6;
-        const double v = flip ? X[6 * b + a] : X[e];
-        if (v != 0.0) atomic_add_d(bsr + 36 * int64_t(slot) + e, v);
-      }
-      return;
-    }
-    double* B = C + p.s_off[slot];
-    const int ldc = p.s_ld[slot], ld = ldc >> 1;
-    for (int e = 0; e < 36; ++e) {
-      const int a = e / 6, b = e % 6;
-      const double v = flip ? X[6 * b + a] : X[e];
-      if (v != 0.0) atomic_add_d(B + ((ldc & 1) ? a * ld + b : b * ld + a), v);
-    }
-  };
-  // H_mm = H_nn = D^T D (+ nu I), H_mn = J_m^T J_n = -D^T D
+  // H_mm = H_nn = D^T D (+ nu I), H_mn = J_m^T J_n = -D^T D (symmetric, so the stored
+  // orientation of the (max, min) block does not matter)
   double Hmm[36], Hmn[36], X[36];
   for (int e = 0; e < 36; ++e) {
     Hmm[e] = DtD[e] + ((e / 6 == e % 6) ? damp : 0.0);
     Hmn[e] = -DtD[e];
   }
-  sandwich(Hmm, cmc, cmc, X);
-  add(dst[0], X, 0);
-  sandwich(Hmm, cnc, cnc, X);
-  add(dst[1], X, 0);
-  sandwich(Hmn, cmc, cnc, X);
-  add(dst[2], X, dst[3]);
-  // neg_grad_m -= J_m^T r~ = +D^T r~ ; neg_grad_n -= D^T r~ ; rhs' = G^T rhs:
-  // [b_w + b_v x c; b_v]
-  auto add_rhs = [&](int cb, double sign, const double c[3]) {
-    const double* v = Dtr;
-    const double r[6] = {v[0] + (v[4] * c[2] - v[5] * c[1]), v[1] + (v[5] * c[0] - v[3] * c[2]),
-                         v[2] + (v[3] * c[1] - v[4] * c[0]), v[3], v[4], v[5]};
-    for (int k = 0; k < 6; ++k) {
-      atomic_add_d(rhs_nat + 6 * cb + k, sign * r[k]);
-      atomic_add_d(grad_cam + 6 * cb + k, sign * v[k]);
+  pose_sandwich(Hmm, cmc, cmc, X);
+  pose_add_block(p, C, bsr, dst[0], X, 0);
+  pose_sandwich(Hmm, cnc, cnc, X);
+  pose_add_block(p, C, bsr, dst[1], X, 0);
+  pose_sandwich(Hmn, cmc, cnc, X);
+  pose_add_block(p, C, bsr, dst[2], X, dst[3]);
+  // neg_grad_m -= J_m^T r~ = +D^T r~ ; neg_grad_n -= D^T r~
+  if (cm >= 0) pose_add_rhs(rhs_nat, grad_cam, cm, 1.0, Dtr, cmc);
+  if (cn >= 0) pose_add_rhs(rhs_nat, grad_cam, cn, -1.0, Dtr, cnc);
+}
+
+// Isometry3PriorCostTerm (isometry3_prior.rs:46-88): r = log(T E^-1) and its Jacobian by
+// forward-mode duals of log(exp(x) T E^-1) at x = 0 (the reference's own derivative, which
+// also fixes the small-angle behaviour).
+__device__ void prior_eval(const double* pose, const double* mean, double r[6], double J[36]) {
+  using D6 = dual::Dual<6>;
+  D6 x[6], ex[7], T[7], Em[7], Ei[7], Tx[7], TE[7], gv[6];
+  for (int k = 0; k < 6; ++k) x[k] = D6::var(0.0, k);
+  for (int k = 0; k < 7; ++k) {
+    T[k] = D6(pose[k]);
+    Em[k] = D6(mean[k]);
+  }
+  se3_inverse(Em, Ei);
+  se3_exp(x, ex);
+  se3_mul(ex, T, Tx);
+  se3_mul(Tx, Ei, TE);
+  se3_log(TE, gv);
+  for (int i = 0; i < 6; ++i) {
+    r[i] = gv[i].re;
+    for (int k = 0; k < 6; ++k) J[6 * i + k] = gv[i].d[k];
+  }
+}
+
+// r~ = w(|r|) r and, with the precision W, lambda = W r~ and the cost r~^T lambda (make,
+// evaluated_term.rs:286-327)
+__device__ __forceinline__ double prior_weighted(const DevPoseGraph& g, int t, const double r[6],
+                                                 double rw[6], double lam[6]) {
+  double nrm = 0;
+  for (int k = 0; k < 6; ++k) nrm += r[k] * r[k];
+  const double w = robust_weight(g.robust, g.delta, sqrt(nrm));
+  for (int k = 0; k < 6; ++k) rw[k] = g.robust == 1 ? w * r[k] : r[k];
+  const double* W = g.prec + 36 * int64_t(t);
+  double c = 0;
+  for (int i = 0; i < 6; ++i) {
+    double s = 0;
+    for (int k = 0; k < 6; ++k) s += W[6 * i + k] * rw[k];
+    lam[i] = s;
+    c += rw[i] * s;
+  }
+  return c;
+}
+
+// One thread per prior term: H = J^T W J (+ nu I on the term's first of a reduction range)
+// into the pose's diagonal S slot, neg_grad -= J^T W r~.
+__global__ void k_prior_linearize(DevPoseGraph g, DevProblem p, const double* __restrict__ poses,
+                                  double nu, double* __restrict__ C, double* __restrict__ rhs_nat,
+                                  double* __restrict__ grad_cam, const double* __restrict__ cw,
+                                  double* __restrict__ bsr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.n) return;
+  const int cm = p.pose_block[g.m[t]];
+  if (cm < 0) return;  // a constant pose: its prior adds cost only
+  double r[6], J[36], rw[6], lam[6];
+  prior_eval(poses + 7 * g.m[t], g.meas + 7 * t, r, J);
+  prior_weighted(g, t, r, rw, lam);
+  const double* W = g.prec + 36 * int64_t(t);
+  double WJ[36], H[36], Jtl[6];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double s = 0;
+      for (int k = 0; k < 6; ++k) s += W[6 * i + k] * J[6 * k + j];
+      WJ[6 * i + j] = s;
     }
-  };
-  if (cm >= 0) add_rhs(cm, 1.0, cmc);
-  if (cn >= 0) add_rhs(cn, -1.0, cnc);
+  const double damp = g.first[t] ? nu : 0.0;
+  for (int i = 0; i < 6; ++i) {
+    double s = 0;
+    for (int k = 0; k < 6; ++k) s += J[6 * k + i] * lam[k];
+    Jtl[i] = s;
+    for (int j = 0; j < 6; ++j) {
+      double q = 0;
+      for (int k = 0; k < 6; ++k) q += J[6 * k + i] * WJ[6 * k + j];
+      H[6 * i + j] = q + (i == j ? damp : 0.0);
+    }
+  }
+  double c[3] = {0, 0, 0}, X[36];
+  if (cw) cam_center(cw + 12 * g.m[t], c);
+  pose_sandwich(H, c, c, X);
+  pose_add_block(p, C, bsr, g.dest[4 * t], X, 0);
+  pose_add_rhs(rhs_nat, grad_cam, cm, -1.0, Jtl, c);
+}
+
+__global__ void k_prior_cost(DevPoseGraph g, const double* __restrict__ poses,
+                             double* __restrict__ partial) {
+  double acc = 0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < g.n; t += gridDim.x * blockDim.x) {
+    double ei[7], te[7], r[6], rw[6], lam[6];
+    se3_inverse(g.meas + 7 * t, ei);
+    se3_mul(poses + 7 * g.m[t], ei, te);
+    se3_log(te, r);
+    acc += prior_weighted(g, t, r, rw, lam);
+  }
+  block_reduce2(acc, 0.0, partial);
 }
 
 // delta = G delta': nu += c x omega for every free camera (centred solves).
@@ -3295,8 +3393,12 @@ void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const Dev
                                  const double* poses, double nu, DevLin& w, DevFactor& f,
                                  const double* centered_cw) {
   if (g.n == 0) return;
-  hipLaunchKernelGGL(k_pose_graph_linearize, dim3(blocks_for(g.n, 128)), dim3(128), 0, s, g, p,
-                     poses, nu, f.front, w.rhs_nat, w.grad_cam, centered_cw, w.bsr);
+  if (g.prior)
+    hipLaunchKernelGGL(k_prior_linearize, dim3(blocks_for(g.n, 64)), dim3(64), 0, s, g, p, poses, nu,
+                       f.front, w.rhs_nat, w.grad_cam, centered_cw, w.bsr);
+  else
+    hipLaunchKernelGGL(k_pose_graph_linearize, dim3(blocks_for(g.n, 128)), dim3(128), 0, s, g, p,
+                       poses, nu, f.front, w.rhs_nat, w.grad_cam, centered_cw, w.bsr);
 }
 
 void launch_uncenter(hipStream_t s, const DevProblem& p, const double* cw, double* dx_cam) {
@@ -3309,7 +3411,10 @@ void launch_pose_graph_cost(hipStream_t s, const DevPoseGraph& g, const double* 
                             double* partial, int32_t partial_cap, DevScalars* sc) {
   (void)partial_cap;
   (void)sc;
-  hipLaunchKernelGGL(k_pose_graph_cost, dim3(kCostBlocks), dim3(kThreads), 0, s, g, poses, partial);
+  if (g.prior)
+    hipLaunchKernelGGL(k_prior_cost, dim3(kCostBlocks), dim3(kThreads), 0, s, g, poses, partial);
+  else
+    hipLaunchKernelGGL(k_pose_graph_cost, dim3(kCostBlocks), dim3(kThreads), 0, s, g, poses, partial);
 }
 
 void launch_eval_pinhole(hipStream_t s, int precision, int64_t n, const double* intr,
@@ -3332,6 +3437,31 @@ void launch_eval_eucm(hipStream_t s, int64_t n, const double* intr, const double
   if (n == 0) return;
   hipLaunchKernelGGL(k_eval_eucm, dim3(blocks_for(n, kThreads)), dim3(kThreads), 0, s, n, intr,
                      pose, pt, uv, robust, delta, res, jintr, jpose, jpt, cost, weight);
+}
+
+// Isometry3 prior terms on host-given inputs: residual, dual Jacobian and cost per term.
+__global__ void k_eval_prior(int64_t n, const double* __restrict__ poses, const double* __restrict__ means,
+                             const double* __restrict__ prec, int robust, double delta, double* res,
+                             double* jac, double* cost) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= n) return;
+  double r[6], J[36], rw[6], lam[6];
+  prior_eval(poses + 7 * t, means + 7 * t, r, J);
+  DevPoseGraph g{};
+  g.robust = robust;
+  g.delta = delta;
+  g.prec = prec;
+  cost[t] = prior_weighted(g, int(t), r, rw, lam);
+  for (int k = 0; k < 6; ++k) res[6 * t + k] = r[k];
+  for (int k = 0; k < 36; ++k) jac[36 * t + k] = J[k];
+}
+
+void launch_eval_prior(hipStream_t s, int64_t n, const double* poses, const double* means,
+                       const double* prec, int robust, double delta, double* res, double* jac,
+                       double* cost) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_eval_prior, dim3(blocks_for(n, 64)), dim3(64), 0, s, n, poses, means, prec,
+                     robust, delta, res, jac, cost);
 }
 
 void launch_eval_pose_graph(hipStream_t s, int64_t n, const double* m, const double* nn,

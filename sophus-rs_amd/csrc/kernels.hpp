@@ -210,6 +210,9 @@ void launch_max_grad(hipStream_t s, const double* grad_cam, int nb, DevScalars* 
 void launch_zero_scalars(hipStream_t s, DevScalars* sc);
 
 // pose-graph terms (SE(3) between factor)
+// Pose terms of one cost function: SE(3) between factors, or (prior) Isometry3 priors, whose
+// term t has pose m[t] (nn[t] = m[t]), the prior mean in meas[7t..] and the precision W in
+// prec[36t..] (row-major).
 struct DevPoseGraph {
   int32_t n;
   const int32_t* m;
@@ -219,6 +222,8 @@ struct DevPoseGraph {
   const int32_t* dest;  // 4 per term: S slots of (m,m), (n,n), (max,min) or -1; flip
   int robust;
   double delta;
+  int prior;
+  const double* prec;
 };
 void launch_pose_graph_linearize(hipStream_t s, const DevPoseGraph& g, const DevProblem& p,
                                  const double* poses, double nu, DevLin& w, DevFactor& f,
@@ -243,6 +248,9 @@ void launch_eval_eucm(hipStream_t s, int64_t n, const double* intr, const double
                       double* jintr, double* jpose, double* jpt, double* cost, double* weight);
 void launch_eval_pose_graph(hipStream_t s, int64_t n, const double* m, const double* nn,
                             const double* mn, double* res, double* jm, double* jn);
+void launch_eval_prior(hipStream_t s, int64_t n, const double* poses, const double* means,
+                       const double* prec, int robust, double delta, double* res, double* jac,
+                       double* cost);
 
 constexpr int kCostBlocks = 1024;
 

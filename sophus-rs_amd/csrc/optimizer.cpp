@@ -450,6 +450,8 @@ struct sgpu_optimizer {
       q.dest = arena.upload(dest, stream);
       q.robust = g.robust;
       q.delta = g.delta;
+      q.prior = g.prior ? 1 : 0;
+      q.prec = g.prior ? arena.upload(g.prec, stream) : nullptr;
       dpg.push_back(q);
     }
     nu = params.initial_lm_damping;

@@ -94,7 +94,7 @@ HostProblem build_host_problem_sorted(const sgpu_var_family* families, int64_t n
           "Schur solver requires at least one active free and one active marginalized variable");
   for (int64_t i = 0; i < num_cost_fns; ++i) {
     const sgpu_cost_fn& c = cost_fns[i];
-    const int arity = is_reprojection(c.kind) ? 3 : 2;
+    const int arity = is_reprojection(c.kind) ? 3 : c.kind == SGPU_COST_ISOMETRY3_PRIOR ? 1 : 2;
     int marg = 0;
     for (int a = 0; a < arity; ++a) {
       if (!c.family_names[a]) raise(SGPU_ERR_INVALID_ARGUMENT, "missing cost family name");
@@ -107,14 +107,15 @@ HostProblem build_host_problem_sorted(const sgpu_var_family* families, int64_t n
   }
   for (int64_t i = 0; i < num_cost_fns; ++i) {
     const sgpu_cost_fn& c = cost_fns[i];
-    if (!is_reprojection(c.kind) && c.kind != SGPU_COST_POSE_GRAPH3)
+    if (!is_reprojection(c.kind) && c.kind != SGPU_COST_POSE_GRAPH3 && c.kind != SGPU_COST_ISOMETRY3_PRIOR)
       raise(SGPU_ERR_INVALID_ARGUMENT, "unknown cost kind");
     if (c.robust_kernel != SGPU_ROBUST_NONE && c.robust_kernel != SGPU_ROBUST_HUBER)
       raise(SGPU_ERR_INVALID_ARGUMENT, "unknown robust kernel");
-    const int arity = is_reprojection(c.kind) ? 3 : 2;
-    const int want[3][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+    const int arity = is_reprojection(c.kind) ? 3 : c.kind == SGPU_COST_ISOMETRY3_PRIOR ? 1 : 2;
+    const int want[4][3] = {{SGPU_VAR_PINHOLE, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
                             {SGPU_VAR_ISOMETRY3, SGPU_VAR_ISOMETRY3, -1},
-                            {SGPU_VAR_EUCM, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3}};
+                            {SGPU_VAR_EUCM, SGPU_VAR_ISOMETRY3, SGPU_VAR_VEC3},
+                            {SGPU_VAR_ISOMETRY3, -1, -1}};
     int fi[3] = {-1, -1, -1};
     for (int a = 0; a < arity; ++a) {
       fi[a] = find(c.family_names[a]);
@@ -192,6 +193,32 @@ HostProblem build_host_problem_sorted(const sgpu_var_family* families, int64_t n
         P.obs_uv[2 * t] = c.measurements[2 * o];
         P.obs_uv[2 * t + 1] = c.measurements[2 * o + 1];
       }
+    } else if (c.kind == SGPU_COST_ISOMETRY3_PRIOR) {
+      // Isometry3PriorCostTerm (isometry3_prior.rs:46-88) on the pose family; terms sorted by
+      // member (CostFn::sort), repeated members reduced into one term (one nu)
+      if (P.fams[fi[0]].kind != SGPU_VAR_FREE)
+        raise(SGPU_ERR_UNSUPPORTED, "Isometry3 priors need the Free SE(3) pose family");
+      if (P.f_pose >= 0 && P.f_pose != fi[0])
+        raise(SGPU_ERR_UNSUPPORTED, "cost functions use different pose families");
+      P.f_pose = fi[0];
+      PoseGraphFn g;
+      g.prior = true;
+      g.robust = c.robust_kernel;
+      g.delta = c.huber_delta;
+      const int64_t nt = c.num_terms;
+      std::vector<int32_t> ord(nt);
+      std::iota(ord.begin(), ord.end(), 0);
+      const int64_t* E = c.entity_indices;
+      std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return E[x] < E[y]; });
+      for (int64_t t = 0; t < nt; ++t) {
+        const int32_t o = ord[t];
+        g.m.push_back(int32_t(E[o]));
+        g.n.push_back(int32_t(E[o]));
+        for (int k = 0; k < 7; ++k) g.meas.push_back(c.measurements[43 * int64_t(o) + k]);
+        for (int k = 0; k < 36; ++k) g.prec.push_back(c.measurements[43 * int64_t(o) + 7 + k]);
+        g.first.push_back(t == 0 || g.m[t] != g.m[t - 1]);
+      }
+      P.pg.push_back(std::move(g));
     } else {
       if (P.fams[fi[0]].kind != SGPU_VAR_FREE || fi[0] != fi[1])
         raise(SGPU_ERR_UNSUPPORTED, "pose-graph terms need one Free SE(3) family");
@@ -672,8 +699,8 @@ void analyze_finish(const HostProblem& P, Symbolic& S) {
     for (size_t t = 0; t < g.m.size(); ++t) {
       const int32_t a = poses.block[g.m[t]], b = poses.block[g.n[t]];
       d[t].mm = a >= 0 ? s_slot(a, a) : -1;
-      d[t].nn = b >= 0 ? s_slot(b, b) : -1;
-      d[t].mn = (a >= 0 && b >= 0) ? s_slot(std::max(a, b), std::min(a, b)) : -1;
+      d[t].nn = b >= 0 && !g.prior ? s_slot(b, b) : -1;
+      d[t].mn = (a >= 0 && b >= 0 && !g.prior) ? s_slot(std::max(a, b), std::min(a, b)) : -1;
       d[t].mn_flip = a < b ? 1 : 0;
     }
     S.pg_dest.push_back(std::move(d));

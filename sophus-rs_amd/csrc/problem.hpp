@@ -26,11 +26,15 @@ struct EqSet {
   std::vector<double> radius;
 };
 
+// SE(3) between factors, or (prior) Isometry3 priors: one pose per term (n = m), the prior
+// mean in meas and the precision in prec.
 struct PoseGraphFn {
   int robust = 0;
   double delta = 0;
+  bool prior = false;
   std::vector<int32_t> m, n;      // pose member indices, sorted by (m, n)
   std::vector<double> meas;       // 7 per term
+  std::vector<double> prec;       // priors: 36 per term (row-major W)
   std::vector<uint8_t> first;     // 1 if the term opens a reduction range
 };
 

@@ -176,6 +176,33 @@ extern "C" sgpu_status sgpu_eval_eucm_terms(int64_t n, const double* intr, const
   });
 }
 
+extern "C" sgpu_status sgpu_eval_isometry3_prior_terms(int64_t n, const double* poses,
+                                                       const double* means, const double* prec,
+                                                       int32_t robust, double delta, double* res,
+                                                       double* jac, double* cost) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!poses || !means || !prec || !res || !jac || !cost)))
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (robust != SGPU_ROBUST_NONE && robust != SGPU_ROBUST_HUBER)
+      throw Error(SGPU_ERR_INVALID_ARGUMENT, "unknown robust kernel");
+    if (n == 0) return;
+    require_device();
+    Buf b1, b2, b3, b4, b5, b6;
+    const double* d1 = upload(b1, poses, 7 * n);
+    const double* d2 = upload(b2, means, 7 * n);
+    const double* d3 = upload(b3, prec, 36 * n);
+    double* d4 = b4.get<double>(6 * n);
+    double* d5 = b5.get<double>(36 * n);
+    double* d6 = b6.get<double>(n);
+    launch_eval_prior(nullptr, n, d1, d2, d3, robust, delta, d4, d5, d6);
+    SGPU_HIP(hipGetLastError());
+    SGPU_HIP(hipDeviceSynchronize());
+    SGPU_HIP(hipMemcpy(res, d4, 6 * n * 8, hipMemcpyDeviceToHost));
+    SGPU_HIP(hipMemcpy(jac, d5, 36 * n * 8, hipMemcpyDeviceToHost));
+    SGPU_HIP(hipMemcpy(cost, d6, n * 8, hipMemcpyDeviceToHost));
+  });
+}
+
 extern "C" sgpu_status sgpu_eval_pose_graph3_terms(int64_t n, const double* m, const double* nn,
                                                    const double* mn, double* res, double* jm,
                                                    double* jn) {

@@ -236,6 +236,8 @@ _SIGNATURES = {
     "sgpu_eval_eucm_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, C.c_int32, C.c_double,
                                        _DP, _DP, _DP, _DP, _DP, _DP]),
     "sgpu_eval_pose_graph3_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "sgpu_eval_isometry3_prior_terms": (C.c_int, [C.c_int64, _DP, _DP, _DP, C.c_int32, C.c_double, _DP, _DP,
+                                                  _DP]),
     "sgpu_block_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_block_ldlt_solve_noalloc": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
     "sgpu_sparse_ldlt_solve": (C.c_int, [C.c_int64, _I64P, _I64P, _DP, _DP, _DP]),
@@ -263,7 +265,7 @@ _SIGNATURES = {
 }
 
 _lib: Optional[C.CDLL] = None
-ABI_VERSION = 10  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
+ABI_VERSION = 11  # SGPU_ABI_VERSION of include/sophus_gpu.h these bindings mirror
 
 
 def library() -> C.CDLL:
@@ -443,6 +445,7 @@ class CostKind(enum.IntEnum):
     PinholeReprojection = 0
     PoseGraph3 = 1
     EucmReprojection = 2
+    Isometry3Prior = 3
 
 
 @dataclass
@@ -450,7 +453,8 @@ class CostFn:
     """CostFn::new_boxed((), CostTerms::new(family_names, terms), robust_kernel).
 
     Terms are stored vectorised: ``entity_indices`` (N x arity) and ``measurements``
-    (N x 2 pixel uv for reprojection, N x 7 pose_m_from_pose_n for the pose graph).
+    (N x 2 pixel uv for reprojection, N x 7 pose_m_from_pose_n for the pose graph, N x 43 for
+    the Isometry3 prior: the mean's 7 parameters, then the 6x6 precision row-major).
     """
 
     kind: CostKind
@@ -461,8 +465,9 @@ class CostFn:
 
     def __post_init__(self):
         reproj = self.kind in (CostKind.PinholeReprojection, CostKind.EucmReprojection)
-        arity = 3 if reproj else 2
-        mlen = 2 if reproj else 7
+        prior = self.kind == CostKind.Isometry3Prior
+        arity = 3 if reproj else 1 if prior else 2
+        mlen = 2 if reproj else 43 if prior else 7
         self.entity_indices = np.ascontiguousarray(self.entity_indices, dtype=np.int64).reshape(-1, arity)
         self.measurements = np.ascontiguousarray(self.measurements, dtype=np.float64).reshape(-1, mlen)
         if len(self.family_names) != arity:
@@ -482,6 +487,18 @@ class CostFn:
     def pose_graph3(family_names, entity_indices, pose_m_from_pose_n, robust_kernel=None) -> "CostFn":
         """PoseGraph3CostTerm terms: (world_from_pose_m, world_from_pose_n)."""
         return CostFn(CostKind.PoseGraph3, family_names, entity_indices, pose_m_from_pose_n, robust_kernel)
+
+    @staticmethod
+    def isometry3_prior(family_names, entity_indices, prior_means, prior_precisions,
+                        robust_kernel=None) -> "CostFn":
+        """Isometry3PriorCostTerm terms (isometry3_prior.rs:36-44): one pose each, the prior
+        mean (N x 7) and precision (N x 6 x 6)."""
+        means = np.asarray(prior_means, dtype=np.float64).reshape(-1, 7)
+        prec = np.asarray(prior_precisions, dtype=np.float64).reshape(-1, 36)
+        if prec.shape[0] == 1 and means.shape[0] > 1:
+            prec = np.repeat(prec, means.shape[0], axis=0)
+        return CostFn(CostKind.Isometry3Prior, family_names, np.asarray(entity_indices).reshape(-1, 1),
+                      np.hstack([means, prec]), robust_kernel)
 
 
 class Solver(enum.IntEnum):
@@ -886,6 +903,21 @@ def eval_pose_graph3_terms(world_from_m, world_from_n, m_from_n) -> dict:
     _check(library().sgpu_eval_pose_graph3_terms(n, a.ctypes.data_as(_DP), b.ctypes.data_as(_DP),
                                                  c.ctypes.data_as(_DP),
                                                  *[out[k].ctypes.data_as(_DP) for k in ("residual", "jac_m", "jac_n")]))
+    return out
+
+
+def eval_isometry3_prior_terms(poses, means, precisions, huber_delta: Optional[float] = None) -> dict:
+    """Isometry3PriorCostTerm per term on the GPU: residual (6), Jacobian (6x6, by duals) and
+    cost r~^T W r~."""
+    a = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, 7)
+    b = np.ascontiguousarray(means, dtype=np.float64).reshape(-1, 7)
+    W = np.ascontiguousarray(precisions, dtype=np.float64).reshape(-1, 36)
+    n = a.shape[0]
+    out = {"residual": np.empty((n, 6)), "jacobian": np.empty((n, 6, 6)), "cost": np.empty(n)}
+    _check(library().sgpu_eval_isometry3_prior_terms(
+        n, a.ctypes.data_as(_DP), b.ctypes.data_as(_DP), W.ctypes.data_as(_DP),
+        0 if huber_delta is None else 1, 0.0 if huber_delta is None else float(huber_delta),
+        *[out[k].ctypes.data_as(_DP) for k in ("residual", "jacobian", "cost")]))
     return out
 
 
