@@ -399,7 +399,7 @@ def eucm_leg(sg, cpu: bool) -> dict:
     return out
 
 
-def batch_leg(sg, n_windows: int, iters: int = 20) -> dict:
+def batch_leg(sg, n_windows: int, iters: int = 20, cpu: bool = False) -> dict:
     """The batched workload: n_windows independent ring windows (5-30 cameras, 100-2000
     points, 2-8 cameras per point, seeded by window id), 20 LM iterations each, FP64 and
     FP32-assembly.  `persistent`: one graph-replayed persistent launch runs every window to
@@ -465,7 +465,46 @@ def batch_leg(sg, n_windows: int, iters: int = 20) -> dict:
                      "final_over_initial_cost_median": float(np.median(res["final_cost"] / res["initial_cost"])),
                      "create_s": create_s, "autotuned_threads": info["threads"],
                      "autotune_ms": info["tune_ms"]}
+    if cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(windows[:64], iters)
     return out
+
+
+def _oracle_windows(args) -> list:
+    """Worker of batch_cpu_baseline: the FP64 oracle on its windows, one core, 20 LM
+    iterations each; returns each window's seconds."""
+    windows, iters = args
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    oracle.set_threads(1)
+    times = []
+    for fams, costs in windows:
+        t = time.perf_counter()
+        oracle.optimize_nlls(fams, costs, oracle.default_params(num_iterations=iters, initial_lm_damping=1.0))
+        times.append(time.perf_counter() - t)
+    return times
+
+
+def batch_cpu_baseline(windows, iters: int) -> dict:
+    """The batched workload's CPU baseline: the FP64 oracle on a sample of its windows, one
+    window per core at a time on host_threads() worker processes (started fresh, not forked
+    from this process, which holds the GPU)."""
+    import multiprocessing as mp
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+    jobs = [(oracle.ba_families(w), oracle.ba_costs(w)) for w in windows]
+    n = max(1, min(host_threads(), len(jobs)))
+    parts = [(jobs[i::n], iters) for i in range(n)]
+    t = time.perf_counter()
+    with mp.get_context("spawn").Pool(n) as pool:
+        per = pool.map(_oracle_windows, parts)
+    wall = time.perf_counter() - t
+    one = [x for p in per for x in p]
+    return {"value": len(jobs) / wall, "unit": "problems/s", "cores": n, "kind": "port",
+            "sample": f"{len(jobs)} of the windows, {iters} LM iterations each, the FP64 oracle on {n} worker "
+                      f"processes (one core each; wall time including the workers' start)",
+            "single_core_problems_per_s": len(one) / sum(one), "median_window_ms": 1e3 * float(np.median(one))}
 
 
 def main() -> int:
@@ -579,7 +618,7 @@ def main() -> int:
     # ---- batched small windows: problems/s and per-window latency -----------------------------
     batch = None
     if not args.no_batch:
-        batch = batch_leg(sg, args.batch_windows)
+        batch = batch_leg(sg, args.batch_windows, cpu=not args.no_cpu)
         log(f"[bench] batch {json.dumps(batch)}")
 
     cpu = None
