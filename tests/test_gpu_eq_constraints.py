@@ -35,8 +35,9 @@ def _eucm_case(sg, nc, npt, seed, sets):
     return p, v, costs, eqs, ofams, ocosts, oeqs
 
 
-def _compare(sg, orc, v, costs, eqs, ofams, ocosts, oeqs, iterations, names):
-    params = sg.OptParams(num_iterations=iterations, initial_lm_damping=1.0)
+def _compare(sg, orc, v, costs, eqs, ofams, ocosts, oeqs, iterations, names, solver=None):
+    solver = sg.Solver.SchurBlockSparseLdlt if solver is None else solver
+    params = sg.OptParams(num_iterations=iterations, initial_lm_damping=1.0, solver=solver)
     opt = sg.Optimizer(v, costs, params, eqs)
     trace = []
     while True:
@@ -51,7 +52,8 @@ def _compare(sg, orc, v, costs, eqs, ofams, ocosts, oeqs, iterations, names):
     orc.set_rcs_order(sg.analyze_rcs_order(v, costs))
     try:
         sol, ovars, otrace = orc.optimize_nlls(ofams, ocosts, orc.default_params(num_iterations=iterations,
-                                                                                 initial_lm_damping=1.0),
+                                                                                 initial_lm_damping=1.0,
+                                                                                 solver=int(solver)),
                                                eqs=oeqs)
     finally:
         orc.set_rcs_order(None)
@@ -72,11 +74,14 @@ def _compare(sg, orc, v, costs, eqs, ofams, ocosts, oeqs, iterations, names):
     return out, res
 
 
-def test_scale_constraint_problem_matches_oracle(sg, orc):
+@pytest.mark.parametrize("solver", ["SchurBlockSparseLdlt", "SchurSparseLdlt"])
+def test_scale_constraint_problem_matches_oracle(sg, orc, solver):
     """BaScaleConstraintProblem::new(4, 20), Schur path, 20 iterations from nu = 1, with the
-    reference's gates (ba_scale_constraint.rs:257-284)."""
+    reference's gates (ba_scale_constraint.rs:257-284); both Schur solvers (the reference's
+    supports_eq_constraints admits both, lib.rs:386-388)."""
     p, v, costs, eqs, ofams, ocosts, oeqs = _eucm_case(sg, 4, 20, 0, [[1]])
-    out, res = _compare(sg, orc, v, costs, eqs, ofams, ocosts, oeqs, 20, ("cams", "poses", "points"))
+    out, res = _compare(sg, orc, v, costs, eqs, ofams, ocosts, oeqs, 20, ("cams", "poses", "points"),
+                        solver=getattr(sg.Solver, solver))
     r = float(np.linalg.norm(p.poses_gt[1, 4:]))
     assert abs(np.linalg.norm(out["poses"][1, 4:]) - r) < 1e-4 and abs(res[0]) < 1e-4
     pose_rms = np.sqrt(np.mean(np.sum((out["poses"][:, 4:] - p.poses_gt[:, 4:]) ** 2, 1)))

@@ -181,16 +181,19 @@ def test_eucm_free_intrinsics_covariance_blocks_match_dense_inverse(sg, orc):
     opt.close()
 
 
-def test_ba_problem_inverse_blocks_match_dense_pseudo_inverse(sg, orc):
+@pytest.mark.parametrize("solver", ["SchurBlockSparseLdlt", "SchurSparseLdlt"])
+def test_ba_problem_inverse_blocks_match_dense_pseudo_inverse(sg, orc, solver):
     """The reference's schur_inverse_block_matches_dense_pseudo_inverse
     (crates/sophus_opt/src/nlls/damped_hessian.rs:610-700): BaProblem::new(4, 20) at its
     initial variables, the damped Hessian at nu = 1 (skip_final_hessian keeps that nu), and
     every free x free block (the shared camera and the free poses), every marginal diagonal
     block and every free x marginal block against the dense pseudo-inverse of H + nu N.
-    The reference's bar is 1e-6 absolute; here 1e-9 relative to the block's largest entry."""
+    The reference's bar is 1e-6 absolute; here 1e-9 relative to the block's largest entry.
+    Both Schur solvers, as the reference's run_all_checks (damped_hessian.rs:558-590)."""
     p = sg.generate_ba_eucm(4, 20, seed=0)
     opt = sg.Optimizer(p.variables(), p.cost_fns(),
-                       sg.OptParams(num_iterations=1, initial_lm_damping=1.0, skip_final_hessian=1))
+                       sg.OptParams(num_iterations=1, initial_lm_damping=1.0, skip_final_hessian=1,
+                                    solver=getattr(sg.Solver, solver)))
     H, _ = orc.linearize_dense(_oracle_fams(p), _oracle_costs(p), 1.0)
     Hinv = np.linalg.pinv(H, rcond=1e-14)
     free = [("cams", 0)] + [("poses", m) for m in range(2, p.poses_init.shape[0])]

@@ -122,3 +122,31 @@ def test_priors_with_fp32_assembly_final_cost(sg, orc):
     trace, out, merit, sol, ovars, otrace = _run_both(sg, orc, p, (), [0, 1], p.poses_gt[:2], precs, 20,
                                                       precision=sg.Precision.Mixed)
     assert abs(merit - sol["final_cost"]) <= 1e-6 * sol["final_cost"]
+
+
+def test_priors_with_pcg_final_cost(sg, orc):
+    """Priors with the iterative Schur solver: the prior's 6x6 block lands on the pose's
+    diagonal slot of S, which the Jacobi preconditioner inverts with it.  At a PCG tolerance
+    of 1e-12 the run accepts the steps the direct oracle accepts and ends at its cost (1e-8)."""
+    p = sg.generate_ba(sg.config_spec(0))
+    precs = np.repeat((1e4 * np.eye(6))[None], 2, axis=0)
+    v = _families(sg, p)
+    costs = p.cost_fns(1.0) + [sg.CostFn.isometry3_prior(["poses"], [0, 1], p.poses_gt[:2], precs, None)]
+    params = sg.OptParams(num_iterations=10, initial_lm_damping=1.0, solver=sg.Solver.SchurPcgJacobi,
+                          pcg_relative_tolerance=1e-12, pcg_max_iterations=1000)
+    opt = sg.Optimizer(v, costs, params)
+    trace = []
+    while True:
+        info = opt.step()
+        trace.append(info)
+        if info.termination is not None:
+            break
+    merit = opt.merit()
+    opt.close()
+    assert all(t.pcg_iterations > 0 for t in trace)
+    prior = np.hstack([p.poses_gt[:2], precs.reshape(-1, 36)])
+    ofams = [("cams", 2, 1, p.intrinsics, []), ("points", 1, 2, p.points_init, []), ("poses", 0, 0, p.poses_init, [])]
+    ocosts = orc.ba_costs(p) + [(3, ["poses"], np.array([[0], [1]]), prior, None)]
+    sol, _, otrace = orc.optimize_nlls(ofams, ocosts, orc.default_params(num_iterations=10, initial_lm_damping=1.0))
+    assert [t.accepted for t in trace] == [o["accepted"] for o in otrace]
+    assert abs(merit - sol["final_cost"]) <= 1e-8 * sol["final_cost"]
