@@ -9,6 +9,7 @@ side cannot hide behind the same mistake on the other.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 from typing import Dict, List, Optional, Sequence, Tuple
@@ -19,7 +20,8 @@ HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
 # variant -> library: "" is the oracle; "fma" its rounding twin (same source, FMA contraction
 # on), used to measure how far the oracle itself moves under a rounding-only change
-VARIANTS = {"": LIB, "fma": HERE / "liboracle_fma.so"}
+# SOPHUS_ORACLE_LIB replaces the oracle's library, e.g. by its sanitizer build (Makefile)
+VARIANTS = {"": Path(os.environ.get("SOPHUS_ORACLE_LIB", LIB)), "fma": HERE / "liboracle_fma.so"}
 
 
 def build() -> Path:
