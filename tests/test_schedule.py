@@ -6,7 +6,8 @@ a parent steps only after every child's last step; children of one parent end in
 rounds (their extend-adds are plain stores); exactly one child per parent, the last to end,
 factors the parent's first diagonal block; each step's tiles are the lower triangle of its
 trailing region; split rounds list every panel-row chunk; the tiles that factor a diagonal
-block come first in their round; a child's update rows keep their order in the parent.
+block come first in their round; a child's update rows keep their order in the parent.  A second build runs the same checks
+under the host sanitizers.
 """
 import shutil
 import subprocess
@@ -18,10 +19,18 @@ ROOT = Path(__file__).resolve().parents[1]
 CSRC = ROOT / "sophus-rs_amd" / "csrc"
 
 
+# The sanitized build runs the same checks with the host code under AddressSanitizer (leaks
+# included) and UndefinedBehaviorSanitizer, undefined behaviour fatal.
+SANITIZE = ["-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+            "-fno-sanitize-recover=undefined"]
+
+
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
-def test_round_schedule_invariants(tmp_path):
+@pytest.mark.parametrize("sanitize", [False, True], ids=["plain", "asan_ubsan"])
+def test_round_schedule_invariants(tmp_path, sanitize):
     exe = tmp_path / "sched_check"
-    cmd = ["g++", "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+    cmd = ["g++", "-O1", *(SANITIZE if sanitize else []), "-std=c++17", "-D__HIP_PLATFORM_AMD__",
+           "-I/opt/rocm/include",
            f"-I{CSRC}", f"-I{ROOT / 'include'}", str(ROOT / "tests" / "sched_check.cpp"),
            str(CSRC / "problem.cpp"), str(CSRC / "generator.cpp"), "-o", str(exe)]
     build = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
