@@ -6,7 +6,8 @@ solve's division by a zero pivot (the component is NaN).  The oracle is pinned o
 (tests/test_oracle_sparse_ldlt.py).  Bars: solves 1e-10 relative (1e-3 where the kept pivot
 makes the condition number 3e12); the LM run of the 10-camera ring (BaProblem-style
 workload of the reference's CPU case, "Schur complement plus sparse LDLt, 20 LM
-iterations") step by step: the same accept sequence, merit 1e-9, parameters 1e-7.
+iterations") and a 60-camera walk step by step: the same accept sequence, merit 1e-9,
+parameters 1e-7.
 """
 import dataclasses
 
@@ -151,3 +152,22 @@ def test_unobserved_camera_under_the_scalar_rule_matches_oracle(sg, orc):
     _compare(trace, out, sol, ovars, otrace)
     assert np.all(np.isnan(out["poses"][10])) and np.all(np.isfinite(out["poses"][:10]))
     assert np.isnan(trace[0].gain_ratio) or trace[0].gain_ratio == 1.0
+
+
+def test_walk_schur_sparse_ldlt_matches_oracle(sg, orc):
+    """A 60-camera random walk (3,000 points, camera 0 constant) under SchurSparseLdlt, ten LM
+    iterations step by step in the GPU's elimination order: the walk's camera graph has long
+    chains of separators, so the supernodal factorization takes many rounds with the scalar
+    pivot rule in every diagonal block."""
+    spec = sg.BaSpec(layout=1, num_cameras=60, num_points=3000, geometric_p=0.15, seed=5)
+    p = sg.generate_ba(spec)
+    out, trace = _gpu_trace(sg, p, sg.OptParams(num_iterations=10, initial_lm_damping=1.0,
+                                                solver=sg.Solver.SchurSparseLdlt))
+    orc.set_rcs_order(sg.analyze_rcs_order(p.variables(), p.cost_fns(1.0)))
+    try:
+        sol, ovars, otrace = orc.optimize_nlls(orc.ba_families(p), orc.ba_costs(p),
+                                               orc.default_params(num_iterations=10, initial_lm_damping=1.0,
+                                                                  solver=int(sg.Solver.SchurSparseLdlt)))
+    finally:
+        orc.set_rcs_order(None)
+    print(f"SchurSparseLdlt walk: merit {_compare(trace, out, sol, ovars, otrace):.3g}")
