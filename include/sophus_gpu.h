@@ -26,7 +26,8 @@
  *   - The cost has no 1/2 factor: cost = sum rho(|r|) (Huber) or |r|^2.
  *
  * Threading: an sgpu_optimizer handle owns its device buffers and one HIP stream;
- * calls on one handle must not overlap; distinct handles are independent.
+ * calls on one handle must not overlap; distinct handles are independent and may be
+ * used from different threads at once; sgpu_last_error() is per calling thread.
  */
 #ifndef SOPHUS_GPU_H
 #define SOPHUS_GPU_H
