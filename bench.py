@@ -513,6 +513,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, help="workload index (0, 1, 2)")
+    ap.add_argument("--solver", default="SchurBlockSparseLdlt",
+                    choices=["SchurBlockSparseLdlt", "SchurSparseLdlt"],
+                    help="direct RCS solver of the per-iteration leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance runs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-window leg")
@@ -541,6 +544,7 @@ def main() -> int:
 
     # ---- per-iteration throughput of the plain FP64 path -----------------------------------
     params = sg.OptParams(num_iterations=1 << 30, initial_lm_damping=1.0, function_tolerance=0.0,
+                          solver=getattr(sg.Solver, args.solver),
                           gradient_tolerance=0.0, parameter_tolerance=0.0,
                           max_consecutive_rejects=1 << 30)
     t = time.perf_counter()
@@ -640,9 +644,13 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"ba_walk_{spec.num_cameras}cams_{spec.num_points}pts",
+        "config": {"workload": f"ba_{'ring' if spec.layout == 0 else 'walk'}_{spec.num_cameras}cams_"
+                               f"{spec.num_points}pts",
                    "num_cameras": spec.num_cameras, "num_points": spec.num_points, "num_obs": n_obs,
-                   "value_path": "f64 Schur + direct block LDLt, per LM iteration",
+                   "value_path": ("f64 Schur + direct block LDLt, per LM iteration"
+                                  if args.solver == "SchurBlockSparseLdlt" else
+                                  "f64 Schur + supernodal LDLt with SparseLdlt's pivot rule, "
+                                  "per LM iteration"),
                    "huber_delta_px": 1.0, "initial_lm_damping": 1.0, "seed": spec.seed},
         "roofline": roof,
         "cpu_baseline": cpu,
